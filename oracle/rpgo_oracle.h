@@ -1,0 +1,120 @@
+/*
+ * rpgo_oracle.h -- FP64 CPU restatement of Kimera-RPGO's PCM path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: tests/, the
+ * smoke() entry point and bench.py's cpu_baseline leg may load it.  The
+ * product (kimera-rpgo_amd/) never links, loads or calls it.
+ *
+ * Every routine names the reference file:line it restates.  Operation
+ * order is fixed (dot products summed k = 0..n-1, left to right) so that
+ * the GPU FP64 replay kernel can reproduce the same rounding.
+ *
+ * Layouts (all row-major doubles):
+ *   SE(3) pose: 12 doubles  R[9] (row-major) then t[3]
+ *   SE(2) pose:  4 doubles  c, s, x, y        (GTSAM Pose2: Rot2 (c,s), Point2)
+ *   SE(3) covariance: 36 doubles, tangent order (omega, v)  (GTSAM Pose3)
+ *   SE(2) covariance:  9 doubles, tangent order (x, y, theta) (GTSAM Pose2)
+ */
+#ifndef RPGO_ORACLE_H
+#define RPGO_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- geometry (exported for known-answer tests) ---- */
+void orc_se3_compose(const double* a, const double* b, double* out);
+void orc_se3_inverse(const double* a, double* out);
+void orc_se3_between(const double* a, const double* b, double* out);
+void orc_se3_adjoint(const double* a, double* ad36);
+void orc_se3_logmap(const double* a, double* xi6);
+void orc_se3_expmap(const double* xi6, double* out);
+void orc_rot3_logmap(const double* R9, double* w3);
+void orc_quat_to_rot(double w, double x, double y, double z, double* R9);
+
+void orc_se2_compose(const double* a, const double* b, double* out);
+void orc_se2_inverse(const double* a, double* out);
+void orc_se2_between(const double* a, const double* b, double* out);
+void orc_se2_adjoint(const double* a, double* ad9);
+void orc_se2_logmap(const double* a, double* xi3);
+
+/* PoseWithCovariance ops (GeometryUtils.h:119-186).  pwc layout:
+ * pose (12 | 4) ++ cov (36 | 9) ++ rot_info flag as double (1.0 / 0.0). */
+int orc_pwc_size(int dim);
+void orc_pwc_from_measurement(int dim, const double* pose, const double* cov, double* pwc);
+void orc_pwc_compose(int dim, const double* a, const double* b, double* out);
+void orc_pwc_inverse(int dim, const double* a, double* out);
+void orc_pwc_between(int dim, const double* a, const double* b, double* out);
+double orc_pwc_mahalanobis(int dim, const double* a);
+
+/* Eigen-style LLT success test and LU inverse, exported for tests. */
+int orc_llt_ok(int n, const double* A);
+void orc_lu_inverse(int n, const double* A, double* Ainv);
+
+/* ---- PCM pipeline ---- */
+
+/* Odometry fold (Pcm.h:516-557): trajectory pwc records for keys 0..N-1.
+ * pose0: initial pose of the chain (from the values); odom: N-1 deltas;
+ * odom_cov: N-1 covariances.  traj_out: N * pwc_size doubles. */
+void orc_fold(int dim, int n_poses, const double* pose0, const double* odom,
+              const double* odom_cov, double* traj_out);
+
+/* Odometry check of one LC (Pcm.h:604-629).  Returns the Mahalanobis norm. */
+double orc_odom_dist(int dim, const double* traj, int from, int to,
+                     const double* lc_pose, const double* lc_cov);
+
+/* Pairwise distance for older LC i=(a,b) and newer LC j=(c,d) (Pcm.h:670-718). */
+double orc_pair_dist(int dim, const double* traj,
+                     int a, int b, const double* zab, const double* sab,
+                     int c, int d, const double* zcd, const double* scd);
+
+/* Whole adjacency stage for one single-prefix group, as the reference does
+ * it for a batch update (Pcm.h:414-502, 725-768):
+ *   odom check of every LC (odom_thr < 0 disables), accepted LCs appended in
+ *   input order, pairwise check of each new LC against all earlier accepted.
+ * accepted_out[n_lc]: index (into the LC input) of each accepted LC, returns
+ * the accepted count m.  odom_dist_out[n_lc] (may be NULL).  adj_out: m*m bytes
+ * (row-major, symmetric, 0 diagonal).  dist_out: m*m doubles (may be NULL).
+ * lc_thr < 0 disables the pairwise check (adjacency left all-zero). */
+int orc_pcm_adjacency(int dim, int n_poses, const double* traj,
+                      int n_lc, const int32_t* lc_from, const int32_t* lc_to,
+                      const double* lc_pose, const double* lc_cov,
+                      double odom_thr, double lc_thr,
+                      int32_t* accepted_out, double* odom_dist_out,
+                      uint8_t* adj_out, double* dist_out);
+
+/* Pairwise distances only (no threshold), for pairs i<j of an already
+ * accepted LC list; dist_out m*m (symmetric, 0 diagonal). Rows [j0, j1). */
+void orc_pcm_dist_rows(int dim, const double* traj, int m,
+                       const int32_t* lc_from, const int32_t* lc_to,
+                       const double* lc_pose, const double* lc_cov,
+                       int j0, int j1, double* dist_out);
+
+/* ---- max clique (findCliqueHeu.cpp / findClique.cpp / graphIO.cpp) ---- */
+
+/* CSR from a dense symmetric byte adjacency (graphIO.cpp:188-230).
+ * offsets[n+1], edges[capacity]; returns number of edge entries. */
+int orc_csr_from_dense(int n, const uint8_t* adj, int32_t* offsets,
+                       int32_t* edges, int capacity);
+
+/* maxCliqueHeu with the stale-buffer output (findCliqueHeu.cpp:32-117).
+ * out: n ints (only the first returned-size entries are meaningful). */
+int orc_max_clique_heu(int n, const int32_t* offsets, const int32_t* edges,
+                       int32_t* out);
+
+/* Incremental heuristic (findCliqueHeu.cpp:119-209, GraphUtils.cpp:30-44):
+ * returns 0 when the previous set should be kept. */
+int orc_max_clique_heu_incremental(int n, const int32_t* offsets,
+                                   const int32_t* edges, int num_new,
+                                   int prev_size, int32_t* out);
+
+/* Exact branch and bound (findClique.cpp:31-145); out receives the clique in
+ * the reference's reverse-discovery order. */
+int orc_max_clique_exact(int n, const int32_t* offsets, const int32_t* edges,
+                         int32_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
