@@ -1,0 +1,138 @@
+"""PCM pair-check throughput on one MI355X (device-resident inputs).
+
+Workload (default): SE(3) helix, 5,000 poses, 1,000 loop closures (20 %
+outliers), odometry check off (odom_threshold = -1) so all 499,500 unique
+pairs (i<j) are evaluated, lc_threshold = 5.0.  A step is one pass of the PCM
+adjacency stage over the uploaded graph: FP64 odometry fold, odometry check and
+compaction, pairwise consistency of every pair, adjacency bit packing.
+
+Prints one JSON line (metric, value, roofline of the dominant kernel, CPU
+baseline of the FP64 oracle on the host cores).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "kimera-rpgo_amd"), os.path.join(ROOT, "oracle")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+# Algorithmic FP32 work per SE(3) pair check (5 congruences, 6 group ops,
+# 5 adjoints, sym add/sub, 2 PSD checks, Log, Cholesky Mahalanobis); SE(2) ~720.
+FLOP_PER_PAIR = {3: 4540.0, 2: 720.0}
+PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}
+
+WORKLOADS = {
+    "se3_5k": dict(kind="se3_helix", n_poses=5000, n_lc=1000, outlier_frac=0.2, odom_thr=-1.0, lc_thr=5.0),
+    "se3_50k": dict(kind="se3_helix", n_poses=50000, n_lc=10000, outlier_frac=0.5, odom_thr=-1.0, lc_thr=5.0),
+}
+
+
+def cpu_baseline(g, odom_thr, lc_thr, budget_s=15.0):
+    """FP64 oracle (single core) on a bounded row sample of the same graph."""
+    import oracle as O
+
+    t0 = time.perf_counter()
+    traj = O.fold(g.dim, g.pose0, g.odom, g.odom_cov)
+    t_fold = time.perf_counter() - t0
+    m = g.n_lc
+    # rows j in [0, j1): pairs j1*(j1-1)/2; grow j1 until the budget is spent
+    pairs = 0
+    t_pairs = 0.0
+    j0 = 1
+    step = 64
+    while j0 < m and t_pairs < budget_s:
+        j1 = min(m, j0 + step)
+        t = time.perf_counter()
+        O.pcm_dist_rows(g.dim, traj, g.lc_from[:j1], g.lc_to[:j1], g.lc_pose[:j1], g.lc_cov[:j1], j0, j1)
+        t_pairs += time.perf_counter() - t
+        pairs += (j1 * (j1 - 1) - j0 * (j0 - 1)) // 2
+        j0 = j1
+        step *= 2
+    return dict(value=pairs / t_pairs, unit="pair-checks/s", cores=1, kind="port",
+                sample=f"FP64 oracle, rows 1..{j0 - 1} of the workload ({pairs} pairs, "
+                       f"{t_pairs:.1f} s; fold {t_fold * 1e3:.1f} ms)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="se3_5k", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--seed", type=int, default=0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
+              "(the multi-GPU sharding is a design only)", file=sys.stderr)
+        sys.exit(2)
+
+    from rpgo_amd import _capi, synth
+
+    w = WORKLOADS[a.workload]
+    g = synth.se3_helix(n_poses=w["n_poses"], n_lc=w["n_lc"], outlier_frac=w["outlier_frac"], seed=a.seed)
+    prec = _capi.PCM_FP64_REPLAY if a.precision == "fp64" else _capi.PCM_FP32_RECHECK
+    ctx = _capi.Context(0)
+    ctx.pcm_upload(g.dim, g.pose0, g.odom, g.odom_cov, g.lc_from, g.lc_to, g.lc_pose, g.lc_cov)
+    for _ in range(a.warmup):
+        ctx.pcm_run(w["odom_thr"], w["lc_thr"], precision=prec)
+    _capi.lib().rpgo_ctx_sync(ctx._h)
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        stats.append(ctx.pcm_run(w["odom_thr"], w["lc_thr"], precision=prec))
+    _capi.lib().rpgo_ctx_sync(ctx._h)
+    wall = time.perf_counter() - t0
+    ms_step = wall * 1e3 / a.steps
+    pairs = stats[-1].n_pairs
+    ms_pairs = float(np.mean([s.ms_pairs for s in stats]))
+    ms_total = float(np.mean([s.ms_total for s in stats]))
+    flop = FLOP_PER_PAIR[g.dim] * pairs
+    achieved = flop / (ms_pairs * 1e-3) / 1e12
+    peak = PEAK_TFLOPS["fp64" if a.precision == "fp64" else "fp32"]
+    out = {
+        "metric": "PCM pair-checks/s",
+        "value": pairs / (ms_step * 1e-3),
+        "unit": "pair-checks/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if a.precision == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"{a.workload}: SE(3) helix {w['n_poses']} poses, {w['n_lc']} loop closures, "
+                               f"{int(w['outlier_frac'] * 100)}% outliers, odom_thr {w['odom_thr']}, "
+                               f"lc_thr {w['lc_thr']}", "pairs": pairs, "precision": a.precision,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "valu", "kernel": "k_pairs64" if a.precision == "fp64" else "k_pairs32",
+                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": None, "flop_per_pair": FLOP_PER_PAIR[g.dim], "ms_kernel": ms_pairs},
+        "stages_ms": {"fold": float(np.mean([s.ms_fold for s in stats])),
+                      "odom_check": float(np.mean([s.ms_odom for s in stats])),
+                      "pairs": ms_pairs, "recheck": float(np.mean([s.ms_recheck for s in stats])),
+                      "device_total": ms_total},
+        "n_consistent": stats[-1].n_consistent,
+        "n_rechecked": stats[-1].n_rechecked,
+    }
+    k, _, ms_clq = ctx.max_clique(_capi.CLIQUE_HEU_REF)
+    out["clique_heu_ref"] = {"size": k, "ms": ms_clq}
+    ctx.close()
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(g, w["odom_thr"], w["lc_thr"])
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

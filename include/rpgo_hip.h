@@ -1,0 +1,131 @@
+/*
+ * rpgo_hip.h -- C ABI of the MI355X-native Kimera-RPGO robust back end.
+ *
+ * Plain C: pointers, sizes and status codes; no C++ or GPU types cross it.
+ * Every entry point returns RPGO_OK (0) or a negative status; the message of
+ * the last failure on the calling thread is available from rpgo_last_error().
+ * A context owns its device memory and one HIP stream; use one context per
+ * host thread.  Host buffers are caller owned.
+ *
+ * Data layouts (row-major doubles):
+ *   SE(3) pose  12 doubles: R (9, row-major) then t (3)     [gtsam::Pose3]
+ *   SE(2) pose   4 doubles: cos, sin, x, y                   [gtsam::Pose2]
+ *   SE(3) cov   36 doubles, tangent order (omega, v)         [Pose3 tangent]
+ *   SE(2) cov    9 doubles, tangent order (x, y, theta)      [Pose2 tangent]
+ *   adjacency   bit-packed rows: W = ceil(m/64) uint64 words per row,
+ *               bit (i & 63) of word [j*W + i/64] is adj(j, i)
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   rpgo_pcm_*        Pcm<poseT,PoseWithCovariance>::parseAndIncrementAdjMatrix
+ *                     include/KimeraRPGO/outlier/Pcm.h:414-502 with updateOdom
+ *                     :516-557, isOdomConsistent :604-629, areLoopsConsistent
+ *                     :670-718 and incrementAdjMatrix :725-768
+ *   rpgo_max_clique*  findMaxCliqueHeu src/utils/GraphUtils.cpp:19-27
+ *                     (FMC::maxCliqueHeu findCliqueHeu.cpp:32-117) and the
+ *                     exact FMC::maxClique findClique.cpp:80-145
+ */
+#ifndef RPGO_HIP_H
+#define RPGO_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RPGO_OK 0
+#define RPGO_ERR_ARG (-1)     /* invalid argument or size                    */
+#define RPGO_ERR_HIP (-2)     /* a HIP runtime call failed                   */
+#define RPGO_ERR_NOMEM (-3)   /* device allocation failed                    */
+#define RPGO_ERR_STATE (-4)   /* call out of order (e.g. run before upload)  */
+
+/* PCM precision modes */
+#define RPGO_PCM_FP64_REPLAY 0  /* every pair in FP64, reference operation order */
+#define RPGO_PCM_FP32_RECHECK 1 /* FP32 pairs, FP64 replay of pairs near threshold */
+
+/* max-clique modes */
+#define RPGO_CLIQUE_HEU_REF 0   /* bit-exact FMC::maxCliqueHeu output (stale buffer) */
+#define RPGO_CLIQUE_EXACT 1     /* exact maximum clique */
+
+typedef struct rpgo_ctx rpgo_ctx;
+
+/* Context on HIP device `device` (flags reserved, pass 0). */
+int rpgo_ctx_create(int device, uint32_t flags, rpgo_ctx** out);
+int rpgo_ctx_destroy(rpgo_ctx* ctx);
+int rpgo_ctx_sync(rpgo_ctx* ctx);
+const char* rpgo_last_error(void);
+/* Library version string ("rpgo-mi355x <semver>"). */
+const char* rpgo_version(void);
+
+/* One single-prefix PCM group: odometry chain 0..n_poses-1 plus loop closures.
+ * Mirrors what Pcm::removeOutliers extracts from one update() call: the first
+ * pose comes from the values (zero covariance, Pcm.h:534-542), odometry k is
+ * the BetweenFactor k -> k+1, loop closure t is BetweenFactor(from, to, pose). */
+typedef struct {
+  int32_t dim;            /* 3 = SE(3) (Pcm3D), 2 = SE(2) (Pcm2D)        */
+  int32_t n_poses;        /* >= 1                                          */
+  const double* pose0;    /* pose_len doubles                             */
+  const double* odom;     /* (n_poses-1) * pose_len                        */
+  const double* odom_cov; /* (n_poses-1) * d*d ; NaN rotation trace drops
+                             the rotation block (GeometryUtils.h:102-113) */
+  int32_t n_lc;           /* >= 0                                          */
+  const int32_t* lc_from; /* n_lc, in [0, n_poses)                         */
+  const int32_t* lc_to;   /* n_lc, in [0, n_poses)                         */
+  const double* lc_pose;  /* n_lc * pose_len                               */
+  const double* lc_cov;   /* n_lc * d*d                                    */
+} rpgo_pcm_problem;
+
+typedef struct {
+  double odom_threshold;  /* PcmParams::odom_threshold; < 0 disables (Pcm.h:74-77)  */
+  double lc_threshold;    /* PcmParams::lc_threshold;   < 0 disables (Pcm.h:79-82)  */
+  int32_t precision;      /* RPGO_PCM_FP64_REPLAY or RPGO_PCM_FP32_RECHECK           */
+  int32_t want_dist;      /* keep the m*m distance matrix for rpgo_pcm_download      */
+  double recheck_margin;  /* FP32 mode: relative band around the threshold that is
+                             replayed in FP64 (<= 0 selects the built-in default)  */
+} rpgo_pcm_params;
+
+typedef struct {
+  int32_t n_accepted;     /* loop closures that passed the odometry check         */
+  int32_t reserved;
+  int64_t n_pairs;        /* unique (i<j) pairs evaluated                          */
+  int64_t n_rechecked;    /* FP32 mode: pairs replayed in FP64                      */
+  int64_t n_flipped;      /* FP32 mode: FP32 decisions the FP64 replay overturned   */
+  int64_t n_consistent;   /* pairs with d < lc_threshold                            */
+  double ms_fold;         /* device time of each stage (HIP events, ctx stream)     */
+  double ms_odom;
+  double ms_pairs;
+  double ms_recheck;
+  double ms_total;
+} rpgo_pcm_stats;
+
+/* Copy a problem to device memory (kept in the context until the next upload). */
+int rpgo_pcm_upload(rpgo_ctx* ctx, const rpgo_pcm_problem* prob);
+/* Run the adjacency stage on the uploaded problem, device-resident; blocks
+ * until done and fills *stats (may be NULL). */
+int rpgo_pcm_run(rpgo_ctx* ctx, const rpgo_pcm_params* params, rpgo_pcm_stats* stats);
+/* Fetch results of the last run.  accepted: n_lc ints (indices of accepted loop
+ * closures, in input order; n_accepted valid).  odom_dist: n_lc doubles or NULL.
+ * adj_bits: m*W words or NULL.  dist: m*m doubles or NULL (needs want_dist). */
+int rpgo_pcm_download(rpgo_ctx* ctx, int32_t* accepted, double* odom_dist,
+                      uint64_t* adj_bits, double* dist);
+/* upload + run + download in one call. */
+int rpgo_pcm_adjacency(rpgo_ctx* ctx, const rpgo_pcm_problem* prob,
+                       const rpgo_pcm_params* params, int32_t* accepted,
+                       double* odom_dist, uint64_t* adj_bits, double* dist,
+                       rpgo_pcm_stats* stats);
+/* The FP64 trajectory fold of the last run: n_poses * (pose_len + d*d + 1)
+ * doubles per pose (pose, covariance, rotation_info flag). */
+int rpgo_pcm_download_traj(rpgo_ctx* ctx, double* traj);
+
+/* Max clique over the adjacency of the last rpgo_pcm_run (device resident).
+ * idx_out: room for m ints; *k_out: clique size (FMC returns -1 for m == 0). */
+int rpgo_max_clique(rpgo_ctx* ctx, int32_t mode, int32_t* idx_out, int32_t* k_out,
+                    double* ms_out);
+/* Max clique of a caller-provided bit-packed symmetric adjacency (n vertices). */
+int rpgo_max_clique_bits(rpgo_ctx* ctx, int32_t n, const uint64_t* adj_bits,
+                         int32_t mode, int32_t* idx_out, int32_t* k_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RPGO_HIP_H */

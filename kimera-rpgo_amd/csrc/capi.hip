@@ -1,0 +1,310 @@
+// C ABI entry points of librpgo_hip (declared in include/rpgo_hip.h).
+#include <cstdarg>
+#include <cstring>
+#include <vector>
+
+#include "rpgo_internal.h"
+
+namespace rpgo {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+static int upload(DevBuf& b, const void* src, size_t bytes, hipStream_t st) {
+  int rc = b.reserve(bytes > 0 ? bytes : 8);
+  if (rc) return rc;
+  if (bytes) RPGO_HIP_CHECK(hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, st));
+  return RPGO_OK;
+}
+
+static float elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, a, b);
+  return ms;
+}
+
+}  // namespace rpgo
+
+using namespace rpgo;
+
+extern "C" {
+
+const char* rpgo_last_error(void) { return rpgo::g_err; }
+
+const char* rpgo_version(void) { return "rpgo-mi355x 0.1.0"; }
+
+int rpgo_ctx_create(int device, uint32_t flags, rpgo_ctx** out) {
+  (void)flags;
+  if (!out) {
+    set_error("rpgo_ctx_create: out is NULL");
+    return RPGO_ERR_ARG;
+  }
+  int count = 0;
+  RPGO_HIP_CHECK(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count) {
+    set_error("rpgo_ctx_create: device %d out of range (%d devices)", device, count);
+    return RPGO_ERR_ARG;
+  }
+  RPGO_HIP_CHECK(hipSetDevice(device));
+  rpgo_ctx* c = new rpgo_ctx();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    set_error("hipStreamCreate failed: %s", hipGetErrorString(e));
+    delete c;
+    return RPGO_ERR_HIP;
+  }
+  for (auto& ev : c->ev) {
+    e = hipEventCreate(&ev);
+    if (e != hipSuccess) {
+      set_error("hipEventCreate failed: %s", hipGetErrorString(e));
+      return RPGO_ERR_HIP;
+    }
+  }
+  if (c->pcm.counters.reserve(256)) {
+    delete c;
+    return RPGO_ERR_NOMEM;
+  }
+  *out = c;
+  return RPGO_OK;
+}
+
+int rpgo_ctx_destroy(rpgo_ctx* ctx) {
+  if (!ctx) return RPGO_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  PcmState& s = ctx->pcm;
+  for (DevBuf* b : {&s.pose0, &s.odom, &s.odom_cov, &s.lc_from, &s.lc_to, &s.lc_pose, &s.lc_cov,
+                    &s.delta, &s.hinv, &s.traj, &s.lcm, &s.odom_dist, &s.flags, &s.accepted,
+                    &s.c_from, &s.c_to, &s.c_lcm, &s.adj, &s.dist, &s.counters, &s.fp32_buf,
+                    &s.queue})
+    b->release();
+  CliqueState& q = ctx->clq;
+  for (DevBuf* b : {&q.adj, &q.deg, &q.dmask, &q.icc, &q.out, &q.scratch, &q.best}) b->release();
+  for (auto& ev : ctx->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return RPGO_OK;
+}
+
+int rpgo_ctx_sync(rpgo_ctx* ctx) {
+  if (!ctx) return RPGO_ERR_ARG;
+  RPGO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return RPGO_OK;
+}
+
+int rpgo_pcm_upload(rpgo_ctx* ctx, const rpgo_pcm_problem* p) {
+  if (!ctx || !p) {
+    set_error("rpgo_pcm_upload: NULL argument");
+    return RPGO_ERR_ARG;
+  }
+  if (p->dim != 2 && p->dim != 3) {
+    set_error("rpgo_pcm_upload: dim must be 2 or 3 (got %d)", p->dim);
+    return RPGO_ERR_ARG;
+  }
+  if (p->n_poses < 1 || p->n_lc < 0 || !p->pose0 ||
+      (p->n_poses > 1 && (!p->odom || !p->odom_cov)) ||
+      (p->n_lc > 0 && (!p->lc_from || !p->lc_to || !p->lc_pose || !p->lc_cov))) {
+    set_error("rpgo_pcm_upload: inconsistent sizes or NULL arrays (n_poses=%d n_lc=%d)",
+              p->n_poses, p->n_lc);
+    return RPGO_ERR_ARG;
+  }
+  for (int k = 0; k < p->n_lc; ++k) {
+    if (p->lc_from[k] < 0 || p->lc_from[k] >= p->n_poses || p->lc_to[k] < 0 ||
+        p->lc_to[k] >= p->n_poses) {
+      set_error("rpgo_pcm_upload: loop closure %d references pose (%d,%d) outside [0,%d)", k,
+                p->lc_from[k], p->lc_to[k], p->n_poses);
+      return RPGO_ERR_ARG;
+    }
+  }
+  RPGO_HIP_CHECK(hipSetDevice(ctx->device));
+  PcmState& s = ctx->pcm;
+  const int PL = pose_len(p->dim), D = tan_dim(p->dim);
+  const size_t nd = (size_t)(p->n_poses - 1), nl = (size_t)p->n_lc;
+  int rc;
+  hipStream_t st = ctx->stream;
+  if ((rc = upload(s.pose0, p->pose0, sizeof(double) * PL, st))) return rc;
+  if ((rc = upload(s.odom, p->odom, sizeof(double) * PL * nd, st))) return rc;
+  if ((rc = upload(s.odom_cov, p->odom_cov, sizeof(double) * D * D * nd, st))) return rc;
+  if ((rc = upload(s.lc_from, p->lc_from, sizeof(int) * nl, st))) return rc;
+  if ((rc = upload(s.lc_to, p->lc_to, sizeof(int) * nl, st))) return rc;
+  if ((rc = upload(s.lc_pose, p->lc_pose, sizeof(double) * PL * nl, st))) return rc;
+  if ((rc = upload(s.lc_cov, p->lc_cov, sizeof(double) * D * D * nl, st))) return rc;
+  RPGO_HIP_CHECK(hipStreamSynchronize(st));
+  s.dim = p->dim;
+  s.n_poses = p->n_poses;
+  s.n_lc = p->n_lc;
+  s.uploaded = true;
+  s.ran = false;
+  return RPGO_OK;
+}
+
+int rpgo_pcm_run(rpgo_ctx* ctx, const rpgo_pcm_params* prm, rpgo_pcm_stats* stats) {
+  if (!ctx || !prm) {
+    set_error("rpgo_pcm_run: NULL argument");
+    return RPGO_ERR_ARG;
+  }
+  PcmState& s = ctx->pcm;
+  if (!s.uploaded) {
+    set_error("rpgo_pcm_run: no problem uploaded");
+    return RPGO_ERR_STATE;
+  }
+  if (prm->precision != RPGO_PCM_FP64_REPLAY && prm->precision != RPGO_PCM_FP32_RECHECK) {
+    set_error("rpgo_pcm_run: unknown precision mode %d", prm->precision);
+    return RPGO_ERR_ARG;
+  }
+  RPGO_HIP_CHECK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  hipEvent_t* ev = ctx->ev;
+  int rc;
+  RPGO_HIP_CHECK(hipMemsetAsync(s.counters.ptr, 0, 256, st));
+  RPGO_HIP_CHECK(hipEventRecord(ev[0], st));
+  if ((rc = pcm_prepare(ctx))) return rc;
+  if ((rc = pcm_fold(ctx))) return rc;
+  RPGO_HIP_CHECK(hipEventRecord(ev[1], st));
+  int m = 0;
+  if ((rc = pcm_odom_and_compact(ctx, prm->odom_threshold, &m))) return rc;
+  RPGO_HIP_CHECK(hipEventRecord(ev[2], st));
+  s.m = m;
+  const int W = (m + 63) / 64;
+  const bool want_dist = prm->want_dist != 0;
+  if ((rc = s.adj.reserve(sizeof(uint64_t) * (size_t)(m > 0 ? m : 1) * (W > 0 ? W : 1)))) return rc;
+  if (want_dist && (rc = s.dist.reserve(sizeof(double) * (size_t)(m > 0 ? m : 1) * (m > 0 ? m : 1))))
+    return rc;
+  if (m > 0) RPGO_HIP_CHECK(hipMemsetAsync(s.adj.ptr, 0, sizeof(uint64_t) * (size_t)m * W, st));
+  if (want_dist && m > 0)
+    RPGO_HIP_CHECK(hipMemsetAsync(s.dist.ptr, 0, sizeof(double) * (size_t)m * m, st));
+  RPGO_HIP_CHECK(hipEventRecord(ev[3], st));
+  int64_t n_recheck = 0, n_flip = 0;
+  float ms_pairs = 0.f, ms_recheck = 0.f;
+  const bool check = prm->lc_threshold >= 0.0;
+  if (check && m > 1) {
+    if (prm->precision == RPGO_PCM_FP64_REPLAY) {
+      if ((rc = pcm_pairs_fp64(ctx, prm->lc_threshold, want_dist))) return rc;
+    } else {
+      if ((rc = pcm_pairs_fp32(ctx, prm->lc_threshold, prm->recheck_margin, want_dist,
+                               &n_recheck, &n_flip, &ms_pairs, &ms_recheck)))
+        return rc;
+    }
+    if ((rc = pcm_symmetrize(ctx))) return rc;
+  }
+  RPGO_HIP_CHECK(hipEventRecord(ev[4], st));
+  RPGO_HIP_CHECK(hipEventSynchronize(ev[4]));
+  s.ran = true;
+  s.have_dist = want_dist;
+  if (stats) {
+    unsigned long long cnt[4] = {0, 0, 0, 0};
+    RPGO_HIP_CHECK(hipMemcpy(cnt, s.counters.ptr, sizeof(cnt), hipMemcpyDeviceToHost));
+    stats->n_accepted = m;
+    stats->reserved = 0;
+    stats->n_pairs = check ? (int64_t)m * (m - 1) / 2 : 0;
+    stats->n_rechecked = n_recheck;
+    stats->n_flipped = n_flip;
+    stats->n_consistent = (int64_t)cnt[0];
+    stats->ms_fold = elapsed(ev[0], ev[1]);
+    stats->ms_odom = elapsed(ev[1], ev[2]);
+    if (prm->precision == RPGO_PCM_FP64_REPLAY) {
+      stats->ms_pairs = elapsed(ev[3], ev[4]);
+      stats->ms_recheck = 0.0;
+    } else {
+      stats->ms_pairs = ms_pairs;
+      stats->ms_recheck = ms_recheck;
+    }
+    stats->ms_total = elapsed(ev[0], ev[4]);
+  }
+  return RPGO_OK;
+}
+
+int rpgo_pcm_download(rpgo_ctx* ctx, int32_t* accepted, double* odom_dist, uint64_t* adj_bits,
+                      double* dist) {
+  if (!ctx) return RPGO_ERR_ARG;
+  PcmState& s = ctx->pcm;
+  if (!s.ran) {
+    set_error("rpgo_pcm_download: nothing computed yet");
+    return RPGO_ERR_STATE;
+  }
+  RPGO_HIP_CHECK(hipSetDevice(ctx->device));
+  const int m = s.m, W = (m + 63) / 64;
+  if (accepted && m > 0)
+    RPGO_HIP_CHECK(hipMemcpy(accepted, s.accepted.ptr, sizeof(int) * m, hipMemcpyDeviceToHost));
+  if (odom_dist && s.n_lc > 0)
+    RPGO_HIP_CHECK(hipMemcpy(odom_dist, s.odom_dist.ptr, sizeof(double) * s.n_lc,
+                             hipMemcpyDeviceToHost));
+  if (adj_bits && m > 0)
+    RPGO_HIP_CHECK(hipMemcpy(adj_bits, s.adj.ptr, sizeof(uint64_t) * (size_t)m * W,
+                             hipMemcpyDeviceToHost));
+  if (dist && m > 0) {
+    if (!s.have_dist) {
+      set_error("rpgo_pcm_download: run without want_dist");
+      return RPGO_ERR_STATE;
+    }
+    RPGO_HIP_CHECK(hipMemcpy(dist, s.dist.ptr, sizeof(double) * (size_t)m * m,
+                             hipMemcpyDeviceToHost));
+  }
+  return RPGO_OK;
+}
+
+int rpgo_pcm_download_traj(rpgo_ctx* ctx, double* traj) {
+  if (!ctx || !traj) return RPGO_ERR_ARG;
+  PcmState& s = ctx->pcm;
+  if (!s.ran) {
+    set_error("rpgo_pcm_download_traj: nothing computed yet");
+    return RPGO_ERR_STATE;
+  }
+  RPGO_HIP_CHECK(hipSetDevice(ctx->device));
+  RPGO_HIP_CHECK(hipMemcpy(traj, s.traj.ptr, sizeof(double) * (size_t)s.n_poses * pwc_len(s.dim),
+                           hipMemcpyDeviceToHost));
+  return RPGO_OK;
+}
+
+int rpgo_pcm_adjacency(rpgo_ctx* ctx, const rpgo_pcm_problem* prob, const rpgo_pcm_params* params,
+                       int32_t* accepted, double* odom_dist, uint64_t* adj_bits, double* dist,
+                       rpgo_pcm_stats* stats) {
+  int rc = rpgo_pcm_upload(ctx, prob);
+  if (rc) return rc;
+  if ((rc = rpgo_pcm_run(ctx, params, stats))) return rc;
+  return rpgo_pcm_download(ctx, accepted, odom_dist, adj_bits, dist);
+}
+
+int rpgo_max_clique(rpgo_ctx* ctx, int32_t mode, int32_t* idx_out, int32_t* k_out,
+                    double* ms_out) {
+  if (!ctx || !k_out) {
+    set_error("rpgo_max_clique: NULL argument");
+    return RPGO_ERR_ARG;
+  }
+  PcmState& s = ctx->pcm;
+  if (!s.ran) {
+    set_error("rpgo_max_clique: no adjacency computed yet");
+    return RPGO_ERR_STATE;
+  }
+  RPGO_HIP_CHECK(hipSetDevice(ctx->device));
+  RPGO_HIP_CHECK(hipEventRecord(ctx->ev[5], ctx->stream));
+  int rc = clique_run(ctx, s.adj.as<uint64_t>(), s.m, mode, idx_out, k_out);
+  if (rc) return rc;
+  RPGO_HIP_CHECK(hipEventRecord(ctx->ev[6], ctx->stream));
+  RPGO_HIP_CHECK(hipEventSynchronize(ctx->ev[6]));
+  if (ms_out) *ms_out = elapsed(ctx->ev[5], ctx->ev[6]);
+  return RPGO_OK;
+}
+
+int rpgo_max_clique_bits(rpgo_ctx* ctx, int32_t n, const uint64_t* adj_bits, int32_t mode,
+                         int32_t* idx_out, int32_t* k_out) {
+  if (!ctx || !k_out || n < 0 || (n > 0 && !adj_bits)) {
+    set_error("rpgo_max_clique_bits: invalid argument");
+    return RPGO_ERR_ARG;
+  }
+  RPGO_HIP_CHECK(hipSetDevice(ctx->device));
+  const int W = (n + 63) / 64;
+  int rc = upload(ctx->clq.adj, adj_bits, sizeof(uint64_t) * (size_t)n * W, ctx->stream);
+  if (rc) return rc;
+  return clique_run(ctx, ctx->clq.adj.as<uint64_t>(), n, mode, idx_out, k_out);
+}
+
+}  // extern "C"

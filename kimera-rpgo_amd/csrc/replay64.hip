@@ -1,0 +1,355 @@
+// FP64 replay of Kimera-RPGO's PCM arithmetic on gfx950.
+//
+// This translation unit is compiled with -ffp-contract=off: every product and
+// sum below rounds exactly as written, in the same order as the reference's
+// dense Eigen expressions are evaluated by the restatement (dot products
+// summed k = 0..n-1).  It provides
+//   - the odometry fold        (Pcm.h:516-557, GeometryUtils.h:119-129)
+//   - the odometry check       (Pcm.h:604-629)
+//   - the pairwise check       (Pcm.h:670-718, GeometryUtils.h:133-186)
+//   - adjacency bit packing    (Pcm.h:725-768, row-packed and symmetrized)
+// and the device functions reused by the FP32 path's near-threshold recheck.
+#include "replay64.h"
+#include "rpgo_internal.h"
+
+namespace rpgo {
+
+// ---------------------------------------------------------------------------
+// per-measurement precompute: PoseWithCovariance(BetweenFactor) records and
+// the fold Jacobians H_k = Ad(delta_k^-1)          (GeometryUtils.h:91-129)
+// ---------------------------------------------------------------------------
+template <int DIM>
+__global__ void k_measurements(int n, const double* __restrict__ pose,
+                               const double* __restrict__ cov, double* __restrict__ pwc,
+                               double* __restrict__ hinv) {
+  constexpr int PL = R64<DIM>::PL, D = R64<DIM>::D, PWC = R64<DIM>::PWC;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double rec[PWC];
+  R64<DIM>::from_measurement(pose + (size_t)k * PL, cov + (size_t)k * D * D, rec);
+  for (int i = 0; i < PWC; ++i) pwc[(size_t)k * PWC + i] = rec[i];
+  if (hinv) {
+    double inv[PL], H[D * D];
+    R64<DIM>::pose_inverse(rec, inv);
+    R64<DIM>::adjoint(inv, H);
+    for (int i = 0; i < D * D; ++i) hinv[(size_t)k * D * D + i] = H[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// sequential fold, one wavefront:  T_{k+1} = T_k * delta_k,
+// Sigma_{k+1} = (H_k Sigma_k) H_k^T + Sigma_delta_k   (same order as the oracle)
+// lanes 0..D*D-1 own the covariance entries, the next lanes the pose entries.
+// ---------------------------------------------------------------------------
+template <int DIM>
+__global__ void __launch_bounds__(64) k_fold(int n_poses, const double* __restrict__ pose0,
+                                            const double* __restrict__ delta,
+                                            const double* __restrict__ hinv,
+                                            double* __restrict__ traj) {
+  constexpr int PL = R64<DIM>::PL, D = R64<DIM>::D, PWC = R64<DIM>::PWC, DD = D * D;
+  __shared__ double S[DD], T[DD], P[PL], Hs[DD], Dl[PWC];
+  __shared__ int ri;
+  const int l = threadIdx.x;
+  if (l < DD) S[l] = 0.0;
+  if (l < PL) P[l] = pose0[l];
+  if (l == 0) ri = 1;
+  __syncthreads();
+  // record 0
+  for (int e = l; e < PWC; e += 64)
+    traj[e] = e < PL ? P[e] : (e < PL + DD ? 0.0 : 1.0);
+  for (int k = 0; k + 1 < n_poses; ++k) {
+    for (int e = l; e < DD; e += 64) Hs[e] = hinv[(size_t)k * DD + e];
+    for (int e = l; e < PWC; e += 64) Dl[e] = delta[(size_t)k * PWC + e];
+    __syncthreads();
+    // T = H * Sigma
+    if (l < DD) {
+      const int r = l / D, c = l % D;
+      double s = Hs[r * D] * S[c];
+#pragma unroll
+      for (int m = 1; m < D; ++m) s += Hs[r * D + m] * S[m * D + c];
+      T[l] = s;
+    }
+    double newpose = 0.0;
+    if (l >= DD && l < DD + PL) newpose = R64<DIM>::compose_entry(P, Dl, l - DD);
+    __syncthreads();
+    double* out = traj + (size_t)(k + 1) * PWC;
+    if (l < DD) {
+      const int r = l / D, c = l % D;
+      double s = T[r * D] * Hs[c * D];
+#pragma unroll
+      for (int m = 1; m < D; ++m) s += T[r * D + m] * Hs[c * D + m];
+      s = s + Dl[PL + l];
+      S[l] = s;
+      out[PL + l] = s;
+    }
+    if (l >= DD && l < DD + PL) {
+      P[l - DD] = newpose;
+      out[l - DD] = newpose;
+    }
+    if (l == 0) {
+      ri = (ri && Dl[PL + DD] != 0.0) ? 1 : 0;
+      out[PL + DD] = ri ? 1.0 : 0.0;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// odometry check (Pcm.h:604-629) : one thread per loop closure
+// ---------------------------------------------------------------------------
+template <int DIM>
+__global__ void k_odom_check(int n_lc, const double* __restrict__ traj,
+                             const int* __restrict__ from, const int* __restrict__ to,
+                             const double* __restrict__ lcm, double thr,
+                             double* __restrict__ dist, int* __restrict__ flag) {
+  constexpr int PWC = R64<DIM>::PWC;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_lc) return;
+  double d = 0.0;
+  int ok = 1;
+  if (thr >= 0.0) {
+    d = R64<DIM>::odom_dist(traj + (size_t)from[k] * PWC, traj + (size_t)to[k] * PWC,
+                            lcm + (size_t)k * PWC);
+    ok = d < thr ? 1 : 0;
+  }
+  dist[k] = d;
+  flag[k] = ok;
+}
+
+// Stable compaction of accepted loop closures (single workgroup scan).
+// accepted[0..m-1] = indices with flag set, in input order; count at accepted[n].
+__global__ void __launch_bounds__(1024) k_compact(int n, const int* __restrict__ flag,
+                                                 int* __restrict__ accepted) {
+  __shared__ int warp_sums[16];
+  __shared__ int base;
+  if (threadIdx.x == 0) base = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int start = 0; start < n; start += 1024) {
+    const int k = start + threadIdx.x;
+    const int f = (k < n) ? flag[k] : 0;
+    const unsigned long long b = __ballot(f);
+    const int prefix = __popcll(b & ((1ull << lane) - 1ull));
+    if (lane == 0) warp_sums[wid] = __popcll(b);
+    __syncthreads();
+    int off = base;
+    for (int w = 0; w < wid; ++w) off += warp_sums[w];
+    if (f) accepted[off + prefix] = k;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int tot = 0;
+      for (int w = 0; w < 16; ++w) tot += warp_sums[w];
+      base += tot;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) accepted[n] = base;
+}
+
+template <int DIM>
+__global__ void k_gather(int m, const int* __restrict__ accepted, const int* __restrict__ from,
+                         const int* __restrict__ to, const double* __restrict__ lcm,
+                         int* __restrict__ cf, int* __restrict__ ct, double* __restrict__ clcm) {
+  constexpr int PWC = R64<DIM>::PWC;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m) return;
+  const int k = accepted[t];
+  cf[t] = from[k];
+  ct[t] = to[k];
+  for (int e = 0; e < PWC; ++e) clcm[(size_t)t * PWC + e] = lcm[(size_t)k * PWC + e];
+}
+
+// ---------------------------------------------------------------------------
+// pairwise FP64 replay.  One wavefront per (row j, word w) tile of the lower
+// triangle: lane = i - 64w, older LC i = (a, b), newer LC j = (c, d).
+// Tiles are enumerated by 64-row groups g: group g has 64 rows of g+1 words.
+// ---------------------------------------------------------------------------
+__device__ inline void tile_of(long long t, int* j, int* w) {
+  // largest g with 32 g (g+1) <= t
+  long long g = (long long)((sqrt(1.0 + (double)t / 8.0) - 1.0) * 0.5);
+  while (32ll * (g + 1) * (g + 2) <= t) ++g;
+  while (g > 0 && 32ll * g * (g + 1) > t) --g;
+  const long long rem = t - 32ll * g * (g + 1);
+  *j = (int)(64 * g + rem / (g + 1));
+  *w = (int)(rem % (g + 1));
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(256) k_pairs64(int m, long long n_tiles,
+                                                 const double* __restrict__ traj,
+                                                 const int* __restrict__ cf,
+                                                 const int* __restrict__ ct,
+                                                 const double* __restrict__ clcm, double thr,
+                                                 int W, unsigned long long* __restrict__ adj,
+                                                 double* __restrict__ dist,
+                                                 unsigned long long* __restrict__ n_consistent) {
+  constexpr int PWC = R64<DIM>::PWC;
+  const long long tile = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (tile >= n_tiles) return;
+  const int lane = threadIdx.x & 63;
+  int j, w;
+  tile_of(tile, &j, &w);
+  if (j >= m) return;
+  const int i = w * 64 + lane;
+  const bool active = i < j;
+  double d = 0.0;
+  if (active) {
+    d = R64<DIM>::pair_dist(traj, cf[i], ct[i], clcm + (size_t)i * PWC, cf[j], ct[j],
+                            clcm + (size_t)j * PWC);
+    if (dist) {
+      dist[(size_t)j * m + i] = d;
+      dist[(size_t)i * m + j] = d;
+    }
+  }
+  const bool cons = active && (d < thr);
+  const unsigned long long bits = __ballot(cons);
+  if (lane == 0) {
+    adj[(size_t)j * W + w] = bits;
+    if (bits) atomicAdd(n_consistent, (unsigned long long)__popcll(bits));
+  }
+}
+
+// Mirror the lower-triangle bits into the upper triangle: one wavefront per
+// 64x64 block (row block J, word W <= J); lane r holds row 64J + r.
+__global__ void __launch_bounds__(64) k_symmetrize(int m, int W, unsigned long long* adj) {
+  const int J = blockIdx.y, Wd = blockIdx.x;
+  if (Wd > J) return;
+  const int lane = threadIdx.x;
+  const int row = 64 * J + lane;
+  const unsigned long long word = row < m ? adj[(size_t)row * W + Wd] : 0ull;
+  unsigned long long mine = 0ull;
+  for (int c = 0; c < 64; ++c) {
+    const unsigned long long b = __ballot((word >> c) & 1ull);
+    if (lane == c) mine = b;
+  }
+  const int orow = 64 * Wd + lane;
+  if (orow < m) {
+    if (Wd == J) adj[(size_t)orow * W + J] |= mine;
+    else adj[(size_t)orow * W + J] = mine;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host drivers
+// ---------------------------------------------------------------------------
+int pcm_prepare(rpgo_ctx* ctx) {
+  PcmState& s = ctx->pcm;
+  const int nd = s.n_poses - 1;
+  const int pwc = pwc_len(s.dim), D = tan_dim(s.dim);
+  int rc;
+  if ((rc = s.delta.reserve(sizeof(double) * (size_t)(nd > 0 ? nd : 1) * pwc))) return rc;
+  if ((rc = s.hinv.reserve(sizeof(double) * (size_t)(nd > 0 ? nd : 1) * D * D))) return rc;
+  if ((rc = s.lcm.reserve(sizeof(double) * (size_t)(s.n_lc > 0 ? s.n_lc : 1) * pwc))) return rc;
+  if (nd > 0) {
+    if (s.dim == 3)
+      k_measurements<3><<<(nd + 255) / 256, 256, 0, ctx->stream>>>(
+          nd, s.odom.as<double>(), s.odom_cov.as<double>(), s.delta.as<double>(), s.hinv.as<double>());
+    else
+      k_measurements<2><<<(nd + 255) / 256, 256, 0, ctx->stream>>>(
+          nd, s.odom.as<double>(), s.odom_cov.as<double>(), s.delta.as<double>(), s.hinv.as<double>());
+    RPGO_HIP_CHECK(hipGetLastError());
+  }
+  if (s.n_lc > 0) {
+    if (s.dim == 3)
+      k_measurements<3><<<(s.n_lc + 255) / 256, 256, 0, ctx->stream>>>(
+          s.n_lc, s.lc_pose.as<double>(), s.lc_cov.as<double>(), s.lcm.as<double>(), nullptr);
+    else
+      k_measurements<2><<<(s.n_lc + 255) / 256, 256, 0, ctx->stream>>>(
+          s.n_lc, s.lc_pose.as<double>(), s.lc_cov.as<double>(), s.lcm.as<double>(), nullptr);
+    RPGO_HIP_CHECK(hipGetLastError());
+  }
+  return RPGO_OK;
+}
+
+int pcm_fold(rpgo_ctx* ctx) {
+  PcmState& s = ctx->pcm;
+  int rc;
+  if ((rc = s.traj.reserve(sizeof(double) * (size_t)s.n_poses * pwc_len(s.dim)))) return rc;
+  if (s.dim == 3)
+    k_fold<3><<<1, 64, 0, ctx->stream>>>(s.n_poses, s.pose0.as<double>(), s.delta.as<double>(),
+                                         s.hinv.as<double>(), s.traj.as<double>());
+  else
+    k_fold<2><<<1, 64, 0, ctx->stream>>>(s.n_poses, s.pose0.as<double>(), s.delta.as<double>(),
+                                         s.hinv.as<double>(), s.traj.as<double>());
+  RPGO_HIP_CHECK(hipGetLastError());
+  return RPGO_OK;
+}
+
+int pcm_odom_and_compact(rpgo_ctx* ctx, double odom_thr, int* m_out) {
+  PcmState& s = ctx->pcm;
+  const int n = s.n_lc;
+  int rc;
+  const int pwc = pwc_len(s.dim);
+  if ((rc = s.odom_dist.reserve(sizeof(double) * (size_t)(n + 1)))) return rc;
+  if ((rc = s.flags.reserve(sizeof(int) * (size_t)(n + 1)))) return rc;
+  if ((rc = s.accepted.reserve(sizeof(int) * (size_t)(n + 1)))) return rc;
+  if ((rc = s.c_from.reserve(sizeof(int) * (size_t)(n + 1)))) return rc;
+  if ((rc = s.c_to.reserve(sizeof(int) * (size_t)(n + 1)))) return rc;
+  if ((rc = s.c_lcm.reserve(sizeof(double) * (size_t)(n + 1) * pwc))) return rc;
+  if (n == 0) {
+    *m_out = 0;
+    return RPGO_OK;
+  }
+  if (s.dim == 3)
+    k_odom_check<3><<<(n + 255) / 256, 256, 0, ctx->stream>>>(
+        n, s.traj.as<double>(), s.lc_from.as<int>(), s.lc_to.as<int>(), s.lcm.as<double>(),
+        odom_thr, s.odom_dist.as<double>(), s.flags.as<int>());
+  else
+    k_odom_check<2><<<(n + 255) / 256, 256, 0, ctx->stream>>>(
+        n, s.traj.as<double>(), s.lc_from.as<int>(), s.lc_to.as<int>(), s.lcm.as<double>(),
+        odom_thr, s.odom_dist.as<double>(), s.flags.as<int>());
+  RPGO_HIP_CHECK(hipGetLastError());
+  k_compact<<<1, 1024, 0, ctx->stream>>>(n, s.flags.as<int>(), s.accepted.as<int>());
+  RPGO_HIP_CHECK(hipGetLastError());
+  int m = 0;
+  RPGO_HIP_CHECK(hipMemcpyAsync(&m, s.accepted.as<int>() + n, sizeof(int), hipMemcpyDeviceToHost,
+                                ctx->stream));
+  RPGO_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  if (m > 0) {
+    if (s.dim == 3)
+      k_gather<3><<<(m + 255) / 256, 256, 0, ctx->stream>>>(
+          m, s.accepted.as<int>(), s.lc_from.as<int>(), s.lc_to.as<int>(), s.lcm.as<double>(),
+          s.c_from.as<int>(), s.c_to.as<int>(), s.c_lcm.as<double>());
+    else
+      k_gather<2><<<(m + 255) / 256, 256, 0, ctx->stream>>>(
+          m, s.accepted.as<int>(), s.lc_from.as<int>(), s.lc_to.as<int>(), s.lcm.as<double>(),
+          s.c_from.as<int>(), s.c_to.as<int>(), s.c_lcm.as<double>());
+    RPGO_HIP_CHECK(hipGetLastError());
+  }
+  *m_out = m;
+  return RPGO_OK;
+}
+
+int pcm_pairs_fp64(rpgo_ctx* ctx, double lc_thr, bool want_dist) {
+  PcmState& s = ctx->pcm;
+  const int m = s.m;
+  if (m < 2) return RPGO_OK;
+  const int W = (m + 63) / 64;
+  const long long G = W;
+  const long long n_tiles = 32ll * G * (G + 1);
+  const long long blocks = (n_tiles + 3) / 4;
+  if (s.dim == 3)
+    k_pairs64<3><<<(unsigned)blocks, 256, 0, ctx->stream>>>(
+        m, n_tiles, s.traj.as<double>(), s.c_from.as<int>(), s.c_to.as<int>(),
+        s.c_lcm.as<double>(), lc_thr, W, s.adj.as<unsigned long long>(),
+        want_dist ? s.dist.as<double>() : nullptr, s.counters.as<unsigned long long>());
+  else
+    k_pairs64<2><<<(unsigned)blocks, 256, 0, ctx->stream>>>(
+        m, n_tiles, s.traj.as<double>(), s.c_from.as<int>(), s.c_to.as<int>(),
+        s.c_lcm.as<double>(), lc_thr, W, s.adj.as<unsigned long long>(),
+        want_dist ? s.dist.as<double>() : nullptr, s.counters.as<unsigned long long>());
+  RPGO_HIP_CHECK(hipGetLastError());
+  return RPGO_OK;
+}
+
+int pcm_symmetrize(rpgo_ctx* ctx) {
+  PcmState& s = ctx->pcm;
+  const int m = s.m;
+  if (m < 2) return RPGO_OK;
+  const int W = (m + 63) / 64;
+  k_symmetrize<<<dim3(W, W), 64, 0, ctx->stream>>>(m, W, s.adj.as<unsigned long long>());
+  RPGO_HIP_CHECK(hipGetLastError());
+  return RPGO_OK;
+}
+
+}  // namespace rpgo

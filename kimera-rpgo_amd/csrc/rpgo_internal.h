@@ -1,0 +1,113 @@
+// Internal declarations shared by the HIP translation units of librpgo_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "rpgo_hip.h"
+
+namespace rpgo {
+
+void set_error(const char* fmt, ...);
+
+#define RPGO_HIP_CHECK(expr)                                                   \
+  do {                                                                         \
+    hipError_t _e = (expr);                                                    \
+    if (_e != hipSuccess) {                                                    \
+      ::rpgo::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+                        __FILE__, __LINE__);                                   \
+      return RPGO_ERR_HIP;                                                     \
+    }                                                                          \
+  } while (0)
+
+// Grow-only device buffer.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  int reserve(size_t n) {
+    if (n <= bytes) return RPGO_OK;
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+    size_t want = n < 256 ? 256 : n;
+    hipError_t e = hipMalloc(&ptr, want);
+    if (e != hipSuccess) {
+      set_error("hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+      ptr = nullptr;
+      return RPGO_ERR_NOMEM;
+    }
+    bytes = want;
+    return RPGO_OK;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(ptr); }
+};
+
+inline int pose_len(int dim) { return dim == 3 ? 12 : 4; }
+inline int tan_dim(int dim) { return dim == 3 ? 6 : 3; }
+// PoseWithCovariance record: pose, covariance, rotation_info flag (as double)
+inline int pwc_len(int dim) { return pose_len(dim) + tan_dim(dim) * tan_dim(dim) + 1; }
+
+struct PcmState {
+  int dim = 0;
+  int n_poses = 0;
+  int n_lc = 0;
+  bool uploaded = false;
+  bool ran = false;
+  int m = 0;  // accepted loop closures of the last run
+  bool have_dist = false;
+  // inputs
+  DevBuf pose0, odom, odom_cov, lc_from, lc_to, lc_pose, lc_cov;
+  // derived
+  DevBuf delta;      // n_poses-1 pwc records of the odometry measurements
+  DevBuf hinv;       // n_poses-1 Ad(delta^-1) matrices
+  DevBuf traj;       // n_poses pwc records (FP64 fold)
+  DevBuf lcm;        // n_lc pwc records of the loop-closure measurements
+  DevBuf odom_dist;  // n_lc doubles
+  DevBuf flags;      // n_lc ints
+  DevBuf accepted;   // n_lc ints (+1 for the count)
+  DevBuf c_from, c_to, c_lcm;  // compacted accepted loop closures
+  DevBuf adj;        // m*W uint64
+  DevBuf dist;       // m*m double
+  DevBuf counters;   // device counters (recheck, flips, consistent)
+  DevBuf fp32_buf;   // FP32-mode per-pose / per-LC tables
+  DevBuf queue;      // FP32-mode recheck queue
+};
+
+struct CliqueState {
+  DevBuf adj;      // caller-provided adjacency (rpgo_max_clique_bits)
+  DevBuf deg, dmask, icc, out, scratch, best;
+};
+
+}  // namespace rpgo
+
+struct rpgo_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[8] = {};
+  rpgo::PcmState pcm;
+  rpgo::CliqueState clq;
+};
+
+namespace rpgo {
+// replay64.hip
+int pcm_prepare(rpgo_ctx* ctx);                  // delta/hinv/lcm precompute
+int pcm_fold(rpgo_ctx* ctx);                     // sequential FP64 fold
+int pcm_odom_and_compact(rpgo_ctx* ctx, double odom_thr, int* m_out);
+int pcm_pairs_fp64(rpgo_ctx* ctx, double lc_thr, bool want_dist);
+int pcm_symmetrize(rpgo_ctx* ctx);
+// pcm_fp32.hip
+int pcm_pairs_fp32(rpgo_ctx* ctx, double lc_thr, double margin, bool want_dist,
+                   int64_t* n_recheck, int64_t* n_flip, float* ms_pairs, float* ms_recheck);
+// clique.hip
+int clique_run(rpgo_ctx* ctx, const uint64_t* d_adj, int n, int mode, int32_t* idx_out,
+               int32_t* k_out);
+}  // namespace rpgo

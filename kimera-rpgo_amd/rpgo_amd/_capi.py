@@ -1,0 +1,201 @@
+"""ctypes binding of librpgo_hip.so (the C ABI in include/rpgo_hip.h).
+
+The HIP library is the only compute path: loading fails loudly when the
+built library is missing, and nothing here computes on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG, "lib", "librpgo_hip.so")
+
+RPGO_OK = 0
+PCM_FP64_REPLAY = 0
+PCM_FP32_RECHECK = 1
+CLIQUE_HEU_REF = 0
+CLIQUE_EXACT = 1
+
+_d = ctypes.POINTER(ctypes.c_double)
+_i32 = ctypes.POINTER(ctypes.c_int32)
+_u64 = ctypes.POINTER(ctypes.c_uint64)
+
+
+class PcmProblem(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int32), ("n_poses", ctypes.c_int32), ("pose0", _d),
+                ("odom", _d), ("odom_cov", _d), ("n_lc", ctypes.c_int32), ("lc_from", _i32),
+                ("lc_to", _i32), ("lc_pose", _d), ("lc_cov", _d)]
+
+
+class PcmParams(ctypes.Structure):
+    _fields_ = [("odom_threshold", ctypes.c_double), ("lc_threshold", ctypes.c_double),
+                ("precision", ctypes.c_int32), ("want_dist", ctypes.c_int32),
+                ("recheck_margin", ctypes.c_double)]
+
+
+class PcmStats(ctypes.Structure):
+    _fields_ = [("n_accepted", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("n_pairs", ctypes.c_int64), ("n_rechecked", ctypes.c_int64),
+                ("n_flipped", ctypes.c_int64), ("n_consistent", ctypes.c_int64),
+                ("ms_fold", ctypes.c_double), ("ms_odom", ctypes.c_double),
+                ("ms_pairs", ctypes.c_double), ("ms_recheck", ctypes.c_double),
+                ("ms_total", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+# (name, argtypes, restype) of every exported entry point
+SIGNATURES = [
+    ("rpgo_ctx_create", [ctypes.c_int, ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+    ("rpgo_ctx_destroy", [ctypes.c_void_p], ctypes.c_int),
+    ("rpgo_ctx_sync", [ctypes.c_void_p], ctypes.c_int),
+    ("rpgo_last_error", [], ctypes.c_char_p),
+    ("rpgo_version", [], ctypes.c_char_p),
+    ("rpgo_pcm_upload", [ctypes.c_void_p, ctypes.POINTER(PcmProblem)], ctypes.c_int),
+    ("rpgo_pcm_run", [ctypes.c_void_p, ctypes.POINTER(PcmParams), ctypes.POINTER(PcmStats)], ctypes.c_int),
+    ("rpgo_pcm_download", [ctypes.c_void_p, _i32, _d, _u64, _d], ctypes.c_int),
+    ("rpgo_pcm_adjacency", [ctypes.c_void_p, ctypes.POINTER(PcmProblem), ctypes.POINTER(PcmParams),
+                            _i32, _d, _u64, _d, ctypes.POINTER(PcmStats)], ctypes.c_int),
+    ("rpgo_pcm_download_traj", [ctypes.c_void_p, _d], ctypes.c_int),
+    ("rpgo_max_clique", [ctypes.c_void_p, ctypes.c_int32, _i32, _i32, _d], ctypes.c_int),
+    ("rpgo_max_clique_bits", [ctypes.c_void_p, ctypes.c_int32, _u64, ctypes.c_int32, _i32, _i32],
+     ctypes.c_int),
+]
+
+_lib = None
+
+
+def lib():
+    """Load librpgo_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with kimera-rpgo_amd/build.py "
+                               "(the HIP library is the only compute path)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, argt, rest in SIGNATURES:
+            f = getattr(L, name)
+            f.argtypes = argt
+            f.restype = rest
+        _lib = L
+    return _lib
+
+
+class RpgoError(RuntimeError):
+    pass
+
+
+def check(rc):
+    if rc != RPGO_OK:
+        msg = lib().rpgo_last_error()
+        raise RpgoError(f"rpgo status {rc}: {msg.decode() if msg else ''}")
+
+
+def _ptr(a, t=_d):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+class Context:
+    """Owns an rpgo_ctx (device memory + one HIP stream)."""
+
+    def __init__(self, device: int = 0):
+        self._h = ctypes.c_void_p()
+        check(lib().rpgo_ctx_create(device, 0, ctypes.byref(self._h)))
+        self._keep = []
+
+    def close(self):
+        if self._h:
+            lib().rpgo_ctx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------- PCM
+    def pcm_upload(self, dim, pose0, odom, odom_cov, lc_from, lc_to, lc_pose, lc_cov):
+        f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+        arrs = dict(pose0=f64(pose0), odom=f64(odom), odom_cov=f64(odom_cov), lc_from=i32(lc_from),
+                    lc_to=i32(lc_to), lc_pose=f64(lc_pose), lc_cov=f64(lc_cov))
+        n_poses = arrs["odom"].reshape(-1, 12 if dim == 3 else 4).shape[0] + 1
+        p = PcmProblem(dim=dim, n_poses=n_poses, pose0=_ptr(arrs["pose0"]), odom=_ptr(arrs["odom"]),
+                       odom_cov=_ptr(arrs["odom_cov"]), n_lc=arrs["lc_from"].shape[0],
+                       lc_from=_ptr(arrs["lc_from"], _i32), lc_to=_ptr(arrs["lc_to"], _i32),
+                       lc_pose=_ptr(arrs["lc_pose"]), lc_cov=_ptr(arrs["lc_cov"]))
+        check(lib().rpgo_pcm_upload(self._h, ctypes.byref(p)))
+        self._n_lc = arrs["lc_from"].shape[0]
+        self._n_poses = n_poses
+        self._dim = dim
+
+    def pcm_run(self, odom_thr, lc_thr, precision=PCM_FP64_REPLAY, want_dist=False, margin=0.0):
+        prm = PcmParams(odom_threshold=odom_thr, lc_threshold=lc_thr, precision=precision,
+                        want_dist=1 if want_dist else 0, recheck_margin=margin)
+        st = PcmStats()
+        check(lib().rpgo_pcm_run(self._h, ctypes.byref(prm), ctypes.byref(st)))
+        self._m = st.n_accepted
+        return st
+
+    def pcm_download(self, want_dist=False):
+        m, n = self._m, self._n_lc
+        W = (m + 63) // 64
+        acc = np.zeros(max(n, 1), np.int32)
+        od = np.zeros(max(n, 1))
+        adj = np.zeros(max(m * W, 1), np.uint64)
+        dist = np.zeros(max(m * m, 1)) if want_dist else None
+        check(lib().rpgo_pcm_download(self._h, _ptr(acc, _i32), _ptr(od), _ptr(adj, _u64), _ptr(dist)))
+        out = dict(accepted=acc[:m].copy(), odom_dist=od[:n].copy(),
+                   adj_bits=adj[: m * W].reshape(m, W).copy())
+        if want_dist:
+            out["dist"] = dist[: m * m].reshape(m, m).copy()
+        return out
+
+    def pcm_traj(self):
+        pl, d = (12, 6) if self._dim == 3 else (4, 3)
+        t = np.zeros((self._n_poses, pl + d * d + 1))
+        check(lib().rpgo_pcm_download_traj(self._h, _ptr(t)))
+        return t
+
+    def max_clique(self, mode=CLIQUE_HEU_REF):
+        idx = np.zeros(max(self._m, 1), np.int32)
+        k = ctypes.c_int32()
+        ms = ctypes.c_double()
+        check(lib().rpgo_max_clique(self._h, mode, _ptr(idx, _i32), ctypes.byref(k), ctypes.byref(ms)))
+        return int(k.value), idx[: max(k.value, 0)].copy(), ms.value
+
+    def max_clique_bits(self, adj_bits, n, mode=CLIQUE_HEU_REF):
+        a = np.ascontiguousarray(adj_bits, dtype=np.uint64).reshape(-1)
+        idx = np.zeros(max(n, 1), np.int32)
+        k = ctypes.c_int32()
+        check(lib().rpgo_max_clique_bits(self._h, n, _ptr(a, _u64), mode, _ptr(idx, _i32), ctypes.byref(k)))
+        return int(k.value), idx[: max(k.value, 0)].copy()
+
+
+def unpack_bits(adj_bits, m):
+    """(m, W) uint64 rows -> (m, m) uint8 matrix."""
+    a = np.ascontiguousarray(adj_bits, dtype=np.uint64).reshape(m, -1)
+    b = np.unpackbits(a.view(np.uint8), axis=1, bitorder="little")
+    return b[:, :m].copy()
+
+
+def pack_bits(adj):
+    """(m, m) 0/1 matrix -> (m, W) uint64 rows (bit i&63 of word i>>6)."""
+    adj = np.asarray(adj, dtype=np.uint8)
+    m = adj.shape[0]
+    W = (m + 63) // 64
+    pad = np.zeros((m, W * 64), np.uint8)
+    pad[:, :m] = adj
+    return np.packbits(pad, axis=1, bitorder="little").view(np.uint64).reshape(m, W)

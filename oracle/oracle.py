@@ -72,6 +72,7 @@ def lib():
             ("orc_max_clique_heu_incremental", [ctypes.c_int, _i32, _i32, ctypes.c_int,
                                                 ctypes.c_int, _i32], ctypes.c_int),
             ("orc_max_clique_exact", [ctypes.c_int, _i32, _i32, _i32], ctypes.c_int),
+            ("orc_max_clique_heu_dense", [ctypes.c_int, _u8, _i32], ctypes.c_int),
         ]:
             f = getattr(L, name)
             f.argtypes = argt
@@ -279,4 +280,13 @@ def max_clique_exact(adj):
     out = np.zeros(max(n, 1), dtype=np.int32)
     e = np.ascontiguousarray(edg) if edg.size else np.zeros(1, np.int32)
     k = lib().orc_max_clique_exact(n, _p(off, _i32), _p(e, _i32), _p(out, _i32))
+    return int(k), out[: max(k, 0)].copy()
+
+
+def max_clique_heu_dense(adj):
+    """maxCliqueHeu semantics evaluated with bitset neighbourhood tests."""
+    adj = np.ascontiguousarray(adj, dtype=np.uint8)
+    n = adj.shape[0]
+    out = np.zeros(max(n, 1), dtype=np.int32)
+    k = lib().orc_max_clique_heu_dense(n, _p(adj, _u8), _p(out, _i32))
     return int(k), out[: max(k, 0)].copy()

@@ -743,3 +743,82 @@ int orc_max_clique_exact(int n, const int32_t* off, const int32_t* edg, int32_t*
   free(inter.data);
   return maxClq;
 }
+
+/* Same semantics as orc_max_clique_heu (findCliqueHeu.cpp:32-117) on a dense
+ * byte adjacency, with neighbourhood tests through a bitset instead of the
+ * CSR scan.  Candidate greedy sizes are computed on bitsets; the winning
+ * candidate is then replayed with the list-based loop to recover the stale
+ * buffer it leaves behind. */
+static int heu_winner_replay(int n, const uint64_t* bits, int W, const int* degv, int c,
+                             int best, int32_t* out) {
+  int* S = (int*)malloc(sizeof(int) * (size_t)(n + 1));
+  int* S1 = (int*)malloc(sizeof(int) * (size_t)(n + 1));
+  int iPos = 0;
+  S[iPos++] = c;
+  for (int u = 0; u < n; ++u)
+    if (u != c && ((bits[(size_t)c * W + (u >> 6)] >> (u & 63)) & 1ull) && best <= degv[u])
+      S[iPos++] = u;
+  int icc = 0;
+  int written = iPos;
+  for (int j = 0; j < iPos; ++j) out[j] = S[j];
+  while (iPos > 0) {
+    ++icc;
+    const int m = S[iPos - 1];
+    int iPos1 = 0;
+    for (int j = 0; j < iPos; ++j) {
+      const int v = S[j];
+      if (((bits[(size_t)m * W + (v >> 6)] >> (v & 63)) & 1ull) && best <= degv[v]) S1[iPos1++] = v;
+    }
+    for (int j = 0; j < iPos1; ++j) { S[j] = S1[j]; out[j] = S1[j]; }
+    iPos = iPos1;
+  }
+  (void)written;
+  free(S);
+  free(S1);
+  return icc;
+}
+
+int orc_max_clique_heu_dense(int n, const uint8_t* adj, int32_t* out) {
+  if (n <= 0) return -1;
+  const int W = (n + 63) / 64;
+  uint64_t* bits = (uint64_t*)calloc((size_t)n * W, sizeof(uint64_t));
+  int* degv = (int*)calloc((size_t)n, sizeof(int));
+  for (int v = 0; v < n; ++v)
+    for (int u = 0; u < n; ++u) {
+      if (u == v) continue;
+      const int r = u > v ? u : v, cc = u > v ? v : u;
+      if (adj[(size_t)r * n + cc]) {
+        bits[(size_t)v * W + (u >> 6)] |= 1ull << (u & 63);
+        ++degv[v];
+      }
+    }
+  uint64_t* S = (uint64_t*)malloc(sizeof(uint64_t) * W);
+  uint64_t* D = (uint64_t*)malloc(sizeof(uint64_t) * W);
+  int best = -1, win = -1, win_best = -1;
+  for (int c = 0; c < n; ++c) {
+    if (best > degv[c]) continue;
+    /* D = { u : deg(u) >= best } */
+    for (int w = 0; w < W; ++w) D[w] = 0;
+    for (int u = 0; u < n; ++u)
+      if (degv[u] >= best) D[u >> 6] |= 1ull << (u & 63);
+    for (int w = 0; w < W; ++w) S[w] = bits[(size_t)c * W + w] & D[w];
+    int icc = 0;
+    for (;;) {
+      ++icc;
+      int top = -1;
+      for (int w = W - 1; w >= 0; --w)
+        if (S[w]) { top = w * 64 + 63 - __builtin_clzll(S[w]); break; }
+      if (top < 0) break; /* only the seed c was left: it is removed, loop ends */
+      const uint64_t* Nm = bits + (size_t)top * W;
+      for (int w = 0; w < W; ++w) S[w] &= Nm[w];
+    }
+    if (best < icc) {
+      win = c;
+      win_best = best;
+      best = icc;
+    }
+  }
+  if (win >= 0) heu_winner_replay(n, bits, W, degv, win, win_best, out);
+  free(S); free(D); free(bits); free(degv);
+  return best;
+}

@@ -103,6 +103,10 @@ int orc_csr_from_dense(int n, const uint8_t* adj, int32_t* offsets,
 int orc_max_clique_heu(int n, const int32_t* offsets, const int32_t* edges,
                        int32_t* out);
 
+/* maxCliqueHeu semantics on a dense byte adjacency (bitset neighbourhood
+ * tests; faster on dense inlier blocks).  out: n ints. */
+int orc_max_clique_heu_dense(int n, const uint8_t* adj, int32_t* out);
+
 /* Incremental heuristic (findCliqueHeu.cpp:119-209, GraphUtils.cpp:30-44):
  * returns 0 when the previous set should be kept. */
 int orc_max_clique_heu_incremental(int n, const int32_t* offsets,

@@ -1,0 +1,49 @@
+"""CPU checks of the C ABI library: it loads and exports every symbol that
+include/rpgo_hip.h declares (no compute calls: there is no GPU here)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+from rpgo_amd import _capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", fn)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            names |= set(re.findall(r"\b(rpgo_[a-z0-9_]+)\s*\(", txt))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = _capi.lib()
+    declared = _declared()
+    assert len(declared) >= 12
+    for name in sorted(declared):
+        assert hasattr(L, name), name
+    bound = {n for n, _, _ in _capi.SIGNATURES}
+    assert declared <= bound, declared - bound
+
+
+def test_version_and_error_string_without_gpu():
+    L = _capi.lib()
+    assert L.rpgo_version().decode().startswith("rpgo-mi355x")
+    assert L.rpgo_last_error() is not None
+
+
+def test_bit_packing_roundtrip():
+    rng = np.random.default_rng(0)
+    for m in (1, 5, 63, 64, 65, 200):
+        A = (rng.uniform(size=(m, m)) < 0.3).astype(np.uint8)
+        bits = _capi.pack_bits(A)
+        assert bits.shape == (m, (m + 63) // 64)
+        assert np.array_equal(_capi.unpack_bits(bits, m), A)
+        # bit (i & 63) of word i >> 6
+        j, i = (int(v) for v in rng.integers(0, m, 2))
+        assert ((int(bits[j, i >> 6]) >> (i & 63)) & 1) == A[j, i]

@@ -334,3 +334,18 @@ def test_heu_greedy_size_matches_python_restatement():
                 best, out = icc, list(buf)
         k, got = O.max_clique_heu(A)
         assert k == best and got.tolist() == out[:best]
+
+
+def test_heu_dense_matches_literal():
+    """The bitset evaluation of maxCliqueHeu equals the literal CSR replay."""
+    rng = np.random.default_rng(13)
+    for trial in range(60):
+        n = int(rng.integers(1, 120))
+        A = (rng.uniform(size=(n, n)) < rng.uniform(0.05, 0.95)).astype(np.uint8)
+        # plant a dense block so the greedy runs deep
+        blk = rng.choice(n, size=min(n, int(rng.integers(1, 40))), replace=False)
+        A[np.ix_(blk, blk)] = (rng.uniform(size=(len(blk), len(blk))) < 0.97)
+        A = np.triu(A, 1); A = A + A.T
+        k1, s1 = O.max_clique_heu(A)
+        k2, s2 = O.max_clique_heu_dense(A)
+        assert k1 == k2 and s1.tolist() == s2.tolist()
