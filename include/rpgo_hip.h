@@ -103,6 +103,11 @@ int rpgo_pcm_upload(rpgo_ctx* ctx, const rpgo_pcm_problem* prob);
 /* Run the adjacency stage on the uploaded problem, device-resident; blocks
  * until done and fills *stats (may be NULL). */
 int rpgo_pcm_run(rpgo_ctx* ctx, const rpgo_pcm_params* params, rpgo_pcm_stats* stats);
+/* Recompute only the pairwise stage -- every pair check of the accepted loop
+ * closures and the adjacency fill (Pcm.h:725-768 / areLoopsConsistent
+ * :670-718) -- on the device-resident trajectory of the last rpgo_pcm_run,
+ * e.g. with another threshold or precision.  Blocks; fills *stats. */
+int rpgo_pcm_pairs(rpgo_ctx* ctx, const rpgo_pcm_params* params, rpgo_pcm_stats* stats);
 /* Fetch results of the last run.  accepted: n_lc ints (indices of accepted loop
  * closures, in input order; n_accepted valid).  odom_dist: n_lc doubles or NULL.
  * adj_bits: m*W words or NULL.  dist: m*m doubles or NULL (needs want_dist). */

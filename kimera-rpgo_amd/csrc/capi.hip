@@ -146,6 +146,77 @@ int rpgo_pcm_upload(rpgo_ctx* ctx, const rpgo_pcm_problem* p) {
   return RPGO_OK;
 }
 
+// The pairwise stage (A3 + A4 of the reference: areLoopsConsistent over every
+// pair, adjacency fill) on the accepted set of the last run.  Events ev[8..11]
+// bracket the pair kernel and the recheck on the context stream.
+static int pairs_pass(rpgo_ctx* ctx, const rpgo_pcm_params* prm, rpgo_pcm_stats* stats) {
+  PcmState& s = ctx->pcm;
+  hipStream_t st = ctx->stream;
+  hipEvent_t* ev = ctx->ev;
+  const int m = s.m, W = (m + 63) / 64;
+  const bool want_dist = prm->want_dist != 0;
+  const bool check = prm->lc_threshold >= 0.0;
+  int rc;
+  if ((rc = s.adj.reserve(sizeof(uint64_t) * (size_t)(m > 0 ? m : 1) * (W > 0 ? W : 1)))) return rc;
+  if (want_dist && (rc = s.dist.reserve(sizeof(double) * (size_t)(m > 0 ? m : 1) * (m > 0 ? m : 1))))
+    return rc;
+  RPGO_HIP_CHECK(hipMemsetAsync(s.counters.ptr, 0, 128, st));
+  if (m > 0) RPGO_HIP_CHECK(hipMemsetAsync(s.adj.ptr, 0, sizeof(uint64_t) * (size_t)m * W, st));
+  if (want_dist && m > 0)
+    RPGO_HIP_CHECK(hipMemsetAsync(s.dist.ptr, 0, sizeof(double) * (size_t)m * m, st));
+  const bool fp32 = prm->precision == RPGO_PCM_FP32_RECHECK && !s.fp32_fallback;
+  int64_t n_recheck = 0;
+  RPGO_HIP_CHECK(hipEventRecord(ev[8], st));
+  RPGO_HIP_CHECK(hipEventRecord(ev[9], st));
+  RPGO_HIP_CHECK(hipEventRecord(ev[10], st));
+  if (check && m > 1) {
+    if (!fp32) {
+      if ((rc = pcm_pairs_fp64(ctx, prm->lc_threshold, want_dist))) return rc;
+      if ((rc = pcm_symmetrize(ctx))) return rc;
+      RPGO_HIP_CHECK(hipEventRecord(ev[9], st));
+      RPGO_HIP_CHECK(hipEventRecord(ev[10], st));
+    } else {
+      if ((rc = pcm_pairs_fp32_launch(ctx, prm->lc_threshold, prm->recheck_margin, want_dist)))
+        return rc;
+      RPGO_HIP_CHECK(hipEventRecord(ev[9], st));
+      unsigned int nq = 0;
+      RPGO_HIP_CHECK(hipMemcpyAsync(&nq, s.counters.as<unsigned int>() + 16, sizeof(nq),
+                                    hipMemcpyDeviceToHost, st));
+      RPGO_HIP_CHECK(hipStreamSynchronize(st));
+      if (nq > (unsigned)kRecheckQueueCap) {
+        // band too crowded for the queue: replay every pair in FP64
+        RPGO_HIP_CHECK(hipMemsetAsync(s.adj.ptr, 0, sizeof(uint64_t) * (size_t)m * W, st));
+        if ((rc = pcm_pairs_fp64(ctx, prm->lc_threshold, want_dist))) return rc;
+        if ((rc = pcm_symmetrize(ctx))) return rc;
+        n_recheck = (int64_t)m * (m - 1) / 2;
+      } else {
+        if ((rc = pcm_recheck64(ctx, (int)nq, prm->lc_threshold, want_dist))) return rc;
+        n_recheck = nq;
+      }
+      RPGO_HIP_CHECK(hipEventRecord(ev[10], st));
+    }
+    if ((rc = pcm_count_consistent(ctx))) return rc;
+  }
+  RPGO_HIP_CHECK(hipEventRecord(ev[11], st));
+  RPGO_HIP_CHECK(hipEventSynchronize(ev[11]));
+  s.ran = true;
+  s.have_dist = want_dist;
+  s.precision = prm->precision;
+  if (stats) {
+    unsigned long long cnt[4] = {0, 0, 0, 0};
+    RPGO_HIP_CHECK(hipMemcpy(cnt, s.counters.ptr, sizeof(cnt), hipMemcpyDeviceToHost));
+    stats->n_accepted = m;
+    stats->reserved = 0;
+    stats->n_pairs = check ? (int64_t)m * (m - 1) / 2 : 0;
+    stats->n_rechecked = n_recheck;
+    stats->n_flipped = (int64_t)cnt[2];
+    stats->n_consistent = (int64_t)(cnt[1] / 2);
+    stats->ms_pairs = elapsed(ev[8], ev[9]);
+    stats->ms_recheck = elapsed(ev[9], ev[10]);
+  }
+  return RPGO_OK;
+}
+
 int rpgo_pcm_run(rpgo_ctx* ctx, const rpgo_pcm_params* prm, rpgo_pcm_stats* stats) {
   if (!ctx || !prm) {
     set_error("rpgo_pcm_run: NULL argument");
@@ -173,51 +244,53 @@ int rpgo_pcm_run(rpgo_ctx* ctx, const rpgo_pcm_params* prm, rpgo_pcm_stats* stat
   if ((rc = pcm_odom_and_compact(ctx, prm->odom_threshold, &m))) return rc;
   RPGO_HIP_CHECK(hipEventRecord(ev[2], st));
   s.m = m;
-  const int W = (m + 63) / 64;
-  const bool want_dist = prm->want_dist != 0;
-  if ((rc = s.adj.reserve(sizeof(uint64_t) * (size_t)(m > 0 ? m : 1) * (W > 0 ? W : 1)))) return rc;
-  if (want_dist && (rc = s.dist.reserve(sizeof(double) * (size_t)(m > 0 ? m : 1) * (m > 0 ? m : 1))))
-    return rc;
-  if (m > 0) RPGO_HIP_CHECK(hipMemsetAsync(s.adj.ptr, 0, sizeof(uint64_t) * (size_t)m * W, st));
-  if (want_dist && m > 0)
-    RPGO_HIP_CHECK(hipMemsetAsync(s.dist.ptr, 0, sizeof(double) * (size_t)m * m, st));
-  RPGO_HIP_CHECK(hipEventRecord(ev[3], st));
-  int64_t n_recheck = 0, n_flip = 0;
-  float ms_pairs = 0.f, ms_recheck = 0.f;
+  s.fp32_ready = false;
+  s.fp32_fallback = false;
   const bool check = prm->lc_threshold >= 0.0;
-  if (check && m > 1) {
-    if (prm->precision == RPGO_PCM_FP64_REPLAY) {
-      if ((rc = pcm_pairs_fp64(ctx, prm->lc_threshold, want_dist))) return rc;
-    } else {
-      if ((rc = pcm_pairs_fp32(ctx, prm->lc_threshold, prm->recheck_margin, want_dist,
-                               &n_recheck, &n_flip, &ms_pairs, &ms_recheck)))
-        return rc;
-    }
-    if ((rc = pcm_symmetrize(ctx))) return rc;
+  if (prm->precision == RPGO_PCM_FP32_RECHECK && check && m > 1) {
+    bool use_fp64 = false;
+    if ((rc = pcm_fp32_prepare(ctx, &use_fp64))) return rc;
+    s.fp32_ready = !use_fp64;
+    s.fp32_fallback = use_fp64;
   }
-  RPGO_HIP_CHECK(hipEventRecord(ev[4], st));
-  RPGO_HIP_CHECK(hipEventSynchronize(ev[4]));
-  s.ran = true;
-  s.have_dist = want_dist;
+  RPGO_HIP_CHECK(hipEventRecord(ev[3], st));
+  if ((rc = pairs_pass(ctx, prm, stats))) return rc;
   if (stats) {
-    unsigned long long cnt[4] = {0, 0, 0, 0};
-    RPGO_HIP_CHECK(hipMemcpy(cnt, s.counters.ptr, sizeof(cnt), hipMemcpyDeviceToHost));
-    stats->n_accepted = m;
-    stats->reserved = 0;
-    stats->n_pairs = check ? (int64_t)m * (m - 1) / 2 : 0;
-    stats->n_rechecked = n_recheck;
-    stats->n_flipped = n_flip;
-    stats->n_consistent = (int64_t)cnt[0];
     stats->ms_fold = elapsed(ev[0], ev[1]);
     stats->ms_odom = elapsed(ev[1], ev[2]);
-    if (prm->precision == RPGO_PCM_FP64_REPLAY) {
-      stats->ms_pairs = elapsed(ev[3], ev[4]);
-      stats->ms_recheck = 0.0;
-    } else {
-      stats->ms_pairs = ms_pairs;
-      stats->ms_recheck = ms_recheck;
-    }
-    stats->ms_total = elapsed(ev[0], ev[4]);
+    stats->ms_total = elapsed(ev[0], ev[11]);
+  }
+  return RPGO_OK;
+}
+
+int rpgo_pcm_pairs(rpgo_ctx* ctx, const rpgo_pcm_params* prm, rpgo_pcm_stats* stats) {
+  if (!ctx || !prm) {
+    set_error("rpgo_pcm_pairs: NULL argument");
+    return RPGO_ERR_ARG;
+  }
+  PcmState& s = ctx->pcm;
+  if (!s.ran) {
+    set_error("rpgo_pcm_pairs: call rpgo_pcm_run first (fold and odometry check)");
+    return RPGO_ERR_STATE;
+  }
+  if (prm->precision != RPGO_PCM_FP64_REPLAY && prm->precision != RPGO_PCM_FP32_RECHECK) {
+    set_error("rpgo_pcm_pairs: unknown precision mode %d", prm->precision);
+    return RPGO_ERR_ARG;
+  }
+  RPGO_HIP_CHECK(hipSetDevice(ctx->device));
+  int rc;
+  if (prm->precision == RPGO_PCM_FP32_RECHECK && !s.fp32_ready && !s.fp32_fallback && s.m > 1) {
+    bool use_fp64 = false;
+    if ((rc = pcm_fp32_prepare(ctx, &use_fp64))) return rc;
+    s.fp32_ready = !use_fp64;
+    s.fp32_fallback = use_fp64;
+  }
+  RPGO_HIP_CHECK(hipEventRecord(ctx->ev[12], ctx->stream));
+  if ((rc = pairs_pass(ctx, prm, stats))) return rc;
+  if (stats) {
+    stats->ms_fold = 0.0;
+    stats->ms_odom = 0.0;
+    stats->ms_total = elapsed(ctx->ev[12], ctx->ev[11]);
   }
   return RPGO_OK;
 }

@@ -1,15 +1,553 @@
-// FP32 PCM pairwise kernel with FP64 near-threshold recheck (placeholder
-// until the anchor-blocked formulation lands).
+// FP32 PCM pairwise kernel (SE(3)) with an FP64 replay of near-threshold pairs.
+//
+// The reference forms the covariance of Trajectory::getBetween(a, c) as a
+// difference of absolute chain covariances, Sigma_c - Ad Sigma_a Ad^T
+// (GeometryUtils.h:148-149; LLT fallback :152-167).  Both terms grow with the
+// trajectory, so the difference is meaningless in FP32.  In exact arithmetic
+// the result is the covariance of the odometry segment between min(a,c) and
+// max(a,c), expressed in the frame of max(a,c) (the LLT fallback covers a > c).
+// This path builds that segment covariance from cancellation-free parts:
+//
+//   anchors every kB poses (A_p = p*kB);
+//   Lcov_k = Seg(A_blk(k), k)      in frame k          (forward, within block)
+//   Rcov_k = Seg(k, A_blk(k)+1)    in frame A_blk(k)+1 (suffix sum, within block)
+//   Sanc(p, q) = Seg(A_p, A_q)     in frame A_q        (anchor table, FP64 sums)
+//
+//   Seg(lo, hi) in frame hi = Ad(Pl_hi^-1) M Ad(Pl_hi^-1)^T + Lcov_hi, with
+//     same block : M = -Ad(Pl_lo) Lcov_lo Ad(Pl_lo)^T          (bounded cancellation)
+//     cross block: M =  Ad(G) Rcov_lo Ad(G)^T + Sanc(blk(lo)+1, blk(hi)),
+//                  G = T_{A_hi}^-1 T_{A_blk(lo)+1}
+//
+// Poses are kept relative to their block anchor (Pl_k = T_A^-1 T_k, FP32) and
+// anchor poses in FP64, so relative poses never subtract large coordinates in
+// FP32.  The rest of the pair check (Pcm.h:703-717: compose with the newer
+// loop closure, inverse, compose with the older one, compose with the second
+// odometry segment, Mahalanobis norm of the Log) runs in FP32.
+//
+// A pair whose squared distance lies within a relative band of the squared
+// threshold, whose Log is near pi, or whose covariance is not numerically PD
+// is queued and replayed in FP64 with the reference's own formula and
+// operation order (replay64.hip), so the adjacency matches the FP64 path bit
+// for bit.  The band is recheck_margin plus a term for the reference's own
+// FP64 cancellation error, which grows with the chain covariance.
+#include <climits>
+
+#include "geom32.h"
+#include "replay64.h"
 #include "rpgo_internal.h"
 
 namespace rpgo {
 
-int pcm_pairs_fp32(rpgo_ctx* ctx, double lc_thr, double margin, bool want_dist,
-                   int64_t* n_recheck, int64_t* n_flip, float* ms_pairs, float* ms_recheck) {
-  (void)ctx; (void)lc_thr; (void)margin; (void)want_dist;
-  (void)n_recheck; (void)n_flip; (void)ms_pairs; (void)ms_recheck;
-  set_error("rpgo_pcm_run: RPGO_PCM_FP32_RECHECK is not available in this build");
-  return RPGO_ERR_ARG;
+constexpr int kB = 64;          // poses per anchor block
+constexpr int kPoseRec = 56;    // floats per pose record
+constexpr int kLcRec = 48;      // floats per loop-closure record
+constexpr int kQueueCap = kRecheckQueueCap;
+
+// pose record layout (floats): Pl[12] Lcov[21] Rcov[21] trc pad
+struct PoseRec {
+  P32 Pl;
+  S6 L;
+  S6 R;
+  float trc;
+  float pad;
+};
+static_assert(sizeof(PoseRec) == kPoseRec * 4, "pose record size");
+
+// loop-closure record: Z[12] Zinv[12] S[21] a b ri
+struct LcRec {
+  P32 Z;
+  P32 Zi;
+  S6 S;
+  int a;
+  int b;
+  int pad;
+};
+static_assert(sizeof(LcRec) == kLcRec * 4, "lc record size");
+
+__device__ __forceinline__ void s6_from36(const double* c, S6& s) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = i; j < 3; ++j) {
+      s.A[s3(i, j)] = (float)c[i * 6 + j];
+      s.C[s3(i, j)] = (float)c[(i + 3) * 6 + j + 3];
+    }
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) s.B[i * 3 + j] = (float)c[i * 6 + j + 3];
+}
+
+__device__ __forceinline__ void p32_from64(const double* p, P32& o) {
+#pragma unroll
+  for (int e = 0; e < 9; ++e) o.R[e] = (float)p[e];
+  o.t[0] = (float)p[9]; o.t[1] = (float)p[10]; o.t[2] = (float)p[11];
+}
+
+__device__ __forceinline__ void add36(double* acc, const double* x) {
+#pragma unroll
+  for (int e = 0; e < 36; ++e) acc[e] += x[e];
+}
+
+// ---------------------------------------------------------------------------
+// precompute
+// ---------------------------------------------------------------------------
+
+// NaN rotation covariances switch the whole run to the FP64 replay.
+__global__ void k_any_no_rotation(int n, const double* __restrict__ rec, int stride, int flag_off,
+                                  int* __restrict__ any) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n && rec[(size_t)k * stride + flag_off] == 0.0) atomicOr(any, 1);
+}
+
+// One thread per anchor block: local poses, forward Lcov, suffix Rcov, and
+// the block's own segment (Bend) and span (Pblock), all accumulated in FP64.
+__global__ void k_block_local(int n_poses, int n_blocks, const double* __restrict__ delta,
+                              const double* __restrict__ hinv, PoseRec* __restrict__ prec,
+                              double* __restrict__ pl64, double* __restrict__ pblock64,
+                              double* __restrict__ bend64) {
+  using R = R64<3>;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_blocks) return;
+  const int k0 = p * kB;
+  const int k1 = min(k0 + kB, n_poses);  // poses of this block: [k0, k1)
+  const bool has_next = k1 < n_poses;    // anchor A_{p+1} = k1 exists
+  double P[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
+  double L[36];
+#pragma unroll
+  for (int e = 0; e < 36; ++e) L[e] = 0.0;
+  const int kend = has_next ? k1 : k1 - 1;  // last step index (exclusive) of the walk
+  for (int k = k0;; ++k) {
+    // record pose k (k < k1) or the continuation at the next anchor
+    if (k < k1) {
+      PoseRec r;
+      p32_from64(P, r.Pl);
+      s6_from36(L, r.L);
+      s6_zero(r.R);
+      r.trc = 0.f;
+      r.pad = 0.f;
+      prec[k] = r;
+#pragma unroll
+      for (int e = 0; e < 12; ++e) pl64[(size_t)k * 12 + e] = P[e];
+    } else {
+#pragma unroll
+      for (int e = 0; e < 12; ++e) pblock64[(size_t)p * 12 + e] = P[e];
+#pragma unroll
+      for (int e = 0; e < 36; ++e) bend64[(size_t)p * 36 + e] = L[e];
+      break;
+    }
+    if (k >= kend) break;
+    // advance: P <- P * delta_k ; L <- H L H^T + Q_k
+    const double* dk = delta + (size_t)k * R::PWC;
+    double Pn[12], HLH[36];
+    R::pose_compose(P, dk, Pn);
+    congruence64<6>(hinv + (size_t)k * 36, L, HLH);
+#pragma unroll
+    for (int e = 0; e < 12; ++e) P[e] = Pn[e];
+#pragma unroll
+    for (int e = 0; e < 36; ++e) L[e] = HLH[e] + dk[12 + e];
+  }
+  if (!has_next) return;
+  // suffix sums Rcov_k = sum_{m >= k} Ad(Pblock^-1 Pl_{m+1}) Q_m Ad(.)^T
+  double Pbi[12];
+  R::pose_inverse(pblock64 + (size_t)p * 12, Pbi);
+  double acc[36];
+#pragma unroll
+  for (int e = 0; e < 36; ++e) acc[e] = 0.0;
+  for (int m = k1 - 1; m >= k0; --m) {
+    const double* Pm1 = (m + 1 < k1) ? pl64 + (size_t)(m + 1) * 12 : pblock64 + (size_t)p * 12;
+    double G[12], Ad[36], term[36];
+    R::pose_compose(Pbi, Pm1, G);
+    R::adjoint(G, Ad);
+    congruence64<6>(Ad, delta + (size_t)m * R::PWC + 12, term);
+    add36(acc, term);
+    S6 s;
+    s6_from36(acc, s);
+    prec[m].R = s;
+  }
+}
+
+// Anchor poses T_{A_p} in FP64 (sequential over blocks).
+__global__ void k_anchor_poses(int n_blocks, const double* __restrict__ pose0,
+                               const double* __restrict__ pblock64, double* __restrict__ ta) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double P[12];
+#pragma unroll
+  for (int e = 0; e < 12; ++e) P[e] = pose0[e];
+  for (int p = 0; p < n_blocks; ++p) {
+#pragma unroll
+    for (int e = 0; e < 12; ++e) ta[(size_t)p * 12 + e] = P[e];
+    if (p + 1 < n_blocks) {
+      double Pn[12];
+      R64<3>::pose_compose(P, pblock64 + (size_t)p * 12, Pn);
+#pragma unroll
+      for (int e = 0; e < 12; ++e) P[e] = Pn[e];
+    }
+  }
+}
+
+// term(r, q) = Ad(T_{A_q}^-1 T_{A_{r+1}}) Bend_r Ad(.)^T for r < q (FP64)
+__global__ void k_anchor_terms(int nb, const double* __restrict__ ta,
+                               const double* __restrict__ bend64, double* __restrict__ terms) {
+  using R = R64<3>;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nb * nb) return;
+  const int q = (int)(t / nb), r = (int)(t % nb);
+  if (r >= q) return;
+  double qi[12], G[12], Ad[36], term[36];
+  R::pose_inverse(ta + (size_t)q * 12, qi);
+  R::pose_compose(qi, ta + (size_t)(r + 1) * 12, G);
+  R::adjoint(G, Ad);
+  congruence64<6>(Ad, bend64 + (size_t)r * 36, term);
+#pragma unroll
+  for (int e = 0; e < 36; ++e) terms[(size_t)t * 36 + e] = term[e];
+}
+
+// Sanc(r, q) = sum_{r' = r}^{q-1} term(r', q)  (suffix sum over r for each q)
+__global__ void k_anchor_suffix(int nb, const double* __restrict__ terms, S6* __restrict__ sanc) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nb) return;
+  double acc[36];
+#pragma unroll
+  for (int e = 0; e < 36; ++e) acc[e] = 0.0;
+  S6 z;
+  s6_zero(z);
+  sanc[(size_t)q * nb + q] = z;
+  for (int r = q - 1; r >= 0; --r) {
+    add36(acc, terms + ((size_t)q * nb + r) * 36);
+    S6 s;
+    s6_from36(acc, s);
+    sanc[(size_t)r * nb + q] = s;
+  }
+}
+
+// Chain covariance trace estimate per pose (margin of the FP64 band).
+__global__ void k_pose_trace(int n_poses, int nb, const S6* __restrict__ sanc,
+                             PoseRec* __restrict__ prec) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_poses) return;
+  const int p = k / kB;
+  PoseRec& r = prec[k];
+  P32 Pi;
+  p32_inv(r.Pl, Pi);
+  S6 T;
+  s6_congruence(Pi, sanc[p], T);  // Sanc(0, p)
+  r.trc = s6_trace(T) + s6_trace(r.L);
+}
+
+__global__ void k_lc_records(int m, const int* __restrict__ cf, const int* __restrict__ ct,
+                             const double* __restrict__ clcm, LcRec* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m) return;
+  const double* r = clcm + (size_t)t * R64<3>::PWC;
+  LcRec o;
+  p32_from64(r, o.Z);
+  double zi[12];
+  R64<3>::pose_inverse(r, zi);
+  p32_from64(zi, o.Zi);
+  s6_from36(r + 12, o.S);
+  o.a = cf[t];
+  o.b = ct[t];
+  o.pad = 0;
+  out[t] = o;
+}
+
+// ---------------------------------------------------------------------------
+// pair check
+// ---------------------------------------------------------------------------
+
+struct Anchors {
+  const double* ta;      // FP64 anchor poses
+  const float* pblock;   // FP32 Pblock_p (12 floats)
+  const S6* sanc;        // nb x nb
+  int nb;
+};
+
+__device__ __forceinline__ void rel_anchor(const Anchors& an, int p, int q, P32& o) {
+  // T_{A_p}^-1 T_{A_q} in FP64, rounded to FP32
+  double pi[12], r[12];
+  R64<3>::pose_inverse(an.ta + (size_t)p * 12, pi);
+  R64<3>::pose_compose(pi, an.ta + (size_t)q * 12, r);
+  p32_from64(r, o);
+}
+
+// between(x, y) of the reference, x older-side key, y newer-side key:
+// pose T_x^-1 T_y and covariance Seg(min, max) in frame max.  Returns the
+// translation norm of the pose (for the error band).
+__device__ __forceinline__ float between32(const Anchors& an, int x, const PoseRec& rx, int y,
+                                           const PoseRec& ry, P32& pose, S6& cov) {
+  const bool xlo = x <= y;
+  const int lo = xlo ? x : y, hi = xlo ? y : x;
+  const PoseRec& rl = xlo ? rx : ry;
+  const PoseRec& rh = xlo ? ry : rx;
+  const int plo = lo / kB, phi = hi / kB;
+  P32 rel;
+  rel_anchor(an, plo, phi, rel);  // T_{A_plo}^-1 T_{A_phi}
+  P32 reli;
+  p32_inv(rel, reli);
+  // pose
+  P32 xi_, tmp;
+  p32_inv(rx.Pl, xi_);
+  p32_mul(xi_, xlo ? rel : reli, tmp);
+  p32_mul(tmp, ry.Pl, pose);
+  // covariance
+  const bool same = plo == phi;
+  P32 G;
+  S6 M;
+  if (same) {
+    s6_congruence(rl.Pl, rl.L, M);
+#pragma unroll
+    for (int e = 0; e < 6; ++e) M.A[e] = -M.A[e], M.C[e] = -M.C[e];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) M.B[e] = -M.B[e];
+  } else {
+    P32 pb;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) pb.R[e] = an.pblock[(size_t)plo * 12 + e];
+    pb.t[0] = an.pblock[(size_t)plo * 12 + 9];
+    pb.t[1] = an.pblock[(size_t)plo * 12 + 10];
+    pb.t[2] = an.pblock[(size_t)plo * 12 + 11];
+    p32_mul(reli, pb, G);
+    s6_congruence(G, rl.R, M);
+    const S6& Q = an.sanc[(size_t)(plo + 1) * an.nb + phi];
+    s6_axpy(1.0f, Q, M);
+  }
+  P32 hinv;
+  p32_inv(rh.Pl, hinv);
+  s6_congruence(hinv, M, cov);
+  s6_axpy(1.0f, rh.L, cov);
+  return sqrtf(pose.t[0] * pose.t[0] + pose.t[1] * pose.t[1] + pose.t[2] * pose.t[2]);
+}
+
+// compose of PoseWithCovariance with a measurement (Z, Zinv, S):
+// pose T Z, covariance Ad(Z^-1) C Ad(Z^-1)^T + S
+__device__ __forceinline__ void compose_meas(const P32& T, const S6& C, const LcRec& z, P32& To,
+                                             S6& Co) {
+  p32_mul(T, z.Z, To);
+  s6_congruence(z.Zi, C, Co);
+  s6_axpy(1.0f, z.S, Co);
+}
+
+template <int ICH>
+__global__ void __launch_bounds__(64) k_pairs32(int m, long long n_tiles, const LcRec* __restrict__ lcs,
+                                                const PoseRec* __restrict__ prec, Anchors an,
+                                                float thr2, float tau, int W,
+                                                unsigned long long* __restrict__ adj,
+                                                double* __restrict__ dist,
+                                                int2* __restrict__ queue,
+                                                unsigned int* __restrict__ qcount) {
+  constexpr int K = 64 / ICH;
+  const long long t = blockIdx.x;
+  if (t >= n_tiles) return;
+  // tile t -> (jt, chunk): row tile jt owns (jt+1)*K chunks
+  long long jt = (long long)((sqrt(1.0 + 8.0 * (double)t / K) - 1.0) * 0.5);
+  while ((long long)K * (jt + 1) * (jt + 2) / 2 <= t) ++jt;
+  while (jt > 0 && (long long)K * jt * (jt + 1) / 2 > t) --jt;
+  const int chunk = (int)(t - (long long)K * jt * (jt + 1) / 2);
+  const int lane = threadIdx.x;
+  const int j = (int)jt * 64 + lane;
+  const int i0 = chunk * ICH;
+  const bool jvalid = j < m;
+  const int jj = jvalid ? j : m - 1;
+  const LcRec lj = lcs[jj];
+  const PoseRec rc = prec[lj.a];
+  const PoseRec rd = prec[lj.b];
+  unsigned long long rowbits = 0ull;
+  for (int ii = 0; ii < ICH; ++ii) {
+    const int i = i0 + ii;
+    if (i >= m) break;
+    const bool active = jvalid && i < j;
+    const LcRec li = lcs[i];
+    const PoseRec ra = prec[li.a];
+    const PoseRec rb = prec[li.b];
+    bool cons = false;
+    if (active) {
+      P32 TA, TX, TY, TW, TB, TL, TBi;
+      S6 CA, CX, CW, CB, CL;
+      const float ta = between32(an, li.a, ra, lj.a, rc, TA, CA);
+      compose_meas(TA, CA, lj, TX, CX);
+      p32_inv(TX, TY);
+      compose_meas(TY, CX, li, TW, CW);
+      const float tb = between32(an, li.b, rb, lj.b, rd, TB, CB);
+      p32_mul(TW, TB, TL);
+      p32_inv(TB, TBi);
+      s6_congruence(TBi, CW, CL);
+      s6_axpy(1.0f, CB, CL);
+      float xi[6];
+      const bool log_ok = se3_log32(TL, xi);
+      float d2, mp;
+      const bool pd = mahal2_32(CL, xi, &d2, &mp);
+      cons = d2 < thr2;
+      // reference FP64 cancellation estimate: eps64 * |Sigma_chain| * lever^2 / pivot
+      const float chain = ra.trc + rc.trc + rb.trc + rd.trc;
+      const float lever = 1.0f + ta * ta + tb * tb;
+      const float band = tau + 1.5e-14f * chain * lever / fmaxf(mp, 1e-30f);
+      const bool near = fabsf(d2 - thr2) <= band * thr2;
+      if (!log_ok || !pd || near || !(d2 == d2) || d2 > 3.0e38f) {
+        const unsigned int q = atomicAdd(qcount, 1u);
+        if (q < (unsigned)kQueueCap) queue[q] = make_int2(i, j);
+      }
+      if (dist) {
+        const double dv = sqrt((double)d2);
+        dist[(size_t)j * m + i] = dv;
+        dist[(size_t)i * m + j] = dv;
+      }
+    }
+    if (cons) rowbits |= 1ull << (i & 63);
+    const unsigned long long colbits = __ballot(cons);
+    if (lane == 0 && colbits) atomicOr(&adj[(size_t)i * W + jt], colbits);
+  }
+  if (jvalid && rowbits) atomicOr(&adj[(size_t)j * W + (i0 >> 6)], rowbits);
+}
+
+// consistent pair count: popcount of the full symmetric matrix / 2
+__global__ void k_popcount(long long nwords, const unsigned long long* __restrict__ adj,
+                           unsigned long long* __restrict__ out) {
+  unsigned long long s = 0;
+  for (long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x; w < nwords;
+       w += (long long)gridDim.x * blockDim.x)
+    s += __popcll(adj[w]);
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if ((threadIdx.x & 63) == 0 && s) atomicAdd(out, s);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+__global__ void k_to_f32(long long n, const double* __restrict__ a, float* __restrict__ b) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) b[k] = (float)a[k];
+}
+
+int pcm_fp32_prepare(rpgo_ctx* ctx, bool* use_fp64) {
+  PcmState& s = ctx->pcm;
+  hipStream_t st = ctx->stream;
+  *use_fp64 = false;
+  if (s.dim != 3) {  // SE(2) runs the FP64 replay
+    *use_fp64 = true;
+    return RPGO_OK;
+  }
+  const int N = s.n_poses, nb = (N + kB - 1) / kB, m = s.m;
+  // NaN rotation covariance anywhere: FP64 replay for every pair
+  int* any = s.counters.as<int>() + 32;
+  RPGO_HIP_CHECK(hipMemsetAsync(any, 0, sizeof(int), st));
+  if (N > 1)
+    k_any_no_rotation<<<(N - 1 + 255) / 256, 256, 0, st>>>(N - 1, s.delta.as<double>(), R64<3>::PWC,
+                                                          48, any);
+  if (s.n_lc > 0)
+    k_any_no_rotation<<<(s.n_lc + 255) / 256, 256, 0, st>>>(s.n_lc, s.lcm.as<double>(), R64<3>::PWC,
+                                                            48, any);
+  RPGO_HIP_CHECK(hipGetLastError());
+  int h_any = 0;
+  RPGO_HIP_CHECK(hipMemcpyAsync(&h_any, any, sizeof(int), hipMemcpyDeviceToHost, st));
+  RPGO_HIP_CHECK(hipStreamSynchronize(st));
+  if (h_any) {
+    *use_fp64 = true;
+    return RPGO_OK;
+  }
+  // layout of fp32_buf: prec | lcrec | pl64 | pblock64 | bend64 | ta | pblock32 | sanc | terms
+  const size_t off_prec = 0;
+  const size_t off_lc = off_prec + sizeof(PoseRec) * (size_t)N;
+  const size_t off_pl64 = off_lc + sizeof(LcRec) * (size_t)(m > 0 ? m : 1);
+  const size_t off_pb64 = off_pl64 + sizeof(double) * 12 * (size_t)N;
+  const size_t off_bend = off_pb64 + sizeof(double) * 12 * (size_t)nb;
+  const size_t off_ta = off_bend + sizeof(double) * 36 * (size_t)nb;
+  const size_t off_pb32 = off_ta + sizeof(double) * 12 * (size_t)nb;
+  const size_t off_sanc = off_pb32 + sizeof(float) * 12 * (size_t)nb;
+  const size_t off_terms = off_sanc + sizeof(S6) * (size_t)nb * nb;
+  const size_t total = off_terms + sizeof(double) * 36 * (size_t)nb * nb;
+  int rc;
+  if ((rc = s.fp32_buf.reserve(total))) return rc;
+  char* base = s.fp32_buf.as<char>();
+  PoseRec* prec = reinterpret_cast<PoseRec*>(base + off_prec);
+  LcRec* lcr = reinterpret_cast<LcRec*>(base + off_lc);
+  double* pl64 = reinterpret_cast<double*>(base + off_pl64);
+  double* pb64 = reinterpret_cast<double*>(base + off_pb64);
+  double* bend = reinterpret_cast<double*>(base + off_bend);
+  double* ta = reinterpret_cast<double*>(base + off_ta);
+  float* pb32 = reinterpret_cast<float*>(base + off_pb32);
+  S6* sanc = reinterpret_cast<S6*>(base + off_sanc);
+  double* terms = reinterpret_cast<double*>(base + off_terms);
+  RPGO_HIP_CHECK(hipMemsetAsync(pb64, 0, sizeof(double) * 12 * (size_t)nb, st));
+  RPGO_HIP_CHECK(hipMemsetAsync(bend, 0, sizeof(double) * 36 * (size_t)nb, st));
+  k_block_local<<<(nb + 63) / 64, 64, 0, st>>>(N, nb, s.delta.as<double>(), s.hinv.as<double>(),
+                                               prec, pl64, pb64, bend);
+  RPGO_HIP_CHECK(hipGetLastError());
+  k_anchor_poses<<<1, 1, 0, st>>>(nb, s.pose0.as<double>(), pb64, ta);
+  RPGO_HIP_CHECK(hipGetLastError());
+  const long long nt = (long long)nb * nb;
+  k_anchor_terms<<<(unsigned)((nt + 255) / 256), 256, 0, st>>>(nb, ta, bend, terms);
+  k_anchor_suffix<<<(nb + 63) / 64, 64, 0, st>>>(nb, terms, sanc);
+  RPGO_HIP_CHECK(hipGetLastError());
+  k_pose_trace<<<(N + 255) / 256, 256, 0, st>>>(N, nb, sanc, prec);
+  if (m > 0)
+    k_lc_records<<<(m + 255) / 256, 256, 0, st>>>(m, s.c_from.as<int>(), s.c_to.as<int>(),
+                                                  s.c_lcm.as<double>(), lcr);
+  RPGO_HIP_CHECK(hipGetLastError());
+  // Pblock FP32
+  k_to_f32<<<(nb * 12 + 255) / 256, 256, 0, st>>>((long long)nb * 12, pb64, pb32);
+  RPGO_HIP_CHECK(hipGetLastError());
+  s.fp32_nb = nb;
+  return RPGO_OK;
+}
+
+int pcm_pairs_fp32_launch(rpgo_ctx* ctx, double lc_thr, double margin, bool want_dist) {
+  PcmState& s = ctx->pcm;
+  const int N = s.n_poses, nb = s.fp32_nb, m = s.m;
+  const size_t off_lc = sizeof(PoseRec) * (size_t)N;
+  const size_t off_pl64 = off_lc + sizeof(LcRec) * (size_t)(m > 0 ? m : 1);
+  const size_t off_pb64 = off_pl64 + sizeof(double) * 12 * (size_t)N;
+  const size_t off_bend = off_pb64 + sizeof(double) * 12 * (size_t)nb;
+  const size_t off_ta = off_bend + sizeof(double) * 36 * (size_t)nb;
+  const size_t off_pb32 = off_ta + sizeof(double) * 12 * (size_t)nb;
+  const size_t off_sanc = off_pb32 + sizeof(float) * 12 * (size_t)nb;
+  char* base = s.fp32_buf.as<char>();
+  Anchors an;
+  an.ta = reinterpret_cast<const double*>(base + off_ta);
+  an.pblock = reinterpret_cast<const float*>(base + off_pb32);
+  an.sanc = reinterpret_cast<const S6*>(base + off_sanc);
+  an.nb = nb;
+  const PoseRec* prec = reinterpret_cast<const PoseRec*>(base);
+  const LcRec* lcr = reinterpret_cast<const LcRec*>(base + off_lc);
+  int rc;
+  if ((rc = s.queue.reserve(sizeof(int2) * (size_t)kQueueCap))) return rc;
+  const int W = (m + 63) / 64;
+  const long long JT = W;
+  const float thr2 = (float)(lc_thr * lc_thr);
+  const float tau = (float)(margin > 0.0 ? margin : 2e-3);
+  unsigned int* qcount = s.counters.as<unsigned int>() + 16;
+  // pick the i-chunk so that the launch has enough wavefronts
+  const long long pairs = (long long)m * (m - 1) / 2;
+  hipStream_t st = ctx->stream;
+  double* dist = want_dist ? s.dist.as<double>() : nullptr;
+  unsigned long long* adj = s.adj.as<unsigned long long>();
+  int2* q = s.queue.as<int2>();
+#define LAUNCH(ICH)                                                                            \
+  do {                                                                                         \
+    const long long tiles = (long long)(64 / ICH) * JT * (JT + 1) / 2;                         \
+    k_pairs32<ICH><<<(unsigned)tiles, 64, 0, st>>>(m, tiles, lcr, prec, an, thr2, tau, W, adj, \
+                                                   dist, q, qcount);                           \
+  } while (0)
+  if (pairs >= (1ll << 26)) LAUNCH(64);
+  else if (pairs >= (1ll << 24)) LAUNCH(16);
+  else LAUNCH(4);
+#undef LAUNCH
+  RPGO_HIP_CHECK(hipGetLastError());
+  return RPGO_OK;
+}
+
+int pcm_count_consistent(rpgo_ctx* ctx) {
+  PcmState& s = ctx->pcm;
+  const int m = s.m, W = (m + 63) / 64;
+  const long long nwords = (long long)m * W;
+  if (nwords == 0) return RPGO_OK;
+  long long blocks = (nwords + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  k_popcount<<<(unsigned)blocks, 256, 0, ctx->stream>>>(nwords, s.adj.as<unsigned long long>(),
+                                                        s.counters.as<unsigned long long>() + 1);
+  RPGO_HIP_CHECK(hipGetLastError());
+  return RPGO_OK;
 }
 
 }  // namespace rpgo

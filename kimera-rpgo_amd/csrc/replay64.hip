@@ -229,9 +229,64 @@ __global__ void __launch_bounds__(64) k_symmetrize(int m, int W, unsigned long l
   }
 }
 
+// FP64 replay of pairs queued by the FP32 kernel: the reference decision
+// d < thr replaces the FP32 one; both triangle bits are patched.
+template <int DIM>
+__global__ void k_recheck64(int n, const int2* __restrict__ queue, const double* __restrict__ traj,
+                            const int* __restrict__ cf, const int* __restrict__ ct,
+                            const double* __restrict__ clcm, double thr, int m, int W,
+                            unsigned long long* __restrict__ adj, double* __restrict__ dist,
+                            unsigned long long* __restrict__ flips) {
+  constexpr int PWC = R64<DIM>::PWC;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int2 q = queue[t];
+  const int i = q.x, j = q.y;
+  if (i < 0 || j <= i || j >= m) return;
+  const double d = R64<DIM>::pair_dist(traj, cf[i], ct[i], clcm + (size_t)i * PWC, cf[j], ct[j],
+                                       clcm + (size_t)j * PWC);
+  const bool cons = d < thr;
+  unsigned long long* wj = adj + (size_t)j * W + (i >> 6);
+  unsigned long long* wi = adj + (size_t)i * W + (j >> 6);
+  const unsigned long long bj = 1ull << (i & 63), bi = 1ull << (j & 63);
+  const bool was = ((*wj) & bj) != 0ull;
+  if (cons != was) {
+    if (cons) {
+      atomicOr(wj, bj);
+      atomicOr(wi, bi);
+    } else {
+      atomicAnd(wj, ~bj);
+      atomicAnd(wi, ~bi);
+    }
+    atomicAdd(flips, 1ull);
+  }
+  if (dist) {
+    dist[(size_t)j * m + i] = d;
+    dist[(size_t)i * m + j] = d;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // host drivers
 // ---------------------------------------------------------------------------
+int pcm_recheck64(rpgo_ctx* ctx, int n_queue, double lc_thr, bool want_dist) {
+  PcmState& s = ctx->pcm;
+  if (n_queue <= 0) return RPGO_OK;
+  const int m = s.m, W = (m + 63) / 64;
+  double* dist = want_dist ? s.dist.as<double>() : nullptr;
+  unsigned long long* flips = s.counters.as<unsigned long long>() + 2;
+  if (s.dim == 3)
+    k_recheck64<3><<<(n_queue + 63) / 64, 64, 0, ctx->stream>>>(
+        n_queue, s.queue.as<int2>(), s.traj.as<double>(), s.c_from.as<int>(), s.c_to.as<int>(),
+        s.c_lcm.as<double>(), lc_thr, m, W, s.adj.as<unsigned long long>(), dist, flips);
+  else
+    k_recheck64<2><<<(n_queue + 63) / 64, 64, 0, ctx->stream>>>(
+        n_queue, s.queue.as<int2>(), s.traj.as<double>(), s.c_from.as<int>(), s.c_to.as<int>(),
+        s.c_lcm.as<double>(), lc_thr, m, W, s.adj.as<unsigned long long>(), dist, flips);
+  RPGO_HIP_CHECK(hipGetLastError());
+  return RPGO_OK;
+}
+
 int pcm_prepare(rpgo_ctx* ctx) {
   PcmState& s = ctx->pcm;
   const int nd = s.n_poses - 1;

@@ -64,6 +64,10 @@ struct PcmState {
   bool ran = false;
   int m = 0;  // accepted loop closures of the last run
   bool have_dist = false;
+  int precision = 0;     // precision of the last run
+  bool fp32_ready = false;  // FP32 tables built for the current accepted set
+  bool fp32_fallback = false;  // FP32 path declined (SE(2) or no rotation info)
+  int fp32_nb = 0;       // anchor blocks of the FP32 tables
   // inputs
   DevBuf pose0, odom, odom_cov, lc_from, lc_to, lc_pose, lc_cov;
   // derived
@@ -92,7 +96,7 @@ struct CliqueState {
 struct rpgo_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[8] = {};
+  hipEvent_t ev[16] = {};
   rpgo::PcmState pcm;
   rpgo::CliqueState clq;
 };
@@ -104,9 +108,18 @@ int pcm_fold(rpgo_ctx* ctx);                     // sequential FP64 fold
 int pcm_odom_and_compact(rpgo_ctx* ctx, double odom_thr, int* m_out);
 int pcm_pairs_fp64(rpgo_ctx* ctx, double lc_thr, bool want_dist);
 int pcm_symmetrize(rpgo_ctx* ctx);
+// FP64 replay of queued (i, j) pairs; patches the adjacency bits, counts flips
+int pcm_recheck64(rpgo_ctx* ctx, int n_queue, double lc_thr, bool want_dist);
 // pcm_fp32.hip
-int pcm_pairs_fp32(rpgo_ctx* ctx, double lc_thr, double margin, bool want_dist,
-                   int64_t* n_recheck, int64_t* n_flip, float* ms_pairs, float* ms_recheck);
+int pcm_fp32_prepare(rpgo_ctx* ctx, bool* use_fp64);
+int pcm_pairs_fp32_launch(rpgo_ctx* ctx, double lc_thr, double margin, bool want_dist);
+int pcm_count_consistent(rpgo_ctx* ctx);
+constexpr int kRecheckQueueCap = 1 << 22;
+
+// counters buffer layout (256 bytes, zeroed per pairwise pass)
+//   u64[0] consistent pairs (FP64 kernel)   u64[1] popcount of the adjacency
+//   u64[2] FP64 recheck flips               u32[16] recheck queue length
+//   i32[32] "some covariance has no rotation information" flag
 // clique.hip
 int clique_run(rpgo_ctx* ctx, const uint64_t* d_adj, int n, int mode, int32_t* idx_out,
                int32_t* k_out);

@@ -58,6 +58,7 @@ SIGNATURES = [
     ("rpgo_version", [], ctypes.c_char_p),
     ("rpgo_pcm_upload", [ctypes.c_void_p, ctypes.POINTER(PcmProblem)], ctypes.c_int),
     ("rpgo_pcm_run", [ctypes.c_void_p, ctypes.POINTER(PcmParams), ctypes.POINTER(PcmStats)], ctypes.c_int),
+    ("rpgo_pcm_pairs", [ctypes.c_void_p, ctypes.POINTER(PcmParams), ctypes.POINTER(PcmStats)], ctypes.c_int),
     ("rpgo_pcm_download", [ctypes.c_void_p, _i32, _d, _u64, _d], ctypes.c_int),
     ("rpgo_pcm_adjacency", [ctypes.c_void_p, ctypes.POINTER(PcmProblem), ctypes.POINTER(PcmParams),
                             _i32, _d, _u64, _d, ctypes.POINTER(PcmStats)], ctypes.c_int),
@@ -147,6 +148,14 @@ class Context:
         st = PcmStats()
         check(lib().rpgo_pcm_run(self._h, ctypes.byref(prm), ctypes.byref(st)))
         self._m = st.n_accepted
+        return st
+
+    def pcm_pairs(self, lc_thr, precision=PCM_FP64_REPLAY, want_dist=False, margin=0.0):
+        """Re-run only the pairwise stage on the last run's device-resident data."""
+        prm = PcmParams(odom_threshold=-1.0, lc_threshold=lc_thr, precision=precision,
+                        want_dist=1 if want_dist else 0, recheck_margin=margin)
+        st = PcmStats()
+        check(lib().rpgo_pcm_pairs(self._h, ctypes.byref(prm), ctypes.byref(st)))
         return st
 
     def pcm_download(self, want_dist=False):

@@ -73,6 +73,8 @@ def lib():
                                                 ctypes.c_int, _i32], ctypes.c_int),
             ("orc_max_clique_exact", [ctypes.c_int, _i32, _i32, _i32], ctypes.c_int),
             ("orc_max_clique_heu_dense", [ctypes.c_int, _u8, _i32], ctypes.c_int),
+            ("orc_pair_dist_list", [ctypes.c_int, _d, _i32, _i32, _d, _d, ctypes.c_int, _i32, _i32, _d],
+             None),
         ]:
             f = getattr(L, name)
             f.argtypes = argt
@@ -290,3 +292,19 @@ def max_clique_heu_dense(adj):
     out = np.zeros(max(n, 1), dtype=np.int32)
     k = lib().orc_max_clique_heu_dense(n, _p(adj, _u8), _p(out, _i32))
     return int(k), out[: max(k, 0)].copy()
+
+
+def pair_dist_list(dim, traj, lc_from, lc_to, lc_pose, lc_cov, pi, pj):
+    """Distances of explicit (older i, newer j) pairs of an accepted LC list."""
+    traj = _f64(traj)
+    lc_from = np.ascontiguousarray(lc_from, dtype=np.int32)
+    lc_to = np.ascontiguousarray(lc_to, dtype=np.int32)
+    m = lc_from.shape[0]
+    lc_pose = _f64(lc_pose).reshape(m, pose_len(dim))
+    lc_cov = _f64(lc_cov).reshape(m, tdim(dim) ** 2)
+    pi = np.ascontiguousarray(pi, dtype=np.int32)
+    pj = np.ascontiguousarray(pj, dtype=np.int32)
+    out = np.zeros(max(len(pi), 1))
+    lib().orc_pair_dist_list(dim, _p(traj), _p(lc_from, _i32), _p(lc_to, _i32), _p(lc_pose),
+                             _p(lc_cov), len(pi), _p(pi, _i32), _p(pj, _i32), _p(out))
+    return out[: len(pi)].copy()

@@ -822,3 +822,17 @@ int orc_max_clique_heu_dense(int n, const uint8_t* adj, int32_t* out) {
   free(S); free(D); free(bits); free(degv);
   return best;
 }
+
+/* Pairwise distances for an explicit list of (i, j) pairs (i older, j newer)
+ * of an accepted loop-closure list; used to spot-check large graphs. */
+void orc_pair_dist_list(int dim, const double* traj, const int32_t* lc_from,
+                        const int32_t* lc_to, const double* lc_pose, const double* lc_cov,
+                        int npairs, const int32_t* pi, const int32_t* pj, double* out) {
+  const int pl = pose_len(dim), dd = tan_dim(dim) * tan_dim(dim);
+  for (int t = 0; t < npairs; ++t) {
+    const int i = pi[t], j = pj[t];
+    out[t] = orc_pair_dist(dim, traj, lc_from[i], lc_to[i], lc_pose + (size_t)i * pl,
+                           lc_cov + (size_t)i * dd, lc_from[j], lc_to[j],
+                           lc_pose + (size_t)j * pl, lc_cov + (size_t)j * dd);
+  }
+}

@@ -91,6 +91,11 @@ void orc_pcm_dist_rows(int dim, const double* traj, int m,
                        const double* lc_pose, const double* lc_cov,
                        int j0, int j1, double* dist_out);
 
+/* Distances of an explicit list of pairs (pi[t] older, pj[t] newer). */
+void orc_pair_dist_list(int dim, const double* traj, const int32_t* lc_from,
+                        const int32_t* lc_to, const double* lc_pose, const double* lc_cov,
+                        int npairs, const int32_t* pi, const int32_t* pj, double* out);
+
 /* ---- max clique (findCliqueHeu.cpp / findClique.cpp / graphIO.cpp) ---- */
 
 /* CSR from a dense symmetric byte adjacency (graphIO.cpp:188-230).
