@@ -39,6 +39,7 @@ def cpu_baseline(g, odom_thr, lc_thr, budget_s=15.0):
     """FP64 oracle (single core) on a bounded row sample of the same graph."""
     import oracle as O
 
+    O.set_threads(1)  # the reference is single-threaded
     t0 = time.perf_counter()
     traj = O.fold(g.dim, g.pose0, g.odom, g.odom_cov)
     t_fold = time.perf_counter() - t0

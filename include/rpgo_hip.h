@@ -80,8 +80,11 @@ typedef struct {
   double lc_threshold;    /* PcmParams::lc_threshold;   < 0 disables (Pcm.h:79-82)  */
   int32_t precision;      /* RPGO_PCM_FP64_REPLAY or RPGO_PCM_FP32_RECHECK           */
   int32_t want_dist;      /* keep the m*m distance matrix for rpgo_pcm_download      */
-  double recheck_margin;  /* FP32 mode: relative band around the threshold that is
-                             replayed in FP64 (<= 0 selects the built-in default)  */
+  double recheck_margin;  /* FP32 mode: relative band around the squared threshold that
+                             is replayed in FP64 (<= 0 selects the default 2e-3)   */
+  double recheck_chain_coef; /* FP32 mode: band term for the reference's own FP64
+                             cancellation, coef * tr(chain covariances) / min pivot
+                             (<= 0 selects the default 16 * 2^-52)                 */
 } rpgo_pcm_params;
 
 typedef struct {

@@ -176,7 +176,8 @@ static int pairs_pass(rpgo_ctx* ctx, const rpgo_pcm_params* prm, rpgo_pcm_stats*
       RPGO_HIP_CHECK(hipEventRecord(ev[9], st));
       RPGO_HIP_CHECK(hipEventRecord(ev[10], st));
     } else {
-      if ((rc = pcm_pairs_fp32_launch(ctx, prm->lc_threshold, prm->recheck_margin, want_dist)))
+      if ((rc = pcm_pairs_fp32_launch(ctx, prm->lc_threshold, prm->recheck_margin,
+                                      prm->recheck_chain_coef, want_dist)))
         return rc;
       RPGO_HIP_CHECK(hipEventRecord(ev[9], st));
       unsigned int nq = 0;

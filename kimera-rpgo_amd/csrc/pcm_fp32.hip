@@ -331,7 +331,7 @@ __device__ __forceinline__ void compose_meas(const P32& T, const S6& C, const Lc
 template <int ICH>
 __global__ void __launch_bounds__(64) k_pairs32(int m, long long n_tiles, const LcRec* __restrict__ lcs,
                                                 const PoseRec* __restrict__ prec, Anchors an,
-                                                float thr2, float tau, int W,
+                                                float thr2, float tau, float kdyn, int W,
                                                 unsigned long long* __restrict__ adj,
                                                 double* __restrict__ dist,
                                                 int2* __restrict__ queue,
@@ -378,10 +378,13 @@ __global__ void __launch_bounds__(64) k_pairs32(int m, long long n_tiles, const 
       float d2, mp;
       const bool pd = mahal2_32(CL, xi, &d2, &mp);
       cons = d2 < thr2;
-      // reference FP64 cancellation estimate: eps64 * |Sigma_chain| * lever^2 / pivot
+      // The reference subtracts chain covariances bounded by |Sigma_c| in PSD
+      // order, so its FP64 error is ~eps64 * tr(chain), measured against the
+      // smallest pivot of Sigma_L (lever arms scale error and covariance alike).
       const float chain = ra.trc + rc.trc + rb.trc + rd.trc;
-      const float lever = 1.0f + ta * ta + tb * tb;
-      const float band = tau + 1.5e-14f * chain * lever / fmaxf(mp, 1e-30f);
+      const float band = tau + kdyn * chain / fmaxf(mp, 1e-30f);
+      (void)ta;
+      (void)tb;
       const bool near = fabsf(d2 - thr2) <= band * thr2;
       if (!log_ok || !pd || near || !(d2 == d2) || d2 > 3.0e38f) {
         const unsigned int q = atomicAdd(qcount, 1u);
@@ -492,7 +495,8 @@ int pcm_fp32_prepare(rpgo_ctx* ctx, bool* use_fp64) {
   return RPGO_OK;
 }
 
-int pcm_pairs_fp32_launch(rpgo_ctx* ctx, double lc_thr, double margin, bool want_dist) {
+int pcm_pairs_fp32_launch(rpgo_ctx* ctx, double lc_thr, double margin, double chain_coef,
+                          bool want_dist) {
   PcmState& s = ctx->pcm;
   const int N = s.n_poses, nb = s.fp32_nb, m = s.m;
   const size_t off_lc = sizeof(PoseRec) * (size_t)N;
@@ -516,6 +520,7 @@ int pcm_pairs_fp32_launch(rpgo_ctx* ctx, double lc_thr, double margin, bool want
   const long long JT = W;
   const float thr2 = (float)(lc_thr * lc_thr);
   const float tau = (float)(margin > 0.0 ? margin : 2e-3);
+  const float kdyn = (float)(chain_coef > 0.0 ? chain_coef : 16.0 * 2.220446049250313e-16);
   unsigned int* qcount = s.counters.as<unsigned int>() + 16;
   // pick the i-chunk so that the launch has enough wavefronts
   const long long pairs = (long long)m * (m - 1) / 2;
@@ -526,7 +531,7 @@ int pcm_pairs_fp32_launch(rpgo_ctx* ctx, double lc_thr, double margin, bool want
 #define LAUNCH(ICH)                                                                            \
   do {                                                                                         \
     const long long tiles = (long long)(64 / ICH) * JT * (JT + 1) / 2;                         \
-    k_pairs32<ICH><<<(unsigned)tiles, 64, 0, st>>>(m, tiles, lcr, prec, an, thr2, tau, W, adj, \
+    k_pairs32<ICH><<<(unsigned)tiles, 64, 0, st>>>(m, tiles, lcr, prec, an, thr2, tau, kdyn, W, adj, \
                                                    dist, q, qcount);                           \
   } while (0)
   if (pairs >= (1ll << 26)) LAUNCH(64);

@@ -112,7 +112,8 @@ int pcm_symmetrize(rpgo_ctx* ctx);
 int pcm_recheck64(rpgo_ctx* ctx, int n_queue, double lc_thr, bool want_dist);
 // pcm_fp32.hip
 int pcm_fp32_prepare(rpgo_ctx* ctx, bool* use_fp64);
-int pcm_pairs_fp32_launch(rpgo_ctx* ctx, double lc_thr, double margin, bool want_dist);
+int pcm_pairs_fp32_launch(rpgo_ctx* ctx, double lc_thr, double margin, double chain_coef,
+                          bool want_dist);
 int pcm_count_consistent(rpgo_ctx* ctx);
 constexpr int kRecheckQueueCap = 1 << 22;
 

@@ -34,7 +34,7 @@ class PcmProblem(ctypes.Structure):
 class PcmParams(ctypes.Structure):
     _fields_ = [("odom_threshold", ctypes.c_double), ("lc_threshold", ctypes.c_double),
                 ("precision", ctypes.c_int32), ("want_dist", ctypes.c_int32),
-                ("recheck_margin", ctypes.c_double)]
+                ("recheck_margin", ctypes.c_double), ("recheck_chain_coef", ctypes.c_double)]
 
 
 class PcmStats(ctypes.Structure):
@@ -142,18 +142,22 @@ class Context:
         self._n_poses = n_poses
         self._dim = dim
 
-    def pcm_run(self, odom_thr, lc_thr, precision=PCM_FP64_REPLAY, want_dist=False, margin=0.0):
+    def pcm_run(self, odom_thr, lc_thr, precision=PCM_FP64_REPLAY, want_dist=False, margin=0.0,
+                chain_coef=0.0):
         prm = PcmParams(odom_threshold=odom_thr, lc_threshold=lc_thr, precision=precision,
-                        want_dist=1 if want_dist else 0, recheck_margin=margin)
+                        want_dist=1 if want_dist else 0, recheck_margin=margin,
+                        recheck_chain_coef=chain_coef)
         st = PcmStats()
         check(lib().rpgo_pcm_run(self._h, ctypes.byref(prm), ctypes.byref(st)))
         self._m = st.n_accepted
         return st
 
-    def pcm_pairs(self, lc_thr, precision=PCM_FP64_REPLAY, want_dist=False, margin=0.0):
+    def pcm_pairs(self, lc_thr, precision=PCM_FP64_REPLAY, want_dist=False, margin=0.0,
+                  chain_coef=0.0):
         """Re-run only the pairwise stage on the last run's device-resident data."""
         prm = PcmParams(odom_threshold=-1.0, lc_threshold=lc_thr, precision=precision,
-                        want_dist=1 if want_dist else 0, recheck_margin=margin)
+                        want_dist=1 if want_dist else 0, recheck_margin=margin,
+                        recheck_chain_coef=chain_coef)
         st = PcmStats()
         check(lib().rpgo_pcm_pairs(self._h, ctypes.byref(prm), ctypes.byref(st)))
         return st

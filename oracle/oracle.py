@@ -73,6 +73,8 @@ def lib():
                                                 ctypes.c_int, _i32], ctypes.c_int),
             ("orc_max_clique_exact", [ctypes.c_int, _i32, _i32, _i32], ctypes.c_int),
             ("orc_max_clique_heu_dense", [ctypes.c_int, _u8, _i32], ctypes.c_int),
+            ("orc_set_threads", [ctypes.c_int], None),
+            ("orc_get_threads", [], ctypes.c_int),
             ("orc_pair_dist_list", [ctypes.c_int, _d, _i32, _i32, _d, _d, ctypes.c_int, _i32, _i32, _d],
              None),
         ]:
@@ -308,3 +310,12 @@ def pair_dist_list(dim, traj, lc_from, lc_to, lc_pose, lc_cov, pi, pj):
     lib().orc_pair_dist_list(dim, _p(traj), _p(lc_from, _i32), _p(lc_to, _i32), _p(lc_pose),
                              _p(lc_cov), len(pi), _p(pi, _i32), _p(pj, _i32), _p(out))
     return out[: len(pi)].copy()
+
+
+def set_threads(n):
+    """OpenMP threads of the distance loops (per-pair arithmetic is unchanged)."""
+    lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return int(lib().orc_get_threads())

@@ -26,8 +26,14 @@
 #include "rpgo_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
+
+/* Threads used by the row-parallel distance loops (1 = the reference's own
+ * single-threaded model; the arithmetic per pair is identical either way). */
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_get_threads(void) { return omp_get_max_threads(); }
 
 #ifndef M_PI
 #define M_PI 3.14159265358979323846
@@ -595,6 +601,7 @@ void orc_pcm_dist_rows(int dim, const double* traj, int m,
                        const double* lc_pose, const double* lc_cov,
                        int j0, int j1, double* dist) {
   const int pl = pose_len(dim), dd = tan_dim(dim) * tan_dim(dim);
+#pragma omp parallel for schedule(dynamic, 4)
   for (int j = j0; j < j1; ++j)
     for (int i = 0; i < j; ++i) {
       const double dv = orc_pair_dist(dim, traj, lc_from[i], lc_to[i],
@@ -829,6 +836,7 @@ void orc_pair_dist_list(int dim, const double* traj, const int32_t* lc_from,
                         const int32_t* lc_to, const double* lc_pose, const double* lc_cov,
                         int npairs, const int32_t* pi, const int32_t* pj, double* out) {
   const int pl = pose_len(dim), dd = tan_dim(dim) * tan_dim(dim);
+#pragma omp parallel for schedule(dynamic, 256)
   for (int t = 0; t < npairs; ++t) {
     const int i = pi[t], j = pj[t];
     out[t] = orc_pair_dist(dim, traj, lc_from[i], lc_to[i], lc_pose + (size_t)i * pl,

@@ -23,6 +23,10 @@
 extern "C" {
 #endif
 
+/* OpenMP threads of the distance loops (orc_pcm_dist_rows, orc_pair_dist_list). */
+void orc_set_threads(int n);
+int orc_get_threads(void);
+
 /* ---- geometry (exported for known-answer tests) ---- */
 void orc_se3_compose(const double* a, const double* b, double* out);
 void orc_se3_inverse(const double* a, double* out);

@@ -111,6 +111,31 @@ def test_pairs_rerun_matches_fresh_run(ctx):
     assert np.array_equal(ctx.pcm_download()["adj_bits"], a2)
 
 
+def test_fp32_large_graph_full_adjacency(ctx):
+    """50,000 poses / 10,000 loop closures: all 49,995,000 pairs of the FP32
+    path against the FP64 oracle (oracle row-parallel on the host cores)."""
+    import os
+    g, ot, lt = synth.workload("se3_50k", seed=1)
+    print(f"[fp32] se3_50k full N={g.n_poses} n_lc={g.n_lc}", flush=True)
+    ctx.pcm_upload(g.dim, g.pose0, g.odom, g.odom_cov, g.lc_from, g.lc_to, g.lc_pose, g.lc_cov)
+    st = ctx.pcm_run(ot, lt, precision=FP32)
+    got = ctx.pcm_download()
+    m = st.n_accepted
+    adj = _capi.unpack_bits(got["adj_bits"], m)
+    traj = O.fold(g.dim, g.pose0, g.odom, g.odom_cov)
+    O.set_threads(min(16, os.cpu_count() or 1))
+    try:
+        dist = O.pcm_dist_rows(3, traj, g.lc_from, g.lc_to, g.lc_pose, g.lc_cov)
+    finally:
+        O.set_threads(1)
+    ref = (dist < lt).astype(np.uint8)
+    np.fill_diagonal(ref, 0)
+    bad = np.argwhere(adj != ref)
+    print(f"[fp32] 50k full: pairs={st.n_pairs} rechecked={st.n_rechecked} flipped={st.n_flipped} "
+          f"mismatches={len(bad)}", flush=True)
+    assert len(bad) == 0, [(int(i), int(j), dist[i, j]) for i, j in bad[:8]]
+
+
 def test_fp32_large_graph_sampled_rows(ctx):
     """50,000-pose graph: adjacency rows spot-checked against the oracle, plus
     size-independent properties (symmetry, zero diagonal, consistent count)."""
