@@ -158,7 +158,9 @@ __device__ __forceinline__ void s6_congruence(const P32& T, const S6& S, S6& o) 
 }
 
 // SO(3) Log in FP32: theta from atan2(|axial|, (tr-1)/2); returns false when
-// the angle is within the near-pi band where FP32 loses the axis.
+// the angle is within ~3e-3 rad of pi (tr + 1 < 1e-5), where the axial vector
+// 2 sin(theta) n no longer fixes the axis to FP32 precision.  Farther from pi
+// the axis error is ~eps32 / sin(theta) < 2e-5, far inside the recheck band.
 __device__ __forceinline__ bool rot_log32(const float* R, float* w) {
   const float tr = R[0] + R[4] + R[8];
   const float ax = R[7] - R[5], ay = R[2] - R[6], az = R[3] - R[1];
@@ -171,7 +173,7 @@ __device__ __forceinline__ bool rot_log32(const float* R, float* w) {
   w[0] = mag * ax;
   w[1] = mag * ay;
   w[2] = mag * az;
-  return (tr + 1.0f) > 2e-2f;
+  return (tr + 1.0f) > 1e-5f;
 }
 
 // Pose3 Logmap in FP32 (same closed form as the reference's Pose3::Logmap)

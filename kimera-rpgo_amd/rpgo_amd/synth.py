@@ -110,7 +110,7 @@ def _dead_reckon3(R0, t0, Rz, tz):
 
 def se3_helix(n_poses=5000, n_lc=1000, outlier_frac=0.2, seed=0, sigma_t=0.02,
               sigma_r=0.005, period=100, pitch=2.0, jitter=0.1, lc_radius=3.0,
-              outlier_range=20.0):
+              outlier_range=20.0, init="odometry"):
     """Helical SE(3) trajectory with seeded loop closures.
 
     Ground truth: pose k sits on a helix of `period` poses per turn (radius
@@ -122,6 +122,10 @@ def se3_helix(n_poses=5000, n_lc=1000, outlier_frac=0.2, seed=0, sigma_t=0.02,
     `lc_radius` m; outliers join random poses |i-j| >= 2 with translation
     U[-outlier_range, outlier_range]^3 and a uniformly random rotation.
     LC direction (later->earlier or earlier->later) and order are shuffled.
+    Initial values: init="odometry" dead-reckons the odometry from pose 0 (what
+    a batch load sees); init="perturbed" perturbs every true pose by
+    Exp(N(0, sigma_r) x3, N(0, sigma_t) x3) (an estimate carried over from an
+    earlier optimisation).  The random stream of the graph is the same for both.
     """
     rng = np.random.default_rng(seed)
     k = np.arange(n_poses, dtype=np.float64)
@@ -189,7 +193,13 @@ def se3_helix(n_poses=5000, n_lc=1000, outlier_frac=0.2, seed=0, sigma_t=0.02,
     perm = rng.permutation(n_lc)
     lc_cov = np.tile(cov, (n_lc, 1))
 
-    Rv, tv = _dead_reckon3(Rgt[0], tgt[0], Rz, tz)
+    if init == "perturbed":
+        rng2 = np.random.default_rng([seed, 1])
+        Rv = Rgt @ rot_exp(rng2.normal(0, sigma_r, (n_poses, 3)))
+        tv = tgt + rng2.normal(0, sigma_t, (n_poses, 3))
+        Rv[0], tv[0] = Rgt[0], tgt[0]
+    else:
+        Rv, tv = _dead_reckon3(Rgt[0], tgt[0], Rz, tz)
     return PoseGraph(
         dim=3, gt=se3_pack(Rgt, tgt), values=se3_pack(Rv, tv), odom=se3_pack(Rz, tz),
         odom_cov=odom_cov, lc_from=frm2[perm], lc_to=to2[perm],
