@@ -133,6 +133,78 @@ int rpgo_max_clique(rpgo_ctx* ctx, int32_t mode, int32_t* idx_out, int32_t* k_ou
 int rpgo_max_clique_bits(rpgo_ctx* ctx, int32_t n, const uint64_t* adj_bits,
                          int32_t mode, int32_t* idx_out, int32_t* k_out);
 
+/* ------------------------------------------------------------------------
+ * Robust solve: GNC-TLS over Levenberg-Marquardt / Gauss-Newton with a
+ * block-Jacobi PCG linear step.  Replaces RobustSolver::optimize
+ * (src/RobustSolver.cpp:115-306): gtsam::GncOptimizer with known inliers
+ * (:105-113, :130-180, :225-275) or plain LevenbergMarquardtOptimizer /
+ * GaussNewtonOptimizer (:181-218, :276-296).  SE(3) poses only.
+ * ---------------------------------------------------------------------- */
+#define RPGO_SOLVER_LM 0         /* Solver::LM (SolverParams.h:13) */
+#define RPGO_SOLVER_GN 1         /* Solver::GN */
+#define RPGO_FACTOR_BETWEEN 0    /* BetweenFactor<Pose3>(i, j, z) */
+#define RPGO_FACTOR_PRIOR 1      /* PriorFactor<Pose3>(i, prior)  */
+#define RPGO_GNC_COST 0          /* GncParams::GncThresholdMode::COST */
+#define RPGO_GNC_PROBABILITY 1   /* GncParams::GncThresholdMode::PROBABILITY */
+
+typedef struct {
+  int32_t n_poses;
+  const double* poses;        /* n_poses * 12 initial values                  */
+  int32_t n_factors;
+  const int32_t* kind;        /* RPGO_FACTOR_BETWEEN | RPGO_FACTOR_PRIOR       */
+  const int32_t* key_i;       /* pose index of the (first) key                 */
+  const int32_t* key_j;       /* second pose of a between factor (else ignored) */
+  const double* measured;     /* n_factors * 12: z of a between, prior pose     */
+  const double* information;  /* n_factors * 36, (omega, v) tangent order       */
+} rpgo_solve_problem;
+
+typedef struct {
+  int32_t solver;               /* RPGO_SOLVER_LM | RPGO_SOLVER_GN                 */
+  int32_t use_gnc;              /* RobustSolverParams::use_gnc_                    */
+  int32_t known_inliers;        /* the first k factors are known inliers (odometry
+                                   then special factors, RobustSolver.cpp:105-113) */
+  int32_t gnc_threshold_mode;   /* RPGO_GNC_COST | RPGO_GNC_PROBABILITY            */
+  double gnc_inlier_threshold;  /* cost, or probability alpha                      */
+  int32_t gnc_max_iterations;   /* GncParams::max_iterations_ (100)               */
+  double gnc_mu_step;           /* 1.4                                            */
+  double gnc_relative_cost_tol; /* 1e-5                                           */
+  double gnc_weights_tol;       /* 1e-4                                           */
+  int32_t gnc_bias_odom;        /* initial weights 1 on known inliers, 0 elsewhere */
+  int32_t lm_diagonal_damping;  /* RobustSolverParams::lm_diagonal_damping (1)     */
+  const double* init_weights;   /* NULL, or n_factors initial GNC weights          */
+  double init_mu;               /* <= 0: GNC's own mu initialisation               */
+  double lambda_initial;        /* LM defaults: 1e-5, 10, 1e5, 1e-3                */
+  double lambda_factor;
+  double lambda_upper_bound;
+  double min_model_fidelity;
+  int32_t lm_max_iterations;    /* 100 */
+  double lm_relative_error_tol; /* 1e-5 */
+  double lm_absolute_error_tol; /* 1e-5 */
+  int32_t pcg_max_iterations;   /* <= 0: 6 * n_poses */
+  double pcg_rel_tol;           /* relative residual of each linear solve */
+  int32_t polish_iterations;    /* final Gauss-Newton refinement steps (0: none) */
+} rpgo_solve_params;
+
+typedef struct {
+  int32_t gnc_iterations;
+  int32_t lm_iterations;
+  int32_t pcg_iterations;
+  int32_t linearizations;
+  double mu_init;
+  double mu_final;
+  double final_cost;            /* weighted cost of the returned poses */
+  double ms_total;              /* device-stream time of the solve     */
+  int32_t n_inliers;            /* truncated weight sum minus known inliers
+                                   (RobustSolver.cpp:170-171)           */
+  int32_t reserved;
+} rpgo_solve_report;
+
+/* Fill *p with the reference defaults (SolverParams.h:60-94, GTSAM LM). */
+void rpgo_solve_params_default(rpgo_solve_params* p);
+/* Solve; poses_out: n_poses * 12, weights_out: n_factors (may be NULL). */
+int rpgo_solve(rpgo_ctx* ctx, const rpgo_solve_problem* prob, const rpgo_solve_params* params,
+               double* poses_out, double* weights_out, rpgo_solve_report* report);
+
 #ifdef __cplusplus
 }
 #endif

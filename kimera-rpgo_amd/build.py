@@ -28,7 +28,8 @@ COMMON = [
 EXTRA = {
     "replay64.hip": ["-ffp-contract=off"],
 }
-SOURCES = ["capi.hip", "replay64.hip", "pcm_fp32.hip", "clique.hip"]
+SOURCES = ["capi.hip", "replay64.hip", "pcm_fp32.hip", "clique.hip", "solver.hip",
+           "solver_capi.hip"]
 
 
 def _compile(src: str) -> str:

@@ -91,6 +91,8 @@ struct CliqueState {
   DevBuf deg, dmask, icc, out, scratch, best;
 };
 
+struct SolverRun;  // solver.hip / rpgo_solver.h
+
 }  // namespace rpgo
 
 struct rpgo_ctx {
@@ -99,6 +101,7 @@ struct rpgo_ctx {
   hipEvent_t ev[16] = {};
   rpgo::PcmState pcm;
   rpgo::CliqueState clq;
+  rpgo::SolverRun* solver = nullptr;  // created on the first rpgo_solve
 };
 
 namespace rpgo {
@@ -121,6 +124,8 @@ constexpr int kRecheckQueueCap = 1 << 22;
 //   u64[0] consistent pairs (FP64 kernel)   u64[1] popcount of the adjacency
 //   u64[2] FP64 recheck flips               u32[16] recheck queue length
 //   i32[32] "some covariance has no rotation information" flag
+// solver_capi.hip
+void solver_destroy(rpgo_ctx* ctx);
 // clique.hip
 int clique_run(rpgo_ctx* ctx, const uint64_t* d_adj, int n, int mode, int32_t* idx_out,
                int32_t* k_out);

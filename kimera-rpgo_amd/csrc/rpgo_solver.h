@@ -1,0 +1,55 @@
+// Internal state of the robust pose-graph solve (solver.hip).
+#pragma once
+
+#include <cstring>
+
+#include "rpgo_internal.h"
+
+namespace rpgo {
+
+struct SolverRun {
+  struct Stats {
+    int lm_iterations = 0;
+    int lm_rejections = 0;
+    int pcg_iterations = 0;
+    int linearizations = 0;
+    double mu_init = -1.0;
+  };
+  int N = 0, F = 0, n_known = 0;
+  rpgo_solve_params prm{};
+  Stats stats;
+  // problem (device)
+  DevBuf x0, x, xn;             // FP64 poses
+  DevBuf kind, fi, fj, z, info;  // factors
+  DevBuf w, barc, known, err, cost;
+  // Gauss-Newton system
+  DevBuf Hii, Hjj, Hij, gi, gj;  // per-factor contributions
+  DevBuf inc_off, inc_cnt, inc;  // pose -> incident factors
+  DevBuf D, Minv, hdiag, b64;
+  DevBuf b, dx, r, zv, p, q, hx;  // PCG vectors (FP32)
+  DevBuf sc;                      // FP64 scalars
+
+  int alloc(int N_, int F_);
+  int build_incidence(hipStream_t st);
+  int linearize(hipStream_t st);
+  void spmv(hipStream_t st, double lambda, const float* x, float* y);
+  int pcg(hipStream_t st, double lambda, int* iters);
+  int model_decrease(hipStream_t st, double* out);
+  int total_cost(hipStream_t st, const double* xp, double* out);
+  int compute_errors(hipStream_t st, const double* xp);
+  int nonlinear_solve(hipStream_t st, bool from_x0, double* cost, int* iters);
+  int read_scalars(hipStream_t st, int count, double* out);
+  int run(hipStream_t st, rpgo_solve_report* rep);
+  double* cost_buf() { return cost.as<double>(); }
+  void release() {
+    for (DevBuf* v : {&x0, &x, &xn, &kind, &fi, &fj, &z, &info, &w, &barc, &known, &err, &cost, &Hii,
+                      &Hjj, &Hij, &gi, &gj, &inc_off, &inc_cnt, &inc, &D, &Minv, &hdiag, &b64, &b,
+                      &dx, &r, &zv, &p, &q, &hx, &sc})
+      v->release();
+  }
+};
+
+// chi^2 quantile (inverse regularized lower incomplete gamma), host side
+double chi2_quantile(double p, double dof);
+
+}  // namespace rpgo

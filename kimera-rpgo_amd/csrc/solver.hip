@@ -1,0 +1,998 @@
+// Robust pose-graph solve on gfx950: GNC-TLS over Levenberg-Marquardt or
+// Gauss-Newton, with block-Jacobi PCG for every linear step.
+//
+// Semantics follow src/RobustSolver.cpp:115-306 (GNC wiring, known inliers,
+// thresholds, bias_odom initial weights) and the GTSAM algorithms it drives:
+// GncOptimizer (TLS weights, mu init/update, convergence tests, every inner
+// solve restarted from the initial values), LevenbergMarquardtOptimizer
+// (diagonal damping, lambda schedule, model-fidelity acceptance) and
+// BetweenFactor<Pose3> with SLOW_BUT_CORRECT_BETWEENFACTOR under the EXPMAP
+// retraction: e = Log(z^-1 T_i^-1 T_j), J_j = Jr^-1(e),
+// J_i = -Jr^-1(e) Ad((T_i^-1 T_j)^-1); PriorFactor e = -Log(x^-1 prior), J = I.
+//
+// Precision split: residuals, Jacobians, gradient and costs in FP64 (so the
+// fixed point of the Gauss-Newton iteration is the FP64 one: each outer step
+// is an iterative-refinement step), the 6x6 Hessian blocks and the PCG
+// vectors in FP32, PCG dot products and norms accumulated in FP64.
+//
+// Layout in HBM: poses as FP64 (R row-major, t); factors SoA-by-record; the
+// Gauss-Newton system in block form: one 6x6 FP32 diagonal block per pose and
+// one off-diagonal block per between factor (its transpose implied), with a
+// pose->incident-factor index (sorted by factor id) for deterministic
+// row-parallel assembly and SpMV.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "replay64.h"
+#include "rpgo_internal.h"
+#include "rpgo_solver.h"
+
+namespace rpgo {
+
+using R3 = R64<3>;
+
+// ------------------------------------------------------------------ FP64 SE(3)
+__device__ __forceinline__ void skew3(const double* w, double* W) {
+  W[0] = 0; W[1] = -w[2]; W[2] = w[1];
+  W[3] = w[2]; W[4] = 0; W[5] = -w[0];
+  W[6] = -w[1]; W[7] = w[0]; W[8] = 0;
+}
+
+__device__ __forceinline__ void mm3(const double* A, const double* B, double* C) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      C[i * 3 + j] = A[i * 3] * B[j] + A[i * 3 + 1] * B[3 + j] + A[i * 3 + 2] * B[6 + j];
+}
+
+// Rodrigues + Pose3::Expmap closed form (t = (w x v - R (w x v) + w w.v) / |w|^2)
+__device__ void se3_exp64(const double* xi, double* T) {
+  const double* w = xi;
+  const double* v = xi + 3;
+  const double th2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+  const double th = sqrt(th2);
+  double A, B;
+  if (th2 < 1e-20) { A = 1.0 - th2 / 6.0; B = 0.5 - th2 / 24.0; }
+  else { A = sin(th) / th; B = (1.0 - cos(th)) / th2; }
+  double W[9], W2[9];
+  skew3(w, W);
+  mm3(W, W, W2);
+#pragma unroll
+  for (int e = 0; e < 9; ++e) T[e] = ((e % 4 == 0) ? 1.0 : 0.0) + A * W[e] + B * W2[e];
+  if (th2 > 2.220446049250313e-16) {
+    const double wv = w[0] * v[0] + w[1] * v[1] + w[2] * v[2];
+    const double c[3] = {w[1] * v[2] - w[2] * v[1], w[2] * v[0] - w[0] * v[2], w[0] * v[1] - w[1] * v[0]};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const double Rc = T[i * 3] * c[0] + T[i * 3 + 1] * c[1] + T[i * 3 + 2] * c[2];
+      T[9 + i] = (c[i] - Rc + w[i] * wv) / th2;
+    }
+  } else {
+    T[9] = v[0]; T[10] = v[1]; T[11] = v[2];
+  }
+}
+
+// SE(3) LogmapDerivative (Jr^-1): [[Jw, 0], [-Jw Q Jw, Jw]], Q of Barfoot14tro eq. 102
+__device__ void se3_logmap_derivative64(const double* xi, double* J) {
+  const double* w = xi;
+  const double* v = xi + 3;
+  const double th2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+  double W[9], W2[9], Jw[9];
+  skew3(w, W);
+  mm3(W, W, W2);
+  if (th2 <= 2.220446049250313e-16) {
+#pragma unroll
+    for (int e = 0; e < 9; ++e) Jw[e] = (e % 4 == 0) ? 1.0 : 0.0;
+  } else {
+    const double th = sqrt(th2);
+    const double coef = 1.0 / th2 - (1.0 + cos(th)) / (2.0 * th * sin(th));
+#pragma unroll
+    for (int e = 0; e < 9; ++e) Jw[e] = ((e % 4 == 0) ? 1.0 : 0.0) + 0.5 * W[e] + coef * W2[e];
+  }
+  double V[9], WV[9], VW[9], WVW[9], WWV[9], VWW[9], WVWW[9], WWVW[9];
+  skew3(v, V);
+  mm3(W, V, WV);
+  mm3(V, W, VW);
+  mm3(WV, W, WVW);
+  mm3(W2, V, WWV);
+  mm3(V, W2, VWW);
+  mm3(WVW, W, WVWW);
+  mm3(W, WVW, WWVW);
+  const double phi = sqrt(th2);
+  double a, b, c;
+  if (phi > 1e-5) {
+    const double s = sin(phi), co = cos(phi);
+    const double p2 = phi * phi, p3 = p2 * phi, p4 = p3 * phi, p5 = p4 * phi;
+    a = (phi - s) / p3;
+    b = (1 - p2 / 2 - co) / p4;
+    c = -0.5 * ((1 - p2 / 2 - co) / p4 - 3 * (phi - s - p3 / 6.0) / p5);
+  } else {
+    a = 1.0 / 6.0;
+    b = -1.0 / 24.0;
+    c = 1.0 / 120.0;
+  }
+  double Q[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e)
+    Q[e] = -0.5 * V[e] + a * (WV[e] + VW[e] - WVW[e]) + b * (WWV[e] + VWW[e] - 3 * WVW[e]) +
+           c * (WVWW[e] + WWVW[e]);
+  double JQ[9], Q2[9];
+  mm3(Jw, Q, JQ);
+  mm3(JQ, Jw, Q2);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      J[i * 6 + j] = Jw[i * 3 + j];
+      J[i * 6 + j + 3] = 0.0;
+      J[(i + 3) * 6 + j] = -Q2[i * 3 + j];
+      J[(i + 3) * 6 + j + 3] = Jw[i * 3 + j];
+    }
+}
+
+// ------------------------------------------------------------------ factors
+// Whitened-cost and Jacobians of one factor at poses x.
+// kind 0: between(i, j), kind 1: prior(i).
+struct FacLin {
+  double e[6];
+  double Ji[36];
+  double Jj[36];
+};
+
+__device__ void factor_error(int kind, const double* Ti, const double* Tj, const double* z,
+                            double* e) {
+  if (kind == 0) {
+    double Tij[12], zi[12], E[12];
+    double Tii[12];
+    R3::pose_inverse(Ti, Tii);
+    R3::pose_compose(Tii, Tj, Tij);
+    R3::pose_inverse(z, zi);
+    R3::pose_compose(zi, Tij, E);
+    R3::logmap(E, e);
+  } else {
+    double Tii[12], E[12];
+    R3::pose_inverse(Ti, Tii);
+    R3::pose_compose(Tii, z, E);
+    R3::logmap(E, e);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) e[k] = -e[k];
+  }
+}
+
+__device__ void factor_linearize(int kind, const double* Ti, const double* Tj, const double* z,
+                                 FacLin& L) {
+  if (kind == 0) {
+    double Tii[12], Tij[12], zi[12], E[12];
+    R3::pose_inverse(Ti, Tii);
+    R3::pose_compose(Tii, Tj, Tij);
+    R3::pose_inverse(z, zi);
+    R3::pose_compose(zi, Tij, E);
+    R3::logmap(E, L.e);
+    double Jr[36], Tji[12], Ad[36];
+    se3_logmap_derivative64(L.e, Jr);
+    R3::pose_inverse(Tij, Tji);
+    R3::adjoint(Tji, Ad);
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) s += Jr[r * 6 + k] * Ad[k * 6 + c];
+        L.Ji[r * 6 + c] = -s;
+        L.Jj[r * 6 + c] = Jr[r * 6 + c];
+      }
+  } else {
+    factor_error(1, Ti, Tj, z, L.e);
+#pragma unroll
+    for (int r = 0; r < 36; ++r) {
+      L.Ji[r] = (r % 7 == 0) ? 1.0 : 0.0;
+      L.Jj[r] = 0.0;
+    }
+  }
+}
+
+__device__ __forceinline__ double quad6(const double* e, const double* L) {
+  double q = 0.0;
+#pragma unroll
+  for (int r = 0; r < 6; ++r) {
+    double s = 0.0;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) s += L[r * 6 + c] * e[c];
+    q += e[r] * s;
+  }
+  return q;
+}
+
+// 0.5 e^T Lambda e per factor (unweighted) at poses x
+__global__ void k_factor_errors(int F, const int* __restrict__ kind, const int* __restrict__ fi,
+                                const int* __restrict__ fj, const double* __restrict__ z,
+                                const double* __restrict__ info, const double* __restrict__ x,
+                                double* __restrict__ err) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  double e[6];
+  const int k = kind[f];
+  factor_error(k, x + (size_t)fi[f] * 12, x + (size_t)(k == 0 ? fj[f] : fi[f]) * 12,
+               z + (size_t)f * 12, e);
+  err[f] = 0.5 * quad6(e, info + (size_t)f * 36);
+}
+
+// per-factor Gauss-Newton contributions: blocks (FP32), gradients and cost (FP64)
+__global__ void k_linearize(int F, const int* __restrict__ kind, const int* __restrict__ fi,
+                            const int* __restrict__ fj, const double* __restrict__ z,
+                            const double* __restrict__ info, const double* __restrict__ w,
+                            const double* __restrict__ x, float* __restrict__ Hii,
+                            float* __restrict__ Hjj, float* __restrict__ Hij,
+                            double* __restrict__ gi, double* __restrict__ gj,
+                            double* __restrict__ cost) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  const int k = kind[f];
+  FacLin L;
+  factor_linearize(k, x + (size_t)fi[f] * 12, x + (size_t)(k == 0 ? fj[f] : fi[f]) * 12,
+                   z + (size_t)f * 12, L);
+  const double wf = w[f];
+  const double* I = info + (size_t)f * 36;
+  // LJ = w Lambda J (6x6) for J = Ji, Jj
+  double LJi[36], LJj[36], Le[6];
+#pragma unroll
+  for (int r = 0; r < 6; ++r) {
+    double se = 0.0;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      double si = 0.0, sj = 0.0;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) {
+        si += I[r * 6 + m] * L.Ji[m * 6 + c];
+        sj += I[r * 6 + m] * L.Jj[m * 6 + c];
+      }
+      LJi[r * 6 + c] = wf * si;
+      LJj[r * 6 + c] = wf * sj;
+      se += I[r * 6 + c] * L.e[c];
+    }
+    Le[r] = wf * se;
+  }
+  double q = 0.0;
+#pragma unroll
+  for (int r = 0; r < 6; ++r) q += L.e[r] * Le[r];
+  cost[f] = 0.5 * q;
+#pragma unroll
+  for (int r = 0; r < 6; ++r) {
+    double a = 0.0, b = 0.0;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      a += L.Ji[m * 6 + r] * Le[m];
+      b += L.Jj[m * 6 + r] * Le[m];
+    }
+    gi[(size_t)f * 6 + r] = a;
+    gj[(size_t)f * 6 + r] = b;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      double sii = 0.0, sjj = 0.0, sij = 0.0;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) {
+        sii += L.Ji[m * 6 + r] * LJi[m * 6 + c];
+        sjj += L.Jj[m * 6 + r] * LJj[m * 6 + c];
+        sij += L.Ji[m * 6 + r] * LJj[m * 6 + c];
+      }
+      Hii[(size_t)f * 36 + r * 6 + c] = (float)sii;
+      Hjj[(size_t)f * 36 + r * 6 + c] = (float)sjj;
+      Hij[(size_t)f * 36 + r * 6 + c] = (float)sij;
+    }
+  }
+}
+
+// Incidence: for pose p, entries inc[off[p]..off[p+1]) = 2 f + side (side 0: f's
+// first key, side 1: second key), sorted by f.
+__global__ void k_inc_count(int F, const int* __restrict__ kind, const int* __restrict__ fi,
+                            const int* __restrict__ fj, int* __restrict__ cnt) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  atomicAdd(&cnt[fi[f]], 1);
+  if (kind[f] == 0) atomicAdd(&cnt[fj[f]], 1);
+}
+
+__global__ void __launch_bounds__(1024) k_exclusive_scan(int n, const int* __restrict__ in,
+                                                         int* __restrict__ out) {
+  __shared__ int wsum[16];
+  __shared__ int base;
+  if (threadIdx.x == 0) base = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int start = 0; start < n; start += 1024) {
+    const int k = start + threadIdx.x;
+    const int v = k < n ? in[k] : 0;
+    int incl = v;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int o = __shfl_up(incl, off);
+      if (lane >= off) incl += o;
+    }
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    int pre = base;
+    for (int w = 0; w < wid; ++w) pre += wsum[w];
+    if (k < n) out[k] = pre + incl - v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int t = 0;
+      for (int w = 0; w < 16; ++w) t += wsum[w];
+      base += t;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[n] = base;
+}
+
+__global__ void k_inc_fill(int F, const int* __restrict__ kind, const int* __restrict__ fi,
+                           const int* __restrict__ fj, const int* __restrict__ off,
+                           int* __restrict__ cursor, int* __restrict__ inc) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  inc[off[fi[f]] + atomicAdd(&cursor[fi[f]], 1)] = 2 * f;
+  if (kind[f] == 0) inc[off[fj[f]] + atomicAdd(&cursor[fj[f]], 1)] = 2 * f + 1;
+}
+
+// insertion sort of each pose's (short) incidence list, for deterministic sums
+__global__ void k_inc_sort(int N, const int* __restrict__ off, int* __restrict__ inc) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const int a = off[p], b = off[p + 1];
+  for (int k = a + 1; k < b; ++k) {
+    const int v = inc[k];
+    int m = k - 1;
+    while (m >= a && inc[m] > v) {
+      inc[m + 1] = inc[m];
+      --m;
+    }
+    inc[m + 1] = v;
+  }
+}
+
+// Diagonal blocks, gradient rhs b = -g (FP32 for PCG), Hessian diagonal and the
+// block-Jacobi inverse of (D + lambda diag(D)) (or + lambda I without diagonal
+// damping), one thread per pose.
+__global__ void k_assemble(int N, const int* __restrict__ off, const int* __restrict__ inc,
+                           const float* __restrict__ Hii, const float* __restrict__ Hjj,
+                           const double* __restrict__ gi, const double* __restrict__ gj,
+                           float* __restrict__ D, double* __restrict__ bvec64,
+                           float* __restrict__ bvec, float* __restrict__ hdiag) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  float acc[36];
+  double g[6];
+#pragma unroll
+  for (int e = 0; e < 36; ++e) acc[e] = 0.f;
+#pragma unroll
+  for (int e = 0; e < 6; ++e) g[e] = 0.0;
+  for (int k = off[p]; k < off[p + 1]; ++k) {
+    const int t = inc[k], f = t >> 1;
+    const float* B = (t & 1) ? Hjj + (size_t)f * 36 : Hii + (size_t)f * 36;
+    const double* gg = (t & 1) ? gj + (size_t)f * 6 : gi + (size_t)f * 6;
+#pragma unroll
+    for (int e = 0; e < 36; ++e) acc[e] += B[e];
+#pragma unroll
+    for (int e = 0; e < 6; ++e) g[e] += gg[e];
+  }
+#pragma unroll
+  for (int e = 0; e < 36; ++e) D[(size_t)p * 36 + e] = acc[e];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) {
+    bvec64[(size_t)p * 6 + e] = -g[e];
+    bvec[(size_t)p * 6 + e] = (float)(-g[e]);
+    hdiag[(size_t)p * 6 + e] = acc[e * 7];
+  }
+}
+
+// block-Jacobi preconditioner: inverse of D_p + lambda * damp_p (FP64 Cholesky)
+__global__ void k_precond(int N, const float* __restrict__ D, const float* __restrict__ hdiag,
+                          double lambda, int diag_damping, double min_diag, double max_diag,
+                          float* __restrict__ Minv) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  double A[36];
+#pragma unroll
+  for (int e = 0; e < 36; ++e) A[e] = D[(size_t)p * 36 + e];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    double d = diag_damping ? fmin(fmax((double)hdiag[(size_t)p * 6 + i], min_diag), max_diag) : 1.0;
+    A[i * 7] += lambda * d;
+  }
+  // Cholesky A = L L^T (lower, in place), tiny pivots floored
+  double L[36];
+#pragma unroll
+  for (int e = 0; e < 36; ++e) L[e] = 0.0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    double x = A[k * 7];
+#pragma unroll
+    for (int j = 0; j < k; ++j) x -= L[k * 6 + j] * L[k * 6 + j];
+    x = sqrt(fmax(x, 1e-300));
+    L[k * 7] = x;
+#pragma unroll
+    for (int i = k + 1; i < 6; ++i) {
+      double v = A[i * 6 + k];
+#pragma unroll
+      for (int j = 0; j < k; ++j) v -= L[i * 6 + j] * L[k * 6 + j];
+      L[i * 6 + k] = v / x;
+    }
+  }
+  // inverse via L^-1
+  double Li[36];
+#pragma unroll
+  for (int e = 0; e < 36; ++e) Li[e] = 0.0;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      if (i < c) continue;
+      double v = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+      for (int j = 0; j < 6; ++j)
+        if (j >= c && j < i) v -= L[i * 6 + j] * Li[j * 6 + c];
+      Li[i * 6 + c] = v / L[i * 7];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) s += Li[k * 6 + i] * Li[k * 6 + j];
+      Minv[(size_t)p * 36 + i * 6 + j] = (float)s;
+    }
+}
+
+// y = (H + lambda damp) x : one thread per pose row (deterministic order)
+__global__ void k_spmv(int N, const int* __restrict__ off, const int* __restrict__ inc,
+                       const int* __restrict__ fi, const int* __restrict__ fj,
+                       const int* __restrict__ kind, const float* __restrict__ D,
+                       const float* __restrict__ Hij, const float* __restrict__ hdiag, double lambda,
+                       int diag_damping, double min_diag, double max_diag,
+                       const float* __restrict__ x, float* __restrict__ y) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  float xp[6], acc[6];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) xp[e] = x[(size_t)p * 6 + e];
+#pragma unroll
+  for (int r = 0; r < 6; ++r) {
+    float s = 0.f;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) s += D[(size_t)p * 36 + r * 6 + c] * xp[c];
+    const double d = diag_damping ? fmin(fmax((double)hdiag[(size_t)p * 6 + r], min_diag), max_diag) : 1.0;
+    acc[r] = s + (float)(lambda * d) * xp[r];
+  }
+  for (int k = off[p]; k < off[p + 1]; ++k) {
+    const int t = inc[k], f = t >> 1;
+    if (kind[f] != 0) continue;
+    const float* B = Hij + (size_t)f * 36;
+    if ((t & 1) == 0) {  // row i: + Hij x_j
+      const float* xq = x + (size_t)fj[f] * 6;
+#pragma unroll
+      for (int r = 0; r < 6; ++r) {
+        float s = 0.f;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) s += B[r * 6 + c] * xq[c];
+        acc[r] += s;
+      }
+    } else {  // row j: + Hij^T x_i
+      const float* xq = x + (size_t)fi[f] * 6;
+#pragma unroll
+      for (int r = 0; r < 6; ++r) {
+        float s = 0.f;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) s += B[c * 6 + r] * xq[c];
+        acc[r] += s;
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 6; ++r) y[(size_t)p * 6 + r] = acc[r];
+}
+
+// FP64 dot of FP32 vectors, wave-reduced then atomically accumulated
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+__global__ void k_dot(long long n, const float* __restrict__ a, const float* __restrict__ b,
+                      double* __restrict__ out) {
+  double s = 0.0;
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (long long)gridDim.x * blockDim.x)
+    s += (double)a[k] * (double)b[k];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) atomicAdd(out, s);
+}
+
+// z = M^-1 r (block-Jacobi), and rz += r . z
+__global__ void k_apply_precond(int N, const float* __restrict__ Minv, const float* __restrict__ r,
+                                float* __restrict__ z, double* __restrict__ rz) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  double s = 0.0;
+  if (p < N) {
+    float rp[6];
+#pragma unroll
+    for (int e = 0; e < 6; ++e) rp[e] = r[(size_t)p * 6 + e];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      float v = 0.f;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) v += Minv[(size_t)p * 36 + i * 6 + j] * rp[j];
+      z[(size_t)p * 6 + i] = v;
+      s += (double)rp[i] * (double)v;
+    }
+  }
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0 && s != 0.0) atomicAdd(rz, s);
+}
+
+// scalars: sc[0] = rz, sc[1] = pq, sc[2] = rz_new, sc[3] = |r|^2
+// x += alpha p ; r -= alpha q ; alpha = rz / pq ; accumulates |r|^2 into sc[3]
+__global__ void k_cg_update_xr(long long n, const double* __restrict__ sc, const float* __restrict__ p,
+                               const float* __restrict__ q, float* __restrict__ x,
+                               float* __restrict__ r, double* __restrict__ rr) {
+  const double pq = sc[1];
+  const float alpha = (float)(pq != 0.0 ? sc[0] / pq : 0.0);
+  double s = 0.0;
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (long long)gridDim.x * blockDim.x) {
+    x[k] += alpha * p[k];
+    const float rv = r[k] - alpha * q[k];
+    r[k] = rv;
+    s += (double)rv * (double)rv;
+  }
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) atomicAdd(rr, s);
+}
+
+// p = z + beta p, beta = rz_new / rz ; then rz <- rz_new, clear pq / rz_new / rr
+__global__ void k_cg_update_p(long long n, const double* __restrict__ sc, const float* __restrict__ z,
+                              float* __restrict__ p) {
+  const double rz = sc[0];
+  const float beta = (float)(rz != 0.0 ? sc[2] / rz : 0.0);
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (long long)gridDim.x * blockDim.x)
+    p[k] = z[k] + beta * p[k];
+}
+
+__global__ void k_cg_roll(double* sc) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    sc[0] = sc[2];
+    sc[1] = 0.0;
+    sc[2] = 0.0;
+    sc[3] = 0.0;
+  }
+}
+
+__global__ void k_copy_f(long long n, const float* __restrict__ a, float* __restrict__ b) {
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (long long)gridDim.x * blockDim.x)
+    b[k] = a[k];
+}
+
+// x_new = x * Exp(delta) per pose
+__global__ void k_retract(int N, const double* __restrict__ x, const float* __restrict__ dx,
+                          double* __restrict__ xn) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  double d[6], E[12], o[12];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) d[e] = dx[(size_t)p * 6 + e];
+  se3_exp64(d, E);
+  R3::pose_compose(x + (size_t)p * 12, E, o);
+#pragma unroll
+  for (int e = 0; e < 12; ++e) xn[(size_t)p * 12 + e] = o[e];
+}
+
+__global__ void k_sum(long long n, const double* __restrict__ a, const double* __restrict__ w,
+                      double* __restrict__ out) {
+  double s = 0.0;
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (long long)gridDim.x * blockDim.x)
+    s += w ? a[k] * w[k] : a[k];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) atomicAdd(out, s);
+}
+
+// b . dx in FP64 (b kept in FP64)
+__global__ void k_dot_df(long long n, const double* __restrict__ a, const float* __restrict__ b,
+                         double* __restrict__ out) {
+  double s = 0.0;
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (long long)gridDim.x * blockDim.x)
+    s += a[k] * (double)b[k];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) atomicAdd(out, s);
+}
+
+// TLS weights (GncOptimizer::calculateWeights), known inliers keep weight 1
+__global__ void k_tls_weights(int F, const double* __restrict__ err, const double* __restrict__ barc,
+                              const unsigned char* __restrict__ known, double mu,
+                              double* __restrict__ w) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  if (known[f]) {
+    w[f] = 1.0;
+    return;
+  }
+  const double u2 = err[f], b = barc[f];
+  const double ub = (mu + 1.0) / mu * b, lb = mu / (mu + 1.0) * b;
+  double wk = sqrt(b * mu * (mu + 1.0) / u2) - mu;
+  if (u2 >= ub || wk < 0.0) wk = 0.0;
+  else if (u2 <= lb || wk > 1.0) wk = 1.0;
+  w[f] = wk;
+}
+
+// candidate mu per factor: barc / (2 r - barc) when positive, +inf otherwise
+__global__ void k_mu_candidates(int F, const double* __restrict__ err,
+                                const double* __restrict__ barc, double* __restrict__ out) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  const double den = 2.0 * err[f] - barc[f];
+  out[f] = den > 0.0 ? barc[f] / den : INFINITY;
+}
+
+__global__ void k_min(long long n, const double* __restrict__ a, unsigned long long* __restrict__ out) {
+  double m = INFINITY;
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (long long)gridDim.x * blockDim.x)
+    m = fmin(m, a[k]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off));
+  // non-negative doubles order like their bit patterns
+  if ((threadIdx.x & 63) == 0) atomicMin(out, (unsigned long long)__double_as_longlong(m));
+}
+
+// weights binary within tol: counts the violations
+__global__ void k_weights_binary(int F, const double* __restrict__ w, double tol,
+                                 unsigned int* __restrict__ bad) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  if (fabs(w[f] - rint(w[f])) > tol) atomicAdd(bad, 1u);
+}
+
+// ------------------------------------------------------------------ host driver
+static inline unsigned grid_for(long long n, int block = 256, long long cap = 2048) {
+  long long g = (n + block - 1) / block;
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+int SolverRun::alloc(int N_, int F_) {
+  N = N_;
+  F = F_;
+  int rc;
+  const size_t n6 = (size_t)N * 6;
+  if ((rc = x0.reserve(sizeof(double) * 12 * N))) return rc;
+  if ((rc = x.reserve(sizeof(double) * 12 * N))) return rc;
+  if ((rc = xn.reserve(sizeof(double) * 12 * N))) return rc;
+  if ((rc = kind.reserve(sizeof(int) * F))) return rc;
+  if ((rc = fi.reserve(sizeof(int) * F))) return rc;
+  if ((rc = fj.reserve(sizeof(int) * F))) return rc;
+  if ((rc = z.reserve(sizeof(double) * 12 * F))) return rc;
+  if ((rc = info.reserve(sizeof(double) * 36 * F))) return rc;
+  if ((rc = w.reserve(sizeof(double) * F))) return rc;
+  if ((rc = barc.reserve(sizeof(double) * F))) return rc;
+  if ((rc = known.reserve(F))) return rc;
+  if ((rc = err.reserve(sizeof(double) * F))) return rc;
+  if ((rc = cost.reserve(sizeof(double) * F))) return rc;
+  if ((rc = Hii.reserve(sizeof(float) * 36 * F))) return rc;
+  if ((rc = Hjj.reserve(sizeof(float) * 36 * F))) return rc;
+  if ((rc = Hij.reserve(sizeof(float) * 36 * F))) return rc;
+  if ((rc = gi.reserve(sizeof(double) * 6 * F))) return rc;
+  if ((rc = gj.reserve(sizeof(double) * 6 * F))) return rc;
+  if ((rc = inc_off.reserve(sizeof(int) * (N + 1)))) return rc;
+  if ((rc = inc_cnt.reserve(sizeof(int) * (N + 1)))) return rc;
+  if ((rc = inc.reserve(sizeof(int) * 2 * F))) return rc;
+  if ((rc = D.reserve(sizeof(float) * 36 * N))) return rc;
+  if ((rc = Minv.reserve(sizeof(float) * 36 * N))) return rc;
+  if ((rc = hdiag.reserve(sizeof(float) * n6))) return rc;
+  if ((rc = b64.reserve(sizeof(double) * n6))) return rc;
+  for (DevBuf* v : {&b, &dx, &r, &zv, &p, &q, &hx})
+    if ((rc = v->reserve(sizeof(float) * n6))) return rc;
+  if ((rc = sc.reserve(sizeof(double) * 64))) return rc;
+  return RPGO_OK;
+}
+
+// pinned host scratch for scalar readbacks
+static double* host_scalars() {
+  static thread_local double* h = nullptr;
+  if (!h) {
+    if (hipHostMalloc((void**)&h, sizeof(double) * 64, hipHostMallocDefault) != hipSuccess) h = nullptr;
+  }
+  return h;
+}
+
+int SolverRun::read_scalars(hipStream_t st, int count, double* out) {
+  double* h = host_scalars();
+  if (!h) {
+    set_error("hipHostMalloc of the scalar readback failed");
+    return RPGO_ERR_NOMEM;
+  }
+  RPGO_HIP_CHECK(hipMemcpyAsync(h, sc.ptr, sizeof(double) * count, hipMemcpyDeviceToHost, st));
+  RPGO_HIP_CHECK(hipStreamSynchronize(st));
+  for (int k = 0; k < count; ++k) out[k] = h[k];
+  return RPGO_OK;
+}
+
+int SolverRun::build_incidence(hipStream_t st) {
+  RPGO_HIP_CHECK(hipMemsetAsync(inc_cnt.ptr, 0, sizeof(int) * (N + 1), st));
+  k_inc_count<<<grid_for(F, 256, 1 << 20), 256, 0, st>>>(F, kind.as<int>(), fi.as<int>(), fj.as<int>(),
+                                                        inc_cnt.as<int>());
+  k_exclusive_scan<<<1, 1024, 0, st>>>(N, inc_cnt.as<int>(), inc_off.as<int>());
+  RPGO_HIP_CHECK(hipMemsetAsync(inc_cnt.ptr, 0, sizeof(int) * (N + 1), st));
+  k_inc_fill<<<grid_for(F, 256, 1 << 20), 256, 0, st>>>(F, kind.as<int>(), fi.as<int>(), fj.as<int>(),
+                                                       inc_off.as<int>(), inc_cnt.as<int>(),
+                                                       inc.as<int>());
+  k_inc_sort<<<grid_for(N, 256, 1 << 20), 256, 0, st>>>(N, inc_off.as<int>(), inc.as<int>());
+  RPGO_HIP_CHECK(hipGetLastError());
+  return RPGO_OK;
+}
+
+// total weighted cost at poses xp (FP64)
+int SolverRun::total_cost(hipStream_t st, const double* xp, double* out) {
+  k_factor_errors<<<grid_for(F, 256, 1 << 20), 256, 0, st>>>(F, kind.as<int>(), fi.as<int>(), fj.as<int>(),
+                                                            z.as<double>(), info.as<double>(), xp,
+                                                            err.as<double>());
+  RPGO_HIP_CHECK(hipMemsetAsync(sc.as<double>() + 8, 0, sizeof(double), st));
+  k_sum<<<grid_for(F), 256, 0, st>>>(F, err.as<double>(), w.as<double>(), sc.as<double>() + 8);
+  RPGO_HIP_CHECK(hipGetLastError());
+  double h[9];
+  int rc = read_scalars(st, 9, h);
+  if (rc) return rc;
+  *out = h[8];
+  return RPGO_OK;
+}
+
+int SolverRun::linearize(hipStream_t st) {
+  k_linearize<<<grid_for(F, 64, 1 << 22), 64, 0, st>>>(
+      F, kind.as<int>(), fi.as<int>(), fj.as<int>(), z.as<double>(), info.as<double>(), w.as<double>(),
+      x.as<double>(), Hii.as<float>(), Hjj.as<float>(), Hij.as<float>(), gi.as<double>(),
+      gj.as<double>(), cost.as<double>());
+  k_assemble<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(
+      N, inc_off.as<int>(), inc.as<int>(), Hii.as<float>(), Hjj.as<float>(), gi.as<double>(),
+      gj.as<double>(), D.as<float>(), b64.as<double>(), b.as<float>(), hdiag.as<float>());
+  RPGO_HIP_CHECK(hipGetLastError());
+  ++stats.linearizations;
+  return RPGO_OK;
+}
+
+// y = (H + lambda damp) x
+void SolverRun::spmv(hipStream_t st, double lambda, const float* xin, float* yout) {
+  k_spmv<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(
+      N, inc_off.as<int>(), inc.as<int>(), fi.as<int>(), fj.as<int>(), kind.as<int>(), D.as<float>(),
+      Hij.as<float>(), hdiag.as<float>(), lambda, prm.lm_diagonal_damping, 1e-6, 1e32, xin, yout);
+}
+
+// Solve (H + lambda damp) dx = b by block-Jacobi PCG (x0 = 0).
+int SolverRun::pcg(hipStream_t st, double lambda, int* iters_out) {
+  const long long n = (long long)N * 6;
+  double* s = sc.as<double>();
+  k_precond<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(N, D.as<float>(), hdiag.as<float>(), lambda,
+                                                      prm.lm_diagonal_damping, 1e-6, 1e32,
+                                                      Minv.as<float>());
+  RPGO_HIP_CHECK(hipMemsetAsync(dx.ptr, 0, sizeof(float) * n, st));
+  k_copy_f<<<grid_for(n), 256, 0, st>>>(n, b.as<float>(), r.as<float>());
+  RPGO_HIP_CHECK(hipMemsetAsync(s, 0, sizeof(double) * 8, st));
+  k_apply_precond<<<grid_for(N, 256, 1 << 22), 256, 0, st>>>(N, Minv.as<float>(), r.as<float>(),
+                                                            zv.as<float>(), s + 0);
+  k_copy_f<<<grid_for(n), 256, 0, st>>>(n, zv.as<float>(), p.as<float>());
+  k_dot<<<grid_for(n), 256, 0, st>>>(n, b.as<float>(), b.as<float>(), s + 4);
+  RPGO_HIP_CHECK(hipGetLastError());
+  double h[8];
+  int rc = read_scalars(st, 8, h);
+  if (rc) return rc;
+  const double bnorm2 = h[4];
+  const double tol2 = prm.pcg_rel_tol * prm.pcg_rel_tol * bnorm2;
+  int it = 0;
+  const int max_it = prm.pcg_max_iterations > 0 ? prm.pcg_max_iterations : 6 * N;
+  if (bnorm2 == 0.0) {
+    *iters_out = 0;
+    return RPGO_OK;
+  }
+  const int check_every = 8;
+  while (it < max_it) {
+    spmv(st, lambda, p.as<float>(), q.as<float>());
+    k_dot<<<grid_for(n), 256, 0, st>>>(n, p.as<float>(), q.as<float>(), s + 1);
+    k_cg_update_xr<<<grid_for(n), 256, 0, st>>>(n, s, p.as<float>(), q.as<float>(), dx.as<float>(),
+                                                r.as<float>(), s + 3);
+    k_apply_precond<<<grid_for(N, 256, 1 << 22), 256, 0, st>>>(N, Minv.as<float>(), r.as<float>(),
+                                                              zv.as<float>(), s + 2);
+    k_cg_update_p<<<grid_for(n), 256, 0, st>>>(n, s, zv.as<float>(), p.as<float>());
+    ++it;
+    if (it % check_every == 0 || it == max_it) {
+      // the residual of this iteration is in s[3]; read it before the roll
+      RPGO_HIP_CHECK(hipGetLastError());
+      if ((rc = read_scalars(st, 4, h))) return rc;
+      if (h[3] <= tol2) {
+        k_cg_roll<<<1, 1, 0, st>>>(s);
+        break;
+      }
+    }
+    k_cg_roll<<<1, 1, 0, st>>>(s);
+  }
+  RPGO_HIP_CHECK(hipGetLastError());
+  stats.pcg_iterations += it;
+  *iters_out = it;
+  return RPGO_OK;
+}
+
+// model decrease L(0) - L(dx) = b.dx - 0.5 dx^T H dx (undamped H)
+int SolverRun::model_decrease(hipStream_t st, double* out) {
+  const long long n = (long long)N * 6;
+  double* s = sc.as<double>();
+  spmv(st, 0.0, dx.as<float>(), hx.as<float>());
+  RPGO_HIP_CHECK(hipMemsetAsync(s + 10, 0, sizeof(double) * 2, st));
+  k_dot_df<<<grid_for(n), 256, 0, st>>>(n, b64.as<double>(), dx.as<float>(), s + 10);
+  k_dot<<<grid_for(n), 256, 0, st>>>(n, dx.as<float>(), hx.as<float>(), s + 11);
+  RPGO_HIP_CHECK(hipGetLastError());
+  double h[12];
+  int rc = read_scalars(st, 12, h);
+  if (rc) return rc;
+  *out = h[10] - 0.5 * h[11];
+  return RPGO_OK;
+}
+
+static bool lm_converged(const rpgo_solve_params& p, double cur, double nw) {
+  if (nw <= 0.0) return true;
+  const double dec = cur - nw;
+  return (dec / cur <= p.lm_relative_error_tol) || (dec <= p.lm_absolute_error_tol);
+}
+
+// One LevenbergMarquardtOptimizer / GaussNewtonOptimizer run from x0 with the
+// current weights; leaves the result in x and its weighted cost in *cost_out.
+int SolverRun::nonlinear_solve(hipStream_t st, bool from_x0, double* cost_out, int* iters_out) {
+  int rc;
+  if (from_x0) RPGO_HIP_CHECK(hipMemcpyAsync(x.ptr, x0.ptr, sizeof(double) * 12 * N, hipMemcpyDeviceToDevice, st));
+  double err;
+  if ((rc = total_cost(st, x.as<double>(), &err))) return rc;
+  double lambda = prm.lambda_initial;
+  int it = 0;
+  const bool gn = prm.solver == RPGO_SOLVER_GN;
+  while (it < prm.lm_max_iterations) {
+    if ((rc = linearize(st))) return rc;
+    if (gn) {
+      int pit;
+      if ((rc = pcg(st, 0.0, &pit))) return rc;
+      k_retract<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(N, x.as<double>(), dx.as<float>(), xn.as<double>());
+      double nerr;
+      if ((rc = total_cost(st, xn.as<double>(), &nerr))) return rc;
+      std::swap(x, xn);
+      ++it;
+      const bool conv = lm_converged(prm, err, nerr);
+      err = nerr;
+      if (conv) break;
+      continue;
+    }
+    bool accepted = false;
+    double nerr = err;
+    while (true) {
+      int pit;
+      if ((rc = pcg(st, lambda, &pit))) return rc;
+      double lin;
+      if ((rc = model_decrease(st, &lin))) return rc;
+      k_retract<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(N, x.as<double>(), dx.as<float>(), xn.as<double>());
+      if ((rc = total_cost(st, xn.as<double>(), &nerr))) return rc;
+      const double fid = lin > 0.0 ? (err - nerr) / lin : -1.0;
+      if (lin >= 0.0 && fid >= prm.min_model_fidelity) {
+        accepted = true;
+        lambda = std::max(lambda / prm.lambda_factor, 0.0);
+        break;
+      }
+      lambda = std::max(lambda * prm.lambda_factor, 1e-12);  // a zero lambda must still grow
+      ++stats.lm_rejections;
+      if (lambda >= prm.lambda_upper_bound) break;
+    }
+    ++it;
+    if (!accepted) break;
+    const bool conv = lm_converged(prm, err, nerr);
+    std::swap(x, xn);
+    err = nerr;
+    if (conv) break;
+  }
+  stats.lm_iterations += it;
+  *cost_out = err;
+  *iters_out = it;
+  return RPGO_OK;
+}
+
+int SolverRun::compute_errors(hipStream_t st, const double* xp) {
+  k_factor_errors<<<grid_for(F, 256, 1 << 20), 256, 0, st>>>(F, kind.as<int>(), fi.as<int>(), fj.as<int>(),
+                                                            z.as<double>(), info.as<double>(), xp,
+                                                            err.as<double>());
+  RPGO_HIP_CHECK(hipGetLastError());
+  return RPGO_OK;
+}
+
+int SolverRun::run(hipStream_t st, rpgo_solve_report* rep) {
+  int rc;
+  stats = Stats{};
+  if ((rc = build_incidence(st))) return rc;
+  int lm_it = 0;
+  double cost = 0.0;
+  int gnc_it = 0;
+  double mu = -1.0;
+  if (!prm.use_gnc) {
+    if ((rc = nonlinear_solve(st, true, &cost, &lm_it))) return rc;
+  } else {
+    // first solve with the initial weights (all 1 or bias_odom / caller weights)
+    if ((rc = nonlinear_solve(st, true, &cost, &lm_it))) return rc;
+    if (prm.init_mu > 0.0) {
+      mu = prm.init_mu;
+    } else {
+      // mu from the residuals at the initial values (TLS, Remark 5)
+      if ((rc = compute_errors(st, x0.as<double>()))) return rc;
+      k_mu_candidates<<<grid_for(F, 256, 1 << 20), 256, 0, st>>>(F, err.as<double>(), barc.as<double>(),
+                                                                cost_buf());
+      const unsigned long long inf_bits = 0x7ff0000000000000ull;
+      RPGO_HIP_CHECK(hipMemcpyAsync(sc.as<double>() + 20, &inf_bits, 8, hipMemcpyHostToDevice, st));
+      k_min<<<grid_for(F), 256, 0, st>>>(F, cost_buf(),
+                                         reinterpret_cast<unsigned long long*>(sc.as<double>() + 20));
+      RPGO_HIP_CHECK(hipGetLastError());
+      double h[21];
+      if ((rc = read_scalars(st, 21, h))) return rc;
+      mu = h[20];
+      if (mu >= 0.0 && mu < 1e-6) mu = 1e-6;
+      mu = (mu > 0.0 && !std::isinf(mu)) ? mu : -1.0;
+    }
+    stats.mu_init = mu;
+    double prev_cost = cost;
+    const int n_unknown = F - n_known;
+    if (!(mu <= 0.0 || n_unknown == 0)) {
+      for (int k = 0; k < prm.gnc_max_iterations; ++k) {
+        if ((rc = compute_errors(st, x.as<double>()))) return rc;
+        k_tls_weights<<<grid_for(F, 256, 1 << 20), 256, 0, st>>>(F, err.as<double>(), barc.as<double>(),
+                                                                known.as<unsigned char>(), mu,
+                                                                w.as<double>());
+        RPGO_HIP_CHECK(hipGetLastError());
+        int it2;
+        if ((rc = nonlinear_solve(st, true, &cost, &it2))) return rc;
+        gnc_it = k + 1;
+        RPGO_HIP_CHECK(hipMemsetAsync(sc.as<double>() + 24, 0, sizeof(double), st));
+        k_weights_binary<<<grid_for(F, 256, 1 << 20), 256, 0, st>>>(
+            F, w.as<double>(), prm.gnc_weights_tol, reinterpret_cast<unsigned int*>(sc.as<double>() + 24));
+        double h[25];
+        if ((rc = read_scalars(st, 25, h))) return rc;
+        unsigned int bad;
+        std::memcpy(&bad, &h[24], sizeof(bad));
+        const bool cost_conv = std::fabs(cost - prev_cost) / std::max(prev_cost, 1e-7) < prm.gnc_relative_cost_tol;
+        if (cost_conv || bad == 0) break;
+        mu *= prm.gnc_mu_step;
+        prev_cost = cost;
+      }
+    }
+  }
+  if (prm.polish_iterations > 0) {
+    // FP64-residual Gauss-Newton refinement of the final weighted problem
+    rpgo_solve_params saved = prm;
+    prm.solver = RPGO_SOLVER_GN;
+    prm.lm_max_iterations = prm.polish_iterations;
+    prm.lm_relative_error_tol = 1e-14;
+    prm.lm_absolute_error_tol = 0.0;
+    prm.pcg_rel_tol = std::min(prm.pcg_rel_tol, 1e-6);
+    int it3;
+    rc = nonlinear_solve(st, false, &cost, &it3);
+    prm = saved;
+    if (rc) return rc;
+  }
+  if (rep) {
+    rep->gnc_iterations = gnc_it;
+    rep->lm_iterations = stats.lm_iterations;
+    rep->pcg_iterations = stats.pcg_iterations;
+    rep->linearizations = stats.linearizations;
+    rep->mu_init = stats.mu_init;
+    rep->mu_final = mu;
+    rep->final_cost = cost;
+  }
+  return RPGO_OK;
+}
+
+}  // namespace rpgo

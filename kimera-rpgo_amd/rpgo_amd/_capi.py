@@ -49,8 +49,51 @@ class PcmStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+SOLVER_LM = 0
+SOLVER_GN = 1
+FACTOR_BETWEEN = 0
+FACTOR_PRIOR = 1
+GNC_COST = 0
+GNC_PROBABILITY = 1
+
+
+class SolveProblem(ctypes.Structure):
+    _fields_ = [("n_poses", ctypes.c_int32), ("poses", _d), ("n_factors", ctypes.c_int32),
+                ("kind", _i32), ("key_i", _i32), ("key_j", _i32), ("measured", _d),
+                ("information", _d)]
+
+
+class SolveParams(ctypes.Structure):
+    _fields_ = [("solver", ctypes.c_int32), ("use_gnc", ctypes.c_int32),
+                ("known_inliers", ctypes.c_int32), ("gnc_threshold_mode", ctypes.c_int32),
+                ("gnc_inlier_threshold", ctypes.c_double), ("gnc_max_iterations", ctypes.c_int32),
+                ("gnc_mu_step", ctypes.c_double), ("gnc_relative_cost_tol", ctypes.c_double),
+                ("gnc_weights_tol", ctypes.c_double), ("gnc_bias_odom", ctypes.c_int32),
+                ("lm_diagonal_damping", ctypes.c_int32), ("init_weights", _d),
+                ("init_mu", ctypes.c_double), ("lambda_initial", ctypes.c_double),
+                ("lambda_factor", ctypes.c_double), ("lambda_upper_bound", ctypes.c_double),
+                ("min_model_fidelity", ctypes.c_double), ("lm_max_iterations", ctypes.c_int32),
+                ("lm_relative_error_tol", ctypes.c_double), ("lm_absolute_error_tol", ctypes.c_double),
+                ("pcg_max_iterations", ctypes.c_int32), ("pcg_rel_tol", ctypes.c_double),
+                ("polish_iterations", ctypes.c_int32)]
+
+
+class SolveReport(ctypes.Structure):
+    _fields_ = [("gnc_iterations", ctypes.c_int32), ("lm_iterations", ctypes.c_int32),
+                ("pcg_iterations", ctypes.c_int32), ("linearizations", ctypes.c_int32),
+                ("mu_init", ctypes.c_double), ("mu_final", ctypes.c_double),
+                ("final_cost", ctypes.c_double), ("ms_total", ctypes.c_double),
+                ("n_inliers", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
 # (name, argtypes, restype) of every exported entry point
 SIGNATURES = [
+    ("rpgo_solve_params_default", [ctypes.POINTER(SolveParams)], None),
+    ("rpgo_solve", [ctypes.c_void_p, ctypes.POINTER(SolveProblem), ctypes.POINTER(SolveParams), _d, _d,
+                    ctypes.POINTER(SolveReport)], ctypes.c_int),
     ("rpgo_ctx_create", [ctypes.c_int, ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
     ("rpgo_ctx_destroy", [ctypes.c_void_p], ctypes.c_int),
     ("rpgo_ctx_sync", [ctypes.c_void_p], ctypes.c_int),
@@ -188,6 +231,37 @@ class Context:
         ms = ctypes.c_double()
         check(lib().rpgo_max_clique(self._h, mode, _ptr(idx, _i32), ctypes.byref(k), ctypes.byref(ms)))
         return int(k.value), idx[: max(k.value, 0)].copy(), ms.value
+
+    # ---------------------------------------------------------------- solve
+    def solve(self, poses, kind, key_i, key_j, measured, information, **params):
+        """rpgo_solve on an SE(3) factor list; params override the reference
+        defaults (rpgo_solve_params_default).  Returns (poses, weights, report)."""
+        f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+        poses, measured, information = f64(poses).reshape(-1, 12), f64(measured), f64(information)
+        kind, key_i, key_j = i32(kind), i32(key_i), i32(key_j)
+        N, F = poses.shape[0], kind.shape[0]
+        prob = SolveProblem(n_poses=N, poses=_ptr(poses), n_factors=F, kind=_ptr(kind, _i32),
+                            key_i=_ptr(key_i, _i32), key_j=_ptr(key_j, _i32), measured=_ptr(measured),
+                            information=_ptr(information))
+        prm = SolveParams()
+        lib().rpgo_solve_params_default(ctypes.byref(prm))
+        keep = None
+        for k, v in params.items():
+            if k == "init_weights":
+                if v is not None:
+                    keep = f64(v)
+                    prm.init_weights = _ptr(keep)
+                continue
+            if not hasattr(prm, k):
+                raise KeyError(k)
+            setattr(prm, k, v)
+        out = np.zeros((N, 12))
+        w = np.zeros(max(F, 1))
+        rep = SolveReport()
+        check(lib().rpgo_solve(self._h, ctypes.byref(prob), ctypes.byref(prm), _ptr(out), _ptr(w),
+                               ctypes.byref(rep)))
+        return out, w[:F].copy(), rep
 
     def max_clique_bits(self, adj_bits, n, mode=CLIQUE_HEU_REF):
         a = np.ascontiguousarray(adj_bits, dtype=np.uint64).reshape(-1)
