@@ -183,6 +183,10 @@ typedef struct {
   int32_t pcg_max_iterations;   /* <= 0: 6 * n_poses */
   double pcg_rel_tol;           /* relative residual of each linear solve */
   int32_t polish_iterations;    /* final Gauss-Newton refinement steps (0: none) */
+  int32_t pcg_segment;          /* poses per block-tridiagonal preconditioner segment
+                                   (<= 0: 128; 1: plain block-Jacobi)             */
+  int32_t gnc_warm_start;       /* 0: every inner solve restarts from the initial
+                                   values (GncOptimizer); 1: from the last result  */
 } rpgo_solve_params;
 
 typedef struct {

@@ -26,6 +26,7 @@ struct SolverRun {
   DevBuf Hii, Hjj, Hij, gi, gj;  // per-factor contributions
   DevBuf inc_off, inc_cnt, inc;  // pose -> incident factors
   DevBuf D, Minv, hdiag, b64;
+  DevBuf chain, Useg;             // odometry chain links, preconditioner off-diagonals
   DevBuf b, dx, r, zv, p, q, hx;  // PCG vectors (FP32)
   DevBuf sc;                      // FP64 scalars
 
@@ -44,6 +45,7 @@ struct SolverRun {
   void release() {
     for (DevBuf* v : {&x0, &x, &xn, &kind, &fi, &fj, &z, &info, &w, &barc, &known, &err, &cost, &Hii,
                       &Hjj, &Hij, &gi, &gj, &inc_off, &inc_cnt, &inc, &D, &Minv, &hdiag, &b64, &b,
+                      &chain, &Useg,
                       &dx, &r, &zv, &p, &q, &hx, &sc})
       v->release();
   }

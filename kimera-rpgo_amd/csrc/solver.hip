@@ -21,6 +21,7 @@
 // pose->incident-factor index (sorted by factor id) for deterministic
 // row-parallel assembly and SpMV.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <vector>
 
@@ -446,6 +447,228 @@ __global__ void k_precond(int N, const float* __restrict__ D, const float* __res
     }
 }
 
+// ---------------------------------------------------------------------------
+// Segment block-tridiagonal preconditioner.  The odometry chain k -> k+1 makes
+// the Gauss-Newton matrix block-tridiagonal up to the loop-closure blocks; the
+// preconditioner solves that chain exactly inside segments of S consecutive
+// poses (block LDL^T, one thread per segment), dropping the loop-closure
+// blocks and the coupling across segment borders.  S = 1 is block-Jacobi.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double wave_sum_pre(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// chain[k] = 2 f + flip for the first odometry factor joining k and k+1
+// (flip = 1 when the factor runs k+1 -> k), -1 when none
+__global__ void k_chain_init(int n, int* __restrict__ chain) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) chain[k] = INT_MAX;
+}
+
+__global__ void k_chain_find(int F, const int* __restrict__ kind, const int* __restrict__ fi,
+                             const int* __restrict__ fj, int* __restrict__ chain) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F || kind[f] != 0) return;
+  const int a = fi[f], b = fj[f];
+  if (b == a + 1) atomicMin(&chain[a], 2 * f);
+  else if (a == b + 1) atomicMin(&chain[b], 2 * f + 1);
+}
+
+__device__ __forceinline__ void damped_block(const float* __restrict__ D, const float* __restrict__ hdiag,
+                                             int p, double lambda, int diag_damping, double* A) {
+#pragma unroll
+  for (int e = 0; e < 36; ++e) A[e] = D[(size_t)p * 36 + e];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    const double d = diag_damping ? fmin(fmax((double)hdiag[(size_t)p * 6 + i], 1e-6), 1e32) : 1.0;
+    A[i * 7] += lambda * d;
+  }
+}
+
+// in-place inverse of an SPD 6x6 (Cholesky; tiny pivots floored)
+__device__ void spd_inverse6(const double* A, double* Ainv) {
+  double L[36];
+#pragma unroll
+  for (int e = 0; e < 36; ++e) L[e] = 0.0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    double x = A[k * 7];
+#pragma unroll
+    for (int j = 0; j < k; ++j) x -= L[k * 6 + j] * L[k * 6 + j];
+    x = sqrt(fmax(x, 1e-300));
+    L[k * 7] = x;
+#pragma unroll
+    for (int i = k + 1; i < 6; ++i) {
+      double v = A[i * 6 + k];
+#pragma unroll
+      for (int j = 0; j < k; ++j) v -= L[i * 6 + j] * L[k * 6 + j];
+      L[i * 6 + k] = v / x;
+    }
+  }
+  double Li[36];
+#pragma unroll
+  for (int e = 0; e < 36; ++e) Li[e] = 0.0;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      if (i < c) continue;
+      double v = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+      for (int j = 0; j < 6; ++j)
+        if (j >= c && j < i) v -= L[i * 6 + j] * Li[j * 6 + c];
+      Li[i * 6 + c] = v / L[i * 7];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) s += Li[k * 6 + i] * Li[k * 6 + j];
+      Ainv[i * 6 + j] = s;
+    }
+}
+
+// Factor: Dp_k = A_kk - U_{k-1}^T Dp_{k-1}^-1 U_{k-1}; stores Dp_k^-1 (Minv)
+// and U_k = H_{k,k+1} (Useg, zero at a segment end or without odometry).
+__global__ void k_seg_factor(int N, int S, const int* __restrict__ chain, const float* __restrict__ D,
+                             const float* __restrict__ hdiag, const float* __restrict__ Hij,
+                             double lambda, int diag_damping, float* __restrict__ Minv,
+                             float* __restrict__ Useg) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k0 = s * S;
+  if (k0 >= N) return;
+  const int k1 = min(k0 + S, N);
+  double Dinv_prev[36], U_prev[36];
+  bool have_prev = false;
+  for (int k = k0; k < k1; ++k) {
+    double A[36];
+    damped_block(D, hdiag, k, lambda, diag_damping, A);
+    if (have_prev) {
+      // A -= U^T (Dinv U)
+      double T[36];
+#pragma unroll
+      for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+          double v = 0.0;
+#pragma unroll
+          for (int m = 0; m < 6; ++m) v += Dinv_prev[i * 6 + m] * U_prev[m * 6 + j];
+          T[i * 6 + j] = v;
+        }
+#pragma unroll
+      for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+          double v = 0.0;
+#pragma unroll
+          for (int m = 0; m < 6; ++m) v += U_prev[m * 6 + i] * T[m * 6 + j];
+          A[i * 6 + j] -= v;
+        }
+    }
+    double Ainv[36];
+    spd_inverse6(A, Ainv);
+#pragma unroll
+    for (int e = 0; e < 36; ++e) {
+      Minv[(size_t)k * 36 + e] = (float)Ainv[e];
+      Dinv_prev[e] = Ainv[e];
+    }
+    // U_k = H_{k,k+1}
+    const int c = (k + 1 < k1) ? chain[k] : INT_MAX;
+    if (c != INT_MAX) {
+      const int f = c >> 1;
+      const bool flip = c & 1;
+      const float* B = Hij + (size_t)f * 36;
+#pragma unroll
+      for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+          const float v = flip ? B[j * 6 + i] : B[i * 6 + j];
+          U_prev[i * 6 + j] = v;
+          Useg[(size_t)k * 36 + i * 6 + j] = v;
+        }
+      have_prev = true;
+    } else {
+#pragma unroll
+      for (int e = 0; e < 36; ++e) Useg[(size_t)k * 36 + e] = 0.f;
+      have_prev = false;
+    }
+  }
+}
+
+// z = M^-1 r by the segment LDL^T, rz += r . z
+__global__ void k_seg_apply(int N, int S, const float* __restrict__ Minv, const float* __restrict__ Useg,
+                            const float* __restrict__ r, float* __restrict__ z, double* __restrict__ rz) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k0 = s * S;
+  double acc = 0.0;
+  if (k0 < N) {
+    const int k1 = min(k0 + S, N);
+    // forward: y_k = r_k - U_{k-1}^T Dinv_{k-1} y_{k-1};  store w_k = Dinv_k y_k in z
+    float y[6], t[6];
+#pragma unroll
+    for (int e = 0; e < 6; ++e) y[e] = r[(size_t)k0 * 6 + e];
+    for (int k = k0; k < k1; ++k) {
+      if (k > k0) {
+        const float* U = Useg + (size_t)(k - 1) * 36;
+        float yn[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+          float v = r[(size_t)k * 6 + i];
+#pragma unroll
+          for (int m = 0; m < 6; ++m) v -= U[m * 6 + i] * t[m];
+          yn[i] = v;
+        }
+#pragma unroll
+        for (int e = 0; e < 6; ++e) y[e] = yn[e];
+      }
+      const float* Di = Minv + (size_t)k * 36;
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        float v = 0.f;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) v += Di[i * 6 + m] * y[m];
+        t[i] = v;
+        z[(size_t)k * 6 + i] = v;
+      }
+    }
+    // backward: z_k = w_k - Dinv_k U_k z_{k+1}
+    float zn[6];
+#pragma unroll
+    for (int e = 0; e < 6; ++e) zn[e] = z[(size_t)(k1 - 1) * 6 + e];
+    for (int k = k1 - 2; k >= k0; --k) {
+      const float* U = Useg + (size_t)k * 36;
+      const float* Di = Minv + (size_t)k * 36;
+      float u[6];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        float v = 0.f;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) v += U[i * 6 + m] * zn[m];
+        u[i] = v;
+      }
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        float v = z[(size_t)k * 6 + i];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) v -= Di[i * 6 + m] * u[m];
+        zn[i] = v;
+        z[(size_t)k * 6 + i] = v;
+      }
+    }
+    for (int k = k0; k < k1; ++k)
+#pragma unroll
+      for (int e = 0; e < 6; ++e) acc += (double)r[(size_t)k * 6 + e] * (double)z[(size_t)k * 6 + e];
+  }
+  acc = wave_sum_pre(acc);
+  if ((threadIdx.x & 63) == 0 && acc != 0.0) atomicAdd(rz, acc);
+}
+
 // y = (H + lambda damp) x : one thread per pose row (deterministic order)
 __global__ void k_spmv(int N, const int* __restrict__ off, const int* __restrict__ inc,
                        const int* __restrict__ fi, const int* __restrict__ fj,
@@ -694,6 +917,8 @@ int SolverRun::alloc(int N_, int F_) {
   if ((rc = inc.reserve(sizeof(int) * 2 * F))) return rc;
   if ((rc = D.reserve(sizeof(float) * 36 * N))) return rc;
   if ((rc = Minv.reserve(sizeof(float) * 36 * N))) return rc;
+  if ((rc = Useg.reserve(sizeof(float) * 36 * N))) return rc;
+  if ((rc = chain.reserve(sizeof(int) * (N + 1)))) return rc;
   if ((rc = hdiag.reserve(sizeof(float) * n6))) return rc;
   if ((rc = b64.reserve(sizeof(double) * n6))) return rc;
   for (DevBuf* v : {&b, &dx, &r, &zv, &p, &q, &hx})
@@ -733,6 +958,9 @@ int SolverRun::build_incidence(hipStream_t st) {
                                                        inc_off.as<int>(), inc_cnt.as<int>(),
                                                        inc.as<int>());
   k_inc_sort<<<grid_for(N, 256, 1 << 20), 256, 0, st>>>(N, inc_off.as<int>(), inc.as<int>());
+  k_chain_init<<<grid_for(N + 1, 256, 1 << 20), 256, 0, st>>>(N + 1, chain.as<int>());
+  k_chain_find<<<grid_for(F, 256, 1 << 20), 256, 0, st>>>(F, kind.as<int>(), fi.as<int>(), fj.as<int>(),
+                                                          chain.as<int>());
   RPGO_HIP_CHECK(hipGetLastError());
   return RPGO_OK;
 }
@@ -772,18 +1000,21 @@ void SolverRun::spmv(hipStream_t st, double lambda, const float* xin, float* you
       Hij.as<float>(), hdiag.as<float>(), lambda, prm.lm_diagonal_damping, 1e-6, 1e32, xin, yout);
 }
 
-// Solve (H + lambda damp) dx = b by block-Jacobi PCG (x0 = 0).
+// Solve (H + lambda damp) dx = b by PCG with the segment block-tridiagonal
+// preconditioner (x0 = 0).
 int SolverRun::pcg(hipStream_t st, double lambda, int* iters_out) {
   const long long n = (long long)N * 6;
   double* s = sc.as<double>();
-  k_precond<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(N, D.as<float>(), hdiag.as<float>(), lambda,
-                                                      prm.lm_diagonal_damping, 1e-6, 1e32,
-                                                      Minv.as<float>());
+  const int S = prm.pcg_segment > 0 ? prm.pcg_segment : 128;
+  const int nseg = (N + S - 1) / S;
+  k_seg_factor<<<(nseg + 63) / 64, 64, 0, st>>>(N, S, chain.as<int>(), D.as<float>(), hdiag.as<float>(),
+                                                Hij.as<float>(), lambda, prm.lm_diagonal_damping,
+                                                Minv.as<float>(), Useg.as<float>());
   RPGO_HIP_CHECK(hipMemsetAsync(dx.ptr, 0, sizeof(float) * n, st));
   k_copy_f<<<grid_for(n), 256, 0, st>>>(n, b.as<float>(), r.as<float>());
   RPGO_HIP_CHECK(hipMemsetAsync(s, 0, sizeof(double) * 8, st));
-  k_apply_precond<<<grid_for(N, 256, 1 << 22), 256, 0, st>>>(N, Minv.as<float>(), r.as<float>(),
-                                                            zv.as<float>(), s + 0);
+  k_seg_apply<<<(nseg + 63) / 64, 64, 0, st>>>(N, S, Minv.as<float>(), Useg.as<float>(), r.as<float>(),
+                                               zv.as<float>(), s + 0);
   k_copy_f<<<grid_for(n), 256, 0, st>>>(n, zv.as<float>(), p.as<float>());
   k_dot<<<grid_for(n), 256, 0, st>>>(n, b.as<float>(), b.as<float>(), s + 4);
   RPGO_HIP_CHECK(hipGetLastError());
@@ -804,8 +1035,8 @@ int SolverRun::pcg(hipStream_t st, double lambda, int* iters_out) {
     k_dot<<<grid_for(n), 256, 0, st>>>(n, p.as<float>(), q.as<float>(), s + 1);
     k_cg_update_xr<<<grid_for(n), 256, 0, st>>>(n, s, p.as<float>(), q.as<float>(), dx.as<float>(),
                                                 r.as<float>(), s + 3);
-    k_apply_precond<<<grid_for(N, 256, 1 << 22), 256, 0, st>>>(N, Minv.as<float>(), r.as<float>(),
-                                                              zv.as<float>(), s + 2);
+    k_seg_apply<<<(nseg + 63) / 64, 64, 0, st>>>(N, S, Minv.as<float>(), Useg.as<float>(),
+                                                 r.as<float>(), zv.as<float>(), s + 2);
     k_cg_update_p<<<grid_for(n), 256, 0, st>>>(n, s, zv.as<float>(), p.as<float>());
     ++it;
     if (it % check_every == 0 || it == max_it) {
@@ -954,7 +1185,7 @@ int SolverRun::run(hipStream_t st, rpgo_solve_report* rep) {
                                                                 w.as<double>());
         RPGO_HIP_CHECK(hipGetLastError());
         int it2;
-        if ((rc = nonlinear_solve(st, true, &cost, &it2))) return rc;
+        if ((rc = nonlinear_solve(st, !prm.gnc_warm_start, &cost, &it2))) return rc;
         gnc_it = k + 1;
         RPGO_HIP_CHECK(hipMemsetAsync(sc.as<double>() + 24, 0, sizeof(double), st));
         k_weights_binary<<<grid_for(F, 256, 1 << 20), 256, 0, st>>>(

@@ -92,6 +92,8 @@ void rpgo_solve_params_default(rpgo_solve_params* p) {
   p->pcg_max_iterations = 0;
   p->pcg_rel_tol = 1e-6;
   p->polish_iterations = 0;
+  p->pcg_segment = 128;
+  p->gnc_warm_start = 0;
 }
 
 int rpgo_solve(rpgo_ctx* ctx, const rpgo_solve_problem* pr, const rpgo_solve_params* prm,
