@@ -7,6 +7,17 @@
 
 namespace rpgo {
 
+constexpr int kPcgChunk = 8;  // PCG iterations per captured graph launch
+
+struct GraphKey {
+  int N = 0, S = 0;
+  const void *hist = nullptr, *p = nullptr, *Minv = nullptr, *Hij = nullptr, *inc = nullptr;
+  bool operator==(const GraphKey& o) const {
+    return N == o.N && S == o.S && hist == o.hist && p == o.p && Minv == o.Minv && Hij == o.Hij &&
+           inc == o.inc;
+  }
+};
+
 struct SolverRun {
   struct Stats {
     int lm_iterations = 0;
@@ -29,8 +40,13 @@ struct SolverRun {
   DevBuf chain, Useg;             // odometry chain links, preconditioner off-diagonals
   DevBuf b, dx, r, zv, p, q, hx;  // PCG vectors (FP32)
   DevBuf sc;                      // FP64 scalars
+  DevBuf hist;                    // PCG residual history (per iteration)
+  hipGraphExec_t graph_exec = nullptr;  // captured PCG chunk
+  GraphKey graph_key;
+  bool use_graphs = true;
 
   int alloc(int N_, int F_);
+  void spmv_dev(hipStream_t st, const double* lam, const float* x, float* y);
   int build_incidence(hipStream_t st);
   int linearize(hipStream_t st);
   void spmv(hipStream_t st, double lambda, const float* x, float* y);
@@ -43,7 +59,9 @@ struct SolverRun {
   int run(hipStream_t st, rpgo_solve_report* rep);
   double* cost_buf() { return cost.as<double>(); }
   void release() {
-    for (DevBuf* v : {&x0, &x, &xn, &kind, &fi, &fj, &z, &info, &w, &barc, &known, &err, &cost, &Hii,
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    graph_exec = nullptr;
+    for (DevBuf* v : {&hist,&x0, &x, &xn, &kind, &fi, &fj, &z, &info, &w, &barc, &known, &err, &cost, &Hii,
                       &Hjj, &Hij, &gi, &gj, &inc_off, &inc_cnt, &inc, &D, &Minv, &hdiag, &b64, &b,
                       &chain, &Useg,
                       &dx, &r, &zv, &p, &q, &hx, &sc})

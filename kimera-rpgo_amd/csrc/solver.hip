@@ -673,11 +673,13 @@ __global__ void k_seg_apply(int N, int S, const float* __restrict__ Minv, const 
 __global__ void k_spmv(int N, const int* __restrict__ off, const int* __restrict__ inc,
                        const int* __restrict__ fi, const int* __restrict__ fj,
                        const int* __restrict__ kind, const float* __restrict__ D,
-                       const float* __restrict__ Hij, const float* __restrict__ hdiag, double lambda,
+                       const float* __restrict__ Hij, const float* __restrict__ hdiag,
+                       const double* __restrict__ lam_ptr, double lambda_imm,
                        int diag_damping, double min_diag, double max_diag,
                        const float* __restrict__ x, float* __restrict__ y) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= N) return;
+  const double lambda = lam_ptr ? *lam_ptr : lambda_imm;
   float xp[6], acc[6];
 #pragma unroll
   for (int e = 0; e < 6; ++e) xp[e] = x[(size_t)p * 6 + e];
@@ -785,8 +787,11 @@ __global__ void k_cg_update_p(long long n, const double* __restrict__ sc, const 
     p[k] = z[k] + beta * p[k];
 }
 
-__global__ void k_cg_roll(double* sc) {
+__global__ void k_cg_roll(double* sc, double* hist) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const int it = (int)sc[32];
+    hist[it] = sc[3];
+    sc[32] = (double)(it + 1);
     sc[0] = sc[2];
     sc[1] = 0.0;
     sc[2] = 0.0;
@@ -997,7 +1002,15 @@ int SolverRun::linearize(hipStream_t st) {
 void SolverRun::spmv(hipStream_t st, double lambda, const float* xin, float* yout) {
   k_spmv<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(
       N, inc_off.as<int>(), inc.as<int>(), fi.as<int>(), fj.as<int>(), kind.as<int>(), D.as<float>(),
-      Hij.as<float>(), hdiag.as<float>(), lambda, prm.lm_diagonal_damping, 1e-6, 1e32, xin, yout);
+      Hij.as<float>(), hdiag.as<float>(), nullptr, lambda, prm.lm_diagonal_damping, 1e-6, 1e32, xin,
+      yout);
+}
+
+// same, lambda read from device memory (for the captured PCG body)
+void SolverRun::spmv_dev(hipStream_t st, const double* lam, const float* xin, float* yout) {
+  k_spmv<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(
+      N, inc_off.as<int>(), inc.as<int>(), fi.as<int>(), fj.as<int>(), kind.as<int>(), D.as<float>(),
+      Hij.as<float>(), hdiag.as<float>(), lam, 0.0, prm.lm_diagonal_damping, 1e-6, 1e32, xin, yout);
 }
 
 // Solve (H + lambda damp) dx = b by PCG with the segment block-tridiagonal
@@ -1024,33 +1037,79 @@ int SolverRun::pcg(hipStream_t st, double lambda, int* iters_out) {
   const double bnorm2 = h[4];
   const double tol2 = prm.pcg_rel_tol * prm.pcg_rel_tol * bnorm2;
   int it = 0;
-  const int max_it = prm.pcg_max_iterations > 0 ? prm.pcg_max_iterations : 6 * N;
+  const int max_it = prm.pcg_max_iterations > 0 ? prm.pcg_max_iterations : std::min(6 * N, 4000);
   if (bnorm2 == 0.0) {
     *iters_out = 0;
     return RPGO_OK;
   }
-  const int check_every = 8;
-  while (it < max_it) {
-    spmv(st, lambda, p.as<float>(), q.as<float>());
-    k_dot<<<grid_for(n), 256, 0, st>>>(n, p.as<float>(), q.as<float>(), s + 1);
-    k_cg_update_xr<<<grid_for(n), 256, 0, st>>>(n, s, p.as<float>(), q.as<float>(), dx.as<float>(),
-                                                r.as<float>(), s + 3);
-    k_seg_apply<<<(nseg + 63) / 64, 64, 0, st>>>(N, S, Minv.as<float>(), Useg.as<float>(),
-                                                 r.as<float>(), zv.as<float>(), s + 2);
-    k_cg_update_p<<<grid_for(n), 256, 0, st>>>(n, s, zv.as<float>(), p.as<float>());
-    ++it;
-    if (it % check_every == 0 || it == max_it) {
-      // the residual of this iteration is in s[3]; read it before the roll
-      RPGO_HIP_CHECK(hipGetLastError());
-      if ((rc = read_scalars(st, 4, h))) return rc;
-      if (h[3] <= tol2) {
-        k_cg_roll<<<1, 1, 0, st>>>(s);
+  // lambda for the SpMV inside the iteration body lives in device memory so the
+  // captured body is independent of it; the iteration counter s[32] indexes the
+  // residual history written by k_cg_roll.
+  const double lam_zero[3] = {lambda, 0.0, 0.0};
+  RPGO_HIP_CHECK(hipMemcpyAsync(s + 30, lam_zero, sizeof(double) * 3, hipMemcpyHostToDevice, st));
+  if ((rc = hist.reserve(sizeof(double) * (size_t)(max_it + kPcgChunk)))) return rc;
+  auto body = [&]() {
+    for (int j = 0; j < kPcgChunk; ++j) {
+      spmv_dev(st, s + 30, p.as<float>(), q.as<float>());
+      k_dot<<<grid_for(n), 256, 0, st>>>(n, p.as<float>(), q.as<float>(), s + 1);
+      k_cg_update_xr<<<grid_for(n), 256, 0, st>>>(n, s, p.as<float>(), q.as<float>(), dx.as<float>(),
+                                                  r.as<float>(), s + 3);
+      k_seg_apply<<<(nseg + 63) / 64, 64, 0, st>>>(N, S, Minv.as<float>(), Useg.as<float>(),
+                                                   r.as<float>(), zv.as<float>(), s + 2);
+      k_cg_update_p<<<grid_for(n), 256, 0, st>>>(n, s, zv.as<float>(), p.as<float>());
+      k_cg_roll<<<1, 1, 0, st>>>(s, hist.as<double>());
+    }
+  };
+  // one captured chunk of kPcgChunk iterations, rebuilt when sizes or buffers change
+  const GraphKey key{N, S, (const void*)hist.ptr, (const void*)p.ptr, (const void*)Minv.ptr,
+                     (const void*)Hij.ptr, (const void*)inc.ptr};
+  if (use_graphs && !(graph_exec && graph_key == key)) {
+    if (graph_exec) {
+      (void)hipGraphExecDestroy(graph_exec);
+      graph_exec = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    RPGO_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    body();
+    RPGO_HIP_CHECK(hipStreamEndCapture(st, &g));
+    hipError_t e = hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      graph_exec = nullptr;
+      use_graphs = false;  // fall back to direct launches of the same kernels
+    } else {
+      graph_key = key;
+    }
+  }
+  double best = bnorm2;
+  int best_it = 0;
+  bool done = false;
+  while (it < max_it && !done) {
+    if (use_graphs && graph_exec) RPGO_HIP_CHECK(hipGraphLaunch(graph_exec, st));
+    else body();
+    RPGO_HIP_CHECK(hipGetLastError());
+    double hh[kPcgChunk];
+    double* hb = host_scalars();
+    RPGO_HIP_CHECK(hipMemcpyAsync(hb, hist.as<double>() + it, sizeof(double) * kPcgChunk,
+                                  hipMemcpyDeviceToHost, st));
+    RPGO_HIP_CHECK(hipStreamSynchronize(st));
+    for (int j = 0; j < kPcgChunk; ++j) hh[j] = hb[j];
+    for (int j = 0; j < kPcgChunk; ++j) {
+      const double rr = hh[j];
+      if (!(rr == rr)) {  // NaN: keep the iterate of the last chunk boundary
+        done = true;
         break;
       }
+      if (rr <= tol2) done = true;
+      if (rr < 0.98 * best) {
+        best = rr;
+        best_it = it + j + 1;
+      }
     }
-    k_cg_roll<<<1, 1, 0, st>>>(s);
+    it += kPcgChunk;
+    // stagnation: FP32 recurrences stop reducing the residual
+    if (it - best_it >= 8 * kPcgChunk) done = true;
   }
-  RPGO_HIP_CHECK(hipGetLastError());
   stats.pcg_iterations += it;
   *iters_out = it;
   return RPGO_OK;

@@ -90,7 +90,7 @@ void rpgo_solve_params_default(rpgo_solve_params* p) {
   p->lm_relative_error_tol = 1e-5;
   p->lm_absolute_error_tol = 1e-5;
   p->pcg_max_iterations = 0;
-  p->pcg_rel_tol = 1e-6;
+  p->pcg_rel_tol = 1e-4;  // inexact Newton: the FP64-residual outer loop refines
   p->polish_iterations = 0;
   p->pcg_segment = 128;
   p->gnc_warm_start = 0;
