@@ -187,6 +187,9 @@ typedef struct {
                                    (<= 0: 128; 1: plain block-Jacobi)             */
   int32_t gnc_warm_start;       /* 0: every inner solve restarts from the initial
                                    values (GncOptimizer); 1: from the last result  */
+  int32_t pcg_coarse;           /* rigid-segment coarse level of the preconditioner:
+                                   < 0 off; 0 auto (up to 128 segments);
+                                   > 0 at most that many segments (<= 128)        */
 } rpgo_solve_params;
 
 typedef struct {

@@ -7,14 +7,17 @@
 
 namespace rpgo {
 
-constexpr int kPcgChunk = 8;  // PCG iterations per captured graph launch
+constexpr int kPcgChunk = 8;       // PCG iterations per captured graph launch
+constexpr int kCoarseSegMax = 128;  // rigid segments of the coarse preconditioner level
+constexpr int kCoarseMax = 6 * kCoarseSegMax;
 
 struct GraphKey {
-  int N = 0, S = 0;
-  const void *hist = nullptr, *p = nullptr, *Minv = nullptr, *Hij = nullptr, *inc = nullptr;
+  int N = 0, S = 0, Sc = 0, nseg_c = 0;
+  const void *hist = nullptr, *p = nullptr, *Minv = nullptr, *Hij = nullptr, *inc = nullptr,
+             *Ainv = nullptr, *Pb = nullptr;
   bool operator==(const GraphKey& o) const {
-    return N == o.N && S == o.S && hist == o.hist && p == o.p && Minv == o.Minv && Hij == o.Hij &&
-           inc == o.inc;
+    return N == o.N && S == o.S && Sc == o.Sc && nseg_c == o.nseg_c && hist == o.hist && p == o.p &&
+           Minv == o.Minv && Hij == o.Hij && inc == o.inc && Ainv == o.Ainv && Pb == o.Pb;
   }
 };
 
@@ -38,19 +41,24 @@ struct SolverRun {
   DevBuf inc_off, inc_cnt, inc;  // pose -> incident factors
   DevBuf D, Minv, hdiag, b64;
   DevBuf chain, Useg;             // odometry chain links, preconditioner off-diagonals
-  DevBuf b, dx, r, zv, p, q, hx;  // PCG vectors (FP32)
+  DevBuf dx, r, zv, p, q, hx;     // PCG vectors (FP64)
   DevBuf sc;                      // FP64 scalars
   DevBuf hist;                    // PCG residual history (per iteration)
+  DevBuf Pb, Ac, Lci, Ainv, cres, ccor;  // coarse level: prolongation blocks, coarse matrix,
+                                     // its inverse factor, FP32 inverse, coarse vectors
+  bool polish = false;               // refinement pass: a step that raises the cost is rejected
   hipGraphExec_t graph_exec = nullptr;  // captured PCG chunk
   GraphKey graph_key;
   bool use_graphs = true;
 
   int alloc(int N_, int F_);
-  void spmv_dev(hipStream_t st, const double* lam, const float* x, float* y);
+  void spmv_dev(hipStream_t st, const double* lam, const double* x, double* y);
   int build_incidence(hipStream_t st);
   int linearize(hipStream_t st);
-  void spmv(hipStream_t st, double lambda, const float* x, float* y);
+  void spmv(hipStream_t st, double lambda, const double* x, double* y);
   int pcg(hipStream_t st, double lambda, int* iters);
+  int coarse_setup(hipStream_t st, double lambda, int* Sc, int* nseg);
+  void precond_apply(hipStream_t st, int S, int Sc, int nseg_c, double* rz);
   int model_decrease(hipStream_t st, double* out);
   int total_cost(hipStream_t st, const double* xp, double* out);
   int compute_errors(hipStream_t st, const double* xp);
@@ -62,8 +70,8 @@ struct SolverRun {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     graph_exec = nullptr;
     for (DevBuf* v : {&hist,&x0, &x, &xn, &kind, &fi, &fj, &z, &info, &w, &barc, &known, &err, &cost, &Hii,
-                      &Hjj, &Hij, &gi, &gj, &inc_off, &inc_cnt, &inc, &D, &Minv, &hdiag, &b64, &b,
-                      &chain, &Useg,
+                      &Hjj, &Hij, &gi, &gj, &inc_off, &inc_cnt, &inc, &D, &Minv, &hdiag, &b64,
+                      &chain, &Useg, &Pb, &Ac, &Lci, &Ainv, &cres, &ccor,
                       &dx, &r, &zv, &p, &q, &hx, &sc})
       v->release();
   }

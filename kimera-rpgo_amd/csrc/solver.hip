@@ -221,12 +221,12 @@ __global__ void k_factor_errors(int F, const int* __restrict__ kind, const int* 
   err[f] = 0.5 * quad6(e, info + (size_t)f * 36);
 }
 
-// per-factor Gauss-Newton contributions: blocks (FP32), gradients and cost (FP64)
+// per-factor Gauss-Newton contributions: blocks, gradients and cost (FP64)
 __global__ void k_linearize(int F, const int* __restrict__ kind, const int* __restrict__ fi,
                             const int* __restrict__ fj, const double* __restrict__ z,
                             const double* __restrict__ info, const double* __restrict__ w,
-                            const double* __restrict__ x, float* __restrict__ Hii,
-                            float* __restrict__ Hjj, float* __restrict__ Hij,
+                            const double* __restrict__ x, double* __restrict__ Hii,
+                            double* __restrict__ Hjj, double* __restrict__ Hij,
                             double* __restrict__ gi, double* __restrict__ gj,
                             double* __restrict__ cost) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
@@ -279,9 +279,9 @@ __global__ void k_linearize(int F, const int* __restrict__ kind, const int* __re
         sjj += L.Jj[m * 6 + r] * LJj[m * 6 + c];
         sij += L.Ji[m * 6 + r] * LJj[m * 6 + c];
       }
-      Hii[(size_t)f * 36 + r * 6 + c] = (float)sii;
-      Hjj[(size_t)f * 36 + r * 6 + c] = (float)sjj;
-      Hij[(size_t)f * 36 + r * 6 + c] = (float)sij;
+      Hii[(size_t)f * 36 + r * 6 + c] = sii;
+      Hjj[(size_t)f * 36 + r * 6 + c] = sjj;
+      Hij[(size_t)f * 36 + r * 6 + c] = sij;
     }
   }
 }
@@ -352,25 +352,23 @@ __global__ void k_inc_sort(int N, const int* __restrict__ off, int* __restrict__
   }
 }
 
-// Diagonal blocks, gradient rhs b = -g (FP32 for PCG), Hessian diagonal and the
-// block-Jacobi inverse of (D + lambda diag(D)) (or + lambda I without diagonal
-// damping), one thread per pose.
+// Diagonal blocks, gradient rhs b = -g and the Hessian diagonal, one thread
+// per pose, summing the incident factors in factor order (deterministic).
 __global__ void k_assemble(int N, const int* __restrict__ off, const int* __restrict__ inc,
-                           const float* __restrict__ Hii, const float* __restrict__ Hjj,
+                           const double* __restrict__ Hii, const double* __restrict__ Hjj,
                            const double* __restrict__ gi, const double* __restrict__ gj,
-                           float* __restrict__ D, double* __restrict__ bvec64,
-                           float* __restrict__ bvec, float* __restrict__ hdiag) {
+                           double* __restrict__ D, double* __restrict__ bvec,
+                           double* __restrict__ hdiag) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= N) return;
-  float acc[36];
-  double g[6];
+  double acc[36], g[6];
 #pragma unroll
-  for (int e = 0; e < 36; ++e) acc[e] = 0.f;
+  for (int e = 0; e < 36; ++e) acc[e] = 0.0;
 #pragma unroll
   for (int e = 0; e < 6; ++e) g[e] = 0.0;
   for (int k = off[p]; k < off[p + 1]; ++k) {
     const int t = inc[k], f = t >> 1;
-    const float* B = (t & 1) ? Hjj + (size_t)f * 36 : Hii + (size_t)f * 36;
+    const double* B = (t & 1) ? Hjj + (size_t)f * 36 : Hii + (size_t)f * 36;
     const double* gg = (t & 1) ? gj + (size_t)f * 6 : gi + (size_t)f * 6;
 #pragma unroll
     for (int e = 0; e < 36; ++e) acc[e] += B[e];
@@ -381,70 +379,9 @@ __global__ void k_assemble(int N, const int* __restrict__ off, const int* __rest
   for (int e = 0; e < 36; ++e) D[(size_t)p * 36 + e] = acc[e];
 #pragma unroll
   for (int e = 0; e < 6; ++e) {
-    bvec64[(size_t)p * 6 + e] = -g[e];
-    bvec[(size_t)p * 6 + e] = (float)(-g[e]);
+    bvec[(size_t)p * 6 + e] = -g[e];
     hdiag[(size_t)p * 6 + e] = acc[e * 7];
   }
-}
-
-// block-Jacobi preconditioner: inverse of D_p + lambda * damp_p (FP64 Cholesky)
-__global__ void k_precond(int N, const float* __restrict__ D, const float* __restrict__ hdiag,
-                          double lambda, int diag_damping, double min_diag, double max_diag,
-                          float* __restrict__ Minv) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= N) return;
-  double A[36];
-#pragma unroll
-  for (int e = 0; e < 36; ++e) A[e] = D[(size_t)p * 36 + e];
-#pragma unroll
-  for (int i = 0; i < 6; ++i) {
-    double d = diag_damping ? fmin(fmax((double)hdiag[(size_t)p * 6 + i], min_diag), max_diag) : 1.0;
-    A[i * 7] += lambda * d;
-  }
-  // Cholesky A = L L^T (lower, in place), tiny pivots floored
-  double L[36];
-#pragma unroll
-  for (int e = 0; e < 36; ++e) L[e] = 0.0;
-#pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    double x = A[k * 7];
-#pragma unroll
-    for (int j = 0; j < k; ++j) x -= L[k * 6 + j] * L[k * 6 + j];
-    x = sqrt(fmax(x, 1e-300));
-    L[k * 7] = x;
-#pragma unroll
-    for (int i = k + 1; i < 6; ++i) {
-      double v = A[i * 6 + k];
-#pragma unroll
-      for (int j = 0; j < k; ++j) v -= L[i * 6 + j] * L[k * 6 + j];
-      L[i * 6 + k] = v / x;
-    }
-  }
-  // inverse via L^-1
-  double Li[36];
-#pragma unroll
-  for (int e = 0; e < 36; ++e) Li[e] = 0.0;
-#pragma unroll
-  for (int c = 0; c < 6; ++c) {
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-      if (i < c) continue;
-      double v = (i == c) ? 1.0 : 0.0;
-#pragma unroll
-      for (int j = 0; j < 6; ++j)
-        if (j >= c && j < i) v -= L[i * 6 + j] * Li[j * 6 + c];
-      Li[i * 6 + c] = v / L[i * 7];
-    }
-  }
-#pragma unroll
-  for (int i = 0; i < 6; ++i)
-#pragma unroll
-    for (int j = 0; j < 6; ++j) {
-      double s = 0.0;
-#pragma unroll
-      for (int k = 0; k < 6; ++k) s += Li[k * 6 + i] * Li[k * 6 + j];
-      Minv[(size_t)p * 36 + i * 6 + j] = (float)s;
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -477,13 +414,13 @@ __global__ void k_chain_find(int F, const int* __restrict__ kind, const int* __r
   else if (a == b + 1) atomicMin(&chain[b], 2 * f + 1);
 }
 
-__device__ __forceinline__ void damped_block(const float* __restrict__ D, const float* __restrict__ hdiag,
+__device__ __forceinline__ void damped_block(const double* __restrict__ D, const double* __restrict__ hdiag,
                                              int p, double lambda, int diag_damping, double* A) {
 #pragma unroll
   for (int e = 0; e < 36; ++e) A[e] = D[(size_t)p * 36 + e];
 #pragma unroll
   for (int i = 0; i < 6; ++i) {
-    const double d = diag_damping ? fmin(fmax((double)hdiag[(size_t)p * 6 + i], 1e-6), 1e32) : 1.0;
+    const double d = diag_damping ? fmin(fmax(hdiag[(size_t)p * 6 + i], 1e-6), 1e32) : 1.0;
     A[i * 7] += lambda * d;
   }
 }
@@ -536,8 +473,8 @@ __device__ void spd_inverse6(const double* A, double* Ainv) {
 
 // Factor: Dp_k = A_kk - U_{k-1}^T Dp_{k-1}^-1 U_{k-1}; stores Dp_k^-1 (Minv)
 // and U_k = H_{k,k+1} (Useg, zero at a segment end or without odometry).
-__global__ void k_seg_factor(int N, int S, const int* __restrict__ chain, const float* __restrict__ D,
-                             const float* __restrict__ hdiag, const float* __restrict__ Hij,
+__global__ void k_seg_factor(int N, int S, const int* __restrict__ chain, const double* __restrict__ D,
+                             const double* __restrict__ hdiag, const double* __restrict__ Hij,
                              double lambda, int diag_damping, float* __restrict__ Minv,
                              float* __restrict__ Useg) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -583,14 +520,14 @@ __global__ void k_seg_factor(int N, int S, const int* __restrict__ chain, const 
     if (c != INT_MAX) {
       const int f = c >> 1;
       const bool flip = c & 1;
-      const float* B = Hij + (size_t)f * 36;
+      const double* B = Hij + (size_t)f * 36;
 #pragma unroll
       for (int i = 0; i < 6; ++i)
 #pragma unroll
         for (int j = 0; j < 6; ++j) {
-          const float v = flip ? B[j * 6 + i] : B[i * 6 + j];
+          const double v = flip ? B[j * 6 + i] : B[i * 6 + j];
           U_prev[i * 6 + j] = v;
-          Useg[(size_t)k * 36 + i * 6 + j] = v;
+          Useg[(size_t)k * 36 + i * 6 + j] = (float)v;
         }
       have_prev = true;
     } else {
@@ -601,9 +538,10 @@ __global__ void k_seg_factor(int N, int S, const int* __restrict__ chain, const 
   }
 }
 
-// z = M^-1 r by the segment LDL^T, rz += r . z
+// z = M^-1 r by the segment LDL^T (FP32 factors and arithmetic, FP64 vectors),
+// rz += r . z
 __global__ void k_seg_apply(int N, int S, const float* __restrict__ Minv, const float* __restrict__ Useg,
-                            const float* __restrict__ r, float* __restrict__ z, double* __restrict__ rz) {
+                            const double* __restrict__ r, double* __restrict__ z, double* __restrict__ rz) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int k0 = s * S;
   double acc = 0.0;
@@ -612,14 +550,14 @@ __global__ void k_seg_apply(int N, int S, const float* __restrict__ Minv, const 
     // forward: y_k = r_k - U_{k-1}^T Dinv_{k-1} y_{k-1};  store w_k = Dinv_k y_k in z
     float y[6], t[6];
 #pragma unroll
-    for (int e = 0; e < 6; ++e) y[e] = r[(size_t)k0 * 6 + e];
+    for (int e = 0; e < 6; ++e) y[e] = (float)r[(size_t)k0 * 6 + e];
     for (int k = k0; k < k1; ++k) {
       if (k > k0) {
         const float* U = Useg + (size_t)(k - 1) * 36;
         float yn[6];
 #pragma unroll
         for (int i = 0; i < 6; ++i) {
-          float v = r[(size_t)k * 6 + i];
+          float v = (float)r[(size_t)k * 6 + i];
 #pragma unroll
           for (int m = 0; m < 6; ++m) v -= U[m * 6 + i] * t[m];
           yn[i] = v;
@@ -640,7 +578,7 @@ __global__ void k_seg_apply(int N, int S, const float* __restrict__ Minv, const 
     // backward: z_k = w_k - Dinv_k U_k z_{k+1}
     float zn[6];
 #pragma unroll
-    for (int e = 0; e < 6; ++e) zn[e] = z[(size_t)(k1 - 1) * 6 + e];
+    for (int e = 0; e < 6; ++e) zn[e] = (float)z[(size_t)(k1 - 1) * 6 + e];
     for (int k = k1 - 2; k >= k0; --k) {
       const float* U = Useg + (size_t)k * 36;
       const float* Di = Minv + (size_t)k * 36;
@@ -654,7 +592,7 @@ __global__ void k_seg_apply(int N, int S, const float* __restrict__ Minv, const 
       }
 #pragma unroll
       for (int i = 0; i < 6; ++i) {
-        float v = z[(size_t)k * 6 + i];
+        float v = (float)z[(size_t)k * 6 + i];
 #pragma unroll
         for (int m = 0; m < 6; ++m) v -= Di[i * 6 + m] * u[m];
         zn[i] = v;
@@ -663,7 +601,254 @@ __global__ void k_seg_apply(int N, int S, const float* __restrict__ Minv, const 
     }
     for (int k = k0; k < k1; ++k)
 #pragma unroll
-      for (int e = 0; e < 6; ++e) acc += (double)r[(size_t)k * 6 + e] * (double)z[(size_t)k * 6 + e];
+      for (int e = 0; e < 6; ++e) acc += r[(size_t)k * 6 + e] * z[(size_t)k * 6 + e];
+  }
+  acc = wave_sum_pre(acc);
+  if ((threadIdx.x & 63) == 0 && acc != 0.0) atomicAdd(rz, acc);
+}
+
+// ---------------------------------------------------------------------------
+// Coarse level of the two-level preconditioner.  Every coarse segment of Sc
+// consecutive poses moves as one rigid body: delta_k = Ad(T_k^-1 T_a) c_s with
+// a the segment's first pose, so intra-segment between factors see no change
+// under the coarse modes (the low-curvature modes block methods converge on
+// slowly).  With P that prolongation, Ac = P^T (H + lambda damp) P
+// (nc = 6 nseg, dense FP64) is assembled per linear solve, factored (dense
+// Cholesky) and inverted explicitly; each PCG iteration applies the additive
+// preconditioner  z = B_fine^-1 r + P Ac^-1 P^T r.
+// ---------------------------------------------------------------------------
+__global__ void k_coarse_basis(int N, int Sc, const double* __restrict__ x, float* __restrict__ Pb) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= N) return;
+  const int a = (k / Sc) * Sc;
+  double ki[12], T[12], Ad[36];
+  R3::pose_inverse(x + (size_t)k * 12, ki);
+  R3::pose_compose(ki, x + (size_t)a * 12, T);
+  R3::adjoint(T, Ad);
+#pragma unroll
+  for (int e = 0; e < 36; ++e) Pb[(size_t)k * 36 + e] = (float)Ad[e];
+}
+
+// out = P1^T M P2 for 6x6 blocks (P in FP32, accumulation in FP64)
+__device__ __forceinline__ void ptmp(const float* P1, const double* M, const float* P2, double* out) {
+  double T[36];
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) s += M[i * 6 + m] * (double)P2[m * 6 + j];
+      T[i * 6 + j] = s;
+    }
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) s += (double)P1[m * 6 + i] * T[m * 6 + j];
+      out[i * 6 + j] = s;
+    }
+}
+
+// Ac(s,s) += P_k^T (D_k + lambda damp_k) P_k
+__global__ void k_coarse_diag(int N, int Sc, int nc, const float* __restrict__ Pb,
+                              const double* __restrict__ D, const double* __restrict__ hdiag,
+                              double lambda, int diag_damping, double* __restrict__ Ac) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= N) return;
+  double B[36], M[36];
+  damped_block(D, hdiag, k, lambda, diag_damping, B);
+  const float* P = Pb + (size_t)k * 36;
+  ptmp(P, B, P, M);
+  const int s = k / Sc;
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) atomicAdd(&Ac[(size_t)(6 * s + i) * nc + 6 * s + j], M[i * 6 + j]);
+}
+
+// Ac(s,t) += P_i^T H_ij P_j and Ac(t,s) += its transpose, per between factor
+__global__ void k_coarse_offdiag(int F, int Sc, int nc, const int* __restrict__ kind,
+                                 const int* __restrict__ fi, const int* __restrict__ fj,
+                                 const double* __restrict__ Hij, const float* __restrict__ Pb,
+                                 double* __restrict__ Ac) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F || kind[f] != 0) return;
+  const int i = fi[f], j = fj[f];
+  double H[36], M[36];
+#pragma unroll
+  for (int e = 0; e < 36; ++e) H[e] = Hij[(size_t)f * 36 + e];
+  ptmp(Pb + (size_t)i * 36, H, Pb + (size_t)j * 36, M);
+  const int s = i / Sc, t = j / Sc;
+#pragma unroll
+  for (int a = 0; a < 6; ++a)
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+      atomicAdd(&Ac[(size_t)(6 * s + a) * nc + 6 * t + b], M[a * 6 + b]);
+      atomicAdd(&Ac[(size_t)(6 * t + b) * nc + 6 * s + a], M[a * 6 + b]);
+    }
+}
+
+// Blocked right-looking Cholesky of the dense nc x nc coarse matrix (lower,
+// in place), panels of kCholNb columns: k_chol_panel factors one panel with a
+// single workgroup, k_chol_update applies its rank-kCholNb update to the
+// trailing lower triangle in 32x32 tiles across the chip.  Pivots are floored
+// at a tiny positive value so a gauge-free (semi-definite) problem keeps a
+// finite preconditioner.
+constexpr int kCholNb = 32;
+
+__global__ void __launch_bounds__(256) k_chol_panel(int n, int k0, double* __restrict__ A) {
+  const int nb = min(kCholNb, n - k0);
+  __shared__ double piv;
+  for (int c = k0; c < k0 + nb; ++c) {
+    if (threadIdx.x == 0) {
+      piv = sqrt(fmax(A[(size_t)c * n + c], 1e-300));
+      A[(size_t)c * n + c] = piv;
+    }
+    __syncthreads();
+    const double d = piv;
+    for (int i = c + 1 + threadIdx.x; i < n; i += blockDim.x) A[(size_t)i * n + c] /= d;
+    __syncthreads();
+    // update the remaining panel columns c+1 .. k0+nb-1 (lower part, rows > c)
+    const int w = k0 + nb - (c + 1);
+    if (w > 0) {
+      for (long long t = threadIdx.x; t < (long long)(n - c - 1) * w; t += blockDim.x) {
+        const int i = c + 1 + (int)(t / w), j = c + 1 + (int)(t % w);
+        if (j <= i) A[(size_t)i * n + j] -= A[(size_t)i * n + c] * A[(size_t)j * n + c];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// trailing update A[i][j] -= sum_{c in panel} L[i][c] L[j][c], j <= i, rows >= k0 + nb
+__global__ void __launch_bounds__(256) k_chol_update(int n, int k0, double* __restrict__ A) {
+  const int nb = min(kCholNb, n - k0);
+  const int r0 = k0 + nb;
+  // lower-triangle tile index -> (ti, tj), tj <= ti
+  const int t = blockIdx.x;
+  int ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+  while (ti * (ti + 1) / 2 > t) --ti;
+  const int tj = t - ti * (ti + 1) / 2;
+  __shared__ double Li[32][kCholNb + 1], Lj[32][kCholNb + 1];
+  for (int e = threadIdx.x; e < 32 * kCholNb; e += blockDim.x) {
+    const int rr = e / kCholNb, cc = e % kCholNb;
+    const int gi = r0 + ti * 32 + rr, gj = r0 + tj * 32 + rr;
+    Li[rr][cc] = (gi < n && cc < nb) ? A[(size_t)gi * n + k0 + cc] : 0.0;
+    Lj[rr][cc] = (gj < n && cc < nb) ? A[(size_t)gj * n + k0 + cc] : 0.0;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 32 * 32; e += blockDim.x) {
+    const int rr = e >> 5, cc = e & 31;
+    const int gi = r0 + ti * 32 + rr, gj = r0 + tj * 32 + cc;
+    if (gi >= n || gj > gi) continue;
+    double sum = 0.0;
+#pragma unroll 8
+    for (int m = 0; m < kCholNb; ++m) sum += Li[rr][m] * Lj[cc][m];
+    A[(size_t)gi * n + gj] -= sum;
+  }
+}
+
+// W = L^-1 (lower): one wave per column c; w_i = (delta_ic - sum_{c<=j<i} L_ij w_j) / L_ii.
+// Lane l owns the entries j = l (mod 64) of its column in LDS, so every LDS
+// value is read back only by the lane that wrote it.
+__global__ void __launch_bounds__(256) k_tri_inv(int n, const double* __restrict__ L, double* __restrict__ W) {
+  __shared__ double wsh[4][kCoarseMax];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int c = blockIdx.x * 4 + wv;
+  if (c >= n) return;
+  double* w = wsh[wv];
+  for (int i = c; i < n; ++i) {
+    const double* Lrow = L + (size_t)i * n;
+    double part = 0.0;
+    for (int j = c + lane; j < i; j += 64) part += Lrow[j] * w[j];
+    part = wave_sum_pre(part);
+    const double wi = ((i == c) ? 1.0 : 0.0) - part;
+    const double v = wi / Lrow[i];
+    if ((i & 63) == lane) w[i] = v;
+    if (lane == 0) W[(size_t)i * n + c] = v;
+  }
+  for (int i = lane; i < c; i += 64) W[(size_t)i * n + c] = 0.0;
+}
+
+// Ainv = Linv^T Linv (symmetric)
+__global__ void k_gram(int n, const double* __restrict__ Li, double* __restrict__ Ainv) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)n * n) return;
+  const int i = (int)(t / n), j = (int)(t % n);
+  const int k0 = i > j ? i : j;
+  double s = 0.0;
+  for (int k = k0; k < n; ++k) s += Li[(size_t)k * n + i] * Li[(size_t)k * n + j];
+  Ainv[t] = s;
+}
+
+// rc = P^T r : one wave per coarse segment (deterministic, no atomics)
+__global__ void k_coarse_restrict(int N, int Sc, int nseg, const float* __restrict__ Pb,
+                                  const double* __restrict__ r, double* __restrict__ rc) {
+  const int s = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (s >= nseg) return;
+  const int k0 = s * Sc, k1 = min(k0 + Sc, N);
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int k = k0 + lane; k < k1; k += 64) {
+    const float* P = Pb + (size_t)k * 36;
+    double rk[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) rk[m] = r[(size_t)k * 6 + m];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      double v = 0.0;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) v += (double)P[m * 6 + i] * rk[m];
+      acc[i] += v;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i) acc[i] = wave_sum_pre(acc[i]);
+  if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < 6; ++i) rc[6 * s + i] = acc[i];
+}
+
+// cc = Ainv rc : 64 rows per workgroup of 256, four row-quarter partials in LDS
+__global__ void __launch_bounds__(256) k_coarse_matvec(int nc, const double* __restrict__ Ainv,
+                                                       const double* __restrict__ rc,
+                                                       double* __restrict__ cc) {
+  __shared__ double part[4][64];
+  const int row = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int q = threadIdx.x >> 6;
+  double s = 0.0;
+  if (row < nc) {
+    const int j0 = (nc * q) / 4, j1 = (nc * (q + 1)) / 4;
+    // Ainv is symmetric: read column `row` (coalesced across the wave)
+    for (int j = j0; j < j1; ++j) s += Ainv[(size_t)j * nc + row] * rc[j];
+  }
+  part[q][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (q == 0 && row < nc)
+    cc[row] = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
+}
+
+// z_k += P_k cc_s ; rz += r_k . (P_k cc_s)
+__global__ void k_coarse_prolong(int N, int Sc, const float* __restrict__ Pb, const double* __restrict__ cc,
+                                 const double* __restrict__ r, double* __restrict__ z,
+                                 double* __restrict__ rz) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  if (k < N) {
+    const float* P = Pb + (size_t)k * 36;
+    const int s = k / Sc;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      double v = 0.0;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) v += (double)P[i * 6 + m] * cc[6 * s + m];
+      z[(size_t)k * 6 + i] += v;
+      acc += r[(size_t)k * 6 + i] * v;
+    }
   }
   acc = wave_sum_pre(acc);
   if ((threadIdx.x & 63) == 0 && acc != 0.0) atomicAdd(rz, acc);
@@ -672,43 +857,43 @@ __global__ void k_seg_apply(int N, int S, const float* __restrict__ Minv, const 
 // y = (H + lambda damp) x : one thread per pose row (deterministic order)
 __global__ void k_spmv(int N, const int* __restrict__ off, const int* __restrict__ inc,
                        const int* __restrict__ fi, const int* __restrict__ fj,
-                       const int* __restrict__ kind, const float* __restrict__ D,
-                       const float* __restrict__ Hij, const float* __restrict__ hdiag,
+                       const int* __restrict__ kind, const double* __restrict__ D,
+                       const double* __restrict__ Hij, const double* __restrict__ hdiag,
                        const double* __restrict__ lam_ptr, double lambda_imm,
                        int diag_damping, double min_diag, double max_diag,
-                       const float* __restrict__ x, float* __restrict__ y) {
+                       const double* __restrict__ x, double* __restrict__ y) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= N) return;
   const double lambda = lam_ptr ? *lam_ptr : lambda_imm;
-  float xp[6], acc[6];
+  double xp[6], acc[6];
 #pragma unroll
   for (int e = 0; e < 6; ++e) xp[e] = x[(size_t)p * 6 + e];
 #pragma unroll
   for (int r = 0; r < 6; ++r) {
-    float s = 0.f;
+    double s = 0.0;
 #pragma unroll
     for (int c = 0; c < 6; ++c) s += D[(size_t)p * 36 + r * 6 + c] * xp[c];
-    const double d = diag_damping ? fmin(fmax((double)hdiag[(size_t)p * 6 + r], min_diag), max_diag) : 1.0;
-    acc[r] = s + (float)(lambda * d) * xp[r];
+    const double d = diag_damping ? fmin(fmax(hdiag[(size_t)p * 6 + r], min_diag), max_diag) : 1.0;
+    acc[r] = s + lambda * d * xp[r];
   }
   for (int k = off[p]; k < off[p + 1]; ++k) {
     const int t = inc[k], f = t >> 1;
     if (kind[f] != 0) continue;
-    const float* B = Hij + (size_t)f * 36;
+    const double* B = Hij + (size_t)f * 36;
     if ((t & 1) == 0) {  // row i: + Hij x_j
-      const float* xq = x + (size_t)fj[f] * 6;
+      const double* xq = x + (size_t)fj[f] * 6;
 #pragma unroll
       for (int r = 0; r < 6; ++r) {
-        float s = 0.f;
+        double s = 0.0;
 #pragma unroll
         for (int c = 0; c < 6; ++c) s += B[r * 6 + c] * xq[c];
         acc[r] += s;
       }
     } else {  // row j: + Hij^T x_i
-      const float* xq = x + (size_t)fi[f] * 6;
+      const double* xq = x + (size_t)fi[f] * 6;
 #pragma unroll
       for (int r = 0; r < 6; ++r) {
-        float s = 0.f;
+        double s = 0.0;
 #pragma unroll
         for (int c = 0; c < 6; ++c) s += B[c * 6 + r] * xq[c];
         acc[r] += s;
@@ -719,74 +904,53 @@ __global__ void k_spmv(int N, const int* __restrict__ off, const int* __restrict
   for (int r = 0; r < 6; ++r) y[(size_t)p * 6 + r] = acc[r];
 }
 
-// FP64 dot of FP32 vectors, wave-reduced then atomically accumulated
+// FP64 dot, wave-reduced then atomically accumulated
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
   return v;
 }
 
-__global__ void k_dot(long long n, const float* __restrict__ a, const float* __restrict__ b,
+__global__ void k_dot(long long n, const double* __restrict__ a, const double* __restrict__ b,
                       double* __restrict__ out) {
   double s = 0.0;
   for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
        k += (long long)gridDim.x * blockDim.x)
-    s += (double)a[k] * (double)b[k];
+    s += a[k] * b[k];
   s = wave_sum(s);
   if ((threadIdx.x & 63) == 0) atomicAdd(out, s);
 }
 
-// z = M^-1 r (block-Jacobi), and rz += r . z
-__global__ void k_apply_precond(int N, const float* __restrict__ Minv, const float* __restrict__ r,
-                                float* __restrict__ z, double* __restrict__ rz) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  double s = 0.0;
-  if (p < N) {
-    float rp[6];
-#pragma unroll
-    for (int e = 0; e < 6; ++e) rp[e] = r[(size_t)p * 6 + e];
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-      float v = 0.f;
-#pragma unroll
-      for (int j = 0; j < 6; ++j) v += Minv[(size_t)p * 36 + i * 6 + j] * rp[j];
-      z[(size_t)p * 6 + i] = v;
-      s += (double)rp[i] * (double)v;
-    }
-  }
-  s = wave_sum(s);
-  if ((threadIdx.x & 63) == 0 && s != 0.0) atomicAdd(rz, s);
-}
-
 // scalars: sc[0] = rz, sc[1] = pq, sc[2] = rz_new, sc[3] = |r|^2
 // x += alpha p ; r -= alpha q ; alpha = rz / pq ; accumulates |r|^2 into sc[3]
-__global__ void k_cg_update_xr(long long n, const double* __restrict__ sc, const float* __restrict__ p,
-                               const float* __restrict__ q, float* __restrict__ x,
-                               float* __restrict__ r, double* __restrict__ rr) {
+__global__ void k_cg_update_xr(long long n, const double* __restrict__ sc, const double* __restrict__ p,
+                               const double* __restrict__ q, double* __restrict__ x,
+                               double* __restrict__ r, double* __restrict__ rr) {
   const double pq = sc[1];
-  const float alpha = (float)(pq != 0.0 ? sc[0] / pq : 0.0);
+  const double alpha = pq != 0.0 ? sc[0] / pq : 0.0;
   double s = 0.0;
   for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
        k += (long long)gridDim.x * blockDim.x) {
     x[k] += alpha * p[k];
-    const float rv = r[k] - alpha * q[k];
+    const double rv = r[k] - alpha * q[k];
     r[k] = rv;
-    s += (double)rv * (double)rv;
+    s += rv * rv;
   }
   s = wave_sum(s);
   if ((threadIdx.x & 63) == 0) atomicAdd(rr, s);
 }
 
-// p = z + beta p, beta = rz_new / rz ; then rz <- rz_new, clear pq / rz_new / rr
-__global__ void k_cg_update_p(long long n, const double* __restrict__ sc, const float* __restrict__ z,
-                              float* __restrict__ p) {
+// p = z + beta p, beta = rz_new / rz
+__global__ void k_cg_update_p(long long n, const double* __restrict__ sc, const double* __restrict__ z,
+                              double* __restrict__ p) {
   const double rz = sc[0];
-  const float beta = (float)(rz != 0.0 ? sc[2] / rz : 0.0);
+  const double beta = rz != 0.0 ? sc[2] / rz : 0.0;
   for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
        k += (long long)gridDim.x * blockDim.x)
     p[k] = z[k] + beta * p[k];
 }
 
+// rz <- rz_new, clear pq / rz_new / rr, record |r|^2 of this iteration
 __global__ void k_cg_roll(double* sc, double* hist) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     const int it = (int)sc[32];
@@ -799,14 +963,14 @@ __global__ void k_cg_roll(double* sc, double* hist) {
   }
 }
 
-__global__ void k_copy_f(long long n, const float* __restrict__ a, float* __restrict__ b) {
+__global__ void k_copy_d(long long n, const double* __restrict__ a, double* __restrict__ b) {
   for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
        k += (long long)gridDim.x * blockDim.x)
     b[k] = a[k];
 }
 
 // x_new = x * Exp(delta) per pose
-__global__ void k_retract(int N, const double* __restrict__ x, const float* __restrict__ dx,
+__global__ void k_retract(int N, const double* __restrict__ x, const double* __restrict__ dx,
                           double* __restrict__ xn) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= N) return;
@@ -830,17 +994,6 @@ __global__ void k_sum(long long n, const double* __restrict__ a, const double* _
 }
 
 // b . dx in FP64 (b kept in FP64)
-__global__ void k_dot_df(long long n, const double* __restrict__ a, const float* __restrict__ b,
-                         double* __restrict__ out) {
-  double s = 0.0;
-  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
-       k += (long long)gridDim.x * blockDim.x)
-    s += a[k] * (double)b[k];
-  s = wave_sum(s);
-  if ((threadIdx.x & 63) == 0) atomicAdd(out, s);
-}
-
-// TLS weights (GncOptimizer::calculateWeights), known inliers keep weight 1
 __global__ void k_tls_weights(int F, const double* __restrict__ err, const double* __restrict__ barc,
                               const unsigned char* __restrict__ known, double mu,
                               double* __restrict__ w) {
@@ -912,22 +1065,21 @@ int SolverRun::alloc(int N_, int F_) {
   if ((rc = known.reserve(F))) return rc;
   if ((rc = err.reserve(sizeof(double) * F))) return rc;
   if ((rc = cost.reserve(sizeof(double) * F))) return rc;
-  if ((rc = Hii.reserve(sizeof(float) * 36 * F))) return rc;
-  if ((rc = Hjj.reserve(sizeof(float) * 36 * F))) return rc;
-  if ((rc = Hij.reserve(sizeof(float) * 36 * F))) return rc;
+  if ((rc = Hii.reserve(sizeof(double) * 36 * F))) return rc;
+  if ((rc = Hjj.reserve(sizeof(double) * 36 * F))) return rc;
+  if ((rc = Hij.reserve(sizeof(double) * 36 * F))) return rc;
   if ((rc = gi.reserve(sizeof(double) * 6 * F))) return rc;
   if ((rc = gj.reserve(sizeof(double) * 6 * F))) return rc;
   if ((rc = inc_off.reserve(sizeof(int) * (N + 1)))) return rc;
   if ((rc = inc_cnt.reserve(sizeof(int) * (N + 1)))) return rc;
   if ((rc = inc.reserve(sizeof(int) * 2 * F))) return rc;
-  if ((rc = D.reserve(sizeof(float) * 36 * N))) return rc;
+  if ((rc = D.reserve(sizeof(double) * 36 * N))) return rc;
   if ((rc = Minv.reserve(sizeof(float) * 36 * N))) return rc;
   if ((rc = Useg.reserve(sizeof(float) * 36 * N))) return rc;
   if ((rc = chain.reserve(sizeof(int) * (N + 1)))) return rc;
-  if ((rc = hdiag.reserve(sizeof(float) * n6))) return rc;
-  if ((rc = b64.reserve(sizeof(double) * n6))) return rc;
-  for (DevBuf* v : {&b, &dx, &r, &zv, &p, &q, &hx})
-    if ((rc = v->reserve(sizeof(float) * n6))) return rc;
+  if ((rc = hdiag.reserve(sizeof(double) * n6))) return rc;
+  for (DevBuf* v : {&b64, &dx, &r, &zv, &p, &q, &hx})
+    if ((rc = v->reserve(sizeof(double) * n6))) return rc;
   if ((rc = sc.reserve(sizeof(double) * 64))) return rc;
   return RPGO_OK;
 }
@@ -988,48 +1140,108 @@ int SolverRun::total_cost(hipStream_t st, const double* xp, double* out) {
 int SolverRun::linearize(hipStream_t st) {
   k_linearize<<<grid_for(F, 64, 1 << 22), 64, 0, st>>>(
       F, kind.as<int>(), fi.as<int>(), fj.as<int>(), z.as<double>(), info.as<double>(), w.as<double>(),
-      x.as<double>(), Hii.as<float>(), Hjj.as<float>(), Hij.as<float>(), gi.as<double>(),
+      x.as<double>(), Hii.as<double>(), Hjj.as<double>(), Hij.as<double>(), gi.as<double>(),
       gj.as<double>(), cost.as<double>());
   k_assemble<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(
-      N, inc_off.as<int>(), inc.as<int>(), Hii.as<float>(), Hjj.as<float>(), gi.as<double>(),
-      gj.as<double>(), D.as<float>(), b64.as<double>(), b.as<float>(), hdiag.as<float>());
+      N, inc_off.as<int>(), inc.as<int>(), Hii.as<double>(), Hjj.as<double>(), gi.as<double>(),
+      gj.as<double>(), D.as<double>(), b64.as<double>(), hdiag.as<double>());
   RPGO_HIP_CHECK(hipGetLastError());
   ++stats.linearizations;
   return RPGO_OK;
 }
 
 // y = (H + lambda damp) x
-void SolverRun::spmv(hipStream_t st, double lambda, const float* xin, float* yout) {
+void SolverRun::spmv(hipStream_t st, double lambda, const double* xin, double* yout) {
   k_spmv<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(
-      N, inc_off.as<int>(), inc.as<int>(), fi.as<int>(), fj.as<int>(), kind.as<int>(), D.as<float>(),
-      Hij.as<float>(), hdiag.as<float>(), nullptr, lambda, prm.lm_diagonal_damping, 1e-6, 1e32, xin,
+      N, inc_off.as<int>(), inc.as<int>(), fi.as<int>(), fj.as<int>(), kind.as<int>(), D.as<double>(),
+      Hij.as<double>(), hdiag.as<double>(), nullptr, lambda, prm.lm_diagonal_damping, 1e-6, 1e32, xin,
       yout);
 }
 
 // same, lambda read from device memory (for the captured PCG body)
-void SolverRun::spmv_dev(hipStream_t st, const double* lam, const float* xin, float* yout) {
+void SolverRun::spmv_dev(hipStream_t st, const double* lam, const double* xin, double* yout) {
   k_spmv<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(
-      N, inc_off.as<int>(), inc.as<int>(), fi.as<int>(), fj.as<int>(), kind.as<int>(), D.as<float>(),
-      Hij.as<float>(), hdiag.as<float>(), lam, 0.0, prm.lm_diagonal_damping, 1e-6, 1e32, xin, yout);
+      N, inc_off.as<int>(), inc.as<int>(), fi.as<int>(), fj.as<int>(), kind.as<int>(), D.as<double>(),
+      Hij.as<double>(), hdiag.as<double>(), lam, 0.0, prm.lm_diagonal_damping, 1e-6, 1e32, xin, yout);
 }
 
-// Solve (H + lambda damp) dx = b by PCG with the segment block-tridiagonal
-// preconditioner (x0 = 0).
+// Coarse level set-up for one linear solve: prolongation blocks at the current
+// poses, Ac = P^T (H + lambda damp) P, its Cholesky factor, L^-1 and the FP32
+// explicit inverse.  *nseg = 0 when the coarse level is off.
+int SolverRun::coarse_setup(hipStream_t st, double lambda, int* Sc_out, int* nseg_out) {
+  *Sc_out = 0;
+  *nseg_out = 0;
+  if (prm.pcg_coarse < 0 || N < 2) return RPGO_OK;
+  const int segs_max = prm.pcg_coarse > 0 ? prm.pcg_coarse : kCoarseSegMax;
+  const int Sfine = prm.pcg_segment > 0 ? prm.pcg_segment : 128;
+  int Sc = std::max(Sfine, (N + segs_max - 1) / segs_max);
+  const int nseg = (N + Sc - 1) / Sc;
+  if (nseg < 2) return RPGO_OK;  // a single rigid segment is the gauge mode only
+  const int nc = 6 * nseg;
+  int rc_;
+  if ((rc_ = Pb.reserve(sizeof(float) * 36 * (size_t)N))) return rc_;
+  if ((rc_ = Ac.reserve(sizeof(double) * (size_t)nc * nc))) return rc_;
+  if ((rc_ = Lci.reserve(sizeof(double) * (size_t)nc * nc))) return rc_;
+  if ((rc_ = Ainv.reserve(sizeof(double) * (size_t)nc * nc))) return rc_;
+  if ((rc_ = cres.reserve(sizeof(double) * nc))) return rc_;
+  if ((rc_ = ccor.reserve(sizeof(double) * nc))) return rc_;
+  k_coarse_basis<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(N, Sc, x.as<double>(), Pb.as<float>());
+  RPGO_HIP_CHECK(hipMemsetAsync(Ac.ptr, 0, sizeof(double) * (size_t)nc * nc, st));
+  k_coarse_diag<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(N, Sc, nc, Pb.as<float>(), D.as<double>(),
+                                                          hdiag.as<double>(), lambda,
+                                                          prm.lm_diagonal_damping, Ac.as<double>());
+  k_coarse_offdiag<<<grid_for(F, 128, 1 << 22), 128, 0, st>>>(F, Sc, nc, kind.as<int>(), fi.as<int>(),
+                                                             fj.as<int>(), Hij.as<double>(),
+                                                             Pb.as<float>(), Ac.as<double>());
+  for (int k0 = 0; k0 < nc; k0 += kCholNb) {
+    k_chol_panel<<<1, 256, 0, st>>>(nc, k0, Ac.as<double>());
+    const int rest = nc - k0 - kCholNb;
+    if (rest > 0) {
+      const int T = (rest + 31) / 32;
+      k_chol_update<<<T * (T + 1) / 2, 256, 0, st>>>(nc, k0, Ac.as<double>());
+    }
+  }
+  k_tri_inv<<<(nc + 3) / 4, 256, 0, st>>>(nc, Ac.as<double>(), Lci.as<double>());
+  k_gram<<<grid_for((long long)nc * nc, 256, 1 << 22), 256, 0, st>>>(nc, Lci.as<double>(), Ainv.as<double>());
+  RPGO_HIP_CHECK(hipGetLastError());
+  *Sc_out = Sc;
+  *nseg_out = nseg;
+  return RPGO_OK;
+}
+
+// z = M^-1 r (fine segment LDL^T plus the coarse correction), rz += r . z
+void SolverRun::precond_apply(hipStream_t st, int S, int Sc, int nseg_c, double* rz) {
+  const int nseg = (N + S - 1) / S;
+  k_seg_apply<<<(nseg + 63) / 64, 64, 0, st>>>(N, S, Minv.as<float>(), Useg.as<float>(), r.as<double>(),
+                                               zv.as<double>(), rz);
+  if (nseg_c > 0) {
+    const int nc = 6 * nseg_c;
+    k_coarse_restrict<<<(nseg_c + 3) / 4, 256, 0, st>>>(N, Sc, nseg_c, Pb.as<float>(), r.as<double>(),
+                                                        cres.as<double>());
+    k_coarse_matvec<<<(nc + 63) / 64, 256, 0, st>>>(nc, Ainv.as<double>(), cres.as<double>(), ccor.as<double>());
+    k_coarse_prolong<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(N, Sc, Pb.as<float>(), ccor.as<double>(),
+                                                               r.as<double>(), zv.as<double>(), rz);
+  }
+}
+
+// Solve (H + lambda damp) dx = b by PCG (x0 = 0) preconditioned by the
+// segment block-tridiagonal solve plus the rigid-segment coarse correction.
 int SolverRun::pcg(hipStream_t st, double lambda, int* iters_out) {
   const long long n = (long long)N * 6;
   double* s = sc.as<double>();
   const int S = prm.pcg_segment > 0 ? prm.pcg_segment : 128;
   const int nseg = (N + S - 1) / S;
-  k_seg_factor<<<(nseg + 63) / 64, 64, 0, st>>>(N, S, chain.as<int>(), D.as<float>(), hdiag.as<float>(),
-                                                Hij.as<float>(), lambda, prm.lm_diagonal_damping,
+  k_seg_factor<<<(nseg + 63) / 64, 64, 0, st>>>(N, S, chain.as<int>(), D.as<double>(), hdiag.as<double>(),
+                                                Hij.as<double>(), lambda, prm.lm_diagonal_damping,
                                                 Minv.as<float>(), Useg.as<float>());
-  RPGO_HIP_CHECK(hipMemsetAsync(dx.ptr, 0, sizeof(float) * n, st));
-  k_copy_f<<<grid_for(n), 256, 0, st>>>(n, b.as<float>(), r.as<float>());
+  int Sc = 0, nseg_c = 0, rc_;
+  if ((rc_ = coarse_setup(st, lambda, &Sc, &nseg_c))) return rc_;
+  RPGO_HIP_CHECK(hipMemsetAsync(dx.ptr, 0, sizeof(double) * n, st));
+  k_copy_d<<<grid_for(n), 256, 0, st>>>(n, b64.as<double>(), r.as<double>());
   RPGO_HIP_CHECK(hipMemsetAsync(s, 0, sizeof(double) * 8, st));
-  k_seg_apply<<<(nseg + 63) / 64, 64, 0, st>>>(N, S, Minv.as<float>(), Useg.as<float>(), r.as<float>(),
-                                               zv.as<float>(), s + 0);
-  k_copy_f<<<grid_for(n), 256, 0, st>>>(n, zv.as<float>(), p.as<float>());
-  k_dot<<<grid_for(n), 256, 0, st>>>(n, b.as<float>(), b.as<float>(), s + 4);
+  precond_apply(st, S, Sc, nseg_c, s + 0);
+  k_copy_d<<<grid_for(n), 256, 0, st>>>(n, zv.as<double>(), p.as<double>());
+  k_dot<<<grid_for(n), 256, 0, st>>>(n, b64.as<double>(), b64.as<double>(), s + 4);
   RPGO_HIP_CHECK(hipGetLastError());
   double h[8];
   int rc = read_scalars(st, 8, h);
@@ -1050,19 +1262,27 @@ int SolverRun::pcg(hipStream_t st, double lambda, int* iters_out) {
   if ((rc = hist.reserve(sizeof(double) * (size_t)(max_it + kPcgChunk)))) return rc;
   auto body = [&]() {
     for (int j = 0; j < kPcgChunk; ++j) {
-      spmv_dev(st, s + 30, p.as<float>(), q.as<float>());
-      k_dot<<<grid_for(n), 256, 0, st>>>(n, p.as<float>(), q.as<float>(), s + 1);
-      k_cg_update_xr<<<grid_for(n), 256, 0, st>>>(n, s, p.as<float>(), q.as<float>(), dx.as<float>(),
-                                                  r.as<float>(), s + 3);
-      k_seg_apply<<<(nseg + 63) / 64, 64, 0, st>>>(N, S, Minv.as<float>(), Useg.as<float>(),
-                                                   r.as<float>(), zv.as<float>(), s + 2);
-      k_cg_update_p<<<grid_for(n), 256, 0, st>>>(n, s, zv.as<float>(), p.as<float>());
+      spmv_dev(st, s + 30, p.as<double>(), q.as<double>());
+      k_dot<<<grid_for(n), 256, 0, st>>>(n, p.as<double>(), q.as<double>(), s + 1);
+      k_cg_update_xr<<<grid_for(n), 256, 0, st>>>(n, s, p.as<double>(), q.as<double>(), dx.as<double>(),
+                                                  r.as<double>(), s + 3);
+      precond_apply(st, S, Sc, nseg_c, s + 2);
+      k_cg_update_p<<<grid_for(n), 256, 0, st>>>(n, s, zv.as<double>(), p.as<double>());
       k_cg_roll<<<1, 1, 0, st>>>(s, hist.as<double>());
     }
   };
   // one captured chunk of kPcgChunk iterations, rebuilt when sizes or buffers change
-  const GraphKey key{N, S, (const void*)hist.ptr, (const void*)p.ptr, (const void*)Minv.ptr,
-                     (const void*)Hij.ptr, (const void*)inc.ptr};
+  const GraphKey key{N,
+                     S,
+                     Sc,
+                     nseg_c,
+                     (const void*)hist.ptr,
+                     (const void*)p.ptr,
+                     (const void*)Minv.ptr,
+                     (const void*)Hij.ptr,
+                     (const void*)inc.ptr,
+                     nseg_c ? (const void*)Ainv.ptr : nullptr,
+                     nseg_c ? (const void*)Pb.ptr : nullptr};
   if (use_graphs && !(graph_exec && graph_key == key)) {
     if (graph_exec) {
       (void)hipGraphExecDestroy(graph_exec);
@@ -1108,7 +1328,7 @@ int SolverRun::pcg(hipStream_t st, double lambda, int* iters_out) {
     }
     it += kPcgChunk;
     // stagnation: FP32 recurrences stop reducing the residual
-    if (it - best_it >= 8 * kPcgChunk) done = true;
+    if (it - best_it >= 32 * kPcgChunk) done = true;
   }
   stats.pcg_iterations += it;
   *iters_out = it;
@@ -1119,10 +1339,10 @@ int SolverRun::pcg(hipStream_t st, double lambda, int* iters_out) {
 int SolverRun::model_decrease(hipStream_t st, double* out) {
   const long long n = (long long)N * 6;
   double* s = sc.as<double>();
-  spmv(st, 0.0, dx.as<float>(), hx.as<float>());
+  spmv(st, 0.0, dx.as<double>(), hx.as<double>());
   RPGO_HIP_CHECK(hipMemsetAsync(s + 10, 0, sizeof(double) * 2, st));
-  k_dot_df<<<grid_for(n), 256, 0, st>>>(n, b64.as<double>(), dx.as<float>(), s + 10);
-  k_dot<<<grid_for(n), 256, 0, st>>>(n, dx.as<float>(), hx.as<float>(), s + 11);
+  k_dot<<<grid_for(n), 256, 0, st>>>(n, b64.as<double>(), dx.as<double>(), s + 10);
+  k_dot<<<grid_for(n), 256, 0, st>>>(n, dx.as<double>(), hx.as<double>(), s + 11);
   RPGO_HIP_CHECK(hipGetLastError());
   double h[12];
   int rc = read_scalars(st, 12, h);
@@ -1152,11 +1372,12 @@ int SolverRun::nonlinear_solve(hipStream_t st, bool from_x0, double* cost_out, i
     if (gn) {
       int pit;
       if ((rc = pcg(st, 0.0, &pit))) return rc;
-      k_retract<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(N, x.as<double>(), dx.as<float>(), xn.as<double>());
+      k_retract<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(N, x.as<double>(), dx.as<double>(), xn.as<double>());
       double nerr;
       if ((rc = total_cost(st, xn.as<double>(), &nerr))) return rc;
-      std::swap(x, xn);
       ++it;
+      if (polish && !(nerr < err)) break;  // refinement keeps the better iterate
+      std::swap(x, xn);
       const bool conv = lm_converged(prm, err, nerr);
       err = nerr;
       if (conv) break;
@@ -1169,7 +1390,7 @@ int SolverRun::nonlinear_solve(hipStream_t st, bool from_x0, double* cost_out, i
       if ((rc = pcg(st, lambda, &pit))) return rc;
       double lin;
       if ((rc = model_decrease(st, &lin))) return rc;
-      k_retract<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(N, x.as<double>(), dx.as<float>(), xn.as<double>());
+      k_retract<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(N, x.as<double>(), dx.as<double>(), xn.as<double>());
       if ((rc = total_cost(st, xn.as<double>(), &nerr))) return rc;
       const double fid = lin > 0.0 ? (err - nerr) / lin : -1.0;
       if (lin >= 0.0 && fid >= prm.min_model_fidelity) {
@@ -1269,7 +1490,9 @@ int SolverRun::run(hipStream_t st, rpgo_solve_report* rep) {
     prm.lm_absolute_error_tol = 0.0;
     prm.pcg_rel_tol = std::min(prm.pcg_rel_tol, 1e-6);
     int it3;
+    polish = true;
     rc = nonlinear_solve(st, false, &cost, &it3);
+    polish = false;
     prm = saved;
     if (rc) return rc;
   }

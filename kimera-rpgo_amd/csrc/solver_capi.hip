@@ -94,6 +94,7 @@ void rpgo_solve_params_default(rpgo_solve_params* p) {
   p->polish_iterations = 0;
   p->pcg_segment = 128;
   p->gnc_warm_start = 0;
+  p->pcg_coarse = 0;
 }
 
 int rpgo_solve(rpgo_ctx* ctx, const rpgo_solve_problem* pr, const rpgo_solve_params* prm,
@@ -115,6 +116,11 @@ int rpgo_solve(rpgo_ctx* ctx, const rpgo_solve_problem* pr, const rpgo_solve_par
   if (!(prm->lambda_factor > 1.0) || !(prm->lambda_upper_bound > 0.0) || prm->lambda_initial < 0.0 ||
       prm->lm_max_iterations < 0 || prm->gnc_max_iterations < 0 || !(prm->gnc_mu_step > 0.0)) {
     set_error("rpgo_solve: invalid LM/GNC parameters (lambda_factor must exceed 1)");
+    return RPGO_ERR_ARG;
+  }
+  if (prm->pcg_coarse > kCoarseSegMax) {
+    set_error("rpgo_solve: pcg_coarse %d above the maximum of %d segments", prm->pcg_coarse,
+              kCoarseSegMax);
     return RPGO_ERR_ARG;
   }
   if (prm->known_inliers < 0 || prm->known_inliers > F) {
