@@ -184,12 +184,17 @@ typedef struct {
   double pcg_rel_tol;           /* relative residual of each linear solve */
   int32_t polish_iterations;    /* final Gauss-Newton refinement steps (0: none) */
   int32_t pcg_segment;          /* poses per block-tridiagonal preconditioner segment
-                                   (<= 0: 128; 1: plain block-Jacobi)             */
+                                   (default 4; 1: plain block-Jacobi)             */
   int32_t gnc_warm_start;       /* 0: every inner solve restarts from the initial
                                    values (GncOptimizer); 1: from the last result  */
   int32_t pcg_coarse;           /* rigid-segment coarse level of the preconditioner:
                                    < 0 off; 0 auto (up to 128 segments);
                                    > 0 at most that many segments (<= 128)        */
+  int32_t pcg_graphs;           /* 1: replay PCG iterations as captured HIP graphs;
+                                   0: direct launches of the same kernels          */
+  int32_t gnc_factor_dim;       /* degrees of freedom of the chi^2 threshold in
+                                   PROBABILITY mode: 6 (Pose3), 3 for Pose2 graphs
+                                   embedded in the plane (<= 0 selects 6)          */
 } rpgo_solve_params;
 
 typedef struct {

@@ -1,7 +1,10 @@
-"""Build librpgo_hip.so (HIP kernels + C ABI) for gfx950 with hipcc, in-tree.
+"""Build the libraries in-tree.
+
+  librpgo_hip.so     HIP kernels for gfx950 + the C ABI (hipcc)
+  libkimera_rpgo.so  C++ RobustSolver facade over the C ABI (g++)
+  tests/cpp/bin/*    C++ facade tests (g++), run by tests/test_facade_gpu.py
 
 Usage:  python kimera-rpgo_amd/build.py [--jobs N]
-Output: kimera-rpgo_amd/lib/librpgo_hip.so
 """
 from __future__ import annotations
 
@@ -30,6 +33,11 @@ EXTRA = {
 }
 SOURCES = ["capi.hip", "replay64.hip", "pcm_fp32.hip", "clique.hip", "solver.hip",
            "solver_capi.hip"]
+HOST = os.path.join(HERE, "host")
+HOST_SOURCES = ["Types.cpp", "Dataset.cpp", "Pcm.cpp", "RobustSolver.cpp"]
+CXX = os.environ.get("CXX", "g++")
+ROCM_LIB = "/opt/rocm/lib"
+TESTS_CPP = os.path.join(ROOT, "tests", "cpp")
 
 
 def _compile(src: str) -> str:
@@ -47,6 +55,51 @@ def _compile(src: str) -> str:
     return obj
 
 
+def _host_compile(src: str) -> str:
+    obj = os.path.join(OBJ, "host_" + src + ".o")
+    cmd = [CXX, "-std=c++17", "-O2", "-fPIC", "-Wall", "-I" + os.path.join(ROOT, "include"), "-c",
+           os.path.join(HOST, src), "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"{CXX} failed on {src}")
+    return obj
+
+
+def build_facade(jobs: int = 4) -> str:
+    """libkimera_rpgo.so: links librpgo_hip.so from the same folder ($ORIGIN)."""
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(_host_compile, HOST_SOURCES))
+    out = os.path.join(LIB, "libkimera_rpgo.so")
+    tmp = out + ".tmp"
+    subprocess.check_call([CXX, "-shared", "-o", tmp, *objs, "-L" + LIB, "-lrpgo_hip",
+                           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link," + ROCM_LIB])
+    os.replace(tmp, out)
+    return out
+
+
+def build_tests(jobs: int = 4) -> list:
+    """C++ facade tests: tests/cpp/test_*.cpp -> tests/cpp/bin/<name>."""
+    bindir = os.path.join(TESTS_CPP, "bin")
+    os.makedirs(bindir, exist_ok=True)
+    srcs = sorted(f for f in os.listdir(TESTS_CPP) if f.startswith("test_") and f.endswith(".cpp"))
+
+    def one(src):
+        exe = os.path.join(bindir, src[:-4])
+        cmd = [CXX, "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), "-I" + TESTS_CPP,
+               os.path.join(TESTS_CPP, src), "-o", exe + ".tmp", "-L" + LIB, "-lkimera_rpgo", "-lrpgo_hip",
+               "-Wl,-rpath,$ORIGIN/../../../kimera-rpgo_amd/lib", "-Wl,-rpath-link," + ROCM_LIB]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"{CXX} failed on {src}")
+        os.replace(exe + ".tmp", exe)
+        return exe
+
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        return list(ex.map(one, srcs))
+
+
 def build(jobs: int = 4) -> str:
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(LIB, exist_ok=True)
@@ -57,6 +110,8 @@ def build(jobs: int = 4) -> str:
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     subprocess.check_call(cmd)
     os.replace(tmp, out)
+    build_facade(jobs)
+    build_tests(jobs)
     return out
 
 

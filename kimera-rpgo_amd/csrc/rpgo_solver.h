@@ -58,7 +58,7 @@ struct SolverRun {
   void spmv(hipStream_t st, double lambda, const double* x, double* y);
   int pcg(hipStream_t st, double lambda, int* iters);
   int coarse_setup(hipStream_t st, double lambda, int* Sc, int* nseg);
-  void precond_apply(hipStream_t st, int S, int Sc, int nseg_c, double* rz);
+  void pcg_step(hipStream_t st, int S, int Sc, int nseg_c, int first, int a);
   int model_decrease(hipStream_t st, double* out);
   int total_cost(hipStream_t st, const double* xp, double* out);
   int compute_errors(hipStream_t st, const double* xp);

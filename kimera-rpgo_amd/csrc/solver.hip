@@ -398,6 +398,21 @@ __device__ __forceinline__ double wave_sum_pre(double v) {
   return v;
 }
 
+// block-wide FP64 sum (all threads call it; the result is valid in thread 0)
+__device__ __forceinline__ double block_sum(double v, double* lds16) {
+  v = wave_sum_pre(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) lds16[wid] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x < 64) {
+    t = threadIdx.x < (blockDim.x >> 6) ? lds16[threadIdx.x] : 0.0;
+    t = wave_sum_pre(t);
+  }
+  return t;  // valid in thread 0
+}
+
 // chain[k] = 2 f + flip for the first odometry factor joining k and k+1
 // (flip = 1 when the factor runs k+1 -> k), -1 when none
 __global__ void k_chain_init(int n, int* __restrict__ chain) {
@@ -536,75 +551,6 @@ __global__ void k_seg_factor(int N, int S, const int* __restrict__ chain, const 
       have_prev = false;
     }
   }
-}
-
-// z = M^-1 r by the segment LDL^T (FP32 factors and arithmetic, FP64 vectors),
-// rz += r . z
-__global__ void k_seg_apply(int N, int S, const float* __restrict__ Minv, const float* __restrict__ Useg,
-                            const double* __restrict__ r, double* __restrict__ z, double* __restrict__ rz) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  const int k0 = s * S;
-  double acc = 0.0;
-  if (k0 < N) {
-    const int k1 = min(k0 + S, N);
-    // forward: y_k = r_k - U_{k-1}^T Dinv_{k-1} y_{k-1};  store w_k = Dinv_k y_k in z
-    float y[6], t[6];
-#pragma unroll
-    for (int e = 0; e < 6; ++e) y[e] = (float)r[(size_t)k0 * 6 + e];
-    for (int k = k0; k < k1; ++k) {
-      if (k > k0) {
-        const float* U = Useg + (size_t)(k - 1) * 36;
-        float yn[6];
-#pragma unroll
-        for (int i = 0; i < 6; ++i) {
-          float v = (float)r[(size_t)k * 6 + i];
-#pragma unroll
-          for (int m = 0; m < 6; ++m) v -= U[m * 6 + i] * t[m];
-          yn[i] = v;
-        }
-#pragma unroll
-        for (int e = 0; e < 6; ++e) y[e] = yn[e];
-      }
-      const float* Di = Minv + (size_t)k * 36;
-#pragma unroll
-      for (int i = 0; i < 6; ++i) {
-        float v = 0.f;
-#pragma unroll
-        for (int m = 0; m < 6; ++m) v += Di[i * 6 + m] * y[m];
-        t[i] = v;
-        z[(size_t)k * 6 + i] = v;
-      }
-    }
-    // backward: z_k = w_k - Dinv_k U_k z_{k+1}
-    float zn[6];
-#pragma unroll
-    for (int e = 0; e < 6; ++e) zn[e] = (float)z[(size_t)(k1 - 1) * 6 + e];
-    for (int k = k1 - 2; k >= k0; --k) {
-      const float* U = Useg + (size_t)k * 36;
-      const float* Di = Minv + (size_t)k * 36;
-      float u[6];
-#pragma unroll
-      for (int i = 0; i < 6; ++i) {
-        float v = 0.f;
-#pragma unroll
-        for (int m = 0; m < 6; ++m) v += U[i * 6 + m] * zn[m];
-        u[i] = v;
-      }
-#pragma unroll
-      for (int i = 0; i < 6; ++i) {
-        float v = (float)z[(size_t)k * 6 + i];
-#pragma unroll
-        for (int m = 0; m < 6; ++m) v -= Di[i * 6 + m] * u[m];
-        zn[i] = v;
-        z[(size_t)k * 6 + i] = v;
-      }
-    }
-    for (int k = k0; k < k1; ++k)
-#pragma unroll
-      for (int e = 0; e < 6; ++e) acc += r[(size_t)k * 6 + e] * z[(size_t)k * 6 + e];
-  }
-  acc = wave_sum_pre(acc);
-  if ((threadIdx.x & 63) == 0 && acc != 0.0) atomicAdd(rz, acc);
 }
 
 // ---------------------------------------------------------------------------
@@ -785,75 +731,6 @@ __global__ void k_gram(int n, const double* __restrict__ Li, double* __restrict_
   Ainv[t] = s;
 }
 
-// rc = P^T r : one wave per coarse segment (deterministic, no atomics)
-__global__ void k_coarse_restrict(int N, int Sc, int nseg, const float* __restrict__ Pb,
-                                  const double* __restrict__ r, double* __restrict__ rc) {
-  const int s = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (s >= nseg) return;
-  const int k0 = s * Sc, k1 = min(k0 + Sc, N);
-  double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int k = k0 + lane; k < k1; k += 64) {
-    const float* P = Pb + (size_t)k * 36;
-    double rk[6];
-#pragma unroll
-    for (int m = 0; m < 6; ++m) rk[m] = r[(size_t)k * 6 + m];
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-      double v = 0.0;
-#pragma unroll
-      for (int m = 0; m < 6; ++m) v += (double)P[m * 6 + i] * rk[m];
-      acc[i] += v;
-    }
-  }
-#pragma unroll
-  for (int i = 0; i < 6; ++i) acc[i] = wave_sum_pre(acc[i]);
-  if (lane == 0)
-#pragma unroll
-    for (int i = 0; i < 6; ++i) rc[6 * s + i] = acc[i];
-}
-
-// cc = Ainv rc : 64 rows per workgroup of 256, four row-quarter partials in LDS
-__global__ void __launch_bounds__(256) k_coarse_matvec(int nc, const double* __restrict__ Ainv,
-                                                       const double* __restrict__ rc,
-                                                       double* __restrict__ cc) {
-  __shared__ double part[4][64];
-  const int row = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int q = threadIdx.x >> 6;
-  double s = 0.0;
-  if (row < nc) {
-    const int j0 = (nc * q) / 4, j1 = (nc * (q + 1)) / 4;
-    // Ainv is symmetric: read column `row` (coalesced across the wave)
-    for (int j = j0; j < j1; ++j) s += Ainv[(size_t)j * nc + row] * rc[j];
-  }
-  part[q][threadIdx.x & 63] = s;
-  __syncthreads();
-  if (q == 0 && row < nc)
-    cc[row] = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
-}
-
-// z_k += P_k cc_s ; rz += r_k . (P_k cc_s)
-__global__ void k_coarse_prolong(int N, int Sc, const float* __restrict__ Pb, const double* __restrict__ cc,
-                                 const double* __restrict__ r, double* __restrict__ z,
-                                 double* __restrict__ rz) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  double acc = 0.0;
-  if (k < N) {
-    const float* P = Pb + (size_t)k * 36;
-    const int s = k / Sc;
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-      double v = 0.0;
-#pragma unroll
-      for (int m = 0; m < 6; ++m) v += (double)P[i * 6 + m] * cc[6 * s + m];
-      z[(size_t)k * 6 + i] += v;
-      acc += r[(size_t)k * 6 + i] * v;
-    }
-  }
-  acc = wave_sum_pre(acc);
-  if ((threadIdx.x & 63) == 0 && acc != 0.0) atomicAdd(rz, acc);
-}
-
 // y = (H + lambda damp) x : one thread per pose row (deterministic order)
 __global__ void k_spmv(int N, const int* __restrict__ off, const int* __restrict__ inc,
                        const int* __restrict__ fi, const int* __restrict__ fj,
@@ -913,54 +790,264 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 __global__ void k_dot(long long n, const double* __restrict__ a, const double* __restrict__ b,
                       double* __restrict__ out) {
+  __shared__ double red[16];
   double s = 0.0;
   for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
        k += (long long)gridDim.x * blockDim.x)
     s += a[k] * b[k];
-  s = wave_sum(s);
-  if ((threadIdx.x & 63) == 0) atomicAdd(out, s);
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) atomicAdd(out, s);
 }
 
-// scalars: sc[0] = rz, sc[1] = pq, sc[2] = rz_new, sc[3] = |r|^2
-// x += alpha p ; r -= alpha q ; alpha = rz / pq ; accumulates |r|^2 into sc[3]
-__global__ void k_cg_update_xr(long long n, const double* __restrict__ sc, const double* __restrict__ p,
-                               const double* __restrict__ q, double* __restrict__ x,
-                               double* __restrict__ r, double* __restrict__ rr) {
-  const double pq = sc[1];
-  const double alpha = pq != 0.0 ? sc[0] / pq : 0.0;
+// ---------------------------------------------------------------------------
+// Fused PCG iteration (4 launches, 3 without the coarse level).  Scalars live
+// in parity slots so that no launch has to reset what a concurrent block of
+// the same launch still reads: for iteration g with a = g & 1,
+//   R[a] = r_g . z_g, R[a^1] accumulates r_{g+1} . z_{g+1}; PQ[a] = p_g . A p_g;
+//   cc[a] accumulates the coarse correction; hist[cnt] = |r_{g+1}|^2.
+// Each launch clears only slots that no block of the same launch touches.
+// ---------------------------------------------------------------------------
+constexpr int kScR = 0, kScPQ = 2, kScCnt = 32, kScLam = 30;
+
+// K1: q = (H + lambda damp) p ; PQ[a] += p . q ; clear R[a^1], PQ[a^1]
+__global__ void __launch_bounds__(128) k_pcg_spmv(int N, int a, const int* __restrict__ off,
+                                                 const int* __restrict__ inc, const int* __restrict__ fi,
+                                                 const int* __restrict__ fj, const int* __restrict__ kind,
+                                                 const double* __restrict__ D, const double* __restrict__ Hij,
+                                                 const double* __restrict__ hdiag, int diag_damping,
+                                                 double* __restrict__ sc, const double* __restrict__ x,
+                                                 double* __restrict__ y) {
+  __shared__ double red[16];
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const double lambda = sc[kScLam];
+  double pq = 0.0;
+  if (p < N) {
+    double xp[6], acc[6];
+#pragma unroll
+    for (int e = 0; e < 6; ++e) xp[e] = x[(size_t)p * 6 + e];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+      double s = 0.0;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) s += D[(size_t)p * 36 + r * 6 + c] * xp[c];
+      const double d = diag_damping ? fmin(fmax(hdiag[(size_t)p * 6 + r], 1e-6), 1e32) : 1.0;
+      acc[r] = s + lambda * d * xp[r];
+    }
+    for (int k = off[p]; k < off[p + 1]; ++k) {
+      const int t = inc[k], f = t >> 1;
+      if (kind[f] != 0) continue;
+      const double* B = Hij + (size_t)f * 36;
+      if ((t & 1) == 0) {
+        const double* xq = x + (size_t)fj[f] * 6;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+          double s = 0.0;
+#pragma unroll
+          for (int c = 0; c < 6; ++c) s += B[r * 6 + c] * xq[c];
+          acc[r] += s;
+        }
+      } else {
+        const double* xq = x + (size_t)fi[f] * 6;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+          double s = 0.0;
+#pragma unroll
+          for (int c = 0; c < 6; ++c) s += B[c * 6 + r] * xq[c];
+          acc[r] += s;
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+      y[(size_t)p * 6 + r] = acc[r];
+      pq += xp[r] * acc[r];
+    }
+  }
+  pq = block_sum(pq, red);
+  if (threadIdx.x == 0) {
+    atomicAdd(&sc[kScPQ + a], pq);
+    if (blockIdx.x == 0) {
+      sc[kScR + (a ^ 1)] = 0.0;
+      sc[kScPQ + (a ^ 1)] = 0.0;
+    }
+  }
+}
+
+// Segment LDL^T solve z = B^-1 r over poses [k0, k1) (FP32 factors and
+// arithmetic, FP64 vectors); returns r . z over the segment.
+__device__ double seg_solve(int k0, int k1, const float* __restrict__ Minv, const float* __restrict__ Useg,
+                            const double* __restrict__ r, double* __restrict__ z) {
+  float y[6], t[6];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) y[e] = (float)r[(size_t)k0 * 6 + e];
+  for (int k = k0; k < k1; ++k) {
+    if (k > k0) {
+      const float* U = Useg + (size_t)(k - 1) * 36;
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        float v = (float)r[(size_t)k * 6 + i];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) v -= U[m * 6 + i] * t[m];
+        y[i] = v;
+      }
+    }
+    const float* Di = Minv + (size_t)k * 36;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      float v = 0.f;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) v += Di[i * 6 + m] * y[m];
+      t[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) z[(size_t)k * 6 + i] = t[i];
+  }
+  float zn[6];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) zn[e] = t[e];
+  for (int k = k1 - 2; k >= k0; --k) {
+    const float* U = Useg + (size_t)k * 36;
+    const float* Di = Minv + (size_t)k * 36;
+    float u[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      float v = 0.f;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) v += U[i * 6 + m] * zn[m];
+      u[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      float v = (float)z[(size_t)k * 6 + i];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) v -= Di[i * 6 + m] * u[m];
+      zn[i] = v;
+      z[(size_t)k * 6 + i] = v;
+    }
+  }
+  double acc = 0.0;
+  for (int k = k0; k < k1; ++k)
+#pragma unroll
+    for (int e = 0; e < 6; ++e) acc += r[(size_t)k * 6 + e] * z[(size_t)k * 6 + e];
+  return acc;
+}
+
+// K2: alpha = R[a] / PQ[a]; x += alpha p; r -= alpha q (skipped when first);
+// z = B_fine^-1 r; R[a^1] += r . z; rc += P^T r (coarse segments are whole
+// multiples of the fine segment S); hist[cnt] += |r|^2.  One thread per fine
+// segment.
+__global__ void __launch_bounds__(128) k_pcg_update(int N, int S, int first, int a,
+                                                   double* __restrict__ sc, const double* __restrict__ p,
+                                                   const double* __restrict__ q, double* __restrict__ x,
+                                                   double* __restrict__ r, double* __restrict__ z,
+                                                   const float* __restrict__ Minv,
+                                                   const float* __restrict__ Useg, int Sc,
+                                                   const float* __restrict__ Pb, double* __restrict__ rc,
+                                                   double* __restrict__ hist) {
+  __shared__ double red[16];
+  const int seg = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k0 = seg * S;
+  double rr = 0.0, rz = 0.0;
+  if (k0 < N) {
+    const int k1 = min(k0 + S, N);
+    if (!first) {
+      const double pq = sc[kScPQ + a];
+      const double alpha = pq != 0.0 ? sc[kScR + a] / pq : 0.0;
+      for (int k = k0; k < k1; ++k)
+#pragma unroll
+        for (int e = 0; e < 6; ++e) {
+          const size_t o = (size_t)k * 6 + e;
+          x[o] += alpha * p[o];
+          const double rv = r[o] - alpha * q[o];
+          r[o] = rv;
+          rr += rv * rv;
+        }
+    }
+    rz = seg_solve(k0, k1, Minv, Useg, r, z);
+    if (Pb) {
+      double acc[6] = {0, 0, 0, 0, 0, 0};
+      for (int k = k0; k < k1; ++k) {
+        const float* P = Pb + (size_t)k * 36;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+          double v = 0.0;
+#pragma unroll
+          for (int m = 0; m < 6; ++m) v += (double)P[m * 6 + i] * r[(size_t)k * 6 + m];
+          acc[i] += v;
+        }
+      }
+      const int cs = k0 / Sc;
+#pragma unroll
+      for (int i = 0; i < 6; ++i) atomicAdd(&rc[6 * cs + i], acc[i]);
+    }
+  }
+  rr = block_sum(rr, red);
+  rz = block_sum(rz, red);
+  if (threadIdx.x == 0) {
+    if (!first) atomicAdd(&hist[(int)sc[kScCnt]], rr);
+    atomicAdd(&sc[kScR + (a ^ 1)], rz);
+  }
+}
+
+// K3: cc[a] += Ainv rc over a (64-row, 64-column) tile per workgroup;
+// R[a^1] += rc . (Ainv rc) (the coarse part of r . z, since r . P c = P^T r . c).
+__global__ void __launch_bounds__(256) k_pcg_coarse(int nc, int a, const double* __restrict__ Ainv,
+                                                   const double* __restrict__ rc, double* __restrict__ cc,
+                                                   double* __restrict__ sc) {
+  __shared__ double part[4][64];
+  __shared__ double red[16];
+  const int row = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int qd = threadIdx.x >> 6;
+  const int j0 = blockIdx.y * 64 + qd * 16, j1 = min(j0 + 16, nc);
   double s = 0.0;
-  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
-       k += (long long)gridDim.x * blockDim.x) {
-    x[k] += alpha * p[k];
-    const double rv = r[k] - alpha * q[k];
-    r[k] = rv;
-    s += rv * rv;
+  if (row < nc)
+    for (int j = j0; j < j1; ++j) s += Ainv[(size_t)j * nc + row] * rc[j];  // Ainv symmetric
+  part[qd][threadIdx.x & 63] = s;
+  __syncthreads();
+  double contrib = 0.0;
+  if (qd == 0 && row < nc) {
+    const double v = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
+    atomicAdd(&cc[row], v);
+    contrib = rc[row] * v;
   }
-  s = wave_sum(s);
-  if ((threadIdx.x & 63) == 0) atomicAdd(rr, s);
+  contrib = block_sum(contrib, red);
+  if (threadIdx.x == 0) atomicAdd(&sc[kScR + (a ^ 1)], contrib);
 }
 
-// p = z + beta p, beta = rz_new / rz
-__global__ void k_cg_update_p(long long n, const double* __restrict__ sc, const double* __restrict__ z,
-                              double* __restrict__ p) {
-  const double rz = sc[0];
-  const double beta = rz != 0.0 ? sc[2] / rz : 0.0;
-  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
-       k += (long long)gridDim.x * blockDim.x)
-    p[k] = z[k] + beta * p[k];
-}
-
-// rz <- rz_new, clear pq / rz_new / rr, record |r|^2 of this iteration
-__global__ void k_cg_roll(double* sc, double* hist) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    const int it = (int)sc[32];
-    hist[it] = sc[3];
-    sc[32] = (double)(it + 1);
-    sc[0] = sc[2];
-    sc[1] = 0.0;
-    sc[2] = 0.0;
-    sc[3] = 0.0;
+// K4: beta = R[a^1] / R[a] (0 when first); z += P cc[a]; p = z + beta p;
+// clear rc and cc[a^1]; advance the iteration counter.
+__global__ void __launch_bounds__(128) k_pcg_direction(int N, int first, int a, double* __restrict__ sc,
+                                                      double* __restrict__ z, double* __restrict__ p,
+                                                      int Sc, const float* __restrict__ Pb,
+                                                      const double* __restrict__ cc_a,
+                                                      double* __restrict__ cc_other,
+                                                      double* __restrict__ rc, int nc) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const double rz = sc[kScR + a];
+  const double beta = (!first && rz != 0.0) ? sc[kScR + (a ^ 1)] / rz : 0.0;
+  if (k < N) {
+    double c[6] = {0, 0, 0, 0, 0, 0};
+    const float* P = Pb ? Pb + (size_t)k * 36 : nullptr;
+    if (P) {
+      const int s = k / Sc;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) c[m] = cc_a[6 * s + m];
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      double v = z[(size_t)k * 6 + i];
+      if (P) {
+#pragma unroll
+        for (int m = 0; m < 6; ++m) v += (double)P[i * 6 + m] * c[m];
+      }
+      p[(size_t)k * 6 + i] = first ? v : v + beta * p[(size_t)k * 6 + i];
+    }
   }
+  if (k < nc) {
+    rc[k] = 0.0;
+    cc_other[k] = 0.0;
+  }
+  if (k == 0 && !first) sc[kScCnt] += 1.0;
 }
 
 __global__ void k_copy_d(long long n, const double* __restrict__ a, double* __restrict__ b) {
@@ -989,8 +1076,9 @@ __global__ void k_sum(long long n, const double* __restrict__ a, const double* _
   for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
        k += (long long)gridDim.x * blockDim.x)
     s += w ? a[k] * w[k] : a[k];
-  s = wave_sum(s);
-  if ((threadIdx.x & 63) == 0) atomicAdd(out, s);
+  __shared__ double red[16];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) atomicAdd(out, s);
 }
 
 // b . dx in FP64 (b kept in FP64)
@@ -1051,6 +1139,10 @@ int SolverRun::alloc(int N_, int F_) {
   N = N_;
   F = F_;
   int rc;
+  // the captured PCG chunk holds buffer addresses: capture it again per solve
+  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+  graph_exec = nullptr;
+  graph_key = GraphKey{};
   const size_t n6 = (size_t)N * 6;
   if ((rc = x0.reserve(sizeof(double) * 12 * N))) return rc;
   if ((rc = x.reserve(sizeof(double) * 12 * N))) return rc;
@@ -1175,6 +1267,7 @@ int SolverRun::coarse_setup(hipStream_t st, double lambda, int* Sc_out, int* nse
   const int segs_max = prm.pcg_coarse > 0 ? prm.pcg_coarse : kCoarseSegMax;
   const int Sfine = prm.pcg_segment > 0 ? prm.pcg_segment : 128;
   int Sc = std::max(Sfine, (N + segs_max - 1) / segs_max);
+  Sc = (Sc + Sfine - 1) / Sfine * Sfine;  // whole fine segments per coarse segment
   const int nseg = (N + Sc - 1) / Sc;
   if (nseg < 2) return RPGO_OK;  // a single rigid segment is the gauge mode only
   const int nc = 6 * nseg;
@@ -1183,8 +1276,6 @@ int SolverRun::coarse_setup(hipStream_t st, double lambda, int* Sc_out, int* nse
   if ((rc_ = Ac.reserve(sizeof(double) * (size_t)nc * nc))) return rc_;
   if ((rc_ = Lci.reserve(sizeof(double) * (size_t)nc * nc))) return rc_;
   if ((rc_ = Ainv.reserve(sizeof(double) * (size_t)nc * nc))) return rc_;
-  if ((rc_ = cres.reserve(sizeof(double) * nc))) return rc_;
-  if ((rc_ = ccor.reserve(sizeof(double) * nc))) return rc_;
   k_coarse_basis<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(N, Sc, x.as<double>(), Pb.as<float>());
   RPGO_HIP_CHECK(hipMemsetAsync(Ac.ptr, 0, sizeof(double) * (size_t)nc * nc, st));
   k_coarse_diag<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(N, Sc, nc, Pb.as<float>(), D.as<double>(),
@@ -1209,19 +1300,30 @@ int SolverRun::coarse_setup(hipStream_t st, double lambda, int* Sc_out, int* nse
   return RPGO_OK;
 }
 
-// z = M^-1 r (fine segment LDL^T plus the coarse correction), rz += r . z
-void SolverRun::precond_apply(hipStream_t st, int S, int Sc, int nseg_c, double* rz) {
+// One PCG iteration (a = iteration parity), or with first = 1 the set-up
+// z0 = M^-1 r0, p0 = z0 written into the slots of iteration 0.
+void SolverRun::pcg_step(hipStream_t st, int S, int Sc, int nseg_c, int first, int a) {
+  double* s = sc.as<double>();
   const int nseg = (N + S - 1) / S;
-  k_seg_apply<<<(nseg + 63) / 64, 64, 0, st>>>(N, S, Minv.as<float>(), Useg.as<float>(), r.as<double>(),
-                                               zv.as<double>(), rz);
-  if (nseg_c > 0) {
-    const int nc = 6 * nseg_c;
-    k_coarse_restrict<<<(nseg_c + 3) / 4, 256, 0, st>>>(N, Sc, nseg_c, Pb.as<float>(), r.as<double>(),
-                                                        cres.as<double>());
-    k_coarse_matvec<<<(nc + 63) / 64, 256, 0, st>>>(nc, Ainv.as<double>(), cres.as<double>(), ccor.as<double>());
-    k_coarse_prolong<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(N, Sc, Pb.as<float>(), ccor.as<double>(),
-                                                               r.as<double>(), zv.as<double>(), rz);
+  const int nc = 6 * nseg_c;
+  const float* P = nseg_c ? Pb.as<float>() : nullptr;
+  double* cc0 = ccor.as<double>();
+  double* cc1 = cc0 + kCoarseMax;
+  if (!first)
+    k_pcg_spmv<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(
+        N, a, inc_off.as<int>(), inc.as<int>(), fi.as<int>(), fj.as<int>(), kind.as<int>(),
+        D.as<double>(), Hij.as<double>(), hdiag.as<double>(), prm.lm_diagonal_damping, s, p.as<double>(),
+        q.as<double>());
+  k_pcg_update<<<grid_for(nseg, 128, 1 << 22), 128, 0, st>>>(
+      N, S, first, a, s, p.as<double>(), q.as<double>(), dx.as<double>(), r.as<double>(), zv.as<double>(),
+      Minv.as<float>(), Useg.as<float>(), Sc, P, cres.as<double>(), hist.as<double>());
+  if (nseg_c) {
+    const dim3 grid((nc + 63) / 64, (nc + 63) / 64);
+    k_pcg_coarse<<<grid, 256, 0, st>>>(nc, a, Ainv.as<double>(), cres.as<double>(), a ? cc1 : cc0, s);
   }
+  k_pcg_direction<<<grid_for(std::max(N, nc), 128, 1 << 22), 128, 0, st>>>(
+      N, first, a, s, zv.as<double>(), p.as<double>(), Sc, P, a ? cc1 : cc0, a ? cc0 : cc1,
+      cres.as<double>(), nc);
 }
 
 // Solve (H + lambda damp) dx = b by PCG (x0 = 0) preconditioned by the
@@ -1236,12 +1338,20 @@ int SolverRun::pcg(hipStream_t st, double lambda, int* iters_out) {
                                                 Minv.as<float>(), Useg.as<float>());
   int Sc = 0, nseg_c = 0, rc_;
   if ((rc_ = coarse_setup(st, lambda, &Sc, &nseg_c))) return rc_;
+  const int max_it = prm.pcg_max_iterations > 0 ? prm.pcg_max_iterations : std::min(6 * N, 4000);
+  if ((rc_ = hist.reserve(sizeof(double) * (size_t)(max_it + kPcgChunk)))) return rc_;
+  if ((rc_ = ccor.reserve(sizeof(double) * 2 * kCoarseMax))) return rc_;
+  if ((rc_ = cres.reserve(sizeof(double) * kCoarseMax))) return rc_;
+  RPGO_HIP_CHECK(hipMemsetAsync(hist.ptr, 0, sizeof(double) * (size_t)(max_it + kPcgChunk), st));
+  RPGO_HIP_CHECK(hipMemsetAsync(ccor.ptr, 0, sizeof(double) * 2 * kCoarseMax, st));
+  RPGO_HIP_CHECK(hipMemsetAsync(cres.ptr, 0, sizeof(double) * kCoarseMax, st));
   RPGO_HIP_CHECK(hipMemsetAsync(dx.ptr, 0, sizeof(double) * n, st));
-  k_copy_d<<<grid_for(n), 256, 0, st>>>(n, b64.as<double>(), r.as<double>());
   RPGO_HIP_CHECK(hipMemsetAsync(s, 0, sizeof(double) * 8, st));
-  precond_apply(st, S, Sc, nseg_c, s + 0);
-  k_copy_d<<<grid_for(n), 256, 0, st>>>(n, zv.as<double>(), p.as<double>());
-  k_dot<<<grid_for(n), 256, 0, st>>>(n, b64.as<double>(), b64.as<double>(), s + 4);
+  RPGO_HIP_CHECK(hipMemsetAsync(s + kScCnt, 0, sizeof(double), st));
+  RPGO_HIP_CHECK(hipMemcpyAsync(s + kScLam, &lambda, sizeof(double), hipMemcpyHostToDevice, st));
+  k_copy_d<<<grid_for(n), 256, 0, st>>>(n, b64.as<double>(), r.as<double>());
+  k_dot<<<grid_for(n, 256, 512), 256, 0, st>>>(n, b64.as<double>(), b64.as<double>(), s + 4);
+  pcg_step(st, S, Sc, nseg_c, 1, 1);  // z0, p0; r0 . z0 lands in R[0]
   RPGO_HIP_CHECK(hipGetLastError());
   double h[8];
   int rc = read_scalars(st, 8, h);
@@ -1249,27 +1359,12 @@ int SolverRun::pcg(hipStream_t st, double lambda, int* iters_out) {
   const double bnorm2 = h[4];
   const double tol2 = prm.pcg_rel_tol * prm.pcg_rel_tol * bnorm2;
   int it = 0;
-  const int max_it = prm.pcg_max_iterations > 0 ? prm.pcg_max_iterations : std::min(6 * N, 4000);
   if (bnorm2 == 0.0) {
     *iters_out = 0;
     return RPGO_OK;
   }
-  // lambda for the SpMV inside the iteration body lives in device memory so the
-  // captured body is independent of it; the iteration counter s[32] indexes the
-  // residual history written by k_cg_roll.
-  const double lam_zero[3] = {lambda, 0.0, 0.0};
-  RPGO_HIP_CHECK(hipMemcpyAsync(s + 30, lam_zero, sizeof(double) * 3, hipMemcpyHostToDevice, st));
-  if ((rc = hist.reserve(sizeof(double) * (size_t)(max_it + kPcgChunk)))) return rc;
   auto body = [&]() {
-    for (int j = 0; j < kPcgChunk; ++j) {
-      spmv_dev(st, s + 30, p.as<double>(), q.as<double>());
-      k_dot<<<grid_for(n), 256, 0, st>>>(n, p.as<double>(), q.as<double>(), s + 1);
-      k_cg_update_xr<<<grid_for(n), 256, 0, st>>>(n, s, p.as<double>(), q.as<double>(), dx.as<double>(),
-                                                  r.as<double>(), s + 3);
-      precond_apply(st, S, Sc, nseg_c, s + 2);
-      k_cg_update_p<<<grid_for(n), 256, 0, st>>>(n, s, zv.as<double>(), p.as<double>());
-      k_cg_roll<<<1, 1, 0, st>>>(s, hist.as<double>());
-    }
+    for (int j = 0; j < kPcgChunk; ++j) pcg_step(st, S, Sc, nseg_c, 0, j & 1);
   };
   // one captured chunk of kPcgChunk iterations, rebuilt when sizes or buffers change
   const GraphKey key{N,
@@ -1283,7 +1378,8 @@ int SolverRun::pcg(hipStream_t st, double lambda, int* iters_out) {
                      (const void*)inc.ptr,
                      nseg_c ? (const void*)Ainv.ptr : nullptr,
                      nseg_c ? (const void*)Pb.ptr : nullptr};
-  if (use_graphs && !(graph_exec && graph_key == key)) {
+  const bool graphs = use_graphs && prm.pcg_graphs != 0;
+  if (graphs && !(graph_exec && graph_key == key)) {
     if (graph_exec) {
       (void)hipGraphExecDestroy(graph_exec);
       graph_exec = nullptr;
@@ -1305,7 +1401,7 @@ int SolverRun::pcg(hipStream_t st, double lambda, int* iters_out) {
   int best_it = 0;
   bool done = false;
   while (it < max_it && !done) {
-    if (use_graphs && graph_exec) RPGO_HIP_CHECK(hipGraphLaunch(graph_exec, st));
+    if (graphs && graph_exec) RPGO_HIP_CHECK(hipGraphLaunch(graph_exec, st));
     else body();
     RPGO_HIP_CHECK(hipGetLastError());
     double hh[kPcgChunk];
@@ -1327,7 +1423,7 @@ int SolverRun::pcg(hipStream_t st, double lambda, int* iters_out) {
       }
     }
     it += kPcgChunk;
-    // stagnation: FP32 recurrences stop reducing the residual
+    // stagnation: the residual stopped decreasing
     if (it - best_it >= 32 * kPcgChunk) done = true;
   }
   stats.pcg_iterations += it;

@@ -90,11 +90,13 @@ void rpgo_solve_params_default(rpgo_solve_params* p) {
   p->lm_relative_error_tol = 1e-5;
   p->lm_absolute_error_tol = 1e-5;
   p->pcg_max_iterations = 0;
-  p->pcg_rel_tol = 1e-4;  // inexact Newton: the FP64-residual outer loop refines
+  p->pcg_rel_tol = 1e-8;  // near-exact steps: LM follows the direct-solve trajectory
   p->polish_iterations = 0;
-  p->pcg_segment = 128;
+  p->pcg_segment = 4;
   p->gnc_warm_start = 0;
   p->pcg_coarse = 0;
+  p->pcg_graphs = 1;
+  p->gnc_factor_dim = 6;
 }
 
 int rpgo_solve(rpgo_ctx* ctx, const rpgo_solve_problem* pr, const rpgo_solve_params* prm,
@@ -151,7 +153,8 @@ int rpgo_solve(rpgo_ctx* ctx, const rpgo_solve_problem* pr, const rpgo_solve_par
   std::vector<double> w0(F), barc(F);
   std::vector<unsigned char> known(F);
   const double thr = prm->gnc_threshold_mode == RPGO_GNC_PROBABILITY
-                         ? 0.5 * chi2_quantile(prm->gnc_inlier_threshold, 6.0)
+                         ? 0.5 * chi2_quantile(prm->gnc_inlier_threshold,
+                                               prm->gnc_factor_dim > 0 ? prm->gnc_factor_dim : 6.0)
                          : prm->gnc_inlier_threshold;
   for (int f = 0; f < F; ++f) {
     kj[f] = pr->kind[f] == RPGO_FACTOR_BETWEEN ? pr->key_j[f] : pr->key_i[f];
