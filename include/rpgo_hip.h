@@ -217,6 +217,43 @@ void rpgo_solve_params_default(rpgo_solve_params* p);
 int rpgo_solve(rpgo_ctx* ctx, const rpgo_solve_problem* prob, const rpgo_solve_params* params,
                double* poses_out, double* weights_out, rpgo_solve_report* report);
 
+/* ------------------------------------------------------------------------
+ * Learned GNC initialisation.  No counterpart in the reference: an addition
+ * of this back end that predicts, per loop closure, the GNC weight the plain
+ * method converges to (as a logit) and log(mu) of the final GNC iteration,
+ * from features of the PCM state.  Plain GNC stays the judge: callers start
+ * rpgo_solve from the prediction (init_weights, init_mu) and fall back to the
+ * plain solve when the result fails their check.
+ * MLP 16 -> 128 -> 128 -> 2 with ReLU; BF16 matrix-core products with FP32
+ * accumulation; Adam on FP32 master weights.  Parameter layout (FP32, this
+ * order): W1[16][128], b1[128], W2[128][128], b2[128], W3[128][2], b3[2].
+ * ---------------------------------------------------------------------- */
+#define RPGO_LC_FEATURES 16
+#define RPGO_MLP_HIDDEN 128
+#define RPGO_MLP_OUT 2
+#define RPGO_MLP_NPARAMS (16 * 128 + 128 + 128 * 128 + 128 + 128 * 2 + 2)
+
+/* Features of the accepted loop closures of the last rpgo_pcm_run (SE(3)),
+ * in the compacted (accepted) order: feat m * RPGO_LC_FEATURES floats.
+ * clique_idx: the k clique members (indices into the accepted set). */
+int rpgo_lc_features(rpgo_ctx* ctx, const int32_t* clique_idx, int32_t k, float* feat);
+/* Load parameters and the feature normalisation (mean[16] then 1/std[16]);
+ * resets the Adam state. */
+int rpgo_mlp_set_params(rpgo_ctx* ctx, const float* params, const float* norm);
+int rpgo_mlp_get_params(rpgo_ctx* ctx, float* params);
+/* y[n][2] = MLP(x[n][16]). */
+int rpgo_mlp_forward(rpgo_ctx* ctx, int32_t n, const float* x, float* y);
+/* Gradient of the mean loss over the n rows (BCE on the weight logit against
+ * target[:,0], mu_weight/2 (y1 - target[:,1])^2 on log mu, each scaled by
+ * sample_w, which may be NULL) with respect to the parameters. */
+int rpgo_mlp_gradients(rpgo_ctx* ctx, int32_t n, const float* x, const float* target,
+                       const float* sample_w, float mu_weight, float* grad, double* loss);
+/* Adam training on the device-resident set: `epochs` passes over shuffled
+ * full batches of `batch` rows; loss_hist[e] = mean batch loss of epoch e. */
+int rpgo_mlp_train(rpgo_ctx* ctx, int32_t n, const float* x, const float* target,
+                   const float* sample_w, float mu_weight, int32_t batch, int32_t epochs,
+                   float lr, uint64_t seed, double* loss_hist);
+
 #ifdef __cplusplus
 }
 #endif

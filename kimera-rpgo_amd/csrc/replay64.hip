@@ -105,15 +105,12 @@ __global__ void k_odom_check(int n_lc, const double* __restrict__ traj,
   constexpr int PWC = R64<DIM>::PWC;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n_lc) return;
-  double d = 0.0;
-  int ok = 1;
-  if (thr >= 0.0) {
-    d = R64<DIM>::odom_dist(traj + (size_t)from[k] * PWC, traj + (size_t)to[k] * PWC,
-                            lcm + (size_t)k * PWC);
-    ok = d < thr ? 1 : 0;
-  }
+  // the distance is computed also when the check is off (thr < 0: every loop
+  // closure passes, Pcm.h:606), as a per-loop-closure feature
+  const double d = R64<DIM>::odom_dist(traj + (size_t)from[k] * PWC, traj + (size_t)to[k] * PWC,
+                                       lcm + (size_t)k * PWC);
   dist[k] = d;
-  flag[k] = ok;
+  flag[k] = (thr < 0.0 || d < thr) ? 1 : 0;
 }
 
 // Stable compaction of accepted loop closures (single workgroup scan).

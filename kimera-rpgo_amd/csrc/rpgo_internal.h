@@ -92,6 +92,7 @@ struct CliqueState {
 };
 
 struct SolverRun;  // solver.hip / rpgo_solver.h
+struct MlpState;   // learned.hip
 
 }  // namespace rpgo
 
@@ -102,6 +103,7 @@ struct rpgo_ctx {
   rpgo::PcmState pcm;
   rpgo::CliqueState clq;
   rpgo::SolverRun* solver = nullptr;  // created on the first rpgo_solve
+  rpgo::MlpState* mlp = nullptr;      // created by rpgo_mlp_set_params
 };
 
 namespace rpgo {
@@ -126,6 +128,7 @@ constexpr int kRecheckQueueCap = 1 << 22;
 //   i32[32] "some covariance has no rotation information" flag
 // solver_capi.hip
 void solver_destroy(rpgo_ctx* ctx);
+void mlp_destroy(rpgo_ctx* ctx);  // learned.hip
 // clique.hip
 int clique_run(rpgo_ctx* ctx, const uint64_t* d_adj, int n, int mode, int32_t* idx_out,
                int32_t* k_out);

@@ -23,6 +23,7 @@ CLIQUE_EXACT = 1
 _d = ctypes.POINTER(ctypes.c_double)
 _i32 = ctypes.POINTER(ctypes.c_int32)
 _u64 = ctypes.POINTER(ctypes.c_uint64)
+_f = ctypes.POINTER(ctypes.c_float)
 
 
 class PcmProblem(ctypes.Structure):
@@ -111,7 +112,20 @@ SIGNATURES = [
     ("rpgo_max_clique", [ctypes.c_void_p, ctypes.c_int32, _i32, _i32, _d], ctypes.c_int),
     ("rpgo_max_clique_bits", [ctypes.c_void_p, ctypes.c_int32, _u64, ctypes.c_int32, _i32, _i32],
      ctypes.c_int),
+    ("rpgo_lc_features", [ctypes.c_void_p, _i32, ctypes.c_int32, _f], ctypes.c_int),
+    ("rpgo_mlp_set_params", [ctypes.c_void_p, _f, _f], ctypes.c_int),
+    ("rpgo_mlp_get_params", [ctypes.c_void_p, _f], ctypes.c_int),
+    ("rpgo_mlp_forward", [ctypes.c_void_p, ctypes.c_int32, _f, _f], ctypes.c_int),
+    ("rpgo_mlp_gradients", [ctypes.c_void_p, ctypes.c_int32, _f, _f, _f, ctypes.c_float, _f, _d],
+     ctypes.c_int),
+    ("rpgo_mlp_train", [ctypes.c_void_p, ctypes.c_int32, _f, _f, _f, ctypes.c_float, ctypes.c_int32,
+                        ctypes.c_int32, ctypes.c_float, ctypes.c_uint64, _d], ctypes.c_int),
 ]
+
+LC_FEATURES = 16
+MLP_HIDDEN = 128
+MLP_OUT = 2
+MLP_NPARAMS = 16 * 128 + 128 + 128 * 128 + 128 + 128 * 2 + 2
 
 _lib = None
 
@@ -264,6 +278,51 @@ class Context:
         check(lib().rpgo_solve(self._h, ctypes.byref(prob), ctypes.byref(prm), _ptr(out), _ptr(w),
                                ctypes.byref(rep)))
         return out, w[:F].copy(), rep
+
+    # ------------------------------------------------------ learned initialisation
+    def lc_features(self, clique_idx):
+        """(m, 16) float32 features of the accepted loop closures of the last PCM run."""
+        ci = np.ascontiguousarray(clique_idx, dtype=np.int32)
+        m = self.pcm_download()["accepted"].shape[0]
+        out = np.zeros((max(m, 1), LC_FEATURES), np.float32)
+        check(lib().rpgo_lc_features(self._h, _ptr(ci, _i32), ci.shape[0], _ptr(out, _f)))
+        return out[:m]
+
+    def mlp_set_params(self, params, norm):
+        p = np.ascontiguousarray(params, dtype=np.float32).reshape(MLP_NPARAMS)
+        n = np.ascontiguousarray(norm, dtype=np.float32).reshape(2 * LC_FEATURES)
+        check(lib().rpgo_mlp_set_params(self._h, _ptr(p, _f), _ptr(n, _f)))
+
+    def mlp_get_params(self):
+        out = np.zeros(MLP_NPARAMS, np.float32)
+        check(lib().rpgo_mlp_get_params(self._h, _ptr(out, _f)))
+        return out
+
+    def mlp_forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, LC_FEATURES)
+        y = np.zeros((x.shape[0], MLP_OUT), np.float32)
+        check(lib().rpgo_mlp_forward(self._h, x.shape[0], _ptr(x, _f), _ptr(y, _f)))
+        return y
+
+    def mlp_gradients(self, x, target, sample_w=None, mu_weight=1.0):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, LC_FEATURES)
+        t = np.ascontiguousarray(target, dtype=np.float32).reshape(-1, MLP_OUT)
+        sw = None if sample_w is None else np.ascontiguousarray(sample_w, dtype=np.float32)
+        g = np.zeros(MLP_NPARAMS, np.float32)
+        loss = ctypes.c_double()
+        check(lib().rpgo_mlp_gradients(self._h, x.shape[0], _ptr(x, _f), _ptr(t, _f), _ptr(sw, _f),
+                                       float(mu_weight), _ptr(g, _f), ctypes.byref(loss)))
+        return g, loss.value
+
+    def mlp_train(self, x, target, sample_w=None, mu_weight=1.0, batch=1024, epochs=10, lr=1e-3, seed=0):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, LC_FEATURES)
+        t = np.ascontiguousarray(target, dtype=np.float32).reshape(-1, MLP_OUT)
+        sw = None if sample_w is None else np.ascontiguousarray(sample_w, dtype=np.float32)
+        hist = np.zeros(max(epochs, 1))
+        check(lib().rpgo_mlp_train(self._h, x.shape[0], _ptr(x, _f), _ptr(t, _f), _ptr(sw, _f),
+                                   float(mu_weight), int(batch), int(epochs), float(lr), int(seed),
+                                   _ptr(hist)))
+        return hist[:epochs]
 
     def max_clique_bits(self, adj_bits, n, mode=CLIQUE_HEU_REF):
         a = np.ascontiguousarray(adj_bits, dtype=np.uint64).reshape(-1)
