@@ -208,7 +208,10 @@ typedef struct {
   double ms_total;              /* device-stream time of the solve     */
   int32_t n_inliers;            /* truncated weight sum minus known inliers
                                    (RobustSolver.cpp:170-171)           */
-  int32_t reserved;
+  int32_t n_tls_disagree;       /* GNC: factors whose final weight differs from
+                                   the TLS decision at the final poses (0 at a
+                                   GNC fixed point; the check a learned start
+                                   must pass)                           */
 } rpgo_solve_report;
 
 /* Fill *p with the reference defaults (SolverParams.h:60-94, GTSAM LM). */

@@ -15,6 +15,7 @@
 // replayed by one wavefront that materialises the reference's stale buffer.
 #include "rpgo_internal.h"
 
+#include <algorithm>
 #include <climits>
 #include <vector>
 
@@ -49,6 +50,8 @@ __global__ void k_dmask(int n, int W, int best, const int* __restrict__ deg,
 
 // words per lane held in registers (n <= 64 * 64 * kMaxWPL)
 constexpr int kMaxWPL = 8;
+// search nodes the exact mode may visit before it reports failure
+constexpr long long kExactNodeBudget = 400000000ll;
 
 // Highest set bit over the wave's distributed bitset; -1 when empty.
 __device__ __forceinline__ int wave_top_bit(const unsigned long long* S, int lane, int W) {
@@ -196,12 +199,12 @@ __global__ void __launch_bounds__(64) k_replay(int n, int W, int c, int best,
 int clique_run(rpgo_ctx* ctx, const uint64_t* d_adj_u64, int n, int mode, int32_t* idx_out,
                int32_t* k_out) {
   CliqueState& s = ctx->clq;
-  if (mode != RPGO_CLIQUE_HEU_REF) {
-    set_error("rpgo_max_clique: mode %d is not available in this build", mode);
+  if (mode != RPGO_CLIQUE_HEU_REF && mode != RPGO_CLIQUE_EXACT) {
+    set_error("rpgo_max_clique: unknown mode %d", mode);
     return RPGO_ERR_ARG;
   }
   if (n <= 0) {
-    *k_out = -1;  // FMC returns -1 for an empty graph
+    *k_out = mode == RPGO_CLIQUE_EXACT ? 0 : -1;  // FMC: maxClique finds nothing, maxCliqueHeu returns -1
     return RPGO_OK;
   }
   const int W = (n + 63) / 64;
@@ -242,6 +245,10 @@ int clique_run(rpgo_ctx* ctx, const uint64_t* d_adj_u64, int n, int mode, int32_
     win_best = best;
     best = icc_c;
     pos = win + 1;
+  }
+  if (mode == RPGO_CLIQUE_EXACT) {
+    // the greedy size is a clique size that exists: the exact search's lower bound
+    return clique_exact_run(ctx, adj, n, std::max(best, 1), idx_out, k_out, kExactNodeBudget);
   }
   if (win >= 0) {
     // dmask for the winner's threshold

@@ -89,6 +89,7 @@ struct PcmState {
 struct CliqueState {
   DevBuf adj;      // caller-provided adjacency (rpgo_max_clique_bits)
   DevBuf deg, dmask, icc, out, scratch, best;
+  DevBuf ex_adj, ex_ctl, ex_stack, ex_path;  // exact mode (clique_exact.hip)
 };
 
 struct SolverRun;  // solver.hip / rpgo_solver.h
@@ -129,6 +130,9 @@ constexpr int kRecheckQueueCap = 1 << 22;
 // solver_capi.hip
 void solver_destroy(rpgo_ctx* ctx);
 void mlp_destroy(rpgo_ctx* ctx);  // learned.hip
+// clique_exact.hip: exact maximum clique given a clique size known to exist
+int clique_exact_run(rpgo_ctx* ctx, const unsigned long long* adj, int n, int lower_bound, int32_t* idx_out,
+                     int32_t* k_out, long long node_budget);
 // clique.hip
 int clique_run(rpgo_ctx* ctx, const uint64_t* d_adj, int n, int mode, int32_t* idx_out,
                int32_t* k_out);

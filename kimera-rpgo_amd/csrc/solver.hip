@@ -643,29 +643,41 @@ __global__ void k_coarse_offdiag(int F, int Sc, int nc, const int* __restrict__ 
 // trailing lower triangle in 32x32 tiles across the chip.  Pivots are floored
 // at a tiny positive value so a gauge-free (semi-definite) problem keeps a
 // finite preconditioner.
-constexpr int kCholNb = 32;
+constexpr int kCholNb = 16;
 
-__global__ void __launch_bounds__(256) k_chol_panel(int n, int k0, double* __restrict__ A) {
-  const int nb = min(kCholNb, n - k0);
+// One panel (columns k0 .. k0+15, rows k0 .. n-1) factored in LDS by one
+// workgroup: the rows' 16 doubles are contiguous in global memory, so the
+// panel is loaded and stored with coalesced row segments.
+__global__ void __launch_bounds__(1024) k_chol_panel(int n, int k0, double* __restrict__ A) {
+  __shared__ double P[kCoarseMax * kCholNb];
   __shared__ double piv;
-  for (int c = k0; c < k0 + nb; ++c) {
+  const int nb = min(kCholNb, n - k0);
+  const int rows = n - k0;
+  for (int e = threadIdx.x; e < rows * kCholNb; e += blockDim.x) {
+    const int r = e / kCholNb, c = e % kCholNb;
+    P[e] = c < nb ? A[(size_t)(k0 + r) * n + k0 + c] : 0.0;
+  }
+  __syncthreads();
+  for (int c = 0; c < nb; ++c) {
     if (threadIdx.x == 0) {
-      piv = sqrt(fmax(A[(size_t)c * n + c], 1e-300));
-      A[(size_t)c * n + c] = piv;
+      piv = sqrt(fmax(P[c * kCholNb + c], 1e-300));
+      P[c * kCholNb + c] = piv;
     }
     __syncthreads();
     const double d = piv;
-    for (int i = c + 1 + threadIdx.x; i < n; i += blockDim.x) A[(size_t)i * n + c] /= d;
+    for (int r = c + 1 + threadIdx.x; r < rows; r += blockDim.x) P[r * kCholNb + c] /= d;
     __syncthreads();
-    // update the remaining panel columns c+1 .. k0+nb-1 (lower part, rows > c)
-    const int w = k0 + nb - (c + 1);
-    if (w > 0) {
-      for (long long t = threadIdx.x; t < (long long)(n - c - 1) * w; t += blockDim.x) {
-        const int i = c + 1 + (int)(t / w), j = c + 1 + (int)(t % w);
-        if (j <= i) A[(size_t)i * n + j] -= A[(size_t)i * n + c] * A[(size_t)j * n + c];
+    const int w = nb - (c + 1);
+    if (w > 0)
+      for (int e = threadIdx.x; e < (rows - c - 1) * w; e += blockDim.x) {
+        const int r = c + 1 + e / w, j = c + 1 + e % w;
+        if (j <= r) P[r * kCholNb + j] -= P[r * kCholNb + c] * P[j * kCholNb + c];
       }
-    }
     __syncthreads();
+  }
+  for (int e = threadIdx.x; e < rows * kCholNb; e += blockDim.x) {
+    const int r = e / kCholNb, c = e % kCholNb;
+    if (c < nb && c <= r) A[(size_t)(k0 + r) * n + k0 + c] = P[e];
   }
 }
 
@@ -700,22 +712,46 @@ __global__ void __launch_bounds__(256) k_chol_update(int n, int k0, double* __re
 
 // W = L^-1 (lower): one wave per column c; w_i = (delta_ic - sum_{c<=j<i} L_ij w_j) / L_ii.
 // Lane l owns the entries j = l (mod 64) of its column in LDS, so every LDS
-// value is read back only by the lane that wrote it.
+// value is read back only by the lane that wrote it.  Row i+1 of L is loaded
+// while row i is reduced.
+constexpr int kTriPL = kCoarseMax / 64;  // row entries per lane
 __global__ void __launch_bounds__(256) k_tri_inv(int n, const double* __restrict__ L, double* __restrict__ W) {
   __shared__ double wsh[4][kCoarseMax];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int c = blockIdx.x * 4 + wv;
   if (c >= n) return;
   double* w = wsh[wv];
+  double cur[kTriPL], nxt[kTriPL];
+#pragma unroll
+  for (int q = 0; q < kTriPL; ++q) {
+    const int j = lane + 64 * q;
+    cur[q] = j < n ? L[(size_t)c * n + j] : 0.0;
+  }
   for (int i = c; i < n; ++i) {
-    const double* Lrow = L + (size_t)i * n;
+    if (i + 1 < n) {
+#pragma unroll
+      for (int q = 0; q < kTriPL; ++q) {
+        const int j = lane + 64 * q;
+        nxt[q] = j < n ? L[(size_t)(i + 1) * n + j] : 0.0;
+      }
+    }
     double part = 0.0;
-    for (int j = c + lane; j < i; j += 64) part += Lrow[j] * w[j];
+#pragma unroll
+    for (int q = 0; q < kTriPL; ++q) {
+      const int j = lane + 64 * q;
+      if (j >= c && j < i) part += cur[q] * w[j];
+    }
     part = wave_sum_pre(part);
-    const double wi = ((i == c) ? 1.0 : 0.0) - part;
-    const double v = wi / Lrow[i];
+    double diag = 0.0;
+#pragma unroll
+    for (int q = 0; q < kTriPL; ++q)
+      if (lane + 64 * q == i) diag = cur[q];
+    diag = __shfl(diag, i & 63);
+    const double v = (((i == c) ? 1.0 : 0.0) - part) / diag;
     if ((i & 63) == lane) w[i] = v;
     if (lane == 0) W[(size_t)i * n + c] = v;
+#pragma unroll
+    for (int q = 0; q < kTriPL; ++q) cur[q] = nxt[q];
   }
   for (int i = lane; i < c; i += 64) W[(size_t)i * n + c] = 0.0;
 }
@@ -948,6 +984,8 @@ __global__ void __launch_bounds__(128) k_pcg_update(int N, int S, int first, int
   const int seg = blockIdx.x * blockDim.x + threadIdx.x;
   const int k0 = seg * S;
   double rr = 0.0, rz = 0.0;
+  int cs_mine = -1;
+  double racc[6] = {0, 0, 0, 0, 0, 0};
   if (k0 < N) {
     const int k1 = min(k0 + S, N);
     if (!first) {
@@ -976,9 +1014,26 @@ __global__ void __launch_bounds__(128) k_pcg_update(int N, int S, int first, int
           acc[i] += v;
         }
       }
-      const int cs = k0 / Sc;
+      cs_mine = k0 / Sc;
 #pragma unroll
-      for (int i = 0; i < 6; ++i) atomicAdd(&rc[6 * cs + i], acc[i]);
+      for (int i = 0; i < 6; ++i) racc[i] = acc[i];
+    }
+  }
+  if (Pb) {
+    // coarse segments span many fine segments: a wavefront whose lanes share
+    // one coarse segment adds its sum once; lanes at a coarse boundary add alone
+    const int cs0 = __shfl(cs_mine, 0), cs1 = __shfl(cs_mine, 63);
+    const bool uniform = (cs0 == cs1) && cs0 >= 0 &&
+                         __shfl(k0 < N ? 1 : 0, 63) != 0;
+    if (uniform) {
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        const double v = wave_sum_pre(racc[i]);
+        if ((threadIdx.x & 63) == 0) atomicAdd(&rc[6 * cs0 + i], v);
+      }
+    } else if (cs_mine >= 0) {
+#pragma unroll
+      for (int i = 0; i < 6; ++i) atomicAdd(&rc[6 * cs_mine + i], racc[i]);
     }
   }
   rr = block_sum(rr, red);
@@ -1120,6 +1175,16 @@ __global__ void k_min(long long n, const double* __restrict__ a, unsigned long l
 }
 
 // weights binary within tol: counts the violations
+// unknown factors whose final weight disagrees with the TLS decision at the
+// final poses (inlier iff 0.5 e^T Lambda e < barc^2): 0 at a GNC fixed point
+__global__ void k_tls_check(int F, const double* __restrict__ err, const double* __restrict__ barc,
+                            const unsigned char* __restrict__ known, const double* __restrict__ w,
+                            unsigned int* __restrict__ bad) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F || known[f]) return;
+  if ((err[f] < barc[f]) != (w[f] > 0.5)) atomicAdd(bad, 1u);
+}
+
 __global__ void k_weights_binary(int F, const double* __restrict__ w, double tol,
                                  unsigned int* __restrict__ bad) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1285,7 +1350,7 @@ int SolverRun::coarse_setup(hipStream_t st, double lambda, int* Sc_out, int* nse
                                                              fj.as<int>(), Hij.as<double>(),
                                                              Pb.as<float>(), Ac.as<double>());
   for (int k0 = 0; k0 < nc; k0 += kCholNb) {
-    k_chol_panel<<<1, 256, 0, st>>>(nc, k0, Ac.as<double>());
+    k_chol_panel<<<1, 1024, 0, st>>>(nc, k0, Ac.as<double>());
     const int rest = nc - k0 - kCholNb;
     if (rest > 0) {
       const int T = (rest + 31) / 32;
@@ -1592,7 +1657,22 @@ int SolverRun::run(hipStream_t st, rpgo_solve_report* rep) {
     prm = saved;
     if (rc) return rc;
   }
+  int disagree = 0;
+  if (prm.use_gnc) {
+    if ((rc = compute_errors(st, x.as<double>()))) return rc;
+    RPGO_HIP_CHECK(hipMemsetAsync(sc.as<double>() + 24, 0, sizeof(double), st));
+    k_tls_check<<<grid_for(F, 256, 1 << 20), 256, 0, st>>>(F, err.as<double>(), barc.as<double>(),
+                                                          known.as<unsigned char>(), w.as<double>(),
+                                                          reinterpret_cast<unsigned int*>(sc.as<double>() + 24));
+    RPGO_HIP_CHECK(hipGetLastError());
+    double h[25];
+    if ((rc = read_scalars(st, 25, h))) return rc;
+    unsigned int bad;
+    std::memcpy(&bad, &h[24], sizeof(bad));
+    disagree = (int)bad;
+  }
   if (rep) {
+    rep->n_tls_disagree = disagree;
     rep->gnc_iterations = gnc_it;
     rep->lm_iterations = stats.lm_iterations;
     rep->pcg_iterations = stats.pcg_iterations;

@@ -197,7 +197,6 @@ int rpgo_solve(rpgo_ctx* ctx, const rpgo_solve_problem* pr, const rpgo_solve_par
     double sum = 0.0;
     for (int f = 0; f < F; ++f) sum += wout[f];
     rep->n_inliers = (int32_t)((size_t)sum - (size_t)prm->known_inliers);
-    rep->reserved = 0;
   }
   return RPGO_OK;
 }

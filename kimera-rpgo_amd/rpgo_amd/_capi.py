@@ -86,10 +86,10 @@ class SolveReport(ctypes.Structure):
                 ("pcg_iterations", ctypes.c_int32), ("linearizations", ctypes.c_int32),
                 ("mu_init", ctypes.c_double), ("mu_final", ctypes.c_double),
                 ("final_cost", ctypes.c_double), ("ms_total", ctypes.c_double),
-                ("n_inliers", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("n_inliers", ctypes.c_int32), ("n_tls_disagree", ctypes.c_int32)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 # (name, argtypes, restype) of every exported entry point

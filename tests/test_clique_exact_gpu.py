@@ -1,0 +1,77 @@
+"""Exact max-clique mode (RPGO_CLIQUE_EXACT) against the literal CPU
+restatement of FMC::maxClique (findClique.cpp:31-145): bit-exact size and
+vertex order on random graphs (with planted cliques), edge cases, and a PCM
+consistency graph small enough for the sequential search."""
+import numpy as np
+import pytest
+
+import oracle as O
+from rpgo_amd import _capi, synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = _capi.Context(0)
+    yield c
+    c.close()
+
+
+def _random_graph(rng, n, p, plant):
+    a = np.triu(rng.random((n, n)) < p, 1)
+    a = a | a.T
+    if plant and n > 4:
+        s = rng.choice(n, int(rng.integers(3, max(4, n // 2))), replace=False)
+        a[np.ix_(s, s)] = True
+        np.fill_diagonal(a, False)
+    return a.astype(np.uint8)
+
+
+def _gpu_exact(ctx, adj):
+    n = adj.shape[0]
+    k, idx = ctx.max_clique_bits(_capi.pack_bits(adj), n, _capi.CLIQUE_EXACT)
+    return k, idx
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_graphs_bit_exact(ctx, seed):
+    rng = np.random.default_rng(100 + seed)
+    for t in range(40):
+        n = int(rng.integers(1, 70))
+        a = _random_graph(rng, n, float(rng.uniform(0.05, 0.9)), t % 3 == 0)
+        kr, ir = O.max_clique_exact(a)
+        k, idx = _gpu_exact(ctx, a)
+        assert k == kr, (seed, t, n, k, kr)
+        assert list(idx[:k]) == list(ir), (seed, t, n, list(idx[:k]), list(ir))
+    print(f"[clique] exact seed {seed}: 40 graphs bit-exact", flush=True)
+
+
+def test_edge_cases(ctx):
+    # no edges: the highest vertex alone
+    a = np.zeros((5, 5), np.uint8)
+    k, idx = _gpu_exact(ctx, a)
+    assert k == 1 and list(idx[:1]) == [4]
+    assert (k, list(idx[:k])) == (O.max_clique_exact(a)[0], list(O.max_clique_exact(a)[1]))
+    # complete graph: every vertex, deepest first
+    a = 1 - np.eye(7, dtype=np.uint8)
+    k, idx = _gpu_exact(ctx, a)
+    kr, ir = O.max_clique_exact(a)
+    assert k == kr == 7 and list(idx[:k]) == list(ir)
+    # more than 64 vertices (several words per row)
+    rng = np.random.default_rng(5)
+    a = _random_graph(rng, 150, 0.3, True)
+    kr, ir = O.max_clique_exact(a)
+    k, idx = _gpu_exact(ctx, a)
+    assert k == kr and list(idx[:k]) == list(ir)
+
+
+def test_pcm_graph_bit_exact(ctx):
+    g = synth.se3_helix(n_poses=2000, n_lc=300, outlier_frac=0.3, seed=1)
+    ctx.pcm_upload(g.dim, g.pose0, g.odom, g.odom_cov, g.lc_from, g.lc_to, g.lc_pose, g.lc_cov)
+    ctx.pcm_run(-1.0, 5.0, precision=_capi.PCM_FP64_REPLAY)
+    adj = _capi.unpack_bits(ctx.pcm_download()["adj_bits"], len(ctx.pcm_download()["accepted"]))
+    k, idx, _ = ctx.max_clique(_capi.CLIQUE_EXACT)
+    kr, ir = O.max_clique_exact(adj)
+    print(f"[clique] PCM m={adj.shape[0]} exact {k} (oracle {kr})", flush=True)
+    assert k == kr and list(idx[:k]) == list(ir)
