@@ -62,6 +62,41 @@ def cpu_baseline(g, odom_thr, lc_thr, budget_s=15.0):
                        f"{t_pairs:.1f} s; fold {t_fold * 1e3:.1f} ms)")
 
 
+def robust_solve(ctx, gl):
+    """Robust-solve ms on the 50,000-pose graph: PCM (FP32 + FP64 recheck, the
+    reference's heuristic clique) then GNC-TLS over LM on odometry + prior +
+    the clique's loop closures; plain GNC (started as GTSAM starts it) against
+    the learned start (features + MLP + GNC from the predicted weights and mu,
+    plain GNC as the fallback), on the same GPU.  Device times (HIP events)."""
+    from rpgo_amd import _capi, learned
+
+    t0 = time.perf_counter()
+    pcm = learned.pcm_stage(ctx, gl)
+    ms_pcm = pcm.ms
+    poses_p, w_p, rep_p, nk = learned.solve_plain(ctx, gl, pcm.inliers)
+    res = {"workload": f"SE(3) helix {gl.n_poses} poses, {gl.n_lc} loop closures, "
+                       f"{int(round(100 * gl.is_outlier.mean()))}% outliers",
+           "pcm_ms": ms_pcm, "solver_loop_closures": int(len(pcm.inliers)),
+           "outliers_after_pcm": int(gl.is_outlier[pcm.inliers].sum()),
+           "plain": {"ms": rep_p.ms_total, "gnc_iterations": rep_p.gnc_iterations,
+                     "lm_iterations": rep_p.lm_iterations, "pcg_iterations": rep_p.pcg_iterations,
+                     "inliers": rep_p.n_inliers},
+           "end_to_end_plain_ms": ms_pcm + rep_p.ms_total}
+    if os.path.exists(learned.MODEL_PATH):
+        model = learned.LearnedModel.load()
+        poses_l, w_l, rep_l, fb, ms_l = learned.solve_learned(ctx, gl, pcm, model)
+        res["learned"] = {"ms": ms_l, "gnc_iterations": rep_l.gnc_iterations,
+                          "lm_iterations": rep_l.lm_iterations, "fallback": bool(fb),
+                          "same_inlier_set": bool(np.array_equal(np.rint(w_p), np.rint(w_l))),
+                          "max_pose_diff_m": float(np.abs(poses_p[:, 9:] - poses_l[:, 9:]).max())}
+        res["end_to_end_learned_ms"] = ms_pcm + ms_l
+        res["speedup_learned_vs_plain"] = (ms_pcm + rep_p.ms_total) / (ms_pcm + ms_l)
+    else:
+        res["learned"] = None
+    res["wall_s"] = time.perf_counter() - t0
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +107,8 @@ def main():
     ap.add_argument("--no-large", dest="large", action="store_false",
                     help="skip the extra 50,000-pose pass reported beside the main line")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-solve", dest="solve", action="store_false",
+                    help="skip the robust-solve measurement on the 50,000-pose graph")
     ap.add_argument("--seed", type=int, default=0)
     a = ap.parse_args()
     if a.gpus != 1:
@@ -154,6 +191,8 @@ def main():
             "ms_recheck": float(np.mean([s.ms_recheck for s in sl])), "n_rechecked": sl[-1].n_rechecked,
             "n_flipped": sl[-1].n_flipped, "ingest_ms": ing.ms_total, "ingest_fold_ms": ing.ms_fold,
             "clique_heu_ref": {"size": kl, "ms": msl}}
+        if a.solve:
+            out["robust_solve_50k"] = robust_solve(ctx, gl)
     ctx.close()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(g, w["odom_thr"], w["lc_thr"])

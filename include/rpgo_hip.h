@@ -132,6 +132,16 @@ int rpgo_max_clique(rpgo_ctx* ctx, int32_t mode, int32_t* idx_out, int32_t* k_ou
 /* Max clique of a caller-provided bit-packed symmetric adjacency (n vertices). */
 int rpgo_max_clique_bits(rpgo_ctx* ctx, int32_t n, const uint64_t* adj_bits,
                          int32_t mode, int32_t* idx_out, int32_t* k_out);
+/* Incremental heuristic over the last rpgo_pcm_run adjacency, replacing
+ * findMaxCliqueHeuIncremental (src/utils/GraphUtils.cpp:30-44, FMC::
+ * maxCliqueHeuIncremental findCliqueHeu.cpp:119-209): only the last num_new
+ * vertices are candidates and the bound starts at prev_size; *k_out is the new
+ * size when it strictly exceeds prev_size, else 0 ("keep the previous set"). */
+int rpgo_max_clique_incremental(rpgo_ctx* ctx, int32_t num_new, int32_t prev_size,
+                                int32_t* idx_out, int32_t* k_out);
+int rpgo_max_clique_incremental_bits(rpgo_ctx* ctx, int32_t n, const uint64_t* adj_bits,
+                                     int32_t num_new, int32_t prev_size, int32_t* idx_out,
+                                     int32_t* k_out);
 
 /* ------------------------------------------------------------------------
  * Robust solve: GNC-TLS over Levenberg-Marquardt / Gauss-Newton with a
@@ -214,6 +224,12 @@ typedef struct {
                                    must pass)                           */
 } rpgo_solve_report;
 
+/* 1/2 e^T Lambda e of every factor of the last rpgo_solve problem at poses
+ * (n_poses * 12 doubles), or at the returned solution when poses is NULL. */
+int rpgo_solve_errors(rpgo_ctx* ctx, const double* poses, double* err_out);
+/* GNC's TLS threshold in PROBABILITY mode: 0.5 * chi^2 quantile(alpha, dof)
+ * (GncOptimizer::setInlierCostThresholdsAtProbability). */
+double rpgo_gnc_threshold(double alpha, int32_t dof);
 /* Fill *p with the reference defaults (SolverParams.h:60-94, GTSAM LM). */
 void rpgo_solve_params_default(rpgo_solve_params* p);
 /* Solve; poses_out: n_poses * 12, weights_out: n_factors (may be NULL). */
@@ -238,8 +254,11 @@ int rpgo_solve(rpgo_ctx* ctx, const rpgo_solve_problem* prob, const rpgo_solve_p
 
 /* Features of the accepted loop closures of the last rpgo_pcm_run (SE(3)),
  * in the compacted (accepted) order: feat m * RPGO_LC_FEATURES floats.
- * clique_idx: the k clique members (indices into the accepted set). */
-int rpgo_lc_features(rpgo_ctx* ctx, const int32_t* clique_idx, int32_t k, float* feat);
+ * clique_idx: the k clique members (indices into the accepted set);
+ * lc_resid: NULL, or m residuals 1/2 e^T Lambda e of the loop closures after
+ * GNC's first solve (all weights 1), with barc the TLS threshold. */
+int rpgo_lc_features(rpgo_ctx* ctx, const int32_t* clique_idx, int32_t k, const double* lc_resid,
+                     double barc, float* feat);
 /* Load parameters and the feature normalisation (mean[16] then 1/std[16]);
  * resets the Adam state. */
 int rpgo_mlp_set_params(rpgo_ctx* ctx, const float* params, const float* norm);

@@ -383,4 +383,32 @@ int rpgo_max_clique_bits(rpgo_ctx* ctx, int32_t n, const uint64_t* adj_bits, int
   return clique_run(ctx, ctx->clq.adj.as<uint64_t>(), n, mode, idx_out, k_out);
 }
 
+int rpgo_max_clique_incremental(rpgo_ctx* ctx, int32_t num_new, int32_t prev_size, int32_t* idx_out,
+                                int32_t* k_out) {
+  if (!ctx || !k_out) {
+    set_error("rpgo_max_clique_incremental: NULL argument");
+    return RPGO_ERR_ARG;
+  }
+  PcmState& s = ctx->pcm;
+  if (!s.ran) {
+    set_error("rpgo_max_clique_incremental: no adjacency computed yet");
+    return RPGO_ERR_STATE;
+  }
+  RPGO_HIP_CHECK(hipSetDevice(ctx->device));
+  return clique_incremental(ctx, s.adj.as<uint64_t>(), s.m, num_new, prev_size, idx_out, k_out);
+}
+
+int rpgo_max_clique_incremental_bits(rpgo_ctx* ctx, int32_t n, const uint64_t* adj_bits, int32_t num_new,
+                                     int32_t prev_size, int32_t* idx_out, int32_t* k_out) {
+  if (!ctx || !k_out || n < 0 || (n > 0 && !adj_bits)) {
+    set_error("rpgo_max_clique_incremental_bits: invalid argument");
+    return RPGO_ERR_ARG;
+  }
+  RPGO_HIP_CHECK(hipSetDevice(ctx->device));
+  const int W = (n + 63) / 64;
+  int rc = upload(ctx->clq.adj, adj_bits, sizeof(uint64_t) * (size_t)n * W, ctx->stream);
+  if (rc) return rc;
+  return clique_incremental(ctx, ctx->clq.adj.as<uint64_t>(), n, num_new, prev_size, idx_out, k_out);
+}
+
 }  // extern "C"

@@ -196,8 +196,40 @@ __global__ void __launch_bounds__(64) k_replay(int n, int W, int c, int best,
   (void)n;
 }
 
+// Greedy scan over candidates pos0 .. n-1 starting from the bound best0;
+// mode EXACT continues with the exact search, mode HEU_REF replays the winner.
+static int clique_scan(rpgo_ctx* ctx, const uint64_t* d_adj_u64, int n, int mode, int pos0, int best0,
+                       int32_t* idx_out, int32_t* k_out);
+
 int clique_run(rpgo_ctx* ctx, const uint64_t* d_adj_u64, int n, int mode, int32_t* idx_out,
                int32_t* k_out) {
+  return clique_scan(ctx, d_adj_u64, n, mode, 0, -1, idx_out, k_out);
+}
+
+// FMC::maxCliqueHeuIncremental (findCliqueHeu.cpp:119-209) through
+// findMaxCliqueHeuIncremental (GraphUtils.cpp:30-44): candidates are only the
+// last num_new vertices, the bound starts at the previous clique size, and
+// the size is reported only when it strictly grew (0 otherwise).
+int clique_incremental(rpgo_ctx* ctx, const uint64_t* d_adj_u64, int n, int num_new, int prev_size,
+                       int32_t* idx_out, int32_t* k_out) {
+  if (num_new < 0 || num_new > n || prev_size < 0) {
+    set_error("rpgo_max_clique_incremental: num_new %d / prev_size %d invalid for %d vertices", num_new,
+              prev_size, n);
+    return RPGO_ERR_ARG;
+  }
+  if (n <= 0 || num_new == 0) {
+    *k_out = 0;
+    return RPGO_OK;
+  }
+  int32_t k = 0;
+  const int rc = clique_scan(ctx, d_adj_u64, n, RPGO_CLIQUE_HEU_REF, n - num_new, prev_size, idx_out, &k);
+  if (rc) return rc;
+  *k_out = k > prev_size ? k : 0;
+  return RPGO_OK;
+}
+
+static int clique_scan(rpgo_ctx* ctx, const uint64_t* d_adj_u64, int n, int mode, int pos0, int best0,
+                       int32_t* idx_out, int32_t* k_out) {
   CliqueState& s = ctx->clq;
   if (mode != RPGO_CLIQUE_HEU_REF && mode != RPGO_CLIQUE_EXACT) {
     set_error("rpgo_max_clique: unknown mode %d", mode);
@@ -222,7 +254,7 @@ int clique_run(rpgo_ctx* ctx, const uint64_t* d_adj_u64, int n, int mode, int32_
   hipStream_t st = ctx->stream;
   k_degrees<<<(n + 255) / 256, 256, 0, st>>>(n, W, adj, s.deg.as<int>());
   RPGO_HIP_CHECK(hipGetLastError());
-  int best = -1, pos = 0, win = -1, win_best = -1;
+  int best = best0, pos = pos0, win = -1, win_best = -1;
   int* d_first = s.best.as<int>();
   while (pos < n) {
     k_dmask<<<(W + 255) / 256, 256, 0, st>>>(n, W, best, s.deg.as<int>(),

@@ -63,7 +63,8 @@ __device__ __forceinline__ double wave_sum_l(double v) {
 //  3 consistent clique members / k
 //  4 clique membership
 //  5 log(1 + chi^2 of the loop closure at the chain poses)
-//  6 log trace of the rotation covariance, 7 of the translation covariance
+//  6 log(1 + r / barc), 7 min(r / barc, 10): residual r = 1/2 e^T Lambda e of
+//    the loop closure after GNC's first solve (all weights 1), when given
 //  8 log(1 + |t| of the measurement), 9 rotation angle of the measurement
 // 10 |i - j| / N, 11 log(1 + |i - j|)
 // 12 log m, 13 log N
@@ -73,7 +74,8 @@ __global__ void k_lc_features(int m, int n_poses, int W, const double* __restric
                               const int* __restrict__ c_from, const int* __restrict__ c_to,
                               const double* __restrict__ c_lcm, const int* __restrict__ accepted,
                               const double* __restrict__ odom_dist, const uint64_t* __restrict__ adj,
-                              const uint64_t* __restrict__ cmask, int k_clique, float* __restrict__ feat) {
+                              const uint64_t* __restrict__ cmask, int k_clique,
+                              const double* __restrict__ resid, double barc, float* __restrict__ feat) {
   using G = R64<3>;
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= m) return;
@@ -104,8 +106,7 @@ __global__ void k_lc_features(int m, int n_poses, int W, const double* __restric
     for (int c = 0; c < 6; ++c) s += Linv[r * 6 + c] * e[c];
     chi2 += e[r] * s;
   }
-  const double trR = Z[12 + 0] + Z[12 + 7] + Z[12 + 14];
-  const double trT = Z[12 + 21] + Z[12 + 28] + Z[12 + 35];
+  const double rb = resid ? fmax(resid[a], 0.0) / fmax(barc, 1e-300) : 0.0;
   const double tn = sqrt(Z[9] * Z[9] + Z[10] * Z[10] + Z[11] * Z[11]);
   const double ang = acos(fmin(1.0, fmax(-1.0, 0.5 * (Z[0] + Z[4] + Z[8] - 1.0))));
   const double gap = fabs((double)(i - j));
@@ -118,8 +119,8 @@ __global__ void k_lc_features(int m, int n_poses, int W, const double* __restric
   f[3] = (float)(inc / fmax(1.0, (double)k_clique));
   f[4] = member ? 1.f : 0.f;
   f[5] = (float)log1p(fmax(chi2, 0.0));
-  f[6] = (float)log(fmax(trR, 1e-300));
-  f[7] = (float)log(fmax(trT, 1e-300));
+  f[6] = (float)log1p(rb);
+  f[7] = (float)fmin(rb, 10.0);
   f[8] = (float)log1p(tn);
   f[9] = (float)ang;
   f[10] = (float)(gap / fmax(1.0, (double)n_poses));
@@ -469,7 +470,8 @@ using namespace rpgo;
 
 extern "C" {
 
-int rpgo_lc_features(rpgo_ctx* ctx, const int32_t* clique_idx, int32_t k, float* feat) {
+int rpgo_lc_features(rpgo_ctx* ctx, const int32_t* clique_idx, int32_t k, const double* lc_resid, double barc,
+                     float* feat) {
   if (!ctx || !feat || k < 0 || (k > 0 && !clique_idx)) {
     set_error("rpgo_lc_features: bad argument");
     return RPGO_ERR_ARG;
@@ -495,20 +497,25 @@ int rpgo_lc_features(rpgo_ctx* ctx, const int32_t* clique_idx, int32_t k, float*
     mask[clique_idx[t] >> 6] |= 1ull << (clique_idx[t] & 63);
   }
   RPGO_HIP_CHECK(hipSetDevice(ctx->device));
-  DevBuf dmask, dfeat;
+  DevBuf dmask, dfeat, dres;
   int rc;
   if ((rc = dmask.reserve(sizeof(uint64_t) * W)) || (rc = dfeat.reserve(sizeof(float) * kF * (size_t)m))) return rc;
+  if (lc_resid && (rc = dres.reserve(sizeof(double) * (size_t)m))) return rc;
   hipStream_t st = ctx->stream;
   RPGO_HIP_CHECK(hipMemcpyAsync(dmask.ptr, mask.data(), sizeof(uint64_t) * W, hipMemcpyHostToDevice, st));
+  if (lc_resid)
+    RPGO_HIP_CHECK(hipMemcpyAsync(dres.ptr, lc_resid, sizeof(double) * (size_t)m, hipMemcpyHostToDevice, st));
   k_lc_features<<<(m + 127) / 128, 128, 0, st>>>(m, s.n_poses, W, s.traj.as<double>(), s.c_from.as<int>(),
                                                  s.c_to.as<int>(), s.c_lcm.as<double>(), s.accepted.as<int>(),
                                                  s.odom_dist.as<double>(), s.adj.as<uint64_t>(),
-                                                 dmask.as<uint64_t>(), k, dfeat.as<float>());
+                                                 dmask.as<uint64_t>(), k, lc_resid ? dres.as<double>() : nullptr,
+                                                 barc, dfeat.as<float>());
   RPGO_HIP_CHECK(hipGetLastError());
   RPGO_HIP_CHECK(hipMemcpyAsync(feat, dfeat.ptr, sizeof(float) * kF * (size_t)m, hipMemcpyDeviceToHost, st));
   RPGO_HIP_CHECK(hipStreamSynchronize(st));
   dmask.release();
   dfeat.release();
+  dres.release();
   return RPGO_OK;
 }
 

@@ -130,6 +130,8 @@ constexpr int kRecheckQueueCap = 1 << 22;
 // solver_capi.hip
 void solver_destroy(rpgo_ctx* ctx);
 void mlp_destroy(rpgo_ctx* ctx);  // learned.hip
+int clique_incremental(rpgo_ctx* ctx, const uint64_t* d_adj_u64, int n, int num_new, int prev_size,
+                       int32_t* idx_out, int32_t* k_out);
 // clique_exact.hip: exact maximum clique given a clique size known to exist
 int clique_exact_run(rpgo_ctx* ctx, const unsigned long long* adj, int n, int lower_bound, int32_t* idx_out,
                      int32_t* k_out, long long node_budget);

@@ -99,6 +99,35 @@ void rpgo_solve_params_default(rpgo_solve_params* p) {
   p->gnc_factor_dim = 6;
 }
 
+double rpgo_gnc_threshold(double alpha, int32_t dof) {
+  return 0.5 * chi2_quantile(alpha, dof > 0 ? dof : 6);
+}
+
+int rpgo_solve_errors(rpgo_ctx* ctx, const double* poses, double* err_out) {
+  if (!ctx || !err_out) {
+    set_error("rpgo_solve_errors: NULL argument");
+    return RPGO_ERR_ARG;
+  }
+  if (!ctx->solver || ctx->solver->N <= 0) {
+    set_error("rpgo_solve_errors: no rpgo_solve problem in this context");
+    return RPGO_ERR_STATE;
+  }
+  RPGO_HIP_CHECK(hipSetDevice(ctx->device));
+  SolverRun& S = *ctx->solver;
+  hipStream_t st = ctx->stream;
+  if (S.F <= 0) return RPGO_OK;
+  const double* xp = S.x.as<double>();
+  if (poses) {
+    RPGO_HIP_CHECK(hipMemcpyAsync(S.xn.ptr, poses, sizeof(double) * 12 * S.N, hipMemcpyHostToDevice, st));
+    xp = S.xn.as<double>();
+  }
+  int rc;
+  if ((rc = S.compute_errors(st, xp))) return rc;
+  RPGO_HIP_CHECK(hipMemcpyAsync(err_out, S.err.ptr, sizeof(double) * S.F, hipMemcpyDeviceToHost, st));
+  RPGO_HIP_CHECK(hipStreamSynchronize(st));
+  return RPGO_OK;
+}
+
 int rpgo_solve(rpgo_ctx* ctx, const rpgo_solve_problem* pr, const rpgo_solve_params* prm,
                double* poses_out, double* weights_out, rpgo_solve_report* rep) {
   if (!ctx || !pr || !prm || !poses_out) {

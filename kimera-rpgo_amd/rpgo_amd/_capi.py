@@ -112,7 +112,13 @@ SIGNATURES = [
     ("rpgo_max_clique", [ctypes.c_void_p, ctypes.c_int32, _i32, _i32, _d], ctypes.c_int),
     ("rpgo_max_clique_bits", [ctypes.c_void_p, ctypes.c_int32, _u64, ctypes.c_int32, _i32, _i32],
      ctypes.c_int),
-    ("rpgo_lc_features", [ctypes.c_void_p, _i32, ctypes.c_int32, _f], ctypes.c_int),
+    ("rpgo_max_clique_incremental", [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _i32, _i32],
+     ctypes.c_int),
+    ("rpgo_max_clique_incremental_bits", [ctypes.c_void_p, ctypes.c_int32, _u64, ctypes.c_int32,
+                                          ctypes.c_int32, _i32, _i32], ctypes.c_int),
+    ("rpgo_lc_features", [ctypes.c_void_p, _i32, ctypes.c_int32, _d, ctypes.c_double, _f], ctypes.c_int),
+    ("rpgo_solve_errors", [ctypes.c_void_p, _d, _d], ctypes.c_int),
+    ("rpgo_gnc_threshold", [ctypes.c_double, ctypes.c_int32], ctypes.c_double),
     ("rpgo_mlp_set_params", [ctypes.c_void_p, _f, _f], ctypes.c_int),
     ("rpgo_mlp_get_params", [ctypes.c_void_p, _f], ctypes.c_int),
     ("rpgo_mlp_forward", [ctypes.c_void_p, ctypes.c_int32, _f, _f], ctypes.c_int),
@@ -280,13 +286,23 @@ class Context:
         return out, w[:F].copy(), rep
 
     # ------------------------------------------------------ learned initialisation
-    def lc_features(self, clique_idx):
-        """(m, 16) float32 features of the accepted loop closures of the last PCM run."""
+    def lc_features(self, clique_idx, lc_resid=None, barc=1.0):
+        """(m, 16) float32 features of the accepted loop closures of the last PCM run;
+        lc_resid: residuals of the m accepted loop closures after GNC's first solve."""
         ci = np.ascontiguousarray(clique_idx, dtype=np.int32)
         m = self.pcm_download()["accepted"].shape[0]
+        r = None if lc_resid is None else np.ascontiguousarray(lc_resid, dtype=np.float64)
+        if r is not None and r.shape[0] != m:
+            raise ValueError(f"lc_resid has {r.shape[0]} entries, expected {m}")
         out = np.zeros((max(m, 1), LC_FEATURES), np.float32)
-        check(lib().rpgo_lc_features(self._h, _ptr(ci, _i32), ci.shape[0], _ptr(out, _f)))
+        check(lib().rpgo_lc_features(self._h, _ptr(ci, _i32), ci.shape[0], _ptr(r), float(barc), _ptr(out, _f)))
         return out[:m]
+
+    def solve_errors(self, n_factors, poses=None):
+        p = None if poses is None else np.ascontiguousarray(poses, dtype=np.float64)
+        out = np.zeros(max(n_factors, 1))
+        check(lib().rpgo_solve_errors(self._h, _ptr(p), _ptr(out)))
+        return out[:n_factors]
 
     def mlp_set_params(self, params, norm):
         p = np.ascontiguousarray(params, dtype=np.float32).reshape(MLP_NPARAMS)
@@ -324,12 +340,31 @@ class Context:
                                    _ptr(hist)))
         return hist[:epochs]
 
+    def max_clique_incremental(self, num_new, prev_size):
+        idx = np.zeros(max(self._m, 1), np.int32)
+        k = ctypes.c_int32()
+        check(lib().rpgo_max_clique_incremental(self._h, int(num_new), int(prev_size), _ptr(idx, _i32),
+                                                ctypes.byref(k)))
+        return int(k.value), idx[: max(k.value, 0)].copy()
+
+    def max_clique_incremental_bits(self, adj_bits, n, num_new, prev_size):
+        return _incremental_bits(self, adj_bits, n, num_new, prev_size)
+
     def max_clique_bits(self, adj_bits, n, mode=CLIQUE_HEU_REF):
         a = np.ascontiguousarray(adj_bits, dtype=np.uint64).reshape(-1)
         idx = np.zeros(max(n, 1), np.int32)
         k = ctypes.c_int32()
         check(lib().rpgo_max_clique_bits(self._h, n, _ptr(a, _u64), mode, _ptr(idx, _i32), ctypes.byref(k)))
         return int(k.value), idx[: max(k.value, 0)].copy()
+
+
+def _incremental_bits(ctx, adj_bits, n, num_new, prev_size):
+    a = np.ascontiguousarray(adj_bits, dtype=np.uint64).reshape(-1)
+    idx = np.zeros(max(n, 1), np.int32)
+    k = ctypes.c_int32()
+    check(lib().rpgo_max_clique_incremental_bits(ctx._h, n, _ptr(a, _u64), int(num_new), int(prev_size),
+                                                 _ptr(idx, _i32), ctypes.byref(k)))
+    return int(k.value), idx[: max(k.value, 0)].copy()
 
 
 def unpack_bits(adj_bits, m):

@@ -77,12 +77,30 @@ def predict(ctx, model: LearnedModel, feats):
     return w0, mu_final
 
 
+GNC_ALPHA = 0.9  # RobustSolverParams default (SolverParams.h:60-68)
+
+
+def clique_features(ctx, gr, pcm: PcmResult, **solve_params):
+    """Features of the solver's loop closures (clique order), with the
+    residuals after GNC's first solve (LM with every weight 1).  Returns
+    (features, device ms of the first solve)."""
+    arrs = problem.solve_arrays(gr, pcm.inliers)
+    nk = arrs[-1]
+    _, _, rep1 = ctx.solve(*arrs[:-1], **solve_params)
+    err = ctx.solve_errors(len(arrs[1]))
+    resid = np.zeros(len(pcm.accepted))
+    resid[pcm.clique] = err[nk:]
+    barc = _capi.lib().rpgo_gnc_threshold(GNC_ALPHA, 6)
+    return ctx.lc_features(pcm.clique, resid, barc)[pcm.clique], rep1.ms_total
+
+
 def solve_learned(ctx, gr, pcm: PcmResult, model: LearnedModel, mu_step=1.4, **solve_params):
     """Learned-start GNC with the plain solve as fallback.
 
     Returns (poses, weights, report, used_fallback, ms_total) where ms_total
-    is the device time of the features, the forward pass and the solve(s)."""
-    feats = ctx.lc_features(pcm.clique)[pcm.clique]
+    is the device time of the first solve, the solve from the learned start
+    and, on fallback, the plain solve."""
+    feats, ms = clique_features(ctx, gr, pcm, **solve_params)
     w0, mu_final = predict(ctx, model, feats)
     arrs = problem.solve_arrays(gr, pcm.inliers)
     nk = arrs[-1]
@@ -90,7 +108,7 @@ def solve_learned(ctx, gr, pcm: PcmResult, model: LearnedModel, mu_step=1.4, **s
     mu0 = max(mu_final / mu_step ** model.back_steps, 1e-6)
     poses, w, rep = ctx.solve(*arrs[:-1], use_gnc=1, known_inliers=nk, init_weights=init_w, init_mu=mu0,
                               **solve_params)
-    ms = rep.ms_total
+    ms += rep.ms_total
     binary = bool(np.all(np.abs(w - np.rint(w)) <= 1e-4))
     if rep.n_tls_disagree == 0 and binary:
         return poses, w, rep, False, ms
@@ -103,7 +121,7 @@ def graph_rows(ctx, gr, **solve_params):
     pcm = pcm_stage(ctx, gr)
     if len(pcm.clique) == 0:
         return None
-    feats = ctx.lc_features(pcm.clique)[pcm.clique]
+    feats, _ = clique_features(ctx, gr, pcm, **solve_params)
     _, w, rep, nk = solve_plain(ctx, gr, pcm.inliers, **solve_params)
     wl = w[nk:]
     mu = rep.mu_final if rep.mu_final > 0 else 1.0

@@ -29,6 +29,7 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--mu-weight", type=float, default=0.1)
     ap.add_argument("--out", default=learned.MODEL_PATH)
+    ap.add_argument("--dump", default="", help="also write the training rows (x, target, graph id) here")
     a = ap.parse_args(argv)
     ctx = _capi.Context(0)
     xs, ts, gid = [], [], []
@@ -49,6 +50,8 @@ def main(argv=None):
     # balance the classes: outliers are the minority among the solver's loop closures
     pos = t[:, 0] > 0.5
     sw = np.where(pos, 0.5 / max(pos.mean(), 1e-3), 0.5 / max(1 - pos.mean(), 1e-3)).astype(np.float32)
+    if a.dump:
+        np.savez(a.dump, x=x, t=t, gid=np.concatenate(gid))
     norm = learned.fit_norm(x)
     model = learned.LearnedModel(learned.init_params(a.seed0), norm)
     ctx.mlp_set_params(model.params, model.norm)

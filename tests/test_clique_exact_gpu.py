@@ -75,3 +75,20 @@ def test_pcm_graph_bit_exact(ctx):
     kr, ir = O.max_clique_exact(adj)
     print(f"[clique] PCM m={adj.shape[0]} exact {k} (oracle {kr})", flush=True)
     assert k == kr and list(idx[:k]) == list(ir)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_incremental_heuristic_bit_exact(ctx, seed):
+    """findMaxCliqueHeuIncremental: size (0 unless it grew) and entries."""
+    rng = np.random.default_rng(200 + seed)
+    for t in range(40):
+        n = int(rng.integers(1, 120))
+        a = _random_graph(rng, n, float(rng.uniform(0.05, 0.9)), t % 3 == 0)
+        num_new = int(rng.integers(0, n + 1))
+        prev = int(rng.integers(0, 8))
+        kr, ir = O.max_clique_heu_incremental(a, num_new, prev)
+        kr = kr if kr > prev else 0
+        k, idx = ctx.max_clique_incremental_bits(_capi.pack_bits(a), n, num_new, prev)
+        assert k == kr, (seed, t, n, num_new, prev, k, kr)
+        assert list(idx[:k]) == list(ir[:kr]), (seed, t)
+    print(f"[clique] incremental seed {seed}: 40 graphs bit-exact", flush=True)

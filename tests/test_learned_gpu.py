@@ -99,6 +99,12 @@ def test_lc_features_match_numpy(ctx):
     assert np.allclose(f[:, 2], np.log1p(deg), atol=1e-5)
     assert np.allclose(f[:, 3], (adj & member[None, :]).sum(1) / max(k, 1), atol=1e-6)
     assert np.array_equal(f[:, 4] > 0.5, member)
+    assert np.all(f[:, 6] == 0) and np.all(f[:, 7] == 0)  # no residuals given
+    resid = np.random.default_rng(3).uniform(0, 50, m)
+    fr = ctx.lc_features(idx[:k], resid, 2.0)
+    assert np.allclose(fr[:, 6], np.log1p(resid / 2.0), rtol=1e-6)
+    assert np.allclose(fr[:, 7], np.minimum(resid / 2.0, 10.0), rtol=1e-6)
+    assert np.array_equal(np.delete(fr, [6, 7], 1), np.delete(f, [6, 7], 1))
     # chi^2 of each loop closure at the chain poses
     R = traj[:, :9].reshape(-1, 3, 3)
     t = traj[:, 9:12]
