@@ -143,6 +143,48 @@ int rpgo_max_clique_incremental_bits(rpgo_ctx* ctx, int32_t n, const uint64_t* a
                                      int32_t num_new, int32_t prev_size, int32_t* idx_out,
                                      int32_t* k_out);
 
+/* Batched PCM: n_graphs independent single-prefix graphs of uneven size,
+ * packed by prefix offsets, one device launch per stage (the C4 batch).
+ * Graph g: poses [pose_off[g], pose_off[g+1]) (pose0 row g), odometry rows
+ * [pose_off[g] - g, pose_off[g+1] - g - 1), loop closures [lc_off[g],
+ * lc_off[g+1]) with pose indices local to the graph.  Every stage is the FP64
+ * replay of the single-graph path (bit-exact with it), so the adjacency of each
+ * graph equals rpgo_pcm_run's on that graph alone. */
+typedef struct {
+  int32_t dim;              /* 3 or 2                                         */
+  int32_t n_graphs;
+  const int64_t* pose_off;  /* n_graphs + 1, pose_off[0] = 0                  */
+  const double* pose0;      /* n_graphs * pose_len                            */
+  const double* odom;       /* (pose_off[n] - n) * pose_len                   */
+  const double* odom_cov;   /* (pose_off[n] - n) * d * d                      */
+  const int64_t* lc_off;    /* n_graphs + 1, lc_off[0] = 0                    */
+  const int32_t* lc_from;   /* lc_off[n], local indices                       */
+  const int32_t* lc_to;
+  const double* lc_pose;
+  const double* lc_cov;
+} rpgo_pcm_batch;
+
+typedef struct {
+  int64_t n_pairs;          /* sum over graphs of m (m - 1) / 2               */
+  int64_t n_consistent;     /* consistent pairs (lower triangle)              */
+  double ms_fold;           /* measurements + fold (all graphs)               */
+  double ms_odom;           /* odometry check + compaction                    */
+  double ms_pairs;          /* pair checks + symmetrize                       */
+  double ms_clique;         /* heuristic clique of every graph                */
+  double ms_total;
+} rpgo_pcm_batch_stats;
+
+/* Run the batch.  n_accepted: n_graphs counts; accepted: lc_off[n] entries,
+ * graph g's accepted local loop-closure indices at [lc_off[g], +n_accepted[g]);
+ * clique_size / clique_idx (may be NULL): FMC::maxCliqueHeu per graph, sizes
+ * per graph and members (indices into that graph's accepted set) at lc_off[g]. */
+int rpgo_pcm_batch_run(rpgo_ctx* ctx, const rpgo_pcm_batch* batch, const rpgo_pcm_params* params,
+                       int32_t* n_accepted, int32_t* accepted, int32_t* clique_size,
+                       int32_t* clique_idx, rpgo_pcm_batch_stats* stats);
+/* Adjacency of graph g of the last batch: n_accepted[g] rows of
+ * ceil(n_accepted[g] / 64) words. */
+int rpgo_pcm_batch_download_adj(rpgo_ctx* ctx, int32_t g, uint64_t* adj_bits);
+
 /* ------------------------------------------------------------------------
  * Robust solve: GNC-TLS over Levenberg-Marquardt / Gauss-Newton with a
  * block-Jacobi PCG linear step.  Replaces RobustSolver::optimize

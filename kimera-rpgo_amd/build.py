@@ -32,7 +32,7 @@ EXTRA = {
     "replay64.hip": ["-ffp-contract=off"],
 }
 SOURCES = ["capi.hip", "replay64.hip", "pcm_fp32.hip", "clique.hip", "solver.hip",
-           "solver_capi.hip", "learned.hip", "clique_exact.hip"]
+           "solver_capi.hip", "learned.hip", "clique_exact.hip", "batch.hip"]
 HOST = os.path.join(HERE, "host")
 HOST_SOURCES = ["Types.cpp", "Dataset.cpp", "Pcm.cpp", "RobustSolver.cpp"]
 CXX = os.environ.get("CXX", "g++")

@@ -1,0 +1,504 @@
+// Batched PCM: many independent single-prefix pose graphs of uneven size in
+// one launch per stage.  Graph g owns poses [pose_off[g], pose_off[g+1]),
+// odometry rows [pose_off[g] - g, pose_off[g+1] - g - 1) and loop closures
+// [lc_off[g], lc_off[g+1]).  Every stage is the single-graph stage of
+// replay64.hip (the FP64 replay of Pcm.h's fold, odometry check and pairwise
+// check, bit-exact with the oracle) with a graph lookup in front:
+//   measurements   one thread per odometry row / loop closure (all graphs)
+//   fold           one wavefront per graph
+//   odometry check one thread per loop closure
+//   compaction     one workgroup per graph (stable, input order)
+//   pairs          one wavefront per (row, 64-column word) tile; the tiles of
+//                  all graphs are numbered by a prefix sum over graphs (sized
+//                  by each graph's loop-closure count, rows beyond the
+//                  accepted count exit) and a wavefront finds its graph by
+//                  binary search
+//   symmetrize     one wavefront per 64 x 64 block, numbered the same way
+// The heuristic clique (FMC::maxCliqueHeu) then runs per graph on the
+// device-resident adjacency.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "replay64.h"
+#include "rpgo_internal.h"
+
+namespace rpgo {
+
+struct BatchState {
+  int dim = 0, n_graphs = 0;
+  long long n_poses = 0, n_lc = 0, n_odom = 0, adj_words = 0;
+  std::vector<long long> pose_off, lc_off, adj_off, tile_off, sym_off;
+  DevBuf d_pose_off, d_lc_off, d_adj_off, d_tile_off, d_sym_off, d_lc_graph;
+  DevBuf pose0, odom, odom_cov, lc_from, lc_to, lc_pose, lc_cov;
+  DevBuf delta, hinv, traj, lcm, odom_dist, flags, accepted, m, c_from, c_to, c_lcm, adj, counters;
+  std::vector<int> m_host;
+};
+
+// PoseWithCovariance records of the measurements (GeometryUtils.h:91-115) and
+// the fold Jacobians Ad(delta^-1), one thread per row of every graph
+template <int DIM>
+__global__ void k_meas_batch(long long n, const double* __restrict__ pose, const double* __restrict__ cov,
+                             double* __restrict__ pwc, double* __restrict__ hinv) {
+  constexpr int PL = R64<DIM>::PL, D = R64<DIM>::D, PWC = R64<DIM>::PWC;
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double rec[PWC];
+  R64<DIM>::from_measurement(pose + (size_t)k * PL, cov + (size_t)k * D * D, rec);
+  for (int i = 0; i < PWC; ++i) pwc[(size_t)k * PWC + i] = rec[i];
+  if (hinv) {
+    double inv[PL], H[D * D];
+    R64<DIM>::pose_inverse(rec, inv);
+    R64<DIM>::adjoint(inv, H);
+    for (int i = 0; i < D * D; ++i) hinv[(size_t)k * D * D + i] = H[i];
+  }
+}
+
+__device__ __forceinline__ int graph_of(const long long* __restrict__ off, int n, long long t) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(64) k_fold_batch(int n_graphs, const long long* __restrict__ pose_off,
+                                                  const double* __restrict__ pose0, const double* __restrict__ delta,
+                                                  const double* __restrict__ hinv, double* __restrict__ traj) {
+  constexpr int PL = R64<DIM>::PL, D = R64<DIM>::D, PWC = R64<DIM>::PWC, DD = D * D;
+  __shared__ double S[DD], T[DD], P[PL], Hs[DD], Dl[PWC];
+  __shared__ int ri;
+  const int g = blockIdx.x;
+  if (g >= n_graphs) return;
+  const long long p0 = pose_off[g];
+  const int n_poses = (int)(pose_off[g + 1] - p0);
+  const long long o0 = p0 - g;  // odometry rows of graph g
+  double* tr = traj + (size_t)p0 * PWC;
+  const int l = threadIdx.x;
+  if (l < DD) S[l] = 0.0;
+  if (l < PL) P[l] = pose0[(size_t)g * PL + l];
+  if (l == 0) ri = 1;
+  __syncthreads();
+  for (int e = l; e < PWC; e += 64) tr[e] = e < PL ? P[e] : (e < PL + DD ? 0.0 : 1.0);
+  for (int k = 0; k + 1 < n_poses; ++k) {
+    for (int e = l; e < DD; e += 64) Hs[e] = hinv[(size_t)(o0 + k) * DD + e];
+    for (int e = l; e < PWC; e += 64) Dl[e] = delta[(size_t)(o0 + k) * PWC + e];
+    __syncthreads();
+    if (l < DD) {
+      const int r = l / D, c = l % D;
+      double s = Hs[r * D] * S[c];
+#pragma unroll
+      for (int q = 1; q < D; ++q) s += Hs[r * D + q] * S[q * D + c];
+      T[l] = s;
+    }
+    double newpose = 0.0;
+    if (l >= DD && l < DD + PL) newpose = R64<DIM>::compose_entry(P, Dl, l - DD);
+    __syncthreads();
+    double* out = tr + (size_t)(k + 1) * PWC;
+    if (l < DD) {
+      const int r = l / D, c = l % D;
+      double s = T[r * D] * Hs[c * D];
+#pragma unroll
+      for (int q = 1; q < D; ++q) s += T[r * D + q] * Hs[c * D + q];
+      s = s + Dl[PL + l];
+      S[l] = s;
+      out[PL + l] = s;
+    }
+    if (l >= DD && l < DD + PL) {
+      P[l - DD] = newpose;
+      out[l - DD] = newpose;
+    }
+    if (l == 0) {
+      ri = (ri && Dl[PL + DD] != 0.0) ? 1 : 0;
+      out[PL + DD] = ri ? 1.0 : 0.0;
+    }
+    __syncthreads();
+  }
+}
+
+template <int DIM>
+__global__ void k_odom_batch(long long n_lc, const int* __restrict__ lc_graph, const long long* __restrict__ pose_off,
+                             const double* __restrict__ traj, const int* __restrict__ from,
+                             const int* __restrict__ to, const double* __restrict__ lcm, double thr,
+                             double* __restrict__ dist, int* __restrict__ flag) {
+  constexpr int PWC = R64<DIM>::PWC;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_lc) return;
+  const double* tr = traj + (size_t)pose_off[lc_graph[t]] * PWC;
+  const double d = R64<DIM>::odom_dist(tr + (size_t)from[t] * PWC, tr + (size_t)to[t] * PWC, lcm + (size_t)t * PWC);
+  dist[t] = d;
+  flag[t] = (thr < 0.0 || d < thr) ? 1 : 0;
+}
+
+// per-graph stable compaction + gather of the accepted loop closures (local indices)
+template <int DIM>
+__global__ void __launch_bounds__(1024) k_compact_batch(int n_graphs, const long long* __restrict__ lc_off,
+                                                        const int* __restrict__ flag, const int* __restrict__ from,
+                                                        const int* __restrict__ to, const double* __restrict__ lcm,
+                                                        int* __restrict__ accepted, int* __restrict__ m_out,
+                                                        int* __restrict__ cf, int* __restrict__ ct,
+                                                        double* __restrict__ clcm) {
+  constexpr int PWC = R64<DIM>::PWC;
+  __shared__ int wsum[16];
+  __shared__ int base;
+  const int g = blockIdx.x;
+  if (g >= n_graphs) return;
+  const long long o = lc_off[g];
+  const int n = (int)(lc_off[g + 1] - o);
+  if (threadIdx.x == 0) base = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int start = 0; start < n; start += 1024) {
+    const int k = start + threadIdx.x;
+    const int v = k < n ? flag[o + k] : 0;
+    int incl = v;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int u = __shfl_up(incl, off);
+      if (lane >= off) incl += u;
+    }
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    int pre = base;
+    for (int w = 0; w < wid; ++w) pre += wsum[w];
+    if (k < n && v) {
+      const int dst = pre + incl - 1;
+      accepted[o + dst] = k;
+      cf[o + dst] = from[o + k];
+      ct[o + dst] = to[o + k];
+      for (int e = 0; e < PWC; ++e) clcm[(size_t)(o + dst) * PWC + e] = lcm[(size_t)(o + k) * PWC + e];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int s = 0;
+      for (int w = 0; w < 16; ++w) s += wsum[w];
+      base += s;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) m_out[g] = base;
+}
+
+__device__ inline void tile_of_b(long long t, int* j, int* w) {
+  long long q = (long long)((sqrt(1.0 + (double)t / 8.0) - 1.0) * 0.5);
+  while (32ll * (q + 1) * (q + 2) <= t) ++q;
+  while (q > 0 && 32ll * q * (q + 1) > t) --q;
+  const long long rem = t - 32ll * q * (q + 1);
+  *j = (int)(64 * q + rem / (q + 1));
+  *w = (int)(rem % (q + 1));
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(256) k_pairs_batch(int n_graphs, long long n_tiles, const long long* __restrict__ tile_off,
+                                                     const long long* __restrict__ pose_off,
+                                                     const long long* __restrict__ lc_off,
+                                                     const long long* __restrict__ adj_off, const int* __restrict__ m_arr,
+                                                     const double* __restrict__ traj, const int* __restrict__ cf,
+                                                     const int* __restrict__ ct, const double* __restrict__ clcm,
+                                                     double thr, int all_consistent,
+                                                     unsigned long long* __restrict__ adj,
+                                                     unsigned long long* __restrict__ n_consistent) {
+  constexpr int PWC = R64<DIM>::PWC;
+  const long long tile = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (tile >= n_tiles) return;
+  const int lane = threadIdx.x & 63;
+  const int g = graph_of(tile_off, n_graphs, tile);
+  int j, w;
+  tile_of_b(tile - tile_off[g], &j, &w);
+  const int m = m_arr[g];
+  if (j >= m) return;
+  const long long o = lc_off[g];
+  const int W = (m + 63) / 64;  // rows of the accepted set, as the clique reads them
+  const double* tr = traj + (size_t)pose_off[g] * PWC;
+  const int i = w * 64 + lane;
+  const bool active = i < j;
+  double d = 0.0;
+  if (active && !all_consistent)
+    d = R64<DIM>::pair_dist(tr, cf[o + i], ct[o + i], clcm + (size_t)(o + i) * PWC, cf[o + j], ct[o + j],
+                            clcm + (size_t)(o + j) * PWC);
+  // lc_threshold < 0: every pair is consistent without a check (Pcm.h:673)
+  const bool cons = active && (all_consistent || d < thr);
+  const unsigned long long bits = __ballot(cons);
+  if (lane == 0) {
+    adj[adj_off[g] + (size_t)j * W + w] = bits;
+    if (bits) atomicAdd(n_consistent, (unsigned long long)__popcll(bits));
+  }
+}
+
+__global__ void __launch_bounds__(64) k_sym_batch(int n_graphs, long long n_blocks, const long long* __restrict__ sym_off,
+                                                  const long long* __restrict__ lc_off,
+                                                  const long long* __restrict__ adj_off, const int* __restrict__ m_arr,
+                                                  unsigned long long* __restrict__ adj) {
+  const long long b = blockIdx.x;
+  if (b >= n_blocks) return;
+  const int g = graph_of(sym_off, n_graphs, b);
+  const long long local = b - sym_off[g];
+  // lower-triangle block (J, Wd), Wd <= J, numbered row by row
+  int J = (int)((sqrt(8.0 * (double)local + 1.0) - 1.0) * 0.5);
+  while ((long long)(J + 1) * (J + 2) / 2 <= local) ++J;
+  while ((long long)J * (J + 1) / 2 > local) --J;
+  const int Wd = (int)(local - (long long)J * (J + 1) / 2);
+  const int m = m_arr[g];
+  if (64 * J >= m) return;  // block beyond the accepted rows
+  const int W = (m + 63) / 64;
+  unsigned long long* A = adj + adj_off[g];
+  const int lane = threadIdx.x;
+  const int row = 64 * J + lane;
+  const unsigned long long word = row < m ? A[(size_t)row * W + Wd] : 0ull;
+  unsigned long long mine = 0ull;
+  for (int c = 0; c < 64; ++c) {
+    const unsigned long long bb = __ballot((word >> c) & 1ull);
+    if (lane == c) mine = bb;
+  }
+  const int orow = 64 * Wd + lane;
+  if (orow < m) {
+    if (Wd == J) A[(size_t)orow * W + J] |= mine;
+    else A[(size_t)orow * W + J] = mine;
+  }
+}
+
+__global__ void k_zero_u64(long long n, unsigned long long* __restrict__ a) {
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
+    a[k] = 0ull;
+}
+
+void batch_destroy(rpgo_ctx* ctx) {
+  if (!ctx->batch) return;
+  BatchState& s = *ctx->batch;
+  for (DevBuf* b : {&s.d_pose_off, &s.d_lc_off, &s.d_adj_off, &s.d_tile_off, &s.d_sym_off, &s.d_lc_graph, &s.pose0,
+                    &s.odom, &s.odom_cov, &s.lc_from, &s.lc_to, &s.lc_pose, &s.lc_cov, &s.delta, &s.hinv, &s.traj,
+                    &s.lcm, &s.odom_dist, &s.flags, &s.accepted, &s.m, &s.c_from, &s.c_to, &s.c_lcm, &s.adj,
+                    &s.counters})
+    b->release();
+  delete ctx->batch;
+  ctx->batch = nullptr;
+}
+
+static int up(DevBuf& b, const void* h, size_t bytes, hipStream_t st) {
+  int rc;
+  if ((rc = b.reserve(bytes ? bytes : 8))) return rc;
+  if (bytes) RPGO_HIP_CHECK(hipMemcpyAsync(b.ptr, h, bytes, hipMemcpyHostToDevice, st));
+  return RPGO_OK;
+}
+
+}  // namespace rpgo
+
+using namespace rpgo;
+
+extern "C" {
+
+int rpgo_pcm_batch_run(rpgo_ctx* ctx, const rpgo_pcm_batch* B, const rpgo_pcm_params* prm, int32_t* n_accepted,
+                       int32_t* accepted, int32_t* clique_size, int32_t* clique_idx, rpgo_pcm_batch_stats* stats) {
+  if (!ctx || !B || !prm || B->n_graphs < 1 || !B->pose_off || !B->lc_off || !B->pose0) {
+    set_error("rpgo_pcm_batch_run: invalid argument");
+    return RPGO_ERR_ARG;
+  }
+  if (B->dim != 3 && B->dim != 2) {
+    set_error("rpgo_pcm_batch_run: dim must be 2 or 3");
+    return RPGO_ERR_ARG;
+  }
+  const int G = B->n_graphs, dim = B->dim;
+  const int PL = dim == 3 ? 12 : 4, D = dim == 3 ? 6 : 3, PWC = dim == 3 ? R64<3>::PWC : R64<2>::PWC;
+  // validate the layout on the host
+  if (B->pose_off[0] != 0 || B->lc_off[0] != 0) {
+    set_error("rpgo_pcm_batch_run: offsets must start at 0");
+    return RPGO_ERR_ARG;
+  }
+  for (int g = 0; g < G; ++g) {
+    const long long np = B->pose_off[g + 1] - B->pose_off[g], nl = B->lc_off[g + 1] - B->lc_off[g];
+    if (np < 1 || nl < 0) {
+      set_error("rpgo_pcm_batch_run: graph %d has %lld poses, %lld loop closures", g, np, nl);
+      return RPGO_ERR_ARG;
+    }
+    for (long long t = B->lc_off[g]; t < B->lc_off[g + 1]; ++t)
+      if (B->lc_from[t] < 0 || B->lc_from[t] >= np || B->lc_to[t] < 0 || B->lc_to[t] >= np) {
+        set_error("rpgo_pcm_batch_run: loop closure %lld of graph %d outside [0, %lld)", t - B->lc_off[g], g, np);
+        return RPGO_ERR_ARG;
+      }
+  }
+  RPGO_HIP_CHECK(hipSetDevice(ctx->device));
+  if (!ctx->batch) ctx->batch = new BatchState();
+  BatchState& s = *ctx->batch;
+  hipStream_t st = ctx->stream;
+  s.dim = dim;
+  s.n_graphs = G;
+  s.pose_off.assign(B->pose_off, B->pose_off + G + 1);
+  s.lc_off.assign(B->lc_off, B->lc_off + G + 1);
+  s.n_poses = s.pose_off[G];
+  s.n_lc = s.lc_off[G];
+  s.n_odom = s.n_poses - G;
+  // adjacency, tile and symmetrize-block offsets sized by each graph's loop-closure count
+  s.adj_off.assign(G + 1, 0);
+  s.tile_off.assign(G + 1, 0);
+  s.sym_off.assign(G + 1, 0);
+  for (int g = 0; g < G; ++g) {
+    const long long nl = s.lc_off[g + 1] - s.lc_off[g], W = (nl + 63) / 64, q = W;  // 64-row groups
+    s.adj_off[g + 1] = s.adj_off[g] + nl * W;
+    s.tile_off[g + 1] = s.tile_off[g] + 32ll * q * (q + 1);
+    s.sym_off[g + 1] = s.sym_off[g] + q * (q + 1) / 2;
+  }
+  std::vector<int> lc_graph((size_t)std::max<long long>(s.n_lc, 1));
+  for (int g = 0; g < G; ++g)
+    for (long long t = s.lc_off[g]; t < s.lc_off[g + 1]; ++t) lc_graph[t] = g;
+  int rc;
+  const size_t nlc = (size_t)s.n_lc, nod = (size_t)s.n_odom;
+  if ((rc = up(s.d_pose_off, s.pose_off.data(), sizeof(long long) * (G + 1), st)) ||
+      (rc = up(s.d_lc_off, s.lc_off.data(), sizeof(long long) * (G + 1), st)) ||
+      (rc = up(s.d_adj_off, s.adj_off.data(), sizeof(long long) * (G + 1), st)) ||
+      (rc = up(s.d_tile_off, s.tile_off.data(), sizeof(long long) * (G + 1), st)) ||
+      (rc = up(s.d_sym_off, s.sym_off.data(), sizeof(long long) * (G + 1), st)) ||
+      (rc = up(s.d_lc_graph, lc_graph.data(), sizeof(int) * nlc, st)) ||
+      (rc = up(s.pose0, B->pose0, sizeof(double) * PL * G, st)) ||
+      (rc = up(s.odom, B->odom, sizeof(double) * PL * nod, st)) ||
+      (rc = up(s.odom_cov, B->odom_cov, sizeof(double) * D * D * nod, st)) ||
+      (rc = up(s.lc_from, B->lc_from, sizeof(int) * nlc, st)) || (rc = up(s.lc_to, B->lc_to, sizeof(int) * nlc, st)) ||
+      (rc = up(s.lc_pose, B->lc_pose, sizeof(double) * PL * nlc, st)) ||
+      (rc = up(s.lc_cov, B->lc_cov, sizeof(double) * D * D * nlc, st)))
+    return rc;
+  const size_t nl1 = nlc ? nlc : 1, no1 = nod ? nod : 1;
+  if ((rc = s.delta.reserve(sizeof(double) * PWC * no1)) || (rc = s.hinv.reserve(sizeof(double) * D * D * no1)) ||
+      (rc = s.traj.reserve(sizeof(double) * PWC * (size_t)s.n_poses)) || (rc = s.lcm.reserve(sizeof(double) * PWC * nl1)) ||
+      (rc = s.odom_dist.reserve(sizeof(double) * nl1)) || (rc = s.flags.reserve(sizeof(int) * nl1)) ||
+      (rc = s.accepted.reserve(sizeof(int) * nl1)) || (rc = s.m.reserve(sizeof(int) * G)) ||
+      (rc = s.c_from.reserve(sizeof(int) * nl1)) || (rc = s.c_to.reserve(sizeof(int) * nl1)) ||
+      (rc = s.c_lcm.reserve(sizeof(double) * PWC * nl1)) ||
+      (rc = s.adj.reserve(sizeof(unsigned long long) * (size_t)std::max<long long>(s.adj_off[G], 1))) ||
+      (rc = s.counters.reserve(sizeof(unsigned long long) * 4)))
+    return rc;
+  RPGO_HIP_CHECK(hipMemsetAsync(s.counters.ptr, 0, sizeof(unsigned long long) * 4, st));
+  RPGO_HIP_CHECK(hipEventRecord(ctx->ev[0], st));
+  // measurements of every odometry row and loop closure
+  if (nod) {
+    if (dim == 3)
+      k_meas_batch<3><<<(unsigned)((nod + 255) / 256), 256, 0, st>>>((long long)nod, s.odom.as<double>(), s.odom_cov.as<double>(),
+                                                                     s.delta.as<double>(), s.hinv.as<double>());
+    else
+      k_meas_batch<2><<<(unsigned)((nod + 255) / 256), 256, 0, st>>>((long long)nod, s.odom.as<double>(), s.odom_cov.as<double>(),
+                                                                     s.delta.as<double>(), s.hinv.as<double>());
+  }
+  if (nlc) {
+    if (dim == 3)
+      k_meas_batch<3><<<(unsigned)((nlc + 255) / 256), 256, 0, st>>>((long long)nlc, s.lc_pose.as<double>(), s.lc_cov.as<double>(),
+                                                                     s.lcm.as<double>(), nullptr);
+    else
+      k_meas_batch<2><<<(unsigned)((nlc + 255) / 256), 256, 0, st>>>((long long)nlc, s.lc_pose.as<double>(), s.lc_cov.as<double>(),
+                                                                     s.lcm.as<double>(), nullptr);
+  }
+  // fold: one wavefront per graph
+  if (dim == 3)
+    k_fold_batch<3><<<G, 64, 0, st>>>(G, s.d_pose_off.as<long long>(), s.pose0.as<double>(), s.delta.as<double>(),
+                                      s.hinv.as<double>(), s.traj.as<double>());
+  else
+    k_fold_batch<2><<<G, 64, 0, st>>>(G, s.d_pose_off.as<long long>(), s.pose0.as<double>(), s.delta.as<double>(),
+                                      s.hinv.as<double>(), s.traj.as<double>());
+  RPGO_HIP_CHECK(hipEventRecord(ctx->ev[1], st));
+  if (nlc) {
+    if (dim == 3) {
+      k_odom_batch<3><<<(unsigned)((nlc + 255) / 256), 256, 0, st>>>(
+          (long long)nlc, s.d_lc_graph.as<int>(), s.d_pose_off.as<long long>(), s.traj.as<double>(), s.lc_from.as<int>(),
+          s.lc_to.as<int>(), s.lcm.as<double>(), prm->odom_threshold, s.odom_dist.as<double>(), s.flags.as<int>());
+      k_compact_batch<3><<<G, 1024, 0, st>>>(G, s.d_lc_off.as<long long>(), s.flags.as<int>(), s.lc_from.as<int>(),
+                                             s.lc_to.as<int>(), s.lcm.as<double>(), s.accepted.as<int>(), s.m.as<int>(),
+                                             s.c_from.as<int>(), s.c_to.as<int>(), s.c_lcm.as<double>());
+    } else {
+      k_odom_batch<2><<<(unsigned)((nlc + 255) / 256), 256, 0, st>>>(
+          (long long)nlc, s.d_lc_graph.as<int>(), s.d_pose_off.as<long long>(), s.traj.as<double>(), s.lc_from.as<int>(),
+          s.lc_to.as<int>(), s.lcm.as<double>(), prm->odom_threshold, s.odom_dist.as<double>(), s.flags.as<int>());
+      k_compact_batch<2><<<G, 1024, 0, st>>>(G, s.d_lc_off.as<long long>(), s.flags.as<int>(), s.lc_from.as<int>(),
+                                             s.lc_to.as<int>(), s.lcm.as<double>(), s.accepted.as<int>(), s.m.as<int>(),
+                                             s.c_from.as<int>(), s.c_to.as<int>(), s.c_lcm.as<double>());
+    }
+  } else {
+    RPGO_HIP_CHECK(hipMemsetAsync(s.m.ptr, 0, sizeof(int) * G, st));
+  }
+  RPGO_HIP_CHECK(hipGetLastError());
+  RPGO_HIP_CHECK(hipEventRecord(ctx->ev[2], st));
+  // pairs: all graphs' tiles in one launch; the lc_threshold < 0 case
+  // (Pcm.h:673) makes every pair consistent
+  const long long n_tiles = s.tile_off[G];
+  const long long adj_words = std::max<long long>(s.adj_off[G], 1);
+  k_zero_u64<<<(unsigned)std::min<long long>((adj_words + 255) / 256, 4096), 256, 0, st>>>(adj_words, s.adj.as<unsigned long long>());
+  const double thr = prm->lc_threshold;
+  const int all_cons = prm->lc_threshold < 0 ? 1 : 0;
+  if (n_tiles > 0) {
+    const unsigned blocks = (unsigned)((n_tiles + 3) / 4);
+    if (dim == 3)
+      k_pairs_batch<3><<<blocks, 256, 0, st>>>(G, n_tiles, s.d_tile_off.as<long long>(), s.d_pose_off.as<long long>(),
+                                               s.d_lc_off.as<long long>(), s.d_adj_off.as<long long>(), s.m.as<int>(),
+                                               s.traj.as<double>(), s.c_from.as<int>(), s.c_to.as<int>(),
+                                               s.c_lcm.as<double>(), thr, all_cons, s.adj.as<unsigned long long>(),
+                                               s.counters.as<unsigned long long>());
+    else
+      k_pairs_batch<2><<<blocks, 256, 0, st>>>(G, n_tiles, s.d_tile_off.as<long long>(), s.d_pose_off.as<long long>(),
+                                               s.d_lc_off.as<long long>(), s.d_adj_off.as<long long>(), s.m.as<int>(),
+                                               s.traj.as<double>(), s.c_from.as<int>(), s.c_to.as<int>(),
+                                               s.c_lcm.as<double>(), thr, all_cons, s.adj.as<unsigned long long>(),
+                                               s.counters.as<unsigned long long>());
+    const long long n_blocks = s.sym_off[G];
+    k_sym_batch<<<(unsigned)n_blocks, 64, 0, st>>>(G, n_blocks, s.d_sym_off.as<long long>(), s.d_lc_off.as<long long>(),
+                                                   s.d_adj_off.as<long long>(), s.m.as<int>(),
+                                                   s.adj.as<unsigned long long>());
+  }
+  RPGO_HIP_CHECK(hipGetLastError());
+  RPGO_HIP_CHECK(hipEventRecord(ctx->ev[3], st));
+  s.m_host.assign(G, 0);
+  RPGO_HIP_CHECK(hipMemcpyAsync(s.m_host.data(), s.m.ptr, sizeof(int) * G, hipMemcpyDeviceToHost, st));
+  unsigned long long cnt[4];
+  RPGO_HIP_CHECK(hipMemcpyAsync(cnt, s.counters.ptr, sizeof(cnt), hipMemcpyDeviceToHost, st));
+  RPGO_HIP_CHECK(hipStreamSynchronize(st));
+  long long pairs = 0;
+  for (int g = 0; g < G; ++g) pairs += (long long)s.m_host[g] * (s.m_host[g] - 1) / 2;
+  if (n_accepted) std::memcpy(n_accepted, s.m_host.data(), sizeof(int) * G);
+  if (accepted && nlc)
+    RPGO_HIP_CHECK(hipMemcpy(accepted, s.accepted.ptr, sizeof(int) * nlc, hipMemcpyDeviceToHost));
+  // heuristic clique per graph on the device-resident adjacency
+  double ms_clique = 0.0;
+  if (clique_size) {
+    RPGO_HIP_CHECK(hipEventRecord(ctx->ev[4], st));
+    for (int g = 0; g < G; ++g) {
+      int32_t k = -1;
+      const int m = s.m_host[g];
+      if ((rc = clique_run(ctx, reinterpret_cast<const uint64_t*>(s.adj.as<unsigned long long>() + s.adj_off[g]), m,
+                           RPGO_CLIQUE_HEU_REF, clique_idx ? clique_idx + s.lc_off[g] : nullptr, &k)))
+        return rc;
+      clique_size[g] = k;
+    }
+    RPGO_HIP_CHECK(hipEventRecord(ctx->ev[5], st));
+    RPGO_HIP_CHECK(hipEventSynchronize(ctx->ev[5]));
+    float msc = 0.f;
+    (void)hipEventElapsedTime(&msc, ctx->ev[4], ctx->ev[5]);
+    ms_clique = msc;
+  }
+  if (stats) {
+    float a = 0.f, b = 0.f, c = 0.f;
+    (void)hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]);
+    (void)hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]);
+    (void)hipEventElapsedTime(&c, ctx->ev[2], ctx->ev[3]);
+    stats->n_pairs = pairs;
+    stats->n_consistent = (long long)cnt[0];
+    stats->ms_fold = a;
+    stats->ms_odom = b;
+    stats->ms_pairs = c;
+    stats->ms_clique = ms_clique;
+    stats->ms_total = a + b + c + ms_clique;
+  }
+  return RPGO_OK;
+}
+
+int rpgo_pcm_batch_download_adj(rpgo_ctx* ctx, int32_t g, uint64_t* adj_bits) {
+  if (!ctx || !ctx->batch || g < 0 || g >= ctx->batch->n_graphs || !adj_bits) {
+    set_error("rpgo_pcm_batch_download_adj: invalid argument or no batch run");
+    return RPGO_ERR_ARG;
+  }
+  BatchState& s = *ctx->batch;
+  const long long words = s.adj_off[g + 1] - s.adj_off[g];
+  if (words > 0)
+    RPGO_HIP_CHECK(hipMemcpy(adj_bits, s.adj.as<unsigned long long>() + s.adj_off[g], sizeof(uint64_t) * words,
+                             hipMemcpyDeviceToHost));
+  return RPGO_OK;
+}
+
+}  // extern "C"

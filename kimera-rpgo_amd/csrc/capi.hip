@@ -89,6 +89,7 @@ int rpgo_ctx_destroy(rpgo_ctx* ctx) {
   for (DevBuf* b : {&q.adj, &q.deg, &q.dmask, &q.icc, &q.out, &q.scratch, &q.best}) b->release();
   solver_destroy(ctx);
   mlp_destroy(ctx);
+  batch_destroy(ctx);
   for (auto& ev : ctx->ev)
     if (ev) (void)hipEventDestroy(ev);
   (void)hipStreamDestroy(ctx->stream);

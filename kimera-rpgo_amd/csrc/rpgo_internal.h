@@ -94,6 +94,7 @@ struct CliqueState {
 
 struct SolverRun;  // solver.hip / rpgo_solver.h
 struct MlpState;   // learned.hip
+struct BatchState; // batch.hip
 
 }  // namespace rpgo
 
@@ -105,6 +106,7 @@ struct rpgo_ctx {
   rpgo::CliqueState clq;
   rpgo::SolverRun* solver = nullptr;  // created on the first rpgo_solve
   rpgo::MlpState* mlp = nullptr;      // created by rpgo_mlp_set_params
+  rpgo::BatchState* batch = nullptr;  // created by rpgo_pcm_batch_run
 };
 
 namespace rpgo {
@@ -130,6 +132,7 @@ constexpr int kRecheckQueueCap = 1 << 22;
 // solver_capi.hip
 void solver_destroy(rpgo_ctx* ctx);
 void mlp_destroy(rpgo_ctx* ctx);  // learned.hip
+void batch_destroy(rpgo_ctx* ctx);  // batch.hip
 int clique_incremental(rpgo_ctx* ctx, const uint64_t* d_adj_u64, int n, int num_new, int prev_size,
                        int32_t* idx_out, int32_t* k_out);
 // clique_exact.hip: exact maximum clique given a clique size known to exist

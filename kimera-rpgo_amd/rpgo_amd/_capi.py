@@ -58,6 +58,21 @@ GNC_COST = 0
 GNC_PROBABILITY = 1
 
 
+class PcmBatch(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int32), ("n_graphs", ctypes.c_int32), ("pose_off", ctypes.POINTER(ctypes.c_int64)),
+                ("pose0", _d), ("odom", _d), ("odom_cov", _d), ("lc_off", ctypes.POINTER(ctypes.c_int64)),
+                ("lc_from", _i32), ("lc_to", _i32), ("lc_pose", _d), ("lc_cov", _d)]
+
+
+class PcmBatchStats(ctypes.Structure):
+    _fields_ = [("n_pairs", ctypes.c_int64), ("n_consistent", ctypes.c_int64), ("ms_fold", ctypes.c_double),
+                ("ms_odom", ctypes.c_double), ("ms_pairs", ctypes.c_double), ("ms_clique", ctypes.c_double),
+                ("ms_total", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class SolveProblem(ctypes.Structure):
     _fields_ = [("n_poses", ctypes.c_int32), ("poses", _d), ("n_factors", ctypes.c_int32),
                 ("kind", _i32), ("key_i", _i32), ("key_j", _i32), ("measured", _d),
@@ -116,6 +131,9 @@ SIGNATURES = [
      ctypes.c_int),
     ("rpgo_max_clique_incremental_bits", [ctypes.c_void_p, ctypes.c_int32, _u64, ctypes.c_int32,
                                           ctypes.c_int32, _i32, _i32], ctypes.c_int),
+    ("rpgo_pcm_batch_run", [ctypes.c_void_p, ctypes.POINTER(PcmBatch), ctypes.POINTER(PcmParams), _i32, _i32,
+                            _i32, _i32, ctypes.POINTER(PcmBatchStats)], ctypes.c_int),
+    ("rpgo_pcm_batch_download_adj", [ctypes.c_void_p, ctypes.c_int32, _u64], ctypes.c_int),
     ("rpgo_lc_features", [ctypes.c_void_p, _i32, ctypes.c_int32, _d, ctypes.c_double, _f], ctypes.c_int),
     ("rpgo_solve_errors", [ctypes.c_void_p, _d, _d], ctypes.c_int),
     ("rpgo_gnc_threshold", [ctypes.c_double, ctypes.c_int32], ctypes.c_double),
@@ -284,6 +302,52 @@ class Context:
         check(lib().rpgo_solve(self._h, ctypes.byref(prob), ctypes.byref(prm), _ptr(out), _ptr(w),
                                ctypes.byref(rep)))
         return out, w[:F].copy(), rep
+
+    # ------------------------------------------------------ batched PCM
+    def pcm_batch_run(self, graphs, odom_thr, lc_thr, clique=True):
+        """graphs: sequence of synth.PoseGraph (same dim).  Returns dict with per-graph
+        accepted lists, clique sizes / members and the batch stats."""
+        dim = graphs[0].dim
+        G = len(graphs)
+        npz = np.array([g.n_poses for g in graphs], np.int64)
+        nlc = np.array([g.n_lc for g in graphs], np.int64)
+        pose_off = np.concatenate([[0], np.cumsum(npz)]).astype(np.int64)
+        lc_off = np.concatenate([[0], np.cumsum(nlc)]).astype(np.int64)
+        cat = lambda xs, dt: np.ascontiguousarray(np.concatenate(xs) if len(xs) else np.zeros(0), dtype=dt)
+        arrs = dict(pose0=cat([np.asarray(g.pose0).reshape(1, -1) for g in graphs], np.float64),
+                    odom=cat([np.asarray(g.odom).reshape(g.n_poses - 1, -1) for g in graphs], np.float64),
+                    odom_cov=cat([np.asarray(g.odom_cov).reshape(g.n_poses - 1, -1) for g in graphs], np.float64),
+                    lc_from=cat([g.lc_from for g in graphs], np.int32), lc_to=cat([g.lc_to for g in graphs], np.int32),
+                    lc_pose=cat([np.asarray(g.lc_pose).reshape(g.n_lc, -1) for g in graphs], np.float64),
+                    lc_cov=cat([np.asarray(g.lc_cov).reshape(g.n_lc, -1) for g in graphs], np.float64))
+        b = PcmBatch(dim=dim, n_graphs=G, pose_off=pose_off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                     pose0=_ptr(arrs["pose0"]), odom=_ptr(arrs["odom"]), odom_cov=_ptr(arrs["odom_cov"]),
+                     lc_off=lc_off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                     lc_from=_ptr(arrs["lc_from"], _i32), lc_to=_ptr(arrs["lc_to"], _i32),
+                     lc_pose=_ptr(arrs["lc_pose"]), lc_cov=_ptr(arrs["lc_cov"]))
+        prm = PcmParams(odom_threshold=odom_thr, lc_threshold=lc_thr, precision=PCM_FP64_REPLAY, want_dist=0)
+        n_acc = np.zeros(G, np.int32)
+        acc = np.zeros(max(int(lc_off[-1]), 1), np.int32)
+        ksz = np.zeros(G, np.int32)
+        kid = np.zeros(max(int(lc_off[-1]), 1), np.int32)
+        st = PcmBatchStats()
+        check(lib().rpgo_pcm_batch_run(self._h, ctypes.byref(b), ctypes.byref(prm), _ptr(n_acc, _i32),
+                                       _ptr(acc, _i32), _ptr(ksz, _i32) if clique else None,
+                                       _ptr(kid, _i32) if clique else None, ctypes.byref(st)))
+        out = {"n_accepted": n_acc, "stats": st,
+               "accepted": [acc[lc_off[g]:lc_off[g] + n_acc[g]].copy() for g in range(G)]}
+        if clique:
+            out["clique_size"] = ksz
+            out["clique_idx"] = [kid[lc_off[g]:lc_off[g] + max(ksz[g], 0)].copy() for g in range(G)]
+        self._batch_m = n_acc
+        return out
+
+    def pcm_batch_adj(self, g):
+        m = int(self._batch_m[g])
+        W = (m + 63) // 64
+        a = np.zeros(max(m * W, 1), np.uint64)
+        check(lib().rpgo_pcm_batch_download_adj(self._h, int(g), _ptr(a, _u64)))
+        return a[: m * W].reshape(m, W) if m else np.zeros((0, 0), np.uint64)
 
     # ------------------------------------------------------ learned initialisation
     def lc_features(self, clique_idx, lc_resid=None, barc=1.0):
