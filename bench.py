@@ -97,6 +97,41 @@ def robust_solve(ctx, gl):
     return res
 
 
+def batch_c4(ctx, seed=0, n_graphs=256, steps=3):
+    """C4 batch: 256 independent SE(3) graphs, poses log-uniform in [200, 20000],
+    loop closures 0.1-0.3 of the poses, outliers 5-40 %, odometry check off and
+    lc_threshold 5.0.  One launch per stage (fold, odometry check, compaction,
+    pairs, symmetrize, clique of every graph), captured in a hipGraph, launch
+    shapes picked by the autotuner on the first call.  Device times."""
+    from rpgo_amd import _capi, synth
+
+    rng = np.random.default_rng(seed)
+    graphs = []
+    for k in range(n_graphs):
+        n = int(np.exp(rng.uniform(np.log(200), np.log(20000))))
+        graphs.append(synth.se3_helix(n_poses=n, n_lc=max(1, int(n * rng.uniform(0.1, 0.3))),
+                                      outlier_frac=float(rng.uniform(0.05, 0.4)), seed=seed * 1000 + k))
+    flags = _capi.BATCH_GRAPH | _capi.BATCH_AUTOTUNE
+    first = ctx.pcm_batch_run(graphs, -1.0, 5.0, flags=flags)
+    staged = ctx.pcm_batch_run(graphs, -1.0, 5.0)
+    t0 = time.perf_counter()
+    reps = [ctx.pcm_batch_run(graphs, -1.0, 5.0, flags=flags) for _ in range(steps)]
+    wall = (time.perf_counter() - t0) / steps
+    ms = float(np.mean([r["stats"].ms_total for r in reps]))
+    st = reps[-1]["stats"]
+    same = all(np.array_equal(reps[-1]["clique_size"], r["clique_size"]) for r in (first, staged))
+    return {"graphs": n_graphs, "poses": int(sum(g.n_poses for g in graphs)),
+            "loop_closures": int(sum(g.n_lc for g in graphs)), "pairs": int(st.n_pairs),
+            "ms_per_batch_device": ms, "ms_per_batch_wall": wall * 1e3,
+            "pair_checks_per_s": st.n_pairs / (ms * 1e-3),
+            "stages_ms": {"fold": staged["stats"].ms_fold, "odom": staged["stats"].ms_odom,
+                          "pairs": staged["stats"].ms_pairs, "clique": staged["stats"].ms_clique,
+                          "total_staged": staged["stats"].ms_total},
+            "tuned": {"pair_waves": st.pair_waves, "clique_waves": st.clique_waves,
+                      "ms_tune": first["stats"].ms_tune},
+            "graph_replayed": bool(st.graph_replayed), "results_identical_across_modes": bool(same)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +144,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-solve", dest="solve", action="store_false",
                     help="skip the robust-solve measurement on the 50,000-pose graph")
+    ap.add_argument("--no-batch", dest="batch", action="store_false",
+                    help="skip the 256-graph batch measurement")
     ap.add_argument("--seed", type=int, default=0)
     a = ap.parse_args()
     if a.gpus != 1:
@@ -193,6 +230,8 @@ def main():
             "clique_heu_ref": {"size": kl, "ms": msl}}
         if a.solve:
             out["robust_solve_50k"] = robust_solve(ctx, gl)
+    if a.batch:
+        out["batch_c4"] = batch_c4(ctx, seed=a.seed)
     ctx.close()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(g, w["odom_thr"], w["lc_thr"])

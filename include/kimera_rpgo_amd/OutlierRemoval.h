@@ -8,6 +8,15 @@
 // consistency graph (FMC::maxCliqueHeu) is the inlier set.  The pairwise
 // matrix and the clique run on the GPU (rpgo_pcm_adjacency, rpgo_max_clique).
 //
+// PcmSimple2D/3D (Pcm<poseT, PoseWithNode>, Pcm.h:1169-1170) replace both
+// distances with the per-node translation and rotation norms of
+// PoseWithNode (GeometryUtils.h:193-289) against odom_trans/rot_threshold
+// and dist_trans/rot_threshold (rpgo_pcm_simple_adjacency).  As in the
+// reference constructor (Pcm.h:74-82) a negative odom_threshold,
+// odom_trans_threshold or odom_rot_threshold turns the odometry check off
+// and a negative lc_threshold, dist_trans_threshold or dist_rot_threshold
+// the pairwise check (every accepted loop closure is then an inlier).
+//
 // Scope: one robot prefix (the intra-robot group of the reference).
 #pragma once
 
@@ -54,9 +63,10 @@ class OutlierRemoval {
 
 class Pcm : public OutlierRemoval {
  public:
-  // dim 3: Pose3 (Pcm3D), dim 2: Pose2 (Pcm2D)
+  // dim 3: Pose3 (Pcm3D), dim 2: Pose2 (Pcm2D); simple: PoseWithNode
+  // distances (PcmSimple3D / PcmSimple2D)
   Pcm(int dim, const PcmParams& params, const std::vector<char>& special_symbols = {},
-      const DeviceParams& device = DeviceParams());
+      const DeviceParams& device = DeviceParams(), bool simple = false);
   ~Pcm() override;
   Pcm(const Pcm&) = delete;
   Pcm& operator=(const Pcm&) = delete;
@@ -87,6 +97,8 @@ class Pcm : public OutlierRemoval {
   void check(int rc, const char* what) const;
 
   int dim_;
+  bool simple_;
+  bool odom_check_, loop_check_;  // Pcm.h:74-82
   PcmParams params_;
   std::vector<char> special_symbols_;
   DeviceParams device_;
@@ -115,6 +127,20 @@ class Pcm2D : public Pcm {
  public:
   explicit Pcm2D(const PcmParams& p, const std::vector<char>& special = {}, const DeviceParams& d = DeviceParams())
       : Pcm(2, p, special, d) {}
+};
+
+class PcmSimple3D : public Pcm {
+ public:
+  explicit PcmSimple3D(const PcmParams& p, const std::vector<char>& special = {},
+                       const DeviceParams& d = DeviceParams())
+      : Pcm(3, p, special, d, true) {}
+};
+
+class PcmSimple2D : public Pcm {
+ public:
+  explicit PcmSimple2D(const PcmParams& p, const std::vector<char>& special = {},
+                       const DeviceParams& d = DeviceParams())
+      : Pcm(2, p, special, d, true) {}
 };
 
 }  // namespace kimera_rpgo_amd

@@ -20,6 +20,10 @@
  *                     include/KimeraRPGO/outlier/Pcm.h:414-502 with updateOdom
  *                     :516-557, isOdomConsistent :604-629, areLoopsConsistent
  *                     :670-718 and incrementAdjMatrix :725-768
+ *   rpgo_pcm_simple_* the same stages of Pcm<poseT,PoseWithNode> (PcmSimple2D/3D,
+ *                     Pcm.h:1169-1170): checkOdomConsistent :581-596,
+ *                     checkLoopConsistent :654-662, PoseWithNode
+ *                     include/KimeraRPGO/utils/GeometryUtils.h:193-289
  *   rpgo_max_clique*  findMaxCliqueHeu src/utils/GraphUtils.cpp:19-27
  *                     (FMC::maxCliqueHeu findCliqueHeu.cpp:32-117) and the
  *                     exact FMC::maxClique findClique.cpp:80-145
@@ -125,6 +129,70 @@ int rpgo_pcm_adjacency(rpgo_ctx* ctx, const rpgo_pcm_problem* prob,
  * doubles per pose (pose, covariance, rotation_info flag). */
 int rpgo_pcm_download_traj(rpgo_ctx* ctx, double* traj);
 
+/* PcmSimple: the same stages with the PoseWithNode distances
+ * (Pcm<poseT, PoseWithNode>, GeometryUtils.h:193-289, Pcm.h:581-596,654-662).
+ * The odometry check passes a loop closure (i, j) when the per-node
+ * translation and rotation norms of getBetween(i,j) * LC^-1 (node |j-i|+1)
+ * are below odom_*; a pair passes when those of the loop
+ * (a->c * LC_cd)^-1 * LC_ab * (b->d) (node |c-a|+|d-b|+2) are below dist_*.
+ * The norms are |tail(translation size)| and |head(rotation dim)| of Logmap
+ * divided by the node count (for Pose2: |(v_y, theta)| and |v_x|); the
+ * rotation norm is 0 without rotation information (NaN rotation variance).
+ * A negative threshold in a pair disables that check (Pcm.h:74-82: every
+ * loop closure passes / no adjacency; the caller keeps every accepted loop
+ * closure as an inlier).  Covariances only feed the rotation_info flags.
+ * Results are read with rpgo_pcm_download (odom_dist and dist hold the
+ * translation norms) and rpgo_max_clique. */
+typedef struct {
+  double odom_trans_threshold; /* PcmParams::odom_trans_threshold */
+  double odom_rot_threshold;   /* PcmParams::odom_rot_threshold   */
+  double dist_trans_threshold; /* PcmParams::dist_trans_threshold */
+  double dist_rot_threshold;   /* PcmParams::dist_rot_threshold   */
+  int32_t want_dist;           /* keep the m*m translation-norm matrix */
+  int32_t reserved;
+} rpgo_pcm_simple_params;
+
+int rpgo_pcm_simple_run(rpgo_ctx* ctx, const rpgo_pcm_simple_params* params,
+                        rpgo_pcm_stats* stats);
+/* upload + rpgo_pcm_simple_run + download in one call. */
+int rpgo_pcm_simple_adjacency(rpgo_ctx* ctx, const rpgo_pcm_problem* prob,
+                              const rpgo_pcm_simple_params* params, int32_t* accepted,
+                              double* odom_dist, uint64_t* adj_bits, double* dist,
+                              rpgo_pcm_stats* stats);
+
+/* Streaming: one reference update() after another (Pcm::removeOutliers
+ * Pcm.h:148-281 called once per increment, parseAndIncrementAdjMatrix
+ * :414-502, incrementAdjMatrix :725-768).  Starts from the state of a full
+ * rpgo_pcm_run / rpgo_pcm_simple_run (which may hold a single pose and no
+ * loop closure).  Each call appends n_odom odometry factors at the chain end
+ * (poses n_poses .. n_poses+n_odom-1, folded from the stored last record),
+ * odometry-checks the new loop closures against the extended chain and adds
+ * the accepted ones, in input order, as new adjacency rows checked against
+ * every earlier accepted loop closure: O(m) pairs per new loop closure
+ * instead of a full O(m^2) pass.  The result equals a full run over the
+ * concatenated problem bit for bit.  Pairs are evaluated in FP64
+ * (params->precision is ignored); want_dist must be 0.  Loop-closure indices
+ * in rpgo_pcm_download's accepted list count every loop closure of the
+ * stream in arrival order.  Stats: n_accepted = total accepted; n_pairs and
+ * n_consistent count this call's pairs.  Follow with rpgo_max_clique, or
+ * rpgo_max_clique_incremental (num_new = accepted loop closures of this call)
+ * for PcmParams::incremental. */
+typedef struct {
+  int32_t n_odom;          /* odometry factors appended at the chain end          */
+  const double* odom;      /* n_odom * pose_len                                   */
+  const double* odom_cov;  /* n_odom * d*d                                        */
+  int32_t n_lc;            /* new loop closures, endpoints in the extended chain  */
+  const int32_t* lc_from;
+  const int32_t* lc_to;
+  const double* lc_pose;   /* n_lc * pose_len */
+  const double* lc_cov;    /* n_lc * d*d      */
+} rpgo_pcm_increment;
+
+int rpgo_pcm_append(rpgo_ctx* ctx, const rpgo_pcm_increment* inc, const rpgo_pcm_params* params,
+                    rpgo_pcm_stats* stats);
+int rpgo_pcm_simple_append(rpgo_ctx* ctx, const rpgo_pcm_increment* inc,
+                           const rpgo_pcm_simple_params* params, rpgo_pcm_stats* stats);
+
 /* Max clique over the adjacency of the last rpgo_pcm_run (device resident).
  * idx_out: room for m ints; *k_out: clique size (FMC returns -1 for m == 0). */
 int rpgo_max_clique(rpgo_ctx* ctx, int32_t mode, int32_t* idx_out, int32_t* k_out,
@@ -162,7 +230,17 @@ typedef struct {
   const int32_t* lc_to;
   const double* lc_pose;
   const double* lc_cov;
+  int32_t flags;            /* RPGO_BATCH_* bits (0: stages launched one by one,
+                               per-stage times in the stats)                  */
 } rpgo_pcm_batch;
+
+/* rpgo_pcm_batch.flags */
+#define RPGO_BATCH_GRAPH 1    /* capture every stage (measurements .. clique) in one
+                                 hipGraph, replayed while the batch layout (sizes,
+                                 launch shapes) is unchanged; stats give ms_total only */
+#define RPGO_BATCH_AUTOTUNE 2 /* before the first run of a dim on this context, time
+                                 the candidate launch shapes of the pair and clique
+                                 stages on this batch and keep the fastest          */
 
 typedef struct {
   int64_t n_pairs;          /* sum over graphs of m (m - 1) / 2               */
@@ -171,13 +249,21 @@ typedef struct {
   double ms_odom;           /* odometry check + compaction                    */
   double ms_pairs;          /* pair checks + symmetrize                       */
   double ms_clique;         /* heuristic clique of every graph                */
-  double ms_total;
+  double ms_total;          /* device time of the whole batch                 */
+  double ms_tune;           /* time spent by RPGO_BATCH_AUTOTUNE in this call  */
+  int32_t pair_waves;       /* wavefronts per workgroup of the pair stage      */
+  int32_t clique_waves;     /* wavefronts per graph of the clique stage        */
+  int32_t graph_replayed;   /* 1 when an existing hipGraph was replayed        */
+  int32_t reserved;
 } rpgo_pcm_batch_stats;
 
 /* Run the batch.  n_accepted: n_graphs counts; accepted: lc_off[n] entries,
  * graph g's accepted local loop-closure indices at [lc_off[g], +n_accepted[g]);
  * clique_size / clique_idx (may be NULL): FMC::maxCliqueHeu per graph, sizes
- * per graph and members (indices into that graph's accepted set) at lc_off[g]. */
+ * per graph and members (indices into that graph's accepted set) at lc_off[g].
+ * The clique stage runs on the device for every graph in one launch (one
+ * workgroup per graph); graphs with more than 32768 accepted loop closures are
+ * rejected.  Blocks until the results are on the host. */
 int rpgo_pcm_batch_run(rpgo_ctx* ctx, const rpgo_pcm_batch* batch, const rpgo_pcm_params* params,
                        int32_t* n_accepted, int32_t* accepted, int32_t* clique_size,
                        int32_t* clique_idx, rpgo_pcm_batch_stats* stats);

@@ -27,9 +27,12 @@ COMMON = [
     "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
     "-Wall", "-Wno-unused-result",
 ]
-# Per-source extra flags: the FP64 replay must round exactly as written.
+# Per-source extra flags: the FP64 replay (replay64.h device functions) must
+# round exactly as written in every translation unit that decides adjacency
+# bits with it.
 EXTRA = {
     "replay64.hip": ["-ffp-contract=off"],
+    "batch.hip": ["-ffp-contract=off"],
 }
 SOURCES = ["capi.hip", "replay64.hip", "pcm_fp32.hip", "clique.hip", "solver.hip",
            "solver_capi.hip", "learned.hip", "clique_exact.hip", "batch.hip"]

@@ -15,9 +15,14 @@
 //                  binary search
 //   symmetrize     one wavefront per 64 x 64 block, numbered the same way
 // The heuristic clique (FMC::maxCliqueHeu) then runs per graph on the
-// device-resident adjacency.
+// device-resident adjacency.  Like replay64.hip this unit is built with
+// -ffp-contract=off: a contracted multiply-add would round the fold and the
+// between covariances (a difference of nearly equal matrices whose LLT test
+// picks the formula) differently from the single-graph path.
 #include <algorithm>
+#include <climits>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <vector>
 
@@ -33,7 +38,14 @@ struct BatchState {
   DevBuf d_pose_off, d_lc_off, d_adj_off, d_tile_off, d_sym_off, d_lc_graph;
   DevBuf pose0, odom, odom_cov, lc_from, lc_to, lc_pose, lc_cov;
   DevBuf delta, hinv, traj, lcm, odom_dist, flags, accepted, m, c_from, c_to, c_lcm, adj, counters;
+  DevBuf deg, cidx, ck;  // clique stage: degrees, members, sizes
   std::vector<int> m_host;
+  // launch shapes chosen by the autotuner, per dim (0: not tuned yet)
+  int tuned_pair[4] = {0, 0, 0, 0}, tuned_clique[4] = {0, 0, 0, 0};
+  // captured stages and the layout they were captured for
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<long long> graph_key;
 };
 
 // PoseWithCovariance records of the measurements (GeometryUtils.h:91-115) and
@@ -191,8 +203,8 @@ __device__ inline void tile_of_b(long long t, int* j, int* w) {
   *w = (int)(rem % (q + 1));
 }
 
-template <int DIM>
-__global__ void __launch_bounds__(256) k_pairs_batch(int n_graphs, long long n_tiles, const long long* __restrict__ tile_off,
+template <int DIM, int WPB>
+__global__ void __launch_bounds__(64 * WPB) k_pairs_batch(int n_graphs, long long n_tiles, const long long* __restrict__ tile_off,
                                                      const long long* __restrict__ pose_off,
                                                      const long long* __restrict__ lc_off,
                                                      const long long* __restrict__ adj_off, const int* __restrict__ m_arr,
@@ -202,7 +214,7 @@ __global__ void __launch_bounds__(256) k_pairs_batch(int n_graphs, long long n_t
                                                      unsigned long long* __restrict__ adj,
                                                      unsigned long long* __restrict__ n_consistent) {
   constexpr int PWC = R64<DIM>::PWC;
-  const long long tile = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const long long tile = (long long)blockIdx.x * WPB + (threadIdx.x >> 6);
   if (tile >= n_tiles) return;
   const int lane = threadIdx.x & 63;
   const int g = graph_of(tile_off, n_graphs, tile);
@@ -265,14 +277,23 @@ __global__ void k_zero_u64(long long n, unsigned long long* __restrict__ a) {
     a[k] = 0ull;
 }
 
+// zero fill as a kernel: the stages may be captured in a hipGraph, where
+// memset nodes are avoided
+static void zero_u64(unsigned long long* a, long long n, hipStream_t st) {
+  if (n <= 0) return;
+  k_zero_u64<<<(unsigned)std::min<long long>((n + 255) / 256, 4096), 256, 0, st>>>(n, a);
+}
+
 void batch_destroy(rpgo_ctx* ctx) {
   if (!ctx->batch) return;
   BatchState& s = *ctx->batch;
   for (DevBuf* b : {&s.d_pose_off, &s.d_lc_off, &s.d_adj_off, &s.d_tile_off, &s.d_sym_off, &s.d_lc_graph, &s.pose0,
                     &s.odom, &s.odom_cov, &s.lc_from, &s.lc_to, &s.lc_pose, &s.lc_cov, &s.delta, &s.hinv, &s.traj,
                     &s.lcm, &s.odom_dist, &s.flags, &s.accepted, &s.m, &s.c_from, &s.c_to, &s.c_lcm, &s.adj,
-                    &s.counters})
+                    &s.counters, &s.deg, &s.cidx, &s.ck})
     b->release();
+  if (s.exec) (void)hipGraphExecDestroy(s.exec);
+  if (s.graph) (void)hipGraphDestroy(s.graph);
   delete ctx->batch;
   ctx->batch = nullptr;
 }
@@ -281,6 +302,178 @@ static int up(DevBuf& b, const void* h, size_t bytes, hipStream_t st) {
   int rc;
   if ((rc = b.reserve(bytes ? bytes : 8))) return rc;
   if (bytes) RPGO_HIP_CHECK(hipMemcpyAsync(b.ptr, h, bytes, hipMemcpyHostToDevice, st));
+  return RPGO_OK;
+}
+
+constexpr int kDefaultPairWaves = 4;
+constexpr int kDefaultCliqueWaves = 16;
+
+template <int DIM>
+static void launch_pairs_dim(rpgo_ctx* ctx, BatchState& s, double thr, int all_cons, int pw) {
+  hipStream_t st = ctx->stream;
+  const long long n_tiles = s.tile_off[s.n_graphs];
+  const unsigned blocks = (unsigned)((n_tiles + pw - 1) / pw);
+#define RPGO_PAIRS(WPB)                                                                                       \
+  k_pairs_batch<DIM, WPB><<<blocks, 64 * WPB, 0, st>>>(                                                      \
+      s.n_graphs, n_tiles, s.d_tile_off.as<long long>(), s.d_pose_off.as<long long>(),                        \
+      s.d_lc_off.as<long long>(), s.d_adj_off.as<long long>(), s.m.as<int>(), s.traj.as<double>(),            \
+      s.c_from.as<int>(), s.c_to.as<int>(), s.c_lcm.as<double>(), thr, all_cons, s.adj.as<unsigned long long>(), \
+      s.counters.as<unsigned long long>())
+  switch (pw) {
+    case 1: RPGO_PAIRS(1); break;
+    case 2: RPGO_PAIRS(2); break;
+    case 8: RPGO_PAIRS(8); break;
+    default: RPGO_PAIRS(4); break;
+  }
+#undef RPGO_PAIRS
+}
+
+// pair checks + symmetrize over every graph's tiles (adjacency zeroed first)
+static int launch_pairs(rpgo_ctx* ctx, BatchState& s, const rpgo_pcm_params* prm, int pw) {
+  hipStream_t st = ctx->stream;
+  const int G = s.n_graphs;
+  const long long n_tiles = s.tile_off[G];
+  zero_u64(s.adj.as<unsigned long long>(), std::max<long long>(s.adj_off[G], 1), st);
+  zero_u64(s.counters.as<unsigned long long>(), 4, st);
+  // lc_threshold < 0: every pair is consistent without a check (Pcm.h:673)
+  const int all_cons = prm->lc_threshold < 0 ? 1 : 0;
+  if (n_tiles > 0) {
+    if (s.dim == 3) launch_pairs_dim<3>(ctx, s, prm->lc_threshold, all_cons, pw);
+    else launch_pairs_dim<2>(ctx, s, prm->lc_threshold, all_cons, pw);
+    const long long n_blocks = s.sym_off[G];
+    k_sym_batch<<<(unsigned)n_blocks, 64, 0, st>>>(G, n_blocks, s.d_sym_off.as<long long>(), s.d_lc_off.as<long long>(),
+                                                   s.d_adj_off.as<long long>(), s.m.as<int>(),
+                                                   s.adj.as<unsigned long long>());
+  }
+  RPGO_HIP_CHECK(hipGetLastError());
+  return RPGO_OK;
+}
+
+// Every stage of the batch, in stream order, with no host synchronisation
+// and no memset/memcpy node (capturable).  events: record ev[0..3], ev[5]
+// between the stages for per-stage times.
+static int launch_stages(rpgo_ctx* ctx, BatchState& s, const rpgo_pcm_params* prm, int pw, int cw, bool events) {
+  hipStream_t st = ctx->stream;
+  hipEvent_t* ev = ctx->ev;
+  const int G = s.n_graphs, dim = s.dim;
+  const size_t nlc = (size_t)s.n_lc, nod = (size_t)s.n_odom;
+  if (events) RPGO_HIP_CHECK(hipEventRecord(ev[0], st));
+  // measurements of every odometry row and loop closure
+  if (nod) {
+    if (dim == 3)
+      k_meas_batch<3><<<(unsigned)((nod + 255) / 256), 256, 0, st>>>((long long)nod, s.odom.as<double>(), s.odom_cov.as<double>(),
+                                                                     s.delta.as<double>(), s.hinv.as<double>());
+    else
+      k_meas_batch<2><<<(unsigned)((nod + 255) / 256), 256, 0, st>>>((long long)nod, s.odom.as<double>(), s.odom_cov.as<double>(),
+                                                                     s.delta.as<double>(), s.hinv.as<double>());
+  }
+  if (nlc) {
+    if (dim == 3)
+      k_meas_batch<3><<<(unsigned)((nlc + 255) / 256), 256, 0, st>>>((long long)nlc, s.lc_pose.as<double>(), s.lc_cov.as<double>(),
+                                                                     s.lcm.as<double>(), nullptr);
+    else
+      k_meas_batch<2><<<(unsigned)((nlc + 255) / 256), 256, 0, st>>>((long long)nlc, s.lc_pose.as<double>(), s.lc_cov.as<double>(),
+                                                                     s.lcm.as<double>(), nullptr);
+  }
+  // fold: one wavefront per graph
+  if (dim == 3)
+    k_fold_batch<3><<<G, 64, 0, st>>>(G, s.d_pose_off.as<long long>(), s.pose0.as<double>(), s.delta.as<double>(),
+                                      s.hinv.as<double>(), s.traj.as<double>());
+  else
+    k_fold_batch<2><<<G, 64, 0, st>>>(G, s.d_pose_off.as<long long>(), s.pose0.as<double>(), s.delta.as<double>(),
+                                      s.hinv.as<double>(), s.traj.as<double>());
+  RPGO_HIP_CHECK(hipGetLastError());
+  if (events) RPGO_HIP_CHECK(hipEventRecord(ev[1], st));
+  // odometry check (one thread per loop closure) and per-graph compaction;
+  // the compaction also writes m[g] = 0 for graphs without loop closures
+  if (nlc) {
+    if (dim == 3)
+      k_odom_batch<3><<<(unsigned)((nlc + 255) / 256), 256, 0, st>>>(
+          (long long)nlc, s.d_lc_graph.as<int>(), s.d_pose_off.as<long long>(), s.traj.as<double>(), s.lc_from.as<int>(),
+          s.lc_to.as<int>(), s.lcm.as<double>(), prm->odom_threshold, s.odom_dist.as<double>(), s.flags.as<int>());
+    else
+      k_odom_batch<2><<<(unsigned)((nlc + 255) / 256), 256, 0, st>>>(
+          (long long)nlc, s.d_lc_graph.as<int>(), s.d_pose_off.as<long long>(), s.traj.as<double>(), s.lc_from.as<int>(),
+          s.lc_to.as<int>(), s.lcm.as<double>(), prm->odom_threshold, s.odom_dist.as<double>(), s.flags.as<int>());
+  }
+  if (dim == 3)
+    k_compact_batch<3><<<G, 1024, 0, st>>>(G, s.d_lc_off.as<long long>(), s.flags.as<int>(), s.lc_from.as<int>(),
+                                           s.lc_to.as<int>(), s.lcm.as<double>(), s.accepted.as<int>(), s.m.as<int>(),
+                                           s.c_from.as<int>(), s.c_to.as<int>(), s.c_lcm.as<double>());
+  else
+    k_compact_batch<2><<<G, 1024, 0, st>>>(G, s.d_lc_off.as<long long>(), s.flags.as<int>(), s.lc_from.as<int>(),
+                                           s.lc_to.as<int>(), s.lcm.as<double>(), s.accepted.as<int>(), s.m.as<int>(),
+                                           s.c_from.as<int>(), s.c_to.as<int>(), s.c_lcm.as<double>());
+  RPGO_HIP_CHECK(hipGetLastError());
+  if (events) RPGO_HIP_CHECK(hipEventRecord(ev[2], st));
+  int rc;
+  if ((rc = launch_pairs(ctx, s, prm, pw))) return rc;
+  if (events) RPGO_HIP_CHECK(hipEventRecord(ev[3], st));
+  if (cw) {
+    if ((rc = clique_batch_launch(ctx, G, cw, s.d_adj_off.as<long long>(), s.d_lc_off.as<long long>(), s.m.as<int>(),
+                                  s.adj.as<unsigned long long>(), s.deg.as<int>(), s.cidx.as<int>(), s.ck.as<int>())))
+      return rc;
+  }
+  if (events) RPGO_HIP_CHECK(hipEventRecord(ev[5], st));
+  return RPGO_OK;
+}
+
+// Launch-shape autotuner: runs the stages once, then times the pair stage for
+// 1, 2, 4 and 8 wavefronts per workgroup and the clique stage for 4, 8 and 16
+// wavefronts per graph on this batch (best of two launches each) and keeps the
+// fastest per dim.  Every candidate computes the same bits.
+static int autotune(rpgo_ctx* ctx, BatchState& s, const rpgo_pcm_params* prm, bool clique, float* ms_tune) {
+  hipStream_t st = ctx->stream;
+  hipEvent_t* ev = ctx->ev;
+  int rc;
+  RPGO_HIP_CHECK(hipEventRecord(ev[6], st));
+  if ((rc = launch_stages(ctx, s, prm, kDefaultPairWaves, clique ? kDefaultCliqueWaves : 0, false))) return rc;
+  float best_t = 1e30f;
+  int best_pw = kDefaultPairWaves;
+  for (int pw : {1, 2, 4, 8}) {
+    float t_min = 1e30f;
+    for (int rep = 0; rep < 2; ++rep) {
+      RPGO_HIP_CHECK(hipEventRecord(ev[7], st));
+      if ((rc = launch_pairs(ctx, s, prm, pw))) return rc;
+      RPGO_HIP_CHECK(hipEventRecord(ev[8], st));
+      RPGO_HIP_CHECK(hipEventSynchronize(ev[8]));
+      float t = 0.f;
+      (void)hipEventElapsedTime(&t, ev[7], ev[8]);
+      t_min = std::min(t_min, t);
+    }
+    if (t_min < best_t) {
+      best_t = t_min;
+      best_pw = pw;
+    }
+  }
+  s.tuned_pair[s.dim] = best_pw;
+  if (clique) {
+    best_t = 1e30f;
+    int best_cw = kDefaultCliqueWaves;
+    for (int cw : {4, 8, 16}) {
+      float t_min = 1e30f;
+      for (int rep = 0; rep < 2; ++rep) {
+        RPGO_HIP_CHECK(hipEventRecord(ev[7], st));
+        if ((rc = clique_batch_launch(ctx, s.n_graphs, cw, s.d_adj_off.as<long long>(), s.d_lc_off.as<long long>(),
+                                      s.m.as<int>(), s.adj.as<unsigned long long>(), s.deg.as<int>(),
+                                      s.cidx.as<int>(), s.ck.as<int>())))
+          return rc;
+        RPGO_HIP_CHECK(hipEventRecord(ev[8], st));
+        RPGO_HIP_CHECK(hipEventSynchronize(ev[8]));
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, ev[7], ev[8]);
+        t_min = std::min(t_min, t);
+      }
+      if (t_min < best_t) {
+        best_t = t_min;
+        best_cw = cw;
+      }
+    }
+    s.tuned_clique[s.dim] = best_cw;
+  }
+  RPGO_HIP_CHECK(hipEventRecord(ev[9], st));
+  RPGO_HIP_CHECK(hipEventSynchronize(ev[9]));
+  (void)hipEventElapsedTime(ms_tune, ev[6], ev[9]);
   return RPGO_OK;
 }
 
@@ -368,122 +561,113 @@ int rpgo_pcm_batch_run(rpgo_ctx* ctx, const rpgo_pcm_batch* B, const rpgo_pcm_pa
       (rc = s.adj.reserve(sizeof(unsigned long long) * (size_t)std::max<long long>(s.adj_off[G], 1))) ||
       (rc = s.counters.reserve(sizeof(unsigned long long) * 4)))
     return rc;
-  RPGO_HIP_CHECK(hipMemsetAsync(s.counters.ptr, 0, sizeof(unsigned long long) * 4, st));
-  RPGO_HIP_CHECK(hipEventRecord(ctx->ev[0], st));
-  // measurements of every odometry row and loop closure
-  if (nod) {
-    if (dim == 3)
-      k_meas_batch<3><<<(unsigned)((nod + 255) / 256), 256, 0, st>>>((long long)nod, s.odom.as<double>(), s.odom_cov.as<double>(),
-                                                                     s.delta.as<double>(), s.hinv.as<double>());
-    else
-      k_meas_batch<2><<<(unsigned)((nod + 255) / 256), 256, 0, st>>>((long long)nod, s.odom.as<double>(), s.odom_cov.as<double>(),
-                                                                     s.delta.as<double>(), s.hinv.as<double>());
+  const bool want_clique = clique_size != nullptr;
+  if (want_clique) {
+    for (int g = 0; g < G; ++g)
+      if (s.lc_off[g + 1] - s.lc_off[g] > clique_max_vertices()) {
+        set_error("rpgo_pcm_batch_run: graph %d has %lld loop closures; the clique stage takes at most %d", g,
+                  s.lc_off[g + 1] - s.lc_off[g], clique_max_vertices());
+        return RPGO_ERR_ARG;
+      }
+    if ((rc = s.deg.reserve(sizeof(int) * nl1)) || (rc = s.cidx.reserve(sizeof(int) * nl1)) ||
+        (rc = s.ck.reserve(sizeof(int) * (size_t)G)))
+      return rc;
   }
-  if (nlc) {
-    if (dim == 3)
-      k_meas_batch<3><<<(unsigned)((nlc + 255) / 256), 256, 0, st>>>((long long)nlc, s.lc_pose.as<double>(), s.lc_cov.as<double>(),
-                                                                     s.lcm.as<double>(), nullptr);
-    else
-      k_meas_batch<2><<<(unsigned)((nlc + 255) / 256), 256, 0, st>>>((long long)nlc, s.lc_pose.as<double>(), s.lc_cov.as<double>(),
-                                                                     s.lcm.as<double>(), nullptr);
+  float ms_tune = 0.f;
+  if ((B->flags & RPGO_BATCH_AUTOTUNE) && (!s.tuned_pair[dim] || (want_clique && !s.tuned_clique[dim]))) {
+    if ((rc = autotune(ctx, s, prm, want_clique, &ms_tune))) return rc;
   }
-  // fold: one wavefront per graph
-  if (dim == 3)
-    k_fold_batch<3><<<G, 64, 0, st>>>(G, s.d_pose_off.as<long long>(), s.pose0.as<double>(), s.delta.as<double>(),
-                                      s.hinv.as<double>(), s.traj.as<double>());
-  else
-    k_fold_batch<2><<<G, 64, 0, st>>>(G, s.d_pose_off.as<long long>(), s.pose0.as<double>(), s.delta.as<double>(),
-                                      s.hinv.as<double>(), s.traj.as<double>());
-  RPGO_HIP_CHECK(hipEventRecord(ctx->ev[1], st));
-  if (nlc) {
-    if (dim == 3) {
-      k_odom_batch<3><<<(unsigned)((nlc + 255) / 256), 256, 0, st>>>(
-          (long long)nlc, s.d_lc_graph.as<int>(), s.d_pose_off.as<long long>(), s.traj.as<double>(), s.lc_from.as<int>(),
-          s.lc_to.as<int>(), s.lcm.as<double>(), prm->odom_threshold, s.odom_dist.as<double>(), s.flags.as<int>());
-      k_compact_batch<3><<<G, 1024, 0, st>>>(G, s.d_lc_off.as<long long>(), s.flags.as<int>(), s.lc_from.as<int>(),
-                                             s.lc_to.as<int>(), s.lcm.as<double>(), s.accepted.as<int>(), s.m.as<int>(),
-                                             s.c_from.as<int>(), s.c_to.as<int>(), s.c_lcm.as<double>());
+  const int pw = s.tuned_pair[dim] ? s.tuned_pair[dim] : kDefaultPairWaves;
+  const int cw = want_clique ? (s.tuned_clique[dim] ? s.tuned_clique[dim] : kDefaultCliqueWaves) : 0;
+  hipEvent_t* ev = ctx->ev;
+  int replayed = 0;
+  if (B->flags & RPGO_BATCH_GRAPH) {
+    // thresholds are baked into the captured launches: the key holds their bits
+    long long bo, bl;
+    std::memcpy(&bo, &prm->odom_threshold, 8);
+    std::memcpy(&bl, &prm->lc_threshold, 8);
+    std::vector<long long> key = {dim, G, (long long)nlc, (long long)nod, s.tile_off[G], s.sym_off[G], pw, cw, bo, bl};
+    for (DevBuf* b : {&s.d_pose_off, &s.d_lc_off, &s.d_adj_off, &s.d_tile_off, &s.d_sym_off, &s.d_lc_graph,
+                      &s.pose0, &s.odom, &s.odom_cov, &s.lc_from, &s.lc_to, &s.lc_pose, &s.lc_cov, &s.delta,
+                      &s.hinv, &s.traj, &s.lcm, &s.odom_dist, &s.flags, &s.accepted, &s.m, &s.c_from, &s.c_to,
+                      &s.c_lcm, &s.adj, &s.counters, &s.deg, &s.cidx, &s.ck})
+      key.push_back((long long)(uintptr_t)b->ptr);
+    if (s.exec && key == s.graph_key) {
+      replayed = 1;
     } else {
-      k_odom_batch<2><<<(unsigned)((nlc + 255) / 256), 256, 0, st>>>(
-          (long long)nlc, s.d_lc_graph.as<int>(), s.d_pose_off.as<long long>(), s.traj.as<double>(), s.lc_from.as<int>(),
-          s.lc_to.as<int>(), s.lcm.as<double>(), prm->odom_threshold, s.odom_dist.as<double>(), s.flags.as<int>());
-      k_compact_batch<2><<<G, 1024, 0, st>>>(G, s.d_lc_off.as<long long>(), s.flags.as<int>(), s.lc_from.as<int>(),
-                                             s.lc_to.as<int>(), s.lcm.as<double>(), s.accepted.as<int>(), s.m.as<int>(),
-                                             s.c_from.as<int>(), s.c_to.as<int>(), s.c_lcm.as<double>());
+      if (s.exec) (void)hipGraphExecDestroy(s.exec);
+      if (s.graph) (void)hipGraphDestroy(s.graph);
+      s.exec = nullptr;
+      s.graph = nullptr;
+      s.graph_key.clear();
+      RPGO_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+      const int rc_cap = launch_stages(ctx, s, prm, pw, cw, false);
+      hipGraph_t gr = nullptr;
+      const hipError_t e_end = hipStreamEndCapture(st, &gr);
+      if (rc_cap) {
+        if (gr) (void)hipGraphDestroy(gr);
+        return rc_cap;
+      }
+      RPGO_HIP_CHECK(e_end);
+      s.graph = gr;
+      RPGO_HIP_CHECK(hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0));
+      s.graph_key = key;
     }
+    RPGO_HIP_CHECK(hipEventRecord(ev[0], st));
+    RPGO_HIP_CHECK(hipGraphLaunch(s.exec, st));
+    RPGO_HIP_CHECK(hipEventRecord(ev[5], st));
   } else {
-    RPGO_HIP_CHECK(hipMemsetAsync(s.m.ptr, 0, sizeof(int) * G, st));
+    if ((rc = launch_stages(ctx, s, prm, pw, cw, true))) return rc;
   }
-  RPGO_HIP_CHECK(hipGetLastError());
-  RPGO_HIP_CHECK(hipEventRecord(ctx->ev[2], st));
-  // pairs: all graphs' tiles in one launch; the lc_threshold < 0 case
-  // (Pcm.h:673) makes every pair consistent
-  const long long n_tiles = s.tile_off[G];
-  const long long adj_words = std::max<long long>(s.adj_off[G], 1);
-  k_zero_u64<<<(unsigned)std::min<long long>((adj_words + 255) / 256, 4096), 256, 0, st>>>(adj_words, s.adj.as<unsigned long long>());
-  const double thr = prm->lc_threshold;
-  const int all_cons = prm->lc_threshold < 0 ? 1 : 0;
-  if (n_tiles > 0) {
-    const unsigned blocks = (unsigned)((n_tiles + 3) / 4);
-    if (dim == 3)
-      k_pairs_batch<3><<<blocks, 256, 0, st>>>(G, n_tiles, s.d_tile_off.as<long long>(), s.d_pose_off.as<long long>(),
-                                               s.d_lc_off.as<long long>(), s.d_adj_off.as<long long>(), s.m.as<int>(),
-                                               s.traj.as<double>(), s.c_from.as<int>(), s.c_to.as<int>(),
-                                               s.c_lcm.as<double>(), thr, all_cons, s.adj.as<unsigned long long>(),
-                                               s.counters.as<unsigned long long>());
-    else
-      k_pairs_batch<2><<<blocks, 256, 0, st>>>(G, n_tiles, s.d_tile_off.as<long long>(), s.d_pose_off.as<long long>(),
-                                               s.d_lc_off.as<long long>(), s.d_adj_off.as<long long>(), s.m.as<int>(),
-                                               s.traj.as<double>(), s.c_from.as<int>(), s.c_to.as<int>(),
-                                               s.c_lcm.as<double>(), thr, all_cons, s.adj.as<unsigned long long>(),
-                                               s.counters.as<unsigned long long>());
-    const long long n_blocks = s.sym_off[G];
-    k_sym_batch<<<(unsigned)n_blocks, 64, 0, st>>>(G, n_blocks, s.d_sym_off.as<long long>(), s.d_lc_off.as<long long>(),
-                                                   s.d_adj_off.as<long long>(), s.m.as<int>(),
-                                                   s.adj.as<unsigned long long>());
-  }
-  RPGO_HIP_CHECK(hipGetLastError());
-  RPGO_HIP_CHECK(hipEventRecord(ctx->ev[3], st));
   s.m_host.assign(G, 0);
   RPGO_HIP_CHECK(hipMemcpyAsync(s.m_host.data(), s.m.ptr, sizeof(int) * G, hipMemcpyDeviceToHost, st));
   unsigned long long cnt[4];
   RPGO_HIP_CHECK(hipMemcpyAsync(cnt, s.counters.ptr, sizeof(cnt), hipMemcpyDeviceToHost, st));
+  if (accepted && nlc)
+    RPGO_HIP_CHECK(hipMemcpyAsync(accepted, s.accepted.ptr, sizeof(int) * nlc, hipMemcpyDeviceToHost, st));
+  std::vector<int> k_host(want_clique ? G : 0);
+  if (want_clique) {
+    RPGO_HIP_CHECK(hipMemcpyAsync(k_host.data(), s.ck.ptr, sizeof(int) * G, hipMemcpyDeviceToHost, st));
+    if (clique_idx && nlc)
+      RPGO_HIP_CHECK(hipMemcpyAsync(clique_idx, s.cidx.ptr, sizeof(int) * nlc, hipMemcpyDeviceToHost, st));
+  }
   RPGO_HIP_CHECK(hipStreamSynchronize(st));
   long long pairs = 0;
   for (int g = 0; g < G; ++g) pairs += (long long)s.m_host[g] * (s.m_host[g] - 1) / 2;
   if (n_accepted) std::memcpy(n_accepted, s.m_host.data(), sizeof(int) * G);
-  if (accepted && nlc)
-    RPGO_HIP_CHECK(hipMemcpy(accepted, s.accepted.ptr, sizeof(int) * nlc, hipMemcpyDeviceToHost));
-  // heuristic clique per graph on the device-resident adjacency
-  double ms_clique = 0.0;
-  if (clique_size) {
-    RPGO_HIP_CHECK(hipEventRecord(ctx->ev[4], st));
+  if (want_clique) {
     for (int g = 0; g < G; ++g) {
-      int32_t k = -1;
-      const int m = s.m_host[g];
-      if ((rc = clique_run(ctx, reinterpret_cast<const uint64_t*>(s.adj.as<unsigned long long>() + s.adj_off[g]), m,
-                           RPGO_CLIQUE_HEU_REF, clique_idx ? clique_idx + s.lc_off[g] : nullptr, &k)))
-        return rc;
-      clique_size[g] = k;
+      if (k_host[g] == INT_MIN) {
+        set_error("rpgo_pcm_batch_run: clique replay of graph %d disagrees with its scan", g);
+        return RPGO_ERR_STATE;
+      }
+      clique_size[g] = k_host[g];
     }
-    RPGO_HIP_CHECK(hipEventRecord(ctx->ev[5], st));
-    RPGO_HIP_CHECK(hipEventSynchronize(ctx->ev[5]));
-    float msc = 0.f;
-    (void)hipEventElapsedTime(&msc, ctx->ev[4], ctx->ev[5]);
-    ms_clique = msc;
   }
   if (stats) {
-    float a = 0.f, b = 0.f, c = 0.f;
-    (void)hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]);
-    (void)hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]);
-    (void)hipEventElapsedTime(&c, ctx->ev[2], ctx->ev[3]);
+    *stats = rpgo_pcm_batch_stats{};
     stats->n_pairs = pairs;
     stats->n_consistent = (long long)cnt[0];
-    stats->ms_fold = a;
-    stats->ms_odom = b;
-    stats->ms_pairs = c;
-    stats->ms_clique = ms_clique;
-    stats->ms_total = a + b + c + ms_clique;
+    if (B->flags & RPGO_BATCH_GRAPH) {
+      float t = 0.f;
+      (void)hipEventElapsedTime(&t, ev[0], ev[5]);
+      stats->ms_total = t;
+    } else {
+      float a = 0.f, b = 0.f, c = 0.f, d = 0.f;
+      (void)hipEventElapsedTime(&a, ev[0], ev[1]);
+      (void)hipEventElapsedTime(&b, ev[1], ev[2]);
+      (void)hipEventElapsedTime(&c, ev[2], ev[3]);
+      (void)hipEventElapsedTime(&d, ev[3], ev[5]);
+      stats->ms_fold = a;
+      stats->ms_odom = b;
+      stats->ms_pairs = c;
+      stats->ms_clique = want_clique ? d : 0.0;
+      stats->ms_total = a + b + c + d;
+    }
+    stats->ms_tune = ms_tune;
+    stats->pair_waves = pw;
+    stats->clique_waves = cw;
+    stats->graph_replayed = replayed;
   }
   return RPGO_OK;
 }
@@ -494,7 +678,9 @@ int rpgo_pcm_batch_download_adj(rpgo_ctx* ctx, int32_t g, uint64_t* adj_bits) {
     return RPGO_ERR_ARG;
   }
   BatchState& s = *ctx->batch;
-  const long long words = s.adj_off[g + 1] - s.adj_off[g];
+  // graph g's region is sized for all its loop closures; the adjacency in it
+  // has the accepted count's rows and stride
+  const long long m = s.m_host[g], words = m * ((m + 63) / 64);
   if (words > 0)
     RPGO_HIP_CHECK(hipMemcpy(adj_bits, s.adj.as<unsigned long long>() + s.adj_off[g], sizeof(uint64_t) * words,
                              hipMemcpyDeviceToHost));

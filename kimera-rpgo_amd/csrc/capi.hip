@@ -83,10 +83,12 @@ int rpgo_ctx_destroy(rpgo_ctx* ctx) {
   for (DevBuf* b : {&s.pose0, &s.odom, &s.odom_cov, &s.lc_from, &s.lc_to, &s.lc_pose, &s.lc_cov,
                     &s.delta, &s.hinv, &s.traj, &s.lcm, &s.odom_dist, &s.flags, &s.accepted,
                     &s.c_from, &s.c_to, &s.c_lcm, &s.adj, &s.dist, &s.counters, &s.fp32_buf,
-                    &s.queue})
+                    &s.queue, &s.tmp})
     b->release();
   CliqueState& q = ctx->clq;
-  for (DevBuf* b : {&q.adj, &q.deg, &q.dmask, &q.icc, &q.out, &q.scratch, &q.best}) b->release();
+  for (DevBuf* b : {&q.adj, &q.deg, &q.dmask, &q.icc, &q.out, &q.scratch, &q.best, &q.ex_adj,
+                    &q.ex_ctl, &q.ex_stack, &q.ex_path})
+    b->release();
   solver_destroy(ctx);
   mlp_destroy(ctx);
   batch_destroy(ctx);
@@ -152,13 +154,14 @@ int rpgo_pcm_upload(rpgo_ctx* ctx, const rpgo_pcm_problem* p) {
 // The pairwise stage (A3 + A4 of the reference: areLoopsConsistent over every
 // pair, adjacency fill) on the accepted set of the last run.  Events ev[8..11]
 // bracket the pair kernel and the recheck on the context stream.
-static int pairs_pass(rpgo_ctx* ctx, const rpgo_pcm_params* prm, rpgo_pcm_stats* stats) {
+static int pairs_pass(rpgo_ctx* ctx, const rpgo_pcm_params* prm, rpgo_pcm_stats* stats,
+                      const SimpleThr* simple = nullptr) {
   PcmState& s = ctx->pcm;
   hipStream_t st = ctx->stream;
   hipEvent_t* ev = ctx->ev;
   const int m = s.m, W = (m + 63) / 64;
   const bool want_dist = prm->want_dist != 0;
-  const bool check = prm->lc_threshold >= 0.0;
+  const bool check = simple ? (simple->trans >= 0.0 && simple->rot >= 0.0) : prm->lc_threshold >= 0.0;
   int rc;
   if ((rc = s.adj.reserve(sizeof(uint64_t) * (size_t)(m > 0 ? m : 1) * (W > 0 ? W : 1)))) return rc;
   if (want_dist && (rc = s.dist.reserve(sizeof(double) * (size_t)(m > 0 ? m : 1) * (m > 0 ? m : 1))))
@@ -167,13 +170,18 @@ static int pairs_pass(rpgo_ctx* ctx, const rpgo_pcm_params* prm, rpgo_pcm_stats*
   if (m > 0) RPGO_HIP_CHECK(hipMemsetAsync(s.adj.ptr, 0, sizeof(uint64_t) * (size_t)m * W, st));
   if (want_dist && m > 0)
     RPGO_HIP_CHECK(hipMemsetAsync(s.dist.ptr, 0, sizeof(double) * (size_t)m * m, st));
-  const bool fp32 = prm->precision == RPGO_PCM_FP32_RECHECK && !s.fp32_fallback;
+  const bool fp32 = !simple && prm->precision == RPGO_PCM_FP32_RECHECK && !s.fp32_fallback;
   int64_t n_recheck = 0;
   RPGO_HIP_CHECK(hipEventRecord(ev[8], st));
   RPGO_HIP_CHECK(hipEventRecord(ev[9], st));
   RPGO_HIP_CHECK(hipEventRecord(ev[10], st));
   if (check && m > 1) {
-    if (!fp32) {
+    if (simple) {
+      if ((rc = pcm_pairs_simple(ctx, *simple, want_dist))) return rc;
+      if ((rc = pcm_symmetrize(ctx))) return rc;
+      RPGO_HIP_CHECK(hipEventRecord(ev[9], st));
+      RPGO_HIP_CHECK(hipEventRecord(ev[10], st));
+    } else if (!fp32) {
       if ((rc = pcm_pairs_fp64(ctx, prm->lc_threshold, want_dist))) return rc;
       if ((rc = pcm_symmetrize(ctx))) return rc;
       RPGO_HIP_CHECK(hipEventRecord(ev[9], st));
@@ -205,7 +213,8 @@ static int pairs_pass(rpgo_ctx* ctx, const rpgo_pcm_params* prm, rpgo_pcm_stats*
   RPGO_HIP_CHECK(hipEventSynchronize(ev[11]));
   s.ran = true;
   s.have_dist = want_dist;
-  s.precision = prm->precision;
+  s.precision = simple ? RPGO_PCM_FP64_REPLAY : prm->precision;
+  s.simple = simple != nullptr;
   if (stats) {
     unsigned long long cnt[4] = {0, 0, 0, 0};
     RPGO_HIP_CHECK(hipMemcpy(cnt, s.counters.ptr, sizeof(cnt), hipMemcpyDeviceToHost));
@@ -297,6 +306,192 @@ int rpgo_pcm_pairs(rpgo_ctx* ctx, const rpgo_pcm_params* prm, rpgo_pcm_stats* st
     stats->ms_total = elapsed(ctx->ev[12], ctx->ev[11]);
   }
   return RPGO_OK;
+}
+
+int rpgo_pcm_simple_run(rpgo_ctx* ctx, const rpgo_pcm_simple_params* prm, rpgo_pcm_stats* stats) {
+  if (!ctx || !prm) {
+    set_error("rpgo_pcm_simple_run: NULL argument");
+    return RPGO_ERR_ARG;
+  }
+  PcmState& s = ctx->pcm;
+  if (!s.uploaded) {
+    set_error("rpgo_pcm_simple_run: no problem uploaded");
+    return RPGO_ERR_STATE;
+  }
+  RPGO_HIP_CHECK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  hipEvent_t* ev = ctx->ev;
+  int rc;
+  const SimpleThr odom{prm->odom_trans_threshold, prm->odom_rot_threshold};
+  const SimpleThr loop{prm->dist_trans_threshold, prm->dist_rot_threshold};
+  RPGO_HIP_CHECK(hipMemsetAsync(s.counters.ptr, 0, 256, st));
+  RPGO_HIP_CHECK(hipEventRecord(ev[0], st));
+  if ((rc = pcm_prepare(ctx))) return rc;
+  if ((rc = pcm_fold(ctx))) return rc;
+  RPGO_HIP_CHECK(hipEventRecord(ev[1], st));
+  int m = 0;
+  if ((rc = pcm_odom_and_compact(ctx, -1.0, &m, &odom))) return rc;
+  RPGO_HIP_CHECK(hipEventRecord(ev[2], st));
+  s.m = m;
+  s.fp32_ready = false;
+  s.fp32_fallback = false;
+  rpgo_pcm_params pp{};
+  pp.lc_threshold = -1.0;
+  pp.precision = RPGO_PCM_FP64_REPLAY;
+  pp.want_dist = prm->want_dist;
+  if ((rc = pairs_pass(ctx, &pp, stats, &loop))) return rc;
+  if (stats) {
+    stats->ms_fold = elapsed(ev[0], ev[1]);
+    stats->ms_odom = elapsed(ev[1], ev[2]);
+    stats->ms_total = elapsed(ev[0], ev[11]);
+  }
+  return RPGO_OK;
+}
+
+int rpgo_pcm_simple_adjacency(rpgo_ctx* ctx, const rpgo_pcm_problem* prob,
+                              const rpgo_pcm_simple_params* params, int32_t* accepted,
+                              double* odom_dist, uint64_t* adj_bits, double* dist,
+                              rpgo_pcm_stats* stats) {
+  int rc = rpgo_pcm_upload(ctx, prob);
+  if (rc) return rc;
+  if ((rc = rpgo_pcm_simple_run(ctx, params, stats))) return rc;
+  return rpgo_pcm_download(ctx, accepted, odom_dist, adj_bits, dist);
+}
+
+}  // extern "C"
+
+namespace {
+
+// One reference update() after another (Pcm.h:148-281 with
+// parseAndIncrementAdjMatrix :414-502): fold the new odometry onto the chain
+// end, odometry-check the new loop closures against the extended chain and
+// grow the adjacency by their rows (incrementAdjMatrix :725-768).
+int pcm_append_impl(rpgo_ctx* ctx, const rpgo_pcm_increment* inc, double odom_thr, double lc_thr,
+                    const SimpleThr* s_odom, const SimpleThr* s_loop, int32_t want_dist,
+                    rpgo_pcm_stats* stats, const char* name) {
+  if (!ctx || !inc) {
+    set_error("%s: NULL argument", name);
+    return RPGO_ERR_ARG;
+  }
+  PcmState& s = ctx->pcm;
+  if (!s.ran) {
+    set_error("%s: start the stream with a full run (rpgo_pcm_run / rpgo_pcm_simple_run)", name);
+    return RPGO_ERR_STATE;
+  }
+  if (s.simple != (s_odom != nullptr)) {
+    set_error("%s: the last run used the %s checks", name, s.simple ? "PcmSimple" : "PCM");
+    return RPGO_ERR_STATE;
+  }
+  if (want_dist) {
+    set_error("%s: the distance matrix is not kept while streaming (want_dist must be 0)", name);
+    return RPGO_ERR_ARG;
+  }
+  const int n0 = s.n_poses, l0 = s.n_lc;
+  if (inc->n_odom < 0 || inc->n_lc < 0 || (inc->n_odom > 0 && (!inc->odom || !inc->odom_cov)) ||
+      (inc->n_lc > 0 && (!inc->lc_from || !inc->lc_to || !inc->lc_pose || !inc->lc_cov))) {
+    set_error("%s: inconsistent sizes or NULL arrays (n_odom=%d n_lc=%d)", name, inc->n_odom, inc->n_lc);
+    return RPGO_ERR_ARG;
+  }
+  const int n1 = n0 + inc->n_odom, l1 = l0 + inc->n_lc;
+  for (int k = 0; k < inc->n_lc; ++k) {
+    if (inc->lc_from[k] < 0 || inc->lc_from[k] >= n1 || inc->lc_to[k] < 0 || inc->lc_to[k] >= n1) {
+      set_error("%s: loop closure %d references pose (%d,%d) outside [0,%d)", name, k, inc->lc_from[k],
+                inc->lc_to[k], n1);
+      return RPGO_ERR_ARG;
+    }
+  }
+  RPGO_HIP_CHECK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  hipEvent_t* ev = ctx->ev;
+  const size_t PL = pose_len(s.dim), D = tan_dim(s.dim), DD = D * D, PWC = pwc_len(s.dim);
+  const size_t d0 = (size_t)(n0 - 1), d1 = (size_t)(n1 - 1);
+  int rc;
+  if ((rc = s.odom.grow_keep(8 * PL * (d1 > 0 ? d1 : 1), 8 * PL * d0, st))) return rc;
+  if ((rc = s.odom_cov.grow_keep(8 * DD * (d1 > 0 ? d1 : 1), 8 * DD * d0, st))) return rc;
+  if ((rc = s.delta.grow_keep(8 * PWC * (d1 > 0 ? d1 : 1), 8 * PWC * d0, st))) return rc;
+  if ((rc = s.hinv.grow_keep(8 * DD * (d1 > 0 ? d1 : 1), 8 * DD * d0, st))) return rc;
+  if ((rc = s.traj.grow_keep(8 * PWC * (size_t)n1, 8 * PWC * (size_t)n0, st))) return rc;
+  const size_t L0 = (size_t)l0, L1 = (size_t)(l1 > 0 ? l1 : 1);
+  if ((rc = s.lc_from.grow_keep(4 * L1, 4 * L0, st))) return rc;
+  if ((rc = s.lc_to.grow_keep(4 * L1, 4 * L0, st))) return rc;
+  if ((rc = s.lc_pose.grow_keep(8 * PL * L1, 8 * PL * L0, st))) return rc;
+  if ((rc = s.lc_cov.grow_keep(8 * DD * L1, 8 * DD * L0, st))) return rc;
+  if ((rc = s.lcm.grow_keep(8 * PWC * L1, 8 * PWC * L0, st))) return rc;
+  if (inc->n_odom > 0) {
+    RPGO_HIP_CHECK(hipMemcpyAsync(s.odom.as<double>() + PL * d0, inc->odom, 8 * PL * inc->n_odom,
+                                  hipMemcpyHostToDevice, st));
+    RPGO_HIP_CHECK(hipMemcpyAsync(s.odom_cov.as<double>() + DD * d0, inc->odom_cov, 8 * DD * inc->n_odom,
+                                  hipMemcpyHostToDevice, st));
+  }
+  if (inc->n_lc > 0) {
+    RPGO_HIP_CHECK(hipMemcpyAsync(s.lc_from.as<int>() + L0, inc->lc_from, 4 * (size_t)inc->n_lc,
+                                  hipMemcpyHostToDevice, st));
+    RPGO_HIP_CHECK(hipMemcpyAsync(s.lc_to.as<int>() + L0, inc->lc_to, 4 * (size_t)inc->n_lc,
+                                  hipMemcpyHostToDevice, st));
+    RPGO_HIP_CHECK(hipMemcpyAsync(s.lc_pose.as<double>() + PL * L0, inc->lc_pose, 8 * PL * inc->n_lc,
+                                  hipMemcpyHostToDevice, st));
+    RPGO_HIP_CHECK(hipMemcpyAsync(s.lc_cov.as<double>() + DD * L0, inc->lc_cov, 8 * DD * inc->n_lc,
+                                  hipMemcpyHostToDevice, st));
+  }
+  s.n_poses = n1;
+  s.n_lc = l1;
+  s.fp32_ready = false;
+  s.fp32_fallback = false;
+  s.have_dist = false;
+  RPGO_HIP_CHECK(hipMemsetAsync(s.counters.ptr, 0, 256, st));
+  RPGO_HIP_CHECK(hipEventRecord(ev[0], st));
+  if ((rc = pcm_measure_range(ctx, n0 - 1, n1 - 1, l0, l1))) return rc;
+  if ((rc = pcm_fold_from(ctx, n0 - 1))) return rc;
+  RPGO_HIP_CHECK(hipEventRecord(ev[1], st));
+  int mn = 0;
+  if ((rc = pcm_odom_append(ctx, l0, odom_thr, s_odom, &mn))) return rc;
+  RPGO_HIP_CHECK(hipEventRecord(ev[2], st));
+  const int m_old = s.m;
+  s.m = m_old + mn;
+  if ((rc = pcm_adj_append(ctx, m_old, lc_thr, s_loop))) return rc;
+  RPGO_HIP_CHECK(hipEventRecord(ev[3], st));
+  RPGO_HIP_CHECK(hipEventSynchronize(ev[3]));
+  if (stats) {
+    const bool check = s_loop ? (s_loop->trans >= 0.0 && s_loop->rot >= 0.0) : lc_thr >= 0.0;
+    unsigned long long cnt = 0;
+    RPGO_HIP_CHECK(hipMemcpy(&cnt, s.counters.ptr, sizeof(cnt), hipMemcpyDeviceToHost));
+    const int m = s.m;
+    *stats = rpgo_pcm_stats{};
+    stats->n_accepted = m;
+    stats->n_pairs = check ? ((int64_t)m * (m - 1) - (int64_t)m_old * (m_old - 1)) / 2 : 0;
+    stats->n_consistent = (int64_t)cnt;
+    stats->ms_fold = elapsed(ev[0], ev[1]);
+    stats->ms_odom = elapsed(ev[1], ev[2]);
+    stats->ms_pairs = elapsed(ev[2], ev[3]);
+    stats->ms_total = elapsed(ev[0], ev[3]);
+  }
+  return RPGO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rpgo_pcm_append(rpgo_ctx* ctx, const rpgo_pcm_increment* inc, const rpgo_pcm_params* prm,
+                    rpgo_pcm_stats* stats) {
+  if (!prm) {
+    set_error("rpgo_pcm_append: NULL params");
+    return RPGO_ERR_ARG;
+  }
+  return pcm_append_impl(ctx, inc, prm->odom_threshold, prm->lc_threshold, nullptr, nullptr,
+                         prm->want_dist, stats, "rpgo_pcm_append");
+}
+
+int rpgo_pcm_simple_append(rpgo_ctx* ctx, const rpgo_pcm_increment* inc,
+                           const rpgo_pcm_simple_params* prm, rpgo_pcm_stats* stats) {
+  if (!prm) {
+    set_error("rpgo_pcm_simple_append: NULL params");
+    return RPGO_ERR_ARG;
+  }
+  const SimpleThr odom{prm->odom_trans_threshold, prm->odom_rot_threshold};
+  const SimpleThr loop{prm->dist_trans_threshold, prm->dist_rot_threshold};
+  return pcm_append_impl(ctx, inc, -1.0, -1.0, &odom, &loop, prm->want_dist, stats,
+                         "rpgo_pcm_simple_append");
 }
 
 int rpgo_pcm_download(rpgo_ctx* ctx, int32_t* accepted, double* odom_dist, uint64_t* adj_bits,

@@ -70,6 +70,49 @@ __device__ __forceinline__ int wave_top_bit(const unsigned long long* S, int lan
   return mine;
 }
 
+__device__ __forceinline__ int wave_popcount(const unsigned long long* S) {
+  int cnt = 0;
+#pragma unroll
+  for (int q = 0; q < kMaxWPL; ++q) cnt += __popcll(S[q]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
+  return cnt;
+}
+
+// Greedy size icc(c, best) of candidate c, one wavefront (S in registers,
+// word w = lane + 64 q); dmask holds the deg >= best filter (LDS or global).
+// Only icc > best matters to the scan, so the walk stops early (INT_MIN) once
+// it cannot get there: every further step removes at least the picked vertex
+// from S, so at most |S| + 1 increments remain.  A candidate that beats best
+// never stops early and returns its exact size.
+__device__ __forceinline__ int icc_wave(int W, int c, const unsigned long long* __restrict__ adj,
+                                        const unsigned long long* dmask, int lane, int best) {
+  unsigned long long S[kMaxWPL];
+#pragma unroll
+  for (int q = 0; q < kMaxWPL; ++q) {
+    const int w = lane + 64 * q;
+    S[q] = (w < W) ? (adj[(size_t)c * W + w] & dmask[w] &
+                      ((w == (c >> 6)) ? ~(1ull << (c & 63)) : ~0ull))
+                   : 0ull;
+  }
+  int icc = 0;
+  if (wave_popcount(S) + 1 <= best) return INT_MIN;
+  for (;;) {
+    ++icc;
+    const int m = wave_top_bit(S, lane, W);
+    if (m < 0) break;  // S held only the seed c: removing it empties S
+    // S /\ N(m); m itself always leaves S, so the loop ends within n steps
+    const unsigned long long* row = adj + (size_t)m * W;
+#pragma unroll
+    for (int q = 0; q < kMaxWPL; ++q) {
+      const int w = lane + 64 * q;
+      if (w < W) S[q] &= row[w] & ((w == (m >> 6)) ? ~(1ull << (m & 63)) : ~0ull);
+    }
+    if ((icc & 7) == 0 && icc + wave_popcount(S) + 1 <= best) return INT_MIN;
+  }
+  return icc;
+}
+
 // One wavefront per candidate c >= pos: greedy size icc(c, best).
 __global__ void __launch_bounds__(256) k_icc(int n, int W, int pos, int best,
                                              const unsigned long long* __restrict__ adj,
@@ -83,27 +126,7 @@ __global__ void __launch_bounds__(256) k_icc(int n, int W, int pos, int best,
     if (lane == 0) icc_out[c] = INT_MIN;
     return;
   }
-  unsigned long long S[kMaxWPL];
-#pragma unroll
-  for (int q = 0; q < kMaxWPL; ++q) {
-    const int w = lane + 64 * q;
-    S[q] = (w < W) ? (adj[(size_t)c * W + w] & dmask[w] &
-                      ((w == (c >> 6)) ? ~(1ull << (c & 63)) : ~0ull))
-                   : 0ull;
-  }
-  int icc = 0;
-  for (;;) {
-    ++icc;
-    const int m = wave_top_bit(S, lane, W);
-    if (m < 0) break;  // S held only the seed c: removing it empties S
-    // S /\ N(m); m itself always leaves S, so the loop ends within n steps
-    const unsigned long long* row = adj + (size_t)m * W;
-#pragma unroll
-    for (int q = 0; q < kMaxWPL; ++q) {
-      const int w = lane + 64 * q;
-      if (w < W) S[q] &= row[w] & ((w == (m >> 6)) ? ~(1ull << (m & 63)) : ~0ull);
-    }
-  }
+  const int icc = icc_wave(W, c, adj, dmask, lane, best);
   if (lane == 0) icc_out[c] = icc;
 }
 
@@ -114,13 +137,11 @@ __global__ void k_first_better(int n, int pos, int best, const int* __restrict__
   if (icc[c] > best) atomicMin(first, c);
 }
 
-// Replay of the winning candidate: writes the reference's buffer positions
-// 0..s0-1 (position p keeps the value last written to it).  One wavefront.
-__global__ void __launch_bounds__(64) k_replay(int n, int W, int c, int best,
-                                               const unsigned long long* __restrict__ adj,
-                                               const unsigned long long* __restrict__ dmask,
-                                               int* __restrict__ out, int* __restrict__ icc_out) {
-  const int lane = threadIdx.x;
+// Replay of the winning candidate c: writes the reference's buffer positions
+// 0..s0-1 (position p keeps the value last written to it) and returns the
+// greedy size.  One wavefront.
+__device__ int replay_wave(int W, int c, const unsigned long long* __restrict__ adj,
+                           const unsigned long long* dmask, int* __restrict__ out, int lane) {
   unsigned long long S[kMaxWPL];
 #pragma unroll
   for (int q = 0; q < kMaxWPL; ++q) {
@@ -129,16 +150,8 @@ __global__ void __launch_bounds__(64) k_replay(int n, int W, int c, int best,
                       ((w == (c >> 6)) ? ~(1ull << (c & 63)) : ~0ull))
                    : 0ull;
   }
-  auto wave_count = [&](const unsigned long long* B) {
-    int cnt = 0;
-#pragma unroll
-    for (int q = 0; q < kMaxWPL; ++q) cnt += __popcll(B[q]);
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
-    return cnt;
-  };
   if (lane == 0) out[0] = c;
-  int s_t = 1 + wave_count(S);  // list size before the iteration
+  int s_t = 1 + wave_popcount(S);  // list size before the iteration
   int icc = 0;
   for (;;) {
     ++icc;
@@ -156,7 +169,7 @@ __global__ void __launch_bounds__(64) k_replay(int n, int W, int c, int best,
       for (int q = 0; q < kMaxWPL; ++q) N[q] = 0ull;
     }
     // list after the iteration: the seed survives unless it was the pick
-    const int s_next = (m >= 0) ? 1 + wave_count(N) : 0;
+    const int s_next = (m >= 0) ? 1 + wave_popcount(N) : 0;
     // positions [max(s_next,1), s_t) keep S_t's entries: ranks p-1 of the set part
     const int lo = s_next > 1 ? s_next : 1;
     if (lo < s_t) {
@@ -191,10 +204,151 @@ __global__ void __launch_bounds__(64) k_replay(int n, int W, int c, int best,
     s_t = s_next;
     if (m < 0) break;
   }
-  if (lane == 0) *icc_out = icc;
+  return icc;
+}
+
+
+__global__ void __launch_bounds__(64) k_replay(int n, int W, int c, int best,
+                                               const unsigned long long* __restrict__ adj,
+                                               const unsigned long long* __restrict__ dmask,
+                                               int* __restrict__ out, int* __restrict__ icc_out) {
+  const int icc = replay_wave(W, c, adj, dmask, out, threadIdx.x);
+  if (threadIdx.x == 0) *icc_out = icc;
   (void)best;
   (void)n;
 }
+
+// ---------------------------------------------------------------------------
+// Batched heuristic: one workgroup of NW wavefronts per graph runs the whole
+// sequential scan of FMC::maxCliqueHeu on the device.  Candidates are taken
+// in rounds of NW consecutive ones (one per wavefront) against the current
+// best; the first candidate of the first round that beats best is the one
+// the sequential scan takes, and the scan resumes after it.  The winner is
+// replayed by wavefront 0.  Graph g's adjacency starts at adj_off[g] with m[g]
+// rows of ceil(m[g]/64) words; its degrees and clique go to lc_off[g].
+// ---------------------------------------------------------------------------
+template <int NW>
+__global__ void __launch_bounds__(64 * NW) k_clique_batch(int n_graphs, const long long* __restrict__ adj_off,
+                                                          const long long* __restrict__ lc_off,
+                                                          const int* __restrict__ m_arr,
+                                                          const unsigned long long* __restrict__ adj_all,
+                                                          int* __restrict__ deg_all, int* __restrict__ out_all,
+                                                          int* __restrict__ k_out) {
+  __shared__ unsigned long long dm[64 * kMaxWPL];
+  __shared__ int s_icc[NW];
+  __shared__ int s_win, s_win_icc;
+  const int g = blockIdx.x;
+  if (g >= n_graphs) return;
+  const int n = m_arr[g];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  if (n <= 0) {
+    if (tid == 0) k_out[g] = -1;  // maxCliqueHeu on an empty graph
+    return;
+  }
+  const int W = (n + 63) / 64;
+  if (W > 64 * kMaxWPL) {
+    if (tid == 0) k_out[g] = INT_MIN;  // rejected on the host before launch
+    return;
+  }
+  const unsigned long long* A = adj_all + adj_off[g];
+  int* deg = deg_all + lc_off[g];
+  int* out = out_all + lc_off[g];
+  for (int v = tid; v < n; v += 64 * NW) {
+    int d = 0;
+    for (int w = 0; w < W; ++w) {
+      unsigned long long b = A[(size_t)v * W + w];
+      if (w == W - 1 && (n & 63)) b &= (1ull << (n & 63)) - 1ull;
+      if (w == (v >> 6)) b &= ~(1ull << (v & 63));
+      d += __popcll(b);
+    }
+    deg[v] = d;
+  }
+  __syncthreads();
+  int best = -1, pos = 0, win = -1, win_best = -1;
+  for (;;) {
+    for (int w = tid; w < W; w += 64 * NW) {
+      unsigned long long b = 0ull;
+      for (int k = 0; k < 64; ++k) {
+        const int u = w * 64 + k;
+        if (u < n && deg[u] >= best) b |= 1ull << k;
+      }
+      dm[w] = b;
+    }
+    __syncthreads();
+    bool found = false;
+    for (int base = pos; base < n; base += NW) {
+      const int c = base + wid;
+      int icc = INT_MIN;
+      if (c < n && !(best > deg[c])) icc = icc_wave(W, c, A, dm, lane, best);  // pruning 1
+      if (lane == 0) s_icc[wid] = icc;
+      __syncthreads();
+      if (tid == 0) {
+        int first = -1, fi = 0;
+        for (int k = 0; k < NW; ++k)
+          if (s_icc[k] > best) {
+            first = base + k;
+            fi = s_icc[k];
+            break;
+          }
+        s_win = first;
+        s_win_icc = fi;
+      }
+      __syncthreads();
+      const int first = s_win;
+      if (first >= 0) {
+        win = first;
+        win_best = best;
+        best = s_win_icc;
+        pos = first + 1;
+        found = true;
+        break;
+      }
+    }
+    __syncthreads();
+    if (!found) break;
+  }
+  if (win >= 0) {
+    for (int w = tid; w < W; w += 64 * NW) {
+      unsigned long long b = 0ull;
+      for (int k = 0; k < 64; ++k) {
+        const int u = w * 64 + k;
+        if (u < n && deg[u] >= win_best) b |= 1ull << k;
+      }
+      dm[w] = b;
+    }
+    __syncthreads();
+    if (wid == 0) {
+      const int icc = replay_wave(W, win, A, dm, out, lane);
+      if (lane == 0) k_out[g] = icc == best ? best : INT_MIN;
+    }
+  } else if (tid == 0) {
+    k_out[g] = best;
+  }
+}
+
+int clique_batch_launch(rpgo_ctx* ctx, int n_graphs, int waves, const long long* d_adj_off,
+                        const long long* d_lc_off, const int* d_m, const unsigned long long* adj, int* deg,
+                        int* out_idx, int* out_k) {
+  hipStream_t st = ctx->stream;
+  switch (waves) {
+    case 4:
+      k_clique_batch<4><<<n_graphs, 256, 0, st>>>(n_graphs, d_adj_off, d_lc_off, d_m, adj, deg, out_idx, out_k);
+      break;
+    case 8:
+      k_clique_batch<8><<<n_graphs, 512, 0, st>>>(n_graphs, d_adj_off, d_lc_off, d_m, adj, deg, out_idx, out_k);
+      break;
+    case 16:
+      k_clique_batch<16><<<n_graphs, 1024, 0, st>>>(n_graphs, d_adj_off, d_lc_off, d_m, adj, deg, out_idx, out_k);
+      break;
+    default:
+      set_error("clique_batch_launch: unsupported wavefront count %d", waves);
+      return RPGO_ERR_ARG;
+  }
+  RPGO_HIP_CHECK(hipGetLastError());
+  return RPGO_OK;
+}
+
+int clique_max_vertices() { return 64 * 64 * kMaxWPL; }
 
 // Greedy scan over candidates pos0 .. n-1 starting from the bound best0;
 // mode EXACT continues with the exact search, mode HEU_REF replays the winner.

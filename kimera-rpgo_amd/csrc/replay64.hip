@@ -8,6 +8,7 @@
 //   - the odometry check       (Pcm.h:604-629)
 //   - the pairwise check       (Pcm.h:670-718, GeometryUtils.h:133-186)
 //   - adjacency bit packing    (Pcm.h:725-768, row-packed and symmetrized)
+//   - the PcmSimple checks     (Pcm.h:581-596, 654-662, PoseWithNode norms)
 // and the device functions reused by the FP32 path's near-threshold recheck.
 #include "replay64.h"
 #include "rpgo_internal.h"
@@ -40,9 +41,12 @@ __global__ void k_measurements(int n, const double* __restrict__ pose,
 // sequential fold, one wavefront:  T_{k+1} = T_k * delta_k,
 // Sigma_{k+1} = (H_k Sigma_k) H_k^T + Sigma_delta_k   (same order as the oracle)
 // lanes 0..D*D-1 own the covariance entries, the next lanes the pose entries.
+// Records start+1 .. n_poses-1 are written; the state at `start` comes from
+// pose0 (chain start, record `start` written too) or from the stored record.
 // ---------------------------------------------------------------------------
 template <int DIM>
-__global__ void __launch_bounds__(64) k_fold(int n_poses, const double* __restrict__ pose0,
+__global__ void __launch_bounds__(64) k_fold(int start, int n_poses,
+                                            const double* __restrict__ pose0,
                                             const double* __restrict__ delta,
                                             const double* __restrict__ hinv,
                                             double* __restrict__ traj) {
@@ -50,14 +54,22 @@ __global__ void __launch_bounds__(64) k_fold(int n_poses, const double* __restri
   __shared__ double S[DD], T[DD], P[PL], Hs[DD], Dl[PWC];
   __shared__ int ri;
   const int l = threadIdx.x;
-  if (l < DD) S[l] = 0.0;
-  if (l < PL) P[l] = pose0[l];
-  if (l == 0) ri = 1;
+  if (pose0) {
+    // chain start: pose from the values, zero covariance (Pcm.h:534-542)
+    if (l < DD) S[l] = 0.0;
+    if (l < PL) P[l] = pose0[l];
+    if (l == 0) ri = 1;
+    for (int e = l; e < PWC; e += 64)
+      traj[(size_t)start * PWC + e] = e < PL ? pose0[e] : (e < PL + DD ? 0.0 : 1.0);
+  } else {
+    // continue from a stored record: the same values the fold held there
+    const double* rec = traj + (size_t)start * PWC;
+    if (l < DD) S[l] = rec[PL + l];
+    if (l < PL) P[l] = rec[l];
+    if (l == 0) ri = rec[PL + DD] != 0.0 ? 1 : 0;
+  }
   __syncthreads();
-  // record 0
-  for (int e = l; e < PWC; e += 64)
-    traj[e] = e < PL ? P[e] : (e < PL + DD ? 0.0 : 1.0);
-  for (int k = 0; k + 1 < n_poses; ++k) {
+  for (int k = start; k + 1 < n_poses; ++k) {
     for (int e = l; e < DD; e += 64) Hs[e] = hinv[(size_t)k * DD + e];
     for (int e = l; e < PWC; e += 64) Dl[e] = delta[(size_t)k * PWC + e];
     __syncthreads();
@@ -111,6 +123,24 @@ __global__ void k_odom_check(int n_lc, const double* __restrict__ traj,
                                        lcm + (size_t)k * PWC);
   dist[k] = d;
   flag[k] = (thr < 0.0 || d < thr) ? 1 : 0;
+}
+
+// PcmSimple odometry check (Pcm.h:581-596, 604-629): node |to - from| + 1;
+// dist receives the translation norm.  check == false passes every loop closure.
+template <int DIM>
+__global__ void k_odom_simple(int n_lc, const double* __restrict__ traj,
+                              const int* __restrict__ from, const int* __restrict__ to,
+                              const double* __restrict__ lcm, bool check, double thr_t,
+                              double thr_r, double* __restrict__ dist, int* __restrict__ flag) {
+  constexpr int PWC = R64<DIM>::PWC;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_lc) return;
+  const int a = from[k], b = to[k];
+  double tn, rn;
+  R64<DIM>::simple_odom(traj + (size_t)a * PWC, traj + (size_t)b * PWC, lcm + (size_t)k * PWC,
+                        abs(b - a) + 1, &tn, &rn);
+  dist[k] = tn;
+  flag[k] = (!check || (tn < thr_t && rn < thr_r)) ? 1 : 0;
 }
 
 // Stable compaction of accepted loop closures (single workgroup scan).
@@ -206,6 +236,42 @@ __global__ void __launch_bounds__(256) k_pairs64(int m, long long n_tiles,
   }
 }
 
+// PcmSimple pairwise check (Pcm.h:654-662, 670-718), same tiling as
+// k_pairs64; consistent iff translation and rotation norms are both below
+// their thresholds; dist receives the translation norm.
+template <int DIM>
+__global__ void __launch_bounds__(256) k_pairs_simple(int m, long long n_tiles,
+                                                      const double* __restrict__ traj,
+                                                      const int* __restrict__ cf,
+                                                      const int* __restrict__ ct,
+                                                      const double* __restrict__ clcm,
+                                                      double thr_t, double thr_r, int W,
+                                                      unsigned long long* __restrict__ adj,
+                                                      double* __restrict__ dist) {
+  constexpr int PWC = R64<DIM>::PWC;
+  const long long tile = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (tile >= n_tiles) return;
+  const int lane = threadIdx.x & 63;
+  int j, w;
+  tile_of(tile, &j, &w);
+  if (j >= m) return;
+  const int i = w * 64 + lane;
+  const bool active = i < j;
+  bool cons = false;
+  if (active) {
+    double tn, rn;
+    R64<DIM>::simple_pair(traj, cf[i], ct[i], clcm + (size_t)i * PWC, cf[j], ct[j],
+                          clcm + (size_t)j * PWC, &tn, &rn);
+    cons = tn < thr_t && rn < thr_r;
+    if (dist) {
+      dist[(size_t)j * m + i] = tn;
+      dist[(size_t)i * m + j] = tn;
+    }
+  }
+  const unsigned long long bits = __ballot(cons);
+  if (lane == 0) adj[(size_t)j * W + w] = bits;
+}
+
 // Mirror the lower-triangle bits into the upper triangle: one wavefront per
 // 64x64 block (row block J, word W <= J); lane r holds row 64J + r.
 __global__ void __launch_bounds__(64) k_symmetrize(int m, int W, unsigned long long* adj) {
@@ -264,6 +330,79 @@ __global__ void k_recheck64(int n, const int2* __restrict__ queue, const double*
 }
 
 // ---------------------------------------------------------------------------
+// streaming append (one update() of the reference after another)
+// ---------------------------------------------------------------------------
+// accepted[m_old + t] = l0 + tmp[t] and the compacted records, t < m_new
+template <int DIM>
+__global__ void k_gather_append(int m_old, int m_new, int l0, const int* __restrict__ tmp,
+                                const int* __restrict__ from, const int* __restrict__ to,
+                                const double* __restrict__ lcm, int* __restrict__ accepted,
+                                int* __restrict__ cf, int* __restrict__ ct,
+                                double* __restrict__ clcm) {
+  constexpr int PWC = R64<DIM>::PWC;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m_new) return;
+  const int k = l0 + tmp[t];
+  const int o = m_old + t;
+  accepted[o] = k;
+  cf[o] = from[k];
+  ct[o] = to[k];
+  for (int e = 0; e < PWC; ++e) clcm[(size_t)o * PWC + e] = lcm[(size_t)k * PWC + e];
+}
+
+// copy rows [0, m_old) from stride W_old to stride W_new, zero-filling the
+// new words; one thread per destination word
+__global__ void k_adj_relayout(int m_old, int W_old, int W_new,
+                               const unsigned long long* __restrict__ src,
+                               unsigned long long* __restrict__ dst) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)m_old * W_new) return;
+  const int r = (int)(t / W_new), w = (int)(t % W_new);
+  dst[t] = w < W_old ? src[(size_t)r * W_old + w] : 0ull;
+}
+
+// New rows j in [m_old, m) against every i < j: one wavefront per (row,
+// word) with word <= j/64, lane = i - 64 word.  Both triangles are written
+// with atomicOr (rows of the new block receive bits from each other); the
+// new rows are zero on entry.  counter += consistent pairs.
+template <int DIM, bool SIMPLE>
+__global__ void __launch_bounds__(256) k_pairs_append(int m_old, int m, int W,
+                                                      const double* __restrict__ traj,
+                                                      const int* __restrict__ cf,
+                                                      const int* __restrict__ ct,
+                                                      const double* __restrict__ clcm,
+                                                      double thr, double thr_r,
+                                                      unsigned long long* __restrict__ adj,
+                                                      unsigned long long* __restrict__ counter) {
+  constexpr int PWC = R64<DIM>::PWC;
+  const long long wave = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (wave >= (long long)(m - m_old) * W) return;
+  const int j = m_old + (int)(wave / W), w = (int)(wave % W);
+  if (w > (j >> 6)) return;
+  const int lane = threadIdx.x & 63;
+  const int i = w * 64 + lane;
+  bool cons = false;
+  if (i < j) {
+    if (SIMPLE) {
+      double tn, rn;
+      R64<DIM>::simple_pair(traj, cf[i], ct[i], clcm + (size_t)i * PWC, cf[j], ct[j],
+                            clcm + (size_t)j * PWC, &tn, &rn);
+      cons = tn < thr && rn < thr_r;
+    } else {
+      const double d = R64<DIM>::pair_dist(traj, cf[i], ct[i], clcm + (size_t)i * PWC, cf[j],
+                                           ct[j], clcm + (size_t)j * PWC);
+      cons = d < thr;
+    }
+  }
+  if (cons) atomicOr(adj + (size_t)i * W + (j >> 6), 1ull << (j & 63));
+  const unsigned long long bits = __ballot(cons);
+  if (lane == 0 && bits) {
+    atomicOr(adj + (size_t)j * W + w, bits);
+    atomicAdd(counter, (unsigned long long)__popcll(bits));
+  }
+}
+
+// ---------------------------------------------------------------------------
 // host drivers
 // ---------------------------------------------------------------------------
 int pcm_recheck64(rpgo_ctx* ctx, int n_queue, double lc_thr, bool want_dist) {
@@ -284,6 +423,34 @@ int pcm_recheck64(rpgo_ctx* ctx, int n_queue, double lc_thr, bool want_dist) {
   return RPGO_OK;
 }
 
+int pcm_measure_range(rpgo_ctx* ctx, int o0, int o1, int l0, int l1) {
+  PcmState& s = ctx->pcm;
+  const int pwc = pwc_len(s.dim), D = tan_dim(s.dim), PL = pose_len(s.dim);
+  const int nd = o1 - o0, nl = l1 - l0;
+  if (nd > 0) {
+    const double* pose = s.odom.as<double>() + (size_t)o0 * PL;
+    const double* cov = s.odom_cov.as<double>() + (size_t)o0 * D * D;
+    double* pwc_out = s.delta.as<double>() + (size_t)o0 * pwc;
+    double* h_out = s.hinv.as<double>() + (size_t)o0 * D * D;
+    if (s.dim == 3)
+      k_measurements<3><<<(nd + 255) / 256, 256, 0, ctx->stream>>>(nd, pose, cov, pwc_out, h_out);
+    else
+      k_measurements<2><<<(nd + 255) / 256, 256, 0, ctx->stream>>>(nd, pose, cov, pwc_out, h_out);
+    RPGO_HIP_CHECK(hipGetLastError());
+  }
+  if (nl > 0) {
+    const double* pose = s.lc_pose.as<double>() + (size_t)l0 * PL;
+    const double* cov = s.lc_cov.as<double>() + (size_t)l0 * D * D;
+    double* pwc_out = s.lcm.as<double>() + (size_t)l0 * pwc;
+    if (s.dim == 3)
+      k_measurements<3><<<(nl + 255) / 256, 256, 0, ctx->stream>>>(nl, pose, cov, pwc_out, nullptr);
+    else
+      k_measurements<2><<<(nl + 255) / 256, 256, 0, ctx->stream>>>(nl, pose, cov, pwc_out, nullptr);
+    RPGO_HIP_CHECK(hipGetLastError());
+  }
+  return RPGO_OK;
+}
+
 int pcm_prepare(rpgo_ctx* ctx) {
   PcmState& s = ctx->pcm;
   const int nd = s.n_poses - 1;
@@ -292,25 +459,7 @@ int pcm_prepare(rpgo_ctx* ctx) {
   if ((rc = s.delta.reserve(sizeof(double) * (size_t)(nd > 0 ? nd : 1) * pwc))) return rc;
   if ((rc = s.hinv.reserve(sizeof(double) * (size_t)(nd > 0 ? nd : 1) * D * D))) return rc;
   if ((rc = s.lcm.reserve(sizeof(double) * (size_t)(s.n_lc > 0 ? s.n_lc : 1) * pwc))) return rc;
-  if (nd > 0) {
-    if (s.dim == 3)
-      k_measurements<3><<<(nd + 255) / 256, 256, 0, ctx->stream>>>(
-          nd, s.odom.as<double>(), s.odom_cov.as<double>(), s.delta.as<double>(), s.hinv.as<double>());
-    else
-      k_measurements<2><<<(nd + 255) / 256, 256, 0, ctx->stream>>>(
-          nd, s.odom.as<double>(), s.odom_cov.as<double>(), s.delta.as<double>(), s.hinv.as<double>());
-    RPGO_HIP_CHECK(hipGetLastError());
-  }
-  if (s.n_lc > 0) {
-    if (s.dim == 3)
-      k_measurements<3><<<(s.n_lc + 255) / 256, 256, 0, ctx->stream>>>(
-          s.n_lc, s.lc_pose.as<double>(), s.lc_cov.as<double>(), s.lcm.as<double>(), nullptr);
-    else
-      k_measurements<2><<<(s.n_lc + 255) / 256, 256, 0, ctx->stream>>>(
-          s.n_lc, s.lc_pose.as<double>(), s.lc_cov.as<double>(), s.lcm.as<double>(), nullptr);
-    RPGO_HIP_CHECK(hipGetLastError());
-  }
-  return RPGO_OK;
+  return pcm_measure_range(ctx, 0, nd, 0, s.n_lc);
 }
 
 int pcm_fold(rpgo_ctx* ctx) {
@@ -318,16 +467,29 @@ int pcm_fold(rpgo_ctx* ctx) {
   int rc;
   if ((rc = s.traj.reserve(sizeof(double) * (size_t)s.n_poses * pwc_len(s.dim)))) return rc;
   if (s.dim == 3)
-    k_fold<3><<<1, 64, 0, ctx->stream>>>(s.n_poses, s.pose0.as<double>(), s.delta.as<double>(),
+    k_fold<3><<<1, 64, 0, ctx->stream>>>(0, s.n_poses, s.pose0.as<double>(), s.delta.as<double>(),
                                          s.hinv.as<double>(), s.traj.as<double>());
   else
-    k_fold<2><<<1, 64, 0, ctx->stream>>>(s.n_poses, s.pose0.as<double>(), s.delta.as<double>(),
+    k_fold<2><<<1, 64, 0, ctx->stream>>>(0, s.n_poses, s.pose0.as<double>(), s.delta.as<double>(),
                                          s.hinv.as<double>(), s.traj.as<double>());
   RPGO_HIP_CHECK(hipGetLastError());
   return RPGO_OK;
 }
 
-int pcm_odom_and_compact(rpgo_ctx* ctx, double odom_thr, int* m_out) {
+int pcm_fold_from(rpgo_ctx* ctx, int start) {
+  PcmState& s = ctx->pcm;
+  if (start + 1 >= s.n_poses) return RPGO_OK;
+  if (s.dim == 3)
+    k_fold<3><<<1, 64, 0, ctx->stream>>>(start, s.n_poses, nullptr, s.delta.as<double>(),
+                                         s.hinv.as<double>(), s.traj.as<double>());
+  else
+    k_fold<2><<<1, 64, 0, ctx->stream>>>(start, s.n_poses, nullptr, s.delta.as<double>(),
+                                         s.hinv.as<double>(), s.traj.as<double>());
+  RPGO_HIP_CHECK(hipGetLastError());
+  return RPGO_OK;
+}
+
+int pcm_odom_and_compact(rpgo_ctx* ctx, double odom_thr, int* m_out, const SimpleThr* simple) {
   PcmState& s = ctx->pcm;
   const int n = s.n_lc;
   int rc;
@@ -342,14 +504,25 @@ int pcm_odom_and_compact(rpgo_ctx* ctx, double odom_thr, int* m_out) {
     *m_out = 0;
     return RPGO_OK;
   }
-  if (s.dim == 3)
+  if (simple) {
+    const bool check = simple->trans >= 0.0 && simple->rot >= 0.0;
+    if (s.dim == 3)
+      k_odom_simple<3><<<(n + 255) / 256, 256, 0, ctx->stream>>>(
+          n, s.traj.as<double>(), s.lc_from.as<int>(), s.lc_to.as<int>(), s.lcm.as<double>(),
+          check, simple->trans, simple->rot, s.odom_dist.as<double>(), s.flags.as<int>());
+    else
+      k_odom_simple<2><<<(n + 255) / 256, 256, 0, ctx->stream>>>(
+          n, s.traj.as<double>(), s.lc_from.as<int>(), s.lc_to.as<int>(), s.lcm.as<double>(),
+          check, simple->trans, simple->rot, s.odom_dist.as<double>(), s.flags.as<int>());
+  } else if (s.dim == 3) {
     k_odom_check<3><<<(n + 255) / 256, 256, 0, ctx->stream>>>(
         n, s.traj.as<double>(), s.lc_from.as<int>(), s.lc_to.as<int>(), s.lcm.as<double>(),
         odom_thr, s.odom_dist.as<double>(), s.flags.as<int>());
-  else
+  } else {
     k_odom_check<2><<<(n + 255) / 256, 256, 0, ctx->stream>>>(
         n, s.traj.as<double>(), s.lc_from.as<int>(), s.lc_to.as<int>(), s.lcm.as<double>(),
         odom_thr, s.odom_dist.as<double>(), s.flags.as<int>());
+  }
   RPGO_HIP_CHECK(hipGetLastError());
   k_compact<<<1, 1024, 0, ctx->stream>>>(n, s.flags.as<int>(), s.accepted.as<int>());
   RPGO_HIP_CHECK(hipGetLastError());
@@ -394,12 +567,145 @@ int pcm_pairs_fp64(rpgo_ctx* ctx, double lc_thr, bool want_dist) {
   return RPGO_OK;
 }
 
+int pcm_pairs_simple(rpgo_ctx* ctx, const SimpleThr& thr, bool want_dist) {
+  PcmState& s = ctx->pcm;
+  const int m = s.m;
+  if (m < 2) return RPGO_OK;
+  const int W = (m + 63) / 64;
+  const long long G = W;
+  const long long n_tiles = 32ll * G * (G + 1);
+  const long long blocks = (n_tiles + 3) / 4;
+  if (s.dim == 3)
+    k_pairs_simple<3><<<(unsigned)blocks, 256, 0, ctx->stream>>>(
+        m, n_tiles, s.traj.as<double>(), s.c_from.as<int>(), s.c_to.as<int>(),
+        s.c_lcm.as<double>(), thr.trans, thr.rot, W, s.adj.as<unsigned long long>(),
+        want_dist ? s.dist.as<double>() : nullptr);
+  else
+    k_pairs_simple<2><<<(unsigned)blocks, 256, 0, ctx->stream>>>(
+        m, n_tiles, s.traj.as<double>(), s.c_from.as<int>(), s.c_to.as<int>(),
+        s.c_lcm.as<double>(), thr.trans, thr.rot, W, s.adj.as<unsigned long long>(),
+        want_dist ? s.dist.as<double>() : nullptr);
+  RPGO_HIP_CHECK(hipGetLastError());
+  return RPGO_OK;
+}
+
 int pcm_symmetrize(rpgo_ctx* ctx) {
   PcmState& s = ctx->pcm;
   const int m = s.m;
   if (m < 2) return RPGO_OK;
   const int W = (m + 63) / 64;
   k_symmetrize<<<dim3(W, W), 64, 0, ctx->stream>>>(m, W, s.adj.as<unsigned long long>());
+  RPGO_HIP_CHECK(hipGetLastError());
+  return RPGO_OK;
+}
+
+int pcm_odom_append(rpgo_ctx* ctx, int l0, double odom_thr, const SimpleThr* simple, int* m_new) {
+  PcmState& s = ctx->pcm;
+  const int n = s.n_lc, nn = n - l0, m_old = s.m, pwc = pwc_len(s.dim);
+  int rc;
+  *m_new = 0;
+  if (nn <= 0) return RPGO_OK;
+  hipStream_t st = ctx->stream;
+  if ((rc = s.odom_dist.grow_keep(sizeof(double) * (size_t)(n + 1), sizeof(double) * l0, st))) return rc;
+  if ((rc = s.flags.grow_keep(sizeof(int) * (size_t)(n + 1), 0, st))) return rc;
+  if ((rc = s.tmp.reserve(sizeof(int) * (size_t)(nn + 1)))) return rc;
+  const size_t mcap = (size_t)m_old + nn + 1;
+  if ((rc = s.accepted.grow_keep(sizeof(int) * (size_t)(n + 1 > (int)mcap ? n + 1 : mcap),
+                                 sizeof(int) * m_old, st)))
+    return rc;
+  if ((rc = s.c_from.grow_keep(sizeof(int) * mcap, sizeof(int) * m_old, st))) return rc;
+  if ((rc = s.c_to.grow_keep(sizeof(int) * mcap, sizeof(int) * m_old, st))) return rc;
+  if ((rc = s.c_lcm.grow_keep(sizeof(double) * mcap * pwc, sizeof(double) * m_old * pwc, st))) return rc;
+  const int* from = s.lc_from.as<int>() + l0;
+  const int* to = s.lc_to.as<int>() + l0;
+  const double* lcm = s.lcm.as<double>() + (size_t)l0 * pwc;
+  double* od = s.odom_dist.as<double>() + l0;
+  int* fl = s.flags.as<int>() + l0;
+  const unsigned g = (unsigned)((nn + 255) / 256);
+  if (simple) {
+    const bool check = simple->trans >= 0.0 && simple->rot >= 0.0;
+    if (s.dim == 3)
+      k_odom_simple<3><<<g, 256, 0, st>>>(nn, s.traj.as<double>(), from, to, lcm, check, simple->trans,
+                                          simple->rot, od, fl);
+    else
+      k_odom_simple<2><<<g, 256, 0, st>>>(nn, s.traj.as<double>(), from, to, lcm, check, simple->trans,
+                                          simple->rot, od, fl);
+  } else if (s.dim == 3) {
+    k_odom_check<3><<<g, 256, 0, st>>>(nn, s.traj.as<double>(), from, to, lcm, odom_thr, od, fl);
+  } else {
+    k_odom_check<2><<<g, 256, 0, st>>>(nn, s.traj.as<double>(), from, to, lcm, odom_thr, od, fl);
+  }
+  RPGO_HIP_CHECK(hipGetLastError());
+  k_compact<<<1, 1024, 0, st>>>(nn, fl, s.tmp.as<int>());
+  RPGO_HIP_CHECK(hipGetLastError());
+  int mn = 0;
+  RPGO_HIP_CHECK(hipMemcpyAsync(&mn, s.tmp.as<int>() + nn, sizeof(int), hipMemcpyDeviceToHost, st));
+  RPGO_HIP_CHECK(hipStreamSynchronize(st));
+  if (mn > 0) {
+    if (s.dim == 3)
+      k_gather_append<3><<<(mn + 255) / 256, 256, 0, st>>>(
+          m_old, mn, l0, s.tmp.as<int>(), s.lc_from.as<int>(), s.lc_to.as<int>(), s.lcm.as<double>(),
+          s.accepted.as<int>(), s.c_from.as<int>(), s.c_to.as<int>(), s.c_lcm.as<double>());
+    else
+      k_gather_append<2><<<(mn + 255) / 256, 256, 0, st>>>(
+          m_old, mn, l0, s.tmp.as<int>(), s.lc_from.as<int>(), s.lc_to.as<int>(), s.lcm.as<double>(),
+          s.accepted.as<int>(), s.c_from.as<int>(), s.c_to.as<int>(), s.c_lcm.as<double>());
+    RPGO_HIP_CHECK(hipGetLastError());
+  }
+  *m_new = mn;
+  return RPGO_OK;
+}
+
+int pcm_adj_append(rpgo_ctx* ctx, int m_old, double lc_thr, const SimpleThr* simple) {
+  PcmState& s = ctx->pcm;
+  const int m = s.m;
+  if (m <= m_old) return RPGO_OK;
+  hipStream_t st = ctx->stream;
+  const int W_old = (m_old + 63) / 64, W = (m + 63) / 64;
+  int rc;
+  const size_t need = sizeof(uint64_t) * (size_t)m * W;
+  if (W != W_old && m_old > 0) {
+    // new stride: re-lay the old rows out into a fresh buffer
+    DevBuf fresh;
+    const size_t cap = need * 2;
+    if ((rc = fresh.reserve(cap))) return rc;
+    const long long words = (long long)m_old * W;
+    k_adj_relayout<<<(unsigned)((words + 255) / 256), 256, 0, st>>>(
+        m_old, W_old, W, s.adj.as<unsigned long long>(), fresh.as<unsigned long long>());
+    RPGO_HIP_CHECK(hipGetLastError());
+    RPGO_HIP_CHECK(hipStreamSynchronize(st));
+    s.adj.release();
+    s.adj = fresh;
+  } else {
+    if ((rc = s.adj.grow_keep(need, sizeof(uint64_t) * (size_t)m_old * W, st))) return rc;
+  }
+  RPGO_HIP_CHECK(hipMemsetAsync(s.adj.as<unsigned long long>() + (size_t)m_old * W, 0,
+                                sizeof(uint64_t) * (size_t)(m - m_old) * W, st));
+  const bool check = simple ? (simple->trans >= 0.0 && simple->rot >= 0.0) : lc_thr >= 0.0;
+  if (!check || m < 2) return RPGO_OK;
+  const long long waves = (long long)(m - m_old) * W;
+  const unsigned blocks = (unsigned)((waves + 3) / 4);
+  unsigned long long* cnt = s.counters.as<unsigned long long>();
+  const double t1 = simple ? simple->trans : lc_thr, t2 = simple ? simple->rot : 0.0;
+  if (s.dim == 3) {
+    if (simple)
+      k_pairs_append<3, true><<<blocks, 256, 0, st>>>(m_old, m, W, s.traj.as<double>(), s.c_from.as<int>(),
+                                                      s.c_to.as<int>(), s.c_lcm.as<double>(), t1, t2,
+                                                      s.adj.as<unsigned long long>(), cnt);
+    else
+      k_pairs_append<3, false><<<blocks, 256, 0, st>>>(m_old, m, W, s.traj.as<double>(), s.c_from.as<int>(),
+                                                       s.c_to.as<int>(), s.c_lcm.as<double>(), t1, t2,
+                                                       s.adj.as<unsigned long long>(), cnt);
+  } else {
+    if (simple)
+      k_pairs_append<2, true><<<blocks, 256, 0, st>>>(m_old, m, W, s.traj.as<double>(), s.c_from.as<int>(),
+                                                      s.c_to.as<int>(), s.c_lcm.as<double>(), t1, t2,
+                                                      s.adj.as<unsigned long long>(), cnt);
+    else
+      k_pairs_append<2, false><<<blocks, 256, 0, st>>>(m_old, m, W, s.traj.as<double>(), s.c_from.as<int>(),
+                                                       s.c_to.as<int>(), s.c_lcm.as<double>(), t1, t2,
+                                                       s.adj.as<unsigned long long>(), cnt);
+  }
   RPGO_HIP_CHECK(hipGetLastError());
   return RPGO_OK;
 }

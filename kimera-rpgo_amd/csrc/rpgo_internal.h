@@ -42,6 +42,31 @@ struct DevBuf {
     bytes = want;
     return RPGO_OK;
   }
+  // grow to at least n bytes (doubling), keeping the first `keep` bytes
+  int grow_keep(size_t n, size_t keep, hipStream_t st) {
+    if (n <= bytes) return RPGO_OK;
+    size_t want = bytes * 2 > n ? bytes * 2 : n;
+    if (want < 256) want = 256;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+      set_error("hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+      return RPGO_ERR_NOMEM;
+    }
+    if (ptr && keep) {
+      e = hipMemcpyAsync(p, ptr, keep < bytes ? keep : bytes, hipMemcpyDeviceToDevice, st);
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+      if (e != hipSuccess) {
+        set_error("grow_keep copy failed: %s", hipGetErrorString(e));
+        (void)hipFree(p);
+        return RPGO_ERR_HIP;
+      }
+    }
+    if (ptr) (void)hipFree(ptr);
+    ptr = p;
+    bytes = want;
+    return RPGO_OK;
+  }
   void release() {
     if (ptr) (void)hipFree(ptr);
     ptr = nullptr;
@@ -65,6 +90,7 @@ struct PcmState {
   int m = 0;  // accepted loop closures of the last run
   bool have_dist = false;
   int precision = 0;     // precision of the last run
+  bool simple = false;   // last run used the PcmSimple (PoseWithNode) checks
   bool fp32_ready = false;  // FP32 tables built for the current accepted set
   bool fp32_fallback = false;  // FP32 path declined (SE(2) or no rotation info)
   int fp32_nb = 0;       // anchor blocks of the FP32 tables
@@ -84,6 +110,7 @@ struct PcmState {
   DevBuf counters;   // device counters (recheck, flips, consistent)
   DevBuf fp32_buf;   // FP32-mode per-pose / per-LC tables
   DevBuf queue;      // FP32-mode recheck queue
+  DevBuf tmp;        // append: compaction of the new loop closures
 };
 
 struct CliqueState {
@@ -110,11 +137,31 @@ struct rpgo_ctx {
 };
 
 namespace rpgo {
+// PcmSimple thresholds of one check (translation, rotation per node);
+// a negative value disables the check
+struct SimpleThr {
+  double trans, rot;
+};
 // replay64.hip
 int pcm_prepare(rpgo_ctx* ctx);                  // delta/hinv/lcm precompute
 int pcm_fold(rpgo_ctx* ctx);                     // sequential FP64 fold
-int pcm_odom_and_compact(rpgo_ctx* ctx, double odom_thr, int* m_out);
+// odometry check (PCM Mahalanobis, or PcmSimple when simple != nullptr) and
+// stable compaction of the accepted loop closures
+int pcm_odom_and_compact(rpgo_ctx* ctx, double odom_thr, int* m_out,
+                         const SimpleThr* simple = nullptr);
 int pcm_pairs_fp64(rpgo_ctx* ctx, double lc_thr, bool want_dist);
+int pcm_pairs_simple(rpgo_ctx* ctx, const SimpleThr& thr, bool want_dist);
+// measurement records of odometry [o0, o1) and loop closures [l0, l1)
+int pcm_measure_range(rpgo_ctx* ctx, int o0, int o1, int l0, int l1);
+// continue the fold from chain record `start` up to record n_poses-1
+int pcm_fold_from(rpgo_ctx* ctx, int start);
+// odometry check of loop closures [l0, s.n_lc); accepted ones are appended
+// to the compacted set (s.accepted, c_from, c_to, c_lcm); *m_new receives
+// their count.  s.m is not changed.
+int pcm_odom_append(rpgo_ctx* ctx, int l0, double odom_thr, const SimpleThr* simple, int* m_new);
+// adjacency rows [m_old, s.m) against every earlier accepted loop closure,
+// both triangles; the adjacency is re-laid out when W = ceil(m/64) grows
+int pcm_adj_append(rpgo_ctx* ctx, int m_old, double lc_thr, const SimpleThr* simple);
 int pcm_symmetrize(rpgo_ctx* ctx);
 // FP64 replay of queued (i, j) pairs; patches the adjacency bits, counts flips
 int pcm_recheck64(rpgo_ctx* ctx, int n_queue, double lc_thr, bool want_dist);
@@ -136,6 +183,12 @@ void batch_destroy(rpgo_ctx* ctx);  // batch.hip
 int clique_incremental(rpgo_ctx* ctx, const uint64_t* d_adj_u64, int n, int num_new, int prev_size,
                        int32_t* idx_out, int32_t* k_out);
 // clique_exact.hip: exact maximum clique given a clique size known to exist
+// batched FMC::maxCliqueHeu, one workgroup of `waves` wavefronts (4, 8, 16)
+// per graph; out_k[g] = -1 for an empty graph, INT_MIN on an internal error
+int clique_batch_launch(rpgo_ctx* ctx, int n_graphs, int waves, const long long* d_adj_off,
+                        const long long* d_lc_off, const int* d_m, const unsigned long long* adj, int* deg,
+                        int* out_idx, int* out_k);
+int clique_max_vertices();
 int clique_exact_run(rpgo_ctx* ctx, const unsigned long long* adj, int n, int lower_bound, int32_t* idx_out,
                      int32_t* k_out, long long node_budget);
 // clique.hip

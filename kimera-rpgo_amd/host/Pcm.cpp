@@ -1,7 +1,7 @@
 // PCM outlier-rejection plugin (see OutlierRemoval.h): host bookkeeping of
 // outlier/Pcm.h:148-1005 for one robot prefix over the device stages
-// rpgo_pcm_adjacency (odometry check + pairwise consistency) and
-// rpgo_max_clique (FMC::maxCliqueHeu).
+// rpgo_pcm_adjacency / rpgo_pcm_simple_adjacency (odometry check + pairwise
+// consistency) and rpgo_max_clique (FMC::maxCliqueHeu).
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -28,8 +28,18 @@ void put_pose(const Factor& f, int dim, double* o) {
 
 }  // namespace
 
-Pcm::Pcm(int dim, const PcmParams& params, const std::vector<char>& special_symbols, const DeviceParams& device)
-    : dim_(dim), params_(params), special_symbols_(special_symbols), device_(device) {
+Pcm::Pcm(int dim, const PcmParams& params, const std::vector<char>& special_symbols, const DeviceParams& device,
+         bool simple)
+    : dim_(dim),
+      simple_(simple),
+      // Pcm.h:74-82: any negative threshold of a check turns that check off
+      odom_check_(!(params.odom_threshold < 0 || params.odom_rot_threshold < 0 ||
+                    params.odom_trans_threshold < 0)),
+      loop_check_(!(params.lc_threshold < 0 || params.dist_rot_threshold < 0 ||
+                    params.dist_trans_threshold < 0)),
+      params_(params),
+      special_symbols_(special_symbols),
+      device_(device) {
   if (dim != 2 && dim != 3) throw std::invalid_argument("Pcm: dim must be 2 or 3");
   check(rpgo_ctx_create(device.device, 0, &ctx_), "rpgo_ctx_create");
 }
@@ -164,21 +174,31 @@ void Pcm::runPcm(const std::vector<FactorPtr>& pending) {
   prob.lc_to = to.data();
   prob.lc_pose = lpose.data();
   prob.lc_cov = lcov.data();
-  rpgo_pcm_params pp{};
-  pp.odom_threshold = params_.odom_threshold;
-  pp.lc_threshold = params_.lc_threshold;
-  pp.precision = device_.pcm_precision;
   rpgo_pcm_stats st{};
   std::vector<int32_t> accepted(m);
-  check(rpgo_pcm_adjacency(ctx_, &prob, &pp, accepted.data(), nullptr, nullptr, nullptr, &st),
-        "rpgo_pcm_adjacency");
+  if (simple_) {
+    rpgo_pcm_simple_params sp{};
+    sp.odom_trans_threshold = odom_check_ ? params_.odom_trans_threshold : -1.0;
+    sp.odom_rot_threshold = odom_check_ ? params_.odom_rot_threshold : -1.0;
+    sp.dist_trans_threshold = loop_check_ ? params_.dist_trans_threshold : -1.0;
+    sp.dist_rot_threshold = loop_check_ ? params_.dist_rot_threshold : -1.0;
+    check(rpgo_pcm_simple_adjacency(ctx_, &prob, &sp, accepted.data(), nullptr, nullptr, nullptr, &st),
+          "rpgo_pcm_simple_adjacency");
+  } else {
+    rpgo_pcm_params pp{};
+    pp.odom_threshold = odom_check_ ? params_.odom_threshold : -1.0;
+    pp.lc_threshold = loop_check_ ? params_.lc_threshold : -1.0;
+    pp.precision = device_.pcm_precision;
+    check(rpgo_pcm_adjacency(ctx_, &prob, &pp, accepted.data(), nullptr, nullptr, nullptr, &st),
+          "rpgo_pcm_adjacency");
+  }
   lc_.clear();
   for (int k = 0; k < st.n_accepted; ++k) lc_.push_back(all[accepted[k]]);
   pcm_ms_ = st.ms_total;
   pairs_ = st.n_pairs;
   inliers_.clear();
   const int ma = (int)lc_.size();
-  if (params_.lc_threshold < 0) {
+  if (!loop_check_) {
     for (int k = 0; k < ma; ++k) inliers_.push_back(k);  // Pcm.h:870-873
   } else if (ma > 0) {
     std::vector<int32_t> idx(ma);

@@ -58,8 +58,12 @@ RobustSolver::RobustSolver(const RobustSolverParams& params) : params_(params) {
     case OutlierRemovalMethod::PCM3D:
       outlier_removal_.reset(new Pcm3D(params.pcm_params, params.specialSymbols, params.device_params));
       break;
-    default:
-      throw std::invalid_argument("RobustSolver: PcmSimple outlier removal is not available in this build");
+    case OutlierRemovalMethod::PCM_Simple2D:
+      outlier_removal_.reset(new PcmSimple2D(params.pcm_params, params.specialSymbols, params.device_params));
+      break;
+    case OutlierRemovalMethod::PCM_Simple3D:
+      outlier_removal_.reset(new PcmSimple3D(params.pcm_params, params.specialSymbols, params.device_params));
+      break;
   }
   if (params.verbosity != Verbosity::VERBOSE && outlier_removal_) outlier_removal_->setQuiet();
   if (params.verbosity == Verbosity::QUIET) debug_ = false;

@@ -38,6 +38,17 @@ class PcmParams(ctypes.Structure):
                 ("recheck_margin", ctypes.c_double), ("recheck_chain_coef", ctypes.c_double)]
 
 
+class PcmSimpleParams(ctypes.Structure):
+    _fields_ = [("odom_trans_threshold", ctypes.c_double), ("odom_rot_threshold", ctypes.c_double),
+                ("dist_trans_threshold", ctypes.c_double), ("dist_rot_threshold", ctypes.c_double),
+                ("want_dist", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class PcmIncrement(ctypes.Structure):
+    _fields_ = [("n_odom", ctypes.c_int32), ("odom", _d), ("odom_cov", _d), ("n_lc", ctypes.c_int32),
+                ("lc_from", _i32), ("lc_to", _i32), ("lc_pose", _d), ("lc_cov", _d)]
+
+
 class PcmStats(ctypes.Structure):
     _fields_ = [("n_accepted", ctypes.c_int32), ("reserved", ctypes.c_int32),
                 ("n_pairs", ctypes.c_int64), ("n_rechecked", ctypes.c_int64),
@@ -61,13 +72,19 @@ GNC_PROBABILITY = 1
 class PcmBatch(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int32), ("n_graphs", ctypes.c_int32), ("pose_off", ctypes.POINTER(ctypes.c_int64)),
                 ("pose0", _d), ("odom", _d), ("odom_cov", _d), ("lc_off", ctypes.POINTER(ctypes.c_int64)),
-                ("lc_from", _i32), ("lc_to", _i32), ("lc_pose", _d), ("lc_cov", _d)]
+                ("lc_from", _i32), ("lc_to", _i32), ("lc_pose", _d), ("lc_cov", _d), ("flags", ctypes.c_int32)]
+
+
+BATCH_GRAPH = 1
+BATCH_AUTOTUNE = 2
 
 
 class PcmBatchStats(ctypes.Structure):
     _fields_ = [("n_pairs", ctypes.c_int64), ("n_consistent", ctypes.c_int64), ("ms_fold", ctypes.c_double),
                 ("ms_odom", ctypes.c_double), ("ms_pairs", ctypes.c_double), ("ms_clique", ctypes.c_double),
-                ("ms_total", ctypes.c_double)]
+                ("ms_total", ctypes.c_double), ("ms_tune", ctypes.c_double), ("pair_waves", ctypes.c_int32),
+                ("clique_waves", ctypes.c_int32), ("graph_replayed", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -124,6 +141,15 @@ SIGNATURES = [
     ("rpgo_pcm_adjacency", [ctypes.c_void_p, ctypes.POINTER(PcmProblem), ctypes.POINTER(PcmParams),
                             _i32, _d, _u64, _d, ctypes.POINTER(PcmStats)], ctypes.c_int),
     ("rpgo_pcm_download_traj", [ctypes.c_void_p, _d], ctypes.c_int),
+    ("rpgo_pcm_append", [ctypes.c_void_p, ctypes.POINTER(PcmIncrement), ctypes.POINTER(PcmParams),
+                         ctypes.POINTER(PcmStats)], ctypes.c_int),
+    ("rpgo_pcm_simple_append", [ctypes.c_void_p, ctypes.POINTER(PcmIncrement),
+                                ctypes.POINTER(PcmSimpleParams), ctypes.POINTER(PcmStats)], ctypes.c_int),
+    ("rpgo_pcm_simple_run", [ctypes.c_void_p, ctypes.POINTER(PcmSimpleParams), ctypes.POINTER(PcmStats)],
+     ctypes.c_int),
+    ("rpgo_pcm_simple_adjacency", [ctypes.c_void_p, ctypes.POINTER(PcmProblem),
+                                   ctypes.POINTER(PcmSimpleParams), _i32, _d, _u64, _d,
+                                   ctypes.POINTER(PcmStats)], ctypes.c_int),
     ("rpgo_max_clique", [ctypes.c_void_p, ctypes.c_int32, _i32, _i32, _d], ctypes.c_int),
     ("rpgo_max_clique_bits", [ctypes.c_void_p, ctypes.c_int32, _u64, ctypes.c_int32, _i32, _i32],
      ctypes.c_int),
@@ -235,6 +261,52 @@ class Context:
         self._m = st.n_accepted
         return st
 
+    def pcm_simple_run(self, odom_trans, odom_rot, dist_trans, dist_rot, want_dist=False):
+        """PcmSimple stages (PoseWithNode distances) on the uploaded problem."""
+        prm = PcmSimpleParams(odom_trans_threshold=odom_trans, odom_rot_threshold=odom_rot,
+                              dist_trans_threshold=dist_trans, dist_rot_threshold=dist_rot,
+                              want_dist=1 if want_dist else 0)
+        st = PcmStats()
+        check(lib().rpgo_pcm_simple_run(self._h, ctypes.byref(prm), ctypes.byref(st)))
+        self._m = st.n_accepted
+        return st
+
+    def _increment(self, odom=None, odom_cov=None, lc_from=(), lc_to=(), lc_pose=None, lc_cov=None):
+        pl, dd = (12, 36) if self._dim == 3 else (4, 9)
+        f64 = lambda a, w: np.ascontiguousarray(np.zeros((0, w)) if a is None else a, dtype=np.float64).reshape(-1, w)
+        i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32).reshape(-1)
+        arrs = dict(odom=f64(odom, pl), odom_cov=f64(odom_cov, dd), lc_from=i32(lc_from), lc_to=i32(lc_to),
+                    lc_pose=f64(lc_pose, pl), lc_cov=f64(lc_cov, dd))
+        inc = PcmIncrement(n_odom=arrs["odom"].shape[0], odom=_ptr(arrs["odom"]), odom_cov=_ptr(arrs["odom_cov"]),
+                           n_lc=arrs["lc_from"].shape[0], lc_from=_ptr(arrs["lc_from"], _i32),
+                           lc_to=_ptr(arrs["lc_to"], _i32), lc_pose=_ptr(arrs["lc_pose"]),
+                           lc_cov=_ptr(arrs["lc_cov"]))
+        return inc, arrs
+
+    def pcm_append(self, odom_thr, lc_thr, **increment):
+        """Stream one increment (new odometry at the chain end, new loop closures) into
+        the device-resident PCM state; returns the stats of this call."""
+        inc, arrs = self._increment(**increment)
+        prm = PcmParams(odom_threshold=odom_thr, lc_threshold=lc_thr, precision=PCM_FP64_REPLAY, want_dist=0)
+        st = PcmStats()
+        check(lib().rpgo_pcm_append(self._h, ctypes.byref(inc), ctypes.byref(prm), ctypes.byref(st)))
+        self._after_append(arrs, st)
+        return st
+
+    def pcm_simple_append(self, odom_trans, odom_rot, dist_trans, dist_rot, **increment):
+        inc, arrs = self._increment(**increment)
+        prm = PcmSimpleParams(odom_trans_threshold=odom_trans, odom_rot_threshold=odom_rot,
+                              dist_trans_threshold=dist_trans, dist_rot_threshold=dist_rot, want_dist=0)
+        st = PcmStats()
+        check(lib().rpgo_pcm_simple_append(self._h, ctypes.byref(inc), ctypes.byref(prm), ctypes.byref(st)))
+        self._after_append(arrs, st)
+        return st
+
+    def _after_append(self, arrs, st):
+        self._n_poses += arrs["odom"].shape[0]
+        self._n_lc += arrs["lc_from"].shape[0]
+        self._m = st.n_accepted
+
     def pcm_pairs(self, lc_thr, precision=PCM_FP64_REPLAY, want_dist=False, margin=0.0,
                   chain_coef=0.0):
         """Re-run only the pairwise stage on the last run's device-resident data."""
@@ -304,7 +376,7 @@ class Context:
         return out, w[:F].copy(), rep
 
     # ------------------------------------------------------ batched PCM
-    def pcm_batch_run(self, graphs, odom_thr, lc_thr, clique=True):
+    def pcm_batch_run(self, graphs, odom_thr, lc_thr, clique=True, flags=0):
         """graphs: sequence of synth.PoseGraph (same dim).  Returns dict with per-graph
         accepted lists, clique sizes / members and the batch stats."""
         dim = graphs[0].dim
@@ -324,7 +396,7 @@ class Context:
                      pose0=_ptr(arrs["pose0"]), odom=_ptr(arrs["odom"]), odom_cov=_ptr(arrs["odom_cov"]),
                      lc_off=lc_off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
                      lc_from=_ptr(arrs["lc_from"], _i32), lc_to=_ptr(arrs["lc_to"], _i32),
-                     lc_pose=_ptr(arrs["lc_pose"]), lc_cov=_ptr(arrs["lc_cov"]))
+                     lc_pose=_ptr(arrs["lc_pose"]), lc_cov=_ptr(arrs["lc_cov"]), flags=int(flags))
         prm = PcmParams(odom_threshold=odom_thr, lc_threshold=lc_thr, precision=PCM_FP64_REPLAY, want_dist=0)
         n_acc = np.zeros(G, np.int32)
         acc = np.zeros(max(int(lc_off[-1]), 1), np.int32)

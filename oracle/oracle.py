@@ -73,6 +73,14 @@ def lib():
                                                 ctypes.c_int, _i32], ctypes.c_int),
             ("orc_max_clique_exact", [ctypes.c_int, _i32, _i32, _i32], ctypes.c_int),
             ("orc_max_clique_heu_dense", [ctypes.c_int, _u8, _i32], ctypes.c_int),
+            ("orc_pwn_norms", [ctypes.c_int, _d, ctypes.c_int, ctypes.c_int, _d, _d], None),
+            ("orc_simple_odom", [ctypes.c_int, _d, ctypes.c_int, ctypes.c_int, _d, _d, _d],
+             ctypes.c_double),
+            ("orc_simple_pair", [ctypes.c_int, _d, ctypes.c_int, ctypes.c_int, _d, _d,
+                                 ctypes.c_int, ctypes.c_int, _d, _d, _d], ctypes.c_double),
+            ("orc_pcm_simple_adjacency", [ctypes.c_int, ctypes.c_int, _d, ctypes.c_int, _i32, _i32,
+                                          _d, _d] + [ctypes.c_double] * 4 + [_i32, _d, _u8, _d],
+             ctypes.c_int),
             ("orc_set_threads", [ctypes.c_int], None),
             ("orc_get_threads", [], ctypes.c_int),
             ("orc_pair_dist_list", [ctypes.c_int, _d, _i32, _i32, _d, _d, ctypes.c_int, _i32, _i32, _d],
@@ -227,6 +235,57 @@ def pcm_adjacency(dim, traj, lc_from, lc_to, lc_pose, lc_cov, odom_thr, lc_thr, 
                                 _p(lc_pose), _p(lc_cov), float(odom_thr), float(lc_thr),
                                 _p(acc, _i32), _p(od),
                                 _p(adj, _u8), _p(dist) if dist is not None else None)
+    adj = adj.reshape(-1)[: m * m].reshape(m, m).copy()
+    if dist is not None:
+        dist = dist.reshape(-1)[: m * m].reshape(m, m).copy()
+    return acc[:m].copy(), od[:n].copy(), adj, dist
+
+
+# ---------------------------------------------------------------- PcmSimple
+def pwn_norms(dim, pose, node, rot_info=True):
+    """(avg_trans_norm, avg_rot_norm) of a PoseWithNode (GeometryUtils.h:275-289)."""
+    t = ctypes.c_double()
+    r = ctypes.c_double()
+    lib().orc_pwn_norms(dim, _p(_f64(pose)), int(node), int(bool(rot_info)), ctypes.byref(t),
+                        ctypes.byref(r))
+    return t.value, r.value
+
+
+def simple_odom(dim, traj, frm, to, lc_pose, lc_cov):
+    """(translation, rotation) norms of the PcmSimple odometry check."""
+    r = ctypes.c_double()
+    t = lib().orc_simple_odom(dim, _p(_f64(traj)), int(frm), int(to), _p(_f64(lc_pose)),
+                              _p(_f64(np.asarray(lc_cov).reshape(-1))), ctypes.byref(r))
+    return t, r.value
+
+
+def simple_pair(dim, traj, a, b, zab, sab, c, d, zcd, scd):
+    r = ctypes.c_double()
+    t = lib().orc_simple_pair(dim, _p(_f64(traj)), int(a), int(b), _p(_f64(zab)),
+                              _p(_f64(np.asarray(sab).reshape(-1))), int(c), int(d),
+                              _p(_f64(zcd)), _p(_f64(np.asarray(scd).reshape(-1))), ctypes.byref(r))
+    return t, r.value
+
+
+def pcm_simple_adjacency(dim, traj, lc_from, lc_to, lc_pose, lc_cov, odom_trans, odom_rot,
+                         dist_trans, dist_rot, want_dist=True):
+    """PcmSimple adjacency stage; same return layout as pcm_adjacency (distances are
+    the translation norms)."""
+    traj = _f64(traj)
+    lc_from = np.ascontiguousarray(lc_from, dtype=np.int32)
+    lc_to = np.ascontiguousarray(lc_to, dtype=np.int32)
+    n = lc_from.shape[0]
+    lc_pose = _f64(lc_pose).reshape(n, pose_len(dim))
+    lc_cov = _f64(lc_cov).reshape(n, tdim(dim) ** 2)
+    acc = np.zeros(max(n, 1), dtype=np.int32)
+    od = np.zeros(max(n, 1))
+    adj = np.zeros((max(n, 1), max(n, 1)), dtype=np.uint8)
+    dist = np.zeros((max(n, 1), max(n, 1))) if want_dist else None
+    m = lib().orc_pcm_simple_adjacency(dim, traj.shape[0], _p(traj), n, _p(lc_from, _i32),
+                                       _p(lc_to, _i32), _p(lc_pose), _p(lc_cov), float(odom_trans),
+                                       float(odom_rot), float(dist_trans), float(dist_rot),
+                                       _p(acc, _i32), _p(od), _p(adj, _u8),
+                                       _p(dist) if dist is not None else None)
     adj = adj.reshape(-1)[: m * m].reshape(m, m).copy()
     if dist is not None:
         dist = dist.reshape(-1)[: m * m].reshape(m, m).copy()

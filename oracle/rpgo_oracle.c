@@ -11,6 +11,8 @@
  *   include/KimeraRPGO/outlier/Pcm.h:604-629          isOdomConsistent
  *   include/KimeraRPGO/outlier/Pcm.h:670-718          areLoopsConsistent
  *   include/KimeraRPGO/outlier/Pcm.h:725-768          incrementAdjMatrix
+ *   include/KimeraRPGO/utils/GeometryUtils.h:193-289  PoseWithNode
+ *   include/KimeraRPGO/outlier/Pcm.h:581-596,654-662  PcmSimple checks
  *   include/KimeraRPGO/max_clique_finder/graphIO.cpp:188-257
  *   include/KimeraRPGO/max_clique_finder/findCliqueHeu.cpp:32-209
  *   include/KimeraRPGO/max_clique_finder/findClique.cpp:31-145
@@ -611,6 +613,139 @@ void orc_pcm_dist_rows(int dim, const double* traj, int m,
       dist[(size_t)j * m + i] = dv;
       dist[(size_t)i * m + j] = dv;
     }
+}
+
+/* ------------------------------------------------------------------ */
+/* PcmSimple (PoseWithNode, GeometryUtils.h:193-289)                    */
+/* ------------------------------------------------------------------ */
+
+/* avg_trans_norm / avg_rot_norm : GeometryUtils.h:275-289.  The norms take
+ * tail(translation size) and head(rotation dimension) of T::Logmap: for
+ * Pose3 (omega, v) that is |v| and |omega|; for Pose2 (v_x, v_y, theta) it is
+ * |(v_y, theta)| and |v_x| (translation size 2, Rot2 dimension 1). */
+void orc_pwn_norms(int dim, const double* pose, int node, int rot_info, double* trans,
+                   double* rot) {
+  const int d = tan_dim(dim), td = dim == 3 ? 3 : 2, rd = dim == 3 ? 3 : 1;
+  double xi[6];
+  pose_logmap(dim, pose, xi);
+  double st = xi[d - td] * xi[d - td];
+  for (int k = d - td + 1; k < d; ++k) st += xi[k] * xi[k];
+  *trans = sqrt(st) / node;
+  if (!rot_info) {
+    *rot = 0.0;
+    return;
+  }
+  double sr = xi[0] * xi[0];
+  for (int k = 1; k < rd; ++k) sr += xi[k] * xi[k];
+  *rot = sqrt(sr) / node;
+}
+
+/* Trajectory records come from orc_fold: the pose chain and rotation_info
+ * flags of PoseWithNode equal those of PoseWithCovariance (compose / between
+ * / inverse move the pose the same way and AND the flags the same way), and
+ * the node count of chain pose k is k (Pcm.h:534-556: node 0 at the chain
+ * start, 1 per odometry factor). */
+static int pwn_flag(int dim, const double* rec) {
+  const int d = tan_dim(dim);
+  return rec[pose_len(dim) + d * d] != 0.0;
+}
+
+/* isOdomConsistent with checkOdomConsistent(PoseWithNode) : Pcm.h:581-596,
+ * 604-629.  result = getBetween(i, j).compose(LC.inverse()), node |j-i| + 1. */
+double orc_simple_odom(int dim, const double* traj, int from, int to, const double* lc_pose,
+                       const double* lc_cov, double* rot_out) {
+  const int ps = orc_pwc_size(dim);
+  double lc[64], ainv[12], pij[12], lcinv[12], res[12];
+  orc_pwc_from_measurement(dim, lc_pose, lc_cov, lc);
+  const double* Pi = traj + (size_t)from * ps;
+  const double* Pj = traj + (size_t)to * ps;
+  pose_inverse(dim, Pi, ainv);
+  pose_compose(dim, ainv, Pj, pij);
+  pose_inverse(dim, lc, lcinv);
+  pose_compose(dim, pij, lcinv, res);
+  const int node = abs(to - from) + 1;
+  const int ri = pwn_flag(dim, Pi) && pwn_flag(dim, Pj) && pwn_flag(dim, lc);
+  double t, r;
+  orc_pwn_norms(dim, res, node, ri, &t, &r);
+  if (rot_out) *rot_out = r;
+  return t;
+}
+
+/* areLoopsConsistent with checkLoopConsistent(PoseWithNode) : Pcm.h:654-662,
+ * 670-718; node |c-a| + 1 + 1 + |d-b|. */
+double orc_simple_pair(int dim, const double* traj, int a, int b, const double* zab,
+                       const double* sab, int c, int d_, const double* zcd, const double* scd,
+                       double* rot_out) {
+  const int ps = orc_pwc_size(dim);
+  double alcb[64], cld[64], inv[12], A[12], B[12], X[12], Y[12], W[12], L[12];
+  orc_pwc_from_measurement(dim, zab, sab, alcb);
+  orc_pwc_from_measurement(dim, zcd, scd, cld);
+  const double* Pa = traj + (size_t)a * ps;
+  const double* Pb = traj + (size_t)b * ps;
+  const double* Pc = traj + (size_t)c * ps;
+  const double* Pd = traj + (size_t)d_ * ps;
+  pose_inverse(dim, Pa, inv);
+  pose_compose(dim, inv, Pc, A);
+  pose_compose(dim, A, cld, X);
+  pose_inverse(dim, X, Y);
+  pose_compose(dim, Y, alcb, W);
+  pose_inverse(dim, Pb, inv);
+  pose_compose(dim, inv, Pd, B);
+  pose_compose(dim, W, B, L);
+  const int node = abs(c - a) + 2 + abs(d_ - b);
+  const int ri = pwn_flag(dim, Pa) && pwn_flag(dim, Pc) && pwn_flag(dim, cld) &&
+                 pwn_flag(dim, alcb) && pwn_flag(dim, Pb) && pwn_flag(dim, Pd);
+  double t, r;
+  orc_pwn_norms(dim, L, node, ri, &t, &r);
+  if (rot_out) *rot_out = r;
+  return t;
+}
+
+int orc_pcm_simple_adjacency(int dim, int n_poses, const double* traj, int n_lc,
+                             const int32_t* lc_from, const int32_t* lc_to, const double* lc_pose,
+                             const double* lc_cov, double odom_trans, double odom_rot,
+                             double dist_trans, double dist_rot, int32_t* accepted,
+                             double* odom_dist_out, uint8_t* adj, double* dist) {
+  const int pl = pose_len(dim), dd = tan_dim(dim) * tan_dim(dim);
+  (void)n_poses;
+  const int odom_check = odom_trans >= 0.0 && odom_rot >= 0.0;
+  const int loop_check = dist_trans >= 0.0 && dist_rot >= 0.0;
+  int m = 0;
+  for (int k = 0; k < n_lc; ++k) {
+    int ok = 1;
+    double od = 0.0;
+    if (odom_check) {
+      double rot = 0.0;
+      od = orc_simple_odom(dim, traj, lc_from[k], lc_to[k], lc_pose + (size_t)k * pl,
+                           lc_cov + (size_t)k * dd, &rot);
+      ok = od < odom_trans && rot < odom_rot;
+    }
+    if (odom_dist_out) odom_dist_out[k] = od;
+    if (ok) accepted[m++] = k;
+  }
+  if (adj) memset(adj, 0, (size_t)m * m);
+  if (dist) memset(dist, 0, sizeof(double) * (size_t)m * m);
+  if (!loop_check) return m;
+  for (int j = 1; j < m; ++j) {
+    const int kj = accepted[j];
+    for (int i = 0; i < j; ++i) {
+      const int ki = accepted[i];
+      double rot = 0.0;
+      const double t = orc_simple_pair(dim, traj, lc_from[ki], lc_to[ki],
+                                       lc_pose + (size_t)ki * pl, lc_cov + (size_t)ki * dd,
+                                       lc_from[kj], lc_to[kj], lc_pose + (size_t)kj * pl,
+                                       lc_cov + (size_t)kj * dd, &rot);
+      if (dist) {
+        dist[(size_t)j * m + i] = t;
+        dist[(size_t)i * m + j] = t;
+      }
+      if (adj && t < dist_trans && rot < dist_rot) {
+        adj[(size_t)j * m + i] = 1;
+        adj[(size_t)i * m + j] = 1;
+      }
+    }
+  }
+  return m;
 }
 
 /* ------------------------------------------------------------------ */

@@ -100,6 +100,28 @@ void orc_pair_dist_list(int dim, const double* traj, const int32_t* lc_from,
                         const int32_t* lc_to, const double* lc_pose, const double* lc_cov,
                         int npairs, const int32_t* pi, const int32_t* pj, double* out);
 
+/* ---- PcmSimple (PoseWithNode: GeometryUtils.h:193-289, Pcm.h:581-596,654-662) ---- */
+
+/* avg_trans_norm / avg_rot_norm of a pose with node count `node`. */
+void orc_pwn_norms(int dim, const double* pose, int node, int rot_info, double* trans,
+                   double* rot);
+/* Odometry check distances of one LC on an orc_fold trajectory: returns the
+ * translation norm, *rot_out the rotation norm. */
+double orc_simple_odom(int dim, const double* traj, int from, int to, const double* lc_pose,
+                       const double* lc_cov, double* rot_out);
+/* Pairwise distances, older LC (a, b) and newer LC (c, d). */
+double orc_simple_pair(int dim, const double* traj, int a, int b, const double* zab,
+                       const double* sab, int c, int d, const double* zcd, const double* scd,
+                       double* rot_out);
+/* Whole adjacency stage with PcmSimple checks: a negative odometry threshold
+ * (either one) disables the odometry check, a negative distance threshold the
+ * pairwise check (Pcm.h:74-82).  dist_out holds translation norms. */
+int orc_pcm_simple_adjacency(int dim, int n_poses, const double* traj, int n_lc,
+                             const int32_t* lc_from, const int32_t* lc_to, const double* lc_pose,
+                             const double* lc_cov, double odom_trans, double odom_rot,
+                             double dist_trans, double dist_rot, int32_t* accepted_out,
+                             double* odom_dist_out, uint8_t* adj_out, double* dist_out);
+
 /* ---- max clique (findCliqueHeu.cpp / findClique.cpp / graphIO.cpp) ---- */
 
 /* CSR from a dense symmetric byte adjacency (graphIO.cpp:188-230).

@@ -21,7 +21,8 @@ def _graphs3(seed, n):
     rng = np.random.default_rng(seed)
     out = []
     for k in range(n):
-        npz = int(np.exp(rng.uniform(np.log(60), np.log(1500))))
+        # the helix generator needs more than one turn (100 poses) for inliers
+        npz = int(np.exp(rng.uniform(np.log(150), np.log(1500))))
         out.append(synth.se3_helix(n_poses=npz, n_lc=max(1, int(npz * rng.uniform(0.1, 0.3))),
                                    outlier_frac=float(rng.uniform(0.05, 0.4)), seed=seed * 100 + k))
     return out
@@ -67,9 +68,53 @@ def test_batch_se2_matches_single(ctx):
 
 def test_batch_single_pose_and_empty_graphs(ctx):
     graphs = _graphs3(9, 3)
-    tiny = synth.se3_helix(n_poses=5, n_lc=1, outlier_frac=0.0, seed=1)
+    tiny = synth.se3_helix(n_poses=5, n_lc=1, outlier_frac=1.0, seed=1)  # one outlier loop closure
     res = ctx.pcm_batch_run([graphs[0], tiny, graphs[1]], -1.0, 5.0)
     for g, gr in enumerate([graphs[0], tiny, graphs[1]]):
         acc, adj, k, idx = _single(ctx, gr, -1.0, 5.0)
         assert np.array_equal(res["accepted"][g], acc)
         assert int(res["clique_size"][g]) == k
+
+
+def _assert_same(res, ref):
+    assert np.array_equal(res["n_accepted"], ref["n_accepted"])
+    for g in range(len(res["accepted"])):
+        assert np.array_equal(res["accepted"][g], ref["accepted"][g]), g
+        assert np.array_equal(res["clique_idx"][g], ref["clique_idx"][g]), g
+    assert np.array_equal(res["clique_size"], ref["clique_size"])
+
+
+def test_batch_graph_replay_and_autotune(ctx):
+    """RPGO_BATCH_GRAPH captures the stages once and replays them while the layout
+    holds; RPGO_BATCH_AUTOTUNE picks launch shapes.  Neither changes a bit, and a
+    replay on new values of the same layout gives the new graphs' results."""
+    graphs = _graphs3(21, 10)
+    print("[batch] graph/autotune: plain run", flush=True)
+    plain = ctx.pcm_batch_run(graphs, 10.0, 5.0)
+    adj_plain = [ctx.pcm_batch_adj(g).copy() for g in range(len(graphs))]
+    print("[batch] graph/autotune: tuned + captured", flush=True)
+    first = ctx.pcm_batch_run(graphs, 10.0, 5.0, flags=_capi.BATCH_GRAPH | _capi.BATCH_AUTOTUNE)
+    assert first["stats"].graph_replayed == 0
+    assert first["stats"].pair_waves in (1, 2, 4, 8) and first["stats"].clique_waves in (4, 8, 16)
+    _assert_same(first, plain)
+    again = ctx.pcm_batch_run(graphs, 10.0, 5.0, flags=_capi.BATCH_GRAPH | _capi.BATCH_AUTOTUNE)
+    assert again["stats"].graph_replayed == 1 and again["stats"].ms_tune == 0.0
+    _assert_same(again, plain)
+    for g in range(len(graphs)):
+        assert np.array_equal(ctx.pcm_batch_adj(g), adj_plain[g]), g
+    # same sizes, other values: the replay must compute the new graphs
+    other = [synth.se3_helix(n_poses=gr.n_poses, n_lc=gr.n_lc, outlier_frac=0.35, seed=500 + k)
+             for k, gr in enumerate(graphs)]
+    print("[batch] graph/autotune: replay on new values", flush=True)
+    rep = ctx.pcm_batch_run(other, 10.0, 5.0, flags=_capi.BATCH_GRAPH | _capi.BATCH_AUTOTUNE)
+    assert rep["stats"].graph_replayed == 1
+    for g, gr in enumerate(other):
+        acc, adj, k, idx = _single(ctx, gr, 10.0, 5.0)
+        assert np.array_equal(rep["accepted"][g], acc), g
+        assert int(rep["clique_size"][g]) == k
+        assert np.array_equal(rep["clique_idx"][g], idx[:max(k, 0)]), g
+    # a changed threshold is part of the layout: captured again
+    thr = ctx.pcm_batch_run(other, 10.0, 3.0, flags=_capi.BATCH_GRAPH)
+    assert thr["stats"].graph_replayed == 0
+    ref = ctx.pcm_batch_run(other, 10.0, 3.0)
+    _assert_same(thr, ref)
