@@ -217,7 +217,8 @@ int rpgo_max_clique_incremental_bits(rpgo_ctx* ctx, int32_t n, const uint64_t* a
  * [pose_off[g] - g, pose_off[g+1] - g - 1), loop closures [lc_off[g],
  * lc_off[g+1]) with pose indices local to the graph.  Every stage is the FP64
  * replay of the single-graph path (bit-exact with it), so the adjacency of each
- * graph equals rpgo_pcm_run's on that graph alone. */
+ * graph equals rpgo_pcm_run's on that graph alone (all zero when
+ * lc_threshold < 0). */
 typedef struct {
   int32_t dim;              /* 3 or 2                                         */
   int32_t n_graphs;

@@ -96,10 +96,21 @@ __global__ void __launch_bounds__(64) k_fold_batch(int n_graphs, const long long
   if (l == 0) ri = 1;
   __syncthreads();
   for (int e = l; e < PWC; e += 64) tr[e] = e < PL ? P[e] : (e < PL + DD ? 0.0 : 1.0);
+  // inputs loaded one step ahead (lane l holds entry l), as in k_fold
+  static_assert(PWC <= 64 && DD <= 64, "one entry per lane");
+  double h_next = 0.0, d_next = 0.0;
+  if (n_poses > 1) {
+    if (l < DD) h_next = hinv[(size_t)o0 * DD + l];
+    if (l < PWC) d_next = delta[(size_t)o0 * PWC + l];
+  }
   for (int k = 0; k + 1 < n_poses; ++k) {
-    for (int e = l; e < DD; e += 64) Hs[e] = hinv[(size_t)(o0 + k) * DD + e];
-    for (int e = l; e < PWC; e += 64) Dl[e] = delta[(size_t)(o0 + k) * PWC + e];
+    if (l < DD) Hs[l] = h_next;
+    if (l < PWC) Dl[l] = d_next;
     __syncthreads();
+    if (k + 2 < n_poses) {
+      if (l < DD) h_next = hinv[(size_t)(o0 + k + 1) * DD + l];
+      if (l < PWC) d_next = delta[(size_t)(o0 + k + 1) * PWC + l];
+    }
     if (l < DD) {
       const int r = l / D, c = l % D;
       double s = Hs[r * D] * S[c];
@@ -210,8 +221,7 @@ __global__ void __launch_bounds__(64 * WPB) k_pairs_batch(int n_graphs, long lon
                                                      const long long* __restrict__ adj_off, const int* __restrict__ m_arr,
                                                      const double* __restrict__ traj, const int* __restrict__ cf,
                                                      const int* __restrict__ ct, const double* __restrict__ clcm,
-                                                     double thr, int all_consistent,
-                                                     unsigned long long* __restrict__ adj,
+                                                     double thr, unsigned long long* __restrict__ adj,
                                                      unsigned long long* __restrict__ n_consistent) {
   constexpr int PWC = R64<DIM>::PWC;
   const long long tile = (long long)blockIdx.x * WPB + (threadIdx.x >> 6);
@@ -228,11 +238,10 @@ __global__ void __launch_bounds__(64 * WPB) k_pairs_batch(int n_graphs, long lon
   const int i = w * 64 + lane;
   const bool active = i < j;
   double d = 0.0;
-  if (active && !all_consistent)
+  if (active)
     d = R64<DIM>::pair_dist(tr, cf[o + i], ct[o + i], clcm + (size_t)(o + i) * PWC, cf[o + j], ct[o + j],
                             clcm + (size_t)(o + j) * PWC);
-  // lc_threshold < 0: every pair is consistent without a check (Pcm.h:673)
-  const bool cons = active && (all_consistent || d < thr);
+  const bool cons = active && d < thr;
   const unsigned long long bits = __ballot(cons);
   if (lane == 0) {
     adj[adj_off[g] + (size_t)j * W + w] = bits;
@@ -309,7 +318,7 @@ constexpr int kDefaultPairWaves = 4;
 constexpr int kDefaultCliqueWaves = 16;
 
 template <int DIM>
-static void launch_pairs_dim(rpgo_ctx* ctx, BatchState& s, double thr, int all_cons, int pw) {
+static void launch_pairs_dim(rpgo_ctx* ctx, BatchState& s, double thr, int pw) {
   hipStream_t st = ctx->stream;
   const long long n_tiles = s.tile_off[s.n_graphs];
   const unsigned blocks = (unsigned)((n_tiles + pw - 1) / pw);
@@ -317,7 +326,7 @@ static void launch_pairs_dim(rpgo_ctx* ctx, BatchState& s, double thr, int all_c
   k_pairs_batch<DIM, WPB><<<blocks, 64 * WPB, 0, st>>>(                                                      \
       s.n_graphs, n_tiles, s.d_tile_off.as<long long>(), s.d_pose_off.as<long long>(),                        \
       s.d_lc_off.as<long long>(), s.d_adj_off.as<long long>(), s.m.as<int>(), s.traj.as<double>(),            \
-      s.c_from.as<int>(), s.c_to.as<int>(), s.c_lcm.as<double>(), thr, all_cons, s.adj.as<unsigned long long>(), \
+      s.c_from.as<int>(), s.c_to.as<int>(), s.c_lcm.as<double>(), thr, s.adj.as<unsigned long long>(),        \
       s.counters.as<unsigned long long>())
   switch (pw) {
     case 1: RPGO_PAIRS(1); break;
@@ -335,11 +344,11 @@ static int launch_pairs(rpgo_ctx* ctx, BatchState& s, const rpgo_pcm_params* prm
   const long long n_tiles = s.tile_off[G];
   zero_u64(s.adj.as<unsigned long long>(), std::max<long long>(s.adj_off[G], 1), st);
   zero_u64(s.counters.as<unsigned long long>(), 4, st);
-  // lc_threshold < 0: every pair is consistent without a check (Pcm.h:673)
-  const int all_cons = prm->lc_threshold < 0 ? 1 : 0;
-  if (n_tiles > 0) {
-    if (s.dim == 3) launch_pairs_dim<3>(ctx, s, prm->lc_threshold, all_cons, pw);
-    else launch_pairs_dim<2>(ctx, s, prm->lc_threshold, all_cons, pw);
+  // lc_threshold < 0 turns the pairwise check off (Pcm.h:79-82): the reference
+  // then never grows the matrix, and as in rpgo_pcm_run the adjacency stays zero
+  if (n_tiles > 0 && prm->lc_threshold >= 0.0) {
+    if (s.dim == 3) launch_pairs_dim<3>(ctx, s, prm->lc_threshold, pw);
+    else launch_pairs_dim<2>(ctx, s, prm->lc_threshold, pw);
     const long long n_blocks = s.sym_off[G];
     k_sym_batch<<<(unsigned)n_blocks, 64, 0, st>>>(G, n_blocks, s.d_sym_off.as<long long>(), s.d_lc_off.as<long long>(),
                                                    s.d_adj_off.as<long long>(), s.m.as<int>(),

@@ -187,43 +187,49 @@ static int pairs_pass(rpgo_ctx* ctx, const rpgo_pcm_params* prm, rpgo_pcm_stats*
       RPGO_HIP_CHECK(hipEventRecord(ev[9], st));
       RPGO_HIP_CHECK(hipEventRecord(ev[10], st));
     } else {
+      // FP32 pairs, then the FP64 replay of the queued near-threshold pairs;
+      // the queue length stays on the device (no host round trip)
       if ((rc = pcm_pairs_fp32_launch(ctx, prm->lc_threshold, prm->recheck_margin,
                                       prm->recheck_chain_coef, want_dist)))
         return rc;
       RPGO_HIP_CHECK(hipEventRecord(ev[9], st));
-      unsigned int nq = 0;
-      RPGO_HIP_CHECK(hipMemcpyAsync(&nq, s.counters.as<unsigned int>() + 16, sizeof(nq),
-                                    hipMemcpyDeviceToHost, st));
-      RPGO_HIP_CHECK(hipStreamSynchronize(st));
-      if (nq > (unsigned)kRecheckQueueCap) {
-        // band too crowded for the queue: replay every pair in FP64
-        RPGO_HIP_CHECK(hipMemsetAsync(s.adj.ptr, 0, sizeof(uint64_t) * (size_t)m * W, st));
-        if ((rc = pcm_pairs_fp64(ctx, prm->lc_threshold, want_dist))) return rc;
-        if ((rc = pcm_symmetrize(ctx))) return rc;
-        n_recheck = (int64_t)m * (m - 1) / 2;
-      } else {
-        if ((rc = pcm_recheck64(ctx, (int)nq, prm->lc_threshold, want_dist))) return rc;
-        n_recheck = nq;
-      }
+      if ((rc = pcm_recheck64_device(ctx, prm->lc_threshold, want_dist))) return rc;
       RPGO_HIP_CHECK(hipEventRecord(ev[10], st));
     }
-    if ((rc = pcm_count_consistent(ctx))) return rc;
   }
   RPGO_HIP_CHECK(hipEventRecord(ev[11], st));
   RPGO_HIP_CHECK(hipEventSynchronize(ev[11]));
+  unsigned long long cnt[4] = {0, 0, 0, 0};
+  unsigned int nq = 0;
+  RPGO_HIP_CHECK(hipMemcpy(cnt, s.counters.ptr, sizeof(cnt), hipMemcpyDeviceToHost));
+  RPGO_HIP_CHECK(hipMemcpy(&nq, s.counters.as<unsigned int>() + 16, sizeof(nq), hipMemcpyDeviceToHost));
+  if (fp32 && check && m > 1) {
+    n_recheck = nq;
+    if (nq > (unsigned)kRecheckQueueCap) {
+      // band too crowded for the queue (the device replay skipped it): replay
+      // every pair in FP64
+      RPGO_HIP_CHECK(hipMemsetAsync(s.adj.ptr, 0, sizeof(uint64_t) * (size_t)m * W, st));
+      RPGO_HIP_CHECK(hipMemsetAsync(s.counters.ptr, 0, sizeof(unsigned long long) * 3, st));
+      if ((rc = pcm_pairs_fp64(ctx, prm->lc_threshold, want_dist))) return rc;
+      if ((rc = pcm_symmetrize(ctx))) return rc;
+      RPGO_HIP_CHECK(hipEventRecord(ev[10], st));
+      RPGO_HIP_CHECK(hipEventRecord(ev[11], st));
+      RPGO_HIP_CHECK(hipEventSynchronize(ev[11]));
+      RPGO_HIP_CHECK(hipMemcpy(cnt, s.counters.ptr, sizeof(cnt), hipMemcpyDeviceToHost));
+      n_recheck = (int64_t)m * (m - 1) / 2;
+    }
+  }
   s.ran = true;
   s.have_dist = want_dist;
   s.precision = simple ? RPGO_PCM_FP64_REPLAY : prm->precision;
   s.simple = simple != nullptr;
   if (stats) {
-    unsigned long long cnt[4] = {0, 0, 0, 0};
-    RPGO_HIP_CHECK(hipMemcpy(cnt, s.counters.ptr, sizeof(cnt), hipMemcpyDeviceToHost));
     stats->n_accepted = m;
     stats->reserved = 0;
     stats->n_pairs = check ? (int64_t)m * (m - 1) / 2 : 0;
     stats->n_rechecked = n_recheck;
     stats->n_flipped = (int64_t)cnt[2];
-    stats->n_consistent = (int64_t)(cnt[1] / 2);
+    stats->n_consistent = (int64_t)cnt[0];
     stats->ms_pairs = elapsed(ev[8], ev[9]);
     stats->ms_recheck = elapsed(ev[9], ev[10]);
   }

@@ -178,6 +178,7 @@ __device__ int replay_wave(int W, int c, const unsigned long long* __restrict__ 
       int base = 0;
 #pragma unroll
       for (int q = 0; q < kMaxWPL; ++q) {
+        if (64 * q >= W) break;  // words beyond W are empty (wave-uniform)
         const int cnt = __popcll(S[q]);
         int incl = cnt;
 #pragma unroll
@@ -186,17 +187,22 @@ __device__ int replay_wave(int W, int c, const unsigned long long* __restrict__ 
           if (lane >= off) incl += o;
         }
         const int total = __shfl(incl, 63);
-        int r = base + incl - cnt;  // rank of the first set bit of this word
-        unsigned long long b = S[q];
-        const int w = lane + 64 * q;
-        while (b) {
-          const int bit = __ffsll((long long)b) - 1;
-          b &= b - 1ull;
-          const int p = r + 1;
-          if (p >= lo && p < s_t) out[p] = w * 64 + bit;
-          ++r;
+        // this word's set bits hold ranks [r0, r0 + cnt), i.e. positions
+        // [r0 + 1, r0 + cnt + 1); only those inside [lo, s_t) are written
+        const int r0 = base + incl - cnt;
+        const int p_lo = lo > r0 + 1 ? lo : r0 + 1;
+        const int p_hi = s_t < r0 + cnt + 1 ? s_t : r0 + cnt + 1;
+        if (p_lo < p_hi) {
+          unsigned long long b = S[q];
+          for (int skip = p_lo - (r0 + 1); skip > 0; --skip) b &= b - 1ull;
+          const int w = lane + 64 * q;
+          for (int p = p_lo; p < p_hi; ++p) {
+            out[p] = w * 64 + (__ffsll((long long)b) - 1);
+            b &= b - 1ull;
+          }
         }
         base += total;
+        if (base + 1 >= s_t) break;  // every later rank lies beyond the range
       }
     }
 #pragma unroll

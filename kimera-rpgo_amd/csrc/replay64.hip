@@ -69,10 +69,22 @@ __global__ void __launch_bounds__(64) k_fold(int start, int n_poses,
     if (l == 0) ri = rec[PL + DD] != 0.0 ? 1 : 0;
   }
   __syncthreads();
+  // step k's inputs are loaded one step ahead into registers (lane l holds
+  // entry l), so the global-memory latency overlaps the previous step
+  static_assert(PWC <= 64 && DD <= 64, "one entry per lane");
+  double h_next = 0.0, d_next = 0.0;
+  if (start + 1 < n_poses) {
+    if (l < DD) h_next = hinv[(size_t)start * DD + l];
+    if (l < PWC) d_next = delta[(size_t)start * PWC + l];
+  }
   for (int k = start; k + 1 < n_poses; ++k) {
-    for (int e = l; e < DD; e += 64) Hs[e] = hinv[(size_t)k * DD + e];
-    for (int e = l; e < PWC; e += 64) Dl[e] = delta[(size_t)k * PWC + e];
+    if (l < DD) Hs[l] = h_next;
+    if (l < PWC) Dl[l] = d_next;
     __syncthreads();
+    if (k + 2 < n_poses) {
+      if (l < DD) h_next = hinv[(size_t)(k + 1) * DD + l];
+      if (l < PWC) d_next = delta[(size_t)(k + 1) * PWC + l];
+    }
     // T = H * Sigma
     if (l < DD) {
       const int r = l / D, c = l % D;
@@ -247,7 +259,8 @@ __global__ void __launch_bounds__(256) k_pairs_simple(int m, long long n_tiles,
                                                       const double* __restrict__ clcm,
                                                       double thr_t, double thr_r, int W,
                                                       unsigned long long* __restrict__ adj,
-                                                      double* __restrict__ dist) {
+                                                      double* __restrict__ dist,
+                                                      unsigned long long* __restrict__ n_consistent) {
   constexpr int PWC = R64<DIM>::PWC;
   const long long tile = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (tile >= n_tiles) return;
@@ -269,7 +282,10 @@ __global__ void __launch_bounds__(256) k_pairs_simple(int m, long long n_tiles,
     }
   }
   const unsigned long long bits = __ballot(cons);
-  if (lane == 0) adj[(size_t)j * W + w] = bits;
+  if (lane == 0) {
+    adj[(size_t)j * W + w] = bits;
+    if (bits) atomicAdd(n_consistent, (unsigned long long)__popcll(bits));
+  }
 }
 
 // Mirror the lower-triangle bits into the upper triangle: one wavefront per
@@ -294,16 +310,16 @@ __global__ void __launch_bounds__(64) k_symmetrize(int m, int W, unsigned long l
 
 // FP64 replay of pairs queued by the FP32 kernel: the reference decision
 // d < thr replaces the FP32 one; both triangle bits are patched.
+// One queued pair (i < j): FP64 distance, patch both bits when the FP32
+// decision differs, count the flip and move the consistent-pair count.
 template <int DIM>
-__global__ void k_recheck64(int n, const int2* __restrict__ queue, const double* __restrict__ traj,
-                            const int* __restrict__ cf, const int* __restrict__ ct,
-                            const double* __restrict__ clcm, double thr, int m, int W,
-                            unsigned long long* __restrict__ adj, double* __restrict__ dist,
-                            unsigned long long* __restrict__ flips) {
+__device__ __forceinline__ void recheck_one(int2 q, const double* __restrict__ traj,
+                                            const int* __restrict__ cf, const int* __restrict__ ct,
+                                            const double* __restrict__ clcm, double thr, int m, int W,
+                                            unsigned long long* __restrict__ adj, double* __restrict__ dist,
+                                            unsigned long long* __restrict__ flips,
+                                            unsigned long long* __restrict__ n_consistent) {
   constexpr int PWC = R64<DIM>::PWC;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int2 q = queue[t];
   const int i = q.x, j = q.y;
   if (i < 0 || j <= i || j >= m) return;
   const double d = R64<DIM>::pair_dist(traj, cf[i], ct[i], clcm + (size_t)i * PWC, cf[j], ct[j],
@@ -317,9 +333,11 @@ __global__ void k_recheck64(int n, const int2* __restrict__ queue, const double*
     if (cons) {
       atomicOr(wj, bj);
       atomicOr(wi, bi);
+      if (n_consistent) atomicAdd(n_consistent, 1ull);
     } else {
       atomicAnd(wj, ~bj);
       atomicAnd(wi, ~bi);
+      if (n_consistent) atomicAdd(n_consistent, ~0ull);  // -1 modulo 2^64
     }
     atomicAdd(flips, 1ull);
   }
@@ -327,6 +345,37 @@ __global__ void k_recheck64(int n, const int2* __restrict__ queue, const double*
     dist[(size_t)j * m + i] = d;
     dist[(size_t)i * m + j] = d;
   }
+}
+
+template <int DIM>
+__global__ void k_recheck64(int n, const int2* __restrict__ queue, const double* __restrict__ traj,
+                            const int* __restrict__ cf, const int* __restrict__ ct,
+                            const double* __restrict__ clcm, double thr, int m, int W,
+                            unsigned long long* __restrict__ adj, double* __restrict__ dist,
+                            unsigned long long* __restrict__ flips) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  recheck_one<DIM>(queue[t], traj, cf, ct, clcm, thr, m, W, adj, dist, flips, nullptr);
+}
+
+// The same with the queue length read on the device (no host round trip):
+// grid-stride over min(*qcount, cap) entries.  A longer queue is left to the
+// host, which sees *qcount > cap after the stage and replays every pair.
+template <int DIM>
+__global__ void __launch_bounds__(64) k_recheck64_dev(const unsigned int* __restrict__ qcount, int cap,
+                                                      const int2* __restrict__ queue,
+                                                      const double* __restrict__ traj,
+                                                      const int* __restrict__ cf, const int* __restrict__ ct,
+                                                      const double* __restrict__ clcm, double thr, int m, int W,
+                                                      unsigned long long* __restrict__ adj,
+                                                      double* __restrict__ dist,
+                                                      unsigned long long* __restrict__ flips,
+                                                      unsigned long long* __restrict__ n_consistent) {
+  const unsigned int nq = *qcount;
+  if (nq > (unsigned)cap) return;
+  const int n = (int)nq;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
+    recheck_one<DIM>(queue[t], traj, cf, ct, clcm, thr, m, W, adj, dist, flips, n_consistent);
 }
 
 // ---------------------------------------------------------------------------
@@ -448,6 +497,29 @@ int pcm_measure_range(rpgo_ctx* ctx, int o0, int o1, int l0, int l1) {
       k_measurements<2><<<(nl + 255) / 256, 256, 0, ctx->stream>>>(nl, pose, cov, pwc_out, nullptr);
     RPGO_HIP_CHECK(hipGetLastError());
   }
+  return RPGO_OK;
+}
+
+int pcm_recheck64_device(rpgo_ctx* ctx, double lc_thr, bool want_dist) {
+  PcmState& s = ctx->pcm;
+  const int m = s.m, W = (m + 63) / 64;
+  double* dist = want_dist ? s.dist.as<double>() : nullptr;
+  unsigned long long* cnt = s.counters.as<unsigned long long>();
+  const unsigned int* qcount = s.counters.as<unsigned int>() + 16;
+  // enough wavefronts for the longest queue seen in practice (0.2-0.3 % of
+  // the pairs at 50k), grid-stride beyond
+  const unsigned blocks = 2048;
+  if (s.dim == 3)
+    k_recheck64_dev<3><<<blocks, 64, 0, ctx->stream>>>(qcount, kRecheckQueueCap, s.queue.as<int2>(),
+                                                       s.traj.as<double>(), s.c_from.as<int>(), s.c_to.as<int>(),
+                                                       s.c_lcm.as<double>(), lc_thr, m, W,
+                                                       s.adj.as<unsigned long long>(), dist, cnt + 2, cnt);
+  else
+    k_recheck64_dev<2><<<blocks, 64, 0, ctx->stream>>>(qcount, kRecheckQueueCap, s.queue.as<int2>(),
+                                                       s.traj.as<double>(), s.c_from.as<int>(), s.c_to.as<int>(),
+                                                       s.c_lcm.as<double>(), lc_thr, m, W,
+                                                       s.adj.as<unsigned long long>(), dist, cnt + 2, cnt);
+  RPGO_HIP_CHECK(hipGetLastError());
   return RPGO_OK;
 }
 
@@ -579,12 +651,12 @@ int pcm_pairs_simple(rpgo_ctx* ctx, const SimpleThr& thr, bool want_dist) {
     k_pairs_simple<3><<<(unsigned)blocks, 256, 0, ctx->stream>>>(
         m, n_tiles, s.traj.as<double>(), s.c_from.as<int>(), s.c_to.as<int>(),
         s.c_lcm.as<double>(), thr.trans, thr.rot, W, s.adj.as<unsigned long long>(),
-        want_dist ? s.dist.as<double>() : nullptr);
+        want_dist ? s.dist.as<double>() : nullptr, s.counters.as<unsigned long long>());
   else
     k_pairs_simple<2><<<(unsigned)blocks, 256, 0, ctx->stream>>>(
         m, n_tiles, s.traj.as<double>(), s.c_from.as<int>(), s.c_to.as<int>(),
         s.c_lcm.as<double>(), thr.trans, thr.rot, W, s.adj.as<unsigned long long>(),
-        want_dist ? s.dist.as<double>() : nullptr);
+        want_dist ? s.dist.as<double>() : nullptr, s.counters.as<unsigned long long>());
   RPGO_HIP_CHECK(hipGetLastError());
   return RPGO_OK;
 }

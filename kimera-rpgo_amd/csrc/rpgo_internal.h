@@ -165,6 +165,10 @@ int pcm_adj_append(rpgo_ctx* ctx, int m_old, double lc_thr, const SimpleThr* sim
 int pcm_symmetrize(rpgo_ctx* ctx);
 // FP64 replay of queued (i, j) pairs; patches the adjacency bits, counts flips
 int pcm_recheck64(rpgo_ctx* ctx, int n_queue, double lc_thr, bool want_dist);
+// the same for the queue the FP32 kernel left on the device: the length is
+// read on the device; a queue longer than kRecheckQueueCap is left untouched
+// (the host sees the length after the stage and replays every pair)
+int pcm_recheck64_device(rpgo_ctx* ctx, double lc_thr, bool want_dist);
 // pcm_fp32.hip
 int pcm_fp32_prepare(rpgo_ctx* ctx, bool* use_fp64);
 int pcm_pairs_fp32_launch(rpgo_ctx* ctx, double lc_thr, double margin, double chain_coef,
@@ -173,7 +177,8 @@ int pcm_count_consistent(rpgo_ctx* ctx);
 constexpr int kRecheckQueueCap = 1 << 22;
 
 // counters buffer layout (256 bytes, zeroed per pairwise pass)
-//   u64[0] consistent pairs (FP64 kernel)   u64[1] popcount of the adjacency
+//   u64[0] consistent pairs i < j (pair kernels, moved by the FP64 recheck)
+//   u64[1] popcount of the adjacency
 //   u64[2] FP64 recheck flips               u32[16] recheck queue length
 //   i32[32] "some covariance has no rotation information" flag
 // solver_capi.hip

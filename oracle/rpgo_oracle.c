@@ -564,13 +564,11 @@ int orc_pcm_adjacency(int dim, int n_poses, const double* traj,
   (void)n_poses;
   int m = 0;
   for (int k = 0; k < n_lc; ++k) {
-    int ok = 1;
-    double od = 0.0;
-    if (odom_thr >= 0.0) {
-      od = orc_odom_dist(dim, traj, lc_from[k], lc_to[k], lc_pose + (size_t)k * pl,
-                         lc_cov + (size_t)k * dd);
-      ok = od < odom_thr;
-    }
+    /* the distance is reported also when the check is off (odom_thr < 0,
+     * every loop closure passes: Pcm.h:606) */
+    const double od = orc_odom_dist(dim, traj, lc_from[k], lc_to[k], lc_pose + (size_t)k * pl,
+                                    lc_cov + (size_t)k * dd);
+    const int ok = odom_thr < 0.0 || od < odom_thr;
     if (odom_dist_out) odom_dist_out[k] = od;
     if (ok) accepted[m++] = k;
   }
@@ -712,14 +710,10 @@ int orc_pcm_simple_adjacency(int dim, int n_poses, const double* traj, int n_lc,
   const int loop_check = dist_trans >= 0.0 && dist_rot >= 0.0;
   int m = 0;
   for (int k = 0; k < n_lc; ++k) {
-    int ok = 1;
-    double od = 0.0;
-    if (odom_check) {
-      double rot = 0.0;
-      od = orc_simple_odom(dim, traj, lc_from[k], lc_to[k], lc_pose + (size_t)k * pl,
-                           lc_cov + (size_t)k * dd, &rot);
-      ok = od < odom_trans && rot < odom_rot;
-    }
+    double rot = 0.0;
+    const double od = orc_simple_odom(dim, traj, lc_from[k], lc_to[k], lc_pose + (size_t)k * pl,
+                                      lc_cov + (size_t)k * dd, &rot);
+    const int ok = !odom_check || (od < odom_trans && rot < odom_rot);
     if (odom_dist_out) odom_dist_out[k] = od;
     if (ok) accepted[m++] = k;
   }

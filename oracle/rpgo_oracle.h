@@ -78,7 +78,8 @@ double orc_pair_dist(int dim, const double* traj,
  *   odom check of every LC (odom_thr < 0 disables), accepted LCs appended in
  *   input order, pairwise check of each new LC against all earlier accepted.
  * accepted_out[n_lc]: index (into the LC input) of each accepted LC, returns
- * the accepted count m.  odom_dist_out[n_lc] (may be NULL).  adj_out: m*m bytes
+ * the accepted count m.  odom_dist_out[n_lc] (may be NULL): the odometry-check
+ * distance of every LC, also when the check is off.  adj_out: m*m bytes
  * (row-major, symmetric, 0 diagonal).  dist_out: m*m doubles (may be NULL).
  * lc_thr < 0 disables the pairwise check (adjacency left all-zero). */
 int orc_pcm_adjacency(int dim, int n_poses, const double* traj,
@@ -115,7 +116,8 @@ double orc_simple_pair(int dim, const double* traj, int a, int b, const double* 
                        double* rot_out);
 /* Whole adjacency stage with PcmSimple checks: a negative odometry threshold
  * (either one) disables the odometry check, a negative distance threshold the
- * pairwise check (Pcm.h:74-82).  dist_out holds translation norms. */
+ * pairwise check (Pcm.h:74-82).  odom_dist_out (every LC, also when the check
+ * is off) and dist_out hold translation norms. */
 int orc_pcm_simple_adjacency(int dim, int n_poses, const double* traj, int n_lc,
                              const int32_t* lc_from, const int32_t* lc_to, const double* lc_pose,
                              const double* lc_cov, double odom_trans, double odom_rot,

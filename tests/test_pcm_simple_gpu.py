@@ -75,12 +75,11 @@ def _run_both(ctx, g, thr):
     return st, got, dict(accepted=acc, odom_dist=od, adj=adj, dist=dist)
 
 
-def _assert_parity(st, got, ref, odom_check=True, loop_check=True):
+def _assert_parity(st, got, ref, loop_check=True):
     assert np.array_equal(got["accepted"], ref["accepted"])
     m = len(ref["accepted"])
     assert st.n_accepted == m
-    if odom_check:
-        np.testing.assert_allclose(got["odom_dist"], ref["odom_dist"], rtol=DIST_RTOL, atol=0)
+    np.testing.assert_allclose(got["odom_dist"], ref["odom_dist"], rtol=DIST_RTOL, atol=0)
     adj = _capi.unpack_bits(got["adj_bits"], m) if m else np.zeros((0, 0), np.uint8)
     assert np.array_equal(adj, ref["adj"]), np.argwhere(adj != ref["adj"])[:10]
     if m and loop_check:
@@ -133,7 +132,7 @@ def test_checks_disabled(ctx):
     thr = (-1.0, 0.1, 0.05, 0.01)
     st, got, ref = _run_both(ctx, g, thr)
     assert len(ref["accepted"]) == g.n_lc
-    _assert_parity(st, got, ref, odom_check=False)
+    _assert_parity(st, got, ref)
     # pair check off: empty adjacency
     thr = (0.5, 0.05, 0.05, -1.0)
     st, got, ref = _run_both(ctx, g, thr)
