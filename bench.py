@@ -74,21 +74,32 @@ def robust_solve(ctx, gl):
     pcm = learned.pcm_stage(ctx, gl)
     ms_pcm = pcm.ms
     poses_p, w_p, rep_p, nk = learned.solve_plain(ctx, gl, pcm.inliers)
+    true_in = ~gl.is_outlier[pcm.inliers]
+
+    def quality(poses, w):
+        """kept true inliers, kept outliers, max translation error vs ground truth (m)"""
+        keep = np.rint(w[nk:]) > 0.5
+        return {"true_inliers_kept": int((keep & true_in).sum()), "outliers_kept": int((keep & ~true_in).sum()),
+                "max_translation_error_vs_truth_m": float(np.abs(poses[:, 9:] - gl.gt[:, 9:]).max())}
+
     res = {"workload": f"SE(3) helix {gl.n_poses} poses, {gl.n_lc} loop closures, "
                        f"{int(round(100 * gl.is_outlier.mean()))}% outliers",
            "pcm_ms": ms_pcm, "solver_loop_closures": int(len(pcm.inliers)),
            "outliers_after_pcm": int(gl.is_outlier[pcm.inliers].sum()),
            "plain": {"ms": rep_p.ms_total, "gnc_iterations": rep_p.gnc_iterations,
                      "lm_iterations": rep_p.lm_iterations, "pcg_iterations": rep_p.pcg_iterations,
-                     "inliers": rep_p.n_inliers},
+                     "inliers": rep_p.n_inliers, **quality(poses_p, w_p)},
            "end_to_end_plain_ms": ms_pcm + rep_p.ms_total}
     if os.path.exists(learned.MODEL_PATH):
         model = learned.LearnedModel.load()
         poses_l, w_l, rep_l, fb, ms_l = learned.solve_learned(ctx, gl, pcm, model)
         res["learned"] = {"ms": ms_l, "gnc_iterations": rep_l.gnc_iterations,
                           "lm_iterations": rep_l.lm_iterations, "fallback": bool(fb),
+                          "tls_fixed_point": rep_l.n_tls_disagree == 0,
                           "same_inlier_set": bool(np.array_equal(np.rint(w_p), np.rint(w_l))),
-                          "max_pose_diff_m": float(np.abs(poses_p[:, 9:] - poses_l[:, 9:]).max())}
+                          "inlier_decisions_differing": int((np.rint(w_p) != np.rint(w_l)).sum()),
+                          "max_pose_diff_vs_plain_m": float(np.abs(poses_p[:, 9:] - poses_l[:, 9:]).max()),
+                          **quality(poses_l, w_l)}
         res["end_to_end_learned_ms"] = ms_pcm + ms_l
         res["speedup_learned_vs_plain"] = (ms_pcm + rep_p.ms_total) / (ms_pcm + ms_l)
     else:
@@ -120,15 +131,18 @@ def batch_c4(ctx, seed=0, n_graphs=256, steps=3):
     ms = float(np.mean([r["stats"].ms_total for r in reps]))
     st = reps[-1]["stats"]
     same = all(np.array_equal(reps[-1]["clique_size"], r["clique_size"]) for r in (first, staged))
+    ss = staged["stats"]
+    adj_ms = ss.ms_fold + ss.ms_odom + ss.ms_pairs
     return {"graphs": n_graphs, "poses": int(sum(g.n_poses for g in graphs)),
             "loop_closures": int(sum(g.n_lc for g in graphs)), "pairs": int(st.n_pairs),
             "ms_per_batch_device": ms, "ms_per_batch_wall": wall * 1e3,
             "pair_checks_per_s": st.n_pairs / (ms * 1e-3),
+            "pair_checks_per_s_adjacency_stage": st.n_pairs / (adj_ms * 1e-3),
             "stages_ms": {"fold": staged["stats"].ms_fold, "odom": staged["stats"].ms_odom,
                           "pairs": staged["stats"].ms_pairs, "clique": staged["stats"].ms_clique,
                           "total_staged": staged["stats"].ms_total},
             "tuned": {"pair_waves": st.pair_waves, "clique_waves": st.clique_waves,
-                      "ms_tune": first["stats"].ms_tune},
+                      "clique_prefetch": st.clique_prefetch, "ms_tune": first["stats"].ms_tune},
             "graph_replayed": bool(st.graph_replayed), "results_identical_across_modes": bool(same)}
 
 

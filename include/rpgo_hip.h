@@ -255,7 +255,7 @@ typedef struct {
   int32_t pair_waves;       /* wavefronts per workgroup of the pair stage      */
   int32_t clique_waves;     /* wavefronts per graph of the clique stage        */
   int32_t graph_replayed;   /* 1 when an existing hipGraph was replayed        */
-  int32_t reserved;
+  int32_t clique_prefetch;  /* rows loaded ahead in the clique stage's walk    */
 } rpgo_pcm_batch_stats;
 
 /* Run the batch.  n_accepted: n_graphs counts; accepted: lc_off[n] entries,

@@ -41,7 +41,7 @@ struct BatchState {
   DevBuf deg, cidx, ck;  // clique stage: degrees, members, sizes
   std::vector<int> m_host;
   // launch shapes chosen by the autotuner, per dim (0: not tuned yet)
-  int tuned_pair[4] = {0, 0, 0, 0}, tuned_clique[4] = {0, 0, 0, 0};
+  int tuned_pair[4] = {0, 0, 0, 0}, tuned_clique[4] = {0, 0, 0, 0}, tuned_prefetch[4] = {0, 0, 0, 0};
   // captured stages and the layout they were captured for
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -361,7 +361,8 @@ static int launch_pairs(rpgo_ctx* ctx, BatchState& s, const rpgo_pcm_params* prm
 // Every stage of the batch, in stream order, with no host synchronisation
 // and no memset/memcpy node (capturable).  events: record ev[0..3], ev[5]
 // between the stages for per-stage times.
-static int launch_stages(rpgo_ctx* ctx, BatchState& s, const rpgo_pcm_params* prm, int pw, int cw, bool events) {
+static int launch_stages(rpgo_ctx* ctx, BatchState& s, const rpgo_pcm_params* prm, int pw, int cw, int pf,
+                         bool events) {
   hipStream_t st = ctx->stream;
   hipEvent_t* ev = ctx->ev;
   const int G = s.n_graphs, dim = s.dim;
@@ -419,66 +420,79 @@ static int launch_stages(rpgo_ctx* ctx, BatchState& s, const rpgo_pcm_params* pr
   if ((rc = launch_pairs(ctx, s, prm, pw))) return rc;
   if (events) RPGO_HIP_CHECK(hipEventRecord(ev[3], st));
   if (cw) {
-    if ((rc = clique_batch_launch(ctx, G, cw, s.d_adj_off.as<long long>(), s.d_lc_off.as<long long>(), s.m.as<int>(),
-                                  s.adj.as<unsigned long long>(), s.deg.as<int>(), s.cidx.as<int>(), s.ck.as<int>())))
+    if ((rc = clique_batch_launch(ctx, G, cw, pf, s.d_adj_off.as<long long>(), s.d_lc_off.as<long long>(),
+                                  s.m.as<int>(), s.adj.as<unsigned long long>(), s.deg.as<int>(), s.cidx.as<int>(),
+                                  s.ck.as<int>())))
       return rc;
   }
   if (events) RPGO_HIP_CHECK(hipEventRecord(ev[5], st));
   return RPGO_OK;
 }
 
-// Launch-shape autotuner: runs the stages once, then times the pair stage for
-// 1, 2, 4 and 8 wavefronts per workgroup and the clique stage for 4, 8 and 16
-// wavefronts per graph on this batch (best of two launches each) and keeps the
+// Launch-shape autotuner: runs the stages once, then times on this batch the
+// pair stage for 1, 2, 4 and 8 wavefronts per workgroup (best of two launches)
+// and the clique stage for 4, 8 and 16 wavefronts per graph, then the fastest
+// of those with row prefetch in the greedy walk (one launch each: a clique
+// launch of a large batch takes long enough to time once), and keeps the
 // fastest per dim.  Every candidate computes the same bits.
 static int autotune(rpgo_ctx* ctx, BatchState& s, const rpgo_pcm_params* prm, bool clique, float* ms_tune) {
   hipStream_t st = ctx->stream;
   hipEvent_t* ev = ctx->ev;
   int rc;
   RPGO_HIP_CHECK(hipEventRecord(ev[6], st));
-  if ((rc = launch_stages(ctx, s, prm, kDefaultPairWaves, clique ? kDefaultCliqueWaves : 0, false))) return rc;
-  float best_t = 1e30f;
-  int best_pw = kDefaultPairWaves;
-  for (int pw : {1, 2, 4, 8}) {
-    float t_min = 1e30f;
-    for (int rep = 0; rep < 2; ++rep) {
+  if ((rc = launch_stages(ctx, s, prm, kDefaultPairWaves, clique ? kDefaultCliqueWaves : 0, 1, false))) return rc;
+  auto timed = [&](auto&& launch, float* t_min, int reps = 2) -> int {
+    *t_min = 1e30f;
+    for (int rep = 0; rep < reps; ++rep) {
       RPGO_HIP_CHECK(hipEventRecord(ev[7], st));
-      if ((rc = launch_pairs(ctx, s, prm, pw))) return rc;
+      const int r = launch();
+      if (r) return r;
       RPGO_HIP_CHECK(hipEventRecord(ev[8], st));
       RPGO_HIP_CHECK(hipEventSynchronize(ev[8]));
       float t = 0.f;
       (void)hipEventElapsedTime(&t, ev[7], ev[8]);
-      t_min = std::min(t_min, t);
+      *t_min = std::min(*t_min, t);
     }
-    if (t_min < best_t) {
-      best_t = t_min;
+    return RPGO_OK;
+  };
+  float best_t = 1e30f;
+  int best_pw = kDefaultPairWaves;
+  for (int pw : {1, 2, 4, 8}) {
+    float t;
+    if ((rc = timed([&] { return launch_pairs(ctx, s, prm, pw); }, &t))) return rc;
+    if (t < best_t) {
+      best_t = t;
       best_pw = pw;
     }
   }
   s.tuned_pair[s.dim] = best_pw;
   if (clique) {
     best_t = 1e30f;
-    int best_cw = kDefaultCliqueWaves;
+    int best_cw = kDefaultCliqueWaves, best_pf = 1;
+    auto clique_time = [&](int cw, int pf, float* t) {
+      return timed(
+          [&] {
+            return clique_batch_launch(ctx, s.n_graphs, cw, pf, s.d_adj_off.as<long long>(),
+                                       s.d_lc_off.as<long long>(), s.m.as<int>(), s.adj.as<unsigned long long>(),
+                                       s.deg.as<int>(), s.cidx.as<int>(), s.ck.as<int>());
+          },
+          t, 1);
+    };
     for (int cw : {4, 8, 16}) {
-      float t_min = 1e30f;
-      for (int rep = 0; rep < 2; ++rep) {
-        RPGO_HIP_CHECK(hipEventRecord(ev[7], st));
-        if ((rc = clique_batch_launch(ctx, s.n_graphs, cw, s.d_adj_off.as<long long>(), s.d_lc_off.as<long long>(),
-                                      s.m.as<int>(), s.adj.as<unsigned long long>(), s.deg.as<int>(),
-                                      s.cidx.as<int>(), s.ck.as<int>())))
-          return rc;
-        RPGO_HIP_CHECK(hipEventRecord(ev[8], st));
-        RPGO_HIP_CHECK(hipEventSynchronize(ev[8]));
-        float t = 0.f;
-        (void)hipEventElapsedTime(&t, ev[7], ev[8]);
-        t_min = std::min(t_min, t);
-      }
-      if (t_min < best_t) {
-        best_t = t_min;
+      float t;
+      if ((rc = clique_time(cw, 1, &t))) return rc;
+      if (t < best_t) {
+        best_t = t;
         best_cw = cw;
       }
     }
+    {
+      float t;
+      if ((rc = clique_time(best_cw, clique_prefetch_depth(), &t))) return rc;
+      if (t < best_t) best_pf = clique_prefetch_depth();
+    }
     s.tuned_clique[s.dim] = best_cw;
+    s.tuned_prefetch[s.dim] = best_pf;
   }
   RPGO_HIP_CHECK(hipEventRecord(ev[9], st));
   RPGO_HIP_CHECK(hipEventSynchronize(ev[9]));
@@ -588,6 +602,7 @@ int rpgo_pcm_batch_run(rpgo_ctx* ctx, const rpgo_pcm_batch* B, const rpgo_pcm_pa
   }
   const int pw = s.tuned_pair[dim] ? s.tuned_pair[dim] : kDefaultPairWaves;
   const int cw = want_clique ? (s.tuned_clique[dim] ? s.tuned_clique[dim] : kDefaultCliqueWaves) : 0;
+  const int pf = s.tuned_prefetch[dim] ? s.tuned_prefetch[dim] : 1;
   hipEvent_t* ev = ctx->ev;
   int replayed = 0;
   if (B->flags & RPGO_BATCH_GRAPH) {
@@ -595,7 +610,8 @@ int rpgo_pcm_batch_run(rpgo_ctx* ctx, const rpgo_pcm_batch* B, const rpgo_pcm_pa
     long long bo, bl;
     std::memcpy(&bo, &prm->odom_threshold, 8);
     std::memcpy(&bl, &prm->lc_threshold, 8);
-    std::vector<long long> key = {dim, G, (long long)nlc, (long long)nod, s.tile_off[G], s.sym_off[G], pw, cw, bo, bl};
+    std::vector<long long> key = {dim, G, (long long)nlc, (long long)nod, s.tile_off[G], s.sym_off[G], pw, cw, pf,
+                                  bo, bl};
     for (DevBuf* b : {&s.d_pose_off, &s.d_lc_off, &s.d_adj_off, &s.d_tile_off, &s.d_sym_off, &s.d_lc_graph,
                       &s.pose0, &s.odom, &s.odom_cov, &s.lc_from, &s.lc_to, &s.lc_pose, &s.lc_cov, &s.delta,
                       &s.hinv, &s.traj, &s.lcm, &s.odom_dist, &s.flags, &s.accepted, &s.m, &s.c_from, &s.c_to,
@@ -610,7 +626,7 @@ int rpgo_pcm_batch_run(rpgo_ctx* ctx, const rpgo_pcm_batch* B, const rpgo_pcm_pa
       s.graph = nullptr;
       s.graph_key.clear();
       RPGO_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
-      const int rc_cap = launch_stages(ctx, s, prm, pw, cw, false);
+      const int rc_cap = launch_stages(ctx, s, prm, pw, cw, pf, false);
       hipGraph_t gr = nullptr;
       const hipError_t e_end = hipStreamEndCapture(st, &gr);
       if (rc_cap) {
@@ -626,7 +642,7 @@ int rpgo_pcm_batch_run(rpgo_ctx* ctx, const rpgo_pcm_batch* B, const rpgo_pcm_pa
     RPGO_HIP_CHECK(hipGraphLaunch(s.exec, st));
     RPGO_HIP_CHECK(hipEventRecord(ev[5], st));
   } else {
-    if ((rc = launch_stages(ctx, s, prm, pw, cw, true))) return rc;
+    if ((rc = launch_stages(ctx, s, prm, pw, cw, pf, true))) return rc;
   }
   s.m_host.assign(G, 0);
   RPGO_HIP_CHECK(hipMemcpyAsync(s.m_host.data(), s.m.ptr, sizeof(int) * G, hipMemcpyDeviceToHost, st));
@@ -676,6 +692,7 @@ int rpgo_pcm_batch_run(rpgo_ctx* ctx, const rpgo_pcm_batch* B, const rpgo_pcm_pa
     stats->ms_tune = ms_tune;
     stats->pair_waves = pw;
     stats->clique_waves = cw;
+    stats->clique_prefetch = want_clique ? pf : 0;
     stats->graph_replayed = replayed;
   }
   return RPGO_OK;

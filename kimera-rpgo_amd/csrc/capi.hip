@@ -71,6 +71,14 @@ int rpgo_ctx_create(int device, uint32_t flags, rpgo_ctx** out) {
     delete c;
     return RPGO_ERR_NOMEM;
   }
+  e = hipHostMalloc(&c->host_pinned, 256, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    set_error("hipHostMalloc failed: %s", hipGetErrorString(e));
+    c->host_pinned = nullptr;
+    c->pcm.counters.release();
+    delete c;
+    return RPGO_ERR_NOMEM;
+  }
   *out = c;
   return RPGO_OK;
 }
@@ -92,6 +100,7 @@ int rpgo_ctx_destroy(rpgo_ctx* ctx) {
   solver_destroy(ctx);
   mlp_destroy(ctx);
   batch_destroy(ctx);
+  if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
   for (auto& ev : ctx->ev)
     if (ev) (void)hipEventDestroy(ev);
   (void)hipStreamDestroy(ctx->stream);
@@ -198,11 +207,14 @@ static int pairs_pass(rpgo_ctx* ctx, const rpgo_pcm_params* prm, rpgo_pcm_stats*
     }
   }
   RPGO_HIP_CHECK(hipEventRecord(ev[11], st));
-  RPGO_HIP_CHECK(hipEventSynchronize(ev[11]));
-  unsigned long long cnt[4] = {0, 0, 0, 0};
+  // counters (u64[0..3]) and the queue length (u32[16]) in one copy to pinned
+  // memory, one wait
+  RPGO_HIP_CHECK(hipMemcpyAsync(ctx->host_pinned, s.counters.ptr, 80, hipMemcpyDeviceToHost, st));
+  RPGO_HIP_CHECK(hipStreamSynchronize(st));
+  unsigned long long cnt[4];
   unsigned int nq = 0;
-  RPGO_HIP_CHECK(hipMemcpy(cnt, s.counters.ptr, sizeof(cnt), hipMemcpyDeviceToHost));
-  RPGO_HIP_CHECK(hipMemcpy(&nq, s.counters.as<unsigned int>() + 16, sizeof(nq), hipMemcpyDeviceToHost));
+  std::memcpy(cnt, ctx->host_pinned, sizeof(cnt));
+  std::memcpy(&nq, static_cast<const char*>(ctx->host_pinned) + 64, sizeof(nq));
   if (fp32 && check && m > 1) {
     n_recheck = nq;
     if (nq > (unsigned)kRecheckQueueCap) {

@@ -79,12 +79,24 @@ __device__ __forceinline__ int wave_popcount(const unsigned long long* S) {
   return cnt;
 }
 
+// rows fetched ahead of the greedy walk (see icc_wave)
+constexpr int kPrefetch = 4;
+
 // Greedy size icc(c, best) of candidate c, one wavefront (S in registers,
 // word w = lane + 64 q); dmask holds the deg >= best filter (LDS or global).
 // Only icc > best matters to the scan, so the walk stops early (INT_MIN) once
 // it cannot get there: every further step removes at least the picked vertex
 // from S, so at most |S| + 1 increments remain.  A candidate that beats best
 // never stops early and returns its exact size.
+// Each step's row load depends on the previous step, so the walk is bound by
+// memory latency.  The rows of the kPrefetch highest vertices of S are loaded
+// together: in a dense block the next pick is the next-highest vertex, and a
+// step whose pick differs from the prediction fetches anew from the current
+// S.  The steps taken are the same as one row at a time.
+// PF = 1 walks one row at a time (the cheaper form when many candidates run
+// at once and the chip is busy anyway); PF = kPrefetch hides the latency of a
+// lone walk.
+template <int PF>
 __device__ __forceinline__ int icc_wave(int W, int c, const unsigned long long* __restrict__ adj,
                                         const unsigned long long* dmask, int lane, int best) {
   unsigned long long S[kMaxWPL];
@@ -97,20 +109,66 @@ __device__ __forceinline__ int icc_wave(int W, int c, const unsigned long long* 
   }
   int icc = 0;
   if (wave_popcount(S) + 1 <= best) return INT_MIN;
-  for (;;) {
-    ++icc;
-    const int m = wave_top_bit(S, lane, W);
-    if (m < 0) break;  // S held only the seed c: removing it empties S
-    // S /\ N(m); m itself always leaves S, so the loop ends within n steps
-    const unsigned long long* row = adj + (size_t)m * W;
+  if (PF == 1) {
+    for (;;) {
+      ++icc;
+      const int m = wave_top_bit(S, lane, W);
+      if (m < 0) return icc;  // S held only the seed c: removing it empties S
+      // S /\ N(m); m itself always leaves S, so the walk ends within n steps
+      const unsigned long long* row = adj + (size_t)m * W;
 #pragma unroll
-    for (int q = 0; q < kMaxWPL; ++q) {
-      const int w = lane + 64 * q;
-      if (w < W) S[q] &= row[w] & ((w == (m >> 6)) ? ~(1ull << (m & 63)) : ~0ull);
+      for (int q = 0; q < kMaxWPL; ++q) {
+        const int w = lane + 64 * q;
+        if (w < W) S[q] &= row[w] & ((w == (m >> 6)) ? ~(1ull << (m & 63)) : ~0ull);
+      }
+      if ((icc & 7) == 0 && icc + wave_popcount(S) + 1 <= best) return INT_MIN;
     }
-    if ((icc & 7) == 0 && icc + wave_popcount(S) + 1 <= best) return INT_MIN;
   }
-  return icc;
+  for (;;) {
+    // predicted picks: the PF highest vertices of S, highest first
+    int cand[PF];
+    {
+      unsigned long long T[kMaxWPL];
+#pragma unroll
+      for (int q = 0; q < kMaxWPL; ++q) T[q] = S[q];
+#pragma unroll
+      for (int p = 0; p < PF; ++p) {
+        const int t = wave_top_bit(T, lane, W);
+        cand[p] = t;
+        if (t >= 0) {
+          const int w = t >> 6;
+#pragma unroll
+          for (int q = 0; q < kMaxWPL; ++q)
+            if (lane + 64 * q == w) T[q] &= ~(1ull << (t & 63));
+        }
+      }
+    }
+    unsigned long long Rw[PF][kMaxWPL];
+#pragma unroll
+    for (int p = 0; p < PF; ++p)
+#pragma unroll
+      for (int q = 0; q < kMaxWPL; ++q) {
+        const int w = lane + 64 * q;
+        Rw[p][q] = (cand[p] >= 0 && w < W) ? adj[(size_t)cand[p] * W + w] : 0ull;
+      }
+#pragma unroll
+    for (int p = 0; p < PF; ++p) {
+      ++icc;
+      const int m = wave_top_bit(S, lane, W);
+      if (m < 0) return icc;  // S held only the seed c: removing it empties S
+      if (m != cand[p]) {     // prediction missed (wave-uniform): fetch again
+        --icc;
+        break;
+      }
+      // S /\ N(m); m itself always leaves S, so the walk ends within n steps
+#pragma unroll
+      for (int q = 0; q < kMaxWPL; ++q) {
+        const int w = lane + 64 * q;
+        if (w < W) S[q] &= Rw[p][q] & ((w == (m >> 6)) ? ~(1ull << (m & 63)) : ~0ull);
+      }
+      if ((icc & 7) == 0 && icc + wave_popcount(S) + 1 <= best) return INT_MIN;
+    }
+  }
 }
 
 // One wavefront per candidate c >= pos: greedy size icc(c, best).
@@ -126,7 +184,7 @@ __global__ void __launch_bounds__(256) k_icc(int n, int W, int pos, int best,
     if (lane == 0) icc_out[c] = INT_MIN;
     return;
   }
-  const int icc = icc_wave(W, c, adj, dmask, lane, best);
+  const int icc = icc_wave<1>(W, c, adj, dmask, lane, best);
   if (lane == 0) icc_out[c] = icc;
 }
 
@@ -233,7 +291,7 @@ __global__ void __launch_bounds__(64) k_replay(int n, int W, int c, int best,
 // replayed by wavefront 0.  Graph g's adjacency starts at adj_off[g] with m[g]
 // rows of ceil(m[g]/64) words; its degrees and clique go to lc_off[g].
 // ---------------------------------------------------------------------------
-template <int NW>
+template <int NW, int PF>
 __global__ void __launch_bounds__(64 * NW) k_clique_batch(int n_graphs, const long long* __restrict__ adj_off,
                                                           const long long* __restrict__ lc_off,
                                                           const int* __restrict__ m_arr,
@@ -285,7 +343,7 @@ __global__ void __launch_bounds__(64 * NW) k_clique_batch(int n_graphs, const lo
     for (int base = pos; base < n; base += NW) {
       const int c = base + wid;
       int icc = INT_MIN;
-      if (c < n && !(best > deg[c])) icc = icc_wave(W, c, A, dm, lane, best);  // pruning 1
+      if (c < n && !(best > deg[c])) icc = icc_wave<PF>(W, c, A, dm, lane, best);  // pruning 1
       if (lane == 0) s_icc[wid] = icc;
       __syncthreads();
       if (tid == 0) {
@@ -332,27 +390,31 @@ __global__ void __launch_bounds__(64 * NW) k_clique_batch(int n_graphs, const lo
   }
 }
 
-int clique_batch_launch(rpgo_ctx* ctx, int n_graphs, int waves, const long long* d_adj_off,
+int clique_batch_launch(rpgo_ctx* ctx, int n_graphs, int waves, int prefetch, const long long* d_adj_off,
                         const long long* d_lc_off, const int* d_m, const unsigned long long* adj, int* deg,
                         int* out_idx, int* out_k) {
   hipStream_t st = ctx->stream;
-  switch (waves) {
-    case 4:
-      k_clique_batch<4><<<n_graphs, 256, 0, st>>>(n_graphs, d_adj_off, d_lc_off, d_m, adj, deg, out_idx, out_k);
-      break;
-    case 8:
-      k_clique_batch<8><<<n_graphs, 512, 0, st>>>(n_graphs, d_adj_off, d_lc_off, d_m, adj, deg, out_idx, out_k);
-      break;
-    case 16:
-      k_clique_batch<16><<<n_graphs, 1024, 0, st>>>(n_graphs, d_adj_off, d_lc_off, d_m, adj, deg, out_idx, out_k);
-      break;
-    default:
-      set_error("clique_batch_launch: unsupported wavefront count %d", waves);
-      return RPGO_ERR_ARG;
+  if ((waves != 4 && waves != 8 && waves != 16) || (prefetch != 1 && prefetch != kPrefetch)) {
+    set_error("clique_batch_launch: unsupported shape (%d wavefronts, prefetch %d)", waves, prefetch);
+    return RPGO_ERR_ARG;
   }
+#define RPGO_CB(NW, PF) \
+  k_clique_batch<NW, PF><<<n_graphs, 64 * NW, 0, st>>>(n_graphs, d_adj_off, d_lc_off, d_m, adj, deg, out_idx, out_k)
+  if (prefetch == 1) {
+    if (waves == 4) RPGO_CB(4, 1);
+    else if (waves == 8) RPGO_CB(8, 1);
+    else RPGO_CB(16, 1);
+  } else {
+    if (waves == 4) RPGO_CB(4, kPrefetch);
+    else if (waves == 8) RPGO_CB(8, kPrefetch);
+    else RPGO_CB(16, kPrefetch);
+  }
+#undef RPGO_CB
   RPGO_HIP_CHECK(hipGetLastError());
   return RPGO_OK;
 }
+
+int clique_prefetch_depth() { return kPrefetch; }
 
 int clique_max_vertices() { return 64 * 64 * kMaxWPL; }
 

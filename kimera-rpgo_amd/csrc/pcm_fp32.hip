@@ -328,88 +328,118 @@ __device__ __forceinline__ void compose_meas(const P32& T, const S6& C, const Lc
   s6_axpy(1.0f, z.S, Co);
 }
 
+// Pair work is split into units: unit u = one older loop closure i against
+// the 64 newer ones j of row tile jt (lane = j - 64 jt), units ordered by jt
+// then i, so row tile jt owns units [64 jt(jt+1)/2, 64 (jt+1)(jt+2)/2).  The
+// grid holds about as many wavefronts as the chip keeps resident and each
+// takes a contiguous, equal range of units: the whole stage ends within one
+// unit of balance (a grid of one wavefront per tile left a second round on a
+// few SIMDs), and the j-side records are reloaded only when a range crosses
+// into the next row tile.
+__device__ __forceinline__ void unit_of(long long u, int* jt, int* i) {
+  long long t = (long long)((sqrt(1.0 + (double)u / 8.0) - 1.0) * 0.5);  // 32 t (t+1) <= u
+  while (32ll * (t + 1) * (t + 2) <= u) ++t;
+  while (t > 0 && 32ll * t * (t + 1) > u) --t;
+  *jt = (int)t;
+  *i = (int)(u - 32ll * t * (t + 1));
+}
+
 // Two wavefronts per SIMD: the compiler fits the pair chain in 256 VGPRs with
 // a small spill instead of 302 registers at one wavefront; 1.29x faster at
-// 5e7 pairs on MI355X (38 vs 30 TFLOP/s), unchanged at 5e5.
-template <int ICH>
+// 5e7 pairs on MI355X (38 vs 30 TFLOP/s).
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
-k_pairs32(int m, long long n_tiles, const LcRec* __restrict__ lcs,
-                                                const PoseRec* __restrict__ prec, Anchors an,
-                                                float thr2, float tau, float kdyn, int W,
-                                                unsigned long long* __restrict__ adj,
-                                                double* __restrict__ dist,
-                                                int2* __restrict__ queue,
-                                                unsigned int* __restrict__ qcount,
-                                                unsigned long long* __restrict__ n_consistent) {
-  constexpr int K = 64 / ICH;
-  const long long t = blockIdx.x;
-  if (t >= n_tiles) return;
-  // tile t -> (jt, chunk): row tile jt owns (jt+1)*K chunks
-  long long jt = (long long)((sqrt(1.0 + 8.0 * (double)t / K) - 1.0) * 0.5);
-  while ((long long)K * (jt + 1) * (jt + 2) / 2 <= t) ++jt;
-  while (jt > 0 && (long long)K * jt * (jt + 1) / 2 > t) --jt;
-  const int chunk = (int)(t - (long long)K * jt * (jt + 1) / 2);
+k_pairs32(int m, long long n_units, int n_waves, const LcRec* __restrict__ lcs,
+          const PoseRec* __restrict__ prec, Anchors an, float thr2, float tau, float kdyn, int W,
+          unsigned long long* __restrict__ adj, double* __restrict__ dist, int2* __restrict__ queue,
+          unsigned int* __restrict__ qcount, unsigned long long* __restrict__ n_consistent) {
+  const int wv = blockIdx.x;
+  if (wv >= n_waves) return;
+  const long long u0 = n_units * wv / n_waves, u1 = n_units * (wv + 1) / n_waves;
+  if (u0 >= u1) return;
   const int lane = threadIdx.x;
-  const int j = (int)jt * 64 + lane;
-  const int i0 = chunk * ICH;
-  const bool jvalid = j < m;
-  const int jj = jvalid ? j : m - 1;
-  const LcRec lj = lcs[jj];
-  const PoseRec rc = prec[lj.a];
-  const PoseRec rd = prec[lj.b];
+  int jt, i;
+  unit_of(u0, &jt, &i);
+  int cur_jt = -1;
+  int j = 0, jj = 0;
+  bool jvalid = false;
+  LcRec lj;
+  PoseRec rc, rd;
   unsigned long long rowbits = 0ull;
-  unsigned int n_cons = 0;  // consistent pairs of this wavefront (lane 0)
-  for (int ii = 0; ii < ICH; ++ii) {
-    const int i = i0 + ii;
-    if (i >= m) break;
-    const bool active = jvalid && i < j;
-    const LcRec li = lcs[i];
-    const PoseRec ra = prec[li.a];
-    const PoseRec rb = prec[li.b];
-    bool cons = false;
-    if (active) {
-      P32 TA, TX, TY, TW, TB, TL, TBi;
-      S6 CA, CX, CW, CB, CL;
-      const float ta = between32(an, li.a, ra, lj.a, rc, TA, CA);
-      compose_meas(TA, CA, lj, TX, CX);
-      p32_inv(TX, TY);
-      compose_meas(TY, CX, li, TW, CW);
-      const float tb = between32(an, li.b, rb, lj.b, rd, TB, CB);
-      p32_mul(TW, TB, TL);
-      p32_inv(TB, TBi);
-      s6_congruence(TBi, CW, CL);
-      s6_axpy(1.0f, CB, CL);
-      float xi[6];
-      const bool log_ok = se3_log32(TL, xi);
-      float d2, mp;
-      const bool pd = mahal2_32(CL, xi, &d2, &mp);
-      cons = d2 < thr2;
-      // The reference subtracts chain covariances bounded by |Sigma_c| in PSD
-      // order, so its FP64 error is ~eps64 * tr(chain), measured against the
-      // smallest pivot of Sigma_L (lever arms scale error and covariance alike).
-      const float chain = ra.trc + rc.trc + rb.trc + rd.trc;
-      const float band = tau + kdyn * chain / fmaxf(mp, 1e-30f);
-      (void)ta;
-      (void)tb;
-      const bool near = fabsf(d2 - thr2) <= band * thr2;
-      if (!log_ok || !pd || near || !(d2 == d2) || d2 > 3.0e38f) {
-        const unsigned int q = atomicAdd(qcount, 1u);
-        if (q < (unsigned)kQueueCap) queue[q] = make_int2(i, j);
+  int roww = -1;  // word of row j the pending rowbits belong to
+  unsigned int n_cons = 0;
+  for (long long u = u0; u < u1; ++u) {
+    if (jt != cur_jt) {
+      if (roww >= 0 && jvalid && rowbits) atomicOr(&adj[(size_t)j * W + roww], rowbits);
+      rowbits = 0ull;
+      roww = -1;
+      cur_jt = jt;
+      j = jt * 64 + lane;
+      jvalid = j < m;
+      jj = jvalid ? j : m - 1;
+      lj = lcs[jj];
+      rc = prec[lj.a];
+      rd = prec[lj.b];
+    }
+    if ((i >> 6) != roww) {
+      if (roww >= 0 && jvalid && rowbits) atomicOr(&adj[(size_t)j * W + roww], rowbits);
+      rowbits = 0ull;
+      roww = i >> 6;
+    }
+    if (i < m) {
+      const bool active = jvalid && i < j;
+      const LcRec li = lcs[i];
+      const PoseRec ra = prec[li.a];
+      const PoseRec rb = prec[li.b];
+      bool cons = false;
+      if (active) {
+        P32 TA, TX, TY, TW, TB, TL, TBi;
+        S6 CA, CX, CW, CB, CL;
+        const float ta = between32(an, li.a, ra, lj.a, rc, TA, CA);
+        compose_meas(TA, CA, lj, TX, CX);
+        p32_inv(TX, TY);
+        compose_meas(TY, CX, li, TW, CW);
+        const float tb = between32(an, li.b, rb, lj.b, rd, TB, CB);
+        p32_mul(TW, TB, TL);
+        p32_inv(TB, TBi);
+        s6_congruence(TBi, CW, CL);
+        s6_axpy(1.0f, CB, CL);
+        float xi[6];
+        const bool log_ok = se3_log32(TL, xi);
+        float d2, mp;
+        const bool pd = mahal2_32(CL, xi, &d2, &mp);
+        cons = d2 < thr2;
+        // The reference subtracts chain covariances bounded by |Sigma_c| in PSD
+        // order, so its FP64 error is ~eps64 * tr(chain), measured against the
+        // smallest pivot of Sigma_L (lever arms scale error and covariance alike).
+        const float chain = ra.trc + rc.trc + rb.trc + rd.trc;
+        const float band = tau + kdyn * chain / fmaxf(mp, 1e-30f);
+        (void)ta;
+        (void)tb;
+        const bool near = fabsf(d2 - thr2) <= band * thr2;
+        if (!log_ok || !pd || near || !(d2 == d2) || d2 > 3.0e38f) {
+          const unsigned int q = atomicAdd(qcount, 1u);
+          if (q < (unsigned)kQueueCap) queue[q] = make_int2(i, j);
+        }
+        if (dist) {
+          const double dv = sqrt((double)d2);
+          dist[(size_t)j * m + i] = dv;
+          dist[(size_t)i * m + j] = dv;
+        }
       }
-      if (dist) {
-        const double dv = sqrt((double)d2);
-        dist[(size_t)j * m + i] = dv;
-        dist[(size_t)i * m + j] = dv;
+      if (cons) rowbits |= 1ull << (i & 63);
+      const unsigned long long colbits = __ballot(cons);
+      if (lane == 0 && colbits) {
+        atomicOr(&adj[(size_t)i * W + jt], colbits);
+        n_cons += (unsigned)__popcll(colbits);
       }
     }
-    if (cons) rowbits |= 1ull << (i & 63);
-    const unsigned long long colbits = __ballot(cons);
-    if (lane == 0 && colbits) {
-      atomicOr(&adj[(size_t)i * W + jt], colbits);
-      n_cons += (unsigned)__popcll(colbits);
+    // next unit
+    if (++i >= 64 * (jt + 1)) {
+      ++jt;
+      i = 0;
     }
   }
-  if (jvalid && rowbits) atomicOr(&adj[(size_t)j * W + (i0 >> 6)], rowbits);
+  if (roww >= 0 && jvalid && rowbits) atomicOr(&adj[(size_t)j * W + roww], rowbits);
   if (lane == 0 && n_cons) atomicAdd(n_consistent, (unsigned long long)n_cons);
 }
 
@@ -533,22 +563,23 @@ int pcm_pairs_fp32_launch(rpgo_ctx* ctx, double lc_thr, double margin, double ch
   const float kdyn = (float)(chain_coef > 0.0 ? chain_coef : 16.0 * 2.220446049250313e-16);
   unsigned int* qcount = s.counters.as<unsigned int>() + 16;
   unsigned long long* n_cons = s.counters.as<unsigned long long>();
-  // pick the i-chunk so that the launch has enough wavefronts
-  const long long pairs = (long long)m * (m - 1) / 2;
   hipStream_t st = ctx->stream;
   double* dist = want_dist ? s.dist.as<double>() : nullptr;
   unsigned long long* adj = s.adj.as<unsigned long long>();
   int2* q = s.queue.as<int2>();
-#define LAUNCH(ICH)                                                                            \
-  do {                                                                                         \
-    const long long tiles = (long long)(64 / ICH) * JT * (JT + 1) / 2;                         \
-    k_pairs32<ICH><<<(unsigned)tiles, 64, 0, st>>>(m, tiles, lcr, prec, an, thr2, tau, kdyn, W, adj, \
-                                                   dist, q, qcount, n_cons);                   \
-  } while (0)
-  if (pairs >= (1ll << 26)) LAUNCH(64);
-  else if (pairs >= (1ll << 24)) LAUNCH(16);
-  else LAUNCH(4);
-#undef LAUNCH
+  // units: one per row tile and older loop closure.  At least as many
+  // wavefronts as stay resident (2 per SIMD) and at most 64 units each, so a
+  // large stage is spread over several waves of wavefronts the hardware
+  // balances, and a small one still ends within one unit of balance.
+  const long long n_units = 32ll * JT * (JT + 1);
+  int dev_cu = 256;
+  (void)hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, ctx->device);
+  long long waves = 8ll * dev_cu;
+  if (n_units > 64 * waves) waves = (n_units + 63) / 64;
+  if (waves > n_units) waves = n_units;
+  if (waves < 1) waves = 1;
+  k_pairs32<<<(unsigned)waves, 64, 0, st>>>(m, n_units, (int)waves, lcr, prec, an, thr2, tau, kdyn, W, adj,
+                                            dist, q, qcount, n_cons);
   RPGO_HIP_CHECK(hipGetLastError());
   return RPGO_OK;
 }

@@ -134,6 +134,7 @@ struct rpgo_ctx {
   rpgo::SolverRun* solver = nullptr;  // created on the first rpgo_solve
   rpgo::MlpState* mlp = nullptr;      // created by rpgo_mlp_set_params
   rpgo::BatchState* batch = nullptr;  // created by rpgo_pcm_batch_run
+  void* host_pinned = nullptr;        // 256 B pinned host buffer for counter readbacks
 };
 
 namespace rpgo {
@@ -189,11 +190,14 @@ int clique_incremental(rpgo_ctx* ctx, const uint64_t* d_adj_u64, int n, int num_
                        int32_t* idx_out, int32_t* k_out);
 // clique_exact.hip: exact maximum clique given a clique size known to exist
 // batched FMC::maxCliqueHeu, one workgroup of `waves` wavefronts (4, 8, 16)
-// per graph; out_k[g] = -1 for an empty graph, INT_MIN on an internal error
-int clique_batch_launch(rpgo_ctx* ctx, int n_graphs, int waves, const long long* d_adj_off,
+// per graph; prefetch: rows loaded ahead in the greedy walk (1 or
+// clique_prefetch_depth()); out_k[g] = -1 for an empty graph, INT_MIN on an
+// internal error
+int clique_batch_launch(rpgo_ctx* ctx, int n_graphs, int waves, int prefetch, const long long* d_adj_off,
                         const long long* d_lc_off, const int* d_m, const unsigned long long* adj, int* deg,
                         int* out_idx, int* out_k);
 int clique_max_vertices();
+int clique_prefetch_depth();
 int clique_exact_run(rpgo_ctx* ctx, const unsigned long long* adj, int n, int lower_bound, int32_t* idx_out,
                      int32_t* k_out, long long node_budget);
 // clique.hip

@@ -84,7 +84,7 @@ class PcmBatchStats(ctypes.Structure):
                 ("ms_odom", ctypes.c_double), ("ms_pairs", ctypes.c_double), ("ms_clique", ctypes.c_double),
                 ("ms_total", ctypes.c_double), ("ms_tune", ctypes.c_double), ("pair_waves", ctypes.c_int32),
                 ("clique_waves", ctypes.c_int32), ("graph_replayed", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("clique_prefetch", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
