@@ -61,10 +61,21 @@ const char* rpgo_last_error(void);
 /* Library version string ("rpgo-mi355x <semver>"). */
 const char* rpgo_version(void);
 
-/* One single-prefix PCM group: odometry chain 0..n_poses-1 plus loop closures.
+/* One PCM loop-closure group: odometry chain 0..n_poses-1 plus loop closures.
  * Mirrors what Pcm::removeOutliers extracts from one update() call: the first
  * pose comes from the values (zero covariance, Pcm.h:534-542), odometry k is
- * the BetweenFactor k -> k+1, loop closure t is BetweenFactor(from, to, pose). */
+ * the BetweenFactor k -> k+1, loop closure t is BetweenFactor(from, to, pose).
+ *
+ * Several robots (n_chains > 1): the poses are the robots' trajectories one
+ * after the other, chain c covering [chain_off[c], chain_off[c+1]) from its
+ * own start pose pose0[c]; odometry rows of chain c are [chain_off[c] - c,
+ * chain_off[c+1] - c - 1) (n_poses - n_chains rows in all).  The loop
+ * closures are one group of the reference (one ObservationId: the same two
+ * robots, or one robot twice).  An inter-robot loop closure passes the
+ * odometry check unchecked (Pcm.h:460-465; its odom_dist is 0), and for a pair
+ * whose first endpoints lie on different chains the newer loop closure's
+ * endpoints are swapped without inverting its measurement (Pcm.h:691-698).
+ * Such problems run the FP64 replay; rpgo_pcm_append takes one chain only. */
 typedef struct {
   int32_t dim;            /* 3 = SE(3) (Pcm3D), 2 = SE(2) (Pcm2D)        */
   int32_t n_poses;        /* >= 1                                          */
@@ -77,6 +88,8 @@ typedef struct {
   const int32_t* lc_to;   /* n_lc, in [0, n_poses)                         */
   const double* lc_pose;  /* n_lc * pose_len                               */
   const double* lc_cov;   /* n_lc * d*d                                    */
+  int32_t n_chains;       /* 0 or 1: one chain; > 1: several robots         */
+  const int32_t* chain_off; /* n_chains + 1 (n_chains > 1 only)            */
 } rpgo_pcm_problem;
 
 typedef struct {

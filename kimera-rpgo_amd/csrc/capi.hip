@@ -91,7 +91,7 @@ int rpgo_ctx_destroy(rpgo_ctx* ctx) {
   for (DevBuf* b : {&s.pose0, &s.odom, &s.odom_cov, &s.lc_from, &s.lc_to, &s.lc_pose, &s.lc_cov,
                     &s.delta, &s.hinv, &s.traj, &s.lcm, &s.odom_dist, &s.flags, &s.accepted,
                     &s.c_from, &s.c_to, &s.c_lcm, &s.adj, &s.dist, &s.counters, &s.fp32_buf,
-                    &s.queue, &s.tmp})
+                    &s.queue, &s.tmp, &s.chain_off})
     b->release();
   CliqueState& q = ctx->clq;
   for (DevBuf* b : {&q.adj, &q.deg, &q.dmask, &q.icc, &q.out, &q.scratch, &q.best, &q.ex_adj,
@@ -123,12 +123,29 @@ int rpgo_pcm_upload(rpgo_ctx* ctx, const rpgo_pcm_problem* p) {
     set_error("rpgo_pcm_upload: dim must be 2 or 3 (got %d)", p->dim);
     return RPGO_ERR_ARG;
   }
-  if (p->n_poses < 1 || p->n_lc < 0 || !p->pose0 ||
-      (p->n_poses > 1 && (!p->odom || !p->odom_cov)) ||
-      (p->n_lc > 0 && (!p->lc_from || !p->lc_to || !p->lc_pose || !p->lc_cov))) {
-    set_error("rpgo_pcm_upload: inconsistent sizes or NULL arrays (n_poses=%d n_lc=%d)",
-              p->n_poses, p->n_lc);
+  const int nc = p->n_chains > 1 ? p->n_chains : 1;
+  if (p->n_poses < nc || p->n_lc < 0 || !p->pose0 ||
+      (p->n_poses > nc && (!p->odom || !p->odom_cov)) ||
+      (p->n_lc > 0 && (!p->lc_from || !p->lc_to || !p->lc_pose || !p->lc_cov)) ||
+      (nc > 1 && !p->chain_off)) {
+    set_error("rpgo_pcm_upload: inconsistent sizes or NULL arrays (n_poses=%d n_lc=%d n_chains=%d)",
+              p->n_poses, p->n_lc, p->n_chains);
     return RPGO_ERR_ARG;
+  }
+  std::vector<int> chain_of;
+  if (nc > 1) {
+    if (p->chain_off[0] != 0 || p->chain_off[nc] != p->n_poses) {
+      set_error("rpgo_pcm_upload: chain_off must run from 0 to n_poses");
+      return RPGO_ERR_ARG;
+    }
+    chain_of.resize(p->n_poses);
+    for (int c = 0; c < nc; ++c) {
+      if (p->chain_off[c + 1] <= p->chain_off[c]) {
+        set_error("rpgo_pcm_upload: chain %d is empty or chain_off decreases", c);
+        return RPGO_ERR_ARG;
+      }
+      for (int k = p->chain_off[c]; k < p->chain_off[c + 1]; ++k) chain_of[k] = c;
+    }
   }
   for (int k = 0; k < p->n_lc; ++k) {
     if (p->lc_from[k] < 0 || p->lc_from[k] >= p->n_poses || p->lc_to[k] < 0 ||
@@ -137,14 +154,25 @@ int rpgo_pcm_upload(rpgo_ctx* ctx, const rpgo_pcm_problem* p) {
                 p->lc_from[k], p->lc_to[k], p->n_poses);
       return RPGO_ERR_ARG;
     }
+    if (nc > 1) {
+      // one group: every loop closure joins the same (unordered) pair of chains
+      const int a0 = chain_of[p->lc_from[0]], b0 = chain_of[p->lc_to[0]];
+      const int a = chain_of[p->lc_from[k]], b = chain_of[p->lc_to[k]];
+      if (!((a == a0 && b == b0) || (a == b0 && b == a0))) {
+        set_error("rpgo_pcm_upload: loop closure %d joins chains (%d,%d), the group joins (%d,%d)", k, a, b,
+                  a0, b0);
+        return RPGO_ERR_ARG;
+      }
+    }
   }
   RPGO_HIP_CHECK(hipSetDevice(ctx->device));
   PcmState& s = ctx->pcm;
   const int PL = pose_len(p->dim), D = tan_dim(p->dim);
-  const size_t nd = (size_t)(p->n_poses - 1), nl = (size_t)p->n_lc;
+  const size_t nd = (size_t)(p->n_poses - nc), nl = (size_t)p->n_lc;
   int rc;
   hipStream_t st = ctx->stream;
-  if ((rc = upload(s.pose0, p->pose0, sizeof(double) * PL, st))) return rc;
+  if ((rc = upload(s.pose0, p->pose0, sizeof(double) * PL * nc, st))) return rc;
+  if (nc > 1 && (rc = upload(s.chain_off, p->chain_off, sizeof(int) * (nc + 1), st))) return rc;
   if ((rc = upload(s.odom, p->odom, sizeof(double) * PL * nd, st))) return rc;
   if ((rc = upload(s.odom_cov, p->odom_cov, sizeof(double) * D * D * nd, st))) return rc;
   if ((rc = upload(s.lc_from, p->lc_from, sizeof(int) * nl, st))) return rc;
@@ -155,6 +183,7 @@ int rpgo_pcm_upload(rpgo_ctx* ctx, const rpgo_pcm_problem* p) {
   s.dim = p->dim;
   s.n_poses = p->n_poses;
   s.n_lc = p->n_lc;
+  s.n_chains = nc;
   s.uploaded = true;
   s.ran = false;
   return RPGO_OK;
@@ -398,6 +427,10 @@ int pcm_append_impl(rpgo_ctx* ctx, const rpgo_pcm_increment* inc, double odom_th
   }
   if (s.simple != (s_odom != nullptr)) {
     set_error("%s: the last run used the %s checks", name, s.simple ? "PcmSimple" : "PCM");
+    return RPGO_ERR_STATE;
+  }
+  if (s.n_chains > 1) {
+    set_error("%s: streaming takes a problem with one chain", name);
     return RPGO_ERR_STATE;
   }
   if (want_dist) {

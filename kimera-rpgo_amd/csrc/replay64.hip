@@ -15,6 +15,23 @@
 
 namespace rpgo {
 
+// Robot trajectories of a problem: chain c covers poses [off[c], off[c+1]).
+// One chain when n <= 1 (off unused).
+struct Chains {
+  const int* off;
+  int n;
+  __device__ __forceinline__ int of(int k) const {
+    if (n <= 1) return 0;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (off[mid] <= k) lo = mid;
+      else hi = mid - 1;
+    }
+    return lo;
+  }
+};
+
 // ---------------------------------------------------------------------------
 // per-measurement precompute: PoseWithCovariance(BetweenFactor) records and
 // the fold Jacobians H_k = Ad(delta_k^-1)          (GeometryUtils.h:91-129)
@@ -118,6 +135,73 @@ __global__ void __launch_bounds__(64) k_fold(int start, int n_poses,
   }
 }
 
+// Fold of several chains, one wavefront per chain (Pcm.h:516-557 per robot
+// prefix): chain c starts at pose0s[c] with zero covariance; its odometry rows
+// are [off[c] - c, off[c+1] - c - 1).  Same arithmetic as k_fold.
+template <int DIM>
+__global__ void __launch_bounds__(64) k_fold_chains(Chains ch, const double* __restrict__ pose0s,
+                                                   const double* __restrict__ delta,
+                                                   const double* __restrict__ hinv,
+                                                   double* __restrict__ traj) {
+  constexpr int PL = R64<DIM>::PL, D = R64<DIM>::D, PWC = R64<DIM>::PWC, DD = D * D;
+  __shared__ double S[DD], T[DD], P[PL], Hs[DD], Dl[PWC];
+  __shared__ int ri;
+  const int c = blockIdx.x;
+  if (c >= ch.n) return;
+  const int p0 = ch.off[c], n_poses = ch.off[c + 1] - p0, o0 = p0 - c;
+  double* tr = traj + (size_t)p0 * PWC;
+  const int l = threadIdx.x;
+  if (l < DD) S[l] = 0.0;
+  if (l < PL) P[l] = pose0s[(size_t)c * PL + l];
+  if (l == 0) ri = 1;
+  __syncthreads();
+  for (int e = l; e < PWC; e += 64) tr[e] = e < PL ? P[e] : (e < PL + DD ? 0.0 : 1.0);
+  static_assert(PWC <= 64 && DD <= 64, "one entry per lane");
+  double h_next = 0.0, d_next = 0.0;
+  if (n_poses > 1) {
+    if (l < DD) h_next = hinv[(size_t)o0 * DD + l];
+    if (l < PWC) d_next = delta[(size_t)o0 * PWC + l];
+  }
+  for (int k = 0; k + 1 < n_poses; ++k) {
+    if (l < DD) Hs[l] = h_next;
+    if (l < PWC) Dl[l] = d_next;
+    __syncthreads();
+    if (k + 2 < n_poses) {
+      if (l < DD) h_next = hinv[(size_t)(o0 + k + 1) * DD + l];
+      if (l < PWC) d_next = delta[(size_t)(o0 + k + 1) * PWC + l];
+    }
+    if (l < DD) {
+      const int r = l / D, cc = l % D;
+      double sacc = Hs[r * D] * S[cc];
+#pragma unroll
+      for (int m = 1; m < D; ++m) sacc += Hs[r * D + m] * S[m * D + cc];
+      T[l] = sacc;
+    }
+    double newpose = 0.0;
+    if (l >= DD && l < DD + PL) newpose = R64<DIM>::compose_entry(P, Dl, l - DD);
+    __syncthreads();
+    double* out = tr + (size_t)(k + 1) * PWC;
+    if (l < DD) {
+      const int r = l / D, cc = l % D;
+      double sacc = T[r * D] * Hs[cc * D];
+#pragma unroll
+      for (int m = 1; m < D; ++m) sacc += T[r * D + m] * Hs[cc * D + m];
+      sacc = sacc + Dl[PL + l];
+      S[l] = sacc;
+      out[PL + l] = sacc;
+    }
+    if (l >= DD && l < DD + PL) {
+      P[l - DD] = newpose;
+      out[l - DD] = newpose;
+    }
+    if (l == 0) {
+      ri = (ri && Dl[PL + DD] != 0.0) ? 1 : 0;
+      out[PL + DD] = ri ? 1.0 : 0.0;
+    }
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------------------
 // odometry check (Pcm.h:604-629) : one thread per loop closure
 // ---------------------------------------------------------------------------
@@ -125,10 +209,15 @@ template <int DIM>
 __global__ void k_odom_check(int n_lc, const double* __restrict__ traj,
                              const int* __restrict__ from, const int* __restrict__ to,
                              const double* __restrict__ lcm, double thr,
-                             double* __restrict__ dist, int* __restrict__ flag) {
+                             double* __restrict__ dist, int* __restrict__ flag, Chains ch) {
   constexpr int PWC = R64<DIM>::PWC;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n_lc) return;
+  if (ch.of(from[k]) != ch.of(to[k])) {  // inter-robot: no odometry check (Pcm.h:460-465)
+    dist[k] = 0.0;
+    flag[k] = 1;
+    return;
+  }
   // the distance is computed also when the check is off (thr < 0: every loop
   // closure passes, Pcm.h:606), as a per-loop-closure feature
   const double d = R64<DIM>::odom_dist(traj + (size_t)from[k] * PWC, traj + (size_t)to[k] * PWC,
@@ -143,11 +232,16 @@ template <int DIM>
 __global__ void k_odom_simple(int n_lc, const double* __restrict__ traj,
                               const int* __restrict__ from, const int* __restrict__ to,
                               const double* __restrict__ lcm, bool check, double thr_t,
-                              double thr_r, double* __restrict__ dist, int* __restrict__ flag) {
+                              double thr_r, double* __restrict__ dist, int* __restrict__ flag, Chains ch) {
   constexpr int PWC = R64<DIM>::PWC;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n_lc) return;
   const int a = from[k], b = to[k];
+  if (ch.of(a) != ch.of(b)) {  // inter-robot: no odometry check (Pcm.h:460-465)
+    dist[k] = 0.0;
+    flag[k] = 1;
+    return;
+  }
   double tn, rn;
   R64<DIM>::simple_odom(traj + (size_t)a * PWC, traj + (size_t)b * PWC, lcm + (size_t)k * PWC,
                         abs(b - a) + 1, &tn, &rn);
@@ -221,7 +315,8 @@ __global__ void __launch_bounds__(256) k_pairs64(int m, long long n_tiles,
                                                  const double* __restrict__ clcm, double thr,
                                                  int W, unsigned long long* __restrict__ adj,
                                                  double* __restrict__ dist,
-                                                 unsigned long long* __restrict__ n_consistent) {
+                                                 unsigned long long* __restrict__ n_consistent,
+                                                 Chains ch) {
   constexpr int PWC = R64<DIM>::PWC;
   const long long tile = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (tile >= n_tiles) return;
@@ -233,7 +328,13 @@ __global__ void __launch_bounds__(256) k_pairs64(int m, long long n_tiles,
   const bool active = i < j;
   double d = 0.0;
   if (active) {
-    d = R64<DIM>::pair_dist(traj, cf[i], ct[i], clcm + (size_t)i * PWC, cf[j], ct[j],
+    int c = cf[j], dd = ct[j];
+    if (ch.of(cf[i]) != ch.of(c)) {  // Pcm.h:691-698: swap, measurement kept
+      const int t = c;
+      c = dd;
+      dd = t;
+    }
+    d = R64<DIM>::pair_dist(traj, cf[i], ct[i], clcm + (size_t)i * PWC, c, dd,
                             clcm + (size_t)j * PWC);
     if (dist) {
       dist[(size_t)j * m + i] = d;
@@ -260,7 +361,8 @@ __global__ void __launch_bounds__(256) k_pairs_simple(int m, long long n_tiles,
                                                       double thr_t, double thr_r, int W,
                                                       unsigned long long* __restrict__ adj,
                                                       double* __restrict__ dist,
-                                                      unsigned long long* __restrict__ n_consistent) {
+                                                      unsigned long long* __restrict__ n_consistent,
+                                                      Chains ch) {
   constexpr int PWC = R64<DIM>::PWC;
   const long long tile = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (tile >= n_tiles) return;
@@ -273,7 +375,13 @@ __global__ void __launch_bounds__(256) k_pairs_simple(int m, long long n_tiles,
   bool cons = false;
   if (active) {
     double tn, rn;
-    R64<DIM>::simple_pair(traj, cf[i], ct[i], clcm + (size_t)i * PWC, cf[j], ct[j],
+    int c = cf[j], dd = ct[j];
+    if (ch.of(cf[i]) != ch.of(c)) {  // Pcm.h:691-698
+      const int t = c;
+      c = dd;
+      dd = t;
+    }
+    R64<DIM>::simple_pair(traj, cf[i], ct[i], clcm + (size_t)i * PWC, c, dd,
                           clcm + (size_t)j * PWC, &tn, &rn);
     cons = tn < thr_t && rn < thr_r;
     if (dist) {
@@ -318,11 +426,17 @@ __device__ __forceinline__ void recheck_one(int2 q, const double* __restrict__ t
                                             const double* __restrict__ clcm, double thr, int m, int W,
                                             unsigned long long* __restrict__ adj, double* __restrict__ dist,
                                             unsigned long long* __restrict__ flips,
-                                            unsigned long long* __restrict__ n_consistent) {
+                                            unsigned long long* __restrict__ n_consistent, Chains ch) {
   constexpr int PWC = R64<DIM>::PWC;
   const int i = q.x, j = q.y;
   if (i < 0 || j <= i || j >= m) return;
-  const double d = R64<DIM>::pair_dist(traj, cf[i], ct[i], clcm + (size_t)i * PWC, cf[j], ct[j],
+  int c = cf[j], dd = ct[j];
+  if (ch.of(cf[i]) != ch.of(c)) {  // Pcm.h:691-698
+    const int t = c;
+    c = dd;
+    dd = t;
+  }
+  const double d = R64<DIM>::pair_dist(traj, cf[i], ct[i], clcm + (size_t)i * PWC, c, dd,
                                        clcm + (size_t)j * PWC);
   const bool cons = d < thr;
   unsigned long long* wj = adj + (size_t)j * W + (i >> 6);
@@ -352,10 +466,10 @@ __global__ void k_recheck64(int n, const int2* __restrict__ queue, const double*
                             const int* __restrict__ cf, const int* __restrict__ ct,
                             const double* __restrict__ clcm, double thr, int m, int W,
                             unsigned long long* __restrict__ adj, double* __restrict__ dist,
-                            unsigned long long* __restrict__ flips) {
+                            unsigned long long* __restrict__ flips, Chains ch) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  recheck_one<DIM>(queue[t], traj, cf, ct, clcm, thr, m, W, adj, dist, flips, nullptr);
+  recheck_one<DIM>(queue[t], traj, cf, ct, clcm, thr, m, W, adj, dist, flips, nullptr, ch);
 }
 
 // The same with the queue length read on the device (no host round trip):
@@ -370,12 +484,13 @@ __global__ void __launch_bounds__(64) k_recheck64_dev(const unsigned int* __rest
                                                       unsigned long long* __restrict__ adj,
                                                       double* __restrict__ dist,
                                                       unsigned long long* __restrict__ flips,
-                                                      unsigned long long* __restrict__ n_consistent) {
+                                                      unsigned long long* __restrict__ n_consistent,
+                                                      Chains ch) {
   const unsigned int nq = *qcount;
   if (nq > (unsigned)cap) return;
   const int n = (int)nq;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
-    recheck_one<DIM>(queue[t], traj, cf, ct, clcm, thr, m, W, adj, dist, flips, n_consistent);
+    recheck_one<DIM>(queue[t], traj, cf, ct, clcm, thr, m, W, adj, dist, flips, n_consistent, ch);
 }
 
 // ---------------------------------------------------------------------------
@@ -454,6 +569,8 @@ __global__ void __launch_bounds__(256) k_pairs_append(int m_old, int m, int W,
 // ---------------------------------------------------------------------------
 // host drivers
 // ---------------------------------------------------------------------------
+static Chains chains_of(const PcmState& s) { return Chains{s.chain_off.as<int>(), s.n_chains}; }
+
 int pcm_recheck64(rpgo_ctx* ctx, int n_queue, double lc_thr, bool want_dist) {
   PcmState& s = ctx->pcm;
   if (n_queue <= 0) return RPGO_OK;
@@ -463,11 +580,11 @@ int pcm_recheck64(rpgo_ctx* ctx, int n_queue, double lc_thr, bool want_dist) {
   if (s.dim == 3)
     k_recheck64<3><<<(n_queue + 63) / 64, 64, 0, ctx->stream>>>(
         n_queue, s.queue.as<int2>(), s.traj.as<double>(), s.c_from.as<int>(), s.c_to.as<int>(),
-        s.c_lcm.as<double>(), lc_thr, m, W, s.adj.as<unsigned long long>(), dist, flips);
+        s.c_lcm.as<double>(), lc_thr, m, W, s.adj.as<unsigned long long>(), dist, flips, chains_of(s));
   else
     k_recheck64<2><<<(n_queue + 63) / 64, 64, 0, ctx->stream>>>(
         n_queue, s.queue.as<int2>(), s.traj.as<double>(), s.c_from.as<int>(), s.c_to.as<int>(),
-        s.c_lcm.as<double>(), lc_thr, m, W, s.adj.as<unsigned long long>(), dist, flips);
+        s.c_lcm.as<double>(), lc_thr, m, W, s.adj.as<unsigned long long>(), dist, flips, chains_of(s));
   RPGO_HIP_CHECK(hipGetLastError());
   return RPGO_OK;
 }
@@ -513,19 +630,19 @@ int pcm_recheck64_device(rpgo_ctx* ctx, double lc_thr, bool want_dist) {
     k_recheck64_dev<3><<<blocks, 64, 0, ctx->stream>>>(qcount, kRecheckQueueCap, s.queue.as<int2>(),
                                                        s.traj.as<double>(), s.c_from.as<int>(), s.c_to.as<int>(),
                                                        s.c_lcm.as<double>(), lc_thr, m, W,
-                                                       s.adj.as<unsigned long long>(), dist, cnt + 2, cnt);
+                                                       s.adj.as<unsigned long long>(), dist, cnt + 2, cnt, chains_of(s));
   else
     k_recheck64_dev<2><<<blocks, 64, 0, ctx->stream>>>(qcount, kRecheckQueueCap, s.queue.as<int2>(),
                                                        s.traj.as<double>(), s.c_from.as<int>(), s.c_to.as<int>(),
                                                        s.c_lcm.as<double>(), lc_thr, m, W,
-                                                       s.adj.as<unsigned long long>(), dist, cnt + 2, cnt);
+                                                       s.adj.as<unsigned long long>(), dist, cnt + 2, cnt, chains_of(s));
   RPGO_HIP_CHECK(hipGetLastError());
   return RPGO_OK;
 }
 
 int pcm_prepare(rpgo_ctx* ctx) {
   PcmState& s = ctx->pcm;
-  const int nd = s.n_poses - 1;
+  const int nd = s.n_poses - (s.n_chains > 1 ? s.n_chains : 1);
   const int pwc = pwc_len(s.dim), D = tan_dim(s.dim);
   int rc;
   if ((rc = s.delta.reserve(sizeof(double) * (size_t)(nd > 0 ? nd : 1) * pwc))) return rc;
@@ -538,6 +655,16 @@ int pcm_fold(rpgo_ctx* ctx) {
   PcmState& s = ctx->pcm;
   int rc;
   if ((rc = s.traj.reserve(sizeof(double) * (size_t)s.n_poses * pwc_len(s.dim)))) return rc;
+  if (s.n_chains > 1) {
+    if (s.dim == 3)
+      k_fold_chains<3><<<s.n_chains, 64, 0, ctx->stream>>>(chains_of(s), s.pose0.as<double>(), s.delta.as<double>(),
+                                                           s.hinv.as<double>(), s.traj.as<double>());
+    else
+      k_fold_chains<2><<<s.n_chains, 64, 0, ctx->stream>>>(chains_of(s), s.pose0.as<double>(), s.delta.as<double>(),
+                                                           s.hinv.as<double>(), s.traj.as<double>());
+    RPGO_HIP_CHECK(hipGetLastError());
+    return RPGO_OK;
+  }
   if (s.dim == 3)
     k_fold<3><<<1, 64, 0, ctx->stream>>>(0, s.n_poses, s.pose0.as<double>(), s.delta.as<double>(),
                                          s.hinv.as<double>(), s.traj.as<double>());
@@ -581,19 +708,19 @@ int pcm_odom_and_compact(rpgo_ctx* ctx, double odom_thr, int* m_out, const Simpl
     if (s.dim == 3)
       k_odom_simple<3><<<(n + 255) / 256, 256, 0, ctx->stream>>>(
           n, s.traj.as<double>(), s.lc_from.as<int>(), s.lc_to.as<int>(), s.lcm.as<double>(),
-          check, simple->trans, simple->rot, s.odom_dist.as<double>(), s.flags.as<int>());
+          check, simple->trans, simple->rot, s.odom_dist.as<double>(), s.flags.as<int>(), chains_of(s));
     else
       k_odom_simple<2><<<(n + 255) / 256, 256, 0, ctx->stream>>>(
           n, s.traj.as<double>(), s.lc_from.as<int>(), s.lc_to.as<int>(), s.lcm.as<double>(),
-          check, simple->trans, simple->rot, s.odom_dist.as<double>(), s.flags.as<int>());
+          check, simple->trans, simple->rot, s.odom_dist.as<double>(), s.flags.as<int>(), chains_of(s));
   } else if (s.dim == 3) {
     k_odom_check<3><<<(n + 255) / 256, 256, 0, ctx->stream>>>(
         n, s.traj.as<double>(), s.lc_from.as<int>(), s.lc_to.as<int>(), s.lcm.as<double>(),
-        odom_thr, s.odom_dist.as<double>(), s.flags.as<int>());
+        odom_thr, s.odom_dist.as<double>(), s.flags.as<int>(), chains_of(s));
   } else {
     k_odom_check<2><<<(n + 255) / 256, 256, 0, ctx->stream>>>(
         n, s.traj.as<double>(), s.lc_from.as<int>(), s.lc_to.as<int>(), s.lcm.as<double>(),
-        odom_thr, s.odom_dist.as<double>(), s.flags.as<int>());
+        odom_thr, s.odom_dist.as<double>(), s.flags.as<int>(), chains_of(s));
   }
   RPGO_HIP_CHECK(hipGetLastError());
   k_compact<<<1, 1024, 0, ctx->stream>>>(n, s.flags.as<int>(), s.accepted.as<int>());
@@ -629,12 +756,12 @@ int pcm_pairs_fp64(rpgo_ctx* ctx, double lc_thr, bool want_dist) {
     k_pairs64<3><<<(unsigned)blocks, 256, 0, ctx->stream>>>(
         m, n_tiles, s.traj.as<double>(), s.c_from.as<int>(), s.c_to.as<int>(),
         s.c_lcm.as<double>(), lc_thr, W, s.adj.as<unsigned long long>(),
-        want_dist ? s.dist.as<double>() : nullptr, s.counters.as<unsigned long long>());
+        want_dist ? s.dist.as<double>() : nullptr, s.counters.as<unsigned long long>(), chains_of(s));
   else
     k_pairs64<2><<<(unsigned)blocks, 256, 0, ctx->stream>>>(
         m, n_tiles, s.traj.as<double>(), s.c_from.as<int>(), s.c_to.as<int>(),
         s.c_lcm.as<double>(), lc_thr, W, s.adj.as<unsigned long long>(),
-        want_dist ? s.dist.as<double>() : nullptr, s.counters.as<unsigned long long>());
+        want_dist ? s.dist.as<double>() : nullptr, s.counters.as<unsigned long long>(), chains_of(s));
   RPGO_HIP_CHECK(hipGetLastError());
   return RPGO_OK;
 }
@@ -651,12 +778,12 @@ int pcm_pairs_simple(rpgo_ctx* ctx, const SimpleThr& thr, bool want_dist) {
     k_pairs_simple<3><<<(unsigned)blocks, 256, 0, ctx->stream>>>(
         m, n_tiles, s.traj.as<double>(), s.c_from.as<int>(), s.c_to.as<int>(),
         s.c_lcm.as<double>(), thr.trans, thr.rot, W, s.adj.as<unsigned long long>(),
-        want_dist ? s.dist.as<double>() : nullptr, s.counters.as<unsigned long long>());
+        want_dist ? s.dist.as<double>() : nullptr, s.counters.as<unsigned long long>(), chains_of(s));
   else
     k_pairs_simple<2><<<(unsigned)blocks, 256, 0, ctx->stream>>>(
         m, n_tiles, s.traj.as<double>(), s.c_from.as<int>(), s.c_to.as<int>(),
         s.c_lcm.as<double>(), thr.trans, thr.rot, W, s.adj.as<unsigned long long>(),
-        want_dist ? s.dist.as<double>() : nullptr, s.counters.as<unsigned long long>());
+        want_dist ? s.dist.as<double>() : nullptr, s.counters.as<unsigned long long>(), chains_of(s));
   RPGO_HIP_CHECK(hipGetLastError());
   return RPGO_OK;
 }
@@ -698,14 +825,14 @@ int pcm_odom_append(rpgo_ctx* ctx, int l0, double odom_thr, const SimpleThr* sim
     const bool check = simple->trans >= 0.0 && simple->rot >= 0.0;
     if (s.dim == 3)
       k_odom_simple<3><<<g, 256, 0, st>>>(nn, s.traj.as<double>(), from, to, lcm, check, simple->trans,
-                                          simple->rot, od, fl);
+                                          simple->rot, od, fl, Chains{nullptr, 1});
     else
       k_odom_simple<2><<<g, 256, 0, st>>>(nn, s.traj.as<double>(), from, to, lcm, check, simple->trans,
-                                          simple->rot, od, fl);
+                                          simple->rot, od, fl, Chains{nullptr, 1});
   } else if (s.dim == 3) {
-    k_odom_check<3><<<g, 256, 0, st>>>(nn, s.traj.as<double>(), from, to, lcm, odom_thr, od, fl);
+    k_odom_check<3><<<g, 256, 0, st>>>(nn, s.traj.as<double>(), from, to, lcm, odom_thr, od, fl, Chains{nullptr, 1});
   } else {
-    k_odom_check<2><<<g, 256, 0, st>>>(nn, s.traj.as<double>(), from, to, lcm, odom_thr, od, fl);
+    k_odom_check<2><<<g, 256, 0, st>>>(nn, s.traj.as<double>(), from, to, lcm, odom_thr, od, fl, Chains{nullptr, 1});
   }
   RPGO_HIP_CHECK(hipGetLastError());
   k_compact<<<1, 1024, 0, st>>>(nn, fl, s.tmp.as<int>());

@@ -85,6 +85,8 @@ struct PcmState {
   int dim = 0;
   int n_poses = 0;
   int n_lc = 0;
+  int n_chains = 1;  // robot trajectories of the uploaded problem
+  DevBuf chain_off;  // n_chains + 1 ints (several chains only)
   bool uploaded = false;
   bool ran = false;
   int m = 0;  // accepted loop closures of the last run

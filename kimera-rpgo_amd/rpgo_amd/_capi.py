@@ -29,7 +29,8 @@ _f = ctypes.POINTER(ctypes.c_float)
 class PcmProblem(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int32), ("n_poses", ctypes.c_int32), ("pose0", _d),
                 ("odom", _d), ("odom_cov", _d), ("n_lc", ctypes.c_int32), ("lc_from", _i32),
-                ("lc_to", _i32), ("lc_pose", _d), ("lc_cov", _d)]
+                ("lc_to", _i32), ("lc_pose", _d), ("lc_cov", _d), ("n_chains", ctypes.c_int32),
+                ("chain_off", _i32)]
 
 
 class PcmParams(ctypes.Structure):
@@ -236,16 +237,24 @@ class Context:
             pass
 
     # ---------------------------------------------------------------- PCM
-    def pcm_upload(self, dim, pose0, odom, odom_cov, lc_from, lc_to, lc_pose, lc_cov):
+    def pcm_upload(self, dim, pose0, odom, odom_cov, lc_from, lc_to, lc_pose, lc_cov, chain_off=None):
+        """chain_off: None for one chain, else the n_chains + 1 chain offsets (several
+        robots; pose0 then holds one start pose per chain)."""
         f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
         i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
         arrs = dict(pose0=f64(pose0), odom=f64(odom), odom_cov=f64(odom_cov), lc_from=i32(lc_from),
                     lc_to=i32(lc_to), lc_pose=f64(lc_pose), lc_cov=f64(lc_cov))
-        n_poses = arrs["odom"].reshape(-1, 12 if dim == 3 else 4).shape[0] + 1
+        nc = 1 if chain_off is None else len(chain_off) - 1
+        n_poses = arrs["odom"].reshape(-1, 12 if dim == 3 else 4).shape[0] + nc
+        if chain_off is not None:
+            arrs["chain_off"] = i32(chain_off)
+            if int(arrs["chain_off"][-1]) != n_poses:
+                raise ValueError("chain_off does not match the odometry rows")
         p = PcmProblem(dim=dim, n_poses=n_poses, pose0=_ptr(arrs["pose0"]), odom=_ptr(arrs["odom"]),
                        odom_cov=_ptr(arrs["odom_cov"]), n_lc=arrs["lc_from"].shape[0],
                        lc_from=_ptr(arrs["lc_from"], _i32), lc_to=_ptr(arrs["lc_to"], _i32),
-                       lc_pose=_ptr(arrs["lc_pose"]), lc_cov=_ptr(arrs["lc_cov"]))
+                       lc_pose=_ptr(arrs["lc_pose"]), lc_cov=_ptr(arrs["lc_cov"]), n_chains=nc,
+                       chain_off=_ptr(arrs["chain_off"], _i32) if chain_off is not None else None)
         check(lib().rpgo_pcm_upload(self._h, ctypes.byref(p)))
         self._n_lc = arrs["lc_from"].shape[0]
         self._n_poses = n_poses

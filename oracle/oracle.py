@@ -81,6 +81,10 @@ def lib():
             ("orc_pcm_simple_adjacency", [ctypes.c_int, ctypes.c_int, _d, ctypes.c_int, _i32, _i32,
                                           _d, _d] + [ctypes.c_double] * 4 + [_i32, _d, _u8, _d],
              ctypes.c_int),
+            ("orc_fold_chains", [ctypes.c_int, ctypes.c_int, _i32, _d, _d, _d, _d], None),
+            ("orc_pcm_adjacency_chains", [ctypes.c_int, ctypes.c_int, _i32, _d, ctypes.c_int, _i32, _i32, _d, _d,
+                                          ctypes.c_int] + [ctypes.c_double] * 4 + [_i32, _d, _u8, _d],
+             ctypes.c_int),
             ("orc_set_threads", [ctypes.c_int], None),
             ("orc_get_threads", [], ctypes.c_int),
             ("orc_pair_dist_list", [ctypes.c_int, _d, _i32, _i32, _d, _d, ctypes.c_int, _i32, _i32, _d],
@@ -286,6 +290,46 @@ def pcm_simple_adjacency(dim, traj, lc_from, lc_to, lc_pose, lc_cov, odom_trans,
                                        float(odom_rot), float(dist_trans), float(dist_rot),
                                        _p(acc, _i32), _p(od), _p(adj, _u8),
                                        _p(dist) if dist is not None else None)
+    adj = adj.reshape(-1)[: m * m].reshape(m, m).copy()
+    if dist is not None:
+        dist = dist.reshape(-1)[: m * m].reshape(m, m).copy()
+    return acc[:m].copy(), od[:n].copy(), adj, dist
+
+
+# ---------------------------------------------------------------- several chains
+def fold_chains(dim, chain_off, pose0s, odom, odom_cov):
+    chain_off = np.ascontiguousarray(chain_off, dtype=np.int32)
+    nc = chain_off.shape[0] - 1
+    odom = _f64(odom).reshape(-1, pose_len(dim))
+    odom_cov = _f64(odom_cov).reshape(-1, tdim(dim) ** 2)
+    n = int(chain_off[-1])
+    assert odom.shape[0] == n - nc
+    out = np.zeros((n, lib().orc_pwc_size(dim)))
+    lib().orc_fold_chains(dim, nc, _p(chain_off, _i32), _p(_f64(pose0s)), _p(odom), _p(odom_cov), _p(out))
+    return out
+
+
+def pcm_adjacency_chains(dim, chain_off, traj, lc_from, lc_to, lc_pose, lc_cov, odom_thr, lc_thr,
+                         simple=None, want_dist=True):
+    """Adjacency stage of one loop-closure group over several chains.  simple:
+    None (Mahalanobis, odom_thr / lc_thr) or (odom_trans, odom_rot, dist_trans,
+    dist_rot).  Same return layout as pcm_adjacency."""
+    chain_off = np.ascontiguousarray(chain_off, dtype=np.int32)
+    traj = _f64(traj)
+    lc_from = np.ascontiguousarray(lc_from, dtype=np.int32)
+    lc_to = np.ascontiguousarray(lc_to, dtype=np.int32)
+    n = lc_from.shape[0]
+    lc_pose = _f64(lc_pose).reshape(n, pose_len(dim))
+    lc_cov = _f64(lc_cov).reshape(n, tdim(dim) ** 2)
+    acc = np.zeros(max(n, 1), dtype=np.int32)
+    od = np.zeros(max(n, 1))
+    adj = np.zeros((max(n, 1), max(n, 1)), dtype=np.uint8)
+    dist = np.zeros((max(n, 1), max(n, 1))) if want_dist else None
+    thr = (odom_thr, 0.0, lc_thr, 0.0) if simple is None else tuple(simple)
+    m = lib().orc_pcm_adjacency_chains(dim, chain_off.shape[0] - 1, _p(chain_off, _i32), _p(traj), n,
+                                       _p(lc_from, _i32), _p(lc_to, _i32), _p(lc_pose), _p(lc_cov),
+                                       0 if simple is None else 1, *[float(v) for v in thr], _p(acc, _i32),
+                                       _p(od), _p(adj, _u8), _p(dist) if dist is not None else None)
     adj = adj.reshape(-1)[: m * m].reshape(m, m).copy()
     if dist is not None:
         dist = dist.reshape(-1)[: m * m].reshape(m, m).copy()

@@ -13,6 +13,7 @@
  *   include/KimeraRPGO/outlier/Pcm.h:725-768          incrementAdjMatrix
  *   include/KimeraRPGO/utils/GeometryUtils.h:193-289  PoseWithNode
  *   include/KimeraRPGO/outlier/Pcm.h:581-596,654-662  PcmSimple checks
+ *   include/KimeraRPGO/outlier/Pcm.h:460-465,691-698  inter-robot loop closures
  *   include/KimeraRPGO/max_clique_finder/graphIO.cpp:188-257
  *   include/KimeraRPGO/max_clique_finder/findCliqueHeu.cpp:32-209
  *   include/KimeraRPGO/max_clique_finder/findClique.cpp:31-145
@@ -588,6 +589,105 @@ int orc_pcm_adjacency(int dim, int n_poses, const double* traj,
         dist[(size_t)i * m + j] = dv;
       }
       if (adj && dv < lc_thr) {
+        adj[(size_t)j * m + i] = 1;
+        adj[(size_t)i * m + j] = 1;
+      }
+    }
+  }
+  return m;
+}
+
+/* ------------------------------------------------------------------ */
+/* Several robot trajectories (Pcm.h:516-557 per prefix, 414-502, 670-718) */
+/* ------------------------------------------------------------------ */
+
+static int chain_of(int n_chains, const int32_t* chain_off, int k) {
+  int lo = 0, hi = n_chains - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (chain_off[mid] <= k) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+/* Fold of every chain: chain c covers poses [chain_off[c], chain_off[c+1]),
+ * starts at pose0s[c] with zero covariance (Pcm.h:534-542) and uses odometry
+ * rows [chain_off[c] - c, chain_off[c+1] - c - 1). */
+void orc_fold_chains(int dim, int n_chains, const int32_t* chain_off, const double* pose0s,
+                     const double* odom, const double* odom_cov, double* traj) {
+  const int pl = pose_len(dim), d = tan_dim(dim), ps = orc_pwc_size(dim);
+  for (int c = 0; c < n_chains; ++c) {
+    const int p0 = chain_off[c], n = chain_off[c + 1] - p0, o0 = p0 - c;
+    orc_fold(dim, n, pose0s + (size_t)c * pl, odom + (size_t)o0 * pl, odom_cov + (size_t)o0 * d * d,
+             traj + (size_t)p0 * ps);
+  }
+}
+
+/* Adjacency stage of one loop-closure group over several chains:
+ *  - a loop closure whose endpoints lie on different chains (an inter-robot
+ *    loop closure) passes the odometry check unchecked, distance 0
+ *    (Pcm.h:460-465);
+ *  - for a pair, when the older loop closure's first endpoint and the newer
+ *    one's lie on different chains, the newer one's endpoints are swapped
+ *    without inverting its measurement (Pcm.h:691-698).
+ * simple != 0 uses the PcmSimple distances (thresholds odom_thr/odom_thr2 and
+ * lc_thr/lc_thr2 as translation/rotation); otherwise the Mahalanobis ones
+ * (odom_thr, lc_thr). */
+int orc_pcm_adjacency_chains(int dim, int n_chains, const int32_t* chain_off, const double* traj,
+                             int n_lc, const int32_t* lc_from, const int32_t* lc_to,
+                             const double* lc_pose, const double* lc_cov, int simple,
+                             double odom_thr, double odom_thr2, double lc_thr, double lc_thr2,
+                             int32_t* accepted, double* odom_dist_out, uint8_t* adj, double* dist) {
+  const int pl = pose_len(dim), dd = tan_dim(dim) * tan_dim(dim);
+  const int odom_check = simple ? (odom_thr >= 0.0 && odom_thr2 >= 0.0) : odom_thr >= 0.0;
+  const int loop_check = simple ? (lc_thr >= 0.0 && lc_thr2 >= 0.0) : lc_thr >= 0.0;
+  int m = 0;
+  for (int k = 0; k < n_lc; ++k) {
+    const int a = lc_from[k], b = lc_to[k];
+    double od = 0.0, rot = 0.0;
+    int ok = 1;
+    if (chain_of(n_chains, chain_off, a) == chain_of(n_chains, chain_off, b)) {
+      if (simple) {
+        od = orc_simple_odom(dim, traj, a, b, lc_pose + (size_t)k * pl, lc_cov + (size_t)k * dd, &rot);
+        ok = !odom_check || (od < odom_thr && rot < odom_thr2);
+      } else {
+        od = orc_odom_dist(dim, traj, a, b, lc_pose + (size_t)k * pl, lc_cov + (size_t)k * dd);
+        ok = !odom_check || od < odom_thr;
+      }
+    }
+    if (odom_dist_out) odom_dist_out[k] = od;
+    if (ok) accepted[m++] = k;
+  }
+  if (adj) memset(adj, 0, (size_t)m * m);
+  if (dist) memset(dist, 0, sizeof(double) * (size_t)m * m);
+  if (!loop_check) return m;
+  for (int j = 1; j < m; ++j) {
+    const int kj = accepted[j];
+    for (int i = 0; i < j; ++i) {
+      const int ki = accepted[i];
+      const int a = lc_from[ki], b = lc_to[ki];
+      int c = lc_from[kj], d_ = lc_to[kj];
+      if (chain_of(n_chains, chain_off, a) != chain_of(n_chains, chain_off, c)) {
+        const int t = c;
+        c = d_;
+        d_ = t;
+      }
+      double dv, rot = 0.0;
+      int cons;
+      if (simple) {
+        dv = orc_simple_pair(dim, traj, a, b, lc_pose + (size_t)ki * pl, lc_cov + (size_t)ki * dd, c, d_,
+                             lc_pose + (size_t)kj * pl, lc_cov + (size_t)kj * dd, &rot);
+        cons = dv < lc_thr && rot < lc_thr2;
+      } else {
+        dv = orc_pair_dist(dim, traj, a, b, lc_pose + (size_t)ki * pl, lc_cov + (size_t)ki * dd, c, d_,
+                           lc_pose + (size_t)kj * pl, lc_cov + (size_t)kj * dd);
+        cons = dv < lc_thr;
+      }
+      if (dist) {
+        dist[(size_t)j * m + i] = dv;
+        dist[(size_t)i * m + j] = dv;
+      }
+      if (adj && cons) {
         adj[(size_t)j * m + i] = 1;
         adj[(size_t)i * m + j] = 1;
       }

@@ -124,6 +124,25 @@ int orc_pcm_simple_adjacency(int dim, int n_poses, const double* traj, int n_lc,
                              double dist_trans, double dist_rot, int32_t* accepted_out,
                              double* odom_dist_out, uint8_t* adj_out, double* dist_out);
 
+/* ---- several robot trajectories (multi-robot groups) ---- */
+
+/* Fold of n_chains chains: chain c covers poses [chain_off[c], chain_off[c+1]),
+ * starts at pose0s[c], odometry rows [chain_off[c] - c, chain_off[c+1] - c - 1). */
+void orc_fold_chains(int dim, int n_chains, const int32_t* chain_off, const double* pose0s,
+                     const double* odom, const double* odom_cov, double* traj_out);
+/* Adjacency stage of one loop-closure group (ObservationId) over several
+ * chains: inter-chain loop closures skip the odometry check (distance 0,
+ * Pcm.h:460-465); a pair whose first endpoints lie on different chains swaps
+ * the newer loop closure's endpoints (Pcm.h:691-698).  simple != 0: PcmSimple
+ * distances with (odom_thr, odom_thr2) and (lc_thr, lc_thr2) as
+ * (translation, rotation) thresholds. */
+int orc_pcm_adjacency_chains(int dim, int n_chains, const int32_t* chain_off, const double* traj,
+                             int n_lc, const int32_t* lc_from, const int32_t* lc_to,
+                             const double* lc_pose, const double* lc_cov, int simple,
+                             double odom_thr, double odom_thr2, double lc_thr, double lc_thr2,
+                             int32_t* accepted_out, double* odom_dist_out, uint8_t* adj_out,
+                             double* dist_out);
+
 /* ---- max clique (findCliqueHeu.cpp / findClique.cpp / graphIO.cpp) ---- */
 
 /* CSR from a dense symmetric byte adjacency (graphIO.cpp:188-230).
