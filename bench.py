@@ -3,8 +3,10 @@
 Workload (default): SE(3) helix, 5,000 poses, 1,000 loop closures (20 %
 outliers), odometry check off (odom_threshold = -1) so all 499,500 unique
 pairs (i<j) are evaluated, lc_threshold = 5.0.  A step is one pass of the PCM
-adjacency stage over the uploaded graph: FP64 odometry fold, odometry check and
-compaction, pairwise consistency of every pair, adjacency bit packing.
+pairwise stage over the uploaded graph: FP32 consistency check of every pair,
+FP64 replay of the pairs near the threshold, adjacency bit packing, consistent
+pair count read back.  The odometry fold and check run once at ingest (the
+reference folds the trajectory as odometry arrives) and are reported apart.
 
 Prints one JSON line (metric, value, roofline of the dominant kernel, CPU
 baseline of the FP64 oracle on the host cores).
@@ -28,6 +30,21 @@ import numpy as np  # noqa: E402
 # 5 adjoints, sym add/sub, 2 PSD checks, Log, Cholesky Mahalanobis); SE(2) ~720.
 FLOP_PER_PAIR = {3: 4540.0, 2: 720.0}
 PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}
+# HBM bytes per k_pairs32 launch on se3_5k, from the rocprofv3 FETCH_SIZE and
+# WRITE_SIZE passes written up in this file (scaled by a plain copy's counts)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_pairs32_se3_5k_traffic.md")
+
+
+def measured_traffic():
+    """bytes per launch recorded in TRAFFIC_FILE ('traffic_bytes_per_launch: N'), or None"""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 WORKLOADS = {
     "se3_5k": dict(kind="se3_helix", n_poses=5000, n_lc=1000, outlier_frac=0.2, odom_thr=-1.0, lc_thr=5.0),
@@ -62,7 +79,7 @@ def cpu_baseline(g, odom_thr, lc_thr, budget_s=15.0):
                        f"{t_pairs:.1f} s; fold {t_fold * 1e3:.1f} ms)")
 
 
-def robust_solve(ctx, gl):
+def robust_solve(ctx, gl, **solve_params):
     """Robust-solve ms on the 50,000-pose graph: PCM (FP32 + FP64 recheck, the
     reference's heuristic clique) then GNC-TLS over LM on odometry + prior +
     the clique's loop closures; plain GNC (started as GTSAM starts it) against
@@ -73,7 +90,7 @@ def robust_solve(ctx, gl):
     t0 = time.perf_counter()
     pcm = learned.pcm_stage(ctx, gl)
     ms_pcm = pcm.ms
-    poses_p, w_p, rep_p, nk = learned.solve_plain(ctx, gl, pcm.inliers)
+    poses_p, w_p, rep_p, nk = learned.solve_plain(ctx, gl, pcm.inliers, **solve_params)
     true_in = ~gl.is_outlier[pcm.inliers]
 
     def quality(poses, w):
@@ -92,7 +109,7 @@ def robust_solve(ctx, gl):
            "end_to_end_plain_ms": ms_pcm + rep_p.ms_total}
     if os.path.exists(learned.MODEL_PATH):
         model = learned.LearnedModel.load()
-        poses_l, w_l, rep_l, fb, ms_l = learned.solve_learned(ctx, gl, pcm, model)
+        poses_l, w_l, rep_l, fb, ms_l = learned.solve_learned(ctx, gl, pcm, model, **solve_params)
         res["learned"] = {"ms": ms_l, "gnc_iterations": rep_l.gnc_iterations,
                           "lm_iterations": rep_l.lm_iterations, "fallback": bool(fb),
                           "tls_fixed_point": rep_l.n_tls_disagree == 0,
@@ -160,6 +177,10 @@ def main():
                     help="skip the robust-solve measurement on the 50,000-pose graph")
     ap.add_argument("--no-batch", dest="batch", action="store_false",
                     help="skip the 256-graph batch measurement")
+    ap.add_argument("--solve-poses", type=int, default=50000,
+                    help="poses of the robust-solve graph (SE(3) helix, loop closures 1/5 of the poses, 50 %% outliers)")
+    ap.add_argument("--pcg-graphs", type=int, default=1, choices=[0, 1],
+                    help="replay PCG iterations as captured HIP graphs (1) or launch them directly (0)")
     ap.add_argument("--seed", type=int, default=0)
     a = ap.parse_args()
     if a.gpus != 1:
@@ -212,7 +233,8 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "kernel": "k_pairs64" if a.precision == "fp64" else "k_pairs32",
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": None, "flop_per_pair": FLOP_PER_PAIR[g.dim], "ms_kernel": ms_pairs},
+                     "traffic": measured_traffic() if (a.workload == "se3_5k" and a.precision == "fp32") else None,
+                     "flop_per_pair": FLOP_PER_PAIR[g.dim], "ms_kernel": ms_pairs},
         "stages_ms": {"pairs_kernel": ms_pairs, "recheck_fp64": float(np.mean([s.ms_recheck for s in stats])),
                       "pairwise_stage_device": ms_total,
                       "ingest_fold_fp64": ingest.ms_fold, "ingest_odom_check": ingest.ms_odom,
@@ -243,7 +265,9 @@ def main():
             "n_flipped": sl[-1].n_flipped, "ingest_ms": ing.ms_total, "ingest_fold_ms": ing.ms_fold,
             "clique_heu_ref": {"size": kl, "ms": msl}}
         if a.solve:
-            out["robust_solve_50k"] = robust_solve(ctx, gl)
+            gs = gl if a.solve_poses == wl["n_poses"] else synth.se3_helix(
+                n_poses=a.solve_poses, n_lc=a.solve_poses // 5, outlier_frac=0.5, seed=a.seed)
+            out["robust_solve_50k" if gs is gl else f"robust_solve_{a.solve_poses}"] = robust_solve(ctx, gs, pcg_graphs=a.pcg_graphs)
     if a.batch:
         out["batch_c4"] = batch_c4(ctx, seed=a.seed)
     ctx.close()
