@@ -17,7 +17,24 @@
 // and a negative lc_threshold, dist_trans_threshold or dist_rot_threshold
 // the pairwise check (every accepted loop closure is then an inlier).
 //
-// Scope: one robot prefix (the intra-robot group of the reference).
+// Several robots (key prefixes): one odometry chain per prefix (robot order
+// = first-seen order), loop closures grouped per unordered prefix pair
+// (ObservationId); inter-robot loop closures skip the odometry check and each
+// group is one device problem over its one or two chains.  Landmarks (keys
+// whose character is a special symbol): the observation that brings the
+// landmark's value starts its group, re-observations are checked pairwise
+// against each other (Pcm.h:775-844) and the group's max clique is kept.
+// With a MultiRobotAlignMethod other than NONE the values of every robot
+// after the first (smallest prefix) are re-initialised from its odometry
+// chain and the GNC pose average of the frame transforms its inter-robot
+// inliers imply (Pcm.h:1011-1102), solved on the device (rpgo_solve).
+//
+// Differences from the reference, where its behaviour is undefined: an
+// odometry factor must extend its robot's chain at the end; a landmark
+// factor must run pose -> landmark and all observations of one landmark come
+// from one robot; loop closures must join poses of known chains.  Each
+// raises std::invalid_argument.  Group and landmark order in the graph to
+// optimize is first-seen order (the reference iterates hash maps).
 #pragma once
 
 #include <memory>
@@ -67,14 +84,19 @@ class Pcm : public OutlierRemoval {
   // distances (PcmSimple3D / PcmSimple2D)
   Pcm(int dim, const PcmParams& params, const std::vector<char>& special_symbols = {},
       const DeviceParams& device = DeviceParams(), bool simple = false);
+  Pcm(int dim, const PcmParams& params, MultiRobotAlignMethod align, double align_gnc_probability = 0.99,
+      const std::vector<char>& special_symbols = {}, const DeviceParams& device = DeviceParams(),
+      bool simple = false);
   ~Pcm() override;
   Pcm(const Pcm&) = delete;
   Pcm& operator=(const Pcm&) = delete;
 
   bool removeOutliers(const NonlinearFactorGraph& new_factors, const Values& new_values,
                       NonlinearFactorGraph* nfg, Values* values) override;
-  size_t getNumLC() const override { return lc_.size(); }
-  size_t getNumLCInliers() const override { return inliers_.size(); }
+  // Pcm.h:135-138: every loop closure and landmark observation taken in
+  // (never decremented), the inliers of the last max-clique pass
+  size_t getNumLC() const override { return total_lc_; }
+  size_t getNumLCInliers() const override { return total_good_lc_; }
   size_t getNumOdomFactors() const override { return odom_.size(); }
   size_t getNumSpecialFactors() const override { return nfg_special_.size(); }
   EdgePtr removeLastLoopClosure(char prefix_1, char prefix_2, NonlinearFactorGraph* nfg) override;
@@ -90,57 +112,75 @@ class Pcm : public OutlierRemoval {
   long long lastPairs() const { return pairs_; }
 
  private:
+  struct Chain {  // one robot's odometry (Pcm.h:516-557 trajectory)
+    char prefix = 0;
+    Key first_key = 0;
+    Pose3 pose0_3;
+    Pose2 pose0_2;
+    std::vector<FactorPtr> odom;
+  };
+  struct Group {  // loop closures of one ObservationId (unordered prefix pair)
+    char id1 = 0, id2 = 0;
+    std::vector<FactorPtr> lc;  // factors that passed the odometry check
+    std::vector<int> inliers;   // consistent factors: clique indices into lc
+  };
+  struct Landmark {  // observations of one landmark (Pcm.h:207-220, 431-448)
+    Key key = 0;
+    std::vector<FactorPtr> obs;
+    std::vector<int> inliers;
+  };
+
   bool isSpecialSymbol(char c) const;
+  int chainOf(char prefix) const;
+  int groupOf(char a, char b) const;
   void updateOdom(const FactorPtr& f, const Values& values);
-  void runPcm(const std::vector<FactorPtr>& pending);
+  // device stages of one group: odometry check of the pending loop closures,
+  // pairwise check of all, max clique; returns how many pending ones passed
+  int runGroup(Group& g, const std::vector<FactorPtr>& pending, bool incremental);
+  void runLandmark(Landmark& l);
+  void countInliers();
   NonlinearFactorGraph buildGraphToOptimize() const;
+  Values multirobotValueInitialization(const Values& values) const;
+  void robotValues(const Chain& c, const Pose3* T3, const Pose2* T2, Values* out) const;
   void check(int rc, const char* what) const;
 
   int dim_;
   bool simple_;
   bool odom_check_, loop_check_;  // Pcm.h:74-82
   PcmParams params_;
+  MultiRobotAlignMethod align_ = MultiRobotAlignMethod::NONE;
+  double align_prob_ = 0.99;
   std::vector<char> special_symbols_;
   DeviceParams device_;
   rpgo_ctx* ctx_ = nullptr;
-  char prefix_ = 0;
-  bool have_traj_ = false;
-  Key first_key_ = 0;
-  Pose3 pose0_3_;
-  Pose2 pose0_2_;
-  std::vector<FactorPtr> odom_;       // odometry factors in chain order
+  std::vector<Chain> chains_;         // robots in first-seen order (robot_order_)
+  std::vector<FactorPtr> odom_;       // every odometry factor in arrival order
   NonlinearFactorGraph nfg_special_;  // non-between factors, in arrival order
-  std::vector<FactorPtr> lc_;         // loop closures that passed the odometry check
-  std::vector<int> inliers_;          // clique output (indices into lc_)
+  std::vector<Group> groups_;
+  std::vector<int> lc_order_;         // group of every loop closure taken in, in order
+  std::vector<Landmark> landmarks_;
+  size_t total_lc_ = 0, total_good_lc_ = 0;
   std::vector<char> ignored_prefixes_;
   double pcm_ms_ = 0.0, clique_ms_ = 0.0;
   long long pairs_ = 0;
 };
 
-class Pcm3D : public Pcm {
- public:
-  explicit Pcm3D(const PcmParams& p, const std::vector<char>& special = {}, const DeviceParams& d = DeviceParams())
-      : Pcm(3, p, special, d) {}
-};
-
-class Pcm2D : public Pcm {
- public:
-  explicit Pcm2D(const PcmParams& p, const std::vector<char>& special = {}, const DeviceParams& d = DeviceParams())
-      : Pcm(2, p, special, d) {}
-};
-
-class PcmSimple3D : public Pcm {
- public:
-  explicit PcmSimple3D(const PcmParams& p, const std::vector<char>& special = {},
-                       const DeviceParams& d = DeviceParams())
-      : Pcm(3, p, special, d, true) {}
-};
-
-class PcmSimple2D : public Pcm {
- public:
-  explicit PcmSimple2D(const PcmParams& p, const std::vector<char>& special = {},
-                       const DeviceParams& d = DeviceParams())
-      : Pcm(2, p, special, d, true) {}
-};
+// Pcm3D(params, align_method, align_probability, special_symbols) as the
+// reference constructs them (Pcm.h:52-95), or without the alignment method.
+#define RPGO_PCM_VARIANT(NAME, DIM, SIMPLE)                                                              \
+  class NAME : public Pcm {                                                                              \
+   public:                                                                                               \
+    explicit NAME(const PcmParams& p, const std::vector<char>& special = {},                             \
+                  const DeviceParams& d = DeviceParams())                                                \
+        : Pcm(DIM, p, special, d, SIMPLE) {}                                                             \
+    NAME(const PcmParams& p, MultiRobotAlignMethod align, double align_gnc_probability = 0.99,           \
+         const std::vector<char>& special = {}, const DeviceParams& d = DeviceParams())                  \
+        : Pcm(DIM, p, align, align_gnc_probability, special, d, SIMPLE) {}                              \
+  };
+RPGO_PCM_VARIANT(Pcm3D, 3, false)
+RPGO_PCM_VARIANT(Pcm2D, 2, false)
+RPGO_PCM_VARIANT(PcmSimple3D, 3, true)
+RPGO_PCM_VARIANT(PcmSimple2D, 2, true)
+#undef RPGO_PCM_VARIANT
 
 }  // namespace kimera_rpgo_amd

@@ -74,6 +74,10 @@ class NoiseModel {
   static std::shared_ptr<NoiseModel> Covariance(int dim, const std::vector<double>& cov);
   static std::shared_ptr<NoiseModel> Sigmas(const std::vector<double>& sigmas);
   static std::shared_ptr<NoiseModel> Variances(const std::vector<double>& variances);
+  // diagonal information; a zero precision leaves that direction unobserved
+  // and, as gtsam's Gaussian::covariance() (R^-1 R^-T by back-substitution)
+  // does, fills the covariance rows and columns that meet it with NaN
+  static std::shared_ptr<NoiseModel> Precisions(const std::vector<double>& precisions);
   static std::shared_ptr<NoiseModel> Isotropic(int dim, double sigma);
   static std::shared_ptr<NoiseModel> IsotropicVariance(int dim, double variance);
   int dim() const { return dim_; }
@@ -95,6 +99,7 @@ struct Isotropic {
 struct Diagonal {
   static SharedNoiseModel Sigmas(const std::vector<double>& s) { return NoiseModel::Sigmas(s); }
   static SharedNoiseModel Variances(const std::vector<double>& v) { return NoiseModel::Variances(v); }
+  static SharedNoiseModel Precisions(const std::vector<double>& p) { return NoiseModel::Precisions(p); }
 };
 struct Gaussian {
   static SharedNoiseModel Information(int dim, const std::vector<double>& m) {

@@ -75,7 +75,15 @@ const char* rpgo_version(void);
  * odometry check unchecked (Pcm.h:460-465; its odom_dist is 0), and for a pair
  * whose first endpoints lie on different chains the newer loop closure's
  * endpoints are swapped without inverting its measurement (Pcm.h:691-698).
- * Such problems run the FP64 replay; rpgo_pcm_append takes one chain only. */
+ * Such problems run the FP64 replay; rpgo_pcm_append takes one chain only.
+ *
+ * Landmark observations (landmark != 0): the loop closures are the
+ * observations of one landmark l, BetweenFactor(pose lc_from[k], l, lc_pose[k])
+ * from one robot's trajectory (n_chains <= 1); lc_to is not read.  No odometry
+ * check (every observation is accepted, odom_dist 0) and the pair check is
+ * incrementLandmarkAdjMatrix's (Pcm.h:775-844): loop = (between(i, j) o
+ * Z_jl)^-1 o Z_il, with the PCM or PcmSimple distance of the run.  FP64 replay;
+ * not streamed. */
 typedef struct {
   int32_t dim;            /* 3 = SE(3) (Pcm3D), 2 = SE(2) (Pcm2D)        */
   int32_t n_poses;        /* >= 1                                          */
@@ -90,6 +98,7 @@ typedef struct {
   const double* lc_cov;   /* n_lc * d*d                                    */
   int32_t n_chains;       /* 0 or 1: one chain; > 1: several robots         */
   const int32_t* chain_off; /* n_chains + 1 (n_chains > 1 only)            */
+  int32_t landmark;       /* nonzero: observations of one landmark         */
 } rpgo_pcm_problem;
 
 typedef struct {

@@ -124,12 +124,17 @@ int rpgo_pcm_upload(rpgo_ctx* ctx, const rpgo_pcm_problem* p) {
     return RPGO_ERR_ARG;
   }
   const int nc = p->n_chains > 1 ? p->n_chains : 1;
+  const bool landmark = p->landmark != 0;
   if (p->n_poses < nc || p->n_lc < 0 || !p->pose0 ||
       (p->n_poses > nc && (!p->odom || !p->odom_cov)) ||
-      (p->n_lc > 0 && (!p->lc_from || !p->lc_to || !p->lc_pose || !p->lc_cov)) ||
+      (p->n_lc > 0 && (!p->lc_from || (!landmark && !p->lc_to) || !p->lc_pose || !p->lc_cov)) ||
       (nc > 1 && !p->chain_off)) {
     set_error("rpgo_pcm_upload: inconsistent sizes or NULL arrays (n_poses=%d n_lc=%d n_chains=%d)",
               p->n_poses, p->n_lc, p->n_chains);
+    return RPGO_ERR_ARG;
+  }
+  if (landmark && nc > 1) {
+    set_error("rpgo_pcm_upload: landmark observations come from one trajectory (n_chains <= 1)");
     return RPGO_ERR_ARG;
   }
   std::vector<int> chain_of;
@@ -148,10 +153,10 @@ int rpgo_pcm_upload(rpgo_ctx* ctx, const rpgo_pcm_problem* p) {
     }
   }
   for (int k = 0; k < p->n_lc; ++k) {
-    if (p->lc_from[k] < 0 || p->lc_from[k] >= p->n_poses || p->lc_to[k] < 0 ||
-        p->lc_to[k] >= p->n_poses) {
+    if (p->lc_from[k] < 0 || p->lc_from[k] >= p->n_poses ||
+        (!landmark && (p->lc_to[k] < 0 || p->lc_to[k] >= p->n_poses))) {
       set_error("rpgo_pcm_upload: loop closure %d references pose (%d,%d) outside [0,%d)", k,
-                p->lc_from[k], p->lc_to[k], p->n_poses);
+                p->lc_from[k], landmark ? p->lc_from[k] : p->lc_to[k], p->n_poses);
       return RPGO_ERR_ARG;
     }
     if (nc > 1) {
@@ -176,7 +181,8 @@ int rpgo_pcm_upload(rpgo_ctx* ctx, const rpgo_pcm_problem* p) {
   if ((rc = upload(s.odom, p->odom, sizeof(double) * PL * nd, st))) return rc;
   if ((rc = upload(s.odom_cov, p->odom_cov, sizeof(double) * D * D * nd, st))) return rc;
   if ((rc = upload(s.lc_from, p->lc_from, sizeof(int) * nl, st))) return rc;
-  if ((rc = upload(s.lc_to, p->lc_to, sizeof(int) * nl, st))) return rc;
+  // landmark observations: lc_to is not read; the pose index stands in for it
+  if ((rc = upload(s.lc_to, landmark ? p->lc_from : p->lc_to, sizeof(int) * nl, st))) return rc;
   if ((rc = upload(s.lc_pose, p->lc_pose, sizeof(double) * PL * nl, st))) return rc;
   if ((rc = upload(s.lc_cov, p->lc_cov, sizeof(double) * D * D * nl, st))) return rc;
   RPGO_HIP_CHECK(hipStreamSynchronize(st));
@@ -184,6 +190,7 @@ int rpgo_pcm_upload(rpgo_ctx* ctx, const rpgo_pcm_problem* p) {
   s.n_poses = p->n_poses;
   s.n_lc = p->n_lc;
   s.n_chains = nc;
+  s.landmark = landmark;
   s.uploaded = true;
   s.ran = false;
   return RPGO_OK;
@@ -429,8 +436,8 @@ int pcm_append_impl(rpgo_ctx* ctx, const rpgo_pcm_increment* inc, double odom_th
     set_error("%s: the last run used the %s checks", name, s.simple ? "PcmSimple" : "PCM");
     return RPGO_ERR_STATE;
   }
-  if (s.n_chains > 1) {
-    set_error("%s: streaming takes a problem with one chain", name);
+  if (s.n_chains > 1 || s.landmark) {
+    set_error("%s: streaming takes a problem with one chain and no landmark", name);
     return RPGO_ERR_STATE;
   }
   if (want_dist) {

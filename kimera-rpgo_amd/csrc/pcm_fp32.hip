@@ -467,7 +467,7 @@ int pcm_fp32_prepare(rpgo_ctx* ctx, bool* use_fp64) {
   PcmState& s = ctx->pcm;
   hipStream_t st = ctx->stream;
   *use_fp64 = false;
-  if (s.dim != 3 || s.n_chains > 1) {  // SE(2) and several chains run the FP64 replay
+  if (s.dim != 3 || s.n_chains > 1 || s.landmark) {  // these run the FP64 replay
     *use_fp64 = true;
     return RPGO_OK;
   }

@@ -16,10 +16,13 @@
 namespace rpgo {
 
 // Robot trajectories of a problem: chain c covers poses [off[c], off[c+1]).
-// One chain when n <= 1 (off unused).
+// One chain when n <= 1 (off unused).  landmark: the loop closures are the
+// observations of one landmark from poses cf[] (no odometry check, the
+// landmark pair check).
 struct Chains {
   const int* off;
   int n;
+  int landmark = 0;
   __device__ __forceinline__ int of(int k) const {
     if (n <= 1) return 0;
     int lo = 0, hi = n - 1;
@@ -213,7 +216,9 @@ __global__ void k_odom_check(int n_lc, const double* __restrict__ traj,
   constexpr int PWC = R64<DIM>::PWC;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n_lc) return;
-  if (ch.of(from[k]) != ch.of(to[k])) {  // inter-robot: no odometry check (Pcm.h:460-465)
+  // inter-robot loop closures and landmark observations: no odometry check
+  // (Pcm.h:449-465)
+  if (ch.landmark || ch.of(from[k]) != ch.of(to[k])) {
     dist[k] = 0.0;
     flag[k] = 1;
     return;
@@ -237,7 +242,7 @@ __global__ void k_odom_simple(int n_lc, const double* __restrict__ traj,
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n_lc) return;
   const int a = from[k], b = to[k];
-  if (ch.of(a) != ch.of(b)) {  // inter-robot: no odometry check (Pcm.h:460-465)
+  if (ch.landmark || ch.of(a) != ch.of(b)) {  // no odometry check (Pcm.h:449-465)
     dist[k] = 0.0;
     flag[k] = 1;
     return;
@@ -334,8 +339,10 @@ __global__ void __launch_bounds__(256) k_pairs64(int m, long long n_tiles,
       c = dd;
       dd = t;
     }
-    d = R64<DIM>::pair_dist(traj, cf[i], ct[i], clcm + (size_t)i * PWC, c, dd,
-                            clcm + (size_t)j * PWC);
+    d = ch.landmark ? R64<DIM>::landmark_pair_dist(traj, cf[i], clcm + (size_t)i * PWC, cf[j],
+                                                   clcm + (size_t)j * PWC)
+                    : R64<DIM>::pair_dist(traj, cf[i], ct[i], clcm + (size_t)i * PWC, c, dd,
+                                          clcm + (size_t)j * PWC);
     if (dist) {
       dist[(size_t)j * m + i] = d;
       dist[(size_t)i * m + j] = d;
@@ -381,8 +388,12 @@ __global__ void __launch_bounds__(256) k_pairs_simple(int m, long long n_tiles,
       c = dd;
       dd = t;
     }
-    R64<DIM>::simple_pair(traj, cf[i], ct[i], clcm + (size_t)i * PWC, c, dd,
-                          clcm + (size_t)j * PWC, &tn, &rn);
+    if (ch.landmark)
+      R64<DIM>::simple_landmark_pair(traj, cf[i], clcm + (size_t)i * PWC, cf[j], clcm + (size_t)j * PWC, &tn,
+                                     &rn);
+    else
+      R64<DIM>::simple_pair(traj, cf[i], ct[i], clcm + (size_t)i * PWC, c, dd, clcm + (size_t)j * PWC, &tn,
+                            &rn);
     cons = tn < thr_t && rn < thr_r;
     if (dist) {
       dist[(size_t)j * m + i] = tn;
@@ -436,8 +447,10 @@ __device__ __forceinline__ void recheck_one(int2 q, const double* __restrict__ t
     c = dd;
     dd = t;
   }
-  const double d = R64<DIM>::pair_dist(traj, cf[i], ct[i], clcm + (size_t)i * PWC, c, dd,
-                                       clcm + (size_t)j * PWC);
+  const double d = ch.landmark ? R64<DIM>::landmark_pair_dist(traj, cf[i], clcm + (size_t)i * PWC, cf[j],
+                                                              clcm + (size_t)j * PWC)
+                               : R64<DIM>::pair_dist(traj, cf[i], ct[i], clcm + (size_t)i * PWC, c, dd,
+                                                     clcm + (size_t)j * PWC);
   const bool cons = d < thr;
   unsigned long long* wj = adj + (size_t)j * W + (i >> 6);
   unsigned long long* wi = adj + (size_t)i * W + (j >> 6);
@@ -569,7 +582,9 @@ __global__ void __launch_bounds__(256) k_pairs_append(int m_old, int m, int W,
 // ---------------------------------------------------------------------------
 // host drivers
 // ---------------------------------------------------------------------------
-static Chains chains_of(const PcmState& s) { return Chains{s.chain_off.as<int>(), s.n_chains}; }
+static Chains chains_of(const PcmState& s) {
+  return Chains{s.chain_off.as<int>(), s.n_chains, s.landmark ? 1 : 0};
+}
 
 int pcm_recheck64(rpgo_ctx* ctx, int n_queue, double lc_thr, bool want_dist) {
   PcmState& s = ctx->pcm;

@@ -86,6 +86,7 @@ struct PcmState {
   int n_poses = 0;
   int n_lc = 0;
   int n_chains = 1;  // robot trajectories of the uploaded problem
+  bool landmark = false;  // loop closures are observations of one landmark
   DevBuf chain_off;  // n_chains + 1 ints (several chains only)
   bool uploaded = false;
   bool ran = false;

@@ -53,16 +53,24 @@ RobustSolver::RobustSolver(const RobustSolverParams& params) : params_(params) {
     case OutlierRemovalMethod::NONE:
       break;
     case OutlierRemovalMethod::PCM2D:
-      outlier_removal_.reset(new Pcm2D(params.pcm_params, params.specialSymbols, params.device_params));
+      outlier_removal_.reset(new Pcm2D(params.pcm_params, params.multirobot_align_method,
+                                          params.multirobot_align_gnc_prob, params.specialSymbols,
+                                          params.device_params));
       break;
     case OutlierRemovalMethod::PCM3D:
-      outlier_removal_.reset(new Pcm3D(params.pcm_params, params.specialSymbols, params.device_params));
+      outlier_removal_.reset(new Pcm3D(params.pcm_params, params.multirobot_align_method,
+                                          params.multirobot_align_gnc_prob, params.specialSymbols,
+                                          params.device_params));
       break;
     case OutlierRemovalMethod::PCM_Simple2D:
-      outlier_removal_.reset(new PcmSimple2D(params.pcm_params, params.specialSymbols, params.device_params));
+      outlier_removal_.reset(new PcmSimple2D(params.pcm_params, params.multirobot_align_method,
+                                          params.multirobot_align_gnc_prob, params.specialSymbols,
+                                          params.device_params));
       break;
     case OutlierRemovalMethod::PCM_Simple3D:
-      outlier_removal_.reset(new PcmSimple3D(params.pcm_params, params.specialSymbols, params.device_params));
+      outlier_removal_.reset(new PcmSimple3D(params.pcm_params, params.multirobot_align_method,
+                                          params.multirobot_align_gnc_prob, params.specialSymbols,
+                                          params.device_params));
       break;
   }
   if (params.verbosity != Verbosity::VERBOSE && outlier_removal_) outlier_removal_->setQuiet();

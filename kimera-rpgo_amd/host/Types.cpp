@@ -158,6 +158,33 @@ std::shared_ptr<NoiseModel> NoiseModel::Variances(const std::vector<double>& v) 
   return m;
 }
 
+std::shared_ptr<NoiseModel> NoiseModel::Precisions(const std::vector<double>& p) {
+  const int d = (int)p.size();
+  auto m = std::make_shared<NoiseModel>();
+  m->dim_ = d;
+  m->info_.assign(d * d, 0.0);
+  for (int i = 0; i < d; ++i) m->info_[i * d + i] = p[i];
+  // Rinv = R^-1 column by column (R = diag(sqrt(p)), upper-triangular
+  // back-substitution in IEEE arithmetic: 0 * inf and 0 / 0 give NaN where a
+  // precision is zero), covariance = Rinv Rinv^T
+  std::vector<double> R(d * d, 0.0), Ri(d * d, 0.0);
+  for (int i = 0; i < d; ++i) R[i * d + i] = std::sqrt(p[i]);
+  for (int c = 0; c < d; ++c)
+    for (int r = d - 1; r >= 0; --r) {
+      double s = (r == c) ? 1.0 : 0.0;
+      for (int k = r + 1; k < d; ++k) s -= R[r * d + k] * Ri[k * d + c];
+      Ri[r * d + c] = s / R[r * d + r];
+    }
+  m->cov_.assign(d * d, 0.0);
+  for (int i = 0; i < d; ++i)
+    for (int j = 0; j < d; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < d; ++k) s += Ri[i * d + k] * Ri[j * d + k];
+      m->cov_[i * d + j] = s;
+    }
+  return m;
+}
+
 std::shared_ptr<NoiseModel> NoiseModel::Sigmas(const std::vector<double>& s) {
   std::vector<double> v(s.size());
   for (size_t i = 0; i < s.size(); ++i) v[i] = s[i] * s[i];

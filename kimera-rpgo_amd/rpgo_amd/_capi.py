@@ -30,7 +30,7 @@ class PcmProblem(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int32), ("n_poses", ctypes.c_int32), ("pose0", _d),
                 ("odom", _d), ("odom_cov", _d), ("n_lc", ctypes.c_int32), ("lc_from", _i32),
                 ("lc_to", _i32), ("lc_pose", _d), ("lc_cov", _d), ("n_chains", ctypes.c_int32),
-                ("chain_off", _i32)]
+                ("chain_off", _i32), ("landmark", ctypes.c_int32)]
 
 
 class PcmParams(ctypes.Structure):
@@ -237,9 +237,14 @@ class Context:
             pass
 
     # ---------------------------------------------------------------- PCM
-    def pcm_upload(self, dim, pose0, odom, odom_cov, lc_from, lc_to, lc_pose, lc_cov, chain_off=None):
+    def pcm_upload(self, dim, pose0, odom, odom_cov, lc_from, lc_to, lc_pose, lc_cov, chain_off=None,
+                   landmark=False):
         """chain_off: None for one chain, else the n_chains + 1 chain offsets (several
-        robots; pose0 then holds one start pose per chain)."""
+        robots; pose0 then holds one start pose per chain).  landmark: the loop
+        closures are observations of one landmark from poses lc_from (lc_to is
+        not read and may be None)."""
+        if landmark and lc_to is None:
+            lc_to = lc_from
         f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
         i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
         arrs = dict(pose0=f64(pose0), odom=f64(odom), odom_cov=f64(odom_cov), lc_from=i32(lc_from),
@@ -254,7 +259,8 @@ class Context:
                        odom_cov=_ptr(arrs["odom_cov"]), n_lc=arrs["lc_from"].shape[0],
                        lc_from=_ptr(arrs["lc_from"], _i32), lc_to=_ptr(arrs["lc_to"], _i32),
                        lc_pose=_ptr(arrs["lc_pose"]), lc_cov=_ptr(arrs["lc_cov"]), n_chains=nc,
-                       chain_off=_ptr(arrs["chain_off"], _i32) if chain_off is not None else None)
+                       chain_off=_ptr(arrs["chain_off"], _i32) if chain_off is not None else None,
+                       landmark=1 if landmark else 0)
         check(lib().rpgo_pcm_upload(self._h, ctypes.byref(p)))
         self._n_lc = arrs["lc_from"].shape[0]
         self._n_poses = n_poses

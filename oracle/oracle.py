@@ -85,6 +85,12 @@ def lib():
             ("orc_pcm_adjacency_chains", [ctypes.c_int, ctypes.c_int, _i32, _d, ctypes.c_int, _i32, _i32, _d, _d,
                                           ctypes.c_int] + [ctypes.c_double] * 4 + [_i32, _d, _u8, _d],
              ctypes.c_int),
+            ("orc_landmark_pair_dist", [ctypes.c_int, _d, ctypes.c_int, _d, _d, ctypes.c_int, _d, _d],
+             ctypes.c_double),
+            ("orc_simple_landmark_pair", [ctypes.c_int, _d, ctypes.c_int, _d, _d, ctypes.c_int, _d, _d, _d],
+             ctypes.c_double),
+            ("orc_pcm_adjacency_landmark", [ctypes.c_int, _d, ctypes.c_int, _i32, _d, _d, ctypes.c_int,
+                                            ctypes.c_double, ctypes.c_double, _u8, _d], None),
             ("orc_set_threads", [ctypes.c_int], None),
             ("orc_get_threads", [], ctypes.c_int),
             ("orc_pair_dist_list", [ctypes.c_int, _d, _i32, _i32, _d, _d, ctypes.c_int, _i32, _i32, _d],
@@ -334,6 +340,39 @@ def pcm_adjacency_chains(dim, chain_off, traj, lc_from, lc_to, lc_pose, lc_cov, 
     if dist is not None:
         dist = dist.reshape(-1)[: m * m].reshape(m, m).copy()
     return acc[:m].copy(), od[:n].copy(), adj, dist
+
+
+def landmark_pair_dist(dim, traj, i, zil, sil, j, zjl, sjl):
+    return lib().orc_landmark_pair_dist(dim, _p(_f64(traj)), int(i), _p(_f64(zil)), _p(_f64(sil)), int(j),
+                                        _p(_f64(zjl)), _p(_f64(sjl)))
+
+
+def simple_landmark_pair(dim, traj, i, zil, sil, j, zjl, sjl):
+    """(translation norm, rotation norm)"""
+    r = ctypes.c_double(0.0)
+    t = lib().orc_simple_landmark_pair(dim, _p(_f64(traj)), int(i), _p(_f64(zil)), _p(_f64(sil)), int(j),
+                                       _p(_f64(zjl)), _p(_f64(sjl)), ctypes.byref(r))
+    return t, r.value
+
+
+def pcm_adjacency_landmark(dim, traj, lc_from, lc_pose, lc_cov, lc_thr, simple=None, want_dist=True):
+    """Adjacency of one landmark's observations from poses lc_from (Pcm.h:775-844).
+    simple: None (Mahalanobis, lc_thr) or (dist_trans, dist_rot).  Returns (adj, dist)."""
+    traj = _f64(traj)
+    lc_from = np.ascontiguousarray(lc_from, dtype=np.int32)
+    n = lc_from.shape[0]
+    lc_pose = _f64(lc_pose).reshape(n, pose_len(dim))
+    lc_cov = _f64(lc_cov).reshape(n, tdim(dim) ** 2)
+    adj = np.zeros((max(n, 1), max(n, 1)), dtype=np.uint8)
+    dist = np.zeros((max(n, 1), max(n, 1))) if want_dist else None
+    thr = (lc_thr, 0.0) if simple is None else tuple(simple)
+    lib().orc_pcm_adjacency_landmark(dim, _p(traj), n, _p(lc_from, _i32), _p(lc_pose), _p(lc_cov),
+                                     0 if simple is None else 1, float(thr[0]), float(thr[1]), _p(adj, _u8),
+                                     _p(dist) if dist is not None else None)
+    adj = adj.reshape(-1)[: n * n].reshape(n, n).copy()
+    if dist is not None:
+        dist = dist.reshape(-1)[: n * n].reshape(n, n).copy()
+    return adj, dist
 
 
 def pcm_dist_rows(dim, traj, lc_from, lc_to, lc_pose, lc_cov, j0=None, j1=None):

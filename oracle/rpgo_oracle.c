@@ -799,6 +799,85 @@ double orc_simple_pair(int dim, const double* traj, int a, int b, const double* 
   return t;
 }
 
+/* ------------------------------------------------------------------ */
+/* Landmark observations (Pcm.h:775-844): loop closures (i, l) and (j, l) */
+/* from poses i, j of one trajectory to one landmark l.                   */
+/* ------------------------------------------------------------------ */
+
+/* incrementLandmarkAdjMatrix, PoseWithCovariance: loop = (between(i, j) o
+ * Z_jl)^-1 o Z_il, Mahalanobis norm (Pcm.h:818-833) */
+double orc_landmark_pair_dist(int dim, const double* traj, int i, const double* zil, const double* sil,
+                              int j, const double* zjl, const double* sjl) {
+  const int ps = orc_pwc_size(dim);
+  double il[64], jl[64], A[64], X[64], Y[64], L[64];
+  orc_pwc_from_measurement(dim, zil, sil, il);
+  orc_pwc_from_measurement(dim, zjl, sjl, jl);
+  orc_pwc_between(dim, traj + (size_t)i * ps, traj + (size_t)j * ps, A);
+  orc_pwc_compose(dim, A, jl, X);
+  orc_pwc_inverse(dim, X, Y);
+  orc_pwc_compose(dim, Y, il, L);
+  return orc_pwc_mahalanobis(dim, L);
+}
+
+/* the same with PoseWithNode: node |j - i| + 1 + 1 */
+double orc_simple_landmark_pair(int dim, const double* traj, int i, const double* zil,
+                                const double* sil, int j, const double* zjl, const double* sjl,
+                                double* rot_out) {
+  const int ps = orc_pwc_size(dim);
+  double il[64], jl[64], inv[12], A[12], X[12], Y[12], L[12];
+  orc_pwc_from_measurement(dim, zil, sil, il);
+  orc_pwc_from_measurement(dim, zjl, sjl, jl);
+  const double* Pi = traj + (size_t)i * ps;
+  const double* Pj = traj + (size_t)j * ps;
+  pose_inverse(dim, Pi, inv);
+  pose_compose(dim, inv, Pj, A);
+  pose_compose(dim, A, jl, X);
+  pose_inverse(dim, X, Y);
+  pose_compose(dim, Y, il, L);
+  const int node = abs(j - i) + 2;
+  const int ri = pwn_flag(dim, Pi) && pwn_flag(dim, Pj) && pwn_flag(dim, jl) && pwn_flag(dim, il);
+  double t, r;
+  orc_pwn_norms(dim, L, node, ri, &t, &r);
+  if (rot_out) *rot_out = r;
+  return t;
+}
+
+/* Adjacency of one landmark's observations from poses lc_from[k]: every
+ * observation is kept (no odometry check), pair (i < j) consistent when its
+ * distance is below lc_thr (and the rotation norm below lc_thr2 for simple);
+ * a negative threshold leaves the adjacency empty. */
+void orc_pcm_adjacency_landmark(int dim, const double* traj, int n_lc, const int32_t* lc_from,
+                                const double* lc_pose, const double* lc_cov, int simple, double lc_thr,
+                                double lc_thr2, uint8_t* adj, double* dist) {
+  const int pl = pose_len(dim), dd = tan_dim(dim) * tan_dim(dim);
+  const int loop_check = simple ? (lc_thr >= 0.0 && lc_thr2 >= 0.0) : lc_thr >= 0.0;
+  if (adj) memset(adj, 0, (size_t)n_lc * n_lc);
+  if (dist) memset(dist, 0, sizeof(double) * (size_t)n_lc * n_lc);
+  if (!loop_check) return;
+  for (int j = 1; j < n_lc; ++j)
+    for (int i = 0; i < j; ++i) {
+      double dv, rot = 0.0;
+      int cons;
+      if (simple) {
+        dv = orc_simple_landmark_pair(dim, traj, lc_from[i], lc_pose + (size_t)i * pl, lc_cov + (size_t)i * dd,
+                                      lc_from[j], lc_pose + (size_t)j * pl, lc_cov + (size_t)j * dd, &rot);
+        cons = dv < lc_thr && rot < lc_thr2;
+      } else {
+        dv = orc_landmark_pair_dist(dim, traj, lc_from[i], lc_pose + (size_t)i * pl, lc_cov + (size_t)i * dd,
+                                    lc_from[j], lc_pose + (size_t)j * pl, lc_cov + (size_t)j * dd);
+        cons = dv < lc_thr;
+      }
+      if (dist) {
+        dist[(size_t)j * n_lc + i] = dv;
+        dist[(size_t)i * n_lc + j] = dv;
+      }
+      if (adj && cons) {
+        adj[(size_t)j * n_lc + i] = 1;
+        adj[(size_t)i * n_lc + j] = 1;
+      }
+    }
+}
+
 int orc_pcm_simple_adjacency(int dim, int n_poses, const double* traj, int n_lc,
                              const int32_t* lc_from, const int32_t* lc_to, const double* lc_pose,
                              const double* lc_cov, double odom_trans, double odom_rot,

@@ -136,6 +136,14 @@ void orc_fold_chains(int dim, int n_chains, const int32_t* chain_off, const doub
  * the newer loop closure's endpoints (Pcm.h:691-698).  simple != 0: PcmSimple
  * distances with (odom_thr, odom_thr2) and (lc_thr, lc_thr2) as
  * (translation, rotation) thresholds. */
+double orc_landmark_pair_dist(int dim, const double* traj, int i, const double* zil, const double* sil,
+                              int j, const double* zjl, const double* sjl);
+double orc_simple_landmark_pair(int dim, const double* traj, int i, const double* zil,
+                                const double* sil, int j, const double* zjl, const double* sjl,
+                                double* rot_out);
+void orc_pcm_adjacency_landmark(int dim, const double* traj, int n_lc, const int32_t* lc_from,
+                                const double* lc_pose, const double* lc_cov, int simple, double lc_thr,
+                                double lc_thr2, uint8_t* adj, double* dist);
 int orc_pcm_adjacency_chains(int dim, int n_chains, const int32_t* chain_off, const double* traj,
                              int n_lc, const int32_t* lc_from, const int32_t* lc_to,
                              const double* lc_pose, const double* lc_cov, int simple,

@@ -1,7 +1,9 @@
 """C++ RobustSolver facade tests (tests/cpp/*.cpp), run on the GPU.
 
-Each binary mirrors single-robot cases of the reference's C++ tests
-(testPcm, testPcmSimple, testPcmDoOptimize, testGnc, testLoadGraph, testTemporary) through
+Each binary mirrors cases of the reference's C++ tests (testPcm,
+testPcmSimple, testPcmDoOptimize, testGnc, testLoadGraph, testTemporary,
+testMultiRobot, testIgnorePrefix, testRemoveLastLoopClosure, testLandmark,
+testMultirobotFrameAlign) through
 the facade library libkimera_rpgo.so, which drives the HIP kernels through
 the C ABI only.
 """
@@ -12,7 +14,7 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "tests", "cpp", "bin")
-CASES = ["test_pcm", "test_pcm_simple", "test_robust_solver"]
+CASES = ["test_pcm", "test_pcm_simple", "test_robust_solver", "test_multirobot"]
 
 
 @pytest.mark.gpu
