@@ -91,6 +91,7 @@ def lib():
              ctypes.c_double),
             ("orc_pcm_adjacency_landmark", [ctypes.c_int, _d, ctypes.c_int, _i32, _d, _d, ctypes.c_int,
                                             ctypes.c_double, ctypes.c_double, _u8, _d], None),
+            ("orc_cross_between", [ctypes.c_int, _d, _d, _d, _d, _d], None),
             ("orc_set_threads", [ctypes.c_int], None),
             ("orc_get_threads", [], ctypes.c_int),
             ("orc_pair_dist_list", [ctypes.c_int, _d, _i32, _i32, _d, _d, ctypes.c_int, _i32, _i32, _d],
@@ -340,6 +341,13 @@ def pcm_adjacency_chains(dim, chain_off, traj, lc_from, lc_to, lc_pose, lc_cov, 
     if dist is not None:
         dist = dist.reshape(-1)[: m * m].reshape(m, m).copy()
     return acc[:m].copy(), od[:n].copy(), adj, dist
+
+
+def cross_between(dim, a0, a, b0, b):
+    """Trajectory::getBetween of keys on two robots (PoseWithCovariance records)."""
+    out = np.zeros(lib().orc_pwc_size(dim))
+    lib().orc_cross_between(dim, _p(_f64(a0)), _p(_f64(a)), _p(_f64(b0)), _p(_f64(b)), _p(out))
+    return out
 
 
 def landmark_pair_dist(dim, traj, i, zil, sil, j, zjl, sjl):

@@ -610,6 +610,21 @@ static int chain_of(int n_chains, const int32_t* chain_off, int k) {
   return lo;
 }
 
+/* Trajectory::getBetween across two prefixes (GraphUtils.h:43-57): with
+ * root poses a0 (of key a's robot) and b0 (of key b's robot), result =
+ * between(a0, a)^-1 o between(a0, b0) o between(b0, b).  The records are
+ * PoseWithCovariance (pose, covariance, rotation_info). */
+void orc_cross_between(int dim, const double* a0, const double* a, const double* b0, const double* b,
+                       double* out) {
+  double pa[64], pb[64], pab[64], ia[64], t[64];
+  orc_pwc_between(dim, a0, a, pa);
+  orc_pwc_between(dim, b0, b, pb);
+  orc_pwc_between(dim, a0, b0, pab);
+  orc_pwc_inverse(dim, pa, ia);
+  orc_pwc_compose(dim, ia, pab, t);
+  orc_pwc_compose(dim, t, pb, out);
+}
+
 /* Fold of every chain: chain c covers poses [chain_off[c], chain_off[c+1]),
  * starts at pose0s[c] with zero covariance (Pcm.h:534-542) and uses odometry
  * rows [chain_off[c] - c, chain_off[c+1] - c - 1). */

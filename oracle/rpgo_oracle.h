@@ -144,6 +144,8 @@ double orc_simple_landmark_pair(int dim, const double* traj, int i, const double
 void orc_pcm_adjacency_landmark(int dim, const double* traj, int n_lc, const int32_t* lc_from,
                                 const double* lc_pose, const double* lc_cov, int simple, double lc_thr,
                                 double lc_thr2, uint8_t* adj, double* dist);
+void orc_cross_between(int dim, const double* a0, const double* a, const double* b0, const double* b,
+                       double* out);
 int orc_pcm_adjacency_chains(int dim, int n_chains, const int32_t* chain_off, const double* traj,
                              int n_lc, const int32_t* lc_from, const int32_t* lc_to,
                              const double* lc_pose, const double* lc_cov, int simple,

@@ -113,3 +113,18 @@ def test_inter_robot_pair_is_swapped_single_chain_check():
     _, _, _, dist = _group(traj, chain_off, [(B(1), A(1), z1), (A(4), B(4), z2)], -1.0, 1e9)
     expect = O.pair_dist(3, traj, B(1), A(1), z1, cov6(0.1), B(4), A(4), z2, cov6(0.1))
     assert dist[1, 0] == expect
+
+
+def test_cross_robot_between_kat():
+    """testTrajectory.cpp:112-162: getBetween(a1, b1) across two robots via the
+    root poses a0, b0 is the pose (Rz(90), (2, -1, 0))."""
+    cov0 = np.zeros(36)
+    a0 = O.pwc_raw(3, pose3(I3, (1, 1, 1)), cov0)
+    b0 = O.pwc_compose(3, a0, O.pwc_raw(3, pose3(I3, (1, 0, 0)), 0.01 * np.eye(6).reshape(36)))
+    a1 = O.pwc_compose(3, a0, O.pwc_raw(3, pose3(I3, (0, 1, 0)), 0.1 * np.eye(6).reshape(36)))
+    b1 = O.pwc_compose(3, b0, O.pwc_raw(3, pose3(R90, (1, 0, 0)), 0.2 * np.eye(6).reshape(36)))
+    r = O.cross_between(3, a0, a1, b0, b1)
+    np.testing.assert_allclose(r[:12], pose3(R90, (2, -1, 0)), atol=1e-12)
+    # same robot: the root-chain route equals the direct between (pose part)
+    direct = O.pwc_between(3, a1, b1)
+    np.testing.assert_allclose(r[:12], direct[:12], atol=1e-12)
