@@ -844,9 +844,67 @@ __global__ void k_dot(long long n, const double* __restrict__ a, const double* _
 // Each launch clears only slots that no block of the same launch touches.
 // ---------------------------------------------------------------------------
 constexpr int kScR = 0, kScPQ = 2, kScCnt = 32, kScLam = 30;
+constexpr int kSpmvPoseMin = 16384;  // poses from which the SpMV runs one thread per pose
 
-// K1: q = (H + lambda damp) p ; PQ[a] += p . q ; clear R[a^1], PQ[a^1]
-__global__ void __launch_bounds__(128) k_pcg_spmv(int N, int a, const int* __restrict__ off,
+// K1: q = (H + lambda damp) p ; PQ[a] += p . q ; clear R[a^1], PQ[a^1].
+// Eight lanes per pose, lane r < 6 owning output row r: block rows (or
+// columns, for the transposed half) are read as 288 contiguous bytes per
+// group, and each row sums its products in the same order as a per-pose loop.
+__global__ void __launch_bounds__(256) k_pcg_spmv_rows(int N, int a, const int* __restrict__ off,
+                                                 const int* __restrict__ inc, const int* __restrict__ fi,
+                                                 const int* __restrict__ fj, const int* __restrict__ kind,
+                                                 const double* __restrict__ D, const double* __restrict__ Hij,
+                                                 const double* __restrict__ hdiag, int diag_damping,
+                                                 double* __restrict__ sc, const double* __restrict__ x,
+                                                 double* __restrict__ y) {
+  __shared__ double red[16];
+  const int lane = threadIdx.x & 63;
+  const int r = lane & 7, base = lane & ~7;
+  const int rr = r < 6 ? r : 0;  // idle rows 6, 7 shadow row 0 (never stored)
+  const int p = (blockIdx.x * blockDim.x + threadIdx.x) >> 3;
+  const double lambda = sc[kScLam];
+  double pq = 0.0;
+  if (p < N) {
+    const double xr = x[(size_t)p * 6 + rr];
+    double s = 0.0;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) s += D[(size_t)p * 36 + rr * 6 + c] * __shfl(xr, base + c);
+    const double d = diag_damping ? fmin(fmax(hdiag[(size_t)p * 6 + rr], 1e-6), 1e32) : 1.0;
+    double acc = s + lambda * d * xr;
+    for (int k = off[p]; k < off[p + 1]; ++k) {
+      const int t = inc[k], f = t >> 1;
+      if (kind[f] != 0) continue;
+      const double* B = Hij + (size_t)f * 36;
+      const bool tr = (t & 1) != 0;
+      const double xq = x[(size_t)(tr ? fi[f] : fj[f]) * 6 + rr];
+      double v = 0.0;
+      if (!tr) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) v += B[rr * 6 + c] * __shfl(xq, base + c);
+      } else {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) v += B[c * 6 + rr] * __shfl(xq, base + c);
+      }
+      acc += v;
+    }
+    if (r < 6) {
+      y[(size_t)p * 6 + r] = acc;
+      pq = xr * acc;
+    }
+  }
+  pq = block_sum(pq, red);
+  if (threadIdx.x == 0) {
+    atomicAdd(&sc[kScPQ + a], pq);
+    if (blockIdx.x == 0) {
+      sc[kScR + (a ^ 1)] = 0.0;
+      sc[kScPQ + (a ^ 1)] = 0.0;
+    }
+  }
+}
+
+// K1, one thread per pose: for large graphs, where the pose count alone fills
+// the chip and each thread streams its rows with many loads in flight.
+__global__ void __launch_bounds__(128) k_pcg_spmv_pose(int N, int a, const int* __restrict__ off,
                                                  const int* __restrict__ inc, const int* __restrict__ fi,
                                                  const int* __restrict__ fj, const int* __restrict__ kind,
                                                  const double* __restrict__ D, const double* __restrict__ Hij,
@@ -909,70 +967,15 @@ __global__ void __launch_bounds__(128) k_pcg_spmv(int N, int a, const int* __res
   }
 }
 
-// Segment LDL^T solve z = B^-1 r over poses [k0, k1) (FP32 factors and
-// arithmetic, FP64 vectors); returns r . z over the segment.
-__device__ double seg_solve(int k0, int k1, const float* __restrict__ Minv, const float* __restrict__ Useg,
-                            const double* __restrict__ r, double* __restrict__ z) {
-  float y[6], t[6];
-#pragma unroll
-  for (int e = 0; e < 6; ++e) y[e] = (float)r[(size_t)k0 * 6 + e];
-  for (int k = k0; k < k1; ++k) {
-    if (k > k0) {
-      const float* U = Useg + (size_t)(k - 1) * 36;
-#pragma unroll
-      for (int i = 0; i < 6; ++i) {
-        float v = (float)r[(size_t)k * 6 + i];
-#pragma unroll
-        for (int m = 0; m < 6; ++m) v -= U[m * 6 + i] * t[m];
-        y[i] = v;
-      }
-    }
-    const float* Di = Minv + (size_t)k * 36;
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-      float v = 0.f;
-#pragma unroll
-      for (int m = 0; m < 6; ++m) v += Di[i * 6 + m] * y[m];
-      t[i] = v;
-    }
-#pragma unroll
-    for (int i = 0; i < 6; ++i) z[(size_t)k * 6 + i] = t[i];
-  }
-  float zn[6];
-#pragma unroll
-  for (int e = 0; e < 6; ++e) zn[e] = t[e];
-  for (int k = k1 - 2; k >= k0; --k) {
-    const float* U = Useg + (size_t)k * 36;
-    const float* Di = Minv + (size_t)k * 36;
-    float u[6];
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-      float v = 0.f;
-#pragma unroll
-      for (int m = 0; m < 6; ++m) v += U[i * 6 + m] * zn[m];
-      u[i] = v;
-    }
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-      float v = (float)z[(size_t)k * 6 + i];
-#pragma unroll
-      for (int m = 0; m < 6; ++m) v -= Di[i * 6 + m] * u[m];
-      zn[i] = v;
-      z[(size_t)k * 6 + i] = v;
-    }
-  }
-  double acc = 0.0;
-  for (int k = k0; k < k1; ++k)
-#pragma unroll
-    for (int e = 0; e < 6; ++e) acc += r[(size_t)k * 6 + e] * z[(size_t)k * 6 + e];
-  return acc;
-}
-
 // K2: alpha = R[a] / PQ[a]; x += alpha p; r -= alpha q (skipped when first);
 // z = B_fine^-1 r; R[a^1] += r . z; rc += P^T r (coarse segments are whole
-// multiples of the fine segment S); hist[cnt] += |r|^2.  One thread per fine
-// segment.
-__global__ void __launch_bounds__(128) k_pcg_update(int N, int S, int first, int a,
+// multiples of the fine segment S); hist[cnt] += |r|^2.
+// Eight lanes per fine segment, lane i < 6 owning row i of every 6-vector:
+// the block-tridiagonal LDL^T solve is sequential along the segment, so its
+// parallelism is the six rows of each 6x6 product (operands exchanged by
+// shuffles within the group).  One thread per segment left most of the chip
+// idle (1,250 segments at 5k poses).
+__global__ void __launch_bounds__(256) k_pcg_update(int N, int S, int first, int a,
                                                    double* __restrict__ sc, const double* __restrict__ p,
                                                    const double* __restrict__ q, double* __restrict__ x,
                                                    double* __restrict__ r, double* __restrict__ z,
@@ -981,59 +984,89 @@ __global__ void __launch_bounds__(128) k_pcg_update(int N, int S, int first, int
                                                    const float* __restrict__ Pb, double* __restrict__ rc,
                                                    double* __restrict__ hist) {
   __shared__ double red[16];
-  const int seg = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int i = lane & 7;          // row of this lane (6, 7: idle rows)
+  const int base = lane & ~7;      // first lane of the group
+  const int seg = (blockIdx.x * blockDim.x + threadIdx.x) >> 3;
   const int k0 = seg * S;
+  const bool row = i < 6;
+  const int ir = row ? i : 0;      // idle rows shadow row 0 (never stored)
   double rr = 0.0, rz = 0.0;
   int cs_mine = -1;
-  double racc[6] = {0, 0, 0, 0, 0, 0};
+  double racc = 0.0;
   if (k0 < N) {
     const int k1 = min(k0 + S, N);
     if (!first) {
       const double pq = sc[kScPQ + a];
       const double alpha = pq != 0.0 ? sc[kScR + a] / pq : 0.0;
-      for (int k = k0; k < k1; ++k)
-#pragma unroll
-        for (int e = 0; e < 6; ++e) {
-          const size_t o = (size_t)k * 6 + e;
-          x[o] += alpha * p[o];
-          const double rv = r[o] - alpha * q[o];
+      for (int k = k0; k < k1; ++k) {
+        const size_t o = (size_t)k * 6 + ir;
+        const double xv = x[o] + alpha * p[o];
+        const double rv = r[o] - alpha * q[o];
+        if (row) {
+          x[o] = xv;
           r[o] = rv;
           rr += rv * rv;
         }
-    }
-    rz = seg_solve(k0, k1, Minv, Useg, r, z);
-    if (Pb) {
-      double acc[6] = {0, 0, 0, 0, 0, 0};
-      for (int k = k0; k < k1; ++k) {
-        const float* P = Pb + (size_t)k * 36;
-#pragma unroll
-        for (int i = 0; i < 6; ++i) {
-          double v = 0.0;
-#pragma unroll
-          for (int m = 0; m < 6; ++m) v += (double)P[m * 6 + i] * r[(size_t)k * 6 + m];
-          acc[i] += v;
-        }
       }
-      cs_mine = k0 / Sc;
-#pragma unroll
-      for (int i = 0; i < 6; ++i) racc[i] = acc[i];
     }
+    // forward: y_k = r_k - U_{k-1}^T t_{k-1}, t_k = Minv_k y_k
+    float t = 0.f;
+    for (int k = k0; k < k1; ++k) {
+      float y = (float)r[(size_t)k * 6 + ir];
+      if (k > k0) {
+        const float* U = Useg + (size_t)(k - 1) * 36;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) y -= U[m * 6 + ir] * __shfl(t, base + m);
+      }
+      const float* Di = Minv + (size_t)k * 36;
+      float v = 0.f;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) v += Di[ir * 6 + m] * __shfl(y, base + m);
+      t = v;
+      if (row) z[(size_t)k * 6 + i] = t;
+    }
+    // backward: z_k -= Minv_k U_k z_{k+1}
+    float zn = t;
+    for (int k = k1 - 2; k >= k0; --k) {
+      const float* U = Useg + (size_t)k * 36;
+      const float* Di = Minv + (size_t)k * 36;
+      float u = 0.f;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) u += U[ir * 6 + m] * __shfl(zn, base + m);
+      float v = (float)z[(size_t)k * 6 + ir];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) v -= Di[ir * 6 + m] * __shfl(u, base + m);
+      zn = v;
+      if (row) z[(size_t)k * 6 + i] = v;
+    }
+    for (int k = k0; k < k1; ++k) {
+      const double rv = r[(size_t)k * 6 + ir];
+      if (row) rz += rv * z[(size_t)k * 6 + i];
+      if (Pb) {
+        // coarse restriction, row i: sum_m P[m][i] r_m
+        const float* P = Pb + (size_t)k * 36;
+        double v = 0.0;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) v += (double)P[m * 6 + ir] * __shfl(rv, base + m);
+        racc += v;
+      }
+    }
+    if (Pb && row) cs_mine = k0 / Sc;
   }
   if (Pb) {
-    // coarse segments span many fine segments: a wavefront whose lanes share
-    // one coarse segment adds its sum once; lanes at a coarse boundary add alone
-    const int cs0 = __shfl(cs_mine, 0), cs1 = __shfl(cs_mine, 63);
-    const bool uniform = (cs0 == cs1) && cs0 >= 0 &&
-                         __shfl(k0 < N ? 1 : 0, 63) != 0;
+    // a wavefront whose segments share one coarse segment sums each row over
+    // its groups and adds once; otherwise every row adds its own
+    const int cs0 = __shfl(cs_mine, 0), cs1 = __shfl(cs_mine, 56);
+    const bool uniform = cs0 == cs1 && cs0 >= 0;
     if (uniform) {
-#pragma unroll
-      for (int i = 0; i < 6; ++i) {
-        const double v = wave_sum_pre(racc[i]);
-        if ((threadIdx.x & 63) == 0) atomicAdd(&rc[6 * cs0 + i], v);
-      }
+      double v = row ? racc : 0.0;
+      v += __shfl_xor(v, 8);
+      v += __shfl_xor(v, 16);
+      v += __shfl_xor(v, 32);
+      if (lane < 6) atomicAdd(&rc[6 * cs0 + lane], v);
     } else if (cs_mine >= 0) {
-#pragma unroll
-      for (int i = 0; i < 6; ++i) atomicAdd(&rc[6 * cs_mine + i], racc[i]);
+      atomicAdd(&rc[6 * cs_mine + i], racc);
     }
   }
   rr = block_sum(rr, red);
@@ -1070,39 +1103,33 @@ __global__ void __launch_bounds__(256) k_pcg_coarse(int nc, int a, const double*
 }
 
 // K4: beta = R[a^1] / R[a] (0 when first); z += P cc[a]; p = z + beta p;
-// clear rc and cc[a^1]; advance the iteration counter.
-__global__ void __launch_bounds__(128) k_pcg_direction(int N, int first, int a, double* __restrict__ sc,
+// clear rc and cc[a^1]; advance the iteration counter.  One thread per
+// vector entry (pose k = e / 6, row e % 6).
+__global__ void __launch_bounds__(256) k_pcg_direction(int N, int first, int a, double* __restrict__ sc,
                                                       double* __restrict__ z, double* __restrict__ p,
                                                       int Sc, const float* __restrict__ Pb,
                                                       const double* __restrict__ cc_a,
                                                       double* __restrict__ cc_other,
                                                       double* __restrict__ rc, int nc) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const double rz = sc[kScR + a];
   const double beta = (!first && rz != 0.0) ? sc[kScR + (a ^ 1)] / rz : 0.0;
-  if (k < N) {
-    double c[6] = {0, 0, 0, 0, 0, 0};
-    const float* P = Pb ? Pb + (size_t)k * 36 : nullptr;
-    if (P) {
-      const int s = k / Sc;
+  if (e < 6ll * N) {
+    const int k = (int)(e / 6), i = (int)(e % 6);
+    double v = z[e];
+    if (Pb) {
+      const float* P = Pb + (size_t)k * 36 + i * 6;
+      const double* c = cc_a + 6 * (k / Sc);
 #pragma unroll
-      for (int m = 0; m < 6; ++m) c[m] = cc_a[6 * s + m];
+      for (int m = 0; m < 6; ++m) v += (double)P[m] * c[m];
     }
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-      double v = z[(size_t)k * 6 + i];
-      if (P) {
-#pragma unroll
-        for (int m = 0; m < 6; ++m) v += (double)P[i * 6 + m] * c[m];
-      }
-      p[(size_t)k * 6 + i] = first ? v : v + beta * p[(size_t)k * 6 + i];
-    }
+    p[e] = first ? v : v + beta * p[e];
   }
-  if (k < nc) {
-    rc[k] = 0.0;
-    cc_other[k] = 0.0;
+  if (e < nc) {
+    rc[e] = 0.0;
+    cc_other[e] = 0.0;
   }
-  if (k == 0 && !first) sc[kScCnt] += 1.0;
+  if (e == 0 && !first) sc[kScCnt] += 1.0;
 }
 
 __global__ void k_copy_d(long long n, const double* __restrict__ a, double* __restrict__ b) {
@@ -1374,19 +1401,27 @@ void SolverRun::pcg_step(hipStream_t st, int S, int Sc, int nseg_c, int first, i
   const float* P = nseg_c ? Pb.as<float>() : nullptr;
   double* cc0 = ccor.as<double>();
   double* cc1 = cc0 + kCoarseMax;
-  if (!first)
-    k_pcg_spmv<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(
+  // both kernels sum every row in the same order (q is the same to the bit);
+  // rows in parallel below kSpmvPoseMin poses, one thread per pose above
+  // (5k poses: 46 vs 49 us per PCG iteration; 50k: 78 vs 69 us)
+  if (!first && N < kSpmvPoseMin)
+    k_pcg_spmv_rows<<<grid_for((long long)N * 8, 256, 1 << 22), 256, 0, st>>>(
         N, a, inc_off.as<int>(), inc.as<int>(), fi.as<int>(), fj.as<int>(), kind.as<int>(),
         D.as<double>(), Hij.as<double>(), hdiag.as<double>(), prm.lm_diagonal_damping, s, p.as<double>(),
         q.as<double>());
-  k_pcg_update<<<grid_for(nseg, 128, 1 << 22), 128, 0, st>>>(
+  else if (!first)
+    k_pcg_spmv_pose<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(
+        N, a, inc_off.as<int>(), inc.as<int>(), fi.as<int>(), fj.as<int>(), kind.as<int>(),
+        D.as<double>(), Hij.as<double>(), hdiag.as<double>(), prm.lm_diagonal_damping, s, p.as<double>(),
+        q.as<double>());
+  k_pcg_update<<<grid_for((long long)nseg * 8, 256, 1 << 22), 256, 0, st>>>(
       N, S, first, a, s, p.as<double>(), q.as<double>(), dx.as<double>(), r.as<double>(), zv.as<double>(),
       Minv.as<float>(), Useg.as<float>(), Sc, P, cres.as<double>(), hist.as<double>());
   if (nseg_c) {
     const dim3 grid((nc + 63) / 64, (nc + 63) / 64);
     k_pcg_coarse<<<grid, 256, 0, st>>>(nc, a, Ainv.as<double>(), cres.as<double>(), a ? cc1 : cc0, s);
   }
-  k_pcg_direction<<<grid_for(std::max(N, nc), 128, 1 << 22), 128, 0, st>>>(
+  k_pcg_direction<<<grid_for(std::max(6ll * N, (long long)nc), 256, 1 << 22), 256, 0, st>>>(
       N, first, a, s, zv.as<double>(), p.as<double>(), Sc, P, a ? cc1 : cc0, a ? cc0 : cc1,
       cres.as<double>(), nc);
 }

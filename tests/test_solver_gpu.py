@@ -40,6 +40,21 @@ def test_lm_matches_oracle_least_squares(ctx):
     assert abs(rep.final_cost - err) <= 1e-6 * err
 
 
+def test_lm_matches_oracle_20k(ctx):
+    """Above 16,384 poses the PCG SpMV runs one thread per pose instead of one
+    lane per row: the same sums, checked at the same bar."""
+    gr = synth.se3_helix(n_poses=20000, n_lc=2000, outlier_frac=0.0, seed=6)
+    g, nk = from_synth(gr)
+    print(f"[solve] LM N={g.N} F={g.F}", flush=True)
+    out, w, rep = _gpu(ctx, g, polish_iterations=10, pcg_rel_tol=1e-6)
+    R, t, err, it = S.least_squares(g, np.ones(g.F))
+    dt, dr = pose_error(out, R, t)
+    print(f"[solve] lm it={rep.lm_iterations} pcg={rep.pcg_iterations} cost={rep.final_cost:.6f} "
+          f"oracle={err:.6f} dt={dt:.2e} dr={dr:.2e} ms={rep.ms_total:.1f}", flush=True)
+    assert dt < POSE_TOL and dr < POSE_TOL
+    assert abs(rep.final_cost - err) <= 1e-6 * err
+
+
 def test_gn_matches_oracle(ctx):
     gr = synth.se3_helix(n_poses=300, n_lc=40, outlier_frac=0.0, seed=5)
     g, nk = from_synth(gr)
