@@ -39,6 +39,7 @@ struct BatchState {
   DevBuf pose0, odom, odom_cov, lc_from, lc_to, lc_pose, lc_cov;
   DevBuf delta, hinv, traj, lcm, odom_dist, flags, accepted, m, c_from, c_to, c_lcm, adj, counters;
   DevBuf deg, cidx, ck;  // clique stage: degrees, members, sizes
+  DevBuf cb_state, cb_icc, cb_dm;  // clique stage: chip-wide rounds
   std::vector<int> m_host;
   // launch shapes chosen by the autotuner, per dim (0: not tuned yet)
   int tuned_pair[4] = {0, 0, 0, 0}, tuned_clique[4] = {0, 0, 0, 0}, tuned_prefetch[4] = {0, 0, 0, 0};
@@ -299,7 +300,7 @@ void batch_destroy(rpgo_ctx* ctx) {
   for (DevBuf* b : {&s.d_pose_off, &s.d_lc_off, &s.d_adj_off, &s.d_tile_off, &s.d_sym_off, &s.d_lc_graph, &s.pose0,
                     &s.odom, &s.odom_cov, &s.lc_from, &s.lc_to, &s.lc_pose, &s.lc_cov, &s.delta, &s.hinv, &s.traj,
                     &s.lcm, &s.odom_dist, &s.flags, &s.accepted, &s.m, &s.c_from, &s.c_to, &s.c_lcm, &s.adj,
-                    &s.counters, &s.deg, &s.cidx, &s.ck})
+                    &s.counters, &s.deg, &s.cidx, &s.ck, &s.cb_state, &s.cb_icc, &s.cb_dm})
     b->release();
   if (s.exec) (void)hipGraphExecDestroy(s.exec);
   if (s.graph) (void)hipGraphDestroy(s.graph);
@@ -361,6 +362,18 @@ static int launch_pairs(rpgo_ctx* ctx, BatchState& s, const rpgo_pcm_params* prm
 // Every stage of the batch, in stream order, with no host synchronisation
 // and no memset/memcpy node (capturable).  events: record ev[0..3], ev[5]
 // between the stages for per-stage times.
+static CliqueSpread spread_of(BatchState& s) {
+  CliqueSpread sp;
+  sp.n_slots = s.n_lc;
+  sp.max_slots = 0;
+  for (int g = 0; g < s.n_graphs; ++g) sp.max_slots = std::max(sp.max_slots, s.lc_off[g + 1] - s.lc_off[g]);
+  sp.slot_graph = s.d_lc_graph.as<int>();
+  sp.state = s.cb_state.ptr;
+  sp.icc = s.cb_icc.as<int>();
+  sp.dm = s.cb_dm.as<unsigned long long>();
+  return sp;
+}
+
 static int launch_stages(rpgo_ctx* ctx, BatchState& s, const rpgo_pcm_params* prm, int pw, int cw, int pf,
                          bool events) {
   hipStream_t st = ctx->stream;
@@ -420,9 +433,10 @@ static int launch_stages(rpgo_ctx* ctx, BatchState& s, const rpgo_pcm_params* pr
   if ((rc = launch_pairs(ctx, s, prm, pw))) return rc;
   if (events) RPGO_HIP_CHECK(hipEventRecord(ev[3], st));
   if (cw) {
+    const CliqueSpread sp = spread_of(s);
     if ((rc = clique_batch_launch(ctx, G, cw, pf, s.d_adj_off.as<long long>(), s.d_lc_off.as<long long>(),
                                   s.m.as<int>(), s.adj.as<unsigned long long>(), s.deg.as<int>(), s.cidx.as<int>(),
-                                  s.ck.as<int>())))
+                                  s.ck.as<int>(), &sp)))
       return rc;
   }
   if (events) RPGO_HIP_CHECK(hipEventRecord(ev[5], st));
@@ -472,9 +486,10 @@ static int autotune(rpgo_ctx* ctx, BatchState& s, const rpgo_pcm_params* prm, bo
     auto clique_time = [&](int cw, int pf, float* t) {
       return timed(
           [&] {
+            const CliqueSpread sp = spread_of(s);
             return clique_batch_launch(ctx, s.n_graphs, cw, pf, s.d_adj_off.as<long long>(),
                                        s.d_lc_off.as<long long>(), s.m.as<int>(), s.adj.as<unsigned long long>(),
-                                       s.deg.as<int>(), s.cidx.as<int>(), s.ck.as<int>());
+                                       s.deg.as<int>(), s.cidx.as<int>(), s.ck.as<int>(), &sp);
           },
           t, 1);
     };
@@ -593,7 +608,9 @@ int rpgo_pcm_batch_run(rpgo_ctx* ctx, const rpgo_pcm_batch* B, const rpgo_pcm_pa
         return RPGO_ERR_ARG;
       }
     if ((rc = s.deg.reserve(sizeof(int) * nl1)) || (rc = s.cidx.reserve(sizeof(int) * nl1)) ||
-        (rc = s.ck.reserve(sizeof(int) * (size_t)G)))
+        (rc = s.ck.reserve(sizeof(int) * (size_t)G)) || (rc = s.cb_state.reserve(clique_spread_state_bytes(G))) ||
+        (rc = s.cb_icc.reserve(clique_spread_icc_bytes(G))) ||
+        (rc = s.cb_dm.reserve(sizeof(unsigned long long) * nl1)))
       return rc;
   }
   float ms_tune = 0.f;
@@ -615,7 +632,7 @@ int rpgo_pcm_batch_run(rpgo_ctx* ctx, const rpgo_pcm_batch* B, const rpgo_pcm_pa
     for (DevBuf* b : {&s.d_pose_off, &s.d_lc_off, &s.d_adj_off, &s.d_tile_off, &s.d_sym_off, &s.d_lc_graph,
                       &s.pose0, &s.odom, &s.odom_cov, &s.lc_from, &s.lc_to, &s.lc_pose, &s.lc_cov, &s.delta,
                       &s.hinv, &s.traj, &s.lcm, &s.odom_dist, &s.flags, &s.accepted, &s.m, &s.c_from, &s.c_to,
-                      &s.c_lcm, &s.adj, &s.counters, &s.deg, &s.cidx, &s.ck})
+                      &s.c_lcm, &s.adj, &s.counters, &s.deg, &s.cidx, &s.ck, &s.cb_state, &s.cb_icc, &s.cb_dm})
       key.push_back((long long)(uintptr_t)b->ptr);
     if (s.exec && key == s.graph_key) {
       replayed = 1;

@@ -291,13 +291,126 @@ __global__ void __launch_bounds__(64) k_replay(int n, int W, int c, int best,
 // replayed by wavefront 0.  Graph g's adjacency starts at adj_off[g] with m[g]
 // rows of ceil(m[g]/64) words; its degrees and clique go to lc_off[g].
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Chip-wide rounds of the batched scan.  One workgroup per graph leaves the
+// largest graphs' scans on one CU each.  A round instead evaluates, for every
+// graph still scanning, a window of kCbWindow candidates from its position
+// with its current best, one wavefront each over the whole chip.  The first
+// candidate of the window whose greedy size beats best is the scan's next
+// winner: every candidate before it saw the same best in the sequential scan
+// and did not beat it.  The position moves past the winner (or past the
+// window), and candidates after a winner are evaluated again next round with
+// the new best.
+// ---------------------------------------------------------------------------
+constexpr int kCbWindow = 1024;  // candidates per graph per round
+constexpr int kCbExtraRounds = 8;  // rounds for winners, beyond one per window of the largest
+                                   // graph; the per-graph scan finishes what is left
+
+struct CbState {
+  int best, pos, win, win_best;
+};
+
+__global__ void k_cb_init(int n_graphs, CbState* __restrict__ st) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < n_graphs) st[g] = CbState{-1, 0, -1, -1};
+}
+
+// deg[v] of every vertex of every graph (vertex slots indexed like the loop
+// closures; slots beyond a graph's accepted count are left alone)
+__global__ void k_cb_degrees(long long n_slots, const int* __restrict__ slot_graph,
+                             const long long* __restrict__ adj_off, const long long* __restrict__ lc_off,
+                             const int* __restrict__ m_arr, const unsigned long long* __restrict__ adj_all,
+                             int* __restrict__ deg_all) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_slots) return;
+  const int g = slot_graph[t];
+  const int v = (int)(t - lc_off[g]), n = m_arr[g];
+  if (v >= n) return;
+  const int W = (n + 63) / 64;
+  const unsigned long long* A = adj_all + adj_off[g];
+  int d = 0;
+  for (int w = 0; w < W; ++w) {
+    unsigned long long b = A[(size_t)v * W + w];
+    if (w == W - 1 && (n & 63)) b &= (1ull << (n & 63)) - 1ull;
+    if (w == (v >> 6)) b &= ~(1ull << (v & 63));
+    d += __popcll(b);
+  }
+  deg_all[t] = d;
+}
+
+// degree filter deg >= best of every graph still scanning; word w of graph g
+// at dm[lc_off[g] + w]
+__global__ void k_cb_mask(long long n_slots, const int* __restrict__ slot_graph,
+                          const long long* __restrict__ lc_off, const int* __restrict__ m_arr,
+                          const int* __restrict__ deg_all, const CbState* __restrict__ st,
+                          unsigned long long* __restrict__ dm) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_slots) return;
+  const int g = slot_graph[t];
+  const int w = (int)(t - lc_off[g]), n = m_arr[g];
+  if (w >= (n + 63) / 64 || st[g].pos >= n) return;
+  const int best = st[g].best;
+  const int* deg = deg_all + lc_off[g];
+  unsigned long long b = 0ull;
+  for (int k = 0; k < 64; ++k) {
+    const int u = w * 64 + k;
+    if (u < n && deg[u] >= best) b |= 1ull << k;
+  }
+  dm[t] = b;
+}
+
+// one wavefront per (graph, window slot): greedy size of candidate pos + k
+__global__ void __launch_bounds__(256) k_cb_icc(int n_graphs, const long long* __restrict__ adj_off,
+                                                const long long* __restrict__ lc_off,
+                                                const int* __restrict__ m_arr,
+                                                const unsigned long long* __restrict__ adj_all,
+                                                const int* __restrict__ deg_all,
+                                                const unsigned long long* __restrict__ dm,
+                                                const CbState* __restrict__ st, int* __restrict__ icc_out) {
+  const long long t = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int g = (int)(t / kCbWindow), k = (int)(t % kCbWindow);
+  if (g >= n_graphs) return;
+  const int n = m_arr[g], pos = st[g].pos, best = st[g].best;
+  const int c = pos + k;
+  if (pos >= n || c >= n) return;  // the select step reads only slots below n - pos
+  const int lane = threadIdx.x & 63;
+  int icc = INT_MIN;
+  if (!(best > deg_all[lc_off[g] + c]))  // pruning 1
+    icc = icc_wave<1>((n + 63) / 64, c, adj_all + adj_off[g], dm + lc_off[g], lane, best);
+  if (lane == 0) icc_out[t] = icc;
+}
+
+// one thread per graph: the window's first winner, or the next window
+__global__ void k_cb_select(int n_graphs, const int* __restrict__ m_arr, const int* __restrict__ icc,
+                            CbState* __restrict__ st) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_graphs) return;
+  CbState s = st[g];
+  const int n = m_arr[g];
+  if (s.pos >= n) return;
+  const int span = min(kCbWindow, n - s.pos);
+  const int* row = icc + (size_t)g * kCbWindow;
+  int k = 0;
+  while (k < span && !(row[k] > s.best)) ++k;
+  if (k < span) {
+    s.win = s.pos + k;
+    s.win_best = s.best;
+    s.best = row[k];
+    s.pos = s.pos + k + 1;
+  } else {
+    s.pos += span;
+  }
+  st[g] = s;
+}
+
 template <int NW, int PF>
 __global__ void __launch_bounds__(64 * NW) k_clique_batch(int n_graphs, const long long* __restrict__ adj_off,
                                                           const long long* __restrict__ lc_off,
                                                           const int* __restrict__ m_arr,
                                                           const unsigned long long* __restrict__ adj_all,
                                                           int* __restrict__ deg_all, int* __restrict__ out_all,
-                                                          int* __restrict__ k_out) {
+                                                          int* __restrict__ k_out,
+                                                          const CbState* __restrict__ start) {
   __shared__ unsigned long long dm[64 * kMaxWPL];
   __shared__ int s_icc[NW];
   __shared__ int s_win, s_win_icc;
@@ -328,7 +441,14 @@ __global__ void __launch_bounds__(64 * NW) k_clique_batch(int n_graphs, const lo
     deg[v] = d;
   }
   __syncthreads();
+  // the scan so far (chip-wide rounds), or its start
   int best = -1, pos = 0, win = -1, win_best = -1;
+  if (start) {
+    best = start[g].best;
+    pos = start[g].pos;
+    win = start[g].win;
+    win_best = start[g].win_best;
+  }
   for (;;) {
     for (int w = tid; w < W; w += 64 * NW) {
       unsigned long long b = 0ull;
@@ -390,16 +510,38 @@ __global__ void __launch_bounds__(64 * NW) k_clique_batch(int n_graphs, const lo
   }
 }
 
+size_t clique_spread_state_bytes(int n_graphs) { return sizeof(CbState) * (size_t)(n_graphs > 0 ? n_graphs : 1); }
+size_t clique_spread_icc_bytes(int n_graphs) {
+  return sizeof(int) * (size_t)kCbWindow * (size_t)(n_graphs > 0 ? n_graphs : 1);
+}
+
 int clique_batch_launch(rpgo_ctx* ctx, int n_graphs, int waves, int prefetch, const long long* d_adj_off,
                         const long long* d_lc_off, const int* d_m, const unsigned long long* adj, int* deg,
-                        int* out_idx, int* out_k) {
+                        int* out_idx, int* out_k, const CliqueSpread* spread) {
   hipStream_t st = ctx->stream;
   if ((waves != 4 && waves != 8 && waves != 16) || (prefetch != 1 && prefetch != kPrefetch)) {
     set_error("clique_batch_launch: unsupported shape (%d wavefronts, prefetch %d)", waves, prefetch);
     return RPGO_ERR_ARG;
   }
-#define RPGO_CB(NW, PF) \
-  k_clique_batch<NW, PF><<<n_graphs, 64 * NW, 0, st>>>(n_graphs, d_adj_off, d_lc_off, d_m, adj, deg, out_idx, out_k)
+  CbState* cs = nullptr;
+  if (spread && spread->n_slots > 0) {
+    cs = static_cast<CbState*>(spread->state);
+    const long long ns = spread->n_slots;
+    const unsigned sg = (unsigned)((ns + 255) / 256);
+    k_cb_init<<<(n_graphs + 255) / 256, 256, 0, st>>>(n_graphs, cs);
+    k_cb_degrees<<<sg, 256, 0, st>>>(ns, spread->slot_graph, d_adj_off, d_lc_off, d_m, adj, deg);
+    const unsigned gi = (unsigned)(((long long)n_graphs * kCbWindow + 3) / 4);
+    const int rounds = (int)((spread->max_slots + kCbWindow - 1) / kCbWindow) + kCbExtraRounds;
+    for (int r = 0; r < rounds; ++r) {
+      k_cb_mask<<<sg, 256, 0, st>>>(ns, spread->slot_graph, d_lc_off, d_m, deg, cs, spread->dm);
+      k_cb_icc<<<gi, 256, 0, st>>>(n_graphs, d_adj_off, d_lc_off, d_m, adj, deg, spread->dm, cs, spread->icc);
+      k_cb_select<<<(n_graphs + 255) / 256, 256, 0, st>>>(n_graphs, d_m, spread->icc, cs);
+    }
+    RPGO_HIP_CHECK(hipGetLastError());
+  }
+#define RPGO_CB(NW, PF)                                                                                       \
+  k_clique_batch<NW, PF><<<n_graphs, 64 * NW, 0, st>>>(n_graphs, d_adj_off, d_lc_off, d_m, adj, deg, out_idx, \
+                                                       out_k, cs)
   if (prefetch == 1) {
     if (waves == 4) RPGO_CB(4, 1);
     else if (waves == 8) RPGO_CB(8, 1);

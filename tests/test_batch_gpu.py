@@ -52,6 +52,21 @@ def test_batch_se3_matches_single(ctx, odom_thr, lc_thr):
           f"{st.ms_pairs:.2f} ms pairs, {st.ms_total:.2f} ms total", flush=True)
 
 
+def test_batch_large_graphs_match_single(ctx):
+    """Graphs of several thousand loop closures: the clique scan runs over
+    several chip-wide rounds (1,024 candidates per graph each) and, for the
+    largest, beyond them in the per-graph scan."""
+    graphs = [synth.se3_helix(n_poses=n, n_lc=l, outlier_frac=f, seed=s)
+              for n, l, f, s in ((9000, 2600, 0.3, 1), (400, 90, 0.2, 2), (40000, 12000, 0.4, 3), (3000, 700, 0.1, 4))]
+    res = ctx.pcm_batch_run(graphs, -1.0, 5.0)
+    for g, gr in enumerate(graphs):
+        acc, adj, k, idx = _single(ctx, gr, -1.0, 5.0)
+        print(f"[batch] large graph {g}: {len(acc)} loop closures, clique {k}", flush=True)
+        assert np.array_equal(res["accepted"][g], acc), g
+        assert int(res["clique_size"][g]) == k
+        assert np.array_equal(res["clique_idx"][g], idx[:max(k, 0)])
+
+
 def test_batch_se2_matches_single(ctx):
     rng = np.random.default_rng(3)
     graphs = [synth.se2_random_walk(n_poses=int(rng.integers(100, 600)), n_inlier=int(rng.integers(10, 60)),
