@@ -427,6 +427,11 @@ __global__ void __launch_bounds__(64) k_symmetrize(int m, int W, unsigned long l
   }
 }
 
+// Queues up to this length are replayed one wavefront per pair (about half
+// the latency of a lone lane-per-pair replay, at 1/64 of the throughput);
+// longer ones one lane per pair, by the same kernel.
+constexpr int kRecheckWaveMax = 2048;
+
 // FP64 replay of pairs queued by the FP32 kernel: the reference decision
 // d < thr replaces the FP32 one; both triangle bits are patched.
 // One queued pair (i < j): FP64 distance, patch both bits when the FP32
@@ -504,6 +509,219 @@ __global__ void __launch_bounds__(64) k_recheck64_dev(const unsigned int* __rest
   const int n = (int)nq;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
     recheck_one<DIM>(queue[t], traj, cf, ct, clcm, thr, m, W, adj, dist, flips, n_consistent, ch);
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront-parallel FP64 replay of one queued SE(3) pair.  The sequential
+// replay is one long dependent chain per lane (about 40 us for a lone pair,
+// which sets the stage's length at 5k poses).  Here one wavefront takes one
+// pair: the 12 pose entries, the 36 adjoint entries and each 6x6 product
+// entry go to their own lanes, each summed in the same order as mm64 /
+// mmt64 / compose_entry, so every value is the sequential one to the bit;
+// the LLT tests and the Mahalanobis norm (logmap, LU inverse) stay
+// sequential on lane 0 with the R64<3> functions themselves.  Workgroups are
+// one wavefront, so __syncthreads() orders the LDS exchanges.
+// ---------------------------------------------------------------------------
+namespace wp {
+
+constexpr int PL = 12, D = 6, PWC = R64<3>::PWC;
+
+struct Scratch {
+  double Pa[PWC], Pb[PWC], Pc[PWC], Pd[PWC], Zab[PWC], Zcd[PWC];
+  double A[PWC], B[PWC], X[PWC], Y[PWC], W[PWC], L[PWC];
+  double t0[PL], t1[PL], t2[PL], H[D * D], T[D * D], HSH[D * D], C[D * D];
+  int ok;
+};
+
+// entry e < 12 of a^-1 (R64<3>::pose_inverse)
+__device__ __forceinline__ double inverse_entry(const double* a, int e) {
+  if (e < 9) return a[(e % 3) * 3 + e / 3];
+  const int i = e - 9;
+  double s = a[0 * 3 + i] * (-a[9]);
+  s += a[1 * 3 + i] * (-a[10]);
+  s += a[2 * 3 + i] * (-a[11]);
+  return s;
+}
+
+// entry (i, j) of Ad(g) (R64<3>::adjoint)
+__device__ __forceinline__ double adjoint_entry(const double* g, int i, int j) {
+  const double* R = g;
+  const double tx = g[9], ty = g[10], tz = g[11];
+  if (i < 3) return j < 3 ? R[i * 3 + j] : 0.0;
+  if (j >= 3) return R[(i - 3) * 3 + (j - 3)];
+  if (i == 3) return -tz * R[3 + j] + ty * R[6 + j];
+  if (i == 4) return tz * R[j] - tx * R[6 + j];
+  return -ty * R[j] + tx * R[3 + j];
+}
+
+__device__ __forceinline__ void pose_inverse(const double* a, double* o, int lane) {
+  if (lane < PL) o[lane] = inverse_entry(a, lane);
+  __syncthreads();
+}
+
+__device__ __forceinline__ void pose_compose(const double* a, const double* b, double* o, int lane) {
+  if (lane < PL) o[lane] = R64<3>::compose_entry(a, b, lane);
+  __syncthreads();
+}
+
+__device__ __forceinline__ void adjoint(const double* g, double* H, int lane) {
+  if (lane < D * D) H[lane] = adjoint_entry(g, lane / D, lane % D);
+  __syncthreads();
+}
+
+// out = (H S) H^T (congruence64<6>: mm64 then mmt64)
+__device__ __forceinline__ void congruence(const double* H, const double* S, double* T, double* out, int lane) {
+  const int i = lane / D, j = lane % D;
+  if (lane < D * D) {
+    double s = H[i * D] * S[j];
+#pragma unroll
+    for (int k = 1; k < D; ++k) s += H[i * D + k] * S[k * D + j];
+    T[lane] = s;
+  }
+  __syncthreads();
+  if (lane < D * D) {
+    double s = T[i * D] * H[j * D];
+#pragma unroll
+    for (int k = 1; k < D; ++k) s += T[i * D + k] * H[j * D + k];
+    out[lane] = s;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ double flag_and(const double* a, const double* b) {
+  return (a[PL + D * D] != 0.0 && b[PL + D * D] != 0.0) ? 1.0 : 0.0;
+}
+
+// pwc_between with the LLT fallback
+__device__ void between(Scratch& w, const double* a, const double* b, double* o, int lane) {
+  pose_inverse(a, w.t0, lane);            // ainv
+  pose_compose(w.t0, b, w.t1, lane);      // pose
+  pose_inverse(w.t1, w.t2, lane);         // rinv
+  adjoint(w.t2, w.H, lane);
+  congruence(w.H, a + PL, w.T, w.HSH, lane);
+  if (lane < D * D) w.C[lane] = b[PL + lane] - w.HSH[lane];
+  __syncthreads();
+  if (lane == 0) w.ok = llt_ok64<D>(w.C) ? 1 : 0;
+  __syncthreads();
+  if (!w.ok) {
+    pose_inverse(b, w.t0, lane);          // binv
+    pose_compose(w.t0, a, w.t2, lane);    // r2
+    pose_inverse(w.t2, w.t0, lane);       // r2inv
+    adjoint(w.t0, w.H, lane);
+    congruence(w.H, b + PL, w.T, w.HSH, lane);
+    if (lane < D * D) w.C[lane] = a[PL + lane] - w.HSH[lane];
+    __syncthreads();
+  }
+  if (lane < PL) o[lane] = w.t1[lane];
+  if (lane < D * D) o[PL + lane] = w.C[lane];
+  if (lane == 0) o[PL + D * D] = flag_and(a, b);
+  __syncthreads();
+}
+
+// pwc_compose
+__device__ void compose(Scratch& w, const double* a, const double* b, double* o, int lane) {
+  pose_compose(a, b, w.t1, lane);         // pose
+  pose_inverse(b, w.t0, lane);            // binv
+  adjoint(w.t0, w.H, lane);
+  congruence(w.H, a + PL, w.T, w.HSH, lane);
+  if (lane < PL) o[lane] = w.t1[lane];
+  if (lane < D * D) o[PL + lane] = w.HSH[lane] + b[PL + lane];
+  if (lane == 0) o[PL + D * D] = flag_and(a, b);
+  __syncthreads();
+}
+
+// pwc_inverse (covariance kept)
+__device__ void inverse(const double* a, double* o, int lane) {
+  if (lane < PL) o[lane] = inverse_entry(a, lane);
+  if (lane < D * D + 1) o[PL + lane] = a[PL + lane];
+  __syncthreads();
+}
+
+__device__ void load(const double* __restrict__ src, double* dst, int lane) {
+  if (lane < PWC) dst[lane] = src[lane];
+}
+
+}  // namespace wp
+
+// One wavefront per queued SE(3) pair (i < j), grid-stride over the queue:
+// the pair_dist of R64<3> computed lane-parallel, then the same patch of both
+// bits, count and flip as the sequential replay.
+__global__ void __launch_bounds__(64) k_recheck64_wave(const unsigned int* __restrict__ qcount, int cap,
+                                                       const int2* __restrict__ queue,
+                                                       const double* __restrict__ traj,
+                                                       const int* __restrict__ cf, const int* __restrict__ ct,
+                                                       const double* __restrict__ clcm, double thr, int m, int W,
+                                                       unsigned long long* __restrict__ adj,
+                                                       double* __restrict__ dist,
+                                                       unsigned long long* __restrict__ flips,
+                                                       unsigned long long* __restrict__ n_consistent, Chains ch) {
+  using namespace wp;
+  __shared__ Scratch w;
+  __shared__ double dsh;
+  const unsigned int nq = *qcount;
+  if (nq > (unsigned)cap) return;
+  const int lane = threadIdx.x;
+  if (nq > (unsigned)kRecheckWaveMax) {  // long queue: one lane per pair
+    for (int t = blockIdx.x * 64 + lane; t < (int)nq; t += gridDim.x * 64)
+      recheck_one<3>(queue[t], traj, cf, ct, clcm, thr, m, W, adj, dist, flips, n_consistent, ch);
+    return;
+  }
+  for (int t = blockIdx.x; t < (int)nq; t += gridDim.x) {
+    const int2 q = queue[t];
+    const int i = q.x, j = q.y;
+    if (i < 0 || j <= i || j >= m) continue;  // uniform over the wavefront
+    int c = cf[j], dd = ct[j];
+    if (ch.of(cf[i]) != ch.of(c)) {  // Pcm.h:691-698
+      const int tt = c;
+      c = dd;
+      dd = tt;
+    }
+    load(traj + (size_t)cf[i] * PWC, w.Pa, lane);
+    load(traj + (size_t)ct[i] * PWC, w.Pb, lane);
+    load(traj + (size_t)c * PWC, w.Pc, lane);
+    load(traj + (size_t)dd * PWC, w.Pd, lane);
+    load(clcm + (size_t)i * PWC, w.Zab, lane);
+    load(clcm + (size_t)j * PWC, w.Zcd, lane);
+    __syncthreads();
+    between(w, w.Pa, w.Pc, w.A, lane);    // a_odom_c
+    compose(w, w.A, w.Zcd, w.X, lane);    // a_path_d
+    inverse(w.X, w.Y, lane);
+    compose(w, w.Y, w.Zab, w.W, lane);    // d_path_b
+    between(w, w.Pb, w.Pd, w.B, lane);    // b_odom_d
+    compose(w, w.W, w.B, w.L, lane);      // loop
+    if (lane == 0) {
+      double rec[PWC];
+#pragma unroll
+      for (int e = 0; e < PWC; ++e) rec[e] = w.L[e];
+      dsh = R64<3>::mahalanobis(rec);
+    }
+    __syncthreads();
+    const double d = dsh;
+    if (lane == 0) {
+      const bool cons = d < thr;
+      unsigned long long* wj = adj + (size_t)j * W + (i >> 6);
+      unsigned long long* wi = adj + (size_t)i * W + (j >> 6);
+      const unsigned long long bj = 1ull << (i & 63), bi = 1ull << (j & 63);
+      const bool was = ((*wj) & bj) != 0ull;
+      if (cons != was) {
+        if (cons) {
+          atomicOr(wj, bj);
+          atomicOr(wi, bi);
+          atomicAdd(n_consistent, 1ull);
+        } else {
+          atomicAnd(wj, ~bj);
+          atomicAnd(wi, ~bi);
+          atomicAdd(n_consistent, ~0ull);  // -1 modulo 2^64
+        }
+        atomicAdd(flips, 1ull);
+      }
+      if (dist) {
+        dist[(size_t)j * m + i] = d;
+        dist[(size_t)i * m + j] = d;
+      }
+    }
+    __syncthreads();  // the scratch is reused by the next pair
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -641,11 +859,14 @@ int pcm_recheck64_device(rpgo_ctx* ctx, double lc_thr, bool want_dist) {
   // enough wavefronts for the longest queue seen in practice (0.2-0.3 % of
   // the pairs at 50k), grid-stride beyond
   const unsigned blocks = 2048;
-  if (s.dim == 3)
-    k_recheck64_dev<3><<<blocks, 64, 0, ctx->stream>>>(qcount, kRecheckQueueCap, s.queue.as<int2>(),
-                                                       s.traj.as<double>(), s.c_from.as<int>(), s.c_to.as<int>(),
-                                                       s.c_lcm.as<double>(), lc_thr, m, W,
-                                                       s.adj.as<unsigned long long>(), dist, cnt + 2, cnt, chains_of(s));
+  if (s.dim == 3) {
+    // the queue length picks on the device: a wavefront or a lane per pair
+    k_recheck64_wave<<<blocks, 64, 0, ctx->stream>>>(qcount, kRecheckQueueCap, s.queue.as<int2>(),
+                                                     s.traj.as<double>(), s.c_from.as<int>(), s.c_to.as<int>(),
+                                                     s.c_lcm.as<double>(), lc_thr, m, W,
+                                                     s.adj.as<unsigned long long>(), dist, cnt + 2, cnt,
+                                                     chains_of(s));
+  }
   else
     k_recheck64_dev<2><<<blocks, 64, 0, ctx->stream>>>(qcount, kRecheckQueueCap, s.queue.as<int2>(),
                                                        s.traj.as<double>(), s.c_from.as<int>(), s.c_to.as<int>(),

@@ -47,6 +47,9 @@ def _fp32_vs_oracle(ctx, g, odom_thr, lc_thr, check_dist=True):
         print(f"[fp32] pairs={st.n_pairs} rechecked={st.n_rechecked} flipped={st.n_flipped} "
               f"max_rel_err={rel:.3e}", flush=True)
         assert rel < 1e-3
+        # replayed pairs carry the FP64 replay's distance: the oracle's up to
+        # the last ulp of the transcendentals (FP32 values are ~1e-6 off)
+        assert int(np.sum(np.abs(d32 - d64) <= 1e-12 * np.abs(d64))) >= st.n_rechecked
     return st, rel
 
 
@@ -54,6 +57,25 @@ def test_fp32_se3_5k(ctx):
     g, ot, lt = synth.workload("se3_5k", seed=0)
     st, rel = _fp32_vs_oracle(ctx, g, ot, lt)
     assert st.n_rechecked < st.n_pairs // 20
+
+
+@pytest.mark.parametrize("n_poses,n_lc", [(5000, 1000), (20000, 3000)])
+def test_fp32_replay_is_the_fp64_path_bitwise(ctx, n_poses, n_lc):
+    """Pairs replayed after the FP32 pass (one wavefront per pair for short
+    queues, one lane per pair for long ones) carry the FP64 path's distance
+    bit for bit."""
+    g = synth.se3_helix(n_poses=n_poses, n_lc=n_lc, outlier_frac=0.3, seed=3)
+    ctx.pcm_upload(g.dim, g.pose0, g.odom, g.odom_cov, g.lc_from, g.lc_to, g.lc_pose, g.lc_cov)
+    ctx.pcm_run(-1.0, 5.0, precision=_capi.PCM_FP64_REPLAY, want_dist=True)
+    ref = ctx.pcm_download(want_dist=True)
+    st = ctx.pcm_pairs(5.0, precision=FP32, want_dist=True)
+    got = ctx.pcm_download(want_dist=True)
+    print(f"[fp32] replay vs FP64 path: N={n_poses} rechecked={st.n_rechecked}", flush=True)
+    assert np.array_equal(got["adj_bits"], ref["adj_bits"])
+    m = len(ref["accepted"])
+    iu = np.triu_indices(m, 1)
+    exact = int(np.sum(got["dist"][iu] == ref["dist"][iu]))
+    assert st.n_rechecked > 0 and exact >= st.n_rechecked
 
 
 def test_fp32_several_seeds_and_thresholds(ctx):
