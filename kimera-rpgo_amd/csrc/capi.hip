@@ -54,6 +54,9 @@ int rpgo_ctx_create(int device, uint32_t flags, rpgo_ctx** out) {
   RPGO_HIP_CHECK(hipSetDevice(device));
   rpgo_ctx* c = new rpgo_ctx();
   c->device = device;
+  if (hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      c->n_cu <= 0)
+    c->n_cu = 256;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     set_error("hipStreamCreate failed: %s", hipGetErrorString(e));
@@ -218,9 +221,10 @@ static int pairs_pass(rpgo_ctx* ctx, const rpgo_pcm_params* prm, rpgo_pcm_stats*
   const bool fp32 = !simple && prm->precision == RPGO_PCM_FP32_RECHECK && !s.fp32_fallback;
   int64_t n_recheck = 0;
   RPGO_HIP_CHECK(hipEventRecord(ev[8], st));
-  RPGO_HIP_CHECK(hipEventRecord(ev[9], st));
-  RPGO_HIP_CHECK(hipEventRecord(ev[10], st));
-  if (check && m > 1) {
+  if (!(check && m > 1)) {  // no pairs: empty stage
+    RPGO_HIP_CHECK(hipEventRecord(ev[9], st));
+    RPGO_HIP_CHECK(hipEventRecord(ev[10], st));
+  } else {
     if (simple) {
       if ((rc = pcm_pairs_simple(ctx, *simple, want_dist))) return rc;
       if ((rc = pcm_symmetrize(ctx))) return rc;

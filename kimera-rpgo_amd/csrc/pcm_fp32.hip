@@ -572,9 +572,7 @@ int pcm_pairs_fp32_launch(rpgo_ctx* ctx, double lc_thr, double margin, double ch
   // large stage is spread over several waves of wavefronts the hardware
   // balances, and a small one still ends within one unit of balance.
   const long long n_units = 32ll * JT * (JT + 1);
-  int dev_cu = 256;
-  (void)hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, ctx->device);
-  long long waves = 8ll * dev_cu;
+  long long waves = 8ll * ctx->n_cu;
   if (n_units > 64 * waves) waves = (n_units + 63) / 64;
   if (waves > n_units) waves = n_units;
   if (waves < 1) waves = 1;

@@ -138,6 +138,7 @@ struct rpgo_ctx {
   rpgo::MlpState* mlp = nullptr;      // created by rpgo_mlp_set_params
   rpgo::BatchState* batch = nullptr;  // created by rpgo_pcm_batch_run
   void* host_pinned = nullptr;        // 256 B pinned host buffer for counter readbacks
+  int n_cu = 256;                     // compute units of the device (launch shapes)
 };
 
 namespace rpgo {
