@@ -345,7 +345,8 @@ typedef struct {
   double pcg_rel_tol;           /* relative residual of each linear solve */
   int32_t polish_iterations;    /* final Gauss-Newton refinement steps (0: none) */
   int32_t pcg_segment;          /* poses per block-tridiagonal preconditioner segment
-                                   (default 4; 1: plain block-Jacobi)             */
+                                   (default 0: automatic, 16 below 32,768 poses and
+                                   32 from there; 1: plain block-Jacobi)          */
   int32_t gnc_warm_start;       /* 0: every inner solve restarts from the initial
                                    values (GncOptimizer); 1: from the last result  */
   int32_t pcg_coarse;           /* rigid-segment coarse level of the preconditioner:

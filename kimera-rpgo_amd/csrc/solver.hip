@@ -967,6 +967,35 @@ __global__ void __launch_bounds__(128) k_pcg_spmv_pose(int N, int a, const int* 
   }
 }
 
+// End of K2: the coarse restriction rc += P^T r (rows summed per coarse
+// segment), |r|^2 into the residual history and r . z into R[a^1].
+__device__ __forceinline__ void pcg_update_tail(int lane, int i, bool row, int cs_mine, double racc, double rr,
+                                                double rz, int first, int a, double* __restrict__ sc,
+                                                bool coarse, double* __restrict__ rc,
+                                                double* __restrict__ hist, double* red) {
+  if (coarse) {
+    // a wavefront whose segments share one coarse segment sums each row over
+    // its groups and adds once; otherwise every row adds its own
+    const int cs0 = __shfl(cs_mine, 0), cs1 = __shfl(cs_mine, 56);
+    const bool uniform = cs0 == cs1 && cs0 >= 0;
+    if (uniform) {
+      double v = row ? racc : 0.0;
+      v += __shfl_xor(v, 8);
+      v += __shfl_xor(v, 16);
+      v += __shfl_xor(v, 32);
+      if (lane < 6) atomicAdd(&rc[6 * cs0 + lane], v);
+    } else if (cs_mine >= 0) {
+      atomicAdd(&rc[6 * cs_mine + i], racc);
+    }
+  }
+  rr = block_sum(rr, red);
+  rz = block_sum(rz, red);
+  if (threadIdx.x == 0) {
+    if (!first) atomicAdd(&hist[(int)sc[kScCnt]], rr);
+    atomicAdd(&sc[kScR + (a ^ 1)], rz);
+  }
+}
+
 // K2: alpha = R[a] / PQ[a]; x += alpha p; r -= alpha q (skipped when first);
 // z = B_fine^-1 r; R[a^1] += r . z; rc += P^T r (coarse segments are whole
 // multiples of the fine segment S); hist[cnt] += |r|^2.
@@ -1054,27 +1083,162 @@ __global__ void __launch_bounds__(256) k_pcg_update(int N, int S, int first, int
     }
     if (Pb && row) cs_mine = k0 / Sc;
   }
-  if (Pb) {
-    // a wavefront whose segments share one coarse segment sums each row over
-    // its groups and adds once; otherwise every row adds its own
-    const int cs0 = __shfl(cs_mine, 0), cs1 = __shfl(cs_mine, 56);
-    const bool uniform = cs0 == cs1 && cs0 >= 0;
-    if (uniform) {
-      double v = row ? racc : 0.0;
-      v += __shfl_xor(v, 8);
-      v += __shfl_xor(v, 16);
-      v += __shfl_xor(v, 32);
-      if (lane < 6) atomicAdd(&rc[6 * cs0 + lane], v);
-    } else if (cs_mine >= 0) {
-      atomicAdd(&rc[6 * cs_mine + i], racc);
+  pcg_update_tail(lane, i, row, cs_mine, racc, rr, rz, first, a, sc, Pb != nullptr, rc, hist, red);
+}
+
+// K2 with the segment length S fixed at compile time, one wavefront per
+// workgroup over 8 fine segments (8 S consecutive poses).  The generic kernel
+// walks each of its four loops one pose at a time and waits on a global load
+// in every step (4 S round trips per PCG iteration), with 256-thread groups
+// that put the few thousand segments of a large graph on a few dozen CUs.
+// Here the wavefront stages its poses' Minv and U blocks into LDS with
+// independent coalesced loads, updates x and r entry-parallel (coalesced,
+// loads issued in chunks), runs the sequential block-tridiagonal sweeps of its
+// segments from LDS, and writes z coalesced.  z, x and r are the generic
+// kernel's to the bit; |r|^2 and r . z are the same sums in another order.
+template <int S>
+__global__ void __launch_bounds__(64) k_pcg_update_w(int N, int first, int a, double* __restrict__ sc,
+                                                    const double* __restrict__ p,
+                                                    const double* __restrict__ q, double* __restrict__ x,
+                                                    double* __restrict__ r, double* __restrict__ z,
+                                                    const float* __restrict__ Minv,
+                                                    const float* __restrict__ Useg, int Sc,
+                                                    const float* __restrict__ Pb, double* __restrict__ rc,
+                                                    double* __restrict__ hist) {
+  constexpr int NP = 8 * S;   // poses of the workgroup
+  constexpr int NV4 = NP * 9;  // float4 per 6x6 block array
+  constexpr int NE = NP * 6;   // vector entries
+  constexpr int NIT = NE / 64;  // entries per lane
+  constexpr int CH = NIT < 6 ? NIT : 6;  // entries per lane per load chunk
+  static_assert(NE % 64 == 0 && NIT % CH == 0, "segment length");
+  __shared__ float4 sM[NV4], sU[NV4];
+  __shared__ double sr[NE];
+  __shared__ float sz[NE];
+  __shared__ double red[16];
+  const int lane = threadIdx.x;
+  const int kb = blockIdx.x * NP;
+  const int np = min(NP, N - kb);  // >= 1 by the grid
+  const int ne = 6 * np, nv = 9 * np;
+  const size_t e0 = (size_t)kb * 6;
+  // stage the blocks; slots past np take copies of the last block, never read
+  {
+    const float4* gM = reinterpret_cast<const float4*>(Minv) + (size_t)kb * 9;
+    const float4* gU = reinterpret_cast<const float4*>(Useg) + (size_t)kb * 9;
+#pragma unroll
+    for (int it = 0; it < (NV4 + 63) / 64; ++it) {
+      const int t = min(lane + 64 * it, NV4 - 1);
+      const int g = min(t, nv - 1);
+      sM[t] = gM[g];
+      sU[t] = gU[g];
     }
   }
-  rr = block_sum(rr, red);
-  rz = block_sum(rz, red);
-  if (threadIdx.x == 0) {
-    if (!first) atomicAdd(&hist[(int)sc[kScCnt]], rr);
-    atomicAdd(&sc[kScR + (a ^ 1)], rz);
+  double rr = 0.0;
+  if (!first) {
+    const double pq = sc[kScPQ + a];
+    const double alpha = pq != 0.0 ? sc[kScR + a] / pq : 0.0;
+#pragma unroll
+    for (int c0 = 0; c0 < NIT; c0 += CH) {
+      double xv[CH], pv[CH], rv[CH], qv[CH];
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        const size_t o = e0 + min(lane + 64 * (c0 + u), ne - 1);
+        xv[u] = x[o];
+        pv[u] = p[o];
+        rv[u] = r[o];
+        qv[u] = q[o];
+      }
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        const int e = lane + 64 * (c0 + u);
+        const double xn = xv[u] + alpha * pv[u];
+        const double rn = rv[u] - alpha * qv[u];
+        sr[e] = rn;
+        if (e < ne) {
+          x[e0 + e] = xn;
+          r[e0 + e] = rn;
+          rr += rn * rn;
+        }
+      }
+    }
+  } else {
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) sr[lane + 64 * it] = r[e0 + min(lane + 64 * it, ne - 1)];
   }
+  __syncthreads();
+  // sweeps: eight lanes per segment, lane i < 6 owning row i
+  const int i = lane & 7, base = lane & ~7;
+  const bool row = i < 6;
+  const int ir = row ? i : 0;
+  const int k0l = (lane >> 3) * S;  // first pose of the segment in the workgroup
+  const int n = np - k0l;           // >= S: whole segment
+  const float* fM = reinterpret_cast<const float*>(sM);
+  const float* fU = reinterpret_cast<const float*>(sU);
+  // forward: y_k = r_k - U_{k-1}^T t_{k-1}, t_k = Minv_k y_k
+  float t = 0.f;
+#pragma unroll
+  for (int kk = 0; kk < S; ++kk) {
+    if (kk < n) {
+      const int kl = k0l + kk;
+      float y = (float)sr[kl * 6 + ir];
+      if (kk > 0) {
+        const float* U = fU + (kl - 1) * 36;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) y -= U[m * 6 + ir] * __shfl(t, base + m);
+      }
+      const float* Di = fM + kl * 36;
+      float v = 0.f;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) v += Di[ir * 6 + m] * __shfl(y, base + m);
+      t = v;
+      if (row) sz[kl * 6 + i] = t;
+    }
+  }
+  // backward: z_k -= Minv_k U_k z_{k+1}
+  float zn = t;
+#pragma unroll
+  for (int kk = S - 2; kk >= 0; --kk) {
+    if (kk < n - 1) {
+      const int kl = k0l + kk;
+      const float* U = fU + kl * 36;
+      const float* Di = fM + kl * 36;
+      float u = 0.f;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) u += U[ir * 6 + m] * __shfl(zn, base + m);
+      float v = sz[kl * 6 + ir];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) v -= Di[ir * 6 + m] * __shfl(u, base + m);
+      zn = v;
+      if (row) sz[kl * 6 + i] = v;
+    }
+  }
+  // coarse restriction, row i of this segment: sum_k sum_m P_k[m][i] r_k[m]
+  // (loads clamped into the workgroup's poses, poses past np add nothing)
+  double racc = 0.0;
+  int cs_mine = -1;
+  if (Pb) {
+#pragma unroll
+    for (int kk = 0; kk < S; ++kk) {
+      const int kl = min(k0l + kk, np - 1);
+      const float* P = Pb + (size_t)(kb + kl) * 36;
+      double v = 0.0;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) v += (double)P[m * 6 + ir] * sr[kl * 6 + m];
+      racc += kk < n ? v : 0.0;
+    }
+    if (row && n > 0) cs_mine = (kb + k0l) / Sc;
+  }
+  __syncthreads();
+  double rz = 0.0;
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int e = lane + 64 * it;
+    if (e < ne) {
+      const float zv = sz[e];
+      z[e0 + e] = zv;
+      rz += sr[e] * (double)zv;
+    }
+  }
+  pcg_update_tail(lane, i, row, cs_mine, racc, rr, rz, first, a, sc, Pb != nullptr, rc, hist, red);
 }
 
 // K3: cc[a] += Ainv rc over a (64-row, 64-column) tile per workgroup;
@@ -1349,6 +1513,17 @@ void SolverRun::spmv_dev(hipStream_t st, const double* lam, const double* xin, d
       Hij.as<double>(), hdiag.as<double>(), lam, 0.0, prm.lm_diagonal_damping, 1e-6, 1e32, xin, yout);
 }
 
+// Poses per fine preconditioner segment.  Automatic (pcg_segment <= 0): 16
+// below 32,768 poses, 32 from there.  Longer segments hold more of the
+// odometry chain (fewer PCG iterations) but make the coarse segments, whole
+// multiples of S, coarser; on SE(3) helices with 50 % outliers, plain GNC:
+// 5k poses S = 4 / 16 / 32 -> 2.04 / 1.87 / 2.07 s, 20k 13.8 / 8.8 / 9.0 s,
+// 50k S = 16 / 32 -> 57.6 / 50.0 s (75.1 s at S = 4).
+static int fine_segment(const rpgo_solve_params& prm, int N) {
+  if (prm.pcg_segment > 0) return prm.pcg_segment;
+  return N < 32768 ? 16 : 32;
+}
+
 // Coarse level set-up for one linear solve: prolongation blocks at the current
 // poses, Ac = P^T (H + lambda damp) P, its Cholesky factor, L^-1 and the FP32
 // explicit inverse.  *nseg = 0 when the coarse level is off.
@@ -1357,7 +1532,7 @@ int SolverRun::coarse_setup(hipStream_t st, double lambda, int* Sc_out, int* nse
   *nseg_out = 0;
   if (prm.pcg_coarse < 0 || N < 2) return RPGO_OK;
   const int segs_max = prm.pcg_coarse > 0 ? prm.pcg_coarse : kCoarseSegMax;
-  const int Sfine = prm.pcg_segment > 0 ? prm.pcg_segment : 128;
+  const int Sfine = fine_segment(prm, N);
   int Sc = std::max(Sfine, (N + segs_max - 1) / segs_max);
   Sc = (Sc + Sfine - 1) / Sfine * Sfine;  // whole fine segments per coarse segment
   const int nseg = (N + Sc - 1) / Sc;
@@ -1414,9 +1589,22 @@ void SolverRun::pcg_step(hipStream_t st, int S, int Sc, int nseg_c, int first, i
         N, a, inc_off.as<int>(), inc.as<int>(), fi.as<int>(), fj.as<int>(), kind.as<int>(),
         D.as<double>(), Hij.as<double>(), hdiag.as<double>(), prm.lm_diagonal_damping, s, p.as<double>(),
         q.as<double>());
-  k_pcg_update<<<grid_for((long long)nseg * 8, 256, 1 << 22), 256, 0, st>>>(
-      N, S, first, a, s, p.as<double>(), q.as<double>(), dx.as<double>(), r.as<double>(), zv.as<double>(),
-      Minv.as<float>(), Useg.as<float>(), Sc, P, cres.as<double>(), hist.as<double>());
+  // segment lengths with a fixed-S kernel: one wavefront per 8 segments
+  const unsigned ugw = (unsigned)((N + 8ll * S - 1) / (8ll * S));
+#define RPGO_UPDATE_ARGS                                                                                  \
+  N, first, a, s, p.as<double>(), q.as<double>(), dx.as<double>(), r.as<double>(), zv.as<double>(),      \
+      Minv.as<float>(), Useg.as<float>(), Sc, P, cres.as<double>(), hist.as<double>()
+  switch (S) {
+    case 4: k_pcg_update_w<4><<<ugw, 64, 0, st>>>(RPGO_UPDATE_ARGS); break;
+    case 8: k_pcg_update_w<8><<<ugw, 64, 0, st>>>(RPGO_UPDATE_ARGS); break;
+    case 16: k_pcg_update_w<16><<<ugw, 64, 0, st>>>(RPGO_UPDATE_ARGS); break;
+    case 32: k_pcg_update_w<32><<<ugw, 64, 0, st>>>(RPGO_UPDATE_ARGS); break;
+    default:
+      k_pcg_update<<<grid_for((long long)nseg * 8, 256, 1 << 22), 256, 0, st>>>(
+          N, S, first, a, s, p.as<double>(), q.as<double>(), dx.as<double>(), r.as<double>(), zv.as<double>(),
+          Minv.as<float>(), Useg.as<float>(), Sc, P, cres.as<double>(), hist.as<double>());
+  }
+#undef RPGO_UPDATE_ARGS
   if (nseg_c) {
     const dim3 grid((nc + 63) / 64, (nc + 63) / 64);
     k_pcg_coarse<<<grid, 256, 0, st>>>(nc, a, Ainv.as<double>(), cres.as<double>(), a ? cc1 : cc0, s);
@@ -1431,7 +1619,7 @@ void SolverRun::pcg_step(hipStream_t st, int S, int Sc, int nseg_c, int first, i
 int SolverRun::pcg(hipStream_t st, double lambda, int* iters_out) {
   const long long n = (long long)N * 6;
   double* s = sc.as<double>();
-  const int S = prm.pcg_segment > 0 ? prm.pcg_segment : 128;
+  const int S = fine_segment(prm, N);
   const int nseg = (N + S - 1) / S;
   k_seg_factor<<<(nseg + 63) / 64, 64, 0, st>>>(N, S, chain.as<int>(), D.as<double>(), hdiag.as<double>(),
                                                 Hij.as<double>(), lambda, prm.lm_diagonal_damping,

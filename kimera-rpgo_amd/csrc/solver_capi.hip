@@ -92,7 +92,7 @@ void rpgo_solve_params_default(rpgo_solve_params* p) {
   p->pcg_max_iterations = 0;
   p->pcg_rel_tol = 1e-8;  // near-exact steps: LM follows the direct-solve trajectory
   p->polish_iterations = 0;
-  p->pcg_segment = 4;
+  p->pcg_segment = 0;
   p->gnc_warm_start = 0;
   p->pcg_coarse = 0;
   p->pcg_graphs = 1;

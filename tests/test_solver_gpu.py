@@ -55,6 +55,24 @@ def test_lm_matches_oracle_20k(ctx):
     assert abs(rep.final_cost - err) <= 1e-6 * err
 
 
+def test_lm_segment_lengths_match_oracle(ctx):
+    """Every preconditioner segment length reaches the oracle's optimum: S = 4,
+    8, 16 and 32 run the fixed-S kernel (one wavefront per 8 segments, LDS
+    staged), S = 6 and 1 the generic one.  2,003 poses: the last workgroup and
+    the last segment are partial."""
+    gr = synth.se3_helix(n_poses=2003, n_lc=250, outlier_frac=0.0, seed=8)
+    g, nk = from_synth(gr)
+    R, t, err, it = S.least_squares(g, np.ones(g.F))
+    for seg in (4, 8, 16, 32, 6, 1):
+        print(f"[solve] LM N={g.N} pcg_segment={seg}", flush=True)
+        out, w, rep = _gpu(ctx, g, polish_iterations=10, pcg_rel_tol=1e-6, pcg_segment=seg)
+        dt, dr = pose_error(out, R, t)
+        print(f"[solve] S={seg} lm it={rep.lm_iterations} pcg={rep.pcg_iterations} cost={rep.final_cost:.6f} "
+              f"oracle={err:.6f} dt={dt:.2e} dr={dr:.2e} ms={rep.ms_total:.1f}", flush=True)
+        assert dt < POSE_TOL and dr < POSE_TOL
+        assert abs(rep.final_cost - err) <= 1e-6 * err
+
+
 def test_gn_matches_oracle(ctx):
     gr = synth.se3_helix(n_poses=300, n_lc=40, outlier_frac=0.0, seed=5)
     g, nk = from_synth(gr)
