@@ -46,7 +46,7 @@ struct GncParams {
 struct DeviceParams {
   int device = 0;
   double pcg_rel_tol = 1e-8;
-  int pcg_segment = 0;  // <= 0: automatic (16 poses, 32 from 32,768 poses)
+  int pcg_segment = 0;  // <= 0: automatic (16 poses, 32 from 10,000 poses)
   int pcg_coarse = 0;
   int pcm_precision = 1;  // 1: FP32 pairs + FP64 recheck, 0: FP64 replay
 };

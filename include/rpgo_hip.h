@@ -345,7 +345,7 @@ typedef struct {
   double pcg_rel_tol;           /* relative residual of each linear solve */
   int32_t polish_iterations;    /* final Gauss-Newton refinement steps (0: none) */
   int32_t pcg_segment;          /* poses per block-tridiagonal preconditioner segment
-                                   (default 0: automatic, 16 below 32,768 poses and
+                                   (default 0: automatic, 16 below 10,000 poses and
                                    32 from there; 1: plain block-Jacobi)          */
   int32_t gnc_warm_start;       /* 0: every inner solve restarts from the initial
                                    values (GncOptimizer); 1: from the last result  */

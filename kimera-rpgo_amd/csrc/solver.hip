@@ -1086,36 +1086,56 @@ __global__ void __launch_bounds__(256) k_pcg_update(int N, int S, int first, int
   pcg_update_tail(lane, i, row, cs_mine, racc, rr, rz, first, a, sc, Pb != nullptr, rc, hist, red);
 }
 
-// K2 with the segment length S fixed at compile time, one wavefront per
-// workgroup over 8 fine segments (8 S consecutive poses).  The generic kernel
-// walks each of its four loops one pose at a time and waits on a global load
-// in every step (4 S round trips per PCG iteration), with 256-thread groups
-// that put the few thousand segments of a large graph on a few dozen CUs.
-// Here the wavefront stages its poses' Minv and U blocks into LDS with
-// independent coalesced loads, updates x and r entry-parallel (coalesced,
-// loads issued in chunks), runs the sequential block-tridiagonal sweeps of its
-// segments from LDS, and writes z coalesced.  z, x and r are the generic
-// kernel's to the bit; |r|^2 and r . z are the same sums in another order.
+// The six row values of an 8-lane group (lanes base .. base + 5) on every
+// lane of the group, by DPP row broadcasts: a 16-lane row holds two groups, so
+// banks 0-1 (lanes 0-7 of the row) take lane m and banks 2-3 lane 8 + m.  VALU
+// moves, where a lane shuffle is an LDS round trip.
+template <int M>
+__device__ __forceinline__ float grp_bcast(float v) {
+  const int iv = __float_as_int(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, iv, 0x150 + M, 0xf, 0x3, false);  // row_newbcast:M
+  return __int_as_float(__builtin_amdgcn_update_dpp(lo, iv, 0x158 + M, 0xf, 0xc, false));
+}
+__device__ __forceinline__ void grp_bcast6(float v, float (&o)[6]) {
+  o[0] = grp_bcast<0>(v);
+  o[1] = grp_bcast<1>(v);
+  o[2] = grp_bcast<2>(v);
+  o[3] = grp_bcast<3>(v);
+  o[4] = grp_bcast<4>(v);
+  o[5] = grp_bcast<5>(v);
+}
+
+// K2 with the segment length S fixed at compile time: a 256-thread workgroup
+// over 8 fine segments (8 S consecutive poses).  The generic kernel walks each
+// of its four loops one pose at a time and waits on a global load in every
+// step (4 S round trips per PCG iteration), with groups that put the few
+// thousand segments of a large graph on a few dozen CUs.  Here the four
+// wavefronts stage the poses' Minv and U blocks into LDS and update x and r
+// entry-parallel (independent coalesced loads, all in flight at once); then
+// wavefront 0 runs the sequential block-tridiagonal sweeps of the 8 segments
+// from LDS (group operands exchanged by DPP row broadcasts) while wavefront 1
+// forms the coarse restriction, and all four write z.  z, x, r and the restriction are the generic kernel's to the bit; |r|^2
+// and r . z are the same sums in another order.
 template <int S>
-__global__ void __launch_bounds__(64) k_pcg_update_w(int N, int first, int a, double* __restrict__ sc,
-                                                    const double* __restrict__ p,
-                                                    const double* __restrict__ q, double* __restrict__ x,
-                                                    double* __restrict__ r, double* __restrict__ z,
-                                                    const float* __restrict__ Minv,
-                                                    const float* __restrict__ Useg, int Sc,
-                                                    const float* __restrict__ Pb, double* __restrict__ rc,
-                                                    double* __restrict__ hist) {
-  constexpr int NP = 8 * S;   // poses of the workgroup
+__global__ void __launch_bounds__(256) k_pcg_update_w(int N, int first, int a, double* __restrict__ sc,
+                                                     const double* __restrict__ p,
+                                                     const double* __restrict__ q, double* __restrict__ x,
+                                                     double* __restrict__ r, double* __restrict__ z,
+                                                     const float* __restrict__ Minv,
+                                                     const float* __restrict__ Useg, int Sc,
+                                                     const float* __restrict__ Pb, double* __restrict__ rc,
+                                                     double* __restrict__ hist) {
+  constexpr int NP = 8 * S;    // poses of the workgroup
   constexpr int NV4 = NP * 9;  // float4 per 6x6 block array
   constexpr int NE = NP * 6;   // vector entries
-  constexpr int NIT = NE / 64;  // entries per lane
-  constexpr int CH = NIT < 6 ? NIT : 6;  // entries per lane per load chunk
-  static_assert(NE % 64 == 0 && NIT % CH == 0, "segment length");
+  constexpr int NIT = (NE + 255) / 256;  // entries per thread
+  constexpr int CH = NIT < 6 ? NIT : 6;  // entries per thread per load chunk
+  static_assert(NIT % CH == 0, "segment length");
   __shared__ float4 sM[NV4], sU[NV4];
   __shared__ double sr[NE];
   __shared__ float sz[NE];
   __shared__ double red[16];
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int kb = blockIdx.x * NP;
   const int np = min(NP, N - kb);  // >= 1 by the grid
   const int ne = 6 * np, nv = 9 * np;
@@ -1125,8 +1145,8 @@ __global__ void __launch_bounds__(64) k_pcg_update_w(int N, int first, int a, do
     const float4* gM = reinterpret_cast<const float4*>(Minv) + (size_t)kb * 9;
     const float4* gU = reinterpret_cast<const float4*>(Useg) + (size_t)kb * 9;
 #pragma unroll
-    for (int it = 0; it < (NV4 + 63) / 64; ++it) {
-      const int t = min(lane + 64 * it, NV4 - 1);
+    for (int it = 0; it < (NV4 + 255) / 256; ++it) {
+      const int t = min(tid + 256 * it, NV4 - 1);
       const int g = min(t, nv - 1);
       sM[t] = gM[g];
       sU[t] = gU[g];
@@ -1141,7 +1161,7 @@ __global__ void __launch_bounds__(64) k_pcg_update_w(int N, int first, int a, do
       double xv[CH], pv[CH], rv[CH], qv[CH];
 #pragma unroll
       for (int u = 0; u < CH; ++u) {
-        const size_t o = e0 + min(lane + 64 * (c0 + u), ne - 1);
+        const size_t o = e0 + min(tid + 256 * (c0 + u), ne - 1);
         xv[u] = x[o];
         pv[u] = p[o];
         rv[u] = r[o];
@@ -1149,10 +1169,10 @@ __global__ void __launch_bounds__(64) k_pcg_update_w(int N, int first, int a, do
       }
 #pragma unroll
       for (int u = 0; u < CH; ++u) {
-        const int e = lane + 64 * (c0 + u);
+        const int e = tid + 256 * (c0 + u);
         const double xn = xv[u] + alpha * pv[u];
         const double rn = rv[u] - alpha * qv[u];
-        sr[e] = rn;
+        if (e < NE) sr[e] = rn;
         if (e < ne) {
           x[e0 + e] = xn;
           r[e0 + e] = rn;
@@ -1162,60 +1182,71 @@ __global__ void __launch_bounds__(64) k_pcg_update_w(int N, int first, int a, do
     }
   } else {
 #pragma unroll
-    for (int it = 0; it < NIT; ++it) sr[lane + 64 * it] = r[e0 + min(lane + 64 * it, ne - 1)];
+    for (int it = 0; it < NIT; ++it) {
+      const int e = tid + 256 * it;
+      const double v = r[e0 + min(e, ne - 1)];
+      if (e < NE) sr[e] = v;
+    }
   }
   __syncthreads();
-  // sweeps: eight lanes per segment, lane i < 6 owning row i
-  const int i = lane & 7, base = lane & ~7;
+  // eight lanes per segment, lane i < 6 owning row i
+  const int i = lane & 7;
   const bool row = i < 6;
   const int ir = row ? i : 0;
   const int k0l = (lane >> 3) * S;  // first pose of the segment in the workgroup
   const int n = np - k0l;           // >= S: whole segment
-  const float* fM = reinterpret_cast<const float*>(sM);
-  const float* fU = reinterpret_cast<const float*>(sU);
-  // forward: y_k = r_k - U_{k-1}^T t_{k-1}, t_k = Minv_k y_k
-  float t = 0.f;
-#pragma unroll
-  for (int kk = 0; kk < S; ++kk) {
-    if (kk < n) {
-      const int kl = k0l + kk;
-      float y = (float)sr[kl * 6 + ir];
-      if (kk > 0) {
-        const float* U = fU + (kl - 1) * 36;
-#pragma unroll
-        for (int m = 0; m < 6; ++m) y -= U[m * 6 + ir] * __shfl(t, base + m);
-      }
-      const float* Di = fM + kl * 36;
-      float v = 0.f;
-#pragma unroll
-      for (int m = 0; m < 6; ++m) v += Di[ir * 6 + m] * __shfl(y, base + m);
-      t = v;
-      if (row) sz[kl * 6 + i] = t;
-    }
-  }
-  // backward: z_k -= Minv_k U_k z_{k+1}
-  float zn = t;
-#pragma unroll
-  for (int kk = S - 2; kk >= 0; --kk) {
-    if (kk < n - 1) {
-      const int kl = k0l + kk;
-      const float* U = fU + kl * 36;
-      const float* Di = fM + kl * 36;
-      float u = 0.f;
-#pragma unroll
-      for (int m = 0; m < 6; ++m) u += U[ir * 6 + m] * __shfl(zn, base + m);
-      float v = sz[kl * 6 + ir];
-#pragma unroll
-      for (int m = 0; m < 6; ++m) v -= Di[ir * 6 + m] * __shfl(u, base + m);
-      zn = v;
-      if (row) sz[kl * 6 + i] = v;
-    }
-  }
-  // coarse restriction, row i of this segment: sum_k sum_m P_k[m][i] r_k[m]
-  // (loads clamped into the workgroup's poses, poses past np add nothing)
   double racc = 0.0;
   int cs_mine = -1;
-  if (Pb) {
+  if (wave == 0) {
+    const float* fM = reinterpret_cast<const float*>(sM);
+    const float* fU = reinterpret_cast<const float*>(sU);
+    // forward: y_k = r_k - U_{k-1}^T t_{k-1}, t_k = Minv_k y_k
+    float t = 0.f;
+#pragma unroll
+    for (int kk = 0; kk < S; ++kk) {
+      if (kk < n) {
+        const int kl = k0l + kk;
+        float y = (float)sr[kl * 6 + ir];
+        float g[6];
+        if (kk > 0) {
+          const float* U = fU + (kl - 1) * 36;
+          grp_bcast6(t, g);
+#pragma unroll
+          for (int m = 0; m < 6; ++m) y -= U[m * 6 + ir] * g[m];
+        }
+        const float* Di = fM + kl * 36;
+        grp_bcast6(y, g);
+        float v = 0.f;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) v += Di[ir * 6 + m] * g[m];
+        t = v;
+        if (row) sz[kl * 6 + i] = t;
+      }
+    }
+    // backward: z_k -= Minv_k U_k z_{k+1}
+    float zn = t;
+#pragma unroll
+    for (int kk = S - 2; kk >= 0; --kk) {
+      if (kk < n - 1) {
+        const int kl = k0l + kk;
+        const float* U = fU + kl * 36;
+        const float* Di = fM + kl * 36;
+        float g[6];
+        grp_bcast6(zn, g);
+        float u = 0.f;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) u += U[ir * 6 + m] * g[m];
+        grp_bcast6(u, g);
+        float v = sz[kl * 6 + ir];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) v -= Di[ir * 6 + m] * g[m];
+        zn = v;
+        if (row) sz[kl * 6 + i] = v;
+      }
+    }
+  } else if (wave == 1 && Pb) {
+    // coarse restriction, row i of each segment: sum_k sum_m P_k[m][i] r_k[m]
+    // (loads clamped into the workgroup's poses, poses past np add nothing)
 #pragma unroll
     for (int kk = 0; kk < S; ++kk) {
       const int kl = min(k0l + kk, np - 1);
@@ -1231,14 +1262,15 @@ __global__ void __launch_bounds__(64) k_pcg_update_w(int N, int first, int a, do
   double rz = 0.0;
 #pragma unroll
   for (int it = 0; it < NIT; ++it) {
-    const int e = lane + 64 * it;
+    const int e = tid + 256 * it;
     if (e < ne) {
       const float zv = sz[e];
       z[e0 + e] = zv;
       rz += sr[e] * (double)zv;
     }
   }
-  pcg_update_tail(lane, i, row, cs_mine, racc, rr, rz, first, a, sc, Pb != nullptr, rc, hist, red);
+  pcg_update_tail(lane, i, row, cs_mine, racc, rr, rz, first, a, sc, Pb != nullptr && wave == 1, rc, hist,
+                  red);
 }
 
 // K3: cc[a] += Ainv rc over a (64-row, 64-column) tile per workgroup;
@@ -1514,14 +1546,14 @@ void SolverRun::spmv_dev(hipStream_t st, const double* lam, const double* xin, d
 }
 
 // Poses per fine preconditioner segment.  Automatic (pcg_segment <= 0): 16
-// below 32,768 poses, 32 from there.  Longer segments hold more of the
+// below 10,000 poses, 32 from there.  Longer segments hold more of the
 // odometry chain (fewer PCG iterations) but make the coarse segments, whole
 // multiples of S, coarser; on SE(3) helices with 50 % outliers, plain GNC:
-// 5k poses S = 4 / 16 / 32 -> 2.04 / 1.87 / 2.07 s, 20k 13.8 / 8.8 / 9.0 s,
-// 50k S = 16 / 32 -> 57.6 / 50.0 s (75.1 s at S = 4).
+// 5k poses S = 16 / 32 -> 1.73 / 1.74 s, 20k 8.23 / 7.64 s, 50k (LM on the
+// outlier-free graph) 1.54 / 1.42 s.
 static int fine_segment(const rpgo_solve_params& prm, int N) {
   if (prm.pcg_segment > 0) return prm.pcg_segment;
-  return N < 32768 ? 16 : 32;
+  return N < 10000 ? 16 : 32;
 }
 
 // Coarse level set-up for one linear solve: prolongation blocks at the current
@@ -1589,16 +1621,16 @@ void SolverRun::pcg_step(hipStream_t st, int S, int Sc, int nseg_c, int first, i
         N, a, inc_off.as<int>(), inc.as<int>(), fi.as<int>(), fj.as<int>(), kind.as<int>(),
         D.as<double>(), Hij.as<double>(), hdiag.as<double>(), prm.lm_diagonal_damping, s, p.as<double>(),
         q.as<double>());
-  // segment lengths with a fixed-S kernel: one wavefront per 8 segments
+  // segment lengths with a fixed-S kernel: one workgroup per 8 segments
   const unsigned ugw = (unsigned)((N + 8ll * S - 1) / (8ll * S));
 #define RPGO_UPDATE_ARGS                                                                                  \
   N, first, a, s, p.as<double>(), q.as<double>(), dx.as<double>(), r.as<double>(), zv.as<double>(),      \
       Minv.as<float>(), Useg.as<float>(), Sc, P, cres.as<double>(), hist.as<double>()
   switch (S) {
-    case 4: k_pcg_update_w<4><<<ugw, 64, 0, st>>>(RPGO_UPDATE_ARGS); break;
-    case 8: k_pcg_update_w<8><<<ugw, 64, 0, st>>>(RPGO_UPDATE_ARGS); break;
-    case 16: k_pcg_update_w<16><<<ugw, 64, 0, st>>>(RPGO_UPDATE_ARGS); break;
-    case 32: k_pcg_update_w<32><<<ugw, 64, 0, st>>>(RPGO_UPDATE_ARGS); break;
+    case 4: k_pcg_update_w<4><<<ugw, 256, 0, st>>>(RPGO_UPDATE_ARGS); break;
+    case 8: k_pcg_update_w<8><<<ugw, 256, 0, st>>>(RPGO_UPDATE_ARGS); break;
+    case 16: k_pcg_update_w<16><<<ugw, 256, 0, st>>>(RPGO_UPDATE_ARGS); break;
+    case 32: k_pcg_update_w<32><<<ugw, 256, 0, st>>>(RPGO_UPDATE_ARGS); break;
     default:
       k_pcg_update<<<grid_for((long long)nseg * 8, 256, 1 << 22), 256, 0, st>>>(
           N, S, first, a, s, p.as<double>(), q.as<double>(), dx.as<double>(), r.as<double>(), zv.as<double>(),

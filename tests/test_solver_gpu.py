@@ -57,8 +57,8 @@ def test_lm_matches_oracle_20k(ctx):
 
 def test_lm_segment_lengths_match_oracle(ctx):
     """Every preconditioner segment length reaches the oracle's optimum: S = 4,
-    8, 16 and 32 run the fixed-S kernel (one wavefront per 8 segments, LDS
-    staged), S = 6 and 1 the generic one.  2,003 poses: the last workgroup and
+    8, 16 and 32 run the fixed-S kernel (one workgroup per 8 segments, LDS
+    staged, DPP sweeps), S = 6 and 1 the generic one.  2,003 poses: the last workgroup and
     the last segment are partial."""
     gr = synth.se3_helix(n_poses=2003, n_lc=250, outlier_frac=0.0, seed=8)
     g, nk = from_synth(gr)
