@@ -46,6 +46,7 @@ struct SolverRun {
   DevBuf hist;                    // PCG residual history (per iteration)
   DevBuf Pb, Ac, Lci, Ainv, cres, ccor;  // coarse level: prolongation blocks, coarse matrix,
                                      // its inverse factor, FP32 inverse, coarse vectors
+  DevBuf LcT;                        // the coarse Cholesky factor transposed (column access)
   bool polish = false;               // refinement pass: a step that raises the cost is rejected
   hipGraphExec_t graph_exec = nullptr;  // captured PCG chunk
   GraphKey graph_key;
@@ -71,7 +72,7 @@ struct SolverRun {
     graph_exec = nullptr;
     for (DevBuf* v : {&hist,&x0, &x, &xn, &kind, &fi, &fj, &z, &info, &w, &barc, &known, &err, &cost, &Hii,
                       &Hjj, &Hij, &gi, &gj, &inc_off, &inc_cnt, &inc, &D, &Minv, &hdiag, &b64,
-                      &chain, &Useg, &Pb, &Ac, &Lci, &Ainv, &cres, &ccor,
+                      &chain, &Useg, &Pb, &Ac, &Lci, &LcT, &Ainv, &cres, &ccor,
                       &dx, &r, &zv, &p, &q, &hx, &sc})
       v->release();
   }

@@ -637,52 +637,66 @@ __global__ void k_coarse_offdiag(int F, int Sc, int nc, const int* __restrict__ 
     }
 }
 
-// Blocked right-looking Cholesky of the dense nc x nc coarse matrix (lower,
-// in place), panels of kCholNb columns: k_chol_panel factors one panel with a
-// single workgroup, k_chol_update applies its rank-kCholNb update to the
-// trailing lower triangle in 32x32 tiles across the chip.  Pivots are floored
-// at a tiny positive value so a gauge-free (semi-definite) problem keeps a
-// finite preconditioner.
 constexpr int kCholNb = 16;
 
-// One panel (columns k0 .. k0+15, rows k0 .. n-1) factored in LDS by one
-// workgroup: the rows' 16 doubles are contiguous in global memory, so the
-// panel is loaded and stored with coalesced row segments.
-__global__ void __launch_bounds__(1024) k_chol_panel(int n, int k0, double* __restrict__ A) {
-  __shared__ double P[kCoarseMax * kCholNb];
-  __shared__ double piv;
+// Blocked right-looking Cholesky of the coarse matrix A (n x n, row-major,
+// lower triangle used), panels of 16 columns.  The factor L is written only
+// transposed, into AT (row j of AT = column j of L), which is what the
+// inverse reads; A keeps the trailing matrix.  Pivots are floored at a tiny
+// positive value so a gauge-free (semi-definite) problem keeps a finite
+// preconditioner.
+//
+// Panel step 1: the 16 x 16 diagonal block at (k0, k0), factored by one
+// wavefront (four entries per lane, row e >> 4, column e & 15), then L_kk
+// into AT.  The rows below the block are independent triangular solves and
+// are done by the trailing update's workgroups (step 2); with the whole
+// panel (up to 768 rows) on one workgroup, a panel took 13-28 us.
+__global__ void __launch_bounds__(64) k_chol_panel(int n, int k0, const double* __restrict__ A,
+                                                   double* __restrict__ AT) {
+  static_assert(kCholNb == 16, "entry e = 16 row + column");
+  __shared__ double blk[kCholNb * kCholNb];
   const int nb = min(kCholNb, n - k0);
-  const int rows = n - k0;
-  for (int e = threadIdx.x; e < rows * kCholNb; e += blockDim.x) {
-    const int r = e / kCholNb, c = e % kCholNb;
-    P[e] = c < nb ? A[(size_t)(k0 + r) * n + k0 + c] : 0.0;
+  const int lane = threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int e = lane + 64 * u, r = e >> 4, j = e & 15;
+    blk[e] = (r < nb && j < nb) ? A[(size_t)(k0 + r) * n + k0 + j] : 0.0;
   }
   __syncthreads();
+  // unscaled right-looking elimination: step c updates columns > c
   for (int c = 0; c < nb; ++c) {
-    if (threadIdx.x == 0) {
-      piv = sqrt(fmax(P[c * kCholNb + c], 1e-300));
-      P[c * kCholNb + c] = piv;
+    const double inv = 1.0 / fmax(blk[c * kCholNb + c], 1e-300);
+    double upd[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = lane + 64 * u, r = e >> 4, j = e & 15;
+      upd[u] = blk[e] - blk[r * kCholNb + c] * blk[j * kCholNb + c] * inv;
     }
     __syncthreads();
-    const double d = piv;
-    for (int r = c + 1 + threadIdx.x; r < rows; r += blockDim.x) P[r * kCholNb + c] /= d;
-    __syncthreads();
-    const int w = nb - (c + 1);
-    if (w > 0)
-      for (int e = threadIdx.x; e < (rows - c - 1) * w; e += blockDim.x) {
-        const int r = c + 1 + e / w, j = c + 1 + e % w;
-        if (j <= r) P[r * kCholNb + j] -= P[r * kCholNb + c] * P[j * kCholNb + c];
-      }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = lane + 64 * u, r = e >> 4, j = e & 15;
+      if (r < nb && j > c && j <= r) blk[e] = upd[u];
+    }
     __syncthreads();
   }
-  for (int e = threadIdx.x; e < rows * kCholNb; e += blockDim.x) {
-    const int r = e / kCholNb, c = e % kCholNb;
-    if (c < nb && c <= r) A[(size_t)(k0 + r) * n + k0 + c] = P[e];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int e = lane + 64 * u, r = e >> 4, j = e & 15;
+    if (r < nb && j <= r) {
+      const double piv = sqrt(fmax(blk[j * kCholNb + j], 1e-300));
+      AT[(size_t)(k0 + j) * n + k0 + r] = r == j ? piv : blk[e] / piv;
+    }
   }
 }
 
-// trailing update A[i][j] -= sum_{c in panel} L[i][c] L[j][c], j <= i, rows >= k0 + nb
-__global__ void __launch_bounds__(256) k_chol_update(int n, int k0, double* __restrict__ A) {
+// Panel step 2, over the rows below the diagonal block (r >= k0 + 16): one
+// 32 x 32 lower tile (ti, tj) of the trailing matrix per workgroup.  The
+// workgroup solves its two row tiles of the panel, l_r L_kk^T = a_r (forward
+// substitution, one thread per row), writes them into AT when ti == tj (each
+// row tile has one diagonal tile), and updates A_ij -= sum_c l_ic l_jc.
+__global__ void __launch_bounds__(256) k_chol_update(int n, int k0, double* __restrict__ A,
+                                                     double* __restrict__ AT) {
   const int nb = min(kCholNb, n - k0);
   const int r0 = k0 + nb;
   // lower-triangle tile index -> (ti, tj), tj <= ti
@@ -691,80 +705,157 @@ __global__ void __launch_bounds__(256) k_chol_update(int n, int k0, double* __re
   while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
   while (ti * (ti + 1) / 2 > t) --ti;
   const int tj = t - ti * (ti + 1) / 2;
-  __shared__ double Li[32][kCholNb + 1], Lj[32][kCholNb + 1];
-  for (int e = threadIdx.x; e < 32 * kCholNb; e += blockDim.x) {
+  __shared__ double Lkk[kCholNb][kCholNb + 1];
+  __shared__ double Lr[64][kCholNb + 1];  // rows 0-31: tile ti, 32-63: tile tj
+  for (int e = threadIdx.x; e < kCholNb * kCholNb; e += blockDim.x) {
+    const int c = e / kCholNb, r = e % kCholNb;  // AT row c = column c of L
+    Lkk[r][c] = (r < nb && c < nb && c <= r) ? AT[(size_t)(k0 + c) * n + k0 + r] : 0.0;
+  }
+  for (int e = threadIdx.x; e < 64 * kCholNb; e += blockDim.x) {
     const int rr = e / kCholNb, cc = e % kCholNb;
-    const int gi = r0 + ti * 32 + rr, gj = r0 + tj * 32 + rr;
-    Li[rr][cc] = (gi < n && cc < nb) ? A[(size_t)gi * n + k0 + cc] : 0.0;
-    Lj[rr][cc] = (gj < n && cc < nb) ? A[(size_t)gj * n + k0 + cc] : 0.0;
+    const int g = r0 + (rr < 32 ? ti * 32 + rr : tj * 32 + rr - 32);
+    Lr[rr][cc] = (g < n && cc < nb) ? A[(size_t)g * n + k0 + cc] : 0.0;
   }
   __syncthreads();
+  if (threadIdx.x < 64) {
+    const int rr = threadIdx.x;
+    for (int c = 0; c < nb; ++c) {
+      double v = Lr[rr][c];
+      for (int k = 0; k < c; ++k) v -= Lr[rr][k] * Lkk[c][k];
+      Lr[rr][c] = v / Lkk[c][c];
+    }
+  }
+  __syncthreads();
+  if (ti == tj)
+    for (int e = threadIdx.x; e < 32 * kCholNb; e += blockDim.x) {
+      const int cc = e >> 5, rr = e & 31;  // AT row k0 + cc, rows coalesced
+      const int g = r0 + ti * 32 + rr;
+      if (g < n && cc < nb) AT[(size_t)(k0 + cc) * n + g] = Lr[rr][cc];
+    }
   for (int e = threadIdx.x; e < 32 * 32; e += blockDim.x) {
     const int rr = e >> 5, cc = e & 31;
     const int gi = r0 + ti * 32 + rr, gj = r0 + tj * 32 + cc;
     if (gi >= n || gj > gi) continue;
     double sum = 0.0;
 #pragma unroll 8
-    for (int m = 0; m < kCholNb; ++m) sum += Li[rr][m] * Lj[cc][m];
+    for (int m = 0; m < kCholNb; ++m) sum += Lr[rr][m] * Lr[32 + cc][m];
     A[(size_t)gi * n + gj] -= sum;
   }
 }
 
-// W = L^-1 (lower): one wave per column c; w_i = (delta_ic - sum_{c<=j<i} L_ij w_j) / L_ii.
-// Lane l owns the entries j = l (mod 64) of its column in LDS, so every LDS
-// value is read back only by the lane that wrote it.  Row i+1 of L is loaded
-// while row i is reduced.
-constexpr int kTriPL = kCoarseMax / 64;  // row entries per lane
-__global__ void __launch_bounds__(256) k_tri_inv(int n, const double* __restrict__ L, double* __restrict__ W) {
-  __shared__ double wsh[4][kCoarseMax];
+// W = L^-1 (lower): one wave per column c, column-oriented substitution:
+// x_i = (delta_ic - s_i) / L_ii, then s_k += L_ki x_i for every k > i.  Lane
+// l keeps the partial sums s_k of the rows k = l (mod 64) in registers, x_i
+// reaches every lane by readlane from the owner of s_i, and column i of L is
+// row i of LT (coalesced), prefetched PF rows ahead.  The row-oriented form
+// (w_i from a dot product over j < i) needed a wave reduction per row:
+// 691 us at 630 unknowns, 5k poses.
+constexpr int kTriPL = kCoarseMax / 64;  // rows per lane
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__global__ void __launch_bounds__(256) k_tri_inv(int n, const double* __restrict__ LT, double* __restrict__ W) {
+  constexpr int PF = 4;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int c = blockIdx.x * 4 + wv;
   if (c >= n) return;
-  double* w = wsh[wv];
-  double cur[kTriPL], nxt[kTriPL];
+  double sk[kTriPL];
+  double buf[PF][kTriPL];
 #pragma unroll
-  for (int q = 0; q < kTriPL; ++q) {
-    const int j = lane + 64 * q;
-    cur[q] = j < n ? L[(size_t)c * n + j] : 0.0;
-  }
-  for (int i = c; i < n; ++i) {
-    if (i + 1 < n) {
+  for (int q = 0; q < kTriPL; ++q) sk[q] = 0.0;
 #pragma unroll
-      for (int q = 0; q < kTriPL; ++q) {
-        const int j = lane + 64 * q;
-        nxt[q] = j < n ? L[(size_t)(i + 1) * n + j] : 0.0;
-      }
-    }
-    double part = 0.0;
+  for (int u = 0; u < PF; ++u)
 #pragma unroll
     for (int q = 0; q < kTriPL; ++q) {
-      const int j = lane + 64 * q;
-      if (j >= c && j < i) part += cur[q] * w[j];
+      const int k = lane + 64 * q, row = min(c + u, n - 1);
+      buf[u][q] = LT[(size_t)row * n + min(k, n - 1)];
     }
-    part = wave_sum_pre(part);
-    double diag = 0.0;
+  for (int i0 = c; i0 < n; i0 += PF) {
 #pragma unroll
-    for (int q = 0; q < kTriPL; ++q)
-      if (lane + 64 * q == i) diag = cur[q];
-    diag = __shfl(diag, i & 63);
-    const double v = (((i == c) ? 1.0 : 0.0) - part) / diag;
-    if ((i & 63) == lane) w[i] = v;
-    if (lane == 0) W[(size_t)i * n + c] = v;
+    for (int u = 0; u < PF; ++u) {
+      const int i = i0 + u;
+      if (i < n) {
+        const int qi = i >> 6, li = i & 63;
+        double si = 0.0, di = 0.0;
 #pragma unroll
-    for (int q = 0; q < kTriPL; ++q) cur[q] = nxt[q];
+        for (int q = 0; q < kTriPL; ++q)
+          if (q == qi) {
+            si = sk[q];
+            di = buf[u][q];
+          }
+        si = readlane_f64(si, li);
+        di = readlane_f64(di, li);  // L_ii
+        const double x = (((i == c) ? 1.0 : 0.0) - si) / di;
+        if (lane == 0) W[(size_t)i * n + c] = x;
+#pragma unroll
+        for (int q = 0; q < kTriPL; ++q) {
+          const int k = lane + 64 * q;
+          const double upd = sk[q] + buf[u][q] * x;
+          sk[q] = k > i ? upd : sk[q];
+        }
+        const int row = min(i + PF, n - 1);
+#pragma unroll
+        for (int q = 0; q < kTriPL; ++q) buf[u][q] = LT[(size_t)row * n + min(lane + 64 * q, n - 1)];
+      }
+    }
   }
   for (int i = lane; i < c; i += 64) W[(size_t)i * n + c] = 0.0;
 }
 
-// Ainv = Linv^T Linv (symmetric)
-__global__ void k_gram(int n, const double* __restrict__ Li, double* __restrict__ Ainv) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long long)n * n) return;
-  const int i = (int)(t / n), j = (int)(t % n);
-  const int k0 = i > j ? i : j;
-  double s = 0.0;
-  for (int k = k0; k < n; ++k) s += Li[(size_t)k * n + i] * Li[(size_t)k * n + j];
-  Ainv[t] = s;
+// Ainv = Linv^T Linv (symmetric), Linv lower triangular (row-major, zeros
+// above the diagonal).  One 32x32 tile (I, J), I <= J, per workgroup, 2x2
+// entries per thread, k from the tile's first nonzero row in chunks of 32
+// staged through LDS; the tile and its mirror are written (a product and its
+// transpose are the same sums in the same order, so Ainv is exactly
+// symmetric).  The former one-thread-per-entry form read every operand from
+// global memory for each entry: 442 us at 630 unknowns, 5k poses.
+__global__ void __launch_bounds__(256) k_gram_tiles(int n, int T, const double* __restrict__ Li,
+                                                   double* __restrict__ Ainv) {
+  __shared__ double sa[32][33], sb[32][33];
+  int b = blockIdx.x, I = 0;
+  while (b >= T - I) {  // upper-triangle tiles numbered row by row
+    b -= T - I;
+    ++I;
+  }
+  const int J = I + b;
+  const int i0 = I * 32, j0 = J * 32;
+  const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15;
+  const int lr = tid >> 3, lc = (tid & 7) * 4;  // loader: row lr, columns lc .. lc + 3
+  double acc00 = 0.0, acc01 = 0.0, acc10 = 0.0, acc11 = 0.0;
+  for (int k0 = j0; k0 < n; k0 += 32) {
+    const int k = k0 + lr;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      sa[lr][lc + e] = (k < n && i0 + lc + e < n) ? Li[(size_t)k * n + i0 + lc + e] : 0.0;
+      sb[lr][lc + e] = (k < n && j0 + lc + e < n) ? Li[(size_t)k * n + j0 + lc + e] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int kk = 0; kk < 32; ++kk) {
+      const double a0 = sa[kk][2 * ti], a1 = sa[kk][2 * ti + 1];
+      const double b0 = sb[kk][2 * tj], b1 = sb[kk][2 * tj + 1];
+      acc00 += a0 * b0;
+      acc01 += a0 * b1;
+      acc10 += a1 * b0;
+      acc11 += a1 * b1;
+    }
+    __syncthreads();
+  }
+  const int ia = i0 + 2 * ti, ja = j0 + 2 * tj;
+  const double v[2][2] = {{acc00, acc01}, {acc10, acc11}};
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) {
+      const int i = ia + x, j = ja + y;
+      if (i < n && j < n) {
+        Ainv[(size_t)i * n + j] = v[x][y];
+        Ainv[(size_t)j * n + i] = v[x][y];
+      }
+    }
 }
 
 // y = (H + lambda damp) x : one thread per pose row (deterministic order)
@@ -1574,6 +1665,7 @@ int SolverRun::coarse_setup(hipStream_t st, double lambda, int* Sc_out, int* nse
   if ((rc_ = Pb.reserve(sizeof(float) * 36 * (size_t)N))) return rc_;
   if ((rc_ = Ac.reserve(sizeof(double) * (size_t)nc * nc))) return rc_;
   if ((rc_ = Lci.reserve(sizeof(double) * (size_t)nc * nc))) return rc_;
+  if ((rc_ = LcT.reserve(sizeof(double) * (size_t)nc * nc))) return rc_;
   if ((rc_ = Ainv.reserve(sizeof(double) * (size_t)nc * nc))) return rc_;
   k_coarse_basis<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(N, Sc, x.as<double>(), Pb.as<float>());
   RPGO_HIP_CHECK(hipMemsetAsync(Ac.ptr, 0, sizeof(double) * (size_t)nc * nc, st));
@@ -1584,15 +1676,18 @@ int SolverRun::coarse_setup(hipStream_t st, double lambda, int* Sc_out, int* nse
                                                              fj.as<int>(), Hij.as<double>(),
                                                              Pb.as<float>(), Ac.as<double>());
   for (int k0 = 0; k0 < nc; k0 += kCholNb) {
-    k_chol_panel<<<1, 1024, 0, st>>>(nc, k0, Ac.as<double>());
+    k_chol_panel<<<1, 64, 0, st>>>(nc, k0, Ac.as<double>(), LcT.as<double>());
     const int rest = nc - k0 - kCholNb;
     if (rest > 0) {
       const int T = (rest + 31) / 32;
-      k_chol_update<<<T * (T + 1) / 2, 256, 0, st>>>(nc, k0, Ac.as<double>());
+      k_chol_update<<<T * (T + 1) / 2, 256, 0, st>>>(nc, k0, Ac.as<double>(), LcT.as<double>());
     }
   }
-  k_tri_inv<<<(nc + 3) / 4, 256, 0, st>>>(nc, Ac.as<double>(), Lci.as<double>());
-  k_gram<<<grid_for((long long)nc * nc, 256, 1 << 22), 256, 0, st>>>(nc, Lci.as<double>(), Ainv.as<double>());
+  k_tri_inv<<<(nc + 3) / 4, 256, 0, st>>>(nc, LcT.as<double>(), Lci.as<double>());
+  {
+    const int T = (nc + 31) / 32;
+    k_gram_tiles<<<T * (T + 1) / 2, 256, 0, st>>>(nc, T, Lci.as<double>(), Ainv.as<double>());
+  }
   RPGO_HIP_CHECK(hipGetLastError());
   *Sc_out = Sc;
   *nseg_out = nseg;
