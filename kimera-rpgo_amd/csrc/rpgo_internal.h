@@ -139,6 +139,7 @@ struct rpgo_ctx {
   rpgo::BatchState* batch = nullptr;  // created by rpgo_pcm_batch_run
   void* host_pinned = nullptr;        // 256 B pinned host buffer for counter readbacks
   int n_cu = 256;                     // compute units of the device (launch shapes)
+  bool stage_timing = true;           // HIP events around the PCM pass stages (ms_* of the stats)
 };
 
 namespace rpgo {

@@ -133,6 +133,7 @@ SIGNATURES = [
     ("rpgo_ctx_create", [ctypes.c_int, ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
     ("rpgo_ctx_destroy", [ctypes.c_void_p], ctypes.c_int),
     ("rpgo_ctx_sync", [ctypes.c_void_p], ctypes.c_int),
+    ("rpgo_ctx_set_stage_timing", [ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
     ("rpgo_last_error", [], ctypes.c_char_p),
     ("rpgo_version", [], ctypes.c_char_p),
     ("rpgo_pcm_upload", [ctypes.c_void_p, ctypes.POINTER(PcmProblem)], ctypes.c_int),
@@ -223,6 +224,13 @@ class Context:
         if self._h:
             lib().rpgo_ctx_destroy(self._h)
             self._h = ctypes.c_void_p()
+
+    def sync(self):
+        check(lib().rpgo_ctx_sync(self._h))
+
+    def set_stage_timing(self, enabled: bool):
+        """HIP events around the PCM pass stages (the ms_* stats; on by default)."""
+        check(lib().rpgo_ctx_set_stage_timing(self._h, 1 if enabled else 0))
 
     def __enter__(self):
         return self

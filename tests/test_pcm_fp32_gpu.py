@@ -133,6 +133,30 @@ def test_pairs_rerun_matches_fresh_run(ctx):
     assert np.array_equal(ctx.pcm_download()["adj_bits"], a2)
 
 
+def test_stage_timing_off_same_results(ctx):
+    """rpgo_ctx_set_stage_timing(0): passes record no events, the ms_* fields of
+    the pass read 0, and adjacency and counts equal the timed passes'."""
+    g, ot, lt = synth.workload("se3_5k", seed=4)
+    ctx.pcm_upload(g.dim, g.pose0, g.odom, g.odom_cov, g.lc_from, g.lc_to, g.lc_pose, g.lc_cov)
+    ctx.pcm_run(ot, lt, precision=FP32)
+    ref = ctx.pcm_download()["adj_bits"].copy()
+    on = ctx.pcm_pairs(lt, precision=FP32)
+    assert on.ms_pairs > 0 and on.ms_total > 0
+    try:
+        ctx.set_stage_timing(False)
+        for prec in (FP32, _capi.PCM_FP64_REPLAY):
+            off = ctx.pcm_pairs(lt, precision=prec)
+            assert off.ms_pairs == 0 and off.ms_recheck == 0 and off.ms_total == 0
+            assert off.n_consistent == on.n_consistent
+            assert np.array_equal(ctx.pcm_download()["adj_bits"], ref)
+        run = ctx.pcm_run(ot, lt, precision=FP32)
+        assert run.ms_total == 0 and run.n_consistent == on.n_consistent
+        assert np.array_equal(ctx.pcm_download()["adj_bits"], ref)
+    finally:
+        ctx.set_stage_timing(True)
+    assert ctx.pcm_pairs(lt, precision=FP32).ms_pairs > 0
+
+
 def test_fp32_large_graph_full_adjacency(ctx):
     """50,000 poses / 10,000 loop closures: all 49,995,000 pairs of the FP32
     path against the FP64 oracle (oracle row-parallel on the host cores)."""
