@@ -15,6 +15,7 @@ PKG = os.path.dirname(HERE)
 LIB_PATH = os.path.join(PKG, "lib", "librpgo_hip.so")
 
 RPGO_OK = 0
+RPGO_ERR_ARG, RPGO_ERR_HIP, RPGO_ERR_NOMEM, RPGO_ERR_STATE = -1, -2, -3, -4
 PCM_FP64_REPLAY = 0
 PCM_FP32_RECHECK = 1
 CLIQUE_HEU_REF = 0

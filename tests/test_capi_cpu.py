@@ -37,6 +37,12 @@ def test_version_and_error_string_without_gpu():
     assert L.rpgo_last_error() is not None
 
 
+def test_stage_timing_rejects_null_context():
+    L = _capi.lib()
+    assert L.rpgo_ctx_set_stage_timing(None, 0) == _capi.RPGO_ERR_ARG
+    assert b"NULL context" in L.rpgo_last_error()
+
+
 def test_bit_packing_roundtrip():
     rng = np.random.default_rng(0)
     for m in (1, 5, 63, 64, 65, 200):
