@@ -399,7 +399,7 @@ int rpgo_solve(rpgo_ctx* ctx, const rpgo_solve_problem* prob, const rpgo_solve_p
  * Learned GNC initialisation.  No counterpart in the reference: an addition
  * of this back end that predicts, per loop closure, the GNC weight the plain
  * method converges to (as a logit) and log(mu) of the final GNC iteration,
- * from features of the PCM state.  Plain GNC stays the judge: callers start
+ * from features of the PCM state.  Plain GNC stays the arbiter: callers start
  * rpgo_solve from the prediction (init_weights, init_mu) and fall back to the
  * plain solve when the result fails their check.
  * MLP 16 -> 128 -> 128 -> 2 with ReLU; BF16 matrix-core products with FP32

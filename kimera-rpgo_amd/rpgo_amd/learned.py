@@ -9,7 +9,7 @@ Pipeline (one SE(3) pose graph, all stages on the GPU through the C ABI):
   3. learned: per-loop-closure features (rpgo_lc_features) -> MLP
      (rpgo_mlp_forward) -> initial weights (predicted inliers 1, outliers 0)
      and a starting mu a few GNC steps before the predicted final mu; the
-     solve starts there.  Plain GNC stays the judge: when the learned start
+     solve starts there.  Plain GNC stays the arbiter: when the learned start
      ends anywhere but at a GNC fixed point (report.n_tls_disagree > 0, or
      non-binary weights) the plain solve is run and returned instead.
 
