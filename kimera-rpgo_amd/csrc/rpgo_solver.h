@@ -14,10 +14,11 @@ constexpr int kCoarseMax = 6 * kCoarseSegMax;
 struct GraphKey {
   int N = 0, S = 0, Sc = 0, nseg_c = 0;
   const void *hist = nullptr, *p = nullptr, *Minv = nullptr, *Hij = nullptr, *inc = nullptr,
-             *Ainv = nullptr, *Pb = nullptr;
+             *inc_q = nullptr, *Ainv = nullptr, *Pb = nullptr;
   bool operator==(const GraphKey& o) const {
     return N == o.N && S == o.S && Sc == o.Sc && nseg_c == o.nseg_c && hist == o.hist && p == o.p &&
-           Minv == o.Minv && Hij == o.Hij && inc == o.inc && Ainv == o.Ainv && Pb == o.Pb;
+           Minv == o.Minv && Hij == o.Hij && inc == o.inc && inc_q == o.inc_q && Ainv == o.Ainv &&
+           Pb == o.Pb;
   }
 };
 
@@ -39,6 +40,7 @@ struct SolverRun {
   // Gauss-Newton system
   DevBuf Hii, Hjj, Hij, gi, gj;  // per-factor contributions
   DevBuf inc_off, inc_cnt, inc;  // pose -> incident factors
+  DevBuf inc_q;                  // per incidence: the factor's other pose (-1: not a between factor)
   DevBuf D, Minv, hdiag, b64;
   DevBuf chain, Useg;             // odometry chain links, preconditioner off-diagonals
   DevBuf dx, r, zv, p, q, hx;     // PCG vectors (FP64)
@@ -71,7 +73,7 @@ struct SolverRun {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     graph_exec = nullptr;
     for (DevBuf* v : {&hist,&x0, &x, &xn, &kind, &fi, &fj, &z, &info, &w, &barc, &known, &err, &cost, &Hii,
-                      &Hjj, &Hij, &gi, &gj, &inc_off, &inc_cnt, &inc, &D, &Minv, &hdiag, &b64,
+                      &Hjj, &Hij, &gi, &gj, &inc_off, &inc_cnt, &inc, &inc_q, &D, &Minv, &hdiag, &b64,
                       &chain, &Useg, &Pb, &Ac, &Lci, &LcT, &Ainv, &cres, &ccor,
                       &dx, &r, &zv, &p, &q, &hx, &sc})
       v->release();

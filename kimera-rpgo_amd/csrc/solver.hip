@@ -352,6 +352,19 @@ __global__ void k_inc_sort(int N, const int* __restrict__ off, int* __restrict__
   }
 }
 
+// the other pose of each incidence, -1 for a factor with no off-diagonal
+// block (kind != 0), so the row SpMV reads it together with the incidence
+__global__ void k_inc_other(int N, const int* __restrict__ off, const int* __restrict__ inc,
+                            const int* __restrict__ kind, const int* __restrict__ fi,
+                            const int* __restrict__ fj, int* __restrict__ inc_q) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  for (int k = off[p]; k < off[p + 1]; ++k) {
+    const int t = inc[k], f = t >> 1;
+    inc_q[k] = kind[f] != 0 ? -1 : ((t & 1) ? fi[f] : fj[f]);
+  }
+}
+
 // Diagonal blocks, gradient rhs b = -g and the Hessian diagonal, one thread
 // per pose, summing the incident factors in factor order (deterministic).
 __global__ void k_assemble(int N, const int* __restrict__ off, const int* __restrict__ inc,
@@ -936,14 +949,14 @@ __global__ void k_dot(long long n, const double* __restrict__ a, const double* _
 // ---------------------------------------------------------------------------
 constexpr int kScR = 0, kScPQ = 2, kScCnt = 32, kScLam = 30;
 constexpr int kSpmvPoseMin = 16384;  // poses from which the SpMV runs one thread per pose
+constexpr int kSpmvBatch = 4;        // incidences per pose whose loads the row SpMV issues together
 
 // K1: q = (H + lambda damp) p ; PQ[a] += p . q ; clear R[a^1], PQ[a^1].
 // Eight lanes per pose, lane r < 6 owning output row r: block rows (or
 // columns, for the transposed half) are read as 288 contiguous bytes per
 // group, and each row sums its products in the same order as a per-pose loop.
 __global__ void __launch_bounds__(256) k_pcg_spmv_rows(int N, int a, const int* __restrict__ off,
-                                                 const int* __restrict__ inc, const int* __restrict__ fi,
-                                                 const int* __restrict__ fj, const int* __restrict__ kind,
+                                                 const int* __restrict__ inc, const int* __restrict__ incq,
                                                  const double* __restrict__ D, const double* __restrict__ Hij,
                                                  const double* __restrict__ hdiag, int diag_damping,
                                                  double* __restrict__ sc, const double* __restrict__ x,
@@ -962,21 +975,41 @@ __global__ void __launch_bounds__(256) k_pcg_spmv_rows(int N, int a, const int* 
     for (int c = 0; c < 6; ++c) s += D[(size_t)p * 36 + rr * 6 + c] * __shfl(xr, base + c);
     const double d = diag_damping ? fmin(fmax(hdiag[(size_t)p * 6 + rr], 1e-6), 1e32) : 1.0;
     double acc = s + lambda * d * xr;
-    for (int k = off[p]; k < off[p + 1]; ++k) {
-      const int t = inc[k], f = t >> 1;
-      if (kind[f] != 0) continue;
-      const double* B = Hij + (size_t)f * 36;
-      const bool tr = (t & 1) != 0;
-      const double xq = x[(size_t)(tr ? fi[f] : fj[f]) * 6 + rr];
-      double v = 0.0;
-      if (!tr) {
+    // incident factors in order; the first kSpmvBatch with every load of the
+    // batch independent of the others (clamped indices, results selected): a
+    // pose's usual 2-4 factors cost three dependent round trips in all, where
+    // one at a time they cost about four each
+    const int k0 = off[p], k1 = off[p + 1];
+    if (k1 > k0) {
+      int tt[kSpmvBatch], qq[kSpmvBatch];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) v += B[rr * 6 + c] * __shfl(xq, base + c);
-      } else {
-#pragma unroll
-        for (int c = 0; c < 6; ++c) v += B[c * 6 + rr] * __shfl(xq, base + c);
+      for (int u = 0; u < kSpmvBatch; ++u) {
+        const int k = min(k0 + u, k1 - 1);
+        tt[u] = inc[k];
+        qq[u] = incq[k];
       }
-      acc += v;
+#pragma unroll
+      for (int u = 0; u < kSpmvBatch; ++u) {
+        const bool use = k0 + u < k1 && qq[u] >= 0;
+        const bool tr = (tt[u] & 1) != 0;
+        const double* B = Hij + (size_t)(tt[u] >> 1) * 36;
+        const double xq = x[(size_t)(qq[u] >= 0 ? qq[u] : p) * 6 + rr];
+        double v = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) v += B[tr ? c * 6 + rr : rr * 6 + c] * __shfl(xq, base + c);
+        acc = use ? acc + v : acc;
+      }
+      for (int k = k0 + kSpmvBatch; k < k1; ++k) {
+        const int t = inc[k], q = incq[k];
+        if (q < 0) continue;
+        const double* B = Hij + (size_t)(t >> 1) * 36;
+        const bool tr = (t & 1) != 0;
+        const double xq = x[(size_t)q * 6 + rr];
+        double v = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) v += B[tr ? c * 6 + rr : rr * 6 + c] * __shfl(xq, base + c);
+        acc += v;
+      }
     }
     if (r < 6) {
       y[(size_t)p * 6 + r] = acc;
@@ -1544,6 +1577,7 @@ int SolverRun::alloc(int N_, int F_) {
   if ((rc = inc_off.reserve(sizeof(int) * (N + 1)))) return rc;
   if ((rc = inc_cnt.reserve(sizeof(int) * (N + 1)))) return rc;
   if ((rc = inc.reserve(sizeof(int) * 2 * F))) return rc;
+  if ((rc = inc_q.reserve(sizeof(int) * 2 * F))) return rc;
   if ((rc = D.reserve(sizeof(double) * 36 * N))) return rc;
   if ((rc = Minv.reserve(sizeof(float) * 36 * N))) return rc;
   if ((rc = Useg.reserve(sizeof(float) * 36 * N))) return rc;
@@ -1586,6 +1620,8 @@ int SolverRun::build_incidence(hipStream_t st) {
                                                        inc_off.as<int>(), inc_cnt.as<int>(),
                                                        inc.as<int>());
   k_inc_sort<<<grid_for(N, 256, 1 << 20), 256, 0, st>>>(N, inc_off.as<int>(), inc.as<int>());
+  k_inc_other<<<grid_for(N, 256, 1 << 20), 256, 0, st>>>(N, inc_off.as<int>(), inc.as<int>(), kind.as<int>(),
+                                                         fi.as<int>(), fj.as<int>(), inc_q.as<int>());
   k_chain_init<<<grid_for(N + 1, 256, 1 << 20), 256, 0, st>>>(N + 1, chain.as<int>());
   k_chain_find<<<grid_for(F, 256, 1 << 20), 256, 0, st>>>(F, kind.as<int>(), fi.as<int>(), fj.as<int>(),
                                                           chain.as<int>());
@@ -1708,7 +1744,7 @@ void SolverRun::pcg_step(hipStream_t st, int S, int Sc, int nseg_c, int first, i
   // (5k poses: 46 vs 49 us per PCG iteration; 50k: 78 vs 69 us)
   if (!first && N < kSpmvPoseMin)
     k_pcg_spmv_rows<<<grid_for((long long)N * 8, 256, 1 << 22), 256, 0, st>>>(
-        N, a, inc_off.as<int>(), inc.as<int>(), fi.as<int>(), fj.as<int>(), kind.as<int>(),
+        N, a, inc_off.as<int>(), inc.as<int>(), inc_q.as<int>(),
         D.as<double>(), Hij.as<double>(), hdiag.as<double>(), prm.lm_diagonal_damping, s, p.as<double>(),
         q.as<double>());
   else if (!first)
@@ -1791,6 +1827,7 @@ int SolverRun::pcg(hipStream_t st, double lambda, int* iters_out) {
                      (const void*)Minv.ptr,
                      (const void*)Hij.ptr,
                      (const void*)inc.ptr,
+                     (const void*)inc_q.ptr,
                      nseg_c ? (const void*)Ainv.ptr : nullptr,
                      nseg_c ? (const void*)Pb.ptr : nullptr};
   const bool graphs = use_graphs && prm.pcg_graphs != 0;
