@@ -357,8 +357,9 @@ typedef struct {
   int32_t gnc_warm_start;       /* 0: every inner solve restarts from the initial
                                    values (GncOptimizer); 1: from the last result  */
   int32_t pcg_coarse;           /* rigid-segment coarse level of the preconditioner:
-                                   < 0 off; 0 auto (up to 128 segments);
-                                   > 0 at most that many segments (<= 128)        */
+                                   < 0 off; 0 auto (up to 128 segments below 10,000
+                                   poses, 256 from there); > 0 at most that many
+                                   segments (<= 256)                              */
   int32_t pcg_graphs;           /* 1: replay PCG iterations as captured HIP graphs;
                                    0: direct launches of the same kernels          */
   int32_t gnc_factor_dim;       /* degrees of freedom of the chi^2 threshold in

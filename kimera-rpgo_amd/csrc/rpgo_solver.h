@@ -8,7 +8,9 @@
 namespace rpgo {
 
 constexpr int kPcgChunk = 8;       // PCG iterations per captured graph launch
-constexpr int kCoarseSegMax = 128;  // rigid segments of the coarse preconditioner level
+constexpr int kCoarseSegMax = 256;  // rigid segments of the coarse preconditioner level (maximum)
+constexpr int kCoarseSegSmall = 128;  // default below kCoarseLargeN poses
+constexpr int kCoarseLargeN = 10000;  // poses from which the default is kCoarseSegMax
 constexpr int kCoarseMax = 6 * kCoarseSegMax;
 
 struct GraphKey {

@@ -763,26 +763,26 @@ __global__ void __launch_bounds__(256) k_chol_update(int n, int k0, double* __re
 // row i of LT (coalesced), prefetched PF rows ahead.  The row-oriented form
 // (w_i from a dot product over j < i) needed a wave reduction per row:
 // 691 us at 630 unknowns, 5k poses.
-constexpr int kTriPL = kCoarseMax / 64;  // rows per lane
 __device__ __forceinline__ double readlane_f64(double v, int lane) {
   const long long b = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
   const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
+template <int PL>  // rows per lane: n <= 64 PL
 __global__ void __launch_bounds__(256) k_tri_inv(int n, const double* __restrict__ LT, double* __restrict__ W) {
   constexpr int PF = 4;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int c = blockIdx.x * 4 + wv;
   if (c >= n) return;
-  double sk[kTriPL];
-  double buf[PF][kTriPL];
+  double sk[PL];
+  double buf[PF][PL];
 #pragma unroll
-  for (int q = 0; q < kTriPL; ++q) sk[q] = 0.0;
+  for (int q = 0; q < PL; ++q) sk[q] = 0.0;
 #pragma unroll
   for (int u = 0; u < PF; ++u)
 #pragma unroll
-    for (int q = 0; q < kTriPL; ++q) {
+    for (int q = 0; q < PL; ++q) {
       const int k = lane + 64 * q, row = min(c + u, n - 1);
       buf[u][q] = LT[(size_t)row * n + min(k, n - 1)];
     }
@@ -794,7 +794,7 @@ __global__ void __launch_bounds__(256) k_tri_inv(int n, const double* __restrict
         const int qi = i >> 6, li = i & 63;
         double si = 0.0, di = 0.0;
 #pragma unroll
-        for (int q = 0; q < kTriPL; ++q)
+        for (int q = 0; q < PL; ++q)
           if (q == qi) {
             si = sk[q];
             di = buf[u][q];
@@ -804,14 +804,14 @@ __global__ void __launch_bounds__(256) k_tri_inv(int n, const double* __restrict
         const double x = (((i == c) ? 1.0 : 0.0) - si) / di;
         if (lane == 0) W[(size_t)i * n + c] = x;
 #pragma unroll
-        for (int q = 0; q < kTriPL; ++q) {
+        for (int q = 0; q < PL; ++q) {
           const int k = lane + 64 * q;
           const double upd = sk[q] + buf[u][q] * x;
           sk[q] = k > i ? upd : sk[q];
         }
         const int row = min(i + PF, n - 1);
 #pragma unroll
-        for (int q = 0; q < kTriPL; ++q) buf[u][q] = LT[(size_t)row * n + min(lane + 64 * q, n - 1)];
+        for (int q = 0; q < PL; ++q) buf[u][q] = LT[(size_t)row * n + min(lane + 64 * q, n - 1)];
       }
     }
   }
@@ -1690,7 +1690,11 @@ int SolverRun::coarse_setup(hipStream_t st, double lambda, int* Sc_out, int* nse
   *Sc_out = 0;
   *nseg_out = 0;
   if (prm.pcg_coarse < 0 || N < 2) return RPGO_OK;
-  const int segs_max = prm.pcg_coarse > 0 ? prm.pcg_coarse : kCoarseSegMax;
+  // more rigid segments cut PCG iterations but cost nc^2 per iteration and
+  // nc^3 per set-up: on SE(3) helices with 50 % outliers, plain GNC with
+  // 128 / 256 segments took 1.44 / 1.48 s at 5k poses, 7.02 / 6.68 s at 20k
+  // and 42.4 / 39.7 s at 50k
+  const int segs_max = prm.pcg_coarse > 0 ? prm.pcg_coarse : (N < kCoarseLargeN ? kCoarseSegSmall : kCoarseSegMax);
   const int Sfine = fine_segment(prm, N);
   int Sc = std::max(Sfine, (N + segs_max - 1) / segs_max);
   Sc = (Sc + Sfine - 1) / Sfine * Sfine;  // whole fine segments per coarse segment
@@ -1719,7 +1723,10 @@ int SolverRun::coarse_setup(hipStream_t st, double lambda, int* Sc_out, int* nse
       k_chol_update<<<T * (T + 1) / 2, 256, 0, st>>>(nc, k0, Ac.as<double>(), LcT.as<double>());
     }
   }
-  k_tri_inv<<<(nc + 3) / 4, 256, 0, st>>>(nc, LcT.as<double>(), Lci.as<double>());
+  if (nc <= 64 * 12)
+    k_tri_inv<12><<<(nc + 3) / 4, 256, 0, st>>>(nc, LcT.as<double>(), Lci.as<double>());
+  else
+    k_tri_inv<kCoarseMax / 64><<<(nc + 3) / 4, 256, 0, st>>>(nc, LcT.as<double>(), Lci.as<double>());
   {
     const int T = (nc + 31) / 32;
     k_gram_tiles<<<T * (T + 1) / 2, 256, 0, st>>>(nc, T, Lci.as<double>(), Ainv.as<double>());
