@@ -379,6 +379,45 @@ def barc_sq(g: Graph, p: GncParams):
     return np.full(g.F, p.inlier_threshold)
 
 
+def tls_mu_init(r0, barc):
+    """GncOptimizer::initializeMu for TLS (Remark 5 of the GNC paper, per-factor
+    thresholds): the smallest barc_k / (2 r_k - barc_k) over factors with
+    r_k > barc_k / 2 at the initial values, floored at 1e-6; -1 when no factor
+    qualifies (GNC then stops before its first iteration)."""
+    mu = math.inf
+    for k in range(len(r0)):
+        if 2 * r0[k] - barc[k] > 0:
+            mu = min(mu, barc[k] / (2 * r0[k] - barc[k]))
+    if 0 <= mu < 1e-6:
+        mu = 1e-6
+    return mu if (mu > 0 and not math.isinf(mu)) else -1.0
+
+
+def tls_weights(r, mu, barc, known):
+    """GncOptimizer::calculateWeights for TLS: w = sqrt(barc mu (mu+1) / r) - mu
+    inside the band (mu/(mu+1) barc, (mu+1)/mu barc), 1 at or below it, 0 at or
+    above it; known inliers keep weight 1."""
+    F = len(r)
+    w = np.ones(F)
+    ub = (mu + 1) / mu * barc
+    lb = mu / (mu + 1) * barc
+    with np.errstate(divide="ignore", invalid="ignore"):
+        wn = np.sqrt(barc * mu * (mu + 1) / r) - mu
+    wn = np.where((r >= ub) | (wn < 0), 0.0, np.where((r <= lb) | (wn > 1), 1.0, wn))
+    unk = ~known
+    w[unk] = wn[unk]
+    return w
+
+
+def gnc_converged(cost, prev_cost, w, p: GncParams):
+    """GncOptimizer::checkConvergence for TLS: relative cost change below
+    relativeCostTol, or every weight within weightsTol of 0 or 1 (the mu test
+    applies to GM only)."""
+    cost_conv = abs(cost - prev_cost) / max(prev_cost, 1e-7) < p.relative_cost_tol
+    w_conv = bool(np.all(np.abs(w - np.round(w)) <= p.weights_tol))
+    return cost_conv or w_conv
+
+
 def gnc_optimize(g: Graph, p: GncParams, lm: LMParams = LMParams(), gauss_newton=False,
                  polish=None):
     """Returns dict(R, t, weights, iterations, mu_trace, cost)."""
@@ -393,35 +432,18 @@ def gnc_optimize(g: Graph, p: GncParams, lm: LMParams = LMParams(), gauss_newton
     if p.init_mu is not None:
         mu = float(p.init_mu)
     else:
-        r0 = factor_errors(g, R0, t0)
-        mu = math.inf
-        for k in range(F):
-            if 2 * r0[k] - barc[k] > 0:
-                mu = min(mu, barc[k] / (2 * r0[k] - barc[k]))
-        if 0 <= mu < 1e-6:
-            mu = 1e-6
-        mu = mu if (mu > 0 and not math.isinf(mu)) else -1.0
+        mu = tls_mu_init(factor_errors(g, R0, t0), barc)
     prev_cost = float(np.dot(w, factor_errors(g, R, t)))
     n_unknown = F - int(known.sum())
     mu_trace = [mu]
     iters = 0
     if not (mu <= 0 or n_unknown == 0):
         for it in range(p.max_iterations):
-            r = factor_errors(g, R, t)
-            w = np.ones(F)
-            unk = ~known
-            ub = (mu + 1) / mu * barc
-            lb = mu / (mu + 1) * barc
-            with np.errstate(divide="ignore", invalid="ignore"):
-                wn = np.sqrt(barc * mu * (mu + 1) / r) - mu
-            wn = np.where((r >= ub) | (wn < 0), 0.0, np.where((r <= lb) | (wn > 1), 1.0, wn))
-            w[unk] = wn[unk]
+            w = tls_weights(factor_errors(g, R, t), mu, barc, known)
             R, t, _, _ = lm_optimize(g, R0, t0, w, lm, gauss_newton)
             cost = float(np.dot(w, factor_errors(g, R, t)))
             iters = it + 1
-            cost_conv = abs(cost - prev_cost) / max(prev_cost, 1e-7) < p.relative_cost_tol
-            w_conv = bool(np.all(np.abs(w - np.round(w)) <= p.weights_tol))
-            if cost_conv or w_conv:
+            if gnc_converged(cost, prev_cost, w, p):
                 break
             mu = mu * p.mu_step
             mu_trace.append(mu)
