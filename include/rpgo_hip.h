@@ -229,6 +229,22 @@ int rpgo_max_clique(rpgo_ctx* ctx, int32_t mode, int32_t* idx_out, int32_t* k_ou
 /* Max clique of a caller-provided bit-packed symmetric adjacency (n vertices). */
 int rpgo_max_clique_bits(rpgo_ctx* ctx, int32_t n, const uint64_t* adj_bits,
                          int32_t mode, int32_t* idx_out, int32_t* k_out);
+/* Allocation-free variant of rpgo_max_clique_bits(ctx, n, adj, RPGO_CLIQUE_HEU_REF,
+ * idx_out, k_out).  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (serving loops, harnesses that forbid
+ * hipMalloc, code that must not grow the device heap).  The adjacency is read
+ * from caller-owned DEVICE memory (d_adj_bits: n rows of ceil(n/64) words,
+ * same layout as above); idx_out (host, room for n ints, may be NULL) and
+ * *k_out receive what rpgo_max_clique_bits returns.  The scan's working
+ * memory (degrees, degree filter, candidate sizes, the replayed buffer) is a
+ * fixed library-owned device array of about 400 KB per GPU, held until the
+ * process ends; no device memory is allocated or freed by the call.  n above
+ * RPGO_NOALLOC_MAX_VERTICES is rejected with RPGO_ERR_ARG.  Every call on one
+ * GPU shares that GPU's array: calls that overlap on one GPU (several
+ * contexts or threads) race and must be serialised by the caller. */
+#define RPGO_NOALLOC_MAX_VERTICES 32768
+int rpgo_max_clique_bits_noalloc(rpgo_ctx* ctx, int32_t n, const uint64_t* d_adj_bits,
+                                 int32_t* idx_out, int32_t* k_out);
 /* Incremental heuristic over the last rpgo_pcm_run adjacency, replacing
  * findMaxCliqueHeuIncremental (src/utils/GraphUtils.cpp:30-44, FMC::
  * maxCliqueHeuIncremental findCliqueHeu.cpp:119-209): only the last num_new

@@ -75,6 +75,11 @@ int rpgo_ctx_create(int device, uint32_t flags, rpgo_ctx** out) {
     delete c;
     return RPGO_ERR_NOMEM;
   }
+  if (int rc = clique_noalloc_init(device)) {
+    c->pcm.counters.release();
+    delete c;
+    return rc;
+  }
   e = hipHostMalloc(&c->host_pinned, 256, hipHostMallocDefault);
   if (e != hipSuccess) {
     set_error("hipHostMalloc failed: %s", hipGetErrorString(e));
@@ -667,6 +672,16 @@ int rpgo_max_clique_bits(rpgo_ctx* ctx, int32_t n, const uint64_t* adj_bits, int
   int rc = upload(ctx->clq.adj, adj_bits, sizeof(uint64_t) * (size_t)n * W, ctx->stream);
   if (rc) return rc;
   return clique_run(ctx, ctx->clq.adj.as<uint64_t>(), n, mode, idx_out, k_out);
+}
+
+int rpgo_max_clique_bits_noalloc(rpgo_ctx* ctx, int32_t n, const uint64_t* d_adj_bits, int32_t* idx_out,
+                                 int32_t* k_out) {
+  if (!ctx || !k_out || n < 0 || (n > 0 && !d_adj_bits)) {
+    set_error("rpgo_max_clique_bits_noalloc: invalid argument");
+    return RPGO_ERR_ARG;
+  }
+  RPGO_HIP_CHECK(hipSetDevice(ctx->device));
+  return clique_run_noalloc(ctx, d_adj_bits, n, idx_out, k_out);
 }
 
 int rpgo_max_clique_incremental(rpgo_ctx* ctx, int32_t num_new, int32_t prev_size, int32_t* idx_out,

@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <mutex>
 #include <vector>
 
 namespace rpgo {
@@ -36,9 +37,12 @@ __global__ void k_degrees(int n, int W, const unsigned long long* __restrict__ a
   deg[v] = s;
 }
 
+// first_reset (may be null): thread 0 also sets *first_reset = n, the "no
+// winner" value k_first_better lowers, so the scan needs no host-to-device copy
 __global__ void k_dmask(int n, int W, int best, const int* __restrict__ deg,
-                        unsigned long long* __restrict__ dmask) {
+                        unsigned long long* __restrict__ dmask, int* __restrict__ first_reset) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (first_reset && w == 0) *first_reset = n;
   if (w >= W) return;
   unsigned long long b = 0ull;
   for (int k = 0; k < 64; ++k) {
@@ -564,14 +568,88 @@ int clique_prefetch_depth() { return kPrefetch; }
 
 int clique_max_vertices() { return 64 * 64 * kMaxWPL; }
 
+// Working memory of one heuristic scan over n vertices (W = ceil(n/64)):
+// deg n ints, dmask W words, icc n ints, out n + 1 ints, best 4 ints.
+struct ScanBufs {
+  int* deg;
+  unsigned long long* dmask;
+  int* icc;
+  int* out;
+  int* best;
+};
+
 // Greedy scan over candidates pos0 .. n-1 starting from the bound best0;
 // mode EXACT continues with the exact search, mode HEU_REF replays the winner.
+// bufs == nullptr: the context's grow-only buffers (allocated on demand).
 static int clique_scan(rpgo_ctx* ctx, const uint64_t* d_adj_u64, int n, int mode, int pos0, int best0,
-                       int32_t* idx_out, int32_t* k_out);
+                       int32_t* idx_out, int32_t* k_out, const ScanBufs* bufs = nullptr);
 
 int clique_run(rpgo_ctx* ctx, const uint64_t* d_adj_u64, int n, int mode, int32_t* idx_out,
                int32_t* k_out) {
   return clique_scan(ctx, d_adj_u64, n, mode, 0, -1, idx_out, k_out);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free heuristic scan (rpgo_max_clique_bits_noalloc).  Its working
+// memory is this file-scope device array, one copy per GPU, held until the
+// process ends.  Host code reaches it only through the address
+// hipGetSymbolAddress returns on each device (the array's name is a host-side
+// stand-in, not a device address); that lookup runs once per device, from
+// rpgo_ctx_create, with the device current.
+// ---------------------------------------------------------------------------
+constexpr int kNoallocMaxVertices = RPGO_NOALLOC_MAX_VERTICES;
+constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t kNaDeg = 0;
+constexpr size_t kNaDmask = kNaDeg + align256(sizeof(int) * kNoallocMaxVertices);
+constexpr size_t kNaIcc = kNaDmask + align256(sizeof(unsigned long long) * ((kNoallocMaxVertices + 63) / 64));
+constexpr size_t kNaOut = kNaIcc + align256(sizeof(int) * kNoallocMaxVertices);
+constexpr size_t kNaBest = kNaOut + align256(sizeof(int) * (kNoallocMaxVertices + 1));
+constexpr size_t kNaBytes = kNaBest + 256;
+static_assert(kNoallocMaxVertices <= 64 * 64 * kMaxWPL, "no-alloc maximum beyond the register bitset");
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int clique_noalloc_init(int device) {
+  if (device < 0 || device >= kMaxDevices) {
+    set_error("clique_noalloc_init: device %d beyond the %d supported", device, kMaxDevices);
+    return RPGO_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return RPGO_OK;
+  void* ptr = nullptr;
+  // the caller (rpgo_ctx_create) has made `device` current
+  RPGO_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[device] = ptr;
+  return RPGO_OK;
+}
+
+size_t clique_noalloc_bytes() { return kNaBytes; }
+
+int clique_run_noalloc(rpgo_ctx* ctx, const uint64_t* d_adj_u64, int n, int32_t* idx_out, int32_t* k_out) {
+  if (n > kNoallocMaxVertices) {
+    set_error("rpgo_max_clique_bits_noalloc: %d vertices exceed the maximum %d", n, kNoallocMaxVertices);
+    return RPGO_ERR_ARG;
+  }
+  void* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (ctx->device >= 0 && ctx->device < kMaxDevices) base = g_scratch_ptr[ctx->device];
+  }
+  if (!base) {
+    set_error("rpgo_max_clique_bits_noalloc: scratch of device %d not set up", ctx->device);
+    return RPGO_ERR_STATE;
+  }
+  unsigned char* b = static_cast<unsigned char*>(base);
+  const ScanBufs bufs{reinterpret_cast<int*>(b + kNaDeg), reinterpret_cast<unsigned long long*>(b + kNaDmask),
+                      reinterpret_cast<int*>(b + kNaIcc), reinterpret_cast<int*>(b + kNaOut),
+                      reinterpret_cast<int*>(b + kNaBest)};
+  return clique_scan(ctx, d_adj_u64, n, RPGO_CLIQUE_HEU_REF, 0, -1, idx_out, k_out, &bufs);
 }
 
 // FMC::maxCliqueHeuIncremental (findCliqueHeu.cpp:119-209) through
@@ -597,7 +675,7 @@ int clique_incremental(rpgo_ctx* ctx, const uint64_t* d_adj_u64, int n, int num_
 }
 
 static int clique_scan(rpgo_ctx* ctx, const uint64_t* d_adj_u64, int n, int mode, int pos0, int best0,
-                       int32_t* idx_out, int32_t* k_out) {
+                       int32_t* idx_out, int32_t* k_out, const ScanBufs* bufs) {
   CliqueState& s = ctx->clq;
   if (mode != RPGO_CLIQUE_HEU_REF && mode != RPGO_CLIQUE_EXACT) {
     set_error("rpgo_max_clique: unknown mode %d", mode);
@@ -613,34 +691,38 @@ static int clique_scan(rpgo_ctx* ctx, const uint64_t* d_adj_u64, int n, int mode
     return RPGO_ERR_ARG;
   }
   const unsigned long long* adj = reinterpret_cast<const unsigned long long*>(d_adj_u64);
-  int rc;
-  if ((rc = s.deg.reserve(sizeof(int) * n))) return rc;
-  if ((rc = s.dmask.reserve(sizeof(unsigned long long) * W))) return rc;
-  if ((rc = s.icc.reserve(sizeof(int) * n))) return rc;
-  if ((rc = s.out.reserve(sizeof(int) * (n + 1)))) return rc;
-  if ((rc = s.best.reserve(sizeof(int) * 4))) return rc;
+  ScanBufs b;
+  if (bufs) {
+    b = *bufs;
+  } else {
+    int rc;
+    if ((rc = s.deg.reserve(sizeof(int) * n))) return rc;
+    if ((rc = s.dmask.reserve(sizeof(unsigned long long) * W))) return rc;
+    if ((rc = s.icc.reserve(sizeof(int) * n))) return rc;
+    if ((rc = s.out.reserve(sizeof(int) * (n + 1)))) return rc;
+    if ((rc = s.best.reserve(sizeof(int) * 4))) return rc;
+    b = ScanBufs{s.deg.as<int>(), s.dmask.as<unsigned long long>(), s.icc.as<int>(), s.out.as<int>(),
+                 s.best.as<int>()};
+  }
   hipStream_t st = ctx->stream;
-  k_degrees<<<(n + 255) / 256, 256, 0, st>>>(n, W, adj, s.deg.as<int>());
+  k_degrees<<<(n + 255) / 256, 256, 0, st>>>(n, W, adj, b.deg);
   RPGO_HIP_CHECK(hipGetLastError());
   int best = best0, pos = pos0, win = -1, win_best = -1;
-  int* d_first = s.best.as<int>();
+  int* d_first = b.best;
   while (pos < n) {
-    k_dmask<<<(W + 255) / 256, 256, 0, st>>>(n, W, best, s.deg.as<int>(),
-                                              s.dmask.as<unsigned long long>());
+    k_dmask<<<(W + 255) / 256, 256, 0, st>>>(n, W, best, b.deg, b.dmask, d_first);
     const int rem = n - pos;
-    k_icc<<<(rem + 3) / 4, 256, 0, st>>>(n, W, pos, best, adj, s.deg.as<int>(),
-                                         s.dmask.as<unsigned long long>(), s.icc.as<int>());
+    k_icc<<<(rem + 3) / 4, 256, 0, st>>>(n, W, pos, best, adj, b.deg, b.dmask, b.icc);
     RPGO_HIP_CHECK(hipGetLastError());
-    int first = n;
-    RPGO_HIP_CHECK(hipMemcpyAsync(d_first, &first, sizeof(int), hipMemcpyHostToDevice, st));
-    k_first_better<<<(rem + 255) / 256, 256, 0, st>>>(n, pos, best, s.icc.as<int>(), d_first);
+    k_first_better<<<(rem + 255) / 256, 256, 0, st>>>(n, pos, best, b.icc, d_first);
     RPGO_HIP_CHECK(hipGetLastError());
     int got[1];
     RPGO_HIP_CHECK(hipMemcpyAsync(got, d_first, sizeof(int), hipMemcpyDeviceToHost, st));
     RPGO_HIP_CHECK(hipStreamSynchronize(st));
     if (got[0] >= n) break;
     int icc_c = 0;
-    RPGO_HIP_CHECK(hipMemcpy(&icc_c, s.icc.as<int>() + got[0], sizeof(int), hipMemcpyDeviceToHost));
+    RPGO_HIP_CHECK(hipMemcpyAsync(&icc_c, b.icc + got[0], sizeof(int), hipMemcpyDeviceToHost, st));
+    RPGO_HIP_CHECK(hipStreamSynchronize(st));
     win = got[0];
     win_best = best;
     best = icc_c;
@@ -652,20 +734,20 @@ static int clique_scan(rpgo_ctx* ctx, const uint64_t* d_adj_u64, int n, int mode
   }
   if (win >= 0) {
     // dmask for the winner's threshold
-    k_dmask<<<(W + 255) / 256, 256, 0, st>>>(n, W, win_best, s.deg.as<int>(),
-                                              s.dmask.as<unsigned long long>());
-    k_replay<<<1, 64, 0, st>>>(n, W, win, win_best, adj, s.dmask.as<unsigned long long>(),
-                               s.out.as<int>(), s.best.as<int>() + 1);
+    k_dmask<<<(W + 255) / 256, 256, 0, st>>>(n, W, win_best, b.deg, b.dmask, nullptr);
+    k_replay<<<1, 64, 0, st>>>(n, W, win, win_best, adj, b.dmask, b.out, b.best + 1);
     RPGO_HIP_CHECK(hipGetLastError());
     int icc_r = 0;
-    RPGO_HIP_CHECK(hipMemcpyAsync(&icc_r, s.best.as<int>() + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    RPGO_HIP_CHECK(hipMemcpyAsync(&icc_r, b.best + 1, sizeof(int), hipMemcpyDeviceToHost, st));
     RPGO_HIP_CHECK(hipStreamSynchronize(st));
     if (icc_r != best) {
       set_error("rpgo_max_clique: replay size %d differs from the scan's %d", icc_r, best);
       return RPGO_ERR_STATE;
     }
-    if (idx_out && best > 0)
-      RPGO_HIP_CHECK(hipMemcpy(idx_out, s.out.as<int>(), sizeof(int) * best, hipMemcpyDeviceToHost));
+    if (idx_out && best > 0) {
+      RPGO_HIP_CHECK(hipMemcpyAsync(idx_out, b.out, sizeof(int) * best, hipMemcpyDeviceToHost, st));
+      RPGO_HIP_CHECK(hipStreamSynchronize(st));
+    }
   }
   *k_out = best;
   return RPGO_OK;

@@ -223,4 +223,9 @@ int clique_exact_run(rpgo_ctx* ctx, const unsigned long long* adj, int n, int lo
 // clique.hip
 int clique_run(rpgo_ctx* ctx, const uint64_t* d_adj, int n, int mode, int32_t* idx_out,
                int32_t* k_out);
+// allocation-free heuristic scan on the per-device scratch array
+// (clique_noalloc_init once per device, with that device current)
+int clique_noalloc_init(int device);
+size_t clique_noalloc_bytes();
+int clique_run_noalloc(rpgo_ctx* ctx, const uint64_t* d_adj, int n, int32_t* idx_out, int32_t* k_out);
 }  // namespace rpgo

@@ -20,6 +20,7 @@ PCM_FP64_REPLAY = 0
 PCM_FP32_RECHECK = 1
 CLIQUE_HEU_REF = 0
 CLIQUE_EXACT = 1
+NOALLOC_MAX_VERTICES = 32768  # RPGO_NOALLOC_MAX_VERTICES
 
 _d = ctypes.POINTER(ctypes.c_double)
 _i32 = ctypes.POINTER(ctypes.c_int32)
@@ -155,6 +156,8 @@ SIGNATURES = [
                                    ctypes.POINTER(PcmStats)], ctypes.c_int),
     ("rpgo_max_clique", [ctypes.c_void_p, ctypes.c_int32, _i32, _i32, _d], ctypes.c_int),
     ("rpgo_max_clique_bits", [ctypes.c_void_p, ctypes.c_int32, _u64, ctypes.c_int32, _i32, _i32],
+     ctypes.c_int),
+    ("rpgo_max_clique_bits_noalloc", [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, _i32, _i32],
      ctypes.c_int),
     ("rpgo_max_clique_incremental", [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _i32, _i32],
      ctypes.c_int),
@@ -515,6 +518,15 @@ class Context:
         idx = np.zeros(max(n, 1), np.int32)
         k = ctypes.c_int32()
         check(lib().rpgo_max_clique_bits(self._h, n, _ptr(a, _u64), mode, _ptr(idx, _i32), ctypes.byref(k)))
+        return int(k.value), idx[: max(k.value, 0)].copy()
+
+    def max_clique_bits_noalloc(self, d_adj_bits, n):
+        """rpgo_max_clique_bits_noalloc: heuristic clique of the adjacency at
+        device address d_adj_bits (n rows of ceil(n/64) words, caller owned)."""
+        idx = np.zeros(max(n, 1), np.int32)
+        k = ctypes.c_int32()
+        check(lib().rpgo_max_clique_bits_noalloc(self._h, n, ctypes.c_void_p(int(d_adj_bits)),
+                                                 _ptr(idx, _i32), ctypes.byref(k)))
         return int(k.value), idx[: max(k.value, 0)].copy()
 
 

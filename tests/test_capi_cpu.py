@@ -53,3 +53,10 @@ def test_bit_packing_roundtrip():
         # bit (i & 63) of word i >> 6
         j, i = (int(v) for v in rng.integers(0, m, 2))
         assert ((int(bits[j, i >> 6]) >> (i & 63)) & 1) == A[j, i]
+
+
+def test_noalloc_clique_rejects_null_context():
+    L = _capi.lib()
+    k = ctypes.c_int32()
+    assert L.rpgo_max_clique_bits_noalloc(None, 10, None, None, ctypes.byref(k)) == _capi.RPGO_ERR_ARG
+    assert b"invalid argument" in L.rpgo_last_error()
