@@ -1,18 +1,24 @@
-"""PCM pair-check throughput on one MI355X (device-resident inputs).
+"""PCM pair-check throughput and robust-solve time on one MI355X
+(device-resident inputs).
 
-Workload (default): SE(3) helix, 5,000 poses, 1,000 loop closures (20 %
-outliers), odometry check off (odom_threshold = -1) so all 499,500 unique
-pairs (i<j) are evaluated, lc_threshold = 5.0.  A step is one pass of the PCM
-pairwise stage over the uploaded graph: FP32 consistency check of every pair,
-FP64 replay of the pairs near the threshold, adjacency bit packing, consistent
-pair count read back.  The odometry fold and check run once at ingest (the
-reference folds the trajectory as odometry arrives) and are reported apart.
-The timed steps run without the library's stage timers (rpgo_ctx_set_stage_
-timing off); the kernel and stage times come from a second pass of the same
-number of steps with the timers on.
+Headline workload (default): SE(3) helix, 50,000 poses, 10,000 loop closures
+(50 % outliers), odometry check off (odom_threshold = -1) so all 49,995,000
+unique pairs (i<j) are evaluated, lc_threshold = 5.0.  A step is one pass of
+the PCM pairwise stage over the uploaded graph: FP32 consistency check of every
+pair, FP64 replay of the pairs near the threshold, adjacency bit packing,
+consistent-pair count read back.  The odometry fold and check run once at
+ingest (the reference folds the trajectory as odometry arrives) and are
+reported apart.  The timed steps run without the library's stage timers
+(rpgo_ctx_set_stage_timing off); the kernel and stage times come from a second
+pass of the same number of steps with the timers on.
 
-Prints one JSON line (metric, value, roofline of the dominant kernel, CPU
-baseline of the FP64 oracle on the host cores).
+Beside the headline: the 5,000-pose graph, the error of every FP32 stage
+against the FP64 path, the exact clique, the robust solve at 50,000 poses
+(PCM + plain GNC against the learned start, poses against the FP64 oracle's
+solution when its fixture is present), the 256-graph batch, and the CPU
+baseline on the host cores.
+
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -33,21 +39,26 @@ import numpy as np  # noqa: E402
 # 5 adjoints, sym add/sub, 2 PSD checks, Log, Cholesky Mahalanobis); SE(2) ~720.
 FLOP_PER_PAIR = {3: 4540.0, 2: 720.0}
 PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}
-# HBM bytes per k_pairs32 launch on se3_5k, from the rocprofv3 FETCH_SIZE and
-# WRITE_SIZE passes written up in this file (scaled by a plain copy's counts)
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_pairs32_se3_5k_traffic.md")
+HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_pairs32 launch, from the rocprofv3 FETCH_SIZE and WRITE_SIZE
+# passes written up in these files (scaled by a plain copy's counts)
+TRAFFIC_FILES = {"se3_5k": os.path.join(ROOT, "profiles", "k_pairs32_se3_5k_traffic.md"),
+                 "se3_50k": os.path.join(ROOT, "profiles", "k_pairs32_se3_50k_traffic.md")}
+# FP64 oracle solution of the 50k robust solve (tests/golden/make_solve_fixtures.py)
+SOLVE_FIXTURE = os.path.join(ROOT, "tests", "golden", "solve_se3_50k.npz")
 
 
-def measured_traffic():
-    """bytes per launch recorded in TRAFFIC_FILE ('traffic_bytes_per_launch: N'), or None"""
+def measured_traffic(workload):
+    """bytes per launch recorded for the workload ('traffic_bytes_per_launch: N'), or None"""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(TRAFFIC_FILES[workload]) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
                     return float(line.split(":", 1)[1])
-    except OSError:
+    except (OSError, KeyError):
         pass
     return None
+
 
 WORKLOADS = {
     "se3_5k": dict(kind="se3_helix", n_poses=5000, n_lc=1000, outlier_frac=0.2, odom_thr=-1.0, lc_thr=5.0),
@@ -55,31 +66,110 @@ WORKLOADS = {
 }
 
 
-def cpu_baseline(g, odom_thr, lc_thr, budget_s=15.0):
-    """FP64 oracle (single core) on a bounded row sample of the same graph."""
-    import oracle as O
+def describe(name):
+    w = WORKLOADS[name]
+    return (f"{name}: SE(3) helix {w['n_poses']} poses, {w['n_lc']} loop closures, "
+            f"{int(w['outlier_frac'] * 100)}% outliers, odom_thr {w['odom_thr']}, lc_thr {w['lc_thr']}")
 
-    O.set_threads(1)  # the reference is single-threaded
+
+def graph(name, seed):
+    from rpgo_amd import synth
+
+    w = WORKLOADS[name]
+    return synth.se3_helix(n_poses=w["n_poses"], n_lc=w["n_lc"], outlier_frac=w["outlier_frac"], seed=seed)
+
+
+def pcm_pass(ctx, g, w, prec, steps, warmup):
+    """Upload + ingest, then `steps` timed pairwise passes without stage timers
+    and `steps` with them (kernel and stage times)."""
+    ctx.pcm_upload(g.dim, g.pose0, g.odom, g.odom_cov, g.lc_from, g.lc_to, g.lc_pose, g.lc_cov)
+    ingest = ctx.pcm_run(w["odom_thr"], w["lc_thr"], precision=prec)
+    ctx.set_stage_timing(False)
+    for _ in range(warmup):
+        ctx.pcm_pairs(w["lc_thr"], precision=prec)
+    ctx.sync()
     t0 = time.perf_counter()
-    traj = O.fold(g.dim, g.pose0, g.odom, g.odom_cov)
-    t_fold = time.perf_counter() - t0
-    m = g.n_lc
-    # rows j in [0, j1): pairs j1*(j1-1)/2; grow j1 until the budget is spent
-    pairs = 0
-    t_pairs = 0.0
-    j0 = 1
-    step = 64
-    while j0 < m and t_pairs < budget_s:
-        j1 = min(m, j0 + step)
-        t = time.perf_counter()
-        O.pcm_dist_rows(g.dim, traj, g.lc_from[:j1], g.lc_to[:j1], g.lc_pose[:j1], g.lc_cov[:j1], j0, j1)
-        t_pairs += time.perf_counter() - t
-        pairs += (j1 * (j1 - 1) - j0 * (j0 - 1)) // 2
-        j0 = j1
-        step *= 2
-    return dict(value=pairs / t_pairs, unit="pair-checks/s", cores=1, kind="port",
-                sample=f"FP64 oracle, rows 1..{j0 - 1} of the workload ({pairs} pairs, "
-                       f"{t_pairs:.1f} s; fold {t_fold * 1e3:.1f} ms)")
+    for _ in range(steps):
+        last = ctx.pcm_pairs(w["lc_thr"], precision=prec)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    ctx.set_stage_timing(True)
+    stats = [ctx.pcm_pairs(w["lc_thr"], precision=prec) for _ in range(steps)]
+    ctx.sync()
+    assert stats[-1].n_consistent == last.n_consistent
+    return dict(ingest=ingest, stats=stats, ms_step=wall * 1e3 / steps, pairs=stats[-1].n_pairs,
+                ms_kernel=float(np.mean([s.ms_pairs for s in stats])),
+                ms_recheck=float(np.mean([s.ms_recheck for s in stats])),
+                ms_stage=float(np.mean([s.ms_total for s in stats])))
+
+
+def fp32_error(ctx, w):
+    """FP32 distances (with their FP64 replays) against the FP64 path on the
+    graph last uploaded.  The FP64 path evaluates every pair with the
+    reference's formula and operation order; tests/test_pcm_fp32_gpu.py shows
+    it equal to the FP64 CPU oracle bit for bit, so this is the error against
+    the FP64 oracle."""
+    from rpgo_amd import _capi
+
+    st32 = ctx.pcm_pairs(w["lc_thr"], precision=_capi.PCM_FP32_RECHECK, want_dist=True)
+    d32 = ctx.pcm_download(want_dist=True)
+    ctx.pcm_pairs(w["lc_thr"], precision=_capi.PCM_FP64_REPLAY, want_dist=True)
+    d64 = ctx.pcm_download(want_dist=True)
+    m = d64["dist"].shape[0]
+    iu = np.triu_indices(m, 1)
+    a, b = d32["dist"][iu], d64["dist"][iu]
+    rel = np.abs(a - b) / np.maximum(b, 1e-300)
+    bits_diff = int(np.count_nonzero(d32["adj_bits"] != d64["adj_bits"]))
+    near = np.abs(b - w["lc_thr"]) <= 0.05 * w["lc_thr"]
+    # restore the FP32 pass's state (adjacency of the headline precision)
+    ctx.pcm_pairs(w["lc_thr"], precision=_capi.PCM_FP32_RECHECK)
+    return {"reference": "FP64 replay path (reference operation order, bit-equal to the FP64 oracle)",
+            "pairs": int(len(b)), "pcm_dist_max_rel_err": float(rel.max()) if len(rel) else 0.0,
+            "pcm_dist_rms_rel_err": float(np.sqrt(np.mean(rel ** 2))) if len(rel) else 0.0,
+            "pcm_dist_max_rel_err_within_5pct_of_threshold": float(rel[near].max()) if near.any() else 0.0,
+            "pairs_replayed_fp64": int(st32.n_rechecked), "fp32_decisions_overturned": int(st32.n_flipped),
+            "adjacency_words_differing": bits_diff,
+            "log_near_pi": "rotation within ~3e-3 rad of pi: replayed in FP64 (counted in pairs_replayed_fp64)"}
+
+
+def exact_clique(ctx, heu_size):
+    """Exact maximum clique (FMC::maxClique semantics) on the last adjacency:
+    size, time, and whether it reached at least the heuristic's size; a node
+    budget overrun is reported as the outcome."""
+    from rpgo_amd import _capi
+
+    t0 = time.perf_counter()
+    try:
+        k, _, ms = ctx.max_clique(_capi.CLIQUE_EXACT)
+        return {"size": k, "ms": ms, "ge_heuristic": bool(k >= heu_size)}
+    except _capi.RpgoError as e:
+        return {"error": str(e), "wall_ms": (time.perf_counter() - t0) * 1e3}
+
+
+def rot_angle_err(Ra, Rb):
+    """largest angle (rad) of Ra^T Rb over the poses (numerically stable form)"""
+    D = np.swapaxes(Ra, -1, -2) @ Rb
+    s = 0.5 * np.linalg.norm(np.stack([D[:, 2, 1] - D[:, 1, 2], D[:, 0, 2] - D[:, 2, 0],
+                                       D[:, 1, 0] - D[:, 0, 1]], -1), axis=-1)
+    c = 0.5 * (D[:, 0, 0] + D[:, 1, 1] + D[:, 2, 2] - 1.0)
+    return float(np.arctan2(s, c).max())
+
+
+def vs_fixture(poses, w, inliers):
+    """Pose and weight differences against the FP64 oracle's GNC solution."""
+    if not os.path.exists(SOLVE_FIXTURE):
+        return None
+    z = np.load(SOLVE_FIXTURE, allow_pickle=False)
+    if not np.array_equal(z["inliers"], inliers):
+        return {"note": "fixture built on another loop-closure set"}
+    from rpgo_amd import synth
+
+    Rz, tz = synth.quat_to_rot(z["q"]), z["t"]
+    Rp = poses[:, :9].reshape(-1, 3, 3)
+    return {"max_translation_err_m": float(np.abs(poses[:, 9:] - tz).max()),
+            "max_rotation_err_rad": rot_angle_err(Rz, Rp),
+            "same_weights": bool(np.array_equal(np.rint(w), np.rint(z["weights"]))),
+            "oracle_gnc_iterations": int(z["iterations"])}
 
 
 def robust_solve(ctx, gl, **solve_params):
@@ -87,8 +177,9 @@ def robust_solve(ctx, gl, **solve_params):
     reference's heuristic clique) then GNC-TLS over LM on odometry + prior +
     the clique's loop closures; plain GNC (started as GTSAM starts it) against
     the learned start (features + MLP + GNC from the predicted weights and mu,
-    plain GNC as the fallback), on the same GPU.  Device times (HIP events)."""
-    from rpgo_amd import _capi, learned
+    accepted only when it passes the cost check against plain GNC's), on the
+    same GPU.  Device times (HIP events)."""
+    from rpgo_amd import learned
 
     t0 = time.perf_counter()
     pcm = learned.pcm_stage(ctx, gl)
@@ -110,21 +201,30 @@ def robust_solve(ctx, gl, **solve_params):
                      "lm_iterations": rep_p.lm_iterations, "pcg_iterations": rep_p.pcg_iterations,
                      "inliers": rep_p.n_inliers, **quality(poses_p, w_p)},
            "end_to_end_plain_ms": ms_pcm + rep_p.ms_total}
+    fx = vs_fixture(poses_p, w_p, pcm.inliers)
+    if fx is not None:
+        res["plain"]["vs_fp64_oracle"] = fx
     if os.path.exists(learned.MODEL_PATH):
         model = learned.LearnedModel.load()
-        poses_l, w_l, rep_l, fb, ms_l = learned.solve_learned(ctx, gl, pcm, model, **solve_params)
-        res["learned"] = {"ms": ms_l, "gnc_iterations": rep_l.gnc_iterations,
-                          "lm_iterations": rep_l.lm_iterations, "fallback": bool(fb),
-                          "tls_fixed_point": rep_l.n_tls_disagree == 0,
+        out = learned.solve_learned(ctx, gl, pcm, model, plain=(poses_p, w_p, rep_p), **solve_params)
+        poses_l, w_l, rep_l = out.poses, out.weights, out.report
+        res["learned"] = {"ms": out.ms, "gnc_iterations": rep_l.gnc_iterations,
+                          "lm_iterations": rep_l.lm_iterations, "accepted": bool(out.accepted),
+                          "check": out.check,
                           "same_inlier_set": bool(np.array_equal(np.rint(w_p), np.rint(w_l))),
                           "inlier_decisions_differing": int((np.rint(w_p) != np.rint(w_l)).sum()),
                           "max_pose_diff_vs_plain_m": float(np.abs(poses_p[:, 9:] - poses_l[:, 9:]).max()),
                           **quality(poses_l, w_l)}
-        res["end_to_end_learned_ms"] = ms_pcm + ms_l
-        res["speedup_learned_vs_plain"] = (ms_pcm + rep_p.ms_total) / (ms_pcm + ms_l)
+        fxl = vs_fixture(poses_l, w_l, pcm.inliers)
+        if fxl is not None:
+            res["learned"]["vs_fp64_oracle"] = fxl
+        res["end_to_end_learned_ms"] = ms_pcm + out.ms
+        res["speedup_learned_vs_plain"] = (ms_pcm + rep_p.ms_total) / (ms_pcm + out.ms)
     else:
         res["learned"] = None
     res["wall_s"] = time.perf_counter() - t0
+    res["_inliers"] = pcm.inliers
+    res["_lm_iterations"] = rep_p.lm_iterations
     return res
 
 
@@ -166,20 +266,125 @@ def batch_c4(ctx, seed=0, n_graphs=256, steps=3):
             "graph_replayed": bool(st.graph_replayed), "results_identical_across_modes": bool(same)}
 
 
+def host_cpu():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def cpu_rate(O, g, traj, budget_s, threads):
+    """pair-checks/s of the FP64 oracle on rows 1..j of the graph, j grown
+    until the budget is spent"""
+    O.set_threads(threads)
+    m = g.n_lc
+    pairs, t_pairs, j0 = 0, 0.0, 1
+    j1 = min(m, 64 * threads)
+    while j0 < m and t_pairs < budget_s:
+        t = time.perf_counter()
+        O.pcm_dist_rows(g.dim, traj, g.lc_from[:j1], g.lc_to[:j1], g.lc_pose[:j1], g.lc_cov[:j1], j0, j1)
+        t_pairs += time.perf_counter() - t
+        pairs += (j1 * (j1 - 1) - j0 * (j0 - 1)) // 2
+        j0 = j1
+        # next rows: about the pairs the remaining budget allows at the rate so far
+        room = (budget_s - t_pairs) * pairs / t_pairs
+        j1 = min(m, int(np.sqrt(j0 * (j0 - 1) + 2.0 * room)) + 1)
+        if j1 <= j0:
+            break
+    return pairs / t_pairs, pairs, t_pairs, j0 - 1
+
+
+def cpu_baseline(name, g, solve_graph=None, solve_inliers=None, gpu_lm_iterations=None, budget_s=15.0):
+    """FP64 oracle on the host cores: one core (the reference's own model) on a
+    bounded row sample of the headline graph, the same with OpenMP over rows,
+    the reference's O(m^3) adjacency growth (Pcm.h:737-767), and one LM
+    iteration of the robust solve's normal equations (linearize + direct
+    sparse solve)."""
+    import oracle as O
+
+    model, nproc = host_cpu()
+    t0 = time.perf_counter()
+    traj = O.fold(g.dim, g.pose0, g.odom, g.odom_cov)
+    t_fold = time.perf_counter() - t0
+    rate1, pairs1, t1, rows1 = cpu_rate(O, g, traj, budget_s, 1)
+    threads = max(1, min(16, nproc or 1))
+    rate_mt, pairs_mt, tmt, rows_mt = cpu_rate(O, g, traj, budget_s / 3, threads)
+    O.set_threads(1)
+    out = dict(value=rate1, unit="pair-checks/s", cores=1, kind="port",
+               sample=f"FP64 oracle on {name}, rows 1..{rows1} ({pairs1} of the workload's pairs, "
+                      f"{t1:.1f} s; fold {t_fold * 1e3:.0f} ms)",
+               host_cpu=model, host_nproc=nproc,
+               openmp={"value": rate_mt, "threads": threads,
+                       "sample": f"rows 1..{rows_mt} ({pairs_mt} pairs, {tmt:.1f} s)"})
+    # faithful incrementAdjMatrix: full on the 5,000-pose graph's 1,000 loop
+    # closures; on the headline graph the storage pattern alone on prefixes,
+    # fitted to c * m^3 and extrapolated to every loop closure, plus the pairs
+    # at the measured one-core rate
+    g5 = graph("se3_5k", 0)
+    traj5 = O.fold(g5.dim, g5.pose0, g5.odom, g5.odom_cov)
+    t = time.perf_counter()
+    O.adj_growth(g5.dim, traj5, g5.lc_from, g5.lc_to, g5.lc_pose, g5.lc_cov, 5.0, True)
+    c2_s = time.perf_counter() - t
+    pref = {}
+    m = g.n_lc
+    for mp in (800, 1200, 1600):
+        if mp >= m:
+            break
+        t = time.perf_counter()
+        O.adj_growth(g.dim, traj, g.lc_from[:mp], g.lc_to[:mp], g.lc_pose[:mp], g.lc_cov[:mp], -1.0, False)
+        pref[mp] = time.perf_counter() - t
+    if not pref:
+        pref[m] = c2_s
+    mp_, tp_ = np.array(list(pref), dtype=np.float64), np.array(list(pref.values()))
+    c = float(np.sum(tp_ * mp_ ** 3) / np.sum(mp_ ** 6))  # least squares of t = c m^3
+    out["adjacency_growth_faithful"] = {
+        "se3_5k_s": c2_s, "storage_only_prefix_s": {str(k): v for k, v in pref.items()},
+        "extrapolated_storage_s": c * m ** 3, "extrapolated_total_s": c * m ** 3 + (m * (m - 1) / 2) / rate1,
+        "note": f"extrapolated to {m} loop closures from the prefixes (c m^3 fit) plus all pairs at the one-core rate"}
+    if solve_graph is not None:
+        import solver_oracle as S
+        from rpgo_amd import problem
+
+        poses, kind, ki, kj, meas, info, nk = problem.solve_arrays(solve_graph, solve_inliers)
+        G = S.graph_from_arrays(poses, kind, ki, kj, meas, info)
+        w = np.ones(G.F)
+        t = time.perf_counter()
+        H, b = S.linearize(G, G.R, G.t, w)
+        t_lin = time.perf_counter() - t
+        t = time.perf_counter()
+        S._solve(H, b)
+        t_sol = time.perf_counter() - t
+        it = gpu_lm_iterations or 0
+        out["robust_solve_cpu"] = {
+            "lm_iteration_s": t_lin + t_sol, "linearize_s": t_lin, "sparse_solve_s": t_sol,
+            "plain_gnc_extrapolated_s": (t_lin + t_sol) * it,
+            "note": f"one LM iteration of the {solve_graph.n_poses}-pose solve (FP64 oracle, SuperLU), "
+                    f"times the {it} LM iterations plain GNC takes (extrapolated)"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="se3_5k", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="se3_50k", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32"])
-    ap.add_argument("--no-large", dest="large", action="store_false",
-                    help="skip the extra 50,000-pose pass reported beside the main line")
+    ap.add_argument("--no-side", "--no-large", dest="side", action="store_false",
+                    help="skip the second PCM graph and the FP32 error report")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-solve", dest="solve", action="store_false",
                     help="skip the robust-solve measurement on the 50,000-pose graph")
     ap.add_argument("--no-batch", dest="batch", action="store_false",
                     help="skip the 256-graph batch measurement")
+    ap.add_argument("--no-exact", dest="exact", action="store_false",
+                    help="skip the exact clique")
     ap.add_argument("--solve-poses", type=int, default=50000,
                     help="poses of the robust-solve graph (SE(3) helix, loop closures 1/5 of the poses, 50 %% outliers)")
     ap.add_argument("--pcg-graphs", type=int, default=1, choices=[0, 1],
@@ -194,95 +399,87 @@ def main():
     from rpgo_amd import _capi, synth
 
     w = WORKLOADS[a.workload]
-    g = synth.se3_helix(n_poses=w["n_poses"], n_lc=w["n_lc"], outlier_frac=w["outlier_frac"], seed=a.seed)
+    g = graph(a.workload, a.seed)
     prec = _capi.PCM_FP64_REPLAY if a.precision == "fp64" else _capi.PCM_FP32_RECHECK
     ctx = _capi.Context(0)
-    ctx.pcm_upload(g.dim, g.pose0, g.odom, g.odom_cov, g.lc_from, g.lc_to, g.lc_pose, g.lc_cov)
-    # ingest: fold, odometry check, FP32 tables (outside the timed region, as in
-    # the reference where the trajectory is folded when odometry arrives)
-    ingest = ctx.pcm_run(w["odom_thr"], w["lc_thr"], precision=prec)
-    # the timed steps run as a caller would, without stage timers (their HIP
-    # events cost a 5k pass about 20 us); a second pass of the same steps with
-    # the timers on gives the kernel and stage times of the roofline
-    ctx.set_stage_timing(False)
-    for _ in range(a.warmup):
-        ctx.pcm_pairs(w["lc_thr"], precision=prec)
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        last = ctx.pcm_pairs(w["lc_thr"], precision=prec)
-    ctx.sync()
-    wall = time.perf_counter() - t0
-    ms_step = wall * 1e3 / a.steps
-    ctx.set_stage_timing(True)
-    stats = [ctx.pcm_pairs(w["lc_thr"], precision=prec) for _ in range(a.steps)]
-    ctx.sync()
-    assert stats[-1].n_consistent == last.n_consistent
-    pairs = stats[-1].n_pairs
-    ms_pairs = float(np.mean([s.ms_pairs for s in stats]))
-    ms_total = float(np.mean([s.ms_total for s in stats]))
-    flop = FLOP_PER_PAIR[g.dim] * pairs
-    achieved = flop / (ms_pairs * 1e-3) / 1e12
+    # ingest (fold, odometry check, FP32 tables) is outside the timed region, as
+    # in the reference where the trajectory is folded when odometry arrives
+    r = pcm_pass(ctx, g, w, prec, a.steps, a.warmup)
+    pairs, ms_k = r["pairs"], r["ms_kernel"]
+    achieved = FLOP_PER_PAIR[g.dim] * pairs / (ms_k * 1e-3) / 1e12
     peak = PEAK_TFLOPS["fp64" if a.precision == "fp64" else "fp32"]
+    traffic = measured_traffic(a.workload) if a.precision == "fp32" else None
     out = {
         "metric": "PCM pair-checks/s",
-        "value": pairs / (ms_step * 1e-3),
+        "value": pairs / (r["ms_step"] * 1e-3),
         "unit": "pair-checks/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": r["ms_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if a.precision == "fp64" else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"{a.workload}: SE(3) helix {w['n_poses']} poses, {w['n_lc']} loop closures, "
-                               f"{int(w['outlier_frac'] * 100)}% outliers, odom_thr {w['odom_thr']}, "
-                               f"lc_thr {w['lc_thr']}", "pairs": pairs, "precision": a.precision,
+        "config": {"workload": describe(a.workload), "pairs": pairs, "precision": a.precision,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "kernel": "k_pairs64" if a.precision == "fp64" else "k_pairs32",
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": measured_traffic() if (a.workload == "se3_5k" and a.precision == "fp32") else None,
-                     "flop_per_pair": FLOP_PER_PAIR[g.dim], "ms_kernel": ms_pairs},
-        "stages_ms": {"pairs_kernel": ms_pairs, "recheck_fp64": float(np.mean([s.ms_recheck for s in stats])),
-                      "pairwise_stage_device": ms_total,
-                      "ingest_fold_fp64": ingest.ms_fold, "ingest_odom_check": ingest.ms_odom,
-                      "ingest_total_device": ingest.ms_total},
-        "n_consistent": stats[-1].n_consistent,
-        "n_rechecked": stats[-1].n_rechecked,
-        "n_flipped": stats[-1].n_flipped,
+                     "traffic": traffic, "flop_per_pair": FLOP_PER_PAIR[g.dim], "ms_kernel": ms_k,
+                     "frac_of_unpacked_fp32_peak": achieved / 78.6 if a.precision == "fp32" else None,
+                     "traffic_gbs": traffic / (ms_k * 1e-3) / 1e9 if traffic else None},
+        "stages_ms": {"pairs_kernel": ms_k, "recheck_fp64": r["ms_recheck"], "pairwise_stage_device": r["ms_stage"],
+                      "ingest_fold_fp64": r["ingest"].ms_fold, "ingest_odom_check": r["ingest"].ms_odom,
+                      "ingest_total_device": r["ingest"].ms_total},
+        "n_consistent": r["stats"][-1].n_consistent,
+        "n_rechecked": r["stats"][-1].n_rechecked,
+        "n_flipped": r["stats"][-1].n_flipped,
     }
     k, _, ms_clq = ctx.max_clique(_capi.CLIQUE_HEU_REF)
     out["clique_heu_ref"] = {"size": k, "ms": ms_clq}
-    if a.large:
-        wl = WORKLOADS["se3_50k"]
-        gl = synth.se3_helix(n_poses=wl["n_poses"], n_lc=wl["n_lc"], outlier_frac=wl["outlier_frac"],
-                             seed=a.seed)
-        ctx.pcm_upload(gl.dim, gl.pose0, gl.odom, gl.odom_cov, gl.lc_from, gl.lc_to, gl.lc_pose, gl.lc_cov)
-        ing = ctx.pcm_run(wl["odom_thr"], wl["lc_thr"], precision=prec)
-        ctx.pcm_pairs(wl["lc_thr"], precision=prec)
-        t1 = time.perf_counter()
-        sl = [ctx.pcm_pairs(wl["lc_thr"], precision=prec) for _ in range(3)]
-        wl_ms = (time.perf_counter() - t1) * 1e3 / 3
-        mk = float(np.mean([s.ms_pairs for s in sl]))
-        kl, _, msl = ctx.max_clique(_capi.CLIQUE_HEU_REF)
-        out["se3_50k"] = {
-            "pairs": sl[-1].n_pairs, "pair_checks_per_s": sl[-1].n_pairs / (wl_ms * 1e-3),
-            "ms_per_pass": wl_ms, "ms_pairs_kernel": mk,
-            "tflops_kernel": FLOP_PER_PAIR[3] * sl[-1].n_pairs / (mk * 1e-3) / 1e12,
-            "ms_recheck": float(np.mean([s.ms_recheck for s in sl])), "n_rechecked": sl[-1].n_rechecked,
-            "n_flipped": sl[-1].n_flipped, "ingest_ms": ing.ms_total, "ingest_fold_ms": ing.ms_fold,
-            "clique_heu_ref": {"size": kl, "ms": msl}}
-        if a.solve:
-            gs = gl if a.solve_poses == wl["n_poses"] else synth.se3_helix(
-                n_poses=a.solve_poses, n_lc=a.solve_poses // 5, outlier_frac=0.5, seed=a.seed)
-            out["robust_solve_50k" if gs is gl else f"robust_solve_{a.solve_poses}"] = robust_solve(ctx, gs, pcg_graphs=a.pcg_graphs)
+    precision = {}
+    if a.side and a.precision == "fp32":
+        precision[a.workload] = fp32_error(ctx, w)
+    if a.exact:
+        out["clique_exact"] = {a.workload: exact_clique(ctx, k)}
+    if a.side:
+        other = "se3_5k" if a.workload == "se3_50k" else "se3_50k"
+        wo = WORKLOADS[other]
+        go = graph(other, a.seed)
+        ro = pcm_pass(ctx, go, wo, prec, a.steps, a.warmup)
+        ko, _, mso = ctx.max_clique(_capi.CLIQUE_HEU_REF)
+        out[other] = {"pairs": ro["pairs"], "pair_checks_per_s": ro["pairs"] / (ro["ms_step"] * 1e-3),
+                      "ms_per_step": ro["ms_step"], "ms_pairs_kernel": ro["ms_kernel"],
+                      "tflops_kernel": FLOP_PER_PAIR[3] * ro["pairs"] / (ro["ms_kernel"] * 1e-3) / 1e12,
+                      "ms_recheck": ro["ms_recheck"], "n_rechecked": ro["stats"][-1].n_rechecked,
+                      "n_flipped": ro["stats"][-1].n_flipped, "ingest_ms": ro["ingest"].ms_total,
+                      "clique_heu_ref": {"size": ko, "ms": mso}}
+        if a.precision == "fp32":
+            precision[other] = fp32_error(ctx, wo)
+        if a.exact:
+            out["clique_exact"][other] = exact_clique(ctx, ko)
+    solve_res = None
+    if a.solve:
+        gs = g if (a.workload == "se3_50k" and a.solve_poses == 50000) else synth.se3_helix(
+            n_poses=a.solve_poses, n_lc=a.solve_poses // 5, outlier_frac=0.5, seed=a.seed)
+        solve_res = robust_solve(ctx, gs, pcg_graphs=a.pcg_graphs)
+        key = "robust_solve_50k" if a.solve_poses == 50000 else f"robust_solve_{a.solve_poses}"
+        out[key] = {k_: v for k_, v in solve_res.items() if not k_.startswith("_")}
+        precision["robust_solve"] = {
+            "operator": "residuals, Jacobians, Hessian blocks, SpMV, dots and cost in FP64; FP32 only "
+                        "in the preconditioner factors (they shape convergence, not the solution)",
+            "final_poses_vs_fp64_oracle": out[key]["plain"].get("vs_fp64_oracle")}
+    if precision:
+        out["precision_vs_fp64"] = precision
     if a.batch:
         out["batch_c4"] = batch_c4(ctx, seed=a.seed)
     ctx.close()
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(g, w["odom_thr"], w["lc_thr"])
+        out["cpu_baseline"] = cpu_baseline(
+            a.workload, g, solve_graph=gs if a.solve else None,
+            solve_inliers=solve_res["_inliers"] if solve_res else None,
+            gpu_lm_iterations=solve_res["_lm_iterations"] if solve_res else None)
     print(json.dumps(out), flush=True)
 
 

@@ -9,9 +9,17 @@ Pipeline (one SE(3) pose graph, all stages on the GPU through the C ABI):
   3. learned: per-loop-closure features (rpgo_lc_features) -> MLP
      (rpgo_mlp_forward) -> initial weights (predicted inliers 1, outliers 0)
      and a starting mu a few GNC steps before the predicted final mu; the
-     solve starts there.  Plain GNC stays the arbiter: when the learned start
-     ends anywhere but at a GNC fixed point (report.n_tls_disagree > 0, or
-     non-binary weights) the plain solve is run and returned instead.
+     solve starts there.  The result is accepted only through the cost check
+     below; otherwise the plain solve is run and returned instead.
+
+Cost check (plain GNC is the arbiter).  The learned result must
+  * be a GNC fixed point: every final weight equals the TLS decision at the
+    final poses (report.n_tls_disagree == 0) and the weights are binary;
+  * not raise the TLS objective  sum_known e_k + sum_lc min(e_k, barc)
+    (e_k = 1/2 e^T Lambda e) above its value at GNC's own first solve (all
+    weights 1), the point plain GNC starts its descent from;
+  * when plain GNC's result is at hand (evaluation: the bench and the tests),
+    keep plain's inlier set with poses within 1e-5 m / 1e-5 rad of plain's.
 
 Labels for training are the plain method's own outputs on the same graphs:
 the final GNC weight of each loop closure and log of the final mu.
@@ -80,10 +88,10 @@ def predict(ctx, model: LearnedModel, feats):
 GNC_ALPHA = 0.9  # RobustSolverParams default (SolverParams.h:60-68)
 
 
-def clique_features(ctx, gr, pcm: PcmResult, **solve_params):
+def clique_features(ctx, gr, pcm: PcmResult, with_cost=False, **solve_params):
     """Features of the solver's loop closures (clique order), with the
     residuals after GNC's first solve (LM with every weight 1).  Returns
-    (features, device ms of the first solve)."""
+    (features, device ms of the first solve[, TLS objective there])."""
     arrs = problem.solve_arrays(gr, pcm.inliers)
     nk = arrs[-1]
     _, _, rep1 = ctx.solve(*arrs[:-1], **solve_params)
@@ -91,16 +99,37 @@ def clique_features(ctx, gr, pcm: PcmResult, **solve_params):
     resid = np.zeros(len(pcm.accepted))
     resid[pcm.clique] = err[nk:]
     barc = _capi.lib().rpgo_gnc_threshold(GNC_ALPHA, 6)
-    return ctx.lc_features(pcm.clique, resid, barc)[pcm.clique], rep1.ms_total
+    feats = ctx.lc_features(pcm.clique, resid, barc)[pcm.clique]
+    if with_cost:
+        return feats, rep1.ms_total, tls_cost(err, nk, barc)
+    return feats, rep1.ms_total
 
 
-def solve_learned(ctx, gr, pcm: PcmResult, model: LearnedModel, mu_step=1.4, **solve_params):
-    """Learned-start GNC with the plain solve as fallback.
+def tls_cost(err, nk, barc):
+    """GNC-TLS objective: known inliers at full cost, loop closures truncated at barc."""
+    return float(np.sum(err[:nk]) + np.sum(np.minimum(err[nk:], barc)))
 
-    Returns (poses, weights, report, used_fallback, ms_total) where ms_total
-    is the device time of the first solve, the solve from the learned start
-    and, on fallback, the plain solve."""
-    feats, ms = clique_features(ctx, gr, pcm, **solve_params)
+
+@dataclass
+class LearnedOutcome:
+    poses: np.ndarray
+    weights: np.ndarray
+    report: object
+    ms: float          # device ms: first solve + learned-start solve (+ plain solve on fallback)
+    accepted: bool     # the learned result passed the cost check
+    check: dict        # the quantities the check looked at
+
+
+POSE_TOL = 1e-5
+
+
+def solve_learned(ctx, gr, pcm: PcmResult, model: LearnedModel, mu_step=1.4, plain=None, **solve_params):
+    """Learned-start GNC, accepted through the cost check, plain GNC as the
+    fallback.  plain: (poses, weights, report) of plain GNC on the same
+    problem when already computed (evaluation); it is then also the judge of
+    the inlier set and the poses, and the fallback reuses it (its device time
+    is added to ms)."""
+    feats, ms, tls_first = clique_features(ctx, gr, pcm, with_cost=True, **solve_params)
     w0, mu_final = predict(ctx, model, feats)
     arrs = problem.solve_arrays(gr, pcm.inliers)
     nk = arrs[-1]
@@ -109,11 +138,32 @@ def solve_learned(ctx, gr, pcm: PcmResult, model: LearnedModel, mu_step=1.4, **s
     poses, w, rep = ctx.solve(*arrs[:-1], use_gnc=1, known_inliers=nk, init_weights=init_w, init_mu=mu0,
                               **solve_params)
     ms += rep.ms_total
+    barc = _capi.lib().rpgo_gnc_threshold(GNC_ALPHA, 6)
+    tls_l = tls_cost(ctx.solve_errors(len(arrs[1])), nk, barc)
     binary = bool(np.all(np.abs(w - np.rint(w)) <= 1e-4))
-    if rep.n_tls_disagree == 0 and binary:
-        return poses, w, rep, False, ms
-    poses, w, rep, _ = solve_plain(ctx, gr, pcm.inliers, **solve_params)
-    return poses, w, rep, True, ms + rep.ms_total
+    check = {"tls_fixed_point": rep.n_tls_disagree == 0, "binary_weights": binary,
+             "tls_cost_learned": tls_l, "tls_cost_first_solve": tls_first,
+             "tls_not_above_first_solve": tls_l <= tls_first}
+    ok = check["tls_fixed_point"] and binary and check["tls_not_above_first_solve"]
+    if plain is not None:
+        pp, wp, rp = plain
+        dt = float(np.abs(pp[:, 9:] - poses[:, 9:]).max())
+        same = bool(np.array_equal(np.rint(wp), np.rint(w)))
+        check.update({"same_inlier_set_as_plain": same, "max_translation_diff_vs_plain_m": dt,
+                      "tls_cost_plain": tls_cost(_errors_at(ctx, arrs, pp), nk, barc)})
+        ok = ok and same and dt <= POSE_TOL
+    if ok:
+        return LearnedOutcome(poses, w, rep, ms, True, check)
+    if plain is None:
+        poses, w, rep, _ = solve_plain(ctx, gr, pcm.inliers, **solve_params)
+    else:
+        poses, w, rep = plain
+    return LearnedOutcome(poses, w, rep, ms + rep.ms_total, False, check)
+
+
+def _errors_at(ctx, arrs, poses):
+    """1/2 e^T Lambda e per factor of the last solve's problem at other poses."""
+    return ctx.solve_errors(len(arrs[1]), poses=poses)
 
 
 def graph_rows(ctx, gr, **solve_params):

@@ -50,6 +50,48 @@ def random_rotations(rng, n):
         np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1)], -2)
 
 
+def quat_to_rot(q):
+    """Unit quaternions (w, x, y, z) (..., 4) -> rotation matrices (..., 3, 3)."""
+    q = np.asarray(q, dtype=np.float64)
+    w, x, y, z = q[..., 0], q[..., 1], q[..., 2], q[..., 3]
+    return np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], -1),
+        np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], -1),
+        np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1)], -2)
+
+
+def rot_to_quat(R):
+    """Rotation matrices (..., 3, 3) -> unit quaternions (w, x, y, z), w >= 0.
+    Shepperd's method: each rotation divides by its largest of 4|w|^2,
+    4|x|^2, 4|y|^2, 4|z|^2, so the result is accurate to rounding at every
+    angle, pi included."""
+    R = np.asarray(R, dtype=np.float64)
+    tr = R[..., 0, 0] + R[..., 1, 1] + R[..., 2, 2]
+    cand = np.stack([tr, R[..., 0, 0], R[..., 1, 1], R[..., 2, 2]], -1)
+    k = np.argmax(cand, -1)
+    q = np.zeros(R.shape[:-2] + (4,))
+    for case in range(4):
+        sel = k == case
+        if not np.any(sel):
+            continue
+        M = R[sel]
+        if case == 0:
+            s = 2.0 * np.sqrt(1.0 + tr[sel])
+            v = [0.25 * s, (M[:, 2, 1] - M[:, 1, 2]) / s, (M[:, 0, 2] - M[:, 2, 0]) / s, (M[:, 1, 0] - M[:, 0, 1]) / s]
+        elif case == 1:
+            s = 2.0 * np.sqrt(1.0 + M[:, 0, 0] - M[:, 1, 1] - M[:, 2, 2])
+            v = [(M[:, 2, 1] - M[:, 1, 2]) / s, 0.25 * s, (M[:, 0, 1] + M[:, 1, 0]) / s, (M[:, 0, 2] + M[:, 2, 0]) / s]
+        elif case == 2:
+            s = 2.0 * np.sqrt(1.0 + M[:, 1, 1] - M[:, 0, 0] - M[:, 2, 2])
+            v = [(M[:, 0, 2] - M[:, 2, 0]) / s, (M[:, 0, 1] + M[:, 1, 0]) / s, 0.25 * s, (M[:, 1, 2] + M[:, 2, 1]) / s]
+        else:
+            s = 2.0 * np.sqrt(1.0 + M[:, 2, 2] - M[:, 0, 0] - M[:, 1, 1])
+            v = [(M[:, 1, 0] - M[:, 0, 1]) / s, (M[:, 0, 2] + M[:, 2, 0]) / s, (M[:, 1, 2] + M[:, 2, 1]) / s, 0.25 * s]
+        q[sel] = np.stack(v, -1)
+    q *= np.where(q[..., :1] < 0, -1.0, 1.0)
+    return q / np.linalg.norm(q, axis=-1, keepdims=True)
+
+
 def se3_pack(R, t):
     R = np.asarray(R); t = np.asarray(t)
     return np.concatenate([R.reshape(R.shape[:-2] + (9,)), t], axis=-1)

@@ -92,6 +92,8 @@ def lib():
             ("orc_pcm_adjacency_landmark", [ctypes.c_int, _d, ctypes.c_int, _i32, _d, _d, ctypes.c_int,
                                             ctypes.c_double, ctypes.c_double, _u8, _d], None),
             ("orc_cross_between", [ctypes.c_int, _d, _d, _d, _d, _d], None),
+            ("orc_adj_growth", [ctypes.c_int, _d, ctypes.c_int, _i32, _i32, _d, _d, ctypes.c_double,
+                                ctypes.c_int], ctypes.c_longlong),
             ("orc_set_threads", [ctypes.c_int], None),
             ("orc_get_threads", [], ctypes.c_int),
             ("orc_pair_dist_list", [ctypes.c_int, _d, _i32, _i32, _d, _d, ctypes.c_int, _i32, _i32, _d],
@@ -396,6 +398,22 @@ def pcm_dist_rows(dim, traj, lc_from, lc_to, lc_pose, lc_cov, j0=None, j1=None):
     lib().orc_pcm_dist_rows(dim, _p(traj), m, _p(lc_from, _i32), _p(lc_to, _i32), _p(lc_pose),
                             _p(lc_cov), j0, j1, _p(dist))
     return dist
+
+
+def adj_growth(dim, traj, lc_from, lc_to, lc_pose, lc_cov, lc_thr, do_pairs=True):
+    """incrementAdjMatrix's allocate/copy/fill/assign pattern (Pcm.h:737-767)
+    over the given loop closures, one thread; returns the consistent pairs."""
+    traj = _f64(traj)
+    lc_from = np.ascontiguousarray(lc_from, dtype=np.int32)
+    lc_to = np.ascontiguousarray(lc_to, dtype=np.int32)
+    m = lc_from.shape[0]
+    lc_pose = _f64(lc_pose).reshape(m, pose_len(dim))
+    lc_cov = _f64(lc_cov).reshape(m, tdim(dim) ** 2)
+    n = lib().orc_adj_growth(dim, _p(traj), m, _p(lc_from, _i32), _p(lc_to, _i32), _p(lc_pose),
+                             _p(lc_cov), float(lc_thr), 1 if do_pairs else 0)
+    if n < 0:
+        raise MemoryError("orc_adj_growth: allocation failed")
+    return int(n)
 
 
 # ---------------------------------------------------------------- cliques

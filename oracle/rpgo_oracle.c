@@ -728,6 +728,73 @@ void orc_pcm_dist_rows(int dim, const double* traj, int m,
     }
 }
 
+/* incrementAdjMatrix's storage pattern (Pcm.h:737-767), one loop closure at
+ * a time, for the CPU baseline's timing: zero-filled k x k adjacency and
+ * distance matrices (column-major doubles, Eigen::MatrixXd::Zero), the
+ * previous (k-1) x (k-1) matrices copied into their top-left corner, row and
+ * column k-1 filled from areLoopsConsistent against every earlier loop closure
+ * (:752-763), then assigned to the stored matrices (:765-766, a second copy of
+ * k x k doubles each).  Single-threaded, like the reference.  do_pairs = 0
+ * moves the same bytes without the pair checks (every pair consistent, as
+ * with lc_threshold < 0, :673).  Returns the consistent pairs (i < j). */
+long long orc_adj_growth(int dim, const double* traj, int m, const int32_t* lc_from,
+                         const int32_t* lc_to, const double* lc_pose, const double* lc_cov,
+                         double lc_thr, int do_pairs) {
+  const int pl = pose_len(dim), dd = tan_dim(dim) * tan_dim(dim);
+  double* adj = NULL;  /* stored matrices, (k-1) x (k-1) before step k */
+  double* dst = NULL;
+  long long n_cons = 0;
+  for (int k = 1; k <= m; ++k) {
+    const size_t kk = (size_t)k * k;
+    double* nadj = (double*)calloc(kk, sizeof(double));
+    double* ndst = (double*)calloc(kk, sizeof(double));
+    if (!nadj || !ndst) {
+      free(nadj); free(ndst); free(adj); free(dst);
+      return -1;
+    }
+    if (k > 1) {
+      for (int c = 0; c < k - 1; ++c) {  /* topLeftCorner(k-1, k-1) = old, column by column */
+        memcpy(nadj + (size_t)c * k, adj + (size_t)c * (k - 1), sizeof(double) * (k - 1));
+        memcpy(ndst + (size_t)c * k, dst + (size_t)c * (k - 1), sizeof(double) * (k - 1));
+      }
+      const int j = k - 1;
+      for (int i = 0; i < j; ++i) {
+        double dv = 0.0;
+        int ok = 1;
+        if (do_pairs && lc_thr >= 0.0) {
+          dv = orc_pair_dist(dim, traj, lc_from[i], lc_to[i], lc_pose + (size_t)i * pl,
+                             lc_cov + (size_t)i * dd, lc_from[j], lc_to[j],
+                             lc_pose + (size_t)j * pl, lc_cov + (size_t)j * dd);
+          ok = dv < lc_thr;
+        }
+        ndst[(size_t)i * k + j] = dv;  /* (row j, col i) and (row i, col j) */
+        ndst[(size_t)j * k + i] = dv;
+        if (ok) {
+          nadj[(size_t)i * k + j] = 1.0;
+          nadj[(size_t)j * k + i] = 1.0;
+          ++n_cons;
+        }
+      }
+    }
+    /* adj_matrix = new_adj_matrix: the stored matrix is reallocated and copied */
+    free(adj);
+    free(dst);
+    adj = (double*)malloc(sizeof(double) * kk);
+    dst = (double*)malloc(sizeof(double) * kk);
+    if (!adj || !dst) {
+      free(nadj); free(ndst); free(adj); free(dst);
+      return -1;
+    }
+    memcpy(adj, nadj, sizeof(double) * kk);
+    memcpy(dst, ndst, sizeof(double) * kk);
+    free(nadj);
+    free(ndst);
+  }
+  free(adj);
+  free(dst);
+  return n_cons;
+}
+
 /* ------------------------------------------------------------------ */
 /* PcmSimple (PoseWithNode, GeometryUtils.h:193-289)                    */
 /* ------------------------------------------------------------------ */

@@ -180,6 +180,13 @@ int orc_max_clique_heu_incremental(int n, const int32_t* offsets,
 int orc_max_clique_exact(int n, const int32_t* offsets, const int32_t* edges,
                          int32_t* out);
 
+/* incrementAdjMatrix's allocate-copy-fill-assign pattern (Pcm.h:737-767) over
+ * m loop closures, single-threaded (CPU baseline timing); do_pairs = 0 skips
+ * the pair checks.  Returns the consistent pairs, -1 if allocation failed. */
+long long orc_adj_growth(int dim, const double* traj, int m, const int32_t* lc_from,
+                         const int32_t* lc_to, const double* lc_pose, const double* lc_cov,
+                         double lc_thr, int do_pairs);
+
 #ifdef __cplusplus
 }
 #endif

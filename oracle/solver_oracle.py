@@ -193,6 +193,26 @@ class Graph:
         self._pidx = np.array([k for kind, k in self.order if kind == "p"], int)
 
 
+def graph_from_arrays(poses, kind, key_i, key_j, measured, information):
+    """Graph from the flat factor list of the rpgo_solve C ABI (factor order
+    kept): kind 0 = BetweenFactor<Pose3>(key_i, key_j, measured), kind 1 =
+    PriorFactor<Pose3>(key_i, measured); poses n x 12 (R row-major, t);
+    information F x 36 in (omega, v) order."""
+    poses = np.asarray(poses, float).reshape(-1, 12)
+    kind = np.asarray(kind)
+    meas = np.asarray(measured, float).reshape(-1, 12)
+    info = np.asarray(information, float).reshape(-1, 6, 6)
+    b = np.nonzero(kind == 0)[0]
+    p = np.nonzero(kind == 1)[0]
+    bpos = {q: n for n, q in enumerate(b)}
+    ppos = {q: n for n, q in enumerate(p)}
+    order = [("b", bpos[q]) if kind[q] == 0 else ("p", ppos[q]) for q in range(len(kind))]
+    ki, kj = np.asarray(key_i, int), np.asarray(key_j, int)
+    return Graph(R=poses[:, :9].reshape(-1, 3, 3).copy(), t=poses[:, 9:].copy(), bi=ki[b], bj=kj[b],
+                 zR=meas[b, :9].reshape(-1, 3, 3), zt=meas[b, 9:], binfo=info[b], pk=ki[p],
+                 pR=meas[p, :9].reshape(-1, 3, 3), pt=meas[p, 9:], pinfo=info[p], order=order)
+
+
 def factor_errors(g: Graph, R, t):
     """Whitened error e and 0.5 e^T Lambda e per factor, in factor order."""
     dims = np.zeros(g.F)
