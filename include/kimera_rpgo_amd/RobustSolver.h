@@ -3,9 +3,14 @@
 // outlier rejection (pairwise consistency + max clique) and GNC-TLS over
 // LM/GN, every heavy stage run on the GPU through the C ABI of rpgo_hip.h.
 //
-// Scope: single-robot graphs (one key prefix for the odometry chain) with
-// Pose3 or Pose2 variables; loop closures between different prefixes and
-// landmark (special-symbol) observations are rejected with an exception.
+// Scope: Pose3 or Pose2 graphs of one or several robots (one odometry chain
+// per key prefix), with inter-robot loop closures grouped per robot pair and
+// landmark (special-symbol) observations grouped per landmark, as the
+// reference's Pcm does (Pcm.h:414-502, 775-844).  std::invalid_argument is
+// thrown where the reference's behaviour is undefined: odometry that does not
+// extend its robot's chain at the end, a landmark factor from landmark to pose,
+// a landmark observed from two robots, a loop closure on a robot without
+// odometry.
 #pragma once
 
 #include <memory>

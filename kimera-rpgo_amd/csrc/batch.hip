@@ -627,8 +627,10 @@ int rpgo_pcm_batch_run(rpgo_ctx* ctx, const rpgo_pcm_batch* B, const rpgo_pcm_pa
     long long bo, bl;
     std::memcpy(&bo, &prm->odom_threshold, 8);
     std::memcpy(&bl, &prm->lc_threshold, 8);
-    std::vector<long long> key = {dim, G, (long long)nlc, (long long)nod, s.tile_off[G], s.sym_off[G], pw, cw, pf,
-                                  bo, bl};
+    // every host-side size a captured launch bakes in: the adjacency words the
+    // zeroing kernel clears (adj_off[G]) and the clique rounds (largest graph)
+    std::vector<long long> key = {dim, G, (long long)nlc, (long long)nod, s.tile_off[G], s.sym_off[G],
+                                  s.adj_off[G], spread_of(s).max_slots, pw, cw, pf, bo, bl};
     for (DevBuf* b : {&s.d_pose_off, &s.d_lc_off, &s.d_adj_off, &s.d_tile_off, &s.d_sym_off, &s.d_lc_graph,
                       &s.pose0, &s.odom, &s.odom_cov, &s.lc_from, &s.lc_to, &s.lc_pose, &s.lc_cov, &s.delta,
                       &s.hinv, &s.traj, &s.lcm, &s.odom_dist, &s.flags, &s.accepted, &s.m, &s.c_from, &s.c_to,

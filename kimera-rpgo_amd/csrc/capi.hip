@@ -667,6 +667,10 @@ int rpgo_max_clique_bits(rpgo_ctx* ctx, int32_t n, const uint64_t* adj_bits, int
     set_error("rpgo_max_clique_bits: invalid argument");
     return RPGO_ERR_ARG;
   }
+  if (n > clique_max_vertices()) {  // checked before the adjacency is copied
+    set_error("rpgo_max_clique_bits: %d vertices exceed the %d supported", n, clique_max_vertices());
+    return RPGO_ERR_ARG;
+  }
   RPGO_HIP_CHECK(hipSetDevice(ctx->device));
   const int W = (n + 63) / 64;
   int rc = upload(ctx->clq.adj, adj_bits, sizeof(uint64_t) * (size_t)n * W, ctx->stream);

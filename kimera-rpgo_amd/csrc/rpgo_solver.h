@@ -16,11 +16,11 @@ constexpr int kCoarseMax = 6 * kCoarseSegMax;
 struct GraphKey {
   int N = 0, S = 0, Sc = 0, nseg_c = 0;
   const void *hist = nullptr, *p = nullptr, *Minv = nullptr, *Hij = nullptr, *inc = nullptr,
-             *inc_q = nullptr, *Ainv = nullptr, *Pb = nullptr;
+             *inc_q = nullptr, *Ainv = nullptr, *Pb = nullptr, *Ds = nullptr, *ext = nullptr;
   bool operator==(const GraphKey& o) const {
     return N == o.N && S == o.S && Sc == o.Sc && nseg_c == o.nseg_c && hist == o.hist && p == o.p &&
            Minv == o.Minv && Hij == o.Hij && inc == o.inc && inc_q == o.inc_q && Ainv == o.Ainv &&
-           Pb == o.Pb;
+           Pb == o.Pb && Ds == o.Ds && ext == o.ext;
   }
 };
 
@@ -45,6 +45,8 @@ struct SolverRun {
   DevBuf inc_q;                  // per incidence: the factor's other pose (-1: not a between factor)
   DevBuf D, Minv, hdiag, b64;
   DevBuf chain, Useg;             // odometry chain links, preconditioner off-diagonals
+  DevBuf Ds, Cs;                  // entry-major diagonal and chain-link blocks (chain SpMV)
+  DevBuf ext_off, ext, ext_q;     // per pose: between incidences that are not chain links
   DevBuf dx, r, zv, p, q, hx;     // PCG vectors (FP64)
   DevBuf sc;                      // FP64 scalars
   DevBuf hist;                    // PCG residual history (per iteration)
@@ -76,7 +78,8 @@ struct SolverRun {
     graph_exec = nullptr;
     for (DevBuf* v : {&hist,&x0, &x, &xn, &kind, &fi, &fj, &z, &info, &w, &barc, &known, &err, &cost, &Hii,
                       &Hjj, &Hij, &gi, &gj, &inc_off, &inc_cnt, &inc, &inc_q, &D, &Minv, &hdiag, &b64,
-                      &chain, &Useg, &Pb, &Ac, &Lci, &LcT, &Ainv, &cres, &ccor,
+                      &chain, &Useg, &Ds, &Cs, &ext_off, &ext, &ext_q, &Pb, &Ac, &Lci, &LcT, &Ainv, &cres,
+                      &ccor,
                       &dx, &r, &zv, &p, &q, &hx, &sc})
       v->release();
   }

@@ -950,6 +950,144 @@ __global__ void k_dot(long long n, const double* __restrict__ a, const double* _
 constexpr int kScR = 0, kScPQ = 2, kScCnt = 32, kScLam = 30;
 constexpr int kSpmvPoseMin = 16384;  // poses from which the SpMV runs one thread per pose
 constexpr int kSpmvBatch = 4;        // incidences per pose whose loads the row SpMV issues together
+constexpr bool kSpmvChain = true;    // PCG SpMV on the entry-major chain layout (k_pcg_spmv_chain)
+
+// ---------------------------------------------------------------------------
+// Chain-structured SpMV.  The odometry chain makes the Gauss-Newton matrix
+// block-tridiagonal along the pose order plus a few loop-closure blocks:
+//   y_p = (D_p + lambda damp_p) x_p + C_p x_{p+1} + C_{p-1}^T x_{p-1}
+//         + sum over p's other between factors (Hij or Hij^T) x_q,
+// with C_p = H_{p,p+1} the block of the chain link p -> p+1 (chain[p]).  The
+// diagonal and link blocks are kept entry-major (entry e of pose p at
+// e * N + p): a wavefront of 64 consecutive poses reads each entry as 512
+// contiguous bytes, where the per-factor layout made every lane fetch its
+// own 288-byte block through a chain of dependent index loads.
+// ---------------------------------------------------------------------------
+
+// extras: per pose, the between factors that are not its chain links (loop
+// closures, repeated odometry), in incidence (factor) order
+__device__ __forceinline__ bool is_link(const int* __restrict__ chain, int N, int p, int f) {
+  const int c0 = chain[p];
+  if (c0 != INT_MAX && (c0 >> 1) == f) return true;
+  if (p > 0) {
+    const int c1 = chain[p - 1];
+    if (c1 != INT_MAX && (c1 >> 1) == f) return true;
+  }
+  (void)N;
+  return false;
+}
+
+__global__ void k_ext_count(int N, const int* __restrict__ off, const int* __restrict__ inc,
+                            const int* __restrict__ kind, const int* __restrict__ chain,
+                            int* __restrict__ cnt) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  int c = 0;
+  for (int k = off[p]; k < off[p + 1]; ++k) {
+    const int f = inc[k] >> 1;
+    if (kind[f] == 0 && !is_link(chain, N, p, f)) ++c;
+  }
+  cnt[p] = c;
+}
+
+__global__ void k_ext_fill(int N, const int* __restrict__ off, const int* __restrict__ inc,
+                           const int* __restrict__ incq, const int* __restrict__ kind,
+                           const int* __restrict__ chain, const int* __restrict__ eoff,
+                           int* __restrict__ ext, int* __restrict__ extq) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  int o = eoff[p];
+  for (int k = off[p]; k < off[p + 1]; ++k) {
+    const int t = inc[k], f = t >> 1;
+    if (kind[f] == 0 && !is_link(chain, N, p, f)) {
+      ext[o] = t;
+      extq[o] = incq[k];
+      ++o;
+    }
+  }
+}
+
+// entry-major diagonal blocks Ds and link blocks Cs (zero where pose p has no
+// link to p + 1); one thread per (entry, pose)
+__global__ void k_chain_blocks(int N, const int* __restrict__ chain, const double* __restrict__ D,
+                               const double* __restrict__ Hij, double* __restrict__ Ds,
+                               double* __restrict__ Cs) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 36ll * N) return;
+  const int e = (int)(t / N), p = (int)(t - (long long)e * N);
+  Ds[t] = D[(size_t)p * 36 + e];
+  double c = 0.0;
+  const int l = (p + 1 < N) ? chain[p] : INT_MAX;
+  if (l != INT_MAX) {
+    const int f = l >> 1, r = e / 6, col = e % 6;
+    // flip: the factor runs p+1 -> p, its Hij is H_{p+1,p} = C_p^T
+    c = (l & 1) ? Hij[(size_t)f * 36 + col * 6 + r] : Hij[(size_t)f * 36 + e];
+  }
+  Cs[t] = c;
+}
+
+// K1 on the chain layout: a workgroup of six wavefronts covers 64 poses,
+// wavefront r computing output row r of each (its lanes 64 consecutive
+// poses).  Row sums in a fixed order: diagonal, damping, link to p+1, link
+// from p-1, extras in factor order.
+__global__ void __launch_bounds__(384) k_pcg_spmv_chain(int N, int a, const int* __restrict__ chain,
+                                                  const double* __restrict__ Ds, const double* __restrict__ Cs,
+                                                  const int* __restrict__ eoff, const int* __restrict__ ext,
+                                                  const int* __restrict__ extq, const double* __restrict__ Hij,
+                                                  const double* __restrict__ hdiag, int diag_damping,
+                                                  double* __restrict__ sc, const double* __restrict__ x,
+                                                  double* __restrict__ y) {
+  __shared__ double red[16];
+  const int r = threadIdx.x >> 6;
+  const int p = blockIdx.x * 64 + (threadIdx.x & 63);
+  const double lambda = sc[kScLam];
+  double pq = 0.0;
+  if (p < N) {
+    const double* xp = x + (size_t)p * 6;
+    const double* xn = x + (size_t)(p + 1 < N ? p + 1 : p) * 6;  // Cs is zero at p = N - 1
+    const double* xb = x + (size_t)(p > 0 ? p - 1 : p) * 6;
+    double s = 0.0;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) s += Ds[(size_t)(r * 6 + c) * N + p] * xp[c];
+    const double d = diag_damping ? fmin(fmax(hdiag[(size_t)p * 6 + r], 1e-6), 1e32) : 1.0;
+    double acc = s + lambda * d * xp[r];
+    double v = 0.0;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) v += Cs[(size_t)(r * 6 + c) * N + p] * xn[c];
+    acc += v;
+    if (p > 0) {
+      v = 0.0;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) v += Cs[(size_t)(c * 6 + r) * N + p - 1] * xb[c];
+      acc += v;
+    }
+    for (int k = eoff[p]; k < eoff[p + 1]; ++k) {
+      const int t = ext[k], q = extq[k];
+      const double* B = Hij + (size_t)(t >> 1) * 36;
+      const double* xq = x + (size_t)q * 6;
+      v = 0.0;
+      if (t & 1) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) v += B[c * 6 + r] * xq[c];
+      } else {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) v += B[r * 6 + c] * xq[c];
+      }
+      acc += v;
+    }
+    y[(size_t)p * 6 + r] = acc;
+    pq = xp[r] * acc;
+  }
+  pq = block_sum(pq, red);
+  if (threadIdx.x == 0) {
+    atomicAdd(&sc[kScPQ + a], pq);
+    if (blockIdx.x == 0) {
+      sc[kScR + (a ^ 1)] = 0.0;
+      sc[kScPQ + (a ^ 1)] = 0.0;
+    }
+  }
+  (void)chain;
+}
 
 // K1: q = (H + lambda damp) p ; PQ[a] += p . q ; clear R[a^1], PQ[a^1].
 // Eight lanes per pose, lane r < 6 owning output row r: block rows (or
@@ -1582,6 +1720,11 @@ int SolverRun::alloc(int N_, int F_) {
   if ((rc = Minv.reserve(sizeof(float) * 36 * N))) return rc;
   if ((rc = Useg.reserve(sizeof(float) * 36 * N))) return rc;
   if ((rc = chain.reserve(sizeof(int) * (N + 1)))) return rc;
+  if ((rc = ext_off.reserve(sizeof(int) * (N + 1)))) return rc;
+  if ((rc = ext.reserve(sizeof(int) * 2 * F))) return rc;
+  if ((rc = ext_q.reserve(sizeof(int) * 2 * F))) return rc;
+  if ((rc = Ds.reserve(sizeof(double) * 36 * N))) return rc;
+  if ((rc = Cs.reserve(sizeof(double) * 36 * N))) return rc;
   if ((rc = hdiag.reserve(sizeof(double) * n6))) return rc;
   for (DevBuf* v : {&b64, &dx, &r, &zv, &p, &q, &hx})
     if ((rc = v->reserve(sizeof(double) * n6))) return rc;
@@ -1625,6 +1768,13 @@ int SolverRun::build_incidence(hipStream_t st) {
   k_chain_init<<<grid_for(N + 1, 256, 1 << 20), 256, 0, st>>>(N + 1, chain.as<int>());
   k_chain_find<<<grid_for(F, 256, 1 << 20), 256, 0, st>>>(F, kind.as<int>(), fi.as<int>(), fj.as<int>(),
                                                           chain.as<int>());
+  // extras of the chain SpMV: every between incidence that is not a chain link
+  k_ext_count<<<grid_for(N, 256, 1 << 20), 256, 0, st>>>(N, inc_off.as<int>(), inc.as<int>(), kind.as<int>(),
+                                                         chain.as<int>(), inc_cnt.as<int>());
+  k_exclusive_scan<<<1, 1024, 0, st>>>(N, inc_cnt.as<int>(), ext_off.as<int>());
+  k_ext_fill<<<grid_for(N, 256, 1 << 20), 256, 0, st>>>(N, inc_off.as<int>(), inc.as<int>(), inc_q.as<int>(),
+                                                        kind.as<int>(), chain.as<int>(), ext_off.as<int>(),
+                                                        ext.as<int>(), ext_q.as<int>());
   RPGO_HIP_CHECK(hipGetLastError());
   return RPGO_OK;
 }
@@ -1652,6 +1802,9 @@ int SolverRun::linearize(hipStream_t st) {
   k_assemble<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(
       N, inc_off.as<int>(), inc.as<int>(), Hii.as<double>(), Hjj.as<double>(), gi.as<double>(),
       gj.as<double>(), D.as<double>(), b64.as<double>(), hdiag.as<double>());
+  k_chain_blocks<<<grid_for(36ll * N, 256, 1 << 22), 256, 0, st>>>(N, chain.as<int>(), D.as<double>(),
+                                                                   Hij.as<double>(), Ds.as<double>(),
+                                                                   Cs.as<double>());
   RPGO_HIP_CHECK(hipGetLastError());
   ++stats.linearizations;
   return RPGO_OK;
@@ -1749,7 +1902,12 @@ void SolverRun::pcg_step(hipStream_t st, int S, int Sc, int nseg_c, int first, i
   // both kernels sum every row in the same order (q is the same to the bit);
   // rows in parallel below kSpmvPoseMin poses, one thread per pose above
   // (5k poses: 46 vs 49 us per PCG iteration; 50k: 78 vs 69 us)
-  if (!first && N < kSpmvPoseMin)
+  if (!first && kSpmvChain)
+    k_pcg_spmv_chain<<<(unsigned)((N + 63) / 64), 384, 0, st>>>(
+        N, a, chain.as<int>(), Ds.as<double>(), Cs.as<double>(), ext_off.as<int>(), ext.as<int>(),
+        ext_q.as<int>(), Hij.as<double>(), hdiag.as<double>(), prm.lm_diagonal_damping, s, p.as<double>(),
+        q.as<double>());
+  else if (!first && N < kSpmvPoseMin)
     k_pcg_spmv_rows<<<grid_for((long long)N * 8, 256, 1 << 22), 256, 0, st>>>(
         N, a, inc_off.as<int>(), inc.as<int>(), inc_q.as<int>(),
         D.as<double>(), Hij.as<double>(), hdiag.as<double>(), prm.lm_diagonal_damping, s, p.as<double>(),
@@ -1836,7 +1994,9 @@ int SolverRun::pcg(hipStream_t st, double lambda, int* iters_out) {
                      (const void*)inc.ptr,
                      (const void*)inc_q.ptr,
                      nseg_c ? (const void*)Ainv.ptr : nullptr,
-                     nseg_c ? (const void*)Pb.ptr : nullptr};
+                     nseg_c ? (const void*)Pb.ptr : nullptr,
+                     (const void*)Ds.ptr,
+                     (const void*)ext.ptr};
   const bool graphs = use_graphs && prm.pcg_graphs != 0;
   if (graphs && !(graph_exec && graph_key == key)) {
     if (graph_exec) {

@@ -262,7 +262,16 @@ class Context:
         arrs = dict(pose0=f64(pose0), odom=f64(odom), odom_cov=f64(odom_cov), lc_from=i32(lc_from),
                     lc_to=i32(lc_to), lc_pose=f64(lc_pose), lc_cov=f64(lc_cov))
         nc = 1 if chain_off is None else len(chain_off) - 1
-        n_poses = arrs["odom"].reshape(-1, 12 if dim == 3 else 4).shape[0] + nc
+        pl, dd = (12, 36) if dim == 3 else (4, 9)
+        n_odom = arrs["odom"].size // pl
+        n_lc = arrs["lc_from"].shape[0]
+        _expect_size("pose0", arrs["pose0"], nc * pl)
+        _expect_size("odom", arrs["odom"], n_odom * pl)
+        _expect_size("odom_cov", arrs["odom_cov"], n_odom * dd)
+        _expect_size("lc_to", arrs["lc_to"], n_lc)
+        _expect_size("lc_pose", arrs["lc_pose"], n_lc * pl)
+        _expect_size("lc_cov", arrs["lc_cov"], n_lc * dd)
+        n_poses = n_odom + nc
         if chain_off is not None:
             arrs["chain_off"] = i32(chain_off)
             if int(arrs["chain_off"][-1]) != n_poses:
@@ -380,6 +389,10 @@ class Context:
         poses, measured, information = f64(poses).reshape(-1, 12), f64(measured), f64(information)
         kind, key_i, key_j = i32(kind), i32(key_i), i32(key_j)
         N, F = poses.shape[0], kind.shape[0]
+        _expect_size("key_i", key_i, F)
+        _expect_size("key_j", key_j, F)
+        _expect_size("measured", measured, 12 * F)
+        _expect_size("information", information, 36 * F)
         prob = SolveProblem(n_poses=N, poses=_ptr(poses), n_factors=F, kind=_ptr(kind, _i32),
                             key_i=_ptr(key_i, _i32), key_j=_ptr(key_j, _i32), measured=_ptr(measured),
                             information=_ptr(information))
@@ -390,6 +403,7 @@ class Context:
             if k == "init_weights":
                 if v is not None:
                     keep = f64(v)
+                    _expect_size("init_weights", keep, F)
                     prm.init_weights = _ptr(keep)
                 continue
             if not hasattr(prm, k):
@@ -515,6 +529,7 @@ class Context:
 
     def max_clique_bits(self, adj_bits, n, mode=CLIQUE_HEU_REF):
         a = np.ascontiguousarray(adj_bits, dtype=np.uint64).reshape(-1)
+        _expect_size("adj_bits", a, n * ((n + 63) // 64), at_least=True)
         idx = np.zeros(max(n, 1), np.int32)
         k = ctypes.c_int32()
         check(lib().rpgo_max_clique_bits(self._h, n, _ptr(a, _u64), mode, _ptr(idx, _i32), ctypes.byref(k)))
@@ -528,6 +543,14 @@ class Context:
         check(lib().rpgo_max_clique_bits_noalloc(self._h, n, ctypes.c_void_p(int(d_adj_bits)),
                                                  _ptr(idx, _i32), ctypes.byref(k)))
         return int(k.value), idx[: max(k.value, 0)].copy()
+
+
+def _expect_size(name, arr, n, at_least=False):
+    """The C ABI reads arrays by count: a size mismatch (or, with at_least, too
+    few entries) is an error here."""
+    size = np.asarray(arr).size
+    if size < n or (size != n and not at_least):
+        raise ValueError(f"{name} has {size} entries, expected {'at least ' if at_least else ''}{n}")
 
 
 def _incremental_bits(ctx, adj_bits, n, num_new, prev_size):

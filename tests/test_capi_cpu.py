@@ -60,3 +60,26 @@ def test_noalloc_clique_rejects_null_context():
     k = ctypes.c_int32()
     assert L.rpgo_max_clique_bits_noalloc(None, 10, None, None, ctypes.byref(k)) == _capi.RPGO_ERR_ARG
     assert b"invalid argument" in L.rpgo_last_error()
+
+
+def test_binding_rejects_mis_sized_arrays_before_the_c_call():
+    """The C ABI reads arrays by count; the binding checks sizes first (no GPU
+    call is reached: the checks run before the context handle is used)."""
+    import pytest
+
+    class NoCtx:
+        _h = None
+
+    eye = np.concatenate([np.eye(3).reshape(9), np.zeros(3)])
+    poses = np.tile(eye, (3, 1))
+    with pytest.raises(ValueError, match="information"):
+        _capi.Context.solve(NoCtx(), poses, [0, 0], [0, 1], [1, 2], np.zeros((2, 12)), np.zeros((1, 36)))
+    with pytest.raises(ValueError, match="init_weights"):
+        _capi.Context.solve(NoCtx(), poses, [0], [0], [1], np.zeros((1, 12)), np.zeros((1, 36)),
+                            init_weights=np.ones(3))
+    with pytest.raises(ValueError, match="lc_cov"):
+        _capi.Context.pcm_upload(NoCtx(), 3, eye, np.tile(eye, (2, 1)), np.zeros((2, 36)), [0], [2],
+                                 eye[None], np.zeros((1, 35)))
+    with pytest.raises(ValueError, match="odom_cov"):
+        _capi.Context.pcm_upload(NoCtx(), 3, eye, np.tile(eye, (2, 1)), np.zeros((1, 36)), [0], [2],
+                                 eye[None], np.zeros((1, 36)))

@@ -153,3 +153,18 @@ def test_clique_heu_ref_random_graphs(ctx):
     # empty graph: FMC returns -1
     k, idx = ctx.max_clique_bits(np.zeros(1, np.uint64), 0)
     assert k == -1
+
+
+def test_clique_above_register_bitset_limit_is_an_argument_error(ctx):
+    """The heuristic keeps a wavefront's candidate set in registers: at most
+    64 x 64 x 8 = 32,768 vertices.  One more is RPGO_ERR_ARG before any
+    device work (the adjacency is not even copied)."""
+    n = 32769
+    W = (n + 63) // 64
+    with pytest.raises(_capi.RpgoError, match="exceed the 32768 supported") as e:
+        ctx.max_clique_bits(np.zeros(n * W, np.uint64), n)
+    assert "status -1" in str(e.value)
+    # the largest supported size still runs (an empty graph: every vertex alone)
+    n = 32768
+    k, idx = ctx.max_clique_bits(np.zeros(n * (n // 64), np.uint64), n)
+    assert k == 1 and list(idx) == [0]
