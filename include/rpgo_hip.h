@@ -381,6 +381,14 @@ typedef struct {
   int32_t gnc_factor_dim;       /* degrees of freedom of the chi^2 threshold in
                                    PROBABILITY mode: 6 (Pose3), 3 for Pose2 graphs
                                    embedded in the plane (<= 0 selects 6)          */
+  int32_t pose2_chart;          /* 0: Pose3 factors (Logmap errors, Expmap
+                                   retraction: the reference's EXPMAP build).
+                                   1: Pose2 graphs embedded in the plane (theta ->
+                                   omega_z, x -> v_x, y -> v_y, R = Rz(theta)) with
+                                   GTSAM's default Pose2 chart: errors are the
+                                   (x, y, theta) of z^-1 T_i^-1 T_j (of x^-1 prior,
+                                   negated, for a prior) and values move by
+                                   x * Pose2(dx) (Pose2::ChartAtOrigin)          */
 } rpgo_solve_params;
 
 typedef struct {

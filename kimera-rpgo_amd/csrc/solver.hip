@@ -142,8 +142,23 @@ struct FacLin {
   double Jj[36];
 };
 
+// Pose2 graphs are embedded in the plane (theta -> omega_z, x -> v_x,
+// y -> v_y).  With chart = 1 their factors keep GTSAM's default Pose2 chart
+// (Pose2::ChartAtOrigin, SLOW_BUT_CORRECT_EXPMAP undefined): the in-plane
+// error is the (x, y, theta) of the relative pose instead of its Logmap, and
+// values move by x * Pose2(dx).  The out-of-plane components (omega_x,
+// omega_y, v_z) keep the SE(3) terms, which vanish and decouple for planar
+// poses.
+constexpr int kPlane[3] = {3, 4, 2};  // tangent index of Pose2 x, y, theta
+
+__device__ __forceinline__ void chart_in_plane(const double* E, double sign, double* e) {
+  e[kPlane[0]] = sign * E[9];
+  e[kPlane[1]] = sign * E[10];
+  e[kPlane[2]] = sign * atan2(E[3], E[0]);
+}
+
 __device__ void factor_error(int kind, const double* Ti, const double* Tj, const double* z,
-                            double* e) {
+                            double* e, int chart) {
   if (kind == 0) {
     double Tij[12], zi[12], E[12];
     double Tii[12];
@@ -152,6 +167,7 @@ __device__ void factor_error(int kind, const double* Ti, const double* Tj, const
     R3::pose_inverse(z, zi);
     R3::pose_compose(zi, Tij, E);
     R3::logmap(E, e);
+    if (chart) chart_in_plane(E, 1.0, e);  // Local(z, hx) = chart(z^-1 hx)
   } else {
     double Tii[12], E[12];
     R3::pose_inverse(Ti, Tii);
@@ -159,11 +175,12 @@ __device__ void factor_error(int kind, const double* Ti, const double* Tj, const
     R3::logmap(E, e);
 #pragma unroll
     for (int k = 0; k < 6; ++k) e[k] = -e[k];
+    if (chart) chart_in_plane(E, -1.0, e);  // -Local(x, prior) = -chart(x^-1 prior)
   }
 }
 
 __device__ void factor_linearize(int kind, const double* Ti, const double* Tj, const double* z,
-                                 FacLin& L) {
+                                 FacLin& L, int chart) {
   if (kind == 0) {
     double Tii[12], Tij[12], zi[12], E[12];
     R3::pose_inverse(Ti, Tii);
@@ -185,8 +202,49 @@ __device__ void factor_linearize(int kind, const double* Ti, const double* Tj, c
         L.Ji[r * 6 + c] = -s;
         L.Jj[r * 6 + c] = Jr[r * 6 + c];
       }
+    if (chart) {
+      // Pose2::between's first Jacobian -Ad(between(Tj, Ti)) (hx = Tij: its
+      // rotation (c, s), dt the world-frame translation Tj - Ti, (c2, s2) the
+      // rotation of Tj) and the chart's Jacobian H_r = [[R(r), 0], [0, 1]] at
+      // r = z^-1 hx: Ji = H_r Hb1, Jj = H_r (BetweenFactor with
+      // LieGroup::localCoordinates)
+      const double c = Tij[0], sn = Tij[3];
+      const double c2 = Tj[0], s2 = Tj[3];
+      const double dxw = Tj[9] - Ti[9], dyw = Tj[10] - Ti[10];
+      const double Hb[9] = {-c, -sn, -s2 * dxw + c2 * dyw, sn, -c, -c2 * dxw - s2 * dyw, 0.0, 0.0, -1.0};
+      const double cr = E[0], sr = E[3];
+      const double Hr[9] = {cr, -sr, 0.0, sr, cr, 0.0, 0.0, 0.0, 1.0};
+      chart_in_plane(E, 1.0, L.e);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        // in-plane rows: only in-plane columns
+#pragma unroll
+        for (int c6 = 0; c6 < 6; ++c6) {
+          L.Ji[kPlane[a] * 6 + c6] = 0.0;
+          L.Jj[kPlane[a] * 6 + c6] = 0.0;
+        }
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          double v = 0.0;
+#pragma unroll
+          for (int m = 0; m < 3; ++m) v += Hr[a * 3 + m] * Hb[m * 3 + b];
+          L.Ji[kPlane[a] * 6 + kPlane[b]] = v;
+          L.Jj[kPlane[a] * 6 + kPlane[b]] = Hr[a * 3 + b];
+        }
+      }
+      // out-of-plane rows: no in-plane columns (zero for planar poses)
+#pragma unroll
+      for (int r = 0; r < 6; ++r) {
+        if (r == kPlane[0] || r == kPlane[1] || r == kPlane[2]) continue;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          L.Ji[r * 6 + kPlane[b]] = 0.0;
+          L.Jj[r * 6 + kPlane[b]] = 0.0;
+        }
+      }
+    }
   } else {
-    factor_error(1, Ti, Tj, z, L.e);
+    factor_error(1, Ti, Tj, z, L.e, chart);
 #pragma unroll
     for (int r = 0; r < 36; ++r) {
       L.Ji[r] = (r % 7 == 0) ? 1.0 : 0.0;
@@ -211,13 +269,13 @@ __device__ __forceinline__ double quad6(const double* e, const double* L) {
 __global__ void k_factor_errors(int F, const int* __restrict__ kind, const int* __restrict__ fi,
                                 const int* __restrict__ fj, const double* __restrict__ z,
                                 const double* __restrict__ info, const double* __restrict__ x,
-                                double* __restrict__ err) {
+                                double* __restrict__ err, int chart) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= F) return;
   double e[6];
   const int k = kind[f];
   factor_error(k, x + (size_t)fi[f] * 12, x + (size_t)(k == 0 ? fj[f] : fi[f]) * 12,
-               z + (size_t)f * 12, e);
+               z + (size_t)f * 12, e, chart);
   err[f] = 0.5 * quad6(e, info + (size_t)f * 36);
 }
 
@@ -228,13 +286,13 @@ __global__ void k_linearize(int F, const int* __restrict__ kind, const int* __re
                             const double* __restrict__ x, double* __restrict__ Hii,
                             double* __restrict__ Hjj, double* __restrict__ Hij,
                             double* __restrict__ gi, double* __restrict__ gj,
-                            double* __restrict__ cost) {
+                            double* __restrict__ cost, int chart) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= F) return;
   const int k = kind[f];
   FacLin L;
   factor_linearize(k, x + (size_t)fi[f] * 12, x + (size_t)(k == 0 ? fj[f] : fi[f]) * 12,
-                   z + (size_t)f * 12, L);
+                   z + (size_t)f * 12, L, chart);
   const double wf = w[f];
   const double* I = info + (size_t)f * 36;
   // LJ = w Lambda J (6x6) for J = Ji, Jj
@@ -1598,13 +1656,22 @@ __global__ void k_copy_d(long long n, const double* __restrict__ a, double* __re
 
 // x_new = x * Exp(delta) per pose
 __global__ void k_retract(int N, const double* __restrict__ x, const double* __restrict__ dx,
-                          double* __restrict__ xn) {
+                          double* __restrict__ xn, int chart) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= N) return;
   double d[6], E[12], o[12];
 #pragma unroll
   for (int e = 0; e < 6; ++e) d[e] = dx[(size_t)p * 6 + e];
-  se3_exp64(d, E);
+  if (chart) {
+    // x * Exp(out-of-plane part) * Pose2(dx_x, dx_y, dx_theta)
+    double dout[6] = {d[0], d[1], 0.0, 0.0, 0.0, d[5]};
+    double Eo[12], P2[12] = {cos(d[2]), -sin(d[2]), 0.0, sin(d[2]), cos(d[2]), 0.0, 0.0, 0.0, 1.0,
+                             d[3], d[4], 0.0};
+    se3_exp64(dout, Eo);
+    R3::pose_compose(Eo, P2, E);
+  } else {
+    se3_exp64(d, E);
+  }
   R3::pose_compose(x + (size_t)p * 12, E, o);
 #pragma unroll
   for (int e = 0; e < 12; ++e) xn[(size_t)p * 12 + e] = o[e];
@@ -1783,7 +1850,7 @@ int SolverRun::build_incidence(hipStream_t st) {
 int SolverRun::total_cost(hipStream_t st, const double* xp, double* out) {
   k_factor_errors<<<grid_for(F, 256, 1 << 20), 256, 0, st>>>(F, kind.as<int>(), fi.as<int>(), fj.as<int>(),
                                                             z.as<double>(), info.as<double>(), xp,
-                                                            err.as<double>());
+                                                            err.as<double>(), prm.pose2_chart);
   RPGO_HIP_CHECK(hipMemsetAsync(sc.as<double>() + 8, 0, sizeof(double), st));
   k_sum<<<grid_for(F), 256, 0, st>>>(F, err.as<double>(), w.as<double>(), sc.as<double>() + 8);
   RPGO_HIP_CHECK(hipGetLastError());
@@ -1798,7 +1865,7 @@ int SolverRun::linearize(hipStream_t st) {
   k_linearize<<<grid_for(F, 64, 1 << 22), 64, 0, st>>>(
       F, kind.as<int>(), fi.as<int>(), fj.as<int>(), z.as<double>(), info.as<double>(), w.as<double>(),
       x.as<double>(), Hii.as<double>(), Hjj.as<double>(), Hij.as<double>(), gi.as<double>(),
-      gj.as<double>(), cost.as<double>());
+      gj.as<double>(), cost.as<double>(), prm.pose2_chart);
   k_assemble<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(
       N, inc_off.as<int>(), inc.as<int>(), Hii.as<double>(), Hjj.as<double>(), gi.as<double>(),
       gj.as<double>(), D.as<double>(), b64.as<double>(), hdiag.as<double>());
@@ -2087,7 +2154,8 @@ int SolverRun::nonlinear_solve(hipStream_t st, bool from_x0, double* cost_out, i
     if (gn) {
       int pit;
       if ((rc = pcg(st, 0.0, &pit))) return rc;
-      k_retract<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(N, x.as<double>(), dx.as<double>(), xn.as<double>());
+      k_retract<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(N, x.as<double>(), dx.as<double>(), xn.as<double>(),
+                                                           prm.pose2_chart);
       double nerr;
       if ((rc = total_cost(st, xn.as<double>(), &nerr))) return rc;
       ++it;
@@ -2105,7 +2173,8 @@ int SolverRun::nonlinear_solve(hipStream_t st, bool from_x0, double* cost_out, i
       if ((rc = pcg(st, lambda, &pit))) return rc;
       double lin;
       if ((rc = model_decrease(st, &lin))) return rc;
-      k_retract<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(N, x.as<double>(), dx.as<double>(), xn.as<double>());
+      k_retract<<<grid_for(N, 128, 1 << 22), 128, 0, st>>>(N, x.as<double>(), dx.as<double>(), xn.as<double>(),
+                                                           prm.pose2_chart);
       if ((rc = total_cost(st, xn.as<double>(), &nerr))) return rc;
       const double fid = lin > 0.0 ? (err - nerr) / lin : -1.0;
       if (lin >= 0.0 && fid >= prm.min_model_fidelity) {
@@ -2133,7 +2202,7 @@ int SolverRun::nonlinear_solve(hipStream_t st, bool from_x0, double* cost_out, i
 int SolverRun::compute_errors(hipStream_t st, const double* xp) {
   k_factor_errors<<<grid_for(F, 256, 1 << 20), 256, 0, st>>>(F, kind.as<int>(), fi.as<int>(), fj.as<int>(),
                                                             z.as<double>(), info.as<double>(), xp,
-                                                            err.as<double>());
+                                                            err.as<double>(), prm.pose2_chart);
   RPGO_HIP_CHECK(hipGetLastError());
   return RPGO_OK;
 }

@@ -97,6 +97,7 @@ void rpgo_solve_params_default(rpgo_solve_params* p) {
   p->pcg_coarse = 0;
   p->pcg_graphs = 1;
   p->gnc_factor_dim = 6;
+  p->pose2_chart = 0;
 }
 
 double rpgo_gnc_threshold(double alpha, int32_t dof) {

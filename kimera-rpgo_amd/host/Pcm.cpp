@@ -548,6 +548,7 @@ Values Pcm::multirobotValueInitialization(const Values& values) const {
         sp.gnc_threshold_mode = RPGO_GNC_PROBABILITY;
         sp.gnc_inlier_threshold = align_prob_;
         sp.gnc_factor_dim = dim_ == 3 ? 6 : 3;
+        sp.pose2_chart = dim_ == 3 ? 0 : 1;
       }
       std::vector<double> res(12);
       rpgo_solve_report rep{};

@@ -167,6 +167,7 @@ void RobustSolver::optimize() {
   sp.pcg_segment = params_.device_params.pcg_segment;
   sp.pcg_coarse = params_.device_params.pcg_coarse;
   sp.gnc_factor_dim = planar ? 3 : 6;
+  sp.pose2_chart = planar ? 1 : 0;  // Pose2 factors keep GTSAM's Pose2 chart
   std::vector<double> w0;
   if (use_gnc) {
     const auto& g = params_.gnc_params;

@@ -113,7 +113,8 @@ class SolveParams(ctypes.Structure):
                 ("pcg_max_iterations", ctypes.c_int32), ("pcg_rel_tol", ctypes.c_double),
                 ("polish_iterations", ctypes.c_int32), ("pcg_segment", ctypes.c_int32),
                 ("gnc_warm_start", ctypes.c_int32), ("pcg_coarse", ctypes.c_int32),
-                ("pcg_graphs", ctypes.c_int32), ("gnc_factor_dim", ctypes.c_int32)]
+                ("pcg_graphs", ctypes.c_int32), ("gnc_factor_dim", ctypes.c_int32),
+                ("pose2_chart", ctypes.c_int32)]
 
 
 class SolveReport(ctypes.Structure):

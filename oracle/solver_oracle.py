@@ -27,6 +27,16 @@ vendored, its published algorithms are restated):
       upper bound 1e5, min model fidelity 1e-3, diagonal damping with the
       Hessian diagonal clamped to [1e-6, 1e32], relative/absolute error
       tolerance 1e-5, 100 iterations.
+  gtsam::BetweenFactor<Pose2> / PriorFactor<Pose2> (Graph2): Pose2 keeps
+      GTSAM's default chart (SLOW_BUT_CORRECT_EXPMAP not defined; the
+      reference's build flags set only the Pose3/Rot3 EXPMAP options,
+      README.md:52): Local(r) = (r.x, r.y, r.theta) and Retract(v) =
+      Pose2(v[0], v[1], v[2]) (Pose2::ChartAtOrigin), so a between factor's
+      error is the (x, y, theta) of z^-1 (T_i^-1 T_j), with Jacobians
+      H_r * (-Ad(hx^-1)) and H_r, H_r = [[R(r.theta), 0], [0, 1]]
+      (Pose2::between, LieGroup::localCoordinates); a prior's error is
+      -Local(x^-1 prior) with Jacobian I (PriorFactor at 686e16aa); values
+      move by x * Pose2(dx).
 Last-bit behaviour (Eigen summation order, CHOLMOD vs SuperLU, Boost chi^2)
 is parity unpinned; only converged results are compared.
 """
@@ -213,8 +223,108 @@ def graph_from_arrays(poses, kind, key_i, key_j, measured, information):
                  pR=meas[p, :9].reshape(-1, 3, 3), pt=meas[p, 9:], pinfo=info[p], order=order)
 
 
+# ------------------------------------------------------------------ SE(2), batched
+def se2_compose(ta, xa, tb, xb):
+    """(theta, xy) of a * b; theta wrapped by atan2 as Rot2 keeps (c, s)."""
+    ca, sa = np.cos(ta), np.sin(ta)
+    x = xa[..., 0] + ca * xb[..., 0] - sa * xb[..., 1]
+    y = xa[..., 1] + sa * xb[..., 0] + ca * xb[..., 1]
+    cb, sb = np.cos(tb), np.sin(tb)
+    th = np.arctan2(sa * cb + ca * sb, ca * cb - sa * sb)
+    return th, np.stack([x, y], -1)
+
+
+def se2_between(ta, xa, tb, xb):
+    """Pose2::between (a^-1 b) and its first Jacobian -Ad(between(b, a))."""
+    c1, s1, c2, s2 = np.cos(ta), np.sin(ta), np.cos(tb), np.sin(tb)
+    c = c1 * c2 + s1 * s2
+    s = -s1 * c2 + c1 * s2
+    dx = xb[..., 0] - xa[..., 0]
+    dy = xb[..., 1] - xa[..., 1]
+    th = np.arctan2(s, c)
+    t = np.stack([c1 * dx + s1 * dy, -s1 * dx + c1 * dy], -1)
+    z = np.zeros_like(c)
+    H1 = np.stack([np.stack([-c, -s, -s2 * dx + c2 * dy], -1),
+                   np.stack([s, -c, -c2 * dx - s2 * dy], -1),
+                   np.stack([z, z, z - 1.0], -1)], -2)
+    return th, t, H1
+
+
+def _chart_H(th):
+    """Jacobian of Pose2::ChartAtOrigin::Local at r: [[R(r.theta), 0], [0, 1]]."""
+    c, s = np.cos(th), np.sin(th)
+    z, o = np.zeros_like(c), np.ones_like(c)
+    return np.stack([np.stack([c, -s, z], -1), np.stack([s, c, z], -1), np.stack([z, z, o], -1)], -2)
+
+
+@dataclass
+class Graph2:
+    """Pose2 graph: N poses (theta, xy) initial values; between factors
+    (i -> j, z = (theta, xy), information 3x3 in (x, y, theta) order); priors;
+    factor order as in Graph."""
+    th: np.ndarray
+    xy: np.ndarray
+    bi: np.ndarray
+    bj: np.ndarray
+    zth: np.ndarray
+    zxy: np.ndarray
+    binfo: np.ndarray
+    pk: np.ndarray = field(default_factory=lambda: np.zeros(0, int))
+    pth: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    pxy: np.ndarray = field(default_factory=lambda: np.zeros((0, 2)))
+    pinfo: np.ndarray = field(default_factory=lambda: np.zeros((0, 3, 3)))
+    order: list = None
+
+    __post_init__ = Graph.__post_init__
+
+    @property
+    def R(self):  # the generic loops carry (R, t) = (theta, xy)
+        return self.th
+
+    @property
+    def t(self):
+        return self.xy
+
+
+def _errors2(g: Graph2, th, xy, jac=False):
+    """Per-factor (x, y, theta) errors in factor order, optionally the Jacobians."""
+    F = g.F
+    E = np.zeros((F, 3))
+    Ji = np.zeros((F, 3, 3))
+    Jj = np.zeros((F, 3, 3))
+    if len(g._bidx):
+        k = g._bidx
+        I, J = g.bi[k], g.bj[k]
+        hth, ht, Hb1 = se2_between(th[I], xy[I], th[J], xy[J])
+        rth, rt, _ = se2_between(g.zth[k], g.zxy[k], hth, ht)
+        E[g._bpos] = np.stack([rt[:, 0], rt[:, 1], rth], -1)
+        if jac:
+            Hl = _chart_H(rth)
+            Ji[g._bpos] = Hl @ Hb1
+            Jj[g._bpos] = Hl
+    if len(g._pidx):
+        k = g._pidx
+        X = g.pk[k]
+        rth, rt, _ = se2_between(th[X], xy[X], g.pth[k], g.pxy[k])
+        E[g._ppos] = -np.stack([rt[:, 0], rt[:, 1], rth], -1)
+        Ji[g._ppos] = np.eye(3)
+    return E, Ji, Jj
+
+
+def _info2(g: Graph2):
+    L = np.zeros((g.F, 3, 3))
+    if len(g._bidx):
+        L[g._bpos] = g.binfo[g._bidx]
+    if len(g._pidx):
+        L[g._ppos] = g.pinfo[g._pidx]
+    return L
+
+
 def factor_errors(g: Graph, R, t):
     """Whitened error e and 0.5 e^T Lambda e per factor, in factor order."""
+    if isinstance(g, Graph2):
+        E, _, _ = _errors2(g, R, t)
+        return 0.5 * np.einsum("fi,fij,fj->f", E, _info2(g), E)
     dims = np.zeros(g.F)
     err = np.zeros(g.F)
     if len(g._bidx):
@@ -237,8 +347,46 @@ def factor_errors(g: Graph, R, t):
     return err
 
 
+def _linearize2(g: Graph2, th, xy, w):
+    N = g.N
+    E, Ji, Jj = _errors2(g, th, xy, jac=True)
+    L = w[:, None, None] * _info2(g)
+    I = np.zeros(g.F, int)
+    J = np.zeros(g.F, int)
+    if len(g._bidx):
+        I[g._bpos], J[g._bpos] = g.bi[g._bidx], g.bj[g._bidx]
+    if len(g._pidx):
+        I[g._ppos] = g.pk[g._pidx]
+    isb = np.zeros(g.F, bool)
+    isb[g._bpos] = True
+    JiT, JjT = np.swapaxes(Ji, -1, -2), np.swapaxes(Jj, -1, -2)
+    rows, cols, vals = [], [], []
+
+    def add_block(P, Q, M):
+        ii = (3 * P)[:, None, None] + np.arange(3)[None, :, None]
+        jj = (3 * Q)[:, None, None] + np.arange(3)[None, None, :]
+        rows.append(np.broadcast_to(ii, M.shape).ravel())
+        cols.append(np.broadcast_to(jj, M.shape).ravel())
+        vals.append(M.ravel())
+
+    add_block(I, I, JiT @ L @ Ji)
+    b = np.zeros(3 * N)
+    Le = (L @ E[..., None])[..., 0]
+    np.add.at(b, (3 * I)[:, None] + np.arange(3), -(JiT @ Le[..., None])[..., 0])
+    if isb.any():
+        Bi, Bj = I[isb], J[isb]
+        add_block(Bj, Bj, (JjT @ L @ Jj)[isb])
+        add_block(Bi, Bj, (JiT @ L @ Jj)[isb])
+        add_block(Bj, Bi, (JjT @ L @ Ji)[isb])
+        np.add.at(b, (3 * Bj)[:, None] + np.arange(3), -(JjT @ Le[..., None])[..., 0][isb])
+    H = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(3 * N, 3 * N))
+    return H, b
+
+
 def linearize(g: Graph, R, t, w):
     """Weighted Gauss-Newton system H (scipy CSR, 6N), gradient b = -J^T W e."""
+    if isinstance(g, Graph2):
+        return _linearize2(g, R, t, w)
     N = g.N
     rows, cols, vals = [], [], []
     b = np.zeros(6 * N)
@@ -291,6 +439,14 @@ def linearize(g: Graph, R, t, w):
     return H, b
 
 
+def _retract(g, R, t, dx):
+    """values.retract(dx): Expmap for Pose3, x * Pose2(dx) for Pose2."""
+    if isinstance(g, Graph2):
+        d = dx.reshape(-1, 3)
+        return se2_compose(R, t, d[:, 2], d[:, :2])
+    return retract(R, t, dx)
+
+
 def retract(R, t, dx):
     dR, dt = se3_exp(dx.reshape(-1, 6))
     Rn = R @ dR
@@ -327,7 +483,7 @@ def lm_optimize(g: Graph, R0, t0, w, p: LMParams = LMParams(), gauss_newton=Fals
         H, b = linearize(g, R, t, w)
         if gauss_newton:
             dx = _solve(H, b)
-            Rn, tn = retract(R, t, dx)
+            Rn, tn = _retract(g, R, t, dx)
             new_err = float(np.dot(w, factor_errors(g, Rn, tn)))
             R, t = Rn, tn
             it += 1
@@ -349,7 +505,7 @@ def lm_optimize(g: Graph, R0, t0, w, p: LMParams = LMParams(), gauss_newton=Fals
             dx = _solve(Hd, b)
             # linearized cost change: b^T dx - 0.5 dx^T H dx  (b = -g)
             lin = float(b @ dx - 0.5 * dx @ (H @ dx))
-            Rn, tn = retract(R, t, dx)
+            Rn, tn = _retract(g, R, t, dx)
             new_err = float(np.dot(w, factor_errors(g, Rn, tn)))
             nonlin = err - new_err
             fid = nonlin / lin if lin > 0 else -1.0
@@ -393,7 +549,7 @@ class GncParams:
 
 
 def barc_sq(g: Graph, p: GncParams):
-    dims = np.full(g.F, 6.0)
+    dims = np.full(g.F, 3.0 if isinstance(g, Graph2) else 6.0)
     if p.threshold_mode == "probability":
         return 0.5 * chi2.ppf(p.inlier_threshold, dims)
     return np.full(g.F, p.inlier_threshold)

@@ -74,3 +74,20 @@ def pose_error(poses, R, t):
     dR = np.swapaxes(R, -1, -2) @ Rp
     ang = np.linalg.norm(S.rot_log(dR), axis=-1).max()
     return float(dt), float(ang)
+
+
+def graph2_from_synth(gr: synth.PoseGraph, lc_idx, prior_var=0.01):
+    """Pose2 oracle graph in the factor order of Pcm.h:977-1005: odometry, the
+    prior on pose 0 (at the ground truth), the given loop closures in order."""
+    n = gr.n_poses
+    lc = np.asarray(lc_idx, dtype=np.int64)
+    th = lambda p: np.arctan2(p[..., 1], p[..., 0])
+    z = np.concatenate([gr.odom, gr.lc_pose[lc]])
+    info = np.linalg.inv(np.concatenate([gr.odom_cov, gr.lc_cov[lc]]).reshape(-1, 3, 3))
+    n_odom = n - 1
+    order = [("b", k) for k in range(n_odom)] + [("p", 0)] + [("b", n_odom + k) for k in range(len(lc))]
+    return S.Graph2(th=th(gr.values).copy(), xy=gr.values[:, 2:].copy(),
+                    bi=np.concatenate([np.arange(n_odom), gr.lc_from[lc]]),
+                    bj=np.concatenate([np.arange(1, n), gr.lc_to[lc]]), zth=th(z), zxy=z[:, 2:],
+                    binfo=info, pk=np.array([0]), pth=th(gr.gt[:1]), pxy=gr.gt[:1, 2:].copy(),
+                    pinfo=(np.eye(3) / prior_var)[None], order=order), n_odom + 1
