@@ -236,12 +236,7 @@ def batch_c4(ctx, seed=0, n_graphs=256, steps=3):
     shapes picked by the autotuner on the first call.  Device times."""
     from rpgo_amd import _capi, synth
 
-    rng = np.random.default_rng(seed)
-    graphs = []
-    for k in range(n_graphs):
-        n = int(np.exp(rng.uniform(np.log(200), np.log(20000))))
-        graphs.append(synth.se3_helix(n_poses=n, n_lc=max(1, int(n * rng.uniform(0.1, 0.3))),
-                                      outlier_frac=float(rng.uniform(0.05, 0.4)), seed=seed * 1000 + k))
+    graphs = synth.c4_batch(seed, n_graphs)
     flags = _capi.BATCH_GRAPH | _capi.BATCH_AUTOTUNE
     first = ctx.pcm_batch_run(graphs, -1.0, 5.0, flags=flags)
     staged = ctx.pcm_batch_run(graphs, -1.0, 5.0)

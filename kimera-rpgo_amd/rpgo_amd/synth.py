@@ -318,6 +318,19 @@ def se2_random_walk(n_poses=500, n_inlier=60, n_outlier=15, seed=0, step=1.0,
         meta=dict(kind="se2_random_walk", n_poses=n_poses, seed=seed))
 
 
+def c4_batch(seed=0, n_graphs=256):
+    """The batched workload: n_graphs independent SE(3) helices with pose
+    counts log-uniform in [200, 20000], loop closures 0.1-0.3 of the poses and
+    outlier fractions uniform in [5 %, 40 %] (graph k seeded seed * 1000 + k)."""
+    rng = np.random.default_rng(seed)
+    graphs = []
+    for k in range(n_graphs):
+        n = int(np.exp(rng.uniform(np.log(200), np.log(20000))))
+        graphs.append(se3_helix(n_poses=n, n_lc=max(1, int(n * rng.uniform(0.1, 0.3))),
+                                outlier_frac=float(rng.uniform(0.05, 0.4)), seed=seed * 1000 + k))
+    return graphs
+
+
 # Named workloads with their PCM thresholds.  odom < 0 disables the odometry
 # check so that every pair (i<j) is evaluated; lc = 5.0 keeps P(chi^2_6 < 25)
 # ~ 0.9997 so the inlier block is nearly complete.

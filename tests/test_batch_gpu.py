@@ -133,3 +133,78 @@ def test_batch_graph_replay_and_autotune(ctx):
     assert thr["stats"].graph_replayed == 0
     ref = ctx.pcm_batch_run(other, 10.0, 3.0)
     _assert_same(thr, ref)
+
+
+def _oracle_graph(gr, odom_thr, lc_thr):
+    import oracle as O
+
+    traj = O.fold(gr.dim, gr.pose0, gr.odom, gr.odom_cov)
+    acc, _, adj, _ = O.pcm_adjacency(gr.dim, traj, gr.lc_from, gr.lc_to, gr.lc_pose, gr.lc_cov, odom_thr, lc_thr,
+                                     want_dist=False)
+    k, idx = O.max_clique_heu_dense(adj)
+    return acc, adj, k, idx
+
+
+@pytest.mark.parametrize("odom_thr,lc_thr", [(-1.0, 5.0), (10.0, 3.0)])
+def test_batch_matches_fp64_oracle(ctx, odom_thr, lc_thr):
+    """Every graph of an uneven batch against the FP64 CPU oracle directly:
+    accepted loop closures, the whole adjacency and the reference heuristic
+    clique."""
+    import oracle as O
+
+    O.set_threads(16)
+    graphs = _graphs3(31, 10)
+    res = ctx.pcm_batch_run(graphs, odom_thr, lc_thr)
+    for g, gr in enumerate(graphs):
+        acc, adj, k, idx = _oracle_graph(gr, odom_thr, lc_thr)
+        got = _capi.unpack_bits(ctx.pcm_batch_adj(g), len(acc))
+        assert np.array_equal(res["accepted"][g], acc), g
+        assert np.array_equal(got, adj), g
+        assert int(res["clique_size"][g]) == k and np.array_equal(res["clique_idx"][g], idx[:max(k, 0)]), g
+
+
+def test_batch_c4_full_size():
+    """The 256-graph batch of the benchmark (poses log-uniform in [200, 20000]):
+    properties of every graph at full size, and oracle rows of eight graphs.
+    Every loop closure accepted (odometry check off), adjacency symmetric with
+    a zero diagonal, its consistent pairs summing to the batch count, the
+    clique's k entries accepted loop closures; sampled rows of eight
+    graphs (the largest among them) equal the FP64 oracle's distances against
+    the threshold bit for bit, and the cliques of the graphs up to 1,200 loop
+    closures equal the oracle's heuristic on the oracle's own adjacency."""
+    import oracle as O
+
+    O.set_threads(16)
+    graphs = synth.c4_batch(0, 256)
+    with _capi.Context(0) as c:
+        print(f"[batch] C4: {len(graphs)} graphs, {sum(g.n_poses for g in graphs)} poses", flush=True)
+        res = c.pcm_batch_run(graphs, -1.0, 5.0, flags=_capi.BATCH_GRAPH)
+        total = 0
+        for g, gr in enumerate(graphs):
+            m = int(res["n_accepted"][g])
+            assert m == gr.n_lc and np.array_equal(res["accepted"][g], np.arange(m)), g
+            A = _capi.unpack_bits(c.pcm_batch_adj(g), m)
+            assert np.array_equal(A, A.T) and not A.diagonal().any(), g
+            total += int(A.sum()) // 2
+            k = int(res["clique_size"][g])
+            ids = res["clique_idx"][g]
+            # FMC::maxCliqueHeu's stale buffer may repeat a vertex (findCliqueHeu.cpp:102,110-113)
+            assert 1 <= k <= m and len(ids) == k and ids.min() >= 0 and ids.max() < m, g
+        assert total == res["stats"].n_consistent
+        order = np.argsort([gr.n_lc for gr in graphs])
+        rng = np.random.default_rng(1)
+        for g in list(order[-3:]) + list(rng.choice(order[:-3], 5, replace=False)):
+            gr = graphs[g]
+            m = gr.n_lc
+            traj = O.fold(gr.dim, gr.pose0, gr.odom, gr.odom_cov)
+            rows = np.sort(rng.choice(np.arange(1, m), min(24, m - 1), replace=False))
+            pi = np.concatenate([np.arange(j) for j in rows]).astype(np.int32)
+            pj = np.concatenate([np.full(j, j) for j in rows]).astype(np.int32)
+            d = O.pair_dist_list(gr.dim, traj, gr.lc_from, gr.lc_to, gr.lc_pose, gr.lc_cov, pi, pj)
+            A = _capi.unpack_bits(c.pcm_batch_adj(g), m)
+            assert np.array_equal(A[pj, pi], (d < 5.0).astype(np.uint8)), g
+            print(f"[batch] C4 graph {g}: {m} loop closures, {len(rows)} oracle rows", flush=True)
+            if m <= 1200:
+                acc, adj, k, idx = _oracle_graph(gr, -1.0, 5.0)
+                assert np.array_equal(A, adj), g
+                assert int(res["clique_size"][g]) == k and np.array_equal(res["clique_idx"][g], idx[:max(k, 0)]), g
