@@ -1002,7 +1002,8 @@ __global__ void k_dot(long long n, const double* __restrict__ a, const double* _
 // in parity slots so that no launch has to reset what a concurrent block of
 // the same launch still reads: for iteration g with a = g & 1,
 //   R[a] = r_g . z_g, R[a^1] accumulates r_{g+1} . z_{g+1}; PQ[a] = p_g . A p_g;
-//   cc[a] accumulates the coarse correction; hist[cnt] = |r_{g+1}|^2.
+//   cc[a] accumulates the coarse correction; the kHistSlots slots of
+//   hist[cnt * kHistRow] sum to |r_{g+1}|^2.
 // Each launch clears only slots that no block of the same launch touches.
 // ---------------------------------------------------------------------------
 constexpr int kScR = 0, kScPQ = 2, kScCnt = 32, kScLam = 30;
@@ -1015,6 +1016,9 @@ constexpr int kScR = 0, kScPQ = 2, kScCnt = 32, kScLam = 30;
 constexpr int kScSlots = 64, kRedSlots = 32, kRedStride = 16;
 constexpr int kScTotal = kScSlots + 4 * kRedSlots * kRedStride;  // doubles in sc
 constexpr int kRedR = 0, kRedPQ = 1;
+// the residual history |r|^2 of each iteration in kHistSlots slots of one
+// 128-byte line each (hist + it * kHistRow), summed on the host
+constexpr int kHistSlots = 8, kHistRow = kHistSlots * kRedStride;
 
 __device__ __forceinline__ double* red_slot(double* sc, int which, int a, int slot) {
   return sc + kScSlots + ((which * 2 + a) * kRedSlots + slot) * kRedStride;
@@ -1345,7 +1349,10 @@ __device__ __forceinline__ void pcg_update_tail(int lane, int i, bool row, int c
   rr = block_sum(rr, red);
   rz = block_sum(rz, red);
   if (threadIdx.x == 0) {
-    if (!first) atomicAdd(&hist[(int)sc[kScCnt]], rr);
+    if (!first) {
+      const int blk = blockIdx.x + blockIdx.y * gridDim.x;
+      atomicAdd(&hist[(size_t)(int)sc[kScCnt] * kHistRow + (blk % kHistSlots) * kRedStride], rr);
+    }
     red_add(sc, kRedR, a ^ 1, rz);
   }
 }
@@ -1842,6 +1849,16 @@ static double* host_scalars() {
   return h;
 }
 
+// pinned host scratch for the residual history of one PCG chunk
+static double* host_history() {
+  static thread_local double* h = nullptr;
+  if (!h) {
+    if (hipHostMalloc((void**)&h, sizeof(double) * kHistRow * kPcgChunk, hipHostMallocDefault) != hipSuccess)
+      h = nullptr;
+  }
+  return h;
+}
+
 int SolverRun::read_scalars(hipStream_t st, int count, double* out) {
   double* h = host_scalars();
   if (!h) {
@@ -2056,10 +2073,10 @@ int SolverRun::pcg(hipStream_t st, double lambda, int* iters_out) {
   int Sc = 0, nseg_c = 0, rc_;
   if ((rc_ = coarse_setup(st, lambda, &Sc, &nseg_c))) return rc_;
   const int max_it = prm.pcg_max_iterations > 0 ? prm.pcg_max_iterations : std::min(6 * N, 4000);
-  if ((rc_ = hist.reserve(sizeof(double) * (size_t)(max_it + kPcgChunk)))) return rc_;
+  if ((rc_ = hist.reserve(sizeof(double) * kHistRow * (size_t)(max_it + kPcgChunk)))) return rc_;
   if ((rc_ = ccor.reserve(sizeof(double) * 2 * kCoarseMax))) return rc_;
   if ((rc_ = cres.reserve(sizeof(double) * kCoarseMax))) return rc_;
-  RPGO_HIP_CHECK(hipMemsetAsync(hist.ptr, 0, sizeof(double) * (size_t)(max_it + kPcgChunk), st));
+  RPGO_HIP_CHECK(hipMemsetAsync(hist.ptr, 0, sizeof(double) * kHistRow * (size_t)(max_it + kPcgChunk), st));
   RPGO_HIP_CHECK(hipMemsetAsync(ccor.ptr, 0, sizeof(double) * 2 * kCoarseMax, st));
   RPGO_HIP_CHECK(hipMemsetAsync(cres.ptr, 0, sizeof(double) * kCoarseMax, st));
   RPGO_HIP_CHECK(hipMemsetAsync(dx.ptr, 0, sizeof(double) * n, st));
@@ -2126,11 +2143,19 @@ int SolverRun::pcg(hipStream_t st, double lambda, int* iters_out) {
     else body();
     RPGO_HIP_CHECK(hipGetLastError());
     double hh[kPcgChunk];
-    double* hb = host_scalars();
-    RPGO_HIP_CHECK(hipMemcpyAsync(hb, hist.as<double>() + it, sizeof(double) * kPcgChunk,
-                                  hipMemcpyDeviceToHost, st));
+    double* hb = host_history();
+    if (!hb) {
+      set_error("hipHostMalloc of the residual history readback failed");
+      return RPGO_ERR_NOMEM;
+    }
+    RPGO_HIP_CHECK(hipMemcpyAsync(hb, hist.as<double>() + (size_t)it * kHistRow,
+                                  sizeof(double) * kHistRow * kPcgChunk, hipMemcpyDeviceToHost, st));
     RPGO_HIP_CHECK(hipStreamSynchronize(st));
-    for (int j = 0; j < kPcgChunk; ++j) hh[j] = hb[j];
+    for (int j = 0; j < kPcgChunk; ++j) {
+      double v = 0.0;
+      for (int k = 0; k < kHistSlots; ++k) v += hb[(size_t)j * kHistRow + k * kRedStride];
+      hh[j] = v;
+    }
     for (int j = 0; j < kPcgChunk; ++j) {
       const double rr = hh[j];
       if (!(rr == rr)) {  // NaN: keep the iterate of the last chunk boundary
