@@ -328,118 +328,118 @@ __device__ __forceinline__ void compose_meas(const P32& T, const S6& C, const Lc
   s6_axpy(1.0f, z.S, Co);
 }
 
-// Pair work is split into units: unit u = one older loop closure i against
-// the 64 newer ones j of row tile jt (lane = j - 64 jt), units ordered by jt
-// then i, so row tile jt owns units [64 jt(jt+1)/2, 64 (jt+1)(jt+2)/2).  The
-// grid holds about as many wavefronts as the chip keeps resident and each
-// takes a contiguous, equal range of units: the whole stage ends within one
-// unit of balance (a grid of one wavefront per tile left a second round on a
-// few SIMDs), and the j-side records are reloaded only when a range crosses
-// into the next row tile.
-__device__ __forceinline__ void unit_of(long long u, int* jt, int* i) {
-  long long t = (long long)((sqrt(1.0 + (double)u / 8.0) - 1.0) * 0.5);  // 32 t (t+1) <= u
-  while (32ll * (t + 1) * (t + 2) <= u) ++t;
-  while (t > 0 && 32ll * t * (t + 1) > u) --t;
-  *jt = (int)t;
-  *i = (int)(u - 32ll * t * (t + 1));
+// Pair work is split by row tile: tile jt holds the 64 newer loop closures
+// j = 64 jt + lane, paired with the older ones i < 64 (jt + 1), which come in
+// words of 64 (i-word w: i in [64 w, 64 w + 64)), each split into S runs of
+// 64 / S (S = 1, 4, 8, ..., 64, chosen per launch so that small stages still
+// fill the chip).  One wavefront takes one run of older loop closures, uniform
+// across the wavefront, against its 64 lanes; a block of four wavefronts takes
+// four consecutive runs of one tile.  The tile's j-side records (the loop
+// closure and its two endpoint poses, 640 B per lane) are staged once per
+// block in LDS and read at their points of use: held in registers for the
+// whole walk they took 157 VGPRs and pushed the pair chain into scratch
+// spills (1.26x slower at 5e7 pairs on MI355X).
+constexpr int kTileWaves = 4;
+
+// 161 floats per lane: an odd stride, so 32-bit LDS reads of the 64 lanes
+// fall in distinct banks (a stride of 160 put all 64 lanes in one bank).
+struct JSide {
+  LcRec lj;
+  PoseRec rc;  // pose lj.a
+  PoseRec rd;  // pose lj.b
+  float pad;
+};
+static_assert(sizeof(JSide) == 161 * 4, "j-side record stride");
+
+// blocks of the tiles before jt: jt / 4 + 1 per tile for S = 1, (jt + 1) S / 4 for S >= 4
+__host__ __device__ __forceinline__ long long tile_blocks_before(long long jt, int S) {
+  if (S == 1) {
+    const long long q = jt / kTileWaves, r = jt % kTileWaves;
+    return jt + (long long)kTileWaves * q * (q - 1) / 2 + r * q;
+  }
+  return (long long)(S / kTileWaves) * jt * (jt + 1) / 2;
 }
 
-// Two wavefronts per SIMD: the compiler fits the pair chain in 256 VGPRs with
-// a small spill instead of 302 registers at one wavefront; 1.29x faster at
-// 5e7 pairs on MI355X (38 vs 30 TFLOP/s).
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
-k_pairs32(int m, long long n_units, int n_waves, const LcRec* __restrict__ lcs,
-          const PoseRec* __restrict__ prec, Anchors an, float thr2, float tau, float kdyn, int W,
-          unsigned long long* __restrict__ adj, double* __restrict__ dist, int2* __restrict__ queue,
-          unsigned int* __restrict__ qcount, unsigned long long* __restrict__ n_consistent) {
-  const int wv = blockIdx.x;
-  if (wv >= n_waves) return;
-  const long long u0 = n_units * wv / n_waves, u1 = n_units * (wv + 1) / n_waves;
-  if (u0 >= u1) return;
-  const int lane = threadIdx.x;
-  int jt, i;
-  unit_of(u0, &jt, &i);
-  int cur_jt = -1;
-  int j = 0, jj = 0;
-  bool jvalid = false;
-  LcRec lj;
-  PoseRec rc, rd;
+// Two wavefronts per SIMD (256 VGPRs each).
+__global__ void __launch_bounds__(64 * kTileWaves) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_pairs32(int m, int n_tiles, int S, const LcRec* __restrict__ lcs, const PoseRec* __restrict__ prec,
+          Anchors an, float thr2, float tau, float kdyn, int W, unsigned long long* __restrict__ adj,
+          double* __restrict__ dist, int2* __restrict__ queue, unsigned int* __restrict__ qcount,
+          unsigned long long* __restrict__ n_consistent) {
+  __shared__ JSide js[64];
+  const long long blk = blockIdx.x;
+  int lo = 0, hi = n_tiles;  // tile_blocks_before(lo) <= blk < tile_blocks_before(hi)
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (tile_blocks_before(mid, S) <= blk) lo = mid;
+    else hi = mid;
+  }
+  const int jt = lo;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+  const int run = (int)(blk - tile_blocks_before(jt, S)) * kTileWaves + wave;  // run of tile jt
+  const int j = jt * 64 + lane;
+  const bool jvalid = j < m;
+  const int jj = jvalid ? j : m - 1;
+  if (wave == 0) js[lane].lj = lcs[jj];
+  else if (wave == 1) js[lane].rc = prec[lcs[jj].a];
+  else if (wave == 2) js[lane].rd = prec[lcs[jj].b];
+  __syncthreads();
+  const int iw = run / S, len = 64 / S;
+  if (iw > jt) return;  // S = 1: the tile's last block may hold fewer than four words
+  const int i0 = iw * 64 + (run % S) * len, i1 = min(i0 + len, m);
   unsigned long long rowbits = 0ull;
-  int roww = -1;  // word of row j the pending rowbits belong to
   unsigned int n_cons = 0;
-  for (long long u = u0; u < u1; ++u) {
-    if (jt != cur_jt) {
-      if (roww >= 0 && jvalid && rowbits) atomicOr(&adj[(size_t)j * W + roww], rowbits);
-      rowbits = 0ull;
-      roww = -1;
-      cur_jt = jt;
-      j = jt * 64 + lane;
-      jvalid = j < m;
-      jj = jvalid ? j : m - 1;
-      lj = lcs[jj];
-      rc = prec[lj.a];
-      rd = prec[lj.b];
-    }
-    if ((i >> 6) != roww) {
-      if (roww >= 0 && jvalid && rowbits) atomicOr(&adj[(size_t)j * W + roww], rowbits);
-      rowbits = 0ull;
-      roww = i >> 6;
-    }
-    if (i < m) {
-      const bool active = jvalid && i < j;
-      const LcRec li = lcs[i];
-      const PoseRec ra = prec[li.a];
-      const PoseRec rb = prec[li.b];
-      bool cons = false;
-      if (active) {
-        P32 TA, TX, TY, TW, TB, TL, TBi;
-        S6 CA, CX, CW, CB, CL;
-        const float ta = between32(an, li.a, ra, lj.a, rc, TA, CA);
-        compose_meas(TA, CA, lj, TX, CX);
-        p32_inv(TX, TY);
-        compose_meas(TY, CX, li, TW, CW);
-        const float tb = between32(an, li.b, rb, lj.b, rd, TB, CB);
-        p32_mul(TW, TB, TL);
-        p32_inv(TB, TBi);
-        s6_congruence(TBi, CW, CL);
-        s6_axpy(1.0f, CB, CL);
-        float xi[6];
-        const bool log_ok = se3_log32(TL, xi);
-        float d2, mp;
-        const bool pd = mahal2_32(CL, xi, &d2, &mp);
-        cons = d2 < thr2;
-        // The reference subtracts chain covariances bounded by |Sigma_c| in PSD
-        // order, so its FP64 error is ~eps64 * tr(chain), measured against the
-        // smallest pivot of Sigma_L (lever arms scale error and covariance alike).
-        const float chain = ra.trc + rc.trc + rb.trc + rd.trc;
-        const float band = tau + kdyn * chain / fmaxf(mp, 1e-30f);
-        (void)ta;
-        (void)tb;
-        const bool near = fabsf(d2 - thr2) <= band * thr2;
-        if (!log_ok || !pd || near || !(d2 == d2) || d2 > 3.0e38f) {
-          const unsigned int q = atomicAdd(qcount, 1u);
-          if (q < (unsigned)kQueueCap) queue[q] = make_int2(i, j);
-        }
-        if (dist) {
-          const double dv = sqrt((double)d2);
-          dist[(size_t)j * m + i] = dv;
-          dist[(size_t)i * m + j] = dv;
-        }
+  for (int i = i0; i < i1; ++i) {
+    // opaque per iteration: the LDS reads below stay at their uses
+    int off = lane * (int)sizeof(JSide);
+    asm volatile("" : "+v"(off));
+    const JSide& J = *reinterpret_cast<const JSide*>(reinterpret_cast<const char*>(js) + off);
+    const bool active = jvalid && i < j;
+    const LcRec& li = lcs[i];
+    const PoseRec& ra = prec[li.a];
+    const PoseRec& rb = prec[li.b];
+    bool cons = false;
+    if (active) {
+      P32 TA, TX, TY, TW, TB, TL, TBi;
+      S6 CA, CX, CW, CB, CL;
+      between32(an, li.a, ra, J.lj.a, J.rc, TA, CA);
+      compose_meas(TA, CA, J.lj, TX, CX);
+      p32_inv(TX, TY);
+      compose_meas(TY, CX, li, TW, CW);
+      between32(an, li.b, rb, J.lj.b, J.rd, TB, CB);
+      p32_mul(TW, TB, TL);
+      p32_inv(TB, TBi);
+      s6_congruence(TBi, CW, CL);
+      s6_axpy(1.0f, CB, CL);
+      float xi[6];
+      const bool log_ok = se3_log32(TL, xi);
+      float d2, mp;
+      const bool pd = mahal2_32(CL, xi, &d2, &mp);
+      cons = d2 < thr2;
+      // The reference subtracts chain covariances bounded by |Sigma_c| in PSD
+      // order, so its FP64 error is ~eps64 * tr(chain), measured against the
+      // smallest pivot of Sigma_L (lever arms scale error and covariance alike).
+      const float chain = ra.trc + J.rc.trc + rb.trc + J.rd.trc;
+      const float band = tau + kdyn * chain / fmaxf(mp, 1e-30f);
+      const bool near = fabsf(d2 - thr2) <= band * thr2;
+      if (!log_ok || !pd || near || !(d2 == d2) || d2 > 3.0e38f) {
+        const unsigned int q = atomicAdd(qcount, 1u);
+        if (q < (unsigned)kQueueCap) queue[q] = make_int2(i, j);
       }
-      if (cons) rowbits |= 1ull << (i & 63);
-      const unsigned long long colbits = __ballot(cons);
-      if (lane == 0 && colbits) {
-        atomicOr(&adj[(size_t)i * W + jt], colbits);
-        n_cons += (unsigned)__popcll(colbits);
+      if (dist) {
+        const double dv = sqrt((double)d2);
+        dist[(size_t)j * m + i] = dv;
+        dist[(size_t)i * m + j] = dv;
       }
     }
-    // next unit
-    if (++i >= 64 * (jt + 1)) {
-      ++jt;
-      i = 0;
+    if (cons) rowbits |= 1ull << (i & 63);
+    const unsigned long long colbits = __ballot(cons);
+    if (lane == 0 && colbits) {
+      atomicOr(&adj[(size_t)i * W + jt], colbits);
+      n_cons += (unsigned)__popcll(colbits);
     }
   }
-  if (roww >= 0 && jvalid && rowbits) atomicOr(&adj[(size_t)j * W + roww], rowbits);
+  if (jvalid && rowbits) atomicOr(&adj[(size_t)j * W + iw], rowbits);
   if (lane == 0 && n_cons) atomicAdd(n_consistent, (unsigned long long)n_cons);
 }
 
@@ -567,17 +567,17 @@ int pcm_pairs_fp32_launch(rpgo_ctx* ctx, double lc_thr, double margin, double ch
   double* dist = want_dist ? s.dist.as<double>() : nullptr;
   unsigned long long* adj = s.adj.as<unsigned long long>();
   int2* q = s.queue.as<int2>();
-  // units: one per row tile and older loop closure.  At least as many
-  // wavefronts as stay resident (2 per SIMD) and at most 64 units each, so a
-  // large stage is spread over several waves of wavefronts the hardware
-  // balances, and a small one still ends within one unit of balance.
-  const long long n_units = 32ll * JT * (JT + 1);
-  long long waves = 8ll * ctx->n_cu;
-  if (n_units > 64 * waves) waves = (n_units + 63) / 64;
-  if (waves > n_units) waves = n_units;
-  if (waves < 1) waves = 1;
-  k_pairs32<<<(unsigned)waves, 64, 0, st>>>(m, n_units, (int)waves, lcr, prec, an, thr2, tau, kdyn, W, adj,
-                                            dist, q, qcount, n_cons);
+  // runs per i-word: the smallest S whose wavefronts fill the chip's resident
+  // slots (8 per CU) once, so a small stage is not left on a few CUs; large
+  // stages take whole words (S = 1).  At 1,000 loop closures S = 16 takes
+  // 93 us on MI355X, S = 64 115 us.
+  const long long words = JT * (JT + 1) / 2;
+  int S = 1;
+  while (S < 64 && words * S < 8ll * ctx->n_cu) S = S == 1 ? 4 : 2 * S;
+  const long long blocks = tile_blocks_before(JT, S);
+  if (blocks > 0)
+    k_pairs32<<<(unsigned)blocks, 64 * kTileWaves, 0, st>>>(m, (int)JT, S, lcr, prec, an, thr2, tau, kdyn,
+                                                            W, adj, dist, q, qcount, n_cons);
   RPGO_HIP_CHECK(hipGetLastError());
   return RPGO_OK;
 }
