@@ -161,15 +161,26 @@ __device__ __forceinline__ void s6_congruence(const P32& T, const S6& S, S6& o) 
 // the angle is within ~3e-3 rad of pi (tr + 1 < 1e-5), where the axial vector
 // 2 sin(theta) n no longer fixes the axis to FP32 precision.  Farther from pi
 // the axis error is ~eps32 / sin(theta) < 2e-5, far inside the recheck band.
-__device__ __forceinline__ bool rot_log32(const float* R, float* w) {
+// Also returns theta and its sine and cosine.
+//
+// Square roots, reciprocals and inverse square roots in this file are the
+// hardware's (v_sqrt_f32, v_rcp_f32, v_rsq_f32, about 1 ulp) rather than the
+// correctly rounded library forms (12-18 instructions each): their error is
+// orders of magnitude below the recheck band, and pairs near the threshold
+// are decided by the FP64 replay.
+__device__ __forceinline__ bool rot_log32(const float* R, float* w, float* th_out, float* s_out,
+                                          float* c_out) {
   const float tr = R[0] + R[4] + R[8];
   const float ax = R[7] - R[5], ay = R[2] - R[6], az = R[3] - R[1];
-  const float s = 0.5f * sqrtf(ax * ax + ay * ay + az * az);  // sin(theta)
-  const float c = 0.5f * (tr - 1.0f);                          // cos(theta)
+  const float s = 0.5f * __builtin_amdgcn_sqrtf(ax * ax + ay * ay + az * az);  // sin(theta)
+  const float c = 0.5f * (tr - 1.0f);                                          // cos(theta)
   const float th = atan2f(s, c);
+  *th_out = th;
+  *s_out = s;
+  *c_out = c;
   float mag;
   if (th < 1e-3f) mag = 0.5f + th * th * (1.0f / 12.0f);
-  else mag = th / (2.0f * s);
+  else mag = 0.5f * th * __builtin_amdgcn_rcpf(s);
   w[0] = mag * ax;
   w[1] = mag * ay;
   w[2] = mag * az;
@@ -178,9 +189,8 @@ __device__ __forceinline__ bool rot_log32(const float* R, float* w) {
 
 // Pose3 Logmap in FP32 (same closed form as the reference's Pose3::Logmap)
 __device__ __forceinline__ bool se3_log32(const P32& T, float* xi) {
-  float w[3];
-  const bool ok = rot_log32(T.R, w);
-  const float th = sqrtf(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  float w[3], th, s, c;
+  const bool ok = rot_log32(T.R, w, &th, &s, &c);
   xi[0] = w[0]; xi[1] = w[1]; xi[2] = w[2];
   const float T0 = T.t[0], T1 = T.t[1], T2 = T.t[2];
   if (th < 1e-10f) {
@@ -190,7 +200,13 @@ __device__ __forceinline__ bool se3_log32(const P32& T, float* xi) {
   // u = T - 1/2 w x T + coef * w x (w x T), coef = (1 - (th/2) cot(th/2)) / th^2
   float coef;
   if (th < 1e-2f) coef = (1.0f / 12.0f) + th * th * (1.0f / 720.0f);
-  else coef = (1.0f - 0.5f * th / tanf(0.5f * th)) / (th * th);
+  else {
+    // (th / 2) cot(th / 2) = (th / 2) (1 + c) / s = (th / 2) s / (1 - c): the
+    // first where c >= 0, the second where c < 0, so neither divides by a
+    // difference that cancels.
+    const float h = c >= 0.f ? (1.0f + c) * __builtin_amdgcn_rcpf(s) : s * __builtin_amdgcn_rcpf(1.0f - c);
+    coef = (1.0f - 0.5f * th * h) * __builtin_amdgcn_rcpf(th * th);
+  }
   const float c0 = w[1] * T2 - w[2] * T1, c1 = w[2] * T0 - w[0] * T2, c2 = w[0] * T1 - w[1] * T0;
   const float d0 = w[1] * c2 - w[2] * c1, d1 = w[2] * c0 - w[0] * c2, d2 = w[0] * c1 - w[1] * c0;
   xi[3] = T0 - 0.5f * c0 + coef * d0;
@@ -226,9 +242,8 @@ __device__ __forceinline__ bool mahal2_32(const S6& S, const float* xi, float* d
     for (int j = 0; j < k; ++j) x -= L[k * (k + 1) / 2 + j] * L[k * (k + 1) / 2 + j];
     ok = ok && (x > 0.f);
     mp = fminf(mp, x);
-    const float lkk = sqrtf(fmaxf(x, 1e-30f));
-    L[k * (k + 1) / 2 + k] = lkk;
-    const float inv = 1.0f / lkk;
+    const float inv = __builtin_amdgcn_rsqf(fmaxf(x, 1e-30f));  // 1 / L_kk
+    L[k * (k + 1) / 2 + k] = inv;
 #pragma unroll
     for (int i = k + 1; i < 6; ++i) {
       float v = M[k * 6 - k * (k - 1) / 2 + (i - k)];  // M(k,i) = M(i,k)
@@ -244,7 +259,7 @@ __device__ __forceinline__ bool mahal2_32(const S6& S, const float* xi, float* d
     float v = xi[i];
 #pragma unroll
     for (int j = 0; j < i; ++j) v -= L[i * (i + 1) / 2 + j] * y[j];
-    y[i] = v / L[i * (i + 1) / 2 + i];
+    y[i] = v * L[i * (i + 1) / 2 + i];
     acc += y[i] * y[i];
   }
   *d2 = acc;
