@@ -308,14 +308,12 @@ __device__ __forceinline__ float between32(const Anchors& an, int x, const PoseR
     pb.t[1] = an.pblock[(size_t)plo * 12 + 10];
     pb.t[2] = an.pblock[(size_t)plo * 12 + 11];
     p32_mul(reli, pb, G);
-    s6_congruence(G, rl.R, M);
     const S6& Q = an.sanc[(size_t)(plo + 1) * an.nb + phi];
-    s6_axpy(1.0f, Q, M);
+    s6_congruence_add(G, rl.R, Q, M);
   }
   P32 hinv;
   p32_inv(rh.Pl, hinv);
-  s6_congruence(hinv, M, cov);
-  s6_axpy(1.0f, rh.L, cov);
+  s6_congruence_add(hinv, M, rh.L, cov);
   return sqrtf(pose.t[0] * pose.t[0] + pose.t[1] * pose.t[1] + pose.t[2] * pose.t[2]);
 }
 
@@ -324,21 +322,20 @@ __device__ __forceinline__ float between32(const Anchors& an, int x, const PoseR
 __device__ __forceinline__ void compose_meas(const P32& T, const S6& C, const LcRec& z, P32& To,
                                              S6& Co) {
   p32_mul(T, z.Z, To);
-  s6_congruence(z.Zi, C, Co);
-  s6_axpy(1.0f, z.S, Co);
+  s6_congruence_add(z.Zi, C, z.S, Co);
 }
 
 // Pair work is split by row tile: tile jt holds the 64 newer loop closures
 // j = 64 jt + lane, paired with the older ones i < 64 (jt + 1), which come in
 // words of 64 (i-word w: i in [64 w, 64 w + 64)), each split into S runs of
-// 64 / S (S = 1, 4, 8, ..., 64, chosen per launch so that small stages still
-// fill the chip).  One wavefront takes one run of older loop closures, uniform
-// across the wavefront, against its 64 lanes; a block of four wavefronts takes
-// four consecutive runs of one tile.  The tile's j-side records (the loop
-// closure and its two endpoint poses, 640 B per lane) are staged once per
-// block in LDS and read at their points of use: held in registers for the
-// whole walk they took 157 VGPRs and pushed the pair chain into scratch
-// spills (1.26x slower at 5e7 pairs on MI355X).
+// 64 / S (S = 4, 8 or 16, chosen per launch from the stage's size).  One
+// wavefront takes one run of older loop closures, uniform across the
+// wavefront, against its 64 lanes; a block of four wavefronts takes four
+// consecutive runs of one tile.  The tile's j-side records (the loop closure
+// and its two endpoint poses, 640 B per lane) are staged once per block in
+// LDS and read at their points of use: held in registers for the whole walk
+// they took 157 VGPRs and pushed the pair chain into scratch spills (1.26x
+// slower at 5e7 pairs on MI355X).
 constexpr int kTileWaves = 4;
 
 // 161 floats per lane: an odd stride, so 32-bit LDS reads of the 64 lanes
@@ -351,12 +348,8 @@ struct JSide {
 };
 static_assert(sizeof(JSide) == 161 * 4, "j-side record stride");
 
-// blocks of the tiles before jt: jt / 4 + 1 per tile for S = 1, (jt + 1) S / 4 for S >= 4
+// blocks of the tiles before jt: tile t has (t + 1) S runs, (t + 1) S / 4 blocks
 __host__ __device__ __forceinline__ long long tile_blocks_before(long long jt, int S) {
-  if (S == 1) {
-    const long long q = jt / kTileWaves, r = jt % kTileWaves;
-    return jt + (long long)kTileWaves * q * (q - 1) / 2 + r * q;
-  }
   return (long long)(S / kTileWaves) * jt * (jt + 1) / 2;
 }
 
@@ -385,7 +378,7 @@ k_pairs32(int m, int n_tiles, int S, const LcRec* __restrict__ lcs, const PoseRe
   else if (wave == 2) js[lane].rd = prec[lcs[jj].b];
   __syncthreads();
   const int iw = run / S, len = 64 / S;
-  if (iw > jt) return;  // S = 1: the tile's last block may hold fewer than four words
+  if (iw > jt) return;  // not reached: a tile's runs fill its blocks
   const int i0 = iw * 64 + (run % S) * len, i1 = min(i0 + len, m);
   unsigned long long rowbits = 0ull;
   unsigned int n_cons = 0;
@@ -409,8 +402,7 @@ k_pairs32(int m, int n_tiles, int S, const LcRec* __restrict__ lcs, const PoseRe
       between32(an, li.b, rb, J.lj.b, J.rd, TB, CB);
       p32_mul(TW, TB, TL);
       p32_inv(TB, TBi);
-      s6_congruence(TBi, CW, CL);
-      s6_axpy(1.0f, CB, CL);
+      s6_congruence_add(TBi, CW, CB, CL);
       float xi[6];
       const bool log_ok = se3_log32(TL, xi);
       float d2, mp;
@@ -567,13 +559,16 @@ int pcm_pairs_fp32_launch(rpgo_ctx* ctx, double lc_thr, double margin, double ch
   double* dist = want_dist ? s.dist.as<double>() : nullptr;
   unsigned long long* adj = s.adj.as<unsigned long long>();
   int2* q = s.queue.as<int2>();
-  // runs per i-word: the smallest S whose wavefronts fill the chip's resident
-  // slots (8 per CU) once, so a small stage is not left on a few CUs; large
-  // stages take whole words (S = 1).  At 1,000 loop closures S = 16 takes
-  // 93 us on MI355X, S = 64 115 us.
+  // runs per i-word: S grows from 4 until the stage holds about 16 rounds of
+  // the chip's resident wavefronts (8 per CU), so the partly filled last
+  // round costs little, and stops at 16 so that a wavefront keeps 4 or more
+  // older loop closures per staged tile.  On MI355X: 10,000 loop closures
+  // take 4.60 / 4.42 / 4.50 / 4.77 ms at S = 1 / 4 / 8 / 16 (S = 1: whole
+  // words, one tile per block of four); 1,000 take 105 / 93 / 93 / 115 us at
+  // S = 8 / 16 / 32 / 64.
   const long long words = JT * (JT + 1) / 2;
-  int S = 1;
-  while (S < 64 && words * S < 8ll * ctx->n_cu) S = S == 1 ? 4 : 2 * S;
+  int S = 4;
+  while (S < 16 && words * S < 128ll * ctx->n_cu) S *= 2;
   const long long blocks = tile_blocks_before(JT, S);
   if (blocks > 0)
     k_pairs32<<<(unsigned)blocks, 64 * kTileWaves, 0, st>>>(m, (int)JT, S, lcr, prec, an, thr2, tau, kdyn,
