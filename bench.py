@@ -258,7 +258,36 @@ def batch_c4(ctx, seed=0, n_graphs=256, steps=3):
                           "total_staged": staged["stats"].ms_total},
             "tuned": {"pair_waves": st.pair_waves, "clique_waves": st.clique_waves,
                       "clique_prefetch": st.clique_prefetch, "ms_tune": first["stats"].ms_tune},
-            "graph_replayed": bool(st.graph_replayed), "results_identical_across_modes": bool(same)}
+            "graph_replayed": bool(st.graph_replayed), "results_identical_across_modes": bool(same),
+            "robust_solve": batch_c4_solve(ctx, graphs, reps[-1])}
+
+
+def batch_c4_solve(ctx, graphs, pcm, stride=16, lanes=4):
+    """Robust solve of the C4 batch's graphs after PCM (every stride-th graph:
+    a bounded sample), GNC-TLS over LM on odometry + prior + each graph's
+    clique: rpgo_solve_batched with `lanes` problems in flight against the same
+    problems in one lane (solved one after another, as rpgo_solve would)."""
+    from rpgo_amd import problem
+
+    sel = list(range(0, len(graphs), stride))
+    probs, known = [], []
+    for g in sel:
+        arrs = problem.solve_arrays(graphs[g], pcm["accepted"][g][pcm["clique_idx"][g]])
+        probs.append(arrs[:-1])
+        known.append(arrs[-1])
+    P1, W1, R1, B1 = ctx.solve_batched(probs, known_inliers=known, lanes=1, use_gnc=1)
+    P, W, R, B = ctx.solve_batched(probs, known_inliers=known, lanes=lanes, use_gnc=1)
+    dt = max(float(np.abs(P[k][:, 9:] - P1[k][:, 9:]).max()) for k in range(len(sel)))
+    same = all(np.array_equal(np.rint(W[k]), np.rint(W1[k])) for k in range(len(sel)))
+    n_poses = int(sum(p[0].shape[0] for p in probs))
+    return {"graphs": len(sel), "sample": f"every {stride}th graph of the batch", "poses": n_poses,
+            "factors": int(sum(len(p[1]) for p in probs)), "lanes": B.lanes,
+            "ms_wall_batched": B.ms_wall, "ms_wall_one_lane": B1.ms_wall,
+            "speedup_vs_one_lane": B1.ms_wall / B.ms_wall if B.ms_wall > 0 else None,
+            "poses_per_s_batched": n_poses / (B.ms_wall * 1e-3) if B.ms_wall > 0 else None,
+            "gnc_iterations": int(sum(r.gnc_iterations for r in R)),
+            "lm_iterations": int(sum(r.lm_iterations for r in R)),
+            "same_weights_as_one_lane": bool(same), "max_translation_diff_vs_one_lane_m": dt}
 
 
 def host_cpu():

@@ -318,8 +318,9 @@ int rpgo_pcm_batch_run(rpgo_ctx* ctx, const rpgo_pcm_batch* batch, const rpgo_pc
 int rpgo_pcm_batch_download_adj(rpgo_ctx* ctx, int32_t g, uint64_t* adj_bits);
 
 /* ------------------------------------------------------------------------
- * Robust solve: GNC-TLS over Levenberg-Marquardt / Gauss-Newton with a
- * block-Jacobi PCG linear step.  Replaces RobustSolver::optimize
+ * Robust solve: GNC-TLS over Levenberg-Marquardt / Gauss-Newton with a PCG
+ * linear step (block-tridiagonal segment preconditioner along the odometry
+ * chain plus a rigid-segment coarse level).  Replaces RobustSolver::optimize
  * (src/RobustSolver.cpp:115-306): gtsam::GncOptimizer with known inliers
  * (:105-113, :130-180, :225-275) or plain LevenbergMarquardtOptimizer /
  * GaussNewtonOptimizer (:181-218, :276-296).  SE(3) poses only.
@@ -419,6 +420,44 @@ void rpgo_solve_params_default(rpgo_solve_params* p);
 /* Solve; poses_out: n_poses * 12, weights_out: n_factors (may be NULL). */
 int rpgo_solve(rpgo_ctx* ctx, const rpgo_solve_problem* prob, const rpgo_solve_params* params,
                double* poses_out, double* weights_out, rpgo_solve_report* report);
+
+/* Batched robust solve: n_problems independent problems of uneven size (the
+ * C4 batch after PCM), packed by prefix offsets.  Problem g: poses
+ * [pose_off[g], pose_off[g+1]) (rows of 12), factors [factor_off[g],
+ * factor_off[g+1]) with pose keys local to the problem.  Each problem is
+ * solved exactly as rpgo_solve solves it alone (RobustSolver::optimize per
+ * graph, src/RobustSolver.cpp:115-306); up to n_lanes problems run at once,
+ * each on its own stream and solver state of this context, largest first. */
+typedef struct {
+  int32_t n_problems;
+  const int64_t* pose_off;      /* n_problems + 1, pose_off[0] = 0              */
+  const int64_t* factor_off;    /* n_problems + 1, factor_off[0] = 0            */
+  const double* poses;          /* pose_off[n] * 12 initial values              */
+  const int32_t* kind;          /* factor_off[n] entries, as rpgo_solve_problem */
+  const int32_t* key_i;         /* local to the problem                         */
+  const int32_t* key_j;
+  const double* measured;       /* factor_off[n] * 12                           */
+  const double* information;    /* factor_off[n] * 36                           */
+  const int32_t* known_inliers; /* n_problems counts, or NULL: params' value    */
+} rpgo_solve_batch;
+
+typedef struct {
+  double ms_wall;               /* host time of the whole call                  */
+  double ms_device_sum;         /* sum of the problems' ms_total                */
+  int32_t lanes;                /* problems in flight at once                   */
+  int32_t failed_problem;       /* -1, or the problem whose solve failed        */
+} rpgo_solve_batch_report;
+
+#define RPGO_SOLVE_MAX_LANES 16
+
+/* params apply to every problem (params->init_weights, when not NULL, holds
+ * factor_off[n] packed weights).  poses_out: pose_off[n] * 12; weights_out:
+ * factor_off[n] (may be NULL); reports: n_problems (may be NULL).  n_lanes in
+ * [1, RPGO_SOLVE_MAX_LANES], 0 selects 4.  The first failing problem stops
+ * the batch; its status is returned and rpgo_last_error() names it. */
+int rpgo_solve_batched(rpgo_ctx* ctx, const rpgo_solve_batch* batch, const rpgo_solve_params* params,
+                       int32_t n_lanes, double* poses_out, double* weights_out,
+                       rpgo_solve_report* reports, rpgo_solve_batch_report* batch_report);
 
 /* ------------------------------------------------------------------------
  * Learned GNC initialisation.  No counterpart in the reference: an addition

@@ -123,6 +123,7 @@ struct CliqueState {
 };
 
 struct SolverRun;  // solver.hip / rpgo_solver.h
+struct SolveLanes; // solver_capi.hip: streams and solver states of rpgo_solve_batched
 struct MlpState;   // learned.hip
 struct BatchState; // batch.hip
 
@@ -135,6 +136,7 @@ struct rpgo_ctx {
   rpgo::PcmState pcm;
   rpgo::CliqueState clq;
   rpgo::SolverRun* solver = nullptr;  // created on the first rpgo_solve
+  rpgo::SolveLanes* solve_lanes = nullptr;  // created by rpgo_solve_batched
   rpgo::MlpState* mlp = nullptr;      // created by rpgo_mlp_set_params
   rpgo::BatchState* batch = nullptr;  // created by rpgo_pcm_batch_run
   void* host_pinned = nullptr;        // 256 B pinned host buffer for counter readbacks

@@ -57,6 +57,10 @@ struct SolverRun {
   hipGraphExec_t graph_exec = nullptr;  // captured PCG chunk
   GraphKey graph_key;
   bool use_graphs = true;
+  // pinned host buffers of this run's readbacks (scalars; one PCG chunk's
+  // residual history), allocated on first use
+  double* h_scalars = nullptr;
+  double* h_history = nullptr;
 
   int alloc(int N_, int F_);
   void spmv_dev(hipStream_t st, const double* lam, const double* x, double* y);
@@ -71,11 +75,16 @@ struct SolverRun {
   int compute_errors(hipStream_t st, const double* xp);
   int nonlinear_solve(hipStream_t st, bool from_x0, double* cost, int* iters);
   int read_scalars(hipStream_t st, int count, double* out);
+  double* pinned_scalars();
+  double* pinned_history();
   int run(hipStream_t st, rpgo_solve_report* rep);
   double* cost_buf() { return cost.as<double>(); }
   void release() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     graph_exec = nullptr;
+    if (h_scalars) (void)hipHostFree(h_scalars);
+    if (h_history) (void)hipHostFree(h_history);
+    h_scalars = h_history = nullptr;
     for (DevBuf* v : {&hist,&x0, &x, &xn, &kind, &fi, &fj, &z, &info, &w, &barc, &known, &err, &cost, &Hii,
                       &Hjj, &Hij, &gi, &gj, &inc_off, &inc_cnt, &inc, &inc_q, &D, &Minv, &hdiag, &b64,
                       &chain, &Useg, &Ds, &Cs, &ext_off, &ext, &ext_q, &Pb, &Ac, &Lci, &LcT, &Ainv, &cres,

@@ -1006,7 +1006,7 @@ __global__ void k_dot(long long n, const double* __restrict__ a, const double* _
 //   hist[cnt * kHistRow] sum to |r_{g+1}|^2.
 // Each launch clears only slots that no block of the same launch touches.
 // ---------------------------------------------------------------------------
-constexpr int kScR = 0, kScPQ = 2, kScCnt = 32, kScLam = 30;
+constexpr int kScCnt = 32, kScLam = 30;  // iteration counter, LM lambda
 // The PCG scalars r.z (R) and p.q (PQ) of each parity a are summed over
 // workgroups in kRedSlots slots 128 bytes apart (sc + kScSlots): a workgroup
 // adds its partial to slot (block id mod kRedSlots), and a reader sums the
@@ -1841,26 +1841,23 @@ int SolverRun::alloc(int N_, int F_) {
 }
 
 // pinned host scratch for scalar readbacks
-static double* host_scalars() {
-  static thread_local double* h = nullptr;
-  if (!h) {
-    if (hipHostMalloc((void**)&h, sizeof(double) * 64, hipHostMallocDefault) != hipSuccess) h = nullptr;
-  }
-  return h;
+double* SolverRun::pinned_scalars() {
+  if (!h_scalars &&
+      hipHostMalloc((void**)&h_scalars, sizeof(double) * 64, hipHostMallocDefault) != hipSuccess)
+    h_scalars = nullptr;
+  return h_scalars;
 }
 
 // pinned host scratch for the residual history of one PCG chunk
-static double* host_history() {
-  static thread_local double* h = nullptr;
-  if (!h) {
-    if (hipHostMalloc((void**)&h, sizeof(double) * kHistRow * kPcgChunk, hipHostMallocDefault) != hipSuccess)
-      h = nullptr;
-  }
-  return h;
+double* SolverRun::pinned_history() {
+  if (!h_history && hipHostMalloc((void**)&h_history, sizeof(double) * kHistRow * kPcgChunk,
+                                  hipHostMallocDefault) != hipSuccess)
+    h_history = nullptr;
+  return h_history;
 }
 
 int SolverRun::read_scalars(hipStream_t st, int count, double* out) {
-  double* h = host_scalars();
+  double* h = pinned_scalars();
   if (!h) {
     set_error("hipHostMalloc of the scalar readback failed");
     return RPGO_ERR_NOMEM;
@@ -2143,7 +2140,7 @@ int SolverRun::pcg(hipStream_t st, double lambda, int* iters_out) {
     else body();
     RPGO_HIP_CHECK(hipGetLastError());
     double hh[kPcgChunk];
-    double* hb = host_history();
+    double* hb = pinned_history();
     if (!hb) {
       set_error("hipHostMalloc of the residual history readback failed");
       return RPGO_ERR_NOMEM;

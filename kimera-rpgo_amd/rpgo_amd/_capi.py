@@ -128,11 +128,33 @@ class SolveReport(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+_i64 = ctypes.POINTER(ctypes.c_int64)
+
+
+class SolveBatch(ctypes.Structure):
+    _fields_ = [("n_problems", ctypes.c_int32), ("pose_off", _i64), ("factor_off", _i64), ("poses", _d),
+                ("kind", _i32), ("key_i", _i32), ("key_j", _i32), ("measured", _d), ("information", _d),
+                ("known_inliers", _i32)]
+
+
+class SolveBatchReport(ctypes.Structure):
+    _fields_ = [("ms_wall", ctypes.c_double), ("ms_device_sum", ctypes.c_double), ("lanes", ctypes.c_int32),
+                ("failed_problem", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+SOLVE_MAX_LANES = 16
+
 # (name, argtypes, restype) of every exported entry point
 SIGNATURES = [
     ("rpgo_solve_params_default", [ctypes.POINTER(SolveParams)], None),
     ("rpgo_solve", [ctypes.c_void_p, ctypes.POINTER(SolveProblem), ctypes.POINTER(SolveParams), _d, _d,
                     ctypes.POINTER(SolveReport)], ctypes.c_int),
+    ("rpgo_solve_batched", [ctypes.c_void_p, ctypes.POINTER(SolveBatch), ctypes.POINTER(SolveParams),
+                            ctypes.c_int32, _d, _d, ctypes.POINTER(SolveReport),
+                            ctypes.POINTER(SolveBatchReport)], ctypes.c_int),
     ("rpgo_ctx_create", [ctypes.c_int, ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
     ("rpgo_ctx_destroy", [ctypes.c_void_p], ctypes.c_int),
     ("rpgo_ctx_sync", [ctypes.c_void_p], ctypes.c_int),
@@ -416,6 +438,60 @@ class Context:
         check(lib().rpgo_solve(self._h, ctypes.byref(prob), ctypes.byref(prm), _ptr(out), _ptr(w),
                                ctypes.byref(rep)))
         return out, w[:F].copy(), rep
+
+    def solve_batched(self, problems, known_inliers=None, lanes=0, **params):
+        """rpgo_solve_batched over a list of (poses, kind, key_i, key_j, measured,
+        information) tuples (the arguments of solve); known_inliers: one count per
+        problem, or None for the params' value.  Returns (list of poses, list of
+        weights, list of SolveReport, SolveBatchReport)."""
+        f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+        G = len(problems)
+        Ns = [np.asarray(p[0]).reshape(-1, 12).shape[0] for p in problems]
+        Fs = [np.asarray(p[1]).shape[0] for p in problems]
+        pose_off = np.zeros(G + 1, np.int64)
+        factor_off = np.zeros(G + 1, np.int64)
+        pose_off[1:] = np.cumsum(Ns)
+        factor_off[1:] = np.cumsum(Fs)
+        cat = lambda k, f, shape: (f(np.concatenate([np.asarray(p[k]).reshape(shape) for p in problems]))
+                                   if G else f(np.zeros((0,) + shape[1:])))
+        poses = cat(0, f64, (-1, 12))
+        kind, key_i, key_j = cat(1, i32, (-1,)), cat(2, i32, (-1,)), cat(3, i32, (-1,))
+        measured, information = cat(4, f64, (-1, 12)), cat(5, f64, (-1, 36))
+        for g, p in enumerate(problems):
+            for name, k, per in (("key_i", 2, 1), ("key_j", 3, 1), ("measured", 4, 12), ("information", 5, 36)):
+                _expect_size(f"problem {g} {name}", np.asarray(p[k]).reshape(-1), per * Fs[g])
+        F = int(factor_off[-1])
+        ki = None
+        if known_inliers is not None:
+            ki = i32(known_inliers)
+            _expect_size("known_inliers", ki, G)
+        b = SolveBatch(n_problems=G, pose_off=pose_off.ctypes.data_as(_i64),
+                       factor_off=factor_off.ctypes.data_as(_i64), poses=_ptr(poses), kind=_ptr(kind, _i32),
+                       key_i=_ptr(key_i, _i32), key_j=_ptr(key_j, _i32), measured=_ptr(measured),
+                       information=_ptr(information), known_inliers=_ptr(ki, _i32) if ki is not None else None)
+        prm = SolveParams()
+        lib().rpgo_solve_params_default(ctypes.byref(prm))
+        keep = None
+        for k, v in params.items():
+            if k == "init_weights":
+                if v is not None:
+                    keep = f64(v)
+                    _expect_size("init_weights", keep, F)
+                    prm.init_weights = _ptr(keep)
+                continue
+            if not hasattr(prm, k):
+                raise KeyError(k)
+            setattr(prm, k, v)
+        out = np.zeros((max(int(pose_off[-1]), 1), 12))
+        w = np.zeros(max(F, 1))
+        reps = (SolveReport * max(G, 1))()
+        brep = SolveBatchReport()
+        check(lib().rpgo_solve_batched(self._h, ctypes.byref(b), ctypes.byref(prm), int(lanes), _ptr(out), _ptr(w),
+                                       reps, ctypes.byref(brep)))
+        poses_l = [out[pose_off[g]:pose_off[g + 1]].copy() for g in range(G)]
+        w_l = [w[factor_off[g]:factor_off[g + 1]].copy() for g in range(G)]
+        return poses_l, w_l, [reps[g] for g in range(G)], brep
 
     # ------------------------------------------------------ batched PCM
     def pcm_batch_run(self, graphs, odom_thr, lc_thr, clique=True, flags=0):
