@@ -232,20 +232,25 @@ def batch_c4(ctx, seed=0, n_graphs=256, steps=3):
     """C4 batch: 256 independent SE(3) graphs, poses log-uniform in [200, 20000],
     loop closures 0.1-0.3 of the poses, outliers 5-40 %, odometry check off and
     lc_threshold 5.0.  One launch per stage (fold, odometry check, compaction,
-    pairs, symmetrize, clique of every graph), captured in a hipGraph, launch
-    shapes picked by the autotuner on the first call.  Device times."""
+    FP32 pairs with the FP64 replay near the threshold, symmetrize, clique of
+    every graph), captured in a hipGraph, launch shapes picked by the autotuner
+    on the first call.  Device times; the FP64 pair stage is timed beside it."""
     from rpgo_amd import _capi, synth
 
     graphs = synth.c4_batch(seed, n_graphs)
     flags = _capi.BATCH_GRAPH | _capi.BATCH_AUTOTUNE
-    first = ctx.pcm_batch_run(graphs, -1.0, 5.0, flags=flags)
-    staged = ctx.pcm_batch_run(graphs, -1.0, 5.0)
+    fp32 = _capi.PCM_FP32_RECHECK
+    first = ctx.pcm_batch_run(graphs, -1.0, 5.0, flags=flags, precision=fp32)
+    staged = ctx.pcm_batch_run(graphs, -1.0, 5.0, precision=fp32)
     t0 = time.perf_counter()
-    reps = [ctx.pcm_batch_run(graphs, -1.0, 5.0, flags=flags) for _ in range(steps)]
+    reps = [ctx.pcm_batch_run(graphs, -1.0, 5.0, flags=flags, precision=fp32) for _ in range(steps)]
     wall = (time.perf_counter() - t0) / steps
     ms = float(np.mean([r["stats"].ms_total for r in reps]))
     st = reps[-1]["stats"]
-    same = all(np.array_equal(reps[-1]["clique_size"], r["clique_size"]) for r in (first, staged))
+    fp64 = [ctx.pcm_batch_run(graphs, -1.0, 5.0, flags=flags) for _ in range(2)][-1]
+    fp64_staged = ctx.pcm_batch_run(graphs, -1.0, 5.0)
+    same = all(np.array_equal(reps[-1]["clique_size"], r["clique_size"]) for r in (first, staged, fp64))
+    same = same and all(np.array_equal(reps[-1]["clique_idx"][g], fp64["clique_idx"][g]) for g in range(n_graphs))
     ss = staged["stats"]
     adj_ms = ss.ms_fold + ss.ms_odom + ss.ms_pairs
     return {"graphs": n_graphs, "poses": int(sum(g.n_poses for g in graphs)),
@@ -259,6 +264,11 @@ def batch_c4(ctx, seed=0, n_graphs=256, steps=3):
             "tuned": {"pair_waves": st.pair_waves, "clique_waves": st.clique_waves,
                       "clique_prefetch": st.clique_prefetch, "ms_tune": first["stats"].ms_tune},
             "graph_replayed": bool(st.graph_replayed), "results_identical_across_modes": bool(same),
+            "fp32_pair_stage": bool(st.fp32), "pairs_replayed_fp64": int(st.n_rechecked),
+            "fp32_decisions_overturned": int(st.n_flipped),
+            "fp64_pair_stage": {"ms_per_batch_device": fp64["stats"].ms_total,
+                                "stages_ms": {"pairs": fp64_staged["stats"].ms_pairs,
+                                              "total_staged": fp64_staged["stats"].ms_total}},
             "robust_solve": batch_c4_solve(ctx, graphs, reps[-1])}
 
 

@@ -260,10 +260,13 @@ int rpgo_max_clique_incremental_bits(rpgo_ctx* ctx, int32_t n, const uint64_t* a
  * packed by prefix offsets, one device launch per stage (the C4 batch).
  * Graph g: poses [pose_off[g], pose_off[g+1]) (pose0 row g), odometry rows
  * [pose_off[g] - g, pose_off[g+1] - g - 1), loop closures [lc_off[g],
- * lc_off[g+1]) with pose indices local to the graph.  Every stage is the FP64
- * replay of the single-graph path (bit-exact with it), so the adjacency of each
- * graph equals rpgo_pcm_run's on that graph alone (all zero when
- * lc_threshold < 0). */
+ * lc_off[g+1]) with pose indices local to the graph.  The fold, odometry
+ * check and compaction are the FP64 replay of the single-graph path; the pairs
+ * are too with params->precision RPGO_PCM_FP64_REPLAY, and with
+ * RPGO_PCM_FP32_RECHECK an SE(3) batch runs the single-graph FP32 stage per
+ * graph with the FP64 replay of the pairs near the threshold.  Either way the
+ * adjacency of each graph equals rpgo_pcm_run's on that graph alone, bit for
+ * bit (all zero when lc_threshold < 0). */
 typedef struct {
   int32_t dim;              /* 3 or 2                                         */
   int32_t n_graphs;
@@ -301,6 +304,11 @@ typedef struct {
   int32_t clique_waves;     /* wavefronts per graph of the clique stage        */
   int32_t graph_replayed;   /* 1 when an existing hipGraph was replayed        */
   int32_t clique_prefetch;  /* rows loaded ahead in the clique stage's walk    */
+  int32_t fp32;             /* 1 when the FP32 pair stage ran (precision FP32_RECHECK,
+                               dim 3, no covariance without rotation information,
+                               at most 2^29 pairs in the batch)                 */
+  int64_t n_rechecked;      /* FP32 stage: pairs replayed in FP64              */
+  int64_t n_flipped;        /* FP32 stage: FP32 decisions the replay overturned */
 } rpgo_pcm_batch_stats;
 
 /* Run the batch.  n_accepted: n_graphs counts; accepted: lc_off[n] entries,

@@ -14,6 +14,11 @@
 //                  accepted count exit) and a wavefront finds its graph by
 //                  binary search
 //   symmetrize     one wavefront per 64 x 64 block, numbered the same way
+// With RPGO_PCM_FP32_RECHECK an SE(3) batch takes the FP32 pair stage of
+// pcm_fp32.hip instead (every graph's anchor tables and records, one pair
+// launch over all graphs writing the lower triangle), and the pairs it queues
+// near the threshold are replayed here in FP64 by the same pair_dist as the
+// FP64 stage, so the adjacency equals the FP64 stage's bit for bit.
 // The heuristic clique (FMC::maxCliqueHeu) then runs per graph on the
 // device-resident adjacency.  Like replay64.hip this unit is built with
 // -ffp-contract=off: a contracted multiply-add would round the fold and the
@@ -41,6 +46,14 @@ struct BatchState {
   DevBuf deg, cidx, ck;  // clique stage: degrees, members, sizes
   DevBuf cb_state, cb_icc, cb_dm;  // clique stage: chip-wide rounds
   std::vector<int> m_host;
+  // FP32 pair stage (SE(3), RPGO_PCM_FP32_RECHECK): per-graph anchor blocks,
+  // anchor-table entries and pair workgroups by prefix, and the stage's tables
+  bool fp32 = false;
+  int fp32_S = 4;
+  std::vector<long long> blk_off, sq_off, pblk_off;
+  DevBuf d_blk_off, d_sq_off, d_pblk_off;
+  DevBuf fp_prec, fp_lcr, fp_pl64, fp_pb64, fp_bend, fp_ta, fp_pb32, fp_sanc, fp_terms, fp_queue;
+  long long fp_qcap = 0;
   // launch shapes chosen by the autotuner, per dim (0: not tuned yet)
   int tuned_pair[4] = {0, 0, 0, 0}, tuned_clique[4] = {0, 0, 0, 0}, tuned_prefetch[4] = {0, 0, 0, 0};
   // captured stages and the layout they were captured for
@@ -223,8 +236,12 @@ __global__ void __launch_bounds__(64 * WPB) k_pairs_batch(int n_graphs, long lon
                                                      const double* __restrict__ traj, const int* __restrict__ cf,
                                                      const int* __restrict__ ct, const double* __restrict__ clcm,
                                                      double thr, unsigned long long* __restrict__ adj,
-                                                     unsigned long long* __restrict__ n_consistent) {
+                                                     unsigned long long* __restrict__ n_consistent,
+                                                     const int* __restrict__ gate) {
   constexpr int PWC = R64<DIM>::PWC;
+  // gate: with the FP32 stage, this FP64 stage runs only when the FP32 one
+  // declined (a covariance without rotation information)
+  if (gate && *gate == 0) return;
   const long long tile = (long long)blockIdx.x * WPB + (threadIdx.x >> 6);
   if (tile >= n_tiles) return;
   const int lane = threadIdx.x & 63;
@@ -282,6 +299,45 @@ __global__ void __launch_bounds__(64) k_sym_batch(int n_graphs, long long n_bloc
   }
 }
 
+// FP64 replay of the pairs the FP32 stage queued: the FP64 stage's pair_dist
+// decides each bit (lower triangle), and the consistent count follows every
+// overturned FP32 decision.  counters: u64[0] consistent pairs, u32[2] queue
+// length, u64[2] overturned decisions.
+template <int DIM>
+__global__ void k_recheck_batch(int n_graphs, const long long* __restrict__ pose_off,
+                                const long long* __restrict__ lc_off, const long long* __restrict__ adj_off,
+                                const int* __restrict__ m_arr, const double* __restrict__ traj,
+                                const int* __restrict__ cf, const int* __restrict__ ct,
+                                const double* __restrict__ clcm, double thr, const int2* __restrict__ queue,
+                                long long qcap, unsigned long long* __restrict__ adj,
+                                unsigned long long* __restrict__ counters) {
+  constexpr int PWC = R64<DIM>::PWC;
+  const unsigned int* qcount = reinterpret_cast<const unsigned int*>(counters + 1);
+  const long long n = min((long long)*qcount, qcap);
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (long long)gridDim.x * blockDim.x) {
+    const int2 e = queue[q];
+    const int g = graph_of(lc_off, n_graphs, e.x);
+    const long long o = lc_off[g];
+    const int i = (int)(e.x - o), j = (int)(e.y - o);
+    const int W = (m_arr[g] + 63) / 64;
+    const double* tr = traj + (size_t)pose_off[g] * PWC;
+    const double d = R64<DIM>::pair_dist(tr, cf[o + i], ct[o + i], clcm + (size_t)(o + i) * PWC, cf[o + j],
+                                         ct[o + j], clcm + (size_t)(o + j) * PWC);
+    const bool bit = d < thr;
+    unsigned long long* word = adj + adj_off[g] + (size_t)j * W + (i >> 6);
+    const unsigned long long mask = 1ull << (i & 63);
+    // only this thread changes this bit: the word's other bits may move meanwhile
+    const bool old = (__hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & mask) != 0ull;
+    if (bit != old) {
+      if (bit) atomicOr(word, mask);
+      else atomicAnd(word, ~mask);
+      atomicAdd(&counters[0], bit ? 1ull : ~0ull);
+      atomicAdd(&counters[2], 1ull);
+    }
+  }
+}
+
 __global__ void k_zero_u64(long long n, unsigned long long* __restrict__ a) {
   for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
     a[k] = 0ull;
@@ -300,7 +356,9 @@ void batch_destroy(rpgo_ctx* ctx) {
   for (DevBuf* b : {&s.d_pose_off, &s.d_lc_off, &s.d_adj_off, &s.d_tile_off, &s.d_sym_off, &s.d_lc_graph, &s.pose0,
                     &s.odom, &s.odom_cov, &s.lc_from, &s.lc_to, &s.lc_pose, &s.lc_cov, &s.delta, &s.hinv, &s.traj,
                     &s.lcm, &s.odom_dist, &s.flags, &s.accepted, &s.m, &s.c_from, &s.c_to, &s.c_lcm, &s.adj,
-                    &s.counters, &s.deg, &s.cidx, &s.ck, &s.cb_state, &s.cb_icc, &s.cb_dm})
+                    &s.counters, &s.deg, &s.cidx, &s.ck, &s.cb_state, &s.cb_icc, &s.cb_dm, &s.d_blk_off,
+                    &s.d_sq_off, &s.d_pblk_off, &s.fp_prec, &s.fp_lcr, &s.fp_pl64, &s.fp_pb64, &s.fp_bend, &s.fp_ta,
+                    &s.fp_pb32, &s.fp_sanc, &s.fp_terms, &s.fp_queue})
     b->release();
   if (s.exec) (void)hipGraphExecDestroy(s.exec);
   if (s.graph) (void)hipGraphDestroy(s.graph);
@@ -316,10 +374,11 @@ static int up(DevBuf& b, const void* h, size_t bytes, hipStream_t st) {
 }
 
 constexpr int kDefaultPairWaves = 4;
+constexpr long long kBatchQueueMax = 1ll << 29;  // recheck queue entries (int2): 4 GiB
 constexpr int kDefaultCliqueWaves = 16;
 
 template <int DIM>
-static void launch_pairs_dim(rpgo_ctx* ctx, BatchState& s, double thr, int pw) {
+static void launch_pairs_dim(rpgo_ctx* ctx, BatchState& s, double thr, int pw, const int* gate) {
   hipStream_t st = ctx->stream;
   const long long n_tiles = s.tile_off[s.n_graphs];
   const unsigned blocks = (unsigned)((n_tiles + pw - 1) / pw);
@@ -328,7 +387,7 @@ static void launch_pairs_dim(rpgo_ctx* ctx, BatchState& s, double thr, int pw) {
       s.n_graphs, n_tiles, s.d_tile_off.as<long long>(), s.d_pose_off.as<long long>(),                        \
       s.d_lc_off.as<long long>(), s.d_adj_off.as<long long>(), s.m.as<int>(), s.traj.as<double>(),            \
       s.c_from.as<int>(), s.c_to.as<int>(), s.c_lcm.as<double>(), thr, s.adj.as<unsigned long long>(),        \
-      s.counters.as<unsigned long long>())
+      s.counters.as<unsigned long long>(), gate)
   switch (pw) {
     case 1: RPGO_PAIRS(1); break;
     case 2: RPGO_PAIRS(2); break;
@@ -336,6 +395,49 @@ static void launch_pairs_dim(rpgo_ctx* ctx, BatchState& s, double thr, int pw) {
     default: RPGO_PAIRS(4); break;
   }
 #undef RPGO_PAIRS
+}
+
+static BatchFp32Args fp32_args(BatchState& s) {
+  BatchFp32Args a;
+  const int G = s.n_graphs;
+  a.G = G;
+  a.n_poses = s.n_poses;
+  a.n_lc = s.n_lc;
+  a.n_odom = s.n_odom;
+  a.n_blocks = s.blk_off[G];
+  a.n_sq = s.sq_off[G];
+  a.n_pair_blocks = s.pblk_off[G];
+  a.S = s.fp32_S;
+  a.pose_off = s.d_pose_off.as<long long>();
+  a.lc_off = s.d_lc_off.as<long long>();
+  a.adj_off = s.d_adj_off.as<long long>();
+  a.blk_off = s.d_blk_off.as<long long>();
+  a.sq_off = s.d_sq_off.as<long long>();
+  a.pblk_off = s.d_pblk_off.as<long long>();
+  a.m = s.m.as<int>();
+  a.delta = s.delta.as<double>();
+  a.hinv = s.hinv.as<double>();
+  a.lcm = s.lcm.as<double>();
+  a.pose0 = s.pose0.as<double>();
+  a.c_from = s.c_from.as<int>();
+  a.c_to = s.c_to.as<int>();
+  a.c_lcm = s.c_lcm.as<double>();
+  a.prec = s.fp_prec.ptr;
+  a.lcr = s.fp_lcr.ptr;
+  a.sanc = s.fp_sanc.ptr;
+  a.pl64 = s.fp_pl64.as<double>();
+  a.pb64 = s.fp_pb64.as<double>();
+  a.bend = s.fp_bend.as<double>();
+  a.ta = s.fp_ta.as<double>();
+  a.terms = s.fp_terms.as<double>();
+  a.pb32 = s.fp_pb32.as<float>();
+  a.adj = s.adj.as<unsigned long long>();
+  a.queue = s.fp_queue.as<int2>();
+  a.qcount = reinterpret_cast<unsigned int*>(s.counters.as<unsigned long long>() + 1);
+  a.qcap = s.fp_qcap;
+  a.n_consistent = s.counters.as<unsigned long long>();
+  a.no_rotation = reinterpret_cast<int*>(s.counters.as<unsigned long long>() + 3);
+  return a;
 }
 
 // pair checks + symmetrize over every graph's tiles (adjacency zeroed first)
@@ -348,8 +450,20 @@ static int launch_pairs(rpgo_ctx* ctx, BatchState& s, const rpgo_pcm_params* prm
   // lc_threshold < 0 turns the pairwise check off (Pcm.h:79-82): the reference
   // then never grows the matrix, and as in rpgo_pcm_run the adjacency stays zero
   if (n_tiles > 0 && prm->lc_threshold >= 0.0) {
-    if (s.dim == 3) launch_pairs_dim<3>(ctx, s, prm->lc_threshold, pw);
-    else launch_pairs_dim<2>(ctx, s, prm->lc_threshold, pw);
+    if (s.fp32) {
+      BatchFp32Args a = fp32_args(s);
+      int rc;
+      if ((rc = pcm_fp32_batch_launch(st, a, prm->lc_threshold, prm->recheck_margin, prm->recheck_chain_coef))) return rc;
+      launch_pairs_dim<3>(ctx, s, prm->lc_threshold, pw, a.no_rotation);
+      k_recheck_batch<3><<<(unsigned)std::min<long long>((s.fp_qcap + 255) / 256, 4096), 256, 0, st>>>(
+          G, s.d_pose_off.as<long long>(), s.d_lc_off.as<long long>(), s.d_adj_off.as<long long>(), s.m.as<int>(),
+          s.traj.as<double>(), s.c_from.as<int>(), s.c_to.as<int>(), s.c_lcm.as<double>(), prm->lc_threshold,
+          s.fp_queue.as<int2>(), s.fp_qcap, s.adj.as<unsigned long long>(), s.counters.as<unsigned long long>());
+    } else if (s.dim == 3) {
+      launch_pairs_dim<3>(ctx, s, prm->lc_threshold, pw, nullptr);
+    } else {
+      launch_pairs_dim<2>(ctx, s, prm->lc_threshold, pw, nullptr);
+    }
     const long long n_blocks = s.sym_off[G];
     k_sym_batch<<<(unsigned)n_blocks, 64, 0, st>>>(G, n_blocks, s.d_sym_off.as<long long>(), s.d_lc_off.as<long long>(),
                                                    s.d_adj_off.as<long long>(), s.m.as<int>(),
@@ -571,6 +685,37 @@ int rpgo_pcm_batch_run(rpgo_ctx* ctx, const rpgo_pcm_batch* B, const rpgo_pcm_pa
     s.tile_off[g + 1] = s.tile_off[g] + 32ll * q * (q + 1);
     s.sym_off[g + 1] = s.sym_off[g] + q * (q + 1) / 2;
   }
+  // FP32 pair stage: SE(3) only, and a recheck queue that holds every pair of
+  // the batch (so no pair near the threshold can be dropped)
+  long long pair_bound = 0;
+  for (int g = 0; g < G; ++g) {
+    const long long nl = s.lc_off[g + 1] - s.lc_off[g];
+    pair_bound += nl * (nl - 1) / 2;
+  }
+  s.fp32 = prm->precision == RPGO_PCM_FP32_RECHECK && dim == 3 && pair_bound <= kBatchQueueMax;
+  if (s.fp32) {
+    s.blk_off.assign(G + 1, 0);
+    s.sq_off.assign(G + 1, 0);
+    s.pblk_off.assign(G + 1, 0);
+    long long words = 0;
+    for (int g = 0; g < G; ++g) {
+      const long long nl = s.lc_off[g + 1] - s.lc_off[g], JT = (nl + 63) / 64;
+      words += JT * (JT + 1) / 2;
+    }
+    // runs per word as in the single-graph stage: about 16 rounds of resident
+    // wavefronts, at most 16 runs
+    int S = 4;
+    while (S < 16 && words * S < 128ll * ctx->n_cu) S *= 2;
+    s.fp32_S = S;
+    for (int g = 0; g < G; ++g) {
+      const long long np = s.pose_off[g + 1] - s.pose_off[g], nb = (np + 63) / 64;
+      const long long nl = s.lc_off[g + 1] - s.lc_off[g], JT = (nl + 63) / 64;
+      s.blk_off[g + 1] = s.blk_off[g] + nb;
+      s.sq_off[g + 1] = s.sq_off[g] + nb * nb;
+      s.pblk_off[g + 1] = s.pblk_off[g] + (long long)(S / 4) * JT * (JT + 1) / 2;
+    }
+    s.fp_qcap = std::max<long long>(pair_bound, 1);
+  }
   std::vector<int> lc_graph((size_t)std::max<long long>(s.n_lc, 1));
   for (int g = 0; g < G; ++g)
     for (long long t = s.lc_off[g]; t < s.lc_off[g + 1]; ++t) lc_graph[t] = g;
@@ -599,6 +744,21 @@ int rpgo_pcm_batch_run(rpgo_ctx* ctx, const rpgo_pcm_batch* B, const rpgo_pcm_pa
       (rc = s.adj.reserve(sizeof(unsigned long long) * (size_t)std::max<long long>(s.adj_off[G], 1))) ||
       (rc = s.counters.reserve(sizeof(unsigned long long) * 4)))
     return rc;
+  if (s.fp32) {
+    const size_t nb = (size_t)std::max<long long>(s.blk_off[G], 1), nsq = (size_t)std::max<long long>(s.sq_off[G], 1);
+    if ((rc = up(s.d_blk_off, s.blk_off.data(), sizeof(long long) * (G + 1), st)) ||
+        (rc = up(s.d_sq_off, s.sq_off.data(), sizeof(long long) * (G + 1), st)) ||
+        (rc = up(s.d_pblk_off, s.pblk_off.data(), sizeof(long long) * (G + 1), st)) ||
+        (rc = s.fp_prec.reserve(pcm_fp32_pose_rec_bytes() * (size_t)s.n_poses)) ||
+        (rc = s.fp_lcr.reserve(pcm_fp32_lc_rec_bytes() * nl1)) ||
+        (rc = s.fp_pl64.reserve(sizeof(double) * 12 * (size_t)s.n_poses)) ||
+        (rc = s.fp_pb64.reserve(sizeof(double) * 12 * nb)) || (rc = s.fp_ta.reserve(sizeof(double) * 12 * nb)) ||
+        (rc = s.fp_bend.reserve(sizeof(double) * 36 * nb)) || (rc = s.fp_pb32.reserve(sizeof(float) * 12 * nb)) ||
+        (rc = s.fp_sanc.reserve(pcm_fp32_anchor_entry_bytes() * nsq)) ||
+        (rc = s.fp_terms.reserve(sizeof(double) * 36 * nsq)) ||
+        (rc = s.fp_queue.reserve(sizeof(int2) * (size_t)s.fp_qcap)))
+      return rc;
+  }
   const bool want_clique = clique_size != nullptr;
   if (want_clique) {
     for (int g = 0; g < G; ++g)
@@ -629,8 +789,18 @@ int rpgo_pcm_batch_run(rpgo_ctx* ctx, const rpgo_pcm_batch* B, const rpgo_pcm_pa
     std::memcpy(&bl, &prm->lc_threshold, 8);
     // every host-side size a captured launch bakes in: the adjacency words the
     // zeroing kernel clears (adj_off[G]) and the clique rounds (largest graph)
+    long long bm, bc;
+    std::memcpy(&bm, &prm->recheck_margin, 8);
+    std::memcpy(&bc, &prm->recheck_chain_coef, 8);
     std::vector<long long> key = {dim, G, (long long)nlc, (long long)nod, s.tile_off[G], s.sym_off[G],
-                                  s.adj_off[G], spread_of(s).max_slots, pw, cw, pf, bo, bl};
+                                  s.adj_off[G], spread_of(s).max_slots, pw, cw, pf, bo, bl, s.fp32 ? 1 : 0,
+                                  bm, bc};
+    if (s.fp32) {
+      key.insert(key.end(), {s.fp32_S, s.blk_off[G], s.sq_off[G], s.pblk_off[G], s.fp_qcap});
+      for (DevBuf* b : {&s.d_blk_off, &s.d_sq_off, &s.d_pblk_off, &s.fp_prec, &s.fp_lcr, &s.fp_pl64, &s.fp_pb64,
+                        &s.fp_bend, &s.fp_ta, &s.fp_pb32, &s.fp_sanc, &s.fp_terms, &s.fp_queue})
+        key.push_back((long long)(uintptr_t)b->ptr);
+    }
     for (DevBuf* b : {&s.d_pose_off, &s.d_lc_off, &s.d_adj_off, &s.d_tile_off, &s.d_sym_off, &s.d_lc_graph,
                       &s.pose0, &s.odom, &s.odom_cov, &s.lc_from, &s.lc_to, &s.lc_pose, &s.lc_cov, &s.delta,
                       &s.hinv, &s.traj, &s.lcm, &s.odom_dist, &s.flags, &s.accepted, &s.m, &s.c_from, &s.c_to,
@@ -692,6 +862,13 @@ int rpgo_pcm_batch_run(rpgo_ctx* ctx, const rpgo_pcm_batch* B, const rpgo_pcm_pa
     *stats = rpgo_pcm_batch_stats{};
     stats->n_pairs = pairs;
     stats->n_consistent = (long long)cnt[0];
+    stats->fp32 = s.fp32 ? 1 : 0;
+    if (s.fp32) {
+      const unsigned int q = (unsigned int)(cnt[1] & 0xffffffffull);
+      stats->n_rechecked = std::min<long long>(q, s.fp_qcap);
+      stats->n_flipped = (long long)cnt[2];
+      if ((int)(cnt[3] & 0xffffffffull)) stats->fp32 = 0;  // a covariance without rotation: FP64 pairs ran
+    }
     if (B->flags & RPGO_BATCH_GRAPH) {
       float t = 0.f;
       (void)hipEventElapsedTime(&t, ev[0], ev[5]);

@@ -100,15 +100,13 @@ __global__ void k_any_no_rotation(int n, const double* __restrict__ rec, int str
   if (k < n && rec[(size_t)k * stride + flag_off] == 0.0) atomicOr(any, 1);
 }
 
-// One thread per anchor block: local poses, forward Lcov, suffix Rcov, and
+// Anchor block p of one graph: local poses, forward Lcov, suffix Rcov, and
 // the block's own segment (Bend) and span (Pblock), all accumulated in FP64.
-__global__ void k_block_local(int n_poses, int n_blocks, const double* __restrict__ delta,
-                              const double* __restrict__ hinv, PoseRec* __restrict__ prec,
-                              double* __restrict__ pl64, double* __restrict__ pblock64,
-                              double* __restrict__ bend64) {
+__device__ __forceinline__ void block_local_body(int p, int n_poses, const double* __restrict__ delta,
+                                                 const double* __restrict__ hinv, PoseRec* __restrict__ prec,
+                                                 double* __restrict__ pl64, double* __restrict__ pblock64,
+                                                 double* __restrict__ bend64) {
   using R = R64<3>;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n_blocks) return;
   const int k0 = p * kB;
   const int k1 = min(k0 + kB, n_poses);  // poses of this block: [k0, k1)
   const bool has_next = k1 < n_poses;    // anchor A_{p+1} = k1 exists
@@ -167,10 +165,19 @@ __global__ void k_block_local(int n_poses, int n_blocks, const double* __restric
   }
 }
 
+// one thread per anchor block
+__global__ void k_block_local(int n_poses, int n_blocks, const double* __restrict__ delta,
+                              const double* __restrict__ hinv, PoseRec* __restrict__ prec,
+                              double* __restrict__ pl64, double* __restrict__ pblock64,
+                              double* __restrict__ bend64) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_blocks) return;
+  block_local_body(p, n_poses, delta, hinv, prec, pl64, pblock64, bend64);
+}
+
 // Anchor poses T_{A_p} in FP64 (sequential over blocks).
-__global__ void k_anchor_poses(int n_blocks, const double* __restrict__ pose0,
-                               const double* __restrict__ pblock64, double* __restrict__ ta) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+__device__ __forceinline__ void anchor_poses_body(int n_blocks, const double* __restrict__ pose0,
+                                                  const double* __restrict__ pblock64, double* __restrict__ ta) {
   double P[12];
 #pragma unroll
   for (int e = 0; e < 12; ++e) P[e] = pose0[e];
@@ -186,12 +193,17 @@ __global__ void k_anchor_poses(int n_blocks, const double* __restrict__ pose0,
   }
 }
 
-// term(r, q) = Ad(T_{A_q}^-1 T_{A_{r+1}}) Bend_r Ad(.)^T for r < q (FP64)
-__global__ void k_anchor_terms(int nb, const double* __restrict__ ta,
-                               const double* __restrict__ bend64, double* __restrict__ terms) {
+__global__ void k_anchor_poses(int n_blocks, const double* __restrict__ pose0,
+                               const double* __restrict__ pblock64, double* __restrict__ ta) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  anchor_poses_body(n_blocks, pose0, pblock64, ta);
+}
+
+// term(r, q) = Ad(T_{A_q}^-1 T_{A_{r+1}}) Bend_r Ad(.)^T for r < q (FP64),
+// entry t = q nb + r
+__device__ __forceinline__ void anchor_terms_body(long long t, int nb, const double* __restrict__ ta,
+                                                  const double* __restrict__ bend64, double* __restrict__ terms) {
   using R = R64<3>;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long long)nb * nb) return;
   const int q = (int)(t / nb), r = (int)(t % nb);
   if (r >= q) return;
   double qi[12], G[12], Ad[36], term[36];
@@ -203,10 +215,16 @@ __global__ void k_anchor_terms(int nb, const double* __restrict__ ta,
   for (int e = 0; e < 36; ++e) terms[(size_t)t * 36 + e] = term[e];
 }
 
+__global__ void k_anchor_terms(int nb, const double* __restrict__ ta,
+                               const double* __restrict__ bend64, double* __restrict__ terms) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nb * nb) return;
+  anchor_terms_body(t, nb, ta, bend64, terms);
+}
+
 // Sanc(r, q) = sum_{r' = r}^{q-1} term(r', q)  (suffix sum over r for each q)
-__global__ void k_anchor_suffix(int nb, const double* __restrict__ terms, S6* __restrict__ sanc) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nb) return;
+__device__ __forceinline__ void anchor_suffix_body(int q, int nb, const double* __restrict__ terms,
+                                                   S6* __restrict__ sanc) {
   double acc[36];
 #pragma unroll
   for (int e = 0; e < 36; ++e) acc[e] = 0.0;
@@ -221,11 +239,14 @@ __global__ void k_anchor_suffix(int nb, const double* __restrict__ terms, S6* __
   }
 }
 
-// Chain covariance trace estimate per pose (margin of the FP64 band).
-__global__ void k_pose_trace(int n_poses, int nb, const S6* __restrict__ sanc,
-                             PoseRec* __restrict__ prec) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n_poses) return;
+__global__ void k_anchor_suffix(int nb, const double* __restrict__ terms, S6* __restrict__ sanc) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nb) return;
+  anchor_suffix_body(q, nb, terms, sanc);
+}
+
+// Chain covariance trace estimate of pose k (margin of the FP64 band).
+__device__ __forceinline__ void pose_trace_body(int k, const S6* __restrict__ sanc, PoseRec* __restrict__ prec) {
   const int p = k / kB;
   PoseRec& r = prec[k];
   P32 Pi;
@@ -235,10 +256,15 @@ __global__ void k_pose_trace(int n_poses, int nb, const S6* __restrict__ sanc,
   r.trc = s6_trace(T) + s6_trace(r.L);
 }
 
-__global__ void k_lc_records(int m, const int* __restrict__ cf, const int* __restrict__ ct,
-                             const double* __restrict__ clcm, LcRec* __restrict__ out) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= m) return;
+__global__ void k_pose_trace(int n_poses, int nb, const S6* __restrict__ sanc,
+                             PoseRec* __restrict__ prec) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_poses) return;
+  pose_trace_body(k, sanc, prec);
+}
+
+__device__ __forceinline__ void lc_record_body(int t, const int* __restrict__ cf, const int* __restrict__ ct,
+                                               const double* __restrict__ clcm, LcRec* __restrict__ out) {
   const double* r = clcm + (size_t)t * R64<3>::PWC;
   LcRec o;
   p32_from64(r, o.Z);
@@ -250,6 +276,13 @@ __global__ void k_lc_records(int m, const int* __restrict__ cf, const int* __res
   o.b = ct[t];
   o.pad = 0;
   out[t] = o;
+}
+
+__global__ void k_lc_records(int m, const int* __restrict__ cf, const int* __restrict__ ct,
+                             const double* __restrict__ clcm, LcRec* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m) return;
+  lc_record_body(t, cf, ct, clcm, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -573,6 +606,195 @@ int pcm_pairs_fp32_launch(rpgo_ctx* ctx, double lc_thr, double margin, double ch
   if (blocks > 0)
     k_pairs32<<<(unsigned)blocks, 64 * kTileWaves, 0, st>>>(m, (int)JT, S, lcr, prec, an, thr2, tau, kdyn,
                                                             W, adj, dist, q, qcount, n_cons);
+  RPGO_HIP_CHECK(hipGetLastError());
+  return RPGO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batched FP32 path: the single-graph tables and pair check per graph of a
+// packed batch (graph g's rows at the offsets of BatchFp32Args)
+// ---------------------------------------------------------------------------
+
+size_t pcm_fp32_pose_rec_bytes() { return sizeof(PoseRec); }
+size_t pcm_fp32_lc_rec_bytes() { return sizeof(LcRec); }
+size_t pcm_fp32_anchor_entry_bytes() { return sizeof(S6); }
+
+// graph of item t: the last g with off[g] <= t
+__device__ __forceinline__ int batch_graph(const long long* __restrict__ off, int n, long long t) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void k_block_local_b(BatchFp32Args a) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n_blocks) return;
+  const int g = batch_graph(a.blk_off, a.G, t);
+  const long long p0 = a.pose_off[g], o0 = p0 - g;  // first pose, first odometry row
+  block_local_body((int)(t - a.blk_off[g]), (int)(a.pose_off[g + 1] - p0), a.delta + o0 * R64<3>::PWC,
+                   a.hinv + o0 * 36, static_cast<PoseRec*>(a.prec) + p0, a.pl64 + p0 * 12,
+                   a.pb64 + a.blk_off[g] * 12, a.bend + a.blk_off[g] * 36);
+}
+
+__global__ void k_anchor_poses_b(BatchFp32Args a) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.G) return;
+  anchor_poses_body((int)(a.blk_off[g + 1] - a.blk_off[g]), a.pose0 + (size_t)g * 12, a.pb64 + a.blk_off[g] * 12,
+                    a.ta + a.blk_off[g] * 12);
+}
+
+__global__ void k_anchor_terms_b(BatchFp32Args a) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n_sq) return;
+  const int g = batch_graph(a.sq_off, a.G, t);
+  anchor_terms_body(t - a.sq_off[g], (int)(a.blk_off[g + 1] - a.blk_off[g]), a.ta + a.blk_off[g] * 12,
+                    a.bend + a.blk_off[g] * 36, a.terms + a.sq_off[g] * 36);
+}
+
+__global__ void k_anchor_suffix_b(BatchFp32Args a) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n_blocks) return;
+  const int g = batch_graph(a.blk_off, a.G, t);
+  anchor_suffix_body((int)(t - a.blk_off[g]), (int)(a.blk_off[g + 1] - a.blk_off[g]), a.terms + a.sq_off[g] * 36,
+                     static_cast<S6*>(a.sanc) + a.sq_off[g]);
+}
+
+__global__ void k_pose_trace_b(BatchFp32Args a) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n_poses) return;
+  const int g = batch_graph(a.pose_off, a.G, t);
+  pose_trace_body((int)(t - a.pose_off[g]), static_cast<const S6*>(a.sanc) + a.sq_off[g],
+                  static_cast<PoseRec*>(a.prec) + a.pose_off[g]);
+}
+
+__global__ void k_lc_records_b(BatchFp32Args a) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n_lc) return;
+  const int g = batch_graph(a.lc_off, a.G, t);
+  const long long o = a.lc_off[g];
+  if (t - o >= a.m[g]) return;  // beyond the accepted loop closures
+  lc_record_body((int)(t - o), a.c_from + o, a.c_to + o, a.c_lcm + o * R64<3>::PWC,
+                 static_cast<LcRec*>(a.lcr) + o);
+}
+
+// The pair check of k_pairs32 over every graph: workgroup b belongs to graph
+// g = the last with pblk_off[g] <= b and takes four runs of one row tile of
+// that graph (tiles numbered from the graph's loop-closure count; tiles at or
+// beyond its accepted count exit).  Bits go to the lower triangle only (row
+// j, bit i < j; the batch's symmetrize fills the upper one), and queued pairs
+// carry batch-wide loop-closure indices.
+__global__ void __launch_bounds__(64 * kTileWaves) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_pairs32_b(BatchFp32Args a, float thr2, float tau, float kdyn) {
+  __shared__ JSide js[64];
+  if (*a.no_rotation) return;
+  const long long blk = blockIdx.x;
+  const int g = batch_graph(a.pblk_off, a.G, blk);
+  const int S = a.S;
+  const long long o = a.lc_off[g];
+  const int n_lc = (int)(a.lc_off[g + 1] - o);
+  const int JTg = (n_lc + 63) / 64;
+  const long long local = blk - a.pblk_off[g];
+  int lo = 0, hi = JTg;  // tile_blocks_before(lo) <= local < tile_blocks_before(hi)
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (tile_blocks_before(mid, S) <= local) lo = mid;
+    else hi = mid;
+  }
+  const int jt = lo;
+  const int m = a.m[g];
+  if (jt * 64 >= m) return;  // tile beyond the accepted loop closures (whole workgroup)
+  const LcRec* __restrict__ lcs = static_cast<const LcRec*>(a.lcr) + o;
+  const PoseRec* __restrict__ prec = static_cast<const PoseRec*>(a.prec) + a.pose_off[g];
+  Anchors an;
+  an.ta = a.ta + a.blk_off[g] * 12;
+  an.pblock = a.pb32 + a.blk_off[g] * 12;
+  an.sanc = static_cast<const S6*>(a.sanc) + a.sq_off[g];
+  an.nb = (int)(a.blk_off[g + 1] - a.blk_off[g]);
+  const int W = (m + 63) / 64;
+  unsigned long long* __restrict__ adj = a.adj + a.adj_off[g];
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+  const int run = (int)(local - tile_blocks_before(jt, S)) * kTileWaves + wave;
+  const int j = jt * 64 + lane;
+  const bool jvalid = j < m;
+  const int jj = jvalid ? j : m - 1;
+  if (wave == 0) js[lane].lj = lcs[jj];
+  else if (wave == 1) js[lane].rc = prec[lcs[jj].a];
+  else if (wave == 2) js[lane].rd = prec[lcs[jj].b];
+  __syncthreads();
+  const int iw = run / S, len = 64 / S;
+  if (iw > jt) return;  // not reached: a tile's runs fill its blocks
+  const int i0 = iw * 64 + (run % S) * len, i1 = min(i0 + len, m);
+  unsigned long long rowbits = 0ull;
+  unsigned int n_cons = 0;
+  for (int i = i0; i < i1; ++i) {
+    int off = lane * (int)sizeof(JSide);
+    asm volatile("" : "+v"(off));
+    const JSide& J = *reinterpret_cast<const JSide*>(reinterpret_cast<const char*>(js) + off);
+    const bool active = jvalid && i < j;
+    const LcRec& li = lcs[i];
+    const PoseRec& ra = prec[li.a];
+    const PoseRec& rb = prec[li.b];
+    bool cons = false;
+    if (active) {
+      P32 TA, TX, TY, TW, TB, TL, TBi;
+      S6 CA, CX, CW, CB, CL;
+      between32(an, li.a, ra, J.lj.a, J.rc, TA, CA);
+      compose_meas(TA, CA, J.lj, TX, CX);
+      p32_inv(TX, TY);
+      compose_meas(TY, CX, li, TW, CW);
+      between32(an, li.b, rb, J.lj.b, J.rd, TB, CB);
+      p32_mul(TW, TB, TL);
+      p32_inv(TB, TBi);
+      s6_congruence_add(TBi, CW, CB, CL);
+      float xi[6];
+      const bool log_ok = se3_log32(TL, xi);
+      float d2, mp;
+      const bool pd = mahal2_32(CL, xi, &d2, &mp);
+      cons = d2 < thr2;
+      const float chain = ra.trc + J.rc.trc + rb.trc + J.rd.trc;
+      const float band = tau + kdyn * chain / fmaxf(mp, 1e-30f);
+      const bool near = fabsf(d2 - thr2) <= band * thr2;
+      if (!log_ok || !pd || near || !(d2 == d2) || d2 > 3.0e38f) {
+        const unsigned int q = atomicAdd(a.qcount, 1u);
+        if ((long long)q < a.qcap) a.queue[q] = make_int2((int)(o + i), (int)(o + j));
+      }
+    }
+    if (cons) rowbits |= 1ull << (i & 63);
+    const unsigned long long colbits = __ballot(cons);
+    if (lane == 0) n_cons += (unsigned)__popcll(colbits);
+  }
+  if (jvalid && rowbits) atomicOr(&adj[(size_t)j * W + iw], rowbits);
+  if (lane == 0 && n_cons) atomicAdd(a.n_consistent, (unsigned long long)n_cons);
+}
+
+int pcm_fp32_batch_launch(hipStream_t st, const BatchFp32Args& a, double lc_thr, double margin,
+                          double chain_coef) {
+  // a covariance without rotation information anywhere: the FP64 pairs instead
+  if (a.n_odom > 0)
+    k_any_no_rotation<<<(unsigned)((a.n_odom + 255) / 256), 256, 0, st>>>((int)a.n_odom, a.delta, R64<3>::PWC, 48,
+                                                                          a.no_rotation);
+  if (a.n_lc > 0)
+    k_any_no_rotation<<<(unsigned)((a.n_lc + 255) / 256), 256, 0, st>>>((int)a.n_lc, a.lcm, R64<3>::PWC, 48,
+                                                                        a.no_rotation);
+  if (a.n_blocks > 0) {
+    k_block_local_b<<<(unsigned)((a.n_blocks + 63) / 64), 64, 0, st>>>(a);
+    k_anchor_poses_b<<<(unsigned)((a.G + 63) / 64), 64, 0, st>>>(a);
+    k_anchor_terms_b<<<(unsigned)((a.n_sq + 255) / 256), 256, 0, st>>>(a);
+    k_anchor_suffix_b<<<(unsigned)((a.n_blocks + 63) / 64), 64, 0, st>>>(a);
+    k_pose_trace_b<<<(unsigned)((a.n_poses + 255) / 256), 256, 0, st>>>(a);
+    k_to_f32<<<(unsigned)((a.n_blocks * 12 + 255) / 256), 256, 0, st>>>(a.n_blocks * 12, a.pb64, a.pb32);
+  }
+  if (a.n_lc > 0) k_lc_records_b<<<(unsigned)((a.n_lc + 255) / 256), 256, 0, st>>>(a);
+  RPGO_HIP_CHECK(hipGetLastError());
+  const float thr2 = (float)(lc_thr * lc_thr);
+  const float tau = (float)(margin > 0.0 ? margin : 2e-3);
+  const float kdyn = (float)(chain_coef > 0.0 ? chain_coef : 16.0 * 2.220446049250313e-16);
+  if (a.n_pair_blocks > 0)
+    k_pairs32_b<<<(unsigned)a.n_pair_blocks, 64 * kTileWaves, 0, st>>>(a, thr2, tau, kdyn);
   RPGO_HIP_CHECK(hipGetLastError());
   return RPGO_OK;
 }

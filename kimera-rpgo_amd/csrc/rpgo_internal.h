@@ -184,6 +184,42 @@ int pcm_pairs_fp32_launch(rpgo_ctx* ctx, double lc_thr, double margin, double ch
 int pcm_count_consistent(rpgo_ctx* ctx);
 constexpr int kRecheckQueueCap = 1 << 22;
 
+// Batched FP32 pair stage (batch.hip drives it, pcm_fp32.hip runs it): every
+// graph of an SE(3) batch gets the single-graph FP32 tables (anchor blocks of
+// 64 poses, records, anchor table) at the offsets below, and one pair launch
+// covers all graphs.  Device pointers; offsets are per-graph prefix sums.
+struct BatchFp32Args {
+  int G = 0;
+  long long n_poses = 0, n_lc = 0, n_odom = 0;
+  long long n_blocks = 0;       // sum of anchor blocks ceil(N_g / 64)
+  long long n_sq = 0;           // sum of nb_g^2 (anchor table entries)
+  long long n_pair_blocks = 0;  // workgroups of the pair launch
+  int S = 4;                    // runs per 64-wide word of older loop closures
+  const long long *pose_off = nullptr, *lc_off = nullptr, *adj_off = nullptr, *blk_off = nullptr,
+                  *sq_off = nullptr, *pblk_off = nullptr;
+  const int* m = nullptr;       // accepted loop closures per graph
+  const double *delta = nullptr, *hinv = nullptr, *lcm = nullptr, *pose0 = nullptr;
+  const int *c_from = nullptr, *c_to = nullptr;
+  const double* c_lcm = nullptr;
+  void *prec = nullptr, *lcr = nullptr, *sanc = nullptr;  // PoseRec, LcRec, S6 (pcm_fp32.hip)
+  double *pl64 = nullptr, *pb64 = nullptr, *bend = nullptr, *ta = nullptr, *terms = nullptr;
+  float* pb32 = nullptr;
+  unsigned long long* adj = nullptr;   // lower triangle written (row j, bits i < j)
+  int2* queue = nullptr;               // (lc_off[g] + i, lc_off[g] + j) of queued pairs
+  unsigned int* qcount = nullptr;
+  long long qcap = 0;
+  unsigned long long* n_consistent = nullptr;
+  int* no_rotation = nullptr;          // set when a covariance lacks rotation information
+};
+// bytes of the per-pose, per-LC, per-block and per-entry records
+size_t pcm_fp32_pose_rec_bytes();
+size_t pcm_fp32_lc_rec_bytes();
+size_t pcm_fp32_anchor_entry_bytes();
+// rotation-flag check, tables and pair launch; the pair launch does nothing
+// when *no_rotation is set (the caller then runs the FP64 pairs)
+int pcm_fp32_batch_launch(hipStream_t st, const BatchFp32Args& a, double lc_thr, double margin,
+                          double chain_coef);
+
 // counters buffer layout (256 bytes, zeroed per pairwise pass)
 //   u64[0] consistent pairs i < j (pair kernels, moved by the FP64 recheck)
 //   u64[1] popcount of the adjacency

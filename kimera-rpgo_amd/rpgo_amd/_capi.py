@@ -87,7 +87,8 @@ class PcmBatchStats(ctypes.Structure):
                 ("ms_odom", ctypes.c_double), ("ms_pairs", ctypes.c_double), ("ms_clique", ctypes.c_double),
                 ("ms_total", ctypes.c_double), ("ms_tune", ctypes.c_double), ("pair_waves", ctypes.c_int32),
                 ("clique_waves", ctypes.c_int32), ("graph_replayed", ctypes.c_int32),
-                ("clique_prefetch", ctypes.c_int32)]
+                ("clique_prefetch", ctypes.c_int32), ("fp32", ctypes.c_int32), ("n_rechecked", ctypes.c_int64),
+                ("n_flipped", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -494,9 +495,10 @@ class Context:
         return poses_l, w_l, [reps[g] for g in range(G)], brep
 
     # ------------------------------------------------------ batched PCM
-    def pcm_batch_run(self, graphs, odom_thr, lc_thr, clique=True, flags=0):
+    def pcm_batch_run(self, graphs, odom_thr, lc_thr, clique=True, flags=0, precision=None):
         """graphs: sequence of synth.PoseGraph (same dim).  Returns dict with per-graph
-        accepted lists, clique sizes / members and the batch stats."""
+        accepted lists, clique sizes / members and the batch stats.  precision:
+        PCM_FP64_REPLAY (the default) or PCM_FP32_RECHECK (SE(3) batches)."""
         dim = graphs[0].dim
         G = len(graphs)
         npz = np.array([g.n_poses for g in graphs], np.int64)
@@ -515,7 +517,8 @@ class Context:
                      lc_off=lc_off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
                      lc_from=_ptr(arrs["lc_from"], _i32), lc_to=_ptr(arrs["lc_to"], _i32),
                      lc_pose=_ptr(arrs["lc_pose"]), lc_cov=_ptr(arrs["lc_cov"]), flags=int(flags))
-        prm = PcmParams(odom_threshold=odom_thr, lc_threshold=lc_thr, precision=PCM_FP64_REPLAY, want_dist=0)
+        prm = PcmParams(odom_threshold=odom_thr, lc_threshold=lc_thr,
+                        precision=PCM_FP64_REPLAY if precision is None else int(precision), want_dist=0)
         n_acc = np.zeros(G, np.int32)
         acc = np.zeros(max(int(lc_off[-1]), 1), np.int32)
         ksz = np.zeros(G, np.int32)

@@ -208,3 +208,57 @@ def test_batch_c4_full_size():
                 acc, adj, k, idx = _oracle_graph(gr, -1.0, 5.0)
                 assert np.array_equal(A, adj), g
                 assert int(res["clique_size"][g]) == k and np.array_equal(res["clique_idx"][g], idx[:max(k, 0)]), g
+
+
+def _adjs(ctx, n):
+    return [ctx.pcm_batch_adj(g).copy() for g in range(n)]
+
+
+def _assert_fp32_equals_fp64(ctx, graphs, odom_thr, lc_thr, flags=0):
+    ref = ctx.pcm_batch_run(graphs, odom_thr, lc_thr, flags=flags)
+    adj_ref = _adjs(ctx, len(graphs))
+    got = ctx.pcm_batch_run(graphs, odom_thr, lc_thr, flags=flags, precision=_capi.PCM_FP32_RECHECK)
+    adj_got = _adjs(ctx, len(graphs))
+    _assert_same(got, ref)
+    for g in range(len(graphs)):
+        assert np.array_equal(adj_got[g], adj_ref[g]), g
+    assert got["stats"].n_consistent == ref["stats"].n_consistent
+    return got["stats"], ref["stats"]
+
+
+@pytest.mark.parametrize("odom_thr,lc_thr", [(-1.0, 5.0), (10.0, 3.0)])
+def test_batch_fp32_equals_fp64_batch(ctx, odom_thr, lc_thr):
+    """The FP32 pair stage of an SE(3) batch (per-graph anchor tables, one pair
+    launch over all graphs, the pairs near the threshold replayed in FP64)
+    gives every graph the FP64 stage's adjacency bit for bit, hence the same
+    cliques; a one-pose-pair graph and graphs of 15-450 loop closures mixed."""
+    graphs = _graphs3(41, 14)
+    graphs.insert(3, synth.se3_helix(n_poses=5, n_lc=1, outlier_frac=1.0, seed=1))
+    st, st_ref = _assert_fp32_equals_fp64(ctx, graphs, odom_thr, lc_thr)
+    print(f"[batch] FP32 stage odom {odom_thr} lc {lc_thr}: {st.n_pairs} pairs, {st.n_rechecked} replayed, "
+          f"{st.n_flipped} overturned; pairs {st.ms_pairs:.2f} ms vs FP64 {st_ref.ms_pairs:.2f} ms", flush=True)
+    assert st.fp32 == 1 and st_ref.fp32 == 0
+    _assert_fp32_equals_fp64(ctx, graphs, odom_thr, lc_thr, flags=_capi.BATCH_GRAPH)
+
+
+def test_batch_fp32_c4_equals_fp64_batch(ctx):
+    """The 256-graph C4 batch (up to several thousand loop closures per graph)
+    in the captured FP32 stage equals the FP64 stage graph by graph."""
+    graphs = synth.c4_batch(0, 256)
+    st, st_ref = _assert_fp32_equals_fp64(ctx, graphs, -1.0, 5.0, flags=_capi.BATCH_GRAPH)
+    print(f"[batch] C4 FP32 stage: {st.n_pairs} pairs, {st.n_rechecked} replayed, {st.n_flipped} overturned; "
+          f"{st.ms_total:.1f} ms vs FP64 {st_ref.ms_total:.1f} ms", flush=True)
+    assert st.fp32 == 1
+
+
+def test_batch_fp32_without_rotation_information_runs_fp64(ctx):
+    """A loop-closure covariance with NaN rotation entries sends the batch's
+    pairs to the FP64 stage (as the single-graph FP32 path does); the results
+    still equal the FP64 batch."""
+    graphs = _graphs3(43, 6)
+    g = graphs[2]
+    cov = g.lc_cov.copy()
+    cov[::5, 7] = np.nan
+    graphs[2] = synth.PoseGraph(**{**g.__dict__, "lc_cov": cov})
+    st, _ = _assert_fp32_equals_fp64(ctx, graphs, -1.0, 4.0)
+    assert st.fp32 == 0 and st.n_rechecked == 0
