@@ -228,7 +228,7 @@ def robust_solve(ctx, gl, **solve_params):
     return res
 
 
-def batch_c4(ctx, seed=0, n_graphs=256, steps=3):
+def batch_c4(ctx, seed=0, n_graphs=256, steps=3, solve=True):
     """C4 batch: 256 independent SE(3) graphs, poses log-uniform in [200, 20000],
     loop closures 0.1-0.3 of the poses, outliers 5-40 %, odometry check off and
     lc_threshold 5.0.  One launch per stage (fold, odometry check, compaction,
@@ -269,7 +269,7 @@ def batch_c4(ctx, seed=0, n_graphs=256, steps=3):
             "fp64_pair_stage": {"ms_per_batch_device": fp64["stats"].ms_total,
                                 "stages_ms": {"pairs": fp64_staged["stats"].ms_pairs,
                                               "total_staged": fp64_staged["stats"].ms_total}},
-            "robust_solve": batch_c4_solve(ctx, graphs, reps[-1])}
+            "robust_solve": batch_c4_solve(ctx, graphs, reps[-1]) if solve else None}
 
 
 def batch_c4_solve(ctx, graphs, pcm, stride=16, lanes=4):
@@ -417,6 +417,8 @@ def main():
                     help="skip the robust-solve measurement on the 50,000-pose graph")
     ap.add_argument("--no-batch", dest="batch", action="store_false",
                     help="skip the 256-graph batch measurement")
+    ap.add_argument("--no-batch-solve", dest="batch_solve", action="store_false",
+                    help="skip the robust solve of the batch's graphs (rpgo_solve_batched)")
     ap.add_argument("--no-exact", dest="exact", action="store_false",
                     help="skip the exact clique")
     ap.add_argument("--solve-poses", type=int, default=50000,
@@ -507,7 +509,7 @@ def main():
     if precision:
         out["precision_vs_fp64"] = precision
     if a.batch:
-        out["batch_c4"] = batch_c4(ctx, seed=a.seed)
+        out["batch_c4"] = batch_c4(ctx, seed=a.seed, solve=a.batch_solve)
     ctx.close()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(

@@ -54,8 +54,8 @@ struct BatchState {
   DevBuf d_blk_off, d_sq_off, d_pblk_off;
   DevBuf fp_prec, fp_lcr, fp_pl64, fp_pb64, fp_bend, fp_ta, fp_pb32, fp_sanc, fp_terms, fp_queue;
   long long fp_qcap = 0;
-  // launch shapes chosen by the autotuner, per dim (0: not tuned yet)
-  int tuned_pair[4] = {0, 0, 0, 0}, tuned_clique[4] = {0, 0, 0, 0}, tuned_prefetch[4] = {0, 0, 0, 0};
+  // launch shapes chosen by the autotuner, per tune_slot() (0: not tuned yet)
+  int tuned_pair[8] = {}, tuned_clique[8] = {}, tuned_prefetch[8] = {};
   // captured stages and the layout they were captured for
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -228,7 +228,10 @@ __device__ inline void tile_of_b(long long t, int* j, int* w) {
   *w = (int)(rem % (q + 1));
 }
 
-template <int DIM, int WPB>
+// STRIDE: a grid smaller than the tiles strides over them (the launch gated
+// behind the FP32 stage); without it one wavefront takes one tile and the
+// kernel keeps its registers (a loop here spills 40 % more)
+template <int DIM, int WPB, bool STRIDE>
 __global__ void __launch_bounds__(64 * WPB) k_pairs_batch(int n_graphs, long long n_tiles, const long long* __restrict__ tile_off,
                                                      const long long* __restrict__ pose_off,
                                                      const long long* __restrict__ lc_off,
@@ -242,28 +245,29 @@ __global__ void __launch_bounds__(64 * WPB) k_pairs_batch(int n_graphs, long lon
   // gate: with the FP32 stage, this FP64 stage runs only when the FP32 one
   // declined (a covariance without rotation information)
   if (gate && *gate == 0) return;
-  const long long tile = (long long)blockIdx.x * WPB + (threadIdx.x >> 6);
-  if (tile >= n_tiles) return;
   const int lane = threadIdx.x & 63;
-  const int g = graph_of(tile_off, n_graphs, tile);
-  int j, w;
-  tile_of_b(tile - tile_off[g], &j, &w);
-  const int m = m_arr[g];
-  if (j >= m) return;
-  const long long o = lc_off[g];
-  const int W = (m + 63) / 64;  // rows of the accepted set, as the clique reads them
-  const double* tr = traj + (size_t)pose_off[g] * PWC;
-  const int i = w * 64 + lane;
-  const bool active = i < j;
-  double d = 0.0;
-  if (active)
-    d = R64<DIM>::pair_dist(tr, cf[o + i], ct[o + i], clcm + (size_t)(o + i) * PWC, cf[o + j], ct[o + j],
-                            clcm + (size_t)(o + j) * PWC);
-  const bool cons = active && d < thr;
-  const unsigned long long bits = __ballot(cons);
-  if (lane == 0) {
-    adj[adj_off[g] + (size_t)j * W + w] = bits;
-    if (bits) atomicAdd(n_consistent, (unsigned long long)__popcll(bits));
+  for (long long tile = (long long)blockIdx.x * WPB + (threadIdx.x >> 6); tile < n_tiles;
+       tile += STRIDE ? (long long)gridDim.x * WPB : n_tiles) {
+    const int g = graph_of(tile_off, n_graphs, tile);
+    int j, w;
+    tile_of_b(tile - tile_off[g], &j, &w);
+    const int m = m_arr[g];
+    if (j >= m) continue;
+    const long long o = lc_off[g];
+    const int W = (m + 63) / 64;  // rows of the accepted set, as the clique reads them
+    const double* tr = traj + (size_t)pose_off[g] * PWC;
+    const int i = w * 64 + lane;
+    const bool active = i < j;
+    double d = 0.0;
+    if (active)
+      d = R64<DIM>::pair_dist(tr, cf[o + i], ct[o + i], clcm + (size_t)(o + i) * PWC, cf[o + j], ct[o + j],
+                              clcm + (size_t)(o + j) * PWC);
+    const bool cons = active && d < thr;
+    const unsigned long long bits = __ballot(cons);
+    if (lane == 0) {
+      adj[adj_off[g] + (size_t)j * W + w] = bits;
+      if (bits) atomicAdd(n_consistent, (unsigned long long)__popcll(bits));
+    }
   }
 }
 
@@ -381,9 +385,19 @@ template <int DIM>
 static void launch_pairs_dim(rpgo_ctx* ctx, BatchState& s, double thr, int pw, const int* gate) {
   hipStream_t st = ctx->stream;
   const long long n_tiles = s.tile_off[s.n_graphs];
-  const unsigned blocks = (unsigned)((n_tiles + pw - 1) / pw);
+  // gated behind the FP32 stage (it runs only when that stage declines): a
+  // small grid that strides over the tiles, so the usual early exit is cheap
+  const long long full = (n_tiles + pw - 1) / pw;
+  const unsigned blocks = (unsigned)(gate ? std::min<long long>(full, 2048) : full);
 #define RPGO_PAIRS(WPB)                                                                                       \
-  k_pairs_batch<DIM, WPB><<<blocks, 64 * WPB, 0, st>>>(                                                      \
+  if (gate)                                                                                                  \
+    k_pairs_batch<DIM, WPB, true><<<blocks, 64 * WPB, 0, st>>>(                                               \
+        s.n_graphs, n_tiles, s.d_tile_off.as<long long>(), s.d_pose_off.as<long long>(),                      \
+        s.d_lc_off.as<long long>(), s.d_adj_off.as<long long>(), s.m.as<int>(), s.traj.as<double>(),          \
+        s.c_from.as<int>(), s.c_to.as<int>(), s.c_lcm.as<double>(), thr, s.adj.as<unsigned long long>(),      \
+        s.counters.as<unsigned long long>(), gate);                                                           \
+  else                                                                                                       \
+  k_pairs_batch<DIM, WPB, false><<<blocks, 64 * WPB, 0, st>>>(                                               \
       s.n_graphs, n_tiles, s.d_tile_off.as<long long>(), s.d_pose_off.as<long long>(),                        \
       s.d_lc_off.as<long long>(), s.d_adj_off.as<long long>(), s.m.as<int>(), s.traj.as<double>(),            \
       s.c_from.as<int>(), s.c_to.as<int>(), s.c_lcm.as<double>(), thr, s.adj.as<unsigned long long>(),        \
@@ -396,6 +410,10 @@ static void launch_pairs_dim(rpgo_ctx* ctx, BatchState& s, double thr, int pw, c
   }
 #undef RPGO_PAIRS
 }
+
+// tuned shapes are kept per dim and pair stage (the FP64 stage's launch
+// shape is only timed when it is the stage that runs)
+static int tune_slot(const BatchState& s) { return s.dim + (s.fp32 ? 4 : 0); }
 
 static BatchFp32Args fp32_args(BatchState& s) {
   BatchFp32Args a;
@@ -586,6 +604,7 @@ static int autotune(rpgo_ctx* ctx, BatchState& s, const rpgo_pcm_params* prm, bo
   float best_t = 1e30f;
   int best_pw = kDefaultPairWaves;
   for (int pw : {1, 2, 4, 8}) {
+    if (s.fp32) break;  // the FP64 stage's shape does not matter there
     float t;
     if ((rc = timed([&] { return launch_pairs(ctx, s, prm, pw); }, &t))) return rc;
     if (t < best_t) {
@@ -593,7 +612,7 @@ static int autotune(rpgo_ctx* ctx, BatchState& s, const rpgo_pcm_params* prm, bo
       best_pw = pw;
     }
   }
-  s.tuned_pair[s.dim] = best_pw;
+  s.tuned_pair[tune_slot(s)] = best_pw;
   if (clique) {
     best_t = 1e30f;
     int best_cw = kDefaultCliqueWaves, best_pf = 1;
@@ -620,8 +639,8 @@ static int autotune(rpgo_ctx* ctx, BatchState& s, const rpgo_pcm_params* prm, bo
       if ((rc = clique_time(best_cw, clique_prefetch_depth(), &t))) return rc;
       if (t < best_t) best_pf = clique_prefetch_depth();
     }
-    s.tuned_clique[s.dim] = best_cw;
-    s.tuned_prefetch[s.dim] = best_pf;
+    s.tuned_clique[tune_slot(s)] = best_cw;
+    s.tuned_prefetch[tune_slot(s)] = best_pf;
   }
   RPGO_HIP_CHECK(hipEventRecord(ev[9], st));
   RPGO_HIP_CHECK(hipEventSynchronize(ev[9]));
@@ -774,12 +793,13 @@ int rpgo_pcm_batch_run(rpgo_ctx* ctx, const rpgo_pcm_batch* B, const rpgo_pcm_pa
       return rc;
   }
   float ms_tune = 0.f;
-  if ((B->flags & RPGO_BATCH_AUTOTUNE) && (!s.tuned_pair[dim] || (want_clique && !s.tuned_clique[dim]))) {
+  const int ts = tune_slot(s);
+  if ((B->flags & RPGO_BATCH_AUTOTUNE) && (!s.tuned_pair[ts] || (want_clique && !s.tuned_clique[ts]))) {
     if ((rc = autotune(ctx, s, prm, want_clique, &ms_tune))) return rc;
   }
-  const int pw = s.tuned_pair[dim] ? s.tuned_pair[dim] : kDefaultPairWaves;
-  const int cw = want_clique ? (s.tuned_clique[dim] ? s.tuned_clique[dim] : kDefaultCliqueWaves) : 0;
-  const int pf = s.tuned_prefetch[dim] ? s.tuned_prefetch[dim] : 1;
+  const int pw = s.tuned_pair[ts] ? s.tuned_pair[ts] : kDefaultPairWaves;
+  const int cw = want_clique ? (s.tuned_clique[ts] ? s.tuned_clique[ts] : kDefaultCliqueWaves) : 0;
+  const int pf = s.tuned_prefetch[ts] ? s.tuned_prefetch[ts] : 1;
   hipEvent_t* ev = ctx->ev;
   int replayed = 0;
   if (B->flags & RPGO_BATCH_GRAPH) {

@@ -687,17 +687,25 @@ __global__ void k_lc_records_b(BatchFp32Args a) {
 // beyond its accepted count exit).  Bits go to the lower triangle only (row
 // j, bit i < j; the batch's symmetrize fills the upper one), and queued pairs
 // carry batch-wide loop-closure indices.
+// The records and offsets come as separate __restrict__ arguments: the older
+// loop closure's records are uniform over the wavefront, and the compiler
+// loads them into SGPRs only from memory it knows the kernel does not write
+// (read through the argument struct they went to VGPRs and spilled).
 __global__ void __launch_bounds__(64 * kTileWaves) __attribute__((amdgpu_waves_per_eu(2, 2)))
-k_pairs32_b(BatchFp32Args a, float thr2, float tau, float kdyn) {
+k_pairs32_b(BatchFp32Args a, const LcRec* __restrict__ lcs_all, const PoseRec* __restrict__ prec_all,
+            const long long* __restrict__ pblk_off, const long long* __restrict__ lc_off,
+            const long long* __restrict__ pose_off, const long long* __restrict__ blk_off,
+            const long long* __restrict__ sq_off, const long long* __restrict__ adj_off,
+            const int* __restrict__ m_arr, float thr2, float tau, float kdyn) {
   __shared__ JSide js[64];
   if (*a.no_rotation) return;
   const long long blk = blockIdx.x;
-  const int g = batch_graph(a.pblk_off, a.G, blk);
+  const int g = batch_graph(pblk_off, a.G, blk);
   const int S = a.S;
-  const long long o = a.lc_off[g];
-  const int n_lc = (int)(a.lc_off[g + 1] - o);
+  const long long o = lc_off[g];
+  const int n_lc = (int)(lc_off[g + 1] - o);
   const int JTg = (n_lc + 63) / 64;
-  const long long local = blk - a.pblk_off[g];
+  const long long local = blk - pblk_off[g];
   int lo = 0, hi = JTg;  // tile_blocks_before(lo) <= local < tile_blocks_before(hi)
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
@@ -705,17 +713,17 @@ k_pairs32_b(BatchFp32Args a, float thr2, float tau, float kdyn) {
     else hi = mid;
   }
   const int jt = lo;
-  const int m = a.m[g];
+  const int m = m_arr[g];
   if (jt * 64 >= m) return;  // tile beyond the accepted loop closures (whole workgroup)
-  const LcRec* __restrict__ lcs = static_cast<const LcRec*>(a.lcr) + o;
-  const PoseRec* __restrict__ prec = static_cast<const PoseRec*>(a.prec) + a.pose_off[g];
+  const LcRec* __restrict__ lcs = lcs_all + o;
+  const PoseRec* __restrict__ prec = prec_all + pose_off[g];
   Anchors an;
-  an.ta = a.ta + a.blk_off[g] * 12;
-  an.pblock = a.pb32 + a.blk_off[g] * 12;
-  an.sanc = static_cast<const S6*>(a.sanc) + a.sq_off[g];
-  an.nb = (int)(a.blk_off[g + 1] - a.blk_off[g]);
+  an.ta = a.ta + blk_off[g] * 12;
+  an.pblock = a.pb32 + blk_off[g] * 12;
+  an.sanc = static_cast<const S6*>(a.sanc) + sq_off[g];
+  an.nb = (int)(blk_off[g + 1] - blk_off[g]);
   const int W = (m + 63) / 64;
-  unsigned long long* __restrict__ adj = a.adj + a.adj_off[g];
+  unsigned long long* __restrict__ adj = a.adj + adj_off[g];
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
   const int run = (int)(local - tile_blocks_before(jt, S)) * kTileWaves + wave;
   const int j = jt * 64 + lane;
@@ -794,7 +802,9 @@ int pcm_fp32_batch_launch(hipStream_t st, const BatchFp32Args& a, double lc_thr,
   const float tau = (float)(margin > 0.0 ? margin : 2e-3);
   const float kdyn = (float)(chain_coef > 0.0 ? chain_coef : 16.0 * 2.220446049250313e-16);
   if (a.n_pair_blocks > 0)
-    k_pairs32_b<<<(unsigned)a.n_pair_blocks, 64 * kTileWaves, 0, st>>>(a, thr2, tau, kdyn);
+    k_pairs32_b<<<(unsigned)a.n_pair_blocks, 64 * kTileWaves, 0, st>>>(
+        a, static_cast<const LcRec*>(a.lcr), static_cast<const PoseRec*>(a.prec), a.pblk_off, a.lc_off, a.pose_off,
+        a.blk_off, a.sq_off, a.adj_off, a.m, thr2, tau, kdyn);
   RPGO_HIP_CHECK(hipGetLastError());
   return RPGO_OK;
 }
