@@ -2,13 +2,14 @@
 // features from the PCM state, and a 16 -> 128 -> 128 -> 2 MLP (ReLU) that
 // predicts (initial-weight logit, log mu) per loop closure, trained on the
 // weights the plain GNC converges to.  Forward, backward and the weight
-// gradients run on the matrix cores (v_mfma_f32_16x16x16_bf16: BF16 operands,
-// FP32 accumulation); the optimizer is Adam on FP32 master weights.
+// gradients run on the matrix cores (gfx950's v_mfma_f32_16x16x32_bf16: BF16
+// operands, FP32 accumulation); the optimizer is Adam on FP32 master weights.
 //
-// MFMA 16x16x16 operand layout (one wavefront, lane l, q = l / 16, r = l % 16):
-//   A (M x K): lane holds A[r][4q + j], j = 0..3
-//   B (K x N): lane holds B[4q + j][r]
-//   D (M x N): lane holds D[4q + j][r]
+// MFMA 16x16x32 operand layout (one wavefront, lane l, q = l / 16, r = l % 16;
+// checked on MI355X against a host product):
+//   A (M x K): lane holds A[r][8q + j], j = 0..7
+//   B (K x N): lane holds B[8q + j][r]
+//   D (M x N): lane holds D[4q + j][r], j = 0..3
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -19,7 +20,8 @@
 
 namespace rpgo {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x8 __attribute__((ext_vector_type(8)));  // BF16 bits
+typedef __bf16 bf16v8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short bf16;
 
@@ -45,8 +47,9 @@ __device__ __forceinline__ bf16 f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(bf16 h) { return __uint_as_float((unsigned)h << 16); }
 
-__device__ __forceinline__ f32x4 mfma16(const bf16x4& a, const bf16x4& b, const f32x4& c) {
-  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+__device__ __forceinline__ f32x4 mfma32(const bf16x8& a, const bf16x8& b, const f32x4& c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16v8, a), __builtin_bit_cast(bf16v8, b), c,
+                                                 0, 0, 0);
 }
 
 __device__ __forceinline__ double wave_sum_l(double v) {
@@ -177,15 +180,15 @@ __global__ void __launch_bounds__(256) k_mlp_forward(int n, const float* __restr
   const float* b1 = P + oB1;
   const float* b2 = P + oB2;
   const float* b3 = P + oB3;
-  // layer 1: A = normalised features (16 x 16), B = W1 (16 x 128)
-  bf16x4 a;
+  // layer 1: A = normalised features (16 x 16, K zero-padded to 32), B = W1 (16 x 128)
+  bf16x8 a;
   {
     const int row = row0 + r;
-    const bool ok = row < n;
+    const bool ok = row < n && q < 2;
     const int src = ok ? (idx ? idx[row] : row) : 0;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int c = 4 * q + j;
+    for (int j = 0; j < 8; ++j) {
+      const int c = (8 * q + j) & (kF - 1);
       const float x = ok ? (X[(size_t)src * kF + c] - norm[c]) * norm[kF + c] : 0.f;
       const bf16 h = f2bf(x);
       a[j] = (short)h;
@@ -194,11 +197,11 @@ __global__ void __launch_bounds__(256) k_mlp_forward(int n, const float* __restr
   }
 #pragma unroll
   for (int nt = 0; nt < kH / 16; ++nt) {
-    bf16x4 b;
+    bf16x8 b;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) b[j] = (short)W1t[(nt * 16 + r) * kF + 4 * q + j];
+    for (int j = 0; j < 8; ++j) b[j] = q < 2 ? (short)W1t[(nt * 16 + r) * kF + 8 * q + j] : (short)0;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    acc = mfma16(a, b, acc);
+    acc = mfma32(a, b, acc);
     const int col = nt * 16 + r;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -214,14 +217,14 @@ __global__ void __launch_bounds__(256) k_mlp_forward(int n, const float* __restr
   for (int nt = 0; nt < kH / 16; ++nt) {
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kt = 0; kt < kH / 16; ++kt) {
-      bf16x4 aa, bb;
+    for (int kt = 0; kt < kH / 32; ++kt) {
+      bf16x8 aa, bb;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        aa[j] = (short)act1[w][r][kt * 16 + 4 * q + j];
-        bb[j] = (short)sW2[(nt * 16 + r) * kH + kt * 16 + 4 * q + j];
+      for (int j = 0; j < 8; ++j) {
+        aa[j] = (short)act1[w][r][kt * 32 + 8 * q + j];
+        bb[j] = (short)sW2[(nt * 16 + r) * kH + kt * 32 + 8 * q + j];
       }
-      acc = mfma16(aa, bb, acc);
+      acc = mfma32(aa, bb, acc);
     }
     const int col = nt * 16 + r;
 #pragma unroll
@@ -236,14 +239,14 @@ __global__ void __launch_bounds__(256) k_mlp_forward(int n, const float* __restr
   // layer 3: A = H2 (16 x 128), B = W3 padded to 128 x 16
   f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int kt = 0; kt < kH / 16; ++kt) {
-    bf16x4 aa, bb;
+  for (int kt = 0; kt < kH / 32; ++kt) {
+    bf16x8 aa, bb;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      aa[j] = (short)act2[w][r][kt * 16 + 4 * q + j];
-      bb[j] = (short)W3t[r * kH + kt * 16 + 4 * q + j];
+    for (int j = 0; j < 8; ++j) {
+      aa[j] = (short)act2[w][r][kt * 32 + 8 * q + j];
+      bb[j] = (short)W3t[r * kH + kt * 32 + 8 * q + j];
     }
-    acc = mfma16(aa, bb, acc);
+    acc = mfma32(aa, bb, acc);
   }
   if (r < kO) {
 #pragma unroll
@@ -308,14 +311,14 @@ __global__ void __launch_bounds__(256) k_mlp_back_h1(int n, const bf16* __restri
   for (int nt = 0; nt < kH / 16; ++nt) {
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kt = 0; kt < kH / 16; ++kt) {
-      bf16x4 aa, bb;
+    for (int kt = 0; kt < kH / 32; ++kt) {
+      bf16x8 aa, bb;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        aa[j] = arow < n ? (short)dH2[(size_t)arow * kH + kt * 16 + 4 * q + j] : (short)0;
-        bb[j] = (short)sW2[(nt * 16 + r) * kH + kt * 16 + 4 * q + j];
+      for (int j = 0; j < 8; ++j) {
+        aa[j] = arow < n ? (short)dH2[(size_t)arow * kH + kt * 32 + 8 * q + j] : (short)0;
+        bb[j] = (short)sW2[(nt * 16 + r) * kH + kt * 32 + 8 * q + j];
       }
-      acc = mfma16(aa, bb, acc);
+      acc = mfma32(aa, bb, acc);
     }
     const int col = nt * 16 + r;
 #pragma unroll
@@ -338,16 +341,16 @@ __global__ void __launch_bounds__(256) k_mlp_wgrad(int n, const bf16* __restrict
   const int chunk = (n + gridDim.z - 1) / gridDim.z;
   const int rb = blockIdx.z * chunk, re = min(n, rb + chunk);
   f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-  for (int r0 = rb + w * 16; r0 < re; r0 += 64) {
-    bf16x4 aa, bb;
+  for (int r0 = rb + w * 32; r0 < re; r0 += 128) {
+    bf16x8 aa, bb;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int row = r0 + 4 * q + j;
+    for (int j = 0; j < 8; ++j) {
+      const int row = r0 + 8 * q + j;
       const bool ok = row < re;
       aa[j] = ok ? (short)A[(size_t)row * lda + m0 + r] : (short)0;
       bb[j] = ok ? (short)B[(size_t)row * ldb + n0 + r] : (short)0;
     }
-    acc = mfma16(aa, bb, acc);
+    acc = mfma32(aa, bb, acc);
   }
 #pragma unroll
   for (int j = 0; j < 4; ++j) part[w][4 * q + j][r] = acc[j];
