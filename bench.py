@@ -48,6 +48,11 @@ TRAFFIC_FILES = {"se3_5k": os.path.join(ROOT, "profiles", "k_pairs32_se3_5k_traf
 SOLVE_FIXTURE = os.path.join(ROOT, "tests", "golden", "solve_se3_50k.npz")
 
 
+def progress(msg):
+    """one progress line per bench leg, on stderr (stdout carries only the JSON line)"""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def measured_traffic(workload):
     """bytes per launch recorded for the workload ('traffic_bytes_per_launch: N'), or None"""
     try:
@@ -139,6 +144,7 @@ def exact_clique(ctx, heu_size):
     from rpgo_amd import _capi
 
     t0 = time.perf_counter()
+    progress(f"exact clique (heuristic size {heu_size})")
     try:
         k, _, ms = ctx.max_clique(_capi.CLIQUE_EXACT)
         return {"size": k, "ms": ms, "ge_heuristic": bool(k >= heu_size)}
@@ -280,6 +286,7 @@ def batch_c4_solve(ctx, graphs, pcm, stride=16, lanes=4):
     from rpgo_amd import problem
 
     sel = list(range(0, len(graphs), stride))
+    progress(f"C4 robust solve: {len(sel)} graphs, {lanes} lanes and one lane")
     probs, known = [], []
     for g in sel:
         arrs = problem.solve_arrays(graphs[g], pcm["accepted"][g][pcm["clique_idx"][g]])
@@ -440,6 +447,7 @@ def main():
     ctx = _capi.Context(0)
     # ingest (fold, odometry check, FP32 tables) is outside the timed region, as
     # in the reference where the trajectory is folded when odometry arrives
+    progress(f"headline: {a.workload} pairwise stage, {a.warmup} + {a.steps} steps")
     r = pcm_pass(ctx, g, w, prec, a.steps, a.warmup)
     pairs, ms_k = r["pairs"], r["ms_kernel"]
     achieved = FLOP_PER_PAIR[g.dim] * pairs / (ms_k * 1e-3) / 1e12
@@ -483,6 +491,7 @@ def main():
         other = "se3_5k" if a.workload == "se3_50k" else "se3_50k"
         wo = WORKLOADS[other]
         go = graph(other, a.seed)
+        progress(f"side workload: {other} pairwise stage")
         ro = pcm_pass(ctx, go, wo, prec, a.steps, a.warmup)
         ko, _, mso = ctx.max_clique(_capi.CLIQUE_HEU_REF)
         out[other] = {"pairs": ro["pairs"], "pair_checks_per_s": ro["pairs"] / (ro["ms_step"] * 1e-3),
@@ -499,6 +508,7 @@ def main():
     if a.solve:
         gs = g if (a.workload == "se3_50k" and a.solve_poses == 50000) else synth.se3_helix(
             n_poses=a.solve_poses, n_lc=a.solve_poses // 5, outlier_frac=0.5, seed=a.seed)
+        progress(f"robust solve: {gs.n_poses} poses, plain GNC and the learned start")
         solve_res = robust_solve(ctx, gs, pcg_graphs=a.pcg_graphs)
         key = "robust_solve_50k" if a.solve_poses == 50000 else f"robust_solve_{a.solve_poses}"
         out[key] = {k_: v for k_, v in solve_res.items() if not k_.startswith("_")}
@@ -509,9 +519,11 @@ def main():
     if precision:
         out["precision_vs_fp64"] = precision
     if a.batch:
+        progress("C4 batch: 256 graphs (PCM FP32 and FP64 stages, then the batched robust solve)")
         out["batch_c4"] = batch_c4(ctx, seed=a.seed, solve=a.batch_solve)
     ctx.close()
     if not a.no_cpu:
+        progress("CPU baseline (FP64 oracle on the host cores)")
         out["cpu_baseline"] = cpu_baseline(
             a.workload, g, solve_graph=gs if a.solve else None,
             solve_inliers=solve_res["_inliers"] if solve_res else None,

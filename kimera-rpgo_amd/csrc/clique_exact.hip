@@ -18,8 +18,21 @@
 //      with best = omega - 1 (branches that cannot reach omega are the only
 //      ones this bound removes, so the first omega-clique met is the one the
 //      sequential search records).
-// A node budget bounds the work; exceeding it is reported as an error.
+// Besides FMC's own bounds every node is also bounded by a greedy colouring of
+// its candidate set: a clique holds at most one vertex of each colour class,
+// so a node whose clique plus its class count cannot beat the bound is
+// pruned.  Like FMC's bounds this removes only branches that cannot hold a
+// larger (phase A) or an omega-sized (phases B, C) clique, so the search
+// order, omega and the first omega-clique met are unchanged; what it changes
+// is the size of the tree on dense inlier blocks, where degrees bound
+// nothing and FMC's search visits every way of dropping one endpoint of each
+// missing edge.
+// A work budget (nodes plus colouring steps) and a time limit per phase
+// (the device's constant-rate wall clock) bound the search; exceeding either
+// is reported as an error.  Every wavefront checks both every 4,096 units
+// of work, so each phase ends within about that much past its limit.
 #include <algorithm>
+#include <climits>
 #include <vector>
 
 #include "rpgo_internal.h"
@@ -29,17 +42,84 @@ namespace rpgo {
 struct ExactCtl {
   unsigned long long best;  // (size << 32) | seed of the best clique so far
   int next_seed;            // seeds still to claim, counted down
-  int aborted;              // node budget exceeded
+  int aborted;              // 1: work budget exceeded, 2: time limit
   unsigned long long nodes;
   unsigned long long found;  // phase B: highest seed with an omega-clique (+1), atomicMax
 };
 
 constexpr int kExactWaves = 256;
+constexpr int kExactPhaseSeconds = 20;  // device time limit of each phase
 
 __device__ __forceinline__ int wsum_i(int v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
   return v;
+}
+
+__device__ __forceinline__ int wmin_i(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const int o = __shfl_xor(v, off);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+constexpr int kExWPL = 8;  // words per lane of a candidate set (n <= 32768)
+
+// Greedy colouring of the candidate set U (W words, word w on lane w % 64):
+// each class is built from the lowest uncoloured vertex up, dropping the
+// neighbours of every vertex it takes, so it is an independent set.  Returns
+// true when U fits in `limit` classes; stops with false as soon as it needs
+// more.  *steps counts the vertices coloured (the work).
+__device__ bool colour_fits(const unsigned long long* __restrict__ U, int W,
+                            const unsigned long long* __restrict__ adj, int lane, int limit, int* steps) {
+  if (limit <= 0) return false;
+  unsigned long long P[kExWPL];
+#pragma unroll
+  for (int k = 0; k < kExWPL; ++k) {
+    const int w = lane + 64 * k;
+    P[k] = w < W ? U[w] : 0ull;
+  }
+  int classes = 0, st = 0;
+  for (;;) {
+    bool left = false;
+#pragma unroll
+    for (int k = 0; k < kExWPL; ++k) left |= P[k] != 0ull;
+    if (__ballot(left) == 0ull) {
+      *steps += st;
+      return true;
+    }
+    if (classes == limit) {
+      *steps += st;
+      return false;
+    }
+    ++classes;
+    unsigned long long Q[kExWPL];
+#pragma unroll
+    for (int k = 0; k < kExWPL; ++k) Q[k] = P[k];
+    for (;;) {
+      int v = INT_MAX;
+#pragma unroll
+      for (int k = kExWPL - 1; k >= 0; --k)
+        if (Q[k]) v = (lane + 64 * k) * 64 + __builtin_ctzll(Q[k]);
+      v = wmin_i(v);
+      if (v == INT_MAX) break;
+      ++st;
+      const int wv = v >> 6;
+      const unsigned long long bit = 1ull << (v & 63);
+      const unsigned long long* __restrict__ row = adj + (size_t)v * W;
+#pragma unroll
+      for (int k = 0; k < kExWPL; ++k) {
+        const int w = lane + 64 * k;
+        if (w == wv) {
+          P[k] &= ~bit;
+          Q[k] &= ~bit;
+        }
+        if (w < W) Q[k] &= ~row[w];
+      }
+    }
+  }
 }
 
 __device__ __forceinline__ int wmax_i(int v) {
@@ -67,8 +147,9 @@ __global__ void __launch_bounds__(256) k_exact(int n, int W, const unsigned long
                                                const unsigned long long* __restrict__ dmask, int om,
                                                int seed_lo, ExactCtl* ctl, unsigned long long* __restrict__ stacks,
                                                int depth_cap, int* __restrict__ path, int* __restrict__ out,
-                                               long long node_budget) {
+                                               long long node_budget, long long clock_limit) {
   const int lane = threadIdx.x & 63;
+  const unsigned long long t_start = wall_clock64();
   const int wid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   unsigned long long* stk = stacks + (size_t)wid * depth_cap * W;
   int* pth = path + (size_t)wid * depth_cap;
@@ -129,6 +210,15 @@ __global__ void __launch_bounds__(256) k_exact(int n, int W, const unsigned long
         --depth;
         continue;
       }
+      {
+        int steps = 0;
+        const bool fits = colour_fits(U, W, adj, lane, bound - size, &steps);
+        local_nodes += steps;
+        if (fits) {
+          --depth;
+          continue;
+        }
+      }
       // pop the highest candidate and build the child's candidate set
       if (lane == ((top >> 6) & 63)) U[top >> 6] &= ~(1ull << (top & 63));
       __builtin_amdgcn_wave_barrier();
@@ -156,8 +246,9 @@ __global__ void __launch_bounds__(256) k_exact(int n, int W, const unsigned long
         if (lane == 0) tot = atomicAdd(&ctl->nodes, (unsigned long long)local_nodes) + local_nodes;
         tot = __shfl(tot, 0);
         local_nodes = 0;
-        if ((long long)tot > node_budget) {
-          if (lane == 0) atomicExch(&ctl->aborted, 1);
+        const bool late = (long long)(wall_clock64() - t_start) > clock_limit;
+        if ((long long)tot > node_budget || late) {
+          if (lane == 0) atomicMax(&ctl->aborted, late ? 2 : 1);
           return;
         }
       }
@@ -247,6 +338,12 @@ int clique_exact_run(rpgo_ctx* ctx, const unsigned long long* adj_in, int n, int
   if ((rc = s.ex_stack.reserve(sizeof(unsigned long long) * stack_words))) return rc;
   if ((rc = s.ex_path.reserve(sizeof(int) * (size_t)kExactWaves * depth_cap))) return rc;
   ExactCtl* ctl = s.ex_ctl.as<ExactCtl>();
+  // time limit of each phase in wall-clock ticks (rate in kHz)
+  int clock_khz = 0;
+  if (hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, ctx->device) != hipSuccess ||
+      clock_khz <= 0)
+    clock_khz = 100000;
+  const long long clock_limit = (long long)clock_khz * 1000ll * kExactPhaseSeconds;
   // phase A: omega, seeded with a clique size known to exist
   const int b0 = std::max(0, lower_bound - 1);
   ExactCtl h{};
@@ -256,12 +353,15 @@ int clique_exact_run(rpgo_ctx* ctx, const unsigned long long* adj_in, int n, int
   k_exact_dmask<<<(W + 255) / 256, 256, 0, st>>>(n, W, b0, s.deg.as<int>(), s.dmask.as<unsigned long long>());
   k_exact<0><<<kExactWaves / 4, 256, 0, st>>>(n, W, adj, s.deg.as<int>(), s.dmask.as<unsigned long long>(), 0, 0,
                                               ctl, s.ex_stack.as<unsigned long long>(), depth_cap,
-                                              s.ex_path.as<int>(), s.out.as<int>(), node_budget);
+                                              s.ex_path.as<int>(), s.out.as<int>(), node_budget, clock_limit);
   RPGO_HIP_CHECK(hipGetLastError());
   RPGO_HIP_CHECK(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, st));
   RPGO_HIP_CHECK(hipStreamSynchronize(st));
   if (h.aborted) {
-    set_error("rpgo_max_clique: exact search exceeded the node budget (%lld)", node_budget);
+    if (h.aborted == 2)
+      set_error("rpgo_max_clique: exact search exceeded its time limit (%d s)", kExactPhaseSeconds);
+    else
+      set_error("rpgo_max_clique: exact search exceeded the node budget (%lld)", node_budget);
     return RPGO_ERR_STATE;
   }
   const int om = (int)(h.best >> 32);
@@ -278,23 +378,33 @@ int clique_exact_run(rpgo_ctx* ctx, const unsigned long long* adj_in, int n, int
   k_exact_dmask<<<(W + 255) / 256, 256, 0, st>>>(n, W, om - 1, s.deg.as<int>(), s.dmask.as<unsigned long long>());
   k_exact<1><<<kExactWaves / 4, 256, 0, st>>>(n, W, adj, s.deg.as<int>(), s.dmask.as<unsigned long long>(), om,
                                               seed_a, ctl, s.ex_stack.as<unsigned long long>(), depth_cap,
-                                              s.ex_path.as<int>(), s.out.as<int>(), node_budget);
+                                              s.ex_path.as<int>(), s.out.as<int>(), node_budget, clock_limit);
   RPGO_HIP_CHECK(hipGetLastError());
   RPGO_HIP_CHECK(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, st));
   RPGO_HIP_CHECK(hipStreamSynchronize(st));
   if (h.aborted || h.found == 0) {
-    set_error("rpgo_max_clique: exact search %s", h.aborted ? "exceeded the node budget" : "lost the maximum clique");
+    set_error("rpgo_max_clique: exact search %s",
+              h.aborted == 2 ? "exceeded its time limit"
+                             : (h.aborted ? "exceeded the node budget" : "lost the maximum clique"));
     return RPGO_ERR_STATE;
   }
   const int seed = (int)(h.found - 1);
+  h.nodes = 0;
+  RPGO_HIP_CHECK(hipMemcpyAsync(ctl, &h, sizeof(h), hipMemcpyHostToDevice, st));
   // phase C: replay the winning seed
   k_exact<2><<<1, 64, 0, st>>>(n, W, adj, s.deg.as<int>(), s.dmask.as<unsigned long long>(), om, seed, ctl,
                                s.ex_stack.as<unsigned long long>(), depth_cap, s.ex_path.as<int>(), s.out.as<int>(),
-                               node_budget);
+                               node_budget, clock_limit);
   RPGO_HIP_CHECK(hipGetLastError());
   int k = 0;
   RPGO_HIP_CHECK(hipMemcpyAsync(&k, s.out.as<int>() + n, sizeof(int), hipMemcpyDeviceToHost, st));
+  RPGO_HIP_CHECK(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, st));
   RPGO_HIP_CHECK(hipStreamSynchronize(st));
+  if (h.aborted) {  // the replay stopped before writing its result
+    set_error("rpgo_max_clique: exact search %s in its replay",
+              h.aborted == 2 ? "exceeded its time limit" : "exceeded the node budget");
+    return RPGO_ERR_STATE;
+  }
   if (k != om) {
     set_error("rpgo_max_clique: replay found size %d, expected %d", k, om);
     return RPGO_ERR_STATE;

@@ -92,3 +92,60 @@ def test_incremental_heuristic_bit_exact(ctx, seed):
         assert k == kr, (seed, t, n, num_new, prev, k, kr)
         assert list(idx[:k]) == list(ir[:kr]), (seed, t)
     print(f"[clique] incremental seed {seed}: 40 graphs bit-exact", flush=True)
+
+
+def _dense_block_graph(rng, n, frac, missing, p_noise):
+    """A near-complete block (frac of the vertices, `missing` random non-edges
+    inside it) in a sparse random graph: the PCM inlier-block shape, where
+    degrees bound nothing and the greedy-colouring bound does the pruning."""
+    a = np.triu(rng.random((n, n)) < p_noise, 1)
+    a = a | a.T
+    blk = np.sort(rng.choice(n, max(2, int(frac * n)), replace=False))
+    a[np.ix_(blk, blk)] = True
+    for _ in range(missing):
+        u, v = rng.choice(blk, 2, replace=False)
+        a[u, v] = a[v, u] = False
+    np.fill_diagonal(a, False)
+    return a.astype(np.uint8)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_dense_blocks_bit_exact(ctx, seed):
+    """Near-complete blocks with up to 12 missing edges (the sequential search
+    still finishes): size and vertex order equal FMC::maxClique's."""
+    rng = np.random.default_rng(300 + seed)
+    for t in range(12):
+        n = int(rng.integers(30, 130))
+        a = _dense_block_graph(rng, n, float(rng.uniform(0.5, 0.95)), int(rng.integers(0, 13)),
+                               float(rng.uniform(0.02, 0.3)))
+        kr, ir = O.max_clique_exact(a)
+        k, idx = _gpu_exact(ctx, a)
+        assert k == kr, (seed, t, n, k, kr)
+        assert list(idx[:k]) == list(ir), (seed, t, n)
+    print(f"[clique] exact dense blocks seed {seed}: 12 graphs bit-exact", flush=True)
+
+
+def test_exact_on_the_5k_pcm_graph(ctx):
+    """The 5k workload's consistency graph (1,000 loop closures, a 768-member
+    heuristic clique in a near-complete inlier block): the exact search
+    finishes, returns a clique of the adjacency at least as large as the
+    heuristic's, listed deepest vertex first and with the seed (its highest
+    vertex) last."""
+    import time
+
+    g, ot, lt = synth.workload("se3_5k", seed=0)
+    ctx.pcm_upload(g.dim, g.pose0, g.odom, g.odom_cov, g.lc_from, g.lc_to, g.lc_pose, g.lc_cov)
+    ctx.pcm_run(ot, lt, precision=_capi.PCM_FP32_RECHECK)
+    m = len(ctx.pcm_download()["accepted"])
+    kh, ih, _ = ctx.max_clique(_capi.CLIQUE_HEU_REF)
+    t = time.perf_counter()
+    k, idx, _ = ctx.max_clique(_capi.CLIQUE_EXACT)
+    dt = time.perf_counter() - t
+    A = _capi.unpack_bits(ctx.pcm_download()["adj_bits"], m)
+    print(f"[clique] 5k exact: size {k} (heuristic {kh}) in {dt:.2f} s", flush=True)
+    assert k >= kh
+    sel = np.asarray(idx[:k])
+    assert len(set(sel.tolist())) == k
+    sub = A[np.ix_(sel, sel)]
+    assert sub.sum() == k * (k - 1), "not a clique"
+    assert sel[-1] == sel.max()
