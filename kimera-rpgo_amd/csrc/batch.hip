@@ -52,7 +52,7 @@ struct BatchState {
   int fp32_S = 4;
   std::vector<long long> blk_off, sq_off, pblk_off;
   DevBuf d_blk_off, d_sq_off, d_pblk_off;
-  DevBuf fp_prec, fp_lcr, fp_pl64, fp_pb64, fp_bend, fp_ta, fp_pb32, fp_sanc, fp_terms, fp_queue;
+  DevBuf fp_prec, fp_lcr, fp_pl64, fp_pb64, fp_bend, fp_ta, fp_sanc, fp_terms, fp_queue;
   long long fp_qcap = 0;
   // launch shapes chosen by the autotuner, per tune_slot() (0: not tuned yet)
   int tuned_pair[8] = {}, tuned_clique[8] = {}, tuned_prefetch[8] = {};
@@ -362,7 +362,7 @@ void batch_destroy(rpgo_ctx* ctx) {
                     &s.lcm, &s.odom_dist, &s.flags, &s.accepted, &s.m, &s.c_from, &s.c_to, &s.c_lcm, &s.adj,
                     &s.counters, &s.deg, &s.cidx, &s.ck, &s.cb_state, &s.cb_icc, &s.cb_dm, &s.d_blk_off,
                     &s.d_sq_off, &s.d_pblk_off, &s.fp_prec, &s.fp_lcr, &s.fp_pl64, &s.fp_pb64, &s.fp_bend, &s.fp_ta,
-                    &s.fp_pb32, &s.fp_sanc, &s.fp_terms, &s.fp_queue})
+                    &s.fp_sanc, &s.fp_terms, &s.fp_queue})
     b->release();
   if (s.exec) (void)hipGraphExecDestroy(s.exec);
   if (s.graph) (void)hipGraphDestroy(s.graph);
@@ -448,7 +448,6 @@ static BatchFp32Args fp32_args(BatchState& s) {
   a.bend = s.fp_bend.as<double>();
   a.ta = s.fp_ta.as<double>();
   a.terms = s.fp_terms.as<double>();
-  a.pb32 = s.fp_pb32.as<float>();
   a.adj = s.adj.as<unsigned long long>();
   a.queue = s.fp_queue.as<int2>();
   a.qcount = reinterpret_cast<unsigned int*>(s.counters.as<unsigned long long>() + 1);
@@ -772,7 +771,7 @@ int rpgo_pcm_batch_run(rpgo_ctx* ctx, const rpgo_pcm_batch* B, const rpgo_pcm_pa
         (rc = s.fp_lcr.reserve(pcm_fp32_lc_rec_bytes() * nl1)) ||
         (rc = s.fp_pl64.reserve(sizeof(double) * 12 * (size_t)s.n_poses)) ||
         (rc = s.fp_pb64.reserve(sizeof(double) * 12 * nb)) || (rc = s.fp_ta.reserve(sizeof(double) * 12 * nb)) ||
-        (rc = s.fp_bend.reserve(sizeof(double) * 36 * nb)) || (rc = s.fp_pb32.reserve(sizeof(float) * 12 * nb)) ||
+        (rc = s.fp_bend.reserve(sizeof(double) * 36 * nb)) ||
         (rc = s.fp_sanc.reserve(pcm_fp32_anchor_entry_bytes() * nsq)) ||
         (rc = s.fp_terms.reserve(sizeof(double) * 36 * nsq)) ||
         (rc = s.fp_queue.reserve(sizeof(int2) * (size_t)s.fp_qcap)))
@@ -818,7 +817,7 @@ int rpgo_pcm_batch_run(rpgo_ctx* ctx, const rpgo_pcm_batch* B, const rpgo_pcm_pa
     if (s.fp32) {
       key.insert(key.end(), {s.fp32_S, s.blk_off[G], s.sq_off[G], s.pblk_off[G], s.fp_qcap});
       for (DevBuf* b : {&s.d_blk_off, &s.d_sq_off, &s.d_pblk_off, &s.fp_prec, &s.fp_lcr, &s.fp_pl64, &s.fp_pb64,
-                        &s.fp_bend, &s.fp_ta, &s.fp_pb32, &s.fp_sanc, &s.fp_terms, &s.fp_queue})
+                        &s.fp_bend, &s.fp_ta, &s.fp_sanc, &s.fp_terms, &s.fp_queue})
         key.push_back((long long)(uintptr_t)b->ptr);
     }
     for (DevBuf* b : {&s.d_pose_off, &s.d_lc_off, &s.d_adj_off, &s.d_tile_off, &s.d_sym_off, &s.d_lc_graph,

@@ -183,6 +183,27 @@ __device__ __forceinline__ void s6_congruence_add(const P32& T, const S6& S, con
   s6_congruence_t<true>(T, S, &Y, o);
 }
 
+// atan2(s, c) for s >= 0: the octant reduction to atan(t), t in [0, 1], and
+// the polynomial of Abramowitz & Stegun 4.4.49 (|error| <= 2e-8 on [0, 1]).
+__device__ __forceinline__ float atan2_pos(float s, float c) {
+  const float ac = fabsf(c);
+  const bool steep = s > ac;
+  const float num = steep ? ac : s, den = steep ? s : ac;
+  const float t = den > 0.f ? num * __builtin_amdgcn_rcpf(den) : 0.f;
+  const float t2 = t * t;
+  float p = 0.0028662257f;
+  p = __builtin_fmaf(p, t2, -0.0161657367f);
+  p = __builtin_fmaf(p, t2, 0.0429096138f);
+  p = __builtin_fmaf(p, t2, -0.0752896400f);
+  p = __builtin_fmaf(p, t2, 0.1065626393f);
+  p = __builtin_fmaf(p, t2, -0.1420889944f);
+  p = __builtin_fmaf(p, t2, 0.1999355085f);
+  p = __builtin_fmaf(p, t2, -0.3333314528f);
+  const float a = __builtin_fmaf(t * t2, p, t);  // atan(t)
+  const float o = steep ? 1.5707963267948966f - a : a;
+  return c < 0.f ? 3.14159265358979f - o : o;
+}
+
 // SO(3) Log in FP32: theta from atan2(|axial|, (tr-1)/2); returns false when
 // the angle is within ~3e-3 rad of pi (tr + 1 < 1e-5), where the axial vector
 // 2 sin(theta) n no longer fixes the axis to FP32 precision.  Farther from pi
@@ -200,7 +221,7 @@ __device__ __forceinline__ bool rot_log32(const float* R, float* w, float* th_ou
   const float ax = R[7] - R[5], ay = R[2] - R[6], az = R[3] - R[1];
   const float s = 0.5f * __builtin_amdgcn_sqrtf(ax * ax + ay * ay + az * az);  // sin(theta)
   const float c = 0.5f * (tr - 1.0f);                                          // cos(theta)
-  const float th = atan2f(s, c);
+  const float th = atan2_pos(s, c);
   *th_out = th;
   *s_out = s;
   *c_out = c;

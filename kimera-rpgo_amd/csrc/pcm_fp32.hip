@@ -20,7 +20,9 @@
 //
 // Poses are kept relative to their block anchor (Pl_k = T_A^-1 T_k, FP32) and
 // anchor poses in FP64, so relative poses never subtract large coordinates in
-// FP32.  The rest of the pair check (Pcm.h:703-717: compose with the newer
+// FP32.  The anchor-to-anchor poses a segment needs (T_{A_p}^-1 T_{A_q} and G)
+// are composed once per anchor pair in FP64, rounded to FP32 and stored beside
+// Sanc, so the pair loop does no FP64 work.  The rest of the pair check (Pcm.h:703-717: compose with the newer
 // loop closure, inverse, compose with the older one, compose with the second
 // odometry segment, Mahalanobis norm of the Log) runs in FP32.
 //
@@ -39,17 +41,19 @@
 namespace rpgo {
 
 constexpr int kB = 64;          // poses per anchor block
-constexpr int kPoseRec = 56;    // floats per pose record
+constexpr int kPoseRec = 60;    // floats per pose record
 constexpr int kLcRec = 48;      // floats per loop-closure record
 constexpr int kQueueCap = kRecheckQueueCap;
 
-// pose record layout (floats): Pl[12] Lcov[21] Rcov[21] trc pad
+// pose record layout (floats): Pl[12] Lcov[21] Rcov[21] trc ti[3] pad[2]
+// (ti: translation of Pl^-1)
 struct PoseRec {
   P32 Pl;
   S6 L;
   S6 R;
   float trc;
-  float pad;
+  float ti[3];
+  float pad[2];
 };
 static_assert(sizeof(PoseRec) == kPoseRec * 4, "pose record size");
 
@@ -63,6 +67,29 @@ struct LcRec {
   int pad;
 };
 static_assert(sizeof(LcRec) == kLcRec * 4, "lc record size");
+
+// Entry (r, q) of the nb x nb anchor-pair table, r <= q:
+//   S   = Sanc(r, q): covariance of the odometry from anchor r to anchor q, in frame q;
+//   rel = T_{A_{r-1}}^-1 T_{A_q} and G = T_{A_q}^-1 T_{A_r} (r >= 1), composed in
+//         FP64 from the anchor poses and rounded once.
+// A segment from block p to a later block q reads entry (p + 1, q): S, rel
+// and G of one gather (192 B, two 128-byte lines).
+struct alignas(64) AncRec {
+  S6 S;
+  P32 rel;
+  P32 G;
+  float pad[3];
+};
+static_assert(sizeof(AncRec) == 192, "anchor entry size");
+
+// Pl^-1 of a pose record: the transpose of its rotation and the stored ti
+__device__ __forceinline__ void rec_inv(const PoseRec& r, P32& o) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) o.R[i * 3 + j] = r.Pl.R[j * 3 + i];
+  o.t[0] = r.ti[0], o.t[1] = r.ti[1], o.t[2] = r.ti[2];
+}
 
 __device__ __forceinline__ void s6_from36(const double* c, S6& s) {
 #pragma unroll
@@ -123,7 +150,12 @@ __device__ __forceinline__ void block_local_body(int p, int n_poses, const doubl
       s6_from36(L, r.L);
       s6_zero(r.R);
       r.trc = 0.f;
-      r.pad = 0.f;
+      {
+        P32 Pi;
+        p32_inv(r.Pl, Pi);
+        r.ti[0] = Pi.t[0], r.ti[1] = Pi.t[1], r.ti[2] = Pi.t[2];
+      }
+      r.pad[0] = r.pad[1] = 0.f;
       prec[k] = r;
 #pragma unroll
       for (int e = 0; e < 12; ++e) pl64[(size_t)k * 12 + e] = P[e];
@@ -200,9 +232,10 @@ __global__ void k_anchor_poses(int n_blocks, const double* __restrict__ pose0,
 }
 
 // term(r, q) = Ad(T_{A_q}^-1 T_{A_{r+1}}) Bend_r Ad(.)^T for r < q (FP64),
-// entry t = q nb + r
+// entry t = q nb + r; also rel and G of anchor-pair entry (r + 1, q)
 __device__ __forceinline__ void anchor_terms_body(long long t, int nb, const double* __restrict__ ta,
-                                                  const double* __restrict__ bend64, double* __restrict__ terms) {
+                                                  const double* __restrict__ bend64, double* __restrict__ terms,
+                                                  AncRec* __restrict__ anc) {
   using R = R64<3>;
   const int q = (int)(t / nb), r = (int)(t % nb);
   if (r >= q) return;
@@ -213,54 +246,61 @@ __device__ __forceinline__ void anchor_terms_body(long long t, int nb, const dou
   congruence64<6>(Ad, bend64 + (size_t)r * 36, term);
 #pragma unroll
   for (int e = 0; e < 36; ++e) terms[(size_t)t * 36 + e] = term[e];
+  AncRec& e = anc[(size_t)(r + 1) * nb + q];
+  p32_from64(G, e.G);
+  double ri[12], rel[12];
+  R::pose_inverse(ta + (size_t)r * 12, ri);
+  R::pose_compose(ri, ta + (size_t)q * 12, rel);
+  p32_from64(rel, e.rel);
+  e.pad[0] = e.pad[1] = e.pad[2] = 0.f;
 }
 
 __global__ void k_anchor_terms(int nb, const double* __restrict__ ta,
-                               const double* __restrict__ bend64, double* __restrict__ terms) {
+                               const double* __restrict__ bend64, double* __restrict__ terms,
+                               AncRec* __restrict__ anc) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)nb * nb) return;
-  anchor_terms_body(t, nb, ta, bend64, terms);
+  anchor_terms_body(t, nb, ta, bend64, terms, anc);
 }
 
 // Sanc(r, q) = sum_{r' = r}^{q-1} term(r', q)  (suffix sum over r for each q)
 __device__ __forceinline__ void anchor_suffix_body(int q, int nb, const double* __restrict__ terms,
-                                                   S6* __restrict__ sanc) {
+                                                   AncRec* __restrict__ anc) {
   double acc[36];
 #pragma unroll
   for (int e = 0; e < 36; ++e) acc[e] = 0.0;
   S6 z;
   s6_zero(z);
-  sanc[(size_t)q * nb + q] = z;
+  anc[(size_t)q * nb + q].S = z;
   for (int r = q - 1; r >= 0; --r) {
     add36(acc, terms + ((size_t)q * nb + r) * 36);
     S6 s;
     s6_from36(acc, s);
-    sanc[(size_t)r * nb + q] = s;
+    anc[(size_t)r * nb + q].S = s;
   }
 }
 
-__global__ void k_anchor_suffix(int nb, const double* __restrict__ terms, S6* __restrict__ sanc) {
+__global__ void k_anchor_suffix(int nb, const double* __restrict__ terms, AncRec* __restrict__ anc) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nb) return;
-  anchor_suffix_body(q, nb, terms, sanc);
+  anchor_suffix_body(q, nb, terms, anc);
 }
 
 // Chain covariance trace estimate of pose k (margin of the FP64 band).
-__device__ __forceinline__ void pose_trace_body(int k, const S6* __restrict__ sanc, PoseRec* __restrict__ prec) {
+__device__ __forceinline__ void pose_trace_body(int k, const AncRec* __restrict__ anc, PoseRec* __restrict__ prec) {
   const int p = k / kB;
   PoseRec& r = prec[k];
   P32 Pi;
-  p32_inv(r.Pl, Pi);
+  rec_inv(r, Pi);
   S6 T;
-  s6_congruence(Pi, sanc[p], T);  // Sanc(0, p)
+  s6_congruence(Pi, anc[p].S, T);  // Sanc(0, p)
   r.trc = s6_trace(T) + s6_trace(r.L);
 }
 
-__global__ void k_pose_trace(int n_poses, int nb, const S6* __restrict__ sanc,
-                             PoseRec* __restrict__ prec) {
+__global__ void k_pose_trace(int n_poses, const AncRec* __restrict__ anc, PoseRec* __restrict__ prec) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n_poses) return;
-  pose_trace_body(k, sanc, prec);
+  pose_trace_body(k, anc, prec);
 }
 
 __device__ __forceinline__ void lc_record_body(int t, const int* __restrict__ cf, const int* __restrict__ ct,
@@ -290,19 +330,9 @@ __global__ void k_lc_records(int m, const int* __restrict__ cf, const int* __res
 // ---------------------------------------------------------------------------
 
 struct Anchors {
-  const double* ta;      // FP64 anchor poses
-  const float* pblock;   // FP32 Pblock_p (12 floats)
-  const S6* sanc;        // nb x nb
+  const AncRec* anc;  // nb x nb
   int nb;
 };
-
-__device__ __forceinline__ void rel_anchor(const Anchors& an, int p, int q, P32& o) {
-  // T_{A_p}^-1 T_{A_q} in FP64, rounded to FP32
-  double pi[12], r[12];
-  R64<3>::pose_inverse(an.ta + (size_t)p * 12, pi);
-  R64<3>::pose_compose(pi, an.ta + (size_t)q * 12, r);
-  p32_from64(r, o);
-}
 
 // between(x, y) of the reference, x older-side key, y newer-side key:
 // pose T_x^-1 T_y and covariance Seg(min, max) in frame max.  Returns the
@@ -314,38 +344,37 @@ __device__ __forceinline__ float between32(const Anchors& an, int x, const PoseR
   const PoseRec& rl = xlo ? rx : ry;
   const PoseRec& rh = xlo ? ry : rx;
   const int plo = lo / kB, phi = hi / kB;
+  const bool same = plo == phi;
+  // T_{A_plo}^-1 T_{A_phi}: the identity within one block
   P32 rel;
-  rel_anchor(an, plo, phi, rel);  // T_{A_plo}^-1 T_{A_phi}
+  const AncRec* e = an.anc + (size_t)(same ? phi : plo + 1) * an.nb + phi;
+  if (same) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) rel.R[k] = (k % 4 == 0) ? 1.f : 0.f;
+    rel.t[0] = rel.t[1] = rel.t[2] = 0.f;
+  } else {
+    rel = e->rel;
+  }
   P32 reli;
   p32_inv(rel, reli);
   // pose
   P32 xi_, tmp;
-  p32_inv(rx.Pl, xi_);
+  rec_inv(rx, xi_);
   p32_mul(xi_, xlo ? rel : reli, tmp);
   p32_mul(tmp, ry.Pl, pose);
   // covariance
-  const bool same = plo == phi;
-  P32 G;
   S6 M;
   if (same) {
     s6_congruence(rl.Pl, rl.L, M);
 #pragma unroll
-    for (int e = 0; e < 6; ++e) M.A[e] = -M.A[e], M.C[e] = -M.C[e];
+    for (int k = 0; k < 6; ++k) M.A[k] = -M.A[k], M.C[k] = -M.C[k];
 #pragma unroll
-    for (int e = 0; e < 9; ++e) M.B[e] = -M.B[e];
+    for (int k = 0; k < 9; ++k) M.B[k] = -M.B[k];
   } else {
-    P32 pb;
-#pragma unroll
-    for (int e = 0; e < 9; ++e) pb.R[e] = an.pblock[(size_t)plo * 12 + e];
-    pb.t[0] = an.pblock[(size_t)plo * 12 + 9];
-    pb.t[1] = an.pblock[(size_t)plo * 12 + 10];
-    pb.t[2] = an.pblock[(size_t)plo * 12 + 11];
-    p32_mul(reli, pb, G);
-    const S6& Q = an.sanc[(size_t)(plo + 1) * an.nb + phi];
-    s6_congruence_add(G, rl.R, Q, M);
+    s6_congruence_add(e->G, rl.R, e->S, M);
   }
   P32 hinv;
-  p32_inv(rh.Pl, hinv);
+  rec_inv(rh, hinv);
   s6_congruence_add(hinv, M, rh.L, cov);
   return sqrtf(pose.t[0] * pose.t[0] + pose.t[1] * pose.t[1] + pose.t[2] * pose.t[2]);
 }
@@ -365,13 +394,13 @@ __device__ __forceinline__ void compose_meas(const P32& T, const S6& C, const Lc
 // wavefront takes one run of older loop closures, uniform across the
 // wavefront, against its 64 lanes; a block of four wavefronts takes four
 // consecutive runs of one tile.  The tile's j-side records (the loop closure
-// and its two endpoint poses, 640 B per lane) are staged once per block in
+// and its two endpoint poses, 676 B per lane) are staged once per block in
 // LDS and read at their points of use: held in registers for the whole walk
 // they took 157 VGPRs and pushed the pair chain into scratch spills (1.26x
 // slower at 5e7 pairs on MI355X).
 constexpr int kTileWaves = 4;
 
-// 161 floats per lane: an odd stride, so 32-bit LDS reads of the 64 lanes
+// 169 floats per lane: an odd stride, so 32-bit LDS reads of the 64 lanes
 // fall in distinct banks (a stride of 160 put all 64 lanes in one bank).
 struct JSide {
   LcRec lj;
@@ -379,7 +408,7 @@ struct JSide {
   PoseRec rd;  // pose lj.b
   float pad;
 };
-static_assert(sizeof(JSide) == 161 * 4, "j-side record stride");
+static_assert(sizeof(JSide) == 169 * 4, "j-side record stride");
 
 // blocks of the tiles before jt: tile t has (t + 1) S runs, (t + 1) S / 4 blocks
 __host__ __device__ __forceinline__ long long tile_blocks_before(long long jt, int S) {
@@ -483,9 +512,23 @@ __global__ void k_popcount(long long nwords, const unsigned long long* __restric
 // host side
 // ---------------------------------------------------------------------------
 
-__global__ void k_to_f32(long long n, const double* __restrict__ a, float* __restrict__ b) {
-  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) b[k] = (float)a[k];
+// layout of fp32_buf: prec | lcrec | pl64 | pblock64 | bend64 | ta | anc | terms
+struct Fp32Layout {
+  size_t prec, lc, pl64, pb64, bend, ta, anc, terms, total;
+};
+
+static Fp32Layout fp32_layout(int N, int m, int nb) {
+  Fp32Layout L;
+  L.prec = 0;
+  L.lc = L.prec + sizeof(PoseRec) * (size_t)N;
+  L.pl64 = L.lc + sizeof(LcRec) * (size_t)(m > 0 ? m : 1);
+  L.pb64 = L.pl64 + sizeof(double) * 12 * (size_t)N;
+  L.bend = L.pb64 + sizeof(double) * 12 * (size_t)nb;
+  L.ta = L.bend + sizeof(double) * 36 * (size_t)nb;
+  L.anc = (L.ta + sizeof(double) * 12 * (size_t)nb + 255) & ~(size_t)255;  // AncRec is 64-byte aligned
+  L.terms = L.anc + sizeof(AncRec) * (size_t)nb * nb;
+  L.total = L.terms + sizeof(double) * 36 * (size_t)nb * nb;
+  return L;
 }
 
 int pcm_fp32_prepare(rpgo_ctx* ctx, bool* use_fp64) {
@@ -514,29 +557,18 @@ int pcm_fp32_prepare(rpgo_ctx* ctx, bool* use_fp64) {
     *use_fp64 = true;
     return RPGO_OK;
   }
-  // layout of fp32_buf: prec | lcrec | pl64 | pblock64 | bend64 | ta | pblock32 | sanc | terms
-  const size_t off_prec = 0;
-  const size_t off_lc = off_prec + sizeof(PoseRec) * (size_t)N;
-  const size_t off_pl64 = off_lc + sizeof(LcRec) * (size_t)(m > 0 ? m : 1);
-  const size_t off_pb64 = off_pl64 + sizeof(double) * 12 * (size_t)N;
-  const size_t off_bend = off_pb64 + sizeof(double) * 12 * (size_t)nb;
-  const size_t off_ta = off_bend + sizeof(double) * 36 * (size_t)nb;
-  const size_t off_pb32 = off_ta + sizeof(double) * 12 * (size_t)nb;
-  const size_t off_sanc = off_pb32 + sizeof(float) * 12 * (size_t)nb;
-  const size_t off_terms = off_sanc + sizeof(S6) * (size_t)nb * nb;
-  const size_t total = off_terms + sizeof(double) * 36 * (size_t)nb * nb;
+  const Fp32Layout L = fp32_layout(N, m, nb);
   int rc;
-  if ((rc = s.fp32_buf.reserve(total))) return rc;
+  if ((rc = s.fp32_buf.reserve(L.total))) return rc;
   char* base = s.fp32_buf.as<char>();
-  PoseRec* prec = reinterpret_cast<PoseRec*>(base + off_prec);
-  LcRec* lcr = reinterpret_cast<LcRec*>(base + off_lc);
-  double* pl64 = reinterpret_cast<double*>(base + off_pl64);
-  double* pb64 = reinterpret_cast<double*>(base + off_pb64);
-  double* bend = reinterpret_cast<double*>(base + off_bend);
-  double* ta = reinterpret_cast<double*>(base + off_ta);
-  float* pb32 = reinterpret_cast<float*>(base + off_pb32);
-  S6* sanc = reinterpret_cast<S6*>(base + off_sanc);
-  double* terms = reinterpret_cast<double*>(base + off_terms);
+  PoseRec* prec = reinterpret_cast<PoseRec*>(base + L.prec);
+  LcRec* lcr = reinterpret_cast<LcRec*>(base + L.lc);
+  double* pl64 = reinterpret_cast<double*>(base + L.pl64);
+  double* pb64 = reinterpret_cast<double*>(base + L.pb64);
+  double* bend = reinterpret_cast<double*>(base + L.bend);
+  double* ta = reinterpret_cast<double*>(base + L.ta);
+  AncRec* anc = reinterpret_cast<AncRec*>(base + L.anc);
+  double* terms = reinterpret_cast<double*>(base + L.terms);
   RPGO_HIP_CHECK(hipMemsetAsync(pb64, 0, sizeof(double) * 12 * (size_t)nb, st));
   RPGO_HIP_CHECK(hipMemsetAsync(bend, 0, sizeof(double) * 36 * (size_t)nb, st));
   k_block_local<<<(nb + 63) / 64, 64, 0, st>>>(N, nb, s.delta.as<double>(), s.hinv.as<double>(),
@@ -545,16 +577,13 @@ int pcm_fp32_prepare(rpgo_ctx* ctx, bool* use_fp64) {
   k_anchor_poses<<<1, 1, 0, st>>>(nb, s.pose0.as<double>(), pb64, ta);
   RPGO_HIP_CHECK(hipGetLastError());
   const long long nt = (long long)nb * nb;
-  k_anchor_terms<<<(unsigned)((nt + 255) / 256), 256, 0, st>>>(nb, ta, bend, terms);
-  k_anchor_suffix<<<(nb + 63) / 64, 64, 0, st>>>(nb, terms, sanc);
+  k_anchor_terms<<<(unsigned)((nt + 255) / 256), 256, 0, st>>>(nb, ta, bend, terms, anc);
+  k_anchor_suffix<<<(nb + 63) / 64, 64, 0, st>>>(nb, terms, anc);
   RPGO_HIP_CHECK(hipGetLastError());
-  k_pose_trace<<<(N + 255) / 256, 256, 0, st>>>(N, nb, sanc, prec);
+  k_pose_trace<<<(N + 255) / 256, 256, 0, st>>>(N, anc, prec);
   if (m > 0)
     k_lc_records<<<(m + 255) / 256, 256, 0, st>>>(m, s.c_from.as<int>(), s.c_to.as<int>(),
                                                   s.c_lcm.as<double>(), lcr);
-  RPGO_HIP_CHECK(hipGetLastError());
-  // Pblock FP32
-  k_to_f32<<<(nb * 12 + 255) / 256, 256, 0, st>>>((long long)nb * 12, pb64, pb32);
   RPGO_HIP_CHECK(hipGetLastError());
   s.fp32_nb = nb;
   return RPGO_OK;
@@ -564,21 +593,13 @@ int pcm_pairs_fp32_launch(rpgo_ctx* ctx, double lc_thr, double margin, double ch
                           bool want_dist) {
   PcmState& s = ctx->pcm;
   const int N = s.n_poses, nb = s.fp32_nb, m = s.m;
-  const size_t off_lc = sizeof(PoseRec) * (size_t)N;
-  const size_t off_pl64 = off_lc + sizeof(LcRec) * (size_t)(m > 0 ? m : 1);
-  const size_t off_pb64 = off_pl64 + sizeof(double) * 12 * (size_t)N;
-  const size_t off_bend = off_pb64 + sizeof(double) * 12 * (size_t)nb;
-  const size_t off_ta = off_bend + sizeof(double) * 36 * (size_t)nb;
-  const size_t off_pb32 = off_ta + sizeof(double) * 12 * (size_t)nb;
-  const size_t off_sanc = off_pb32 + sizeof(float) * 12 * (size_t)nb;
+  const Fp32Layout L = fp32_layout(N, m, nb);
   char* base = s.fp32_buf.as<char>();
   Anchors an;
-  an.ta = reinterpret_cast<const double*>(base + off_ta);
-  an.pblock = reinterpret_cast<const float*>(base + off_pb32);
-  an.sanc = reinterpret_cast<const S6*>(base + off_sanc);
+  an.anc = reinterpret_cast<const AncRec*>(base + L.anc);
   an.nb = nb;
-  const PoseRec* prec = reinterpret_cast<const PoseRec*>(base);
-  const LcRec* lcr = reinterpret_cast<const LcRec*>(base + off_lc);
+  const PoseRec* prec = reinterpret_cast<const PoseRec*>(base + L.prec);
+  const LcRec* lcr = reinterpret_cast<const LcRec*>(base + L.lc);
   int rc;
   if ((rc = s.queue.reserve(sizeof(int2) * (size_t)kQueueCap))) return rc;
   const int W = (m + 63) / 64;
@@ -617,7 +638,7 @@ int pcm_pairs_fp32_launch(rpgo_ctx* ctx, double lc_thr, double margin, double ch
 
 size_t pcm_fp32_pose_rec_bytes() { return sizeof(PoseRec); }
 size_t pcm_fp32_lc_rec_bytes() { return sizeof(LcRec); }
-size_t pcm_fp32_anchor_entry_bytes() { return sizeof(S6); }
+size_t pcm_fp32_anchor_entry_bytes() { return sizeof(AncRec); }
 
 // graph of item t: the last g with off[g] <= t
 __device__ __forceinline__ int batch_graph(const long long* __restrict__ off, int n, long long t) {
@@ -652,7 +673,8 @@ __global__ void k_anchor_terms_b(BatchFp32Args a) {
   if (t >= a.n_sq) return;
   const int g = batch_graph(a.sq_off, a.G, t);
   anchor_terms_body(t - a.sq_off[g], (int)(a.blk_off[g + 1] - a.blk_off[g]), a.ta + a.blk_off[g] * 12,
-                    a.bend + a.blk_off[g] * 36, a.terms + a.sq_off[g] * 36);
+                    a.bend + a.blk_off[g] * 36, a.terms + a.sq_off[g] * 36,
+                    static_cast<AncRec*>(a.sanc) + a.sq_off[g]);
 }
 
 __global__ void k_anchor_suffix_b(BatchFp32Args a) {
@@ -660,14 +682,14 @@ __global__ void k_anchor_suffix_b(BatchFp32Args a) {
   if (t >= a.n_blocks) return;
   const int g = batch_graph(a.blk_off, a.G, t);
   anchor_suffix_body((int)(t - a.blk_off[g]), (int)(a.blk_off[g + 1] - a.blk_off[g]), a.terms + a.sq_off[g] * 36,
-                     static_cast<S6*>(a.sanc) + a.sq_off[g]);
+                     static_cast<AncRec*>(a.sanc) + a.sq_off[g]);
 }
 
 __global__ void k_pose_trace_b(BatchFp32Args a) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.n_poses) return;
   const int g = batch_graph(a.pose_off, a.G, t);
-  pose_trace_body((int)(t - a.pose_off[g]), static_cast<const S6*>(a.sanc) + a.sq_off[g],
+  pose_trace_body((int)(t - a.pose_off[g]), static_cast<const AncRec*>(a.sanc) + a.sq_off[g],
                   static_cast<PoseRec*>(a.prec) + a.pose_off[g]);
 }
 
@@ -718,9 +740,7 @@ k_pairs32_b(BatchFp32Args a, const LcRec* __restrict__ lcs_all, const PoseRec* _
   const LcRec* __restrict__ lcs = lcs_all + o;
   const PoseRec* __restrict__ prec = prec_all + pose_off[g];
   Anchors an;
-  an.ta = a.ta + blk_off[g] * 12;
-  an.pblock = a.pb32 + blk_off[g] * 12;
-  an.sanc = static_cast<const S6*>(a.sanc) + sq_off[g];
+  an.anc = static_cast<const AncRec*>(a.sanc) + sq_off[g];
   an.nb = (int)(blk_off[g + 1] - blk_off[g]);
   const int W = (m + 63) / 64;
   unsigned long long* __restrict__ adj = a.adj + adj_off[g];
@@ -794,7 +814,6 @@ int pcm_fp32_batch_launch(hipStream_t st, const BatchFp32Args& a, double lc_thr,
     k_anchor_terms_b<<<(unsigned)((a.n_sq + 255) / 256), 256, 0, st>>>(a);
     k_anchor_suffix_b<<<(unsigned)((a.n_blocks + 63) / 64), 64, 0, st>>>(a);
     k_pose_trace_b<<<(unsigned)((a.n_poses + 255) / 256), 256, 0, st>>>(a);
-    k_to_f32<<<(unsigned)((a.n_blocks * 12 + 255) / 256), 256, 0, st>>>(a.n_blocks * 12, a.pb64, a.pb32);
   }
   if (a.n_lc > 0) k_lc_records_b<<<(unsigned)((a.n_lc + 255) / 256), 256, 0, st>>>(a);
   RPGO_HIP_CHECK(hipGetLastError());

@@ -201,9 +201,8 @@ struct BatchFp32Args {
   const double *delta = nullptr, *hinv = nullptr, *lcm = nullptr, *pose0 = nullptr;
   const int *c_from = nullptr, *c_to = nullptr;
   const double* c_lcm = nullptr;
-  void *prec = nullptr, *lcr = nullptr, *sanc = nullptr;  // PoseRec, LcRec, S6 (pcm_fp32.hip)
+  void *prec = nullptr, *lcr = nullptr, *sanc = nullptr;  // PoseRec, LcRec, AncRec (pcm_fp32.hip)
   double *pl64 = nullptr, *pb64 = nullptr, *bend = nullptr, *ta = nullptr, *terms = nullptr;
-  float* pb32 = nullptr;
   unsigned long long* adj = nullptr;   // lower triangle written (row j, bits i < j)
   int2* queue = nullptr;               // (lc_off[g] + i, lc_off[g] + j) of queued pairs
   unsigned int* qcount = nullptr;
