@@ -118,12 +118,13 @@ __device__ __forceinline__ int icc_wave(int W, int c, const unsigned long long* 
       ++icc;
       const int m = wave_top_bit(S, lane, W);
       if (m < 0) return icc;  // S held only the seed c: removing it empties S
-      // S /\ N(m); m itself always leaves S, so the walk ends within n steps
+      // S /\ N(m); m itself always leaves S, so the walk ends within n steps.
+      // Words of S already empty stay empty: their row words are not read.
       const unsigned long long* row = adj + (size_t)m * W;
 #pragma unroll
       for (int q = 0; q < kMaxWPL; ++q) {
         const int w = lane + 64 * q;
-        if (w < W) S[q] &= row[w] & ((w == (m >> 6)) ? ~(1ull << (m & 63)) : ~0ull);
+        if (w < W && S[q]) S[q] &= row[w] & ((w == (m >> 6)) ? ~(1ull << (m & 63)) : ~0ull);
       }
       if ((icc & 7) == 0 && icc + wave_popcount(S) + 1 <= best) return INT_MIN;
     }
@@ -153,7 +154,7 @@ __device__ __forceinline__ int icc_wave(int W, int c, const unsigned long long* 
 #pragma unroll
       for (int q = 0; q < kMaxWPL; ++q) {
         const int w = lane + 64 * q;
-        Rw[p][q] = (cand[p] >= 0 && w < W) ? adj[(size_t)cand[p] * W + w] : 0ull;
+        Rw[p][q] = (cand[p] >= 0 && w < W && S[q]) ? adj[(size_t)cand[p] * W + w] : 0ull;  // empty words stay empty
       }
 #pragma unroll
     for (int p = 0; p < PF; ++p) {
@@ -201,9 +202,13 @@ __global__ void k_first_better(int n, int pos, int best, const int* __restrict__
 
 // Replay of the winning candidate c: writes the reference's buffer positions
 // 0..s0-1 (position p keeps the value last written to it) and returns the
-// greedy size.  One wavefront.
-__device__ int replay_wave(int W, int c, const unsigned long long* __restrict__ adj,
-                           const unsigned long long* dmask, int* __restrict__ out, int lane) {
+// greedy size.  One wavefront.  The rows of the PF highest vertices of S
+// are fetched together, as in icc_wave; a step whose pick differs from the
+// prediction fetches again, so the steps and writes are those of a walk one
+// row at a time.
+template <int PF>
+__device__ __forceinline__ int replay_wave(int W, int c, const unsigned long long* __restrict__ adj,
+                                           const unsigned long long* dmask, int* __restrict__ out, int lane) {
   unsigned long long S[kMaxWPL];
 #pragma unroll
   for (int q = 0; q < kMaxWPL; ++q) {
@@ -216,71 +221,94 @@ __device__ int replay_wave(int W, int c, const unsigned long long* __restrict__ 
   int s_t = 1 + wave_popcount(S);  // list size before the iteration
   int icc = 0;
   for (;;) {
-    ++icc;
-    const int m = wave_top_bit(S, lane, W);
-    unsigned long long N[kMaxWPL];
-    if (m >= 0) {
-      const unsigned long long* row = adj + (size_t)m * W;
+    // predicted picks: the PF highest vertices of S, highest first
+    int cand[PF];
+    {
+      unsigned long long T[kMaxWPL];
+#pragma unroll
+      for (int q = 0; q < kMaxWPL; ++q) T[q] = S[q];
+#pragma unroll
+      for (int f = 0; f < PF; ++f) {
+        const int t = wave_top_bit(T, lane, W);
+        cand[f] = t;
+        if (t >= 0) {
+          const int w = t >> 6;
+#pragma unroll
+          for (int q = 0; q < kMaxWPL; ++q)
+            if (lane + 64 * q == w) T[q] &= ~(1ull << (t & 63));
+        }
+      }
+    }
+    unsigned long long Rw[PF][kMaxWPL];
+#pragma unroll
+    for (int f = 0; f < PF; ++f)
 #pragma unroll
       for (int q = 0; q < kMaxWPL; ++q) {
         const int w = lane + 64 * q;
-        N[q] = (w < W) ? (S[q] & row[w] & ((w == (m >> 6)) ? ~(1ull << (m & 63)) : ~0ull)) : 0ull;
+        Rw[f][q] = (cand[f] >= 0 && w < W && S[q]) ? adj[(size_t)cand[f] * W + w] : 0ull;  // empty words stay empty
       }
-    } else {
 #pragma unroll
-      for (int q = 0; q < kMaxWPL; ++q) N[q] = 0ull;
-    }
-    // list after the iteration: the seed survives unless it was the pick
-    const int s_next = (m >= 0) ? 1 + wave_popcount(N) : 0;
-    // positions [max(s_next,1), s_t) keep S_t's entries: ranks p-1 of the set part
-    const int lo = s_next > 1 ? s_next : 1;
-    if (lo < s_t) {
-      // rank of each of this lane's words: exclusive scan in word order
-      // (word w = lane + 64 q; order is q-major across lanes)
-      int base = 0;
+    for (int f = 0; f < PF; ++f) {
+      const int m = wave_top_bit(S, lane, W);
+      if (m >= 0 && m != cand[f]) break;  // prediction missed (wave-uniform): fetch again
+      ++icc;
+      unsigned long long N[kMaxWPL];
 #pragma unroll
       for (int q = 0; q < kMaxWPL; ++q) {
-        if (64 * q >= W) break;  // words beyond W are empty (wave-uniform)
-        const int cnt = __popcll(S[q]);
-        int incl = cnt;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-          const int o = __shfl_up(incl, off);
-          if (lane >= off) incl += o;
-        }
-        const int total = __shfl(incl, 63);
-        // this word's set bits hold ranks [r0, r0 + cnt), i.e. positions
-        // [r0 + 1, r0 + cnt + 1); only those inside [lo, s_t) are written
-        const int r0 = base + incl - cnt;
-        const int p_lo = lo > r0 + 1 ? lo : r0 + 1;
-        const int p_hi = s_t < r0 + cnt + 1 ? s_t : r0 + cnt + 1;
-        if (p_lo < p_hi) {
-          unsigned long long b = S[q];
-          for (int skip = p_lo - (r0 + 1); skip > 0; --skip) b &= b - 1ull;
-          const int w = lane + 64 * q;
-          for (int p = p_lo; p < p_hi; ++p) {
-            out[p] = w * 64 + (__ffsll((long long)b) - 1);
-            b &= b - 1ull;
-          }
-        }
-        base += total;
-        if (base + 1 >= s_t) break;  // every later rank lies beyond the range
+        const int w = lane + 64 * q;
+        N[q] = (m >= 0 && w < W) ? (S[q] & Rw[f][q] & ((w == (m >> 6)) ? ~(1ull << (m & 63)) : ~0ull)) : 0ull;
       }
-    }
+      // list after the iteration: the seed survives unless it was the pick
+      const int s_next = (m >= 0) ? 1 + wave_popcount(N) : 0;
+      // positions [max(s_next,1), s_t) keep S_t's entries: ranks p-1 of the set part
+      const int lo = s_next > 1 ? s_next : 1;
+      if (lo < s_t) {
+        // rank of each of this lane's words: exclusive scan in word order
+        // (word w = lane + 64 q; order is q-major across lanes)
+        int base = 0;
 #pragma unroll
-    for (int q = 0; q < kMaxWPL; ++q) S[q] = N[q];
-    s_t = s_next;
-    if (m < 0) break;
+        for (int q = 0; q < kMaxWPL; ++q) {
+          if (64 * q >= W) break;  // words beyond W are empty (wave-uniform)
+          const int cnt = __popcll(S[q]);
+          int incl = cnt;
+#pragma unroll
+          for (int off = 1; off < 64; off <<= 1) {
+            const int o = __shfl_up(incl, off);
+            if (lane >= off) incl += o;
+          }
+          const int total = __shfl(incl, 63);
+          // this word's set bits hold ranks [r0, r0 + cnt), i.e. positions
+          // [r0 + 1, r0 + cnt + 1); only those inside [lo, s_t) are written
+          const int r0 = base + incl - cnt;
+          const int p_lo = lo > r0 + 1 ? lo : r0 + 1;
+          const int p_hi = s_t < r0 + cnt + 1 ? s_t : r0 + cnt + 1;
+          if (p_lo < p_hi) {
+            unsigned long long bits = S[q];
+            for (int skip = p_lo - (r0 + 1); skip > 0; --skip) bits &= bits - 1ull;
+            const int w = lane + 64 * q;
+            for (int p = p_lo; p < p_hi; ++p) {
+              out[p] = w * 64 + (__ffsll((long long)bits) - 1);
+              bits &= bits - 1ull;
+            }
+          }
+          base += total;
+          if (base + 1 >= s_t) break;  // every later rank lies beyond the range
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kMaxWPL; ++q) S[q] = N[q];
+      s_t = s_next;
+      if (m < 0) return icc;
+    }
   }
-  return icc;
 }
 
 
-__global__ void __launch_bounds__(64) k_replay(int n, int W, int c, int best,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) k_replay(int n, int W, int c, int best,
                                                const unsigned long long* __restrict__ adj,
                                                const unsigned long long* __restrict__ dmask,
                                                int* __restrict__ out, int* __restrict__ icc_out) {
-  const int icc = replay_wave(W, c, adj, dmask, out, threadIdx.x);
+  const int icc = replay_wave<kPrefetch>(W, c, adj, dmask, out, threadIdx.x);
   if (threadIdx.x == 0) *icc_out = icc;
   (void)best;
   (void)n;
@@ -383,7 +411,9 @@ __global__ void __launch_bounds__(256) k_cb_icc(int n_graphs, const long long* _
   if (pos >= n || c >= n) return;  // the select step reads only slots below n - pos
   const int lane = threadIdx.x & 63;
   int icc = INT_MIN;
-  if (!(best > deg_all[lc_off[g] + c]))  // pruning 1
+  // best < 1: the window's first candidate wins (every greedy size is >= 1),
+  // so the others need no walk
+  if (!(best > deg_all[lc_off[g] + c]) && (best >= 1 || k == 0))  // pruning 1
     icc = icc_wave<1>((n + 63) / 64, c, adj_all + adj_off[g], dm + lc_off[g], lane, best);
   if (lane == 0) icc_out[t] = icc;
 }
@@ -438,7 +468,8 @@ __global__ void __launch_bounds__(64 * NW) k_clique_batch(int n_graphs, const lo
   const unsigned long long* A = adj_all + adj_off[g];
   int* deg = deg_all + lc_off[g];
   int* out = out_all + lc_off[g];
-  for (int v = tid; v < n; v += 64 * NW) {
+  // after the chip-wide rounds the degrees are already there (k_cb_degrees)
+  for (int v = start ? n : tid; v < n; v += 64 * NW) {
     int d = 0;
     for (int w = 0; w < W; ++w) {
       unsigned long long b = A[(size_t)v * W + w];
@@ -471,7 +502,8 @@ __global__ void __launch_bounds__(64 * NW) k_clique_batch(int n_graphs, const lo
     for (int base = pos; base < n; base += NW) {
       const int c = base + wid;
       int icc = INT_MIN;
-      if (c < n && !(best > deg[c])) icc = icc_wave<PF>(W, c, A, dm, lane, best);  // pruning 1
+      // pruning 1; with best < 1 the first candidate wins (greedy sizes are >= 1)
+      if (c < n && !(best > deg[c]) && (best >= 1 || wid == 0)) icc = icc_wave<PF>(W, c, A, dm, lane, best);
       if (lane == 0) s_icc[wid] = icc;
       __syncthreads();
       if (tid == 0) {
@@ -510,7 +542,7 @@ __global__ void __launch_bounds__(64 * NW) k_clique_batch(int n_graphs, const lo
     }
     __syncthreads();
     if (wid == 0) {
-      const int icc = replay_wave(W, win, A, dm, out, lane);
+      const int icc = replay_wave<PF>(W, win, A, dm, out, lane);
       if (lane == 0) k_out[g] = icc == best ? best : INT_MIN;
     }
   } else if (tid == 0) {
@@ -711,7 +743,8 @@ static int clique_scan(rpgo_ctx* ctx, const uint64_t* d_adj_u64, int n, int mode
   int* d_first = b.best;
   while (pos < n) {
     k_dmask<<<(W + 255) / 256, 256, 0, st>>>(n, W, best, b.deg, b.dmask, d_first);
-    const int rem = n - pos;
+    // best < 1: candidate pos wins (every greedy size is >= 1), so it alone is walked
+    const int rem = best < 1 ? 1 : n - pos;
     k_icc<<<(rem + 3) / 4, 256, 0, st>>>(n, W, pos, best, adj, b.deg, b.dmask, b.icc);
     RPGO_HIP_CHECK(hipGetLastError());
     k_first_better<<<(rem + 255) / 256, 256, 0, st>>>(n, pos, best, b.icc, d_first);
