@@ -58,29 +58,29 @@ constexpr int kMaxWPL = 8;
 constexpr long long kExactNodeBudget = 400000000ll;
 
 // Highest set bit over the wave's distributed bitset; -1 when empty.
+// Word w = lane + 64 q, so the highest vertex lies in the highest q with a
+// non-empty word, on the highest such lane: one ballot per q (scalar), then
+// that lane's top bit by v_readlane, instead of a six-step shuffle reduction.
+// Called by whole wavefronts (wave-uniform control flow).
 __device__ __forceinline__ int wave_top_bit(const unsigned long long* S, int lane, int W) {
-  int mine = -1;
 #pragma unroll
   for (int q = kMaxWPL - 1; q >= 0; --q) {
+    if (64 * q >= W) continue;  // no words here (wave-uniform)
     const int w = lane + 64 * q;
-    if (mine < 0 && w < W && S[q]) mine = w * 64 + 63 - __clzll(S[q]);
+    const unsigned long long lanes = __ballot(w < W && S[q] != 0ull);
+    if (lanes) {
+      const int L = 63 - __clzll(lanes);
+      return __builtin_amdgcn_readlane(w * 64 + 63 - __clzll(S[q]), L);
+    }
   }
-  // wave max
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const int o = __shfl_xor(mine, off);
-    mine = o > mine ? o : mine;
-  }
-  return mine;
+  return -1;
 }
 
 __device__ __forceinline__ int wave_popcount(const unsigned long long* S) {
   int cnt = 0;
 #pragma unroll
   for (int q = 0; q < kMaxWPL; ++q) cnt += __popcll(S[q]);
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
-  return cnt;
+  return __ockl_wfred_add_i32(cnt);  // DPP wavefront sum
 }
 
 // rows fetched ahead of the greedy walk (see icc_wave)
@@ -260,39 +260,22 @@ __device__ __forceinline__ int replay_wave(int W, int c, const unsigned long lon
       }
       // list after the iteration: the seed survives unless it was the pick
       const int s_next = (m >= 0) ? 1 + wave_popcount(N) : 0;
-      // positions [max(s_next,1), s_t) keep S_t's entries: ranks p-1 of the set part
+      // positions [max(s_next,1), s_t) keep S_t's entries: position p holds the
+      // element of rank p-1 (ascending vertex order), i.e. the top s_t - lo
+      // vertices of S_t, highest at s_t - 1; the pick and its non-neighbours
+      // leave S, so there are as many as vertices removed at this step
       const int lo = s_next > 1 ? s_next : 1;
       if (lo < s_t) {
-        // rank of each of this lane's words: exclusive scan in word order
-        // (word w = lane + 64 q; order is q-major across lanes)
-        int base = 0;
+        unsigned long long T[kMaxWPL];
 #pragma unroll
-        for (int q = 0; q < kMaxWPL; ++q) {
-          if (64 * q >= W) break;  // words beyond W are empty (wave-uniform)
-          const int cnt = __popcll(S[q]);
-          int incl = cnt;
+        for (int q = 0; q < kMaxWPL; ++q) T[q] = S[q];
+        for (int p = s_t - 1; p >= lo; --p) {
+          const int t = wave_top_bit(T, lane, W);
+          if (lane == 0) out[p] = t;
+          const int w = t >> 6;
 #pragma unroll
-          for (int off = 1; off < 64; off <<= 1) {
-            const int o = __shfl_up(incl, off);
-            if (lane >= off) incl += o;
-          }
-          const int total = __shfl(incl, 63);
-          // this word's set bits hold ranks [r0, r0 + cnt), i.e. positions
-          // [r0 + 1, r0 + cnt + 1); only those inside [lo, s_t) are written
-          const int r0 = base + incl - cnt;
-          const int p_lo = lo > r0 + 1 ? lo : r0 + 1;
-          const int p_hi = s_t < r0 + cnt + 1 ? s_t : r0 + cnt + 1;
-          if (p_lo < p_hi) {
-            unsigned long long bits = S[q];
-            for (int skip = p_lo - (r0 + 1); skip > 0; --skip) bits &= bits - 1ull;
-            const int w = lane + 64 * q;
-            for (int p = p_lo; p < p_hi; ++p) {
-              out[p] = w * 64 + (__ffsll((long long)bits) - 1);
-              bits &= bits - 1ull;
-            }
-          }
-          base += total;
-          if (base + 1 >= s_t) break;  // every later rank lies beyond the range
+          for (int q = 0; q < kMaxWPL; ++q)
+            if (lane + 64 * q == w) T[q] &= ~(1ull << (t & 63));
         }
       }
 #pragma unroll

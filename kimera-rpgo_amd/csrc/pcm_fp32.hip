@@ -256,7 +256,7 @@ __device__ __forceinline__ void anchor_terms_body(long long t, int nb, const dou
   e.pad[0] = e.pad[1] = e.pad[2] = 0.f;
 }
 
-__global__ void k_anchor_terms(int nb, const double* __restrict__ ta,
+__global__ void __launch_bounds__(256) k_anchor_terms(int nb, const double* __restrict__ ta,
                                const double* __restrict__ bend64, double* __restrict__ terms,
                                AncRec* __restrict__ anc) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -680,7 +680,7 @@ __global__ void k_anchor_poses_b(BatchFp32Args a) {
                     a.ta + a.blk_off[g] * 12);
 }
 
-__global__ void k_anchor_terms_b(BatchFp32Args a) {
+__global__ void __launch_bounds__(256) k_anchor_terms_b(BatchFp32Args a) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.n_sq) return;
   const int g = batch_graph(a.sq_off, a.G, t);
