@@ -50,20 +50,10 @@ struct ExactCtl {
 constexpr int kExactWaves = 256;
 constexpr int kExactPhaseSeconds = 20;  // device time limit of each phase
 
-__device__ __forceinline__ int wsum_i(int v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
-}
-
-__device__ __forceinline__ int wmin_i(int v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const int o = __shfl_xor(v, off);
-    v = o < v ? o : v;
-  }
-  return v;
-}
+// wavefront sum and max by DPP (whole wavefronts only: every call site runs
+// with all 64 lanes active)
+__device__ __forceinline__ int wsum_i(int v) { return __ockl_wfred_add_i32(v); }
+__device__ __forceinline__ int wmax_i(int v) { return __ockl_wfred_max_i32(v); }
 
 constexpr int kExWPL = 8;  // words per lane of a candidate set (n <= 32768)
 
@@ -99,11 +89,18 @@ __device__ bool colour_fits(const unsigned long long* __restrict__ U, int W,
 #pragma unroll
     for (int k = 0; k < kExWPL; ++k) Q[k] = P[k];
     for (;;) {
+      // lowest vertex of Q: word w = lane + 64 k, so the lowest k with a
+      // non-empty word, its lowest such lane (one ballot per k), that lane's
+      // lowest bit by v_readlane
       int v = INT_MAX;
 #pragma unroll
-      for (int k = kExWPL - 1; k >= 0; --k)
-        if (Q[k]) v = (lane + 64 * k) * 64 + __builtin_ctzll(Q[k]);
-      v = wmin_i(v);
+      for (int k = 0; k < kExWPL; ++k) {
+        const unsigned long long lanes = __ballot(Q[k] != 0ull);
+        if (lanes) {
+          v = __builtin_amdgcn_readlane((lane + 64 * k) * 64 + __builtin_ctzll(Q[k]), __builtin_ctzll(lanes));
+          break;
+        }
+      }
       if (v == INT_MAX) break;
       ++st;
       const int wv = v >> 6;
@@ -116,19 +113,10 @@ __device__ bool colour_fits(const unsigned long long* __restrict__ U, int W,
           P[k] &= ~bit;
           Q[k] &= ~bit;
         }
-        if (w < W) Q[k] &= ~row[w];
+        if (w < W && Q[k]) Q[k] &= ~row[w];  // empty words stay empty: their row words are not read
       }
     }
   }
-}
-
-__device__ __forceinline__ int wmax_i(int v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const int o = __shfl_xor(v, off);
-    v = o > v ? o : v;
-  }
-  return v;
 }
 
 // bits u < seed of word w
