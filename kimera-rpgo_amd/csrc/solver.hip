@@ -280,13 +280,16 @@ __global__ void k_factor_errors(int F, const int* __restrict__ kind, const int* 
 }
 
 // per-factor Gauss-Newton contributions: blocks, gradients and cost (FP64)
-__global__ void k_linearize(int F, const int* __restrict__ kind, const int* __restrict__ fi,
-                            const int* __restrict__ fj, const double* __restrict__ z,
-                            const double* __restrict__ info, const double* __restrict__ w,
-                            const double* __restrict__ x, double* __restrict__ Hii,
-                            double* __restrict__ Hjj, double* __restrict__ Hij,
-                            double* __restrict__ gi, double* __restrict__ gj,
-                            double* __restrict__ cost, int chart) {
+// 64-thread workgroups (the launch shape): the register budget of one
+// wavefront per SIMD.  The products are formed in an order that keeps at
+// most Ji, Jj and one w Lambda J product live (216 VGPRs), without scratch.
+__global__ void __launch_bounds__(64) k_linearize(int F, const int* __restrict__ kind, const int* __restrict__ fi,
+                                                  const int* __restrict__ fj, const double* __restrict__ z,
+                                                  const double* __restrict__ info, const double* __restrict__ w,
+                                                  const double* __restrict__ x, double* __restrict__ Hii,
+                                                  double* __restrict__ Hjj, double* __restrict__ Hij,
+                                                  double* __restrict__ gi, double* __restrict__ gj,
+                                                  double* __restrict__ cost, int chart) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= F) return;
   const int k = kind[f];
@@ -295,23 +298,13 @@ __global__ void k_linearize(int F, const int* __restrict__ kind, const int* __re
                    z + (size_t)f * 12, L, chart);
   const double wf = w[f];
   const double* I = info + (size_t)f * 36;
-  // LJ = w Lambda J (6x6) for J = Ji, Jj
-  double LJi[36], LJj[36], Le[6];
+  // Le = w Lambda e, cost, gradients
+  double Le[6];
 #pragma unroll
   for (int r = 0; r < 6; ++r) {
     double se = 0.0;
 #pragma unroll
-    for (int c = 0; c < 6; ++c) {
-      double si = 0.0, sj = 0.0;
-#pragma unroll
-      for (int m = 0; m < 6; ++m) {
-        si += I[r * 6 + m] * L.Ji[m * 6 + c];
-        sj += I[r * 6 + m] * L.Jj[m * 6 + c];
-      }
-      LJi[r * 6 + c] = wf * si;
-      LJj[r * 6 + c] = wf * sj;
-      se += I[r * 6 + c] * L.e[c];
-    }
+    for (int c = 0; c < 6; ++c) se += I[r * 6 + c] * L.e[c];
     Le[r] = wf * se;
   }
   double q = 0.0;
@@ -328,19 +321,54 @@ __global__ void k_linearize(int F, const int* __restrict__ kind, const int* __re
     }
     gi[(size_t)f * 6 + r] = a;
     gj[(size_t)f * 6 + r] = b;
+  }
+  // LJ = w Lambda Jj: Hjj = Jj^T LJ, Hij = Ji^T LJ
+  {
+    double LJ[36];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) {
-      double sii = 0.0, sjj = 0.0, sij = 0.0;
+    for (int r = 0; r < 6; ++r)
 #pragma unroll
-      for (int m = 0; m < 6; ++m) {
-        sii += L.Ji[m * 6 + r] * LJi[m * 6 + c];
-        sjj += L.Jj[m * 6 + r] * LJj[m * 6 + c];
-        sij += L.Ji[m * 6 + r] * LJj[m * 6 + c];
+      for (int c = 0; c < 6; ++c) {
+        double sj = 0.0;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) sj += I[r * 6 + m] * L.Jj[m * 6 + c];
+        LJ[r * 6 + c] = wf * sj;
       }
-      Hii[(size_t)f * 36 + r * 6 + c] = sii;
-      Hjj[(size_t)f * 36 + r * 6 + c] = sjj;
-      Hij[(size_t)f * 36 + r * 6 + c] = sij;
-    }
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        double sjj = 0.0, sij = 0.0;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+          sjj += L.Jj[m * 6 + r] * LJ[m * 6 + c];
+          sij += L.Ji[m * 6 + r] * LJ[m * 6 + c];
+        }
+        Hjj[(size_t)f * 36 + r * 6 + c] = sjj;
+        Hij[(size_t)f * 36 + r * 6 + c] = sij;
+      }
+  }
+  // LJ = w Lambda Ji: Hii = Ji^T LJ
+  {
+    double LJ[36];
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        double si = 0.0;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) si += I[r * 6 + m] * L.Ji[m * 6 + c];
+        LJ[r * 6 + c] = wf * si;
+      }
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        double sii = 0.0;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) sii += L.Ji[m * 6 + r] * LJ[m * 6 + c];
+        Hii[(size_t)f * 36 + r * 6 + c] = sii;
+      }
   }
 }
 
