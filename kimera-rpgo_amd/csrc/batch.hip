@@ -45,6 +45,10 @@ struct BatchState {
   DevBuf delta, hinv, traj, lcm, odom_dist, flags, accepted, m, c_from, c_to, c_lcm, adj, counters;
   DevBuf deg, cidx, ck;  // clique stage: degrees, members, sizes
   DevBuf cb_state, cb_icc, cb_dm;  // clique stage: chip-wide rounds
+  // clique rounds' graph placement: graphs grouped by XCD (G entries), then
+  // the 9 group offsets; cb_xmax = graphs of the largest group
+  DevBuf cb_xorder;
+  int cb_xmax = 0;
   std::vector<int> m_host;
   // FP32 pair stage (SE(3), RPGO_PCM_FP32_RECHECK): per-graph anchor blocks,
   // anchor-table entries and pair workgroups by prefix, and the stage's tables
@@ -360,7 +364,7 @@ void batch_destroy(rpgo_ctx* ctx) {
   for (DevBuf* b : {&s.d_pose_off, &s.d_lc_off, &s.d_adj_off, &s.d_tile_off, &s.d_sym_off, &s.d_lc_graph, &s.pose0,
                     &s.odom, &s.odom_cov, &s.lc_from, &s.lc_to, &s.lc_pose, &s.lc_cov, &s.delta, &s.hinv, &s.traj,
                     &s.lcm, &s.odom_dist, &s.flags, &s.accepted, &s.m, &s.c_from, &s.c_to, &s.c_lcm, &s.adj,
-                    &s.counters, &s.deg, &s.cidx, &s.ck, &s.cb_state, &s.cb_icc, &s.cb_dm, &s.d_blk_off,
+                    &s.counters, &s.deg, &s.cidx, &s.ck, &s.cb_state, &s.cb_icc, &s.cb_dm, &s.cb_xorder, &s.d_blk_off,
                     &s.d_sq_off, &s.d_pblk_off, &s.fp_prec, &s.fp_lcr, &s.fp_pl64, &s.fp_pb64, &s.fp_bend, &s.fp_ta,
                     &s.fp_sanc, &s.fp_terms, &s.fp_queue})
     b->release();
@@ -502,6 +506,8 @@ static CliqueSpread spread_of(BatchState& s) {
   sp.state = s.cb_state.ptr;
   sp.icc = s.cb_icc.as<int>();
   sp.dm = s.cb_dm.as<unsigned long long>();
+  sp.xorder = s.cb_xorder.as<int>();
+  sp.xmax = s.cb_xmax;
   return sp;
 }
 
@@ -790,6 +796,36 @@ int rpgo_pcm_batch_run(rpgo_ctx* ctx, const rpgo_pcm_batch* B, const rpgo_pcm_pa
         (rc = s.cb_icc.reserve(clique_spread_icc_bytes(G))) ||
         (rc = s.cb_dm.reserve(sizeof(unsigned long long) * nl1)))
       return rc;
+    // The rounds walk each graph's adjacency rows many times over.  Each graph
+    // goes to one XCD, so its rows stay in that XCD's L2, and the graphs are
+    // spread over the 8 XCDs largest first onto the least loaded (work ~ n^2).
+    std::vector<int> xorder((size_t)G + 9);
+    {
+      std::vector<int> byn(G);
+      for (int g = 0; g < G; ++g) byn[g] = g;
+      std::stable_sort(byn.begin(), byn.end(), [&](int a, int b) {
+        return s.lc_off[a + 1] - s.lc_off[a] > s.lc_off[b + 1] - s.lc_off[b];
+      });
+      std::vector<std::vector<int>> grp(8);
+      double load[8] = {};
+      for (int g : byn) {
+        int x = 0;
+        for (int k = 1; k < 8; ++k)
+          if (load[k] < load[x]) x = k;
+        const double n = (double)(s.lc_off[g + 1] - s.lc_off[g]);
+        load[x] += n * n;
+        grp[x].push_back(g);
+      }
+      int o = 0;
+      s.cb_xmax = 0;
+      for (int x = 0; x < 8; ++x) {
+        xorder[(size_t)G + x] = o;
+        for (int g : grp[x]) xorder[o++] = g;
+        s.cb_xmax = std::max<int>(s.cb_xmax, (int)grp[x].size());
+      }
+      xorder[(size_t)G + 8] = o;
+    }
+    if ((rc = up(s.cb_xorder, xorder.data(), sizeof(int) * xorder.size(), st))) return rc;
   }
   float ms_tune = 0.f;
   const int ts = tune_slot(s);
@@ -813,7 +849,7 @@ int rpgo_pcm_batch_run(rpgo_ctx* ctx, const rpgo_pcm_batch* B, const rpgo_pcm_pa
     std::memcpy(&bc, &prm->recheck_chain_coef, 8);
     std::vector<long long> key = {dim, G, (long long)nlc, (long long)nod, s.tile_off[G], s.sym_off[G],
                                   s.adj_off[G], spread_of(s).max_slots, pw, cw, pf, bo, bl, s.fp32 ? 1 : 0,
-                                  bm, bc};
+                                  bm, bc, s.cb_xmax};
     if (s.fp32) {
       key.insert(key.end(), {s.fp32_S, s.blk_off[G], s.sq_off[G], s.pblk_off[G], s.fp_qcap});
       for (DevBuf* b : {&s.d_blk_off, &s.d_sq_off, &s.d_pblk_off, &s.fp_prec, &s.fp_lcr, &s.fp_pl64, &s.fp_pb64,
@@ -823,7 +859,8 @@ int rpgo_pcm_batch_run(rpgo_ctx* ctx, const rpgo_pcm_batch* B, const rpgo_pcm_pa
     for (DevBuf* b : {&s.d_pose_off, &s.d_lc_off, &s.d_adj_off, &s.d_tile_off, &s.d_sym_off, &s.d_lc_graph,
                       &s.pose0, &s.odom, &s.odom_cov, &s.lc_from, &s.lc_to, &s.lc_pose, &s.lc_cov, &s.delta,
                       &s.hinv, &s.traj, &s.lcm, &s.odom_dist, &s.flags, &s.accepted, &s.m, &s.c_from, &s.c_to,
-                      &s.c_lcm, &s.adj, &s.counters, &s.deg, &s.cidx, &s.ck, &s.cb_state, &s.cb_icc, &s.cb_dm})
+                      &s.c_lcm, &s.adj, &s.counters, &s.deg, &s.cidx, &s.ck, &s.cb_state, &s.cb_icc, &s.cb_dm,
+                      &s.cb_xorder})
       key.push_back((long long)(uintptr_t)b->ptr);
     if (s.exec && key == s.graph_key) {
       replayed = 1;

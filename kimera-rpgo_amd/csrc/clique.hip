@@ -378,17 +378,26 @@ __global__ void k_cb_mask(long long n_slots, const int* __restrict__ slot_graph,
   dm[t] = b;
 }
 
-// one wavefront per (graph, window slot): greedy size of candidate pos + k
+// one wavefront per (graph, window slot): greedy size of candidate pos + k.
+// Workgroup b runs on XCD x = b % 8; the XCD's i-th workgroup takes window
+// quarter i % (kCbWindow / 4) of the (i / (kCbWindow / 4))-th graph of XCD x's
+// group in xorder, so each graph's walks share one L2.
 __global__ void __launch_bounds__(256) k_cb_icc(int n_graphs, const long long* __restrict__ adj_off,
                                                 const long long* __restrict__ lc_off,
                                                 const int* __restrict__ m_arr,
                                                 const unsigned long long* __restrict__ adj_all,
                                                 const int* __restrict__ deg_all,
                                                 const unsigned long long* __restrict__ dm,
-                                                const CbState* __restrict__ st, int* __restrict__ icc_out) {
-  const long long t = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int g = (int)(t / kCbWindow), k = (int)(t % kCbWindow);
-  if (g >= n_graphs) return;
+                                                const CbState* __restrict__ st, const int* __restrict__ xorder,
+                                                int* __restrict__ icc_out) {
+  constexpr int kBlkPerGraph = kCbWindow / 4;
+  const int x = blockIdx.x & 7;
+  const int i = (int)(blockIdx.x >> 3);
+  const int gi = xorder[n_graphs + x] + i / kBlkPerGraph;
+  if (gi >= xorder[n_graphs + x + 1]) return;  // beyond this XCD's group
+  const int g = xorder[gi];
+  const int k = (i % kBlkPerGraph) * 4 + (threadIdx.x >> 6);
+  const long long t = (long long)g * kCbWindow + k;
   const int n = m_arr[g], pos = st[g].pos, best = st[g].best;
   const int c = pos + k;
   if (pos >= n || c >= n) return;  // the select step reads only slots below n - pos
@@ -553,11 +562,12 @@ int clique_batch_launch(rpgo_ctx* ctx, int n_graphs, int waves, int prefetch, co
     const unsigned sg = (unsigned)((ns + 255) / 256);
     k_cb_init<<<(n_graphs + 255) / 256, 256, 0, st>>>(n_graphs, cs);
     k_cb_degrees<<<sg, 256, 0, st>>>(ns, spread->slot_graph, d_adj_off, d_lc_off, d_m, adj, deg);
-    const unsigned gi = (unsigned)(((long long)n_graphs * kCbWindow + 3) / 4);
+    const unsigned gi = (unsigned)(8ll * spread->xmax * (kCbWindow / 4));
     const int rounds = (int)((spread->max_slots + kCbWindow - 1) / kCbWindow) + kCbExtraRounds;
     for (int r = 0; r < rounds; ++r) {
       k_cb_mask<<<sg, 256, 0, st>>>(ns, spread->slot_graph, d_lc_off, d_m, deg, cs, spread->dm);
-      k_cb_icc<<<gi, 256, 0, st>>>(n_graphs, d_adj_off, d_lc_off, d_m, adj, deg, spread->dm, cs, spread->icc);
+      k_cb_icc<<<gi, 256, 0, st>>>(n_graphs, d_adj_off, d_lc_off, d_m, adj, deg, spread->dm, cs, spread->xorder,
+                                   spread->icc);
       k_cb_select<<<(n_graphs + 255) / 256, 256, 0, st>>>(n_graphs, d_m, spread->icc, cs);
     }
     RPGO_HIP_CHECK(hipGetLastError());

@@ -247,6 +247,8 @@ struct CliqueSpread {
   void* state;
   int* icc;
   unsigned long long* dm;
+  const int* xorder;  // graphs grouped by XCD (n_graphs entries), then 9 group offsets
+  int xmax;           // graphs of the largest group
 };
 size_t clique_spread_state_bytes(int n_graphs);
 size_t clique_spread_icc_bytes(int n_graphs);
