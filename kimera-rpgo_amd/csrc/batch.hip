@@ -100,7 +100,7 @@ __global__ void __launch_bounds__(64) k_fold_batch(int n_graphs, const long long
                                                   const double* __restrict__ pose0, const double* __restrict__ delta,
                                                   const double* __restrict__ hinv, double* __restrict__ traj) {
   constexpr int PL = R64<DIM>::PL, D = R64<DIM>::D, PWC = R64<DIM>::PWC, DD = D * D;
-  __shared__ double S[DD], T[DD], P[PL], Hs[DD], Dl[PWC];
+  __shared__ double S[DD], T[DD], P[PL], Hs[DD], Dl[PWC], ob[kFoldAhead * PWC];
   __shared__ int ri;
   const int g = blockIdx.x;
   if (g >= n_graphs) return;
@@ -114,51 +114,9 @@ __global__ void __launch_bounds__(64) k_fold_batch(int n_graphs, const long long
   if (l == 0) ri = 1;
   __syncthreads();
   for (int e = l; e < PWC; e += 64) tr[e] = e < PL ? P[e] : (e < PL + DD ? 0.0 : 1.0);
-  // inputs loaded one step ahead (lane l holds entry l), as in k_fold
-  static_assert(PWC <= 64 && DD <= 64, "one entry per lane");
-  double h_next = 0.0, d_next = 0.0;
-  if (n_poses > 1) {
-    if (l < DD) h_next = hinv[(size_t)o0 * DD + l];
-    if (l < PWC) d_next = delta[(size_t)o0 * PWC + l];
-  }
-  for (int k = 0; k + 1 < n_poses; ++k) {
-    if (l < DD) Hs[l] = h_next;
-    if (l < PWC) Dl[l] = d_next;
-    __syncthreads();
-    if (k + 2 < n_poses) {
-      if (l < DD) h_next = hinv[(size_t)(o0 + k + 1) * DD + l];
-      if (l < PWC) d_next = delta[(size_t)(o0 + k + 1) * PWC + l];
-    }
-    if (l < DD) {
-      const int r = l / D, c = l % D;
-      double s = Hs[r * D] * S[c];
-#pragma unroll
-      for (int q = 1; q < D; ++q) s += Hs[r * D + q] * S[q * D + c];
-      T[l] = s;
-    }
-    double newpose = 0.0;
-    if (l >= DD && l < DD + PL) newpose = R64<DIM>::compose_entry(P, Dl, l - DD);
-    __syncthreads();
-    double* out = tr + (size_t)(k + 1) * PWC;
-    if (l < DD) {
-      const int r = l / D, c = l % D;
-      double s = T[r * D] * Hs[c * D];
-#pragma unroll
-      for (int q = 1; q < D; ++q) s += T[r * D + q] * Hs[c * D + q];
-      s = s + Dl[PL + l];
-      S[l] = s;
-      out[PL + l] = s;
-    }
-    if (l >= DD && l < DD + PL) {
-      P[l - DD] = newpose;
-      out[l - DD] = newpose;
-    }
-    if (l == 0) {
-      ri = (ri && Dl[PL + DD] != 0.0) ? 1 : 0;
-      out[PL + DD] = ri ? 1.0 : 0.0;
-    }
-    __syncthreads();
-  }
+  if (n_poses > 1)
+    fold_run<DIM>(n_poses - 1, hinv + (size_t)o0 * DD, delta + (size_t)o0 * PWC, tr + PWC, S, T, P, Hs, Dl, &ri,
+                  ob, l);
 }
 
 template <int DIM>

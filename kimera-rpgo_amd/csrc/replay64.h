@@ -358,4 +358,102 @@ struct R64<2> {
 #undef RPGO_R64_COMMON
 };
 
+// Sequential odometry fold of one chain by one wavefront (Pcm.h:516-557):
+// for step j < n_steps, T_{j+1} = T_j * delta_j and
+// Sigma_{j+1} = (H_j Sigma_j) H_j^T + Sigma_delta_j, in the oracle's order.
+// Lanes 0..D*D-1 own the covariance entries, the next lanes the pose entries;
+// S, P and ri hold the state on entry (step 0's start) and on exit.  hk, dk:
+// H_j and the delta record of step 0 (rows of D*D and PWC doubles); out:
+// the trajectory record written by step 0 (then one per step); ob: LDS for
+// kFoldAhead records.
+// The steps form one dependent chain, so the time per step is the latency of
+// its LDS round trips; no global memory access may wait inside it.  Steps go
+// in blocks of kFoldAhead: the next block's inputs are loaded into registers
+// while this block runs, the block's records collect in LDS and are stored
+// after it, and the only wait for global memory is at the block boundary.
+// The workgroup is this one wavefront, and LDS operations of a wavefront are
+// performed in issue order, so the steps order their LDS traffic with
+// wavefront-scope fences (no instruction; they keep the compiler from moving
+// memory operations across them).  A __syncthreads() per step is a
+// workgroup-scope fence that waits for every global load and store in
+// flight.  Callers compiled with -ffp-contract=off, launched with 64 threads.
+constexpr int kFoldAhead = 8;
+
+__device__ __forceinline__ void fold_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <int DIM>
+__device__ __forceinline__ void fold_run(int n_steps, const double* __restrict__ hk,
+                                         const double* __restrict__ dk, double* __restrict__ out,
+                                         double* S, double* T, double* P, double* Hs, double* Dl, int* ri,
+                                         double* ob, int l) {
+  constexpr int PL = R64<DIM>::PL, D = R64<DIM>::D, PWC = R64<DIM>::PWC, DD = D * D;
+  static_assert(PWC <= 64 && DD <= 64, "one entry per lane");
+  constexpr int A = kFoldAhead;
+  const int lh = l < DD ? l : DD - 1, ld = l < PWC ? l : PWC - 1;  // in-row addresses for every lane
+  double hx[A], dx[A], hy[A], dy[A];
+#pragma unroll
+  for (int u = 0; u < A; ++u) {
+    const int j = u < n_steps ? u : 0;
+    hx[u] = hk[(size_t)j * DD + lh];
+    dx[u] = dk[(size_t)j * PWC + ld];
+  }
+  for (int j0 = 0; j0 < n_steps; j0 += A) {
+    // the next block's inputs: independent of the chain, in flight meanwhile
+#pragma unroll
+    for (int u = 0; u < A; ++u) {
+      const int j = j0 + A + u < n_steps ? j0 + A + u : 0;
+      hy[u] = hk[(size_t)j * DD + lh];
+      dy[u] = dk[(size_t)j * PWC + ld];
+    }
+    const int nb = n_steps - j0 < A ? n_steps - j0 : A;
+#pragma unroll
+    for (int u = 0; u < A; ++u) {
+      if (u >= nb) break;
+      if (l < DD) Hs[l] = hx[u];
+      if (l < PWC) Dl[l] = dx[u];
+      fold_wave_sync();
+      // T = H * Sigma
+      if (l < DD) {
+        const int r = l / D, c = l % D;
+        double s = Hs[r * D] * S[c];
+#pragma unroll
+        for (int m = 1; m < D; ++m) s += Hs[r * D + m] * S[m * D + c];
+        T[l] = s;
+      }
+      double newpose = 0.0;
+      if (l >= DD && l < DD + PL) newpose = R64<DIM>::compose_entry(P, Dl, l - DD);
+      fold_wave_sync();
+      double* o = ob + u * PWC;
+      if (l < DD) {
+        const int r = l / D, c = l % D;
+        double s = T[r * D] * Hs[c * D];
+#pragma unroll
+        for (int m = 1; m < D; ++m) s += T[r * D + m] * Hs[c * D + m];
+        s = s + Dl[PL + l];
+        S[l] = s;
+        o[PL + l] = s;
+      }
+      if (l >= DD && l < DD + PL) {
+        P[l - DD] = newpose;
+        o[l - DD] = newpose;
+      }
+      if (l == 0) {
+        *ri = (*ri && Dl[PL + DD] != 0.0) ? 1 : 0;
+        o[PL + DD] = *ri ? 1.0 : 0.0;
+      }
+      fold_wave_sync();
+    }
+    // the block's records, stored contiguously
+    for (int e = l; e < nb * PWC; e += 64) out[(size_t)j0 * PWC + e] = ob[e];
+#pragma unroll
+    for (int u = 0; u < A; ++u) {
+      hx[u] = hy[u];
+      dx[u] = dy[u];
+    }
+  }
+}
+
 }  // namespace rpgo

@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(64) k_fold(int start, int n_poses,
                                             const double* __restrict__ hinv,
                                             double* __restrict__ traj) {
   constexpr int PL = R64<DIM>::PL, D = R64<DIM>::D, PWC = R64<DIM>::PWC, DD = D * D;
-  __shared__ double S[DD], T[DD], P[PL], Hs[DD], Dl[PWC];
+  __shared__ double S[DD], T[DD], P[PL], Hs[DD], Dl[PWC], ob[kFoldAhead * PWC];
   __shared__ int ri;
   const int l = threadIdx.x;
   if (pose0) {
@@ -89,53 +89,9 @@ __global__ void __launch_bounds__(64) k_fold(int start, int n_poses,
     if (l == 0) ri = rec[PL + DD] != 0.0 ? 1 : 0;
   }
   __syncthreads();
-  // step k's inputs are loaded one step ahead into registers (lane l holds
-  // entry l), so the global-memory latency overlaps the previous step
-  static_assert(PWC <= 64 && DD <= 64, "one entry per lane");
-  double h_next = 0.0, d_next = 0.0;
-  if (start + 1 < n_poses) {
-    if (l < DD) h_next = hinv[(size_t)start * DD + l];
-    if (l < PWC) d_next = delta[(size_t)start * PWC + l];
-  }
-  for (int k = start; k + 1 < n_poses; ++k) {
-    if (l < DD) Hs[l] = h_next;
-    if (l < PWC) Dl[l] = d_next;
-    __syncthreads();
-    if (k + 2 < n_poses) {
-      if (l < DD) h_next = hinv[(size_t)(k + 1) * DD + l];
-      if (l < PWC) d_next = delta[(size_t)(k + 1) * PWC + l];
-    }
-    // T = H * Sigma
-    if (l < DD) {
-      const int r = l / D, c = l % D;
-      double s = Hs[r * D] * S[c];
-#pragma unroll
-      for (int m = 1; m < D; ++m) s += Hs[r * D + m] * S[m * D + c];
-      T[l] = s;
-    }
-    double newpose = 0.0;
-    if (l >= DD && l < DD + PL) newpose = R64<DIM>::compose_entry(P, Dl, l - DD);
-    __syncthreads();
-    double* out = traj + (size_t)(k + 1) * PWC;
-    if (l < DD) {
-      const int r = l / D, c = l % D;
-      double s = T[r * D] * Hs[c * D];
-#pragma unroll
-      for (int m = 1; m < D; ++m) s += T[r * D + m] * Hs[c * D + m];
-      s = s + Dl[PL + l];
-      S[l] = s;
-      out[PL + l] = s;
-    }
-    if (l >= DD && l < DD + PL) {
-      P[l - DD] = newpose;
-      out[l - DD] = newpose;
-    }
-    if (l == 0) {
-      ri = (ri && Dl[PL + DD] != 0.0) ? 1 : 0;
-      out[PL + DD] = ri ? 1.0 : 0.0;
-    }
-    __syncthreads();
-  }
+  if (start + 1 < n_poses)
+    fold_run<DIM>(n_poses - 1 - start, hinv + (size_t)start * DD, delta + (size_t)start * PWC,
+                  traj + (size_t)(start + 1) * PWC, S, T, P, Hs, Dl, &ri, ob, l);
 }
 
 // Fold of several chains, one wavefront per chain (Pcm.h:516-557 per robot
@@ -147,7 +103,7 @@ __global__ void __launch_bounds__(64) k_fold_chains(Chains ch, const double* __r
                                                    const double* __restrict__ hinv,
                                                    double* __restrict__ traj) {
   constexpr int PL = R64<DIM>::PL, D = R64<DIM>::D, PWC = R64<DIM>::PWC, DD = D * D;
-  __shared__ double S[DD], T[DD], P[PL], Hs[DD], Dl[PWC];
+  __shared__ double S[DD], T[DD], P[PL], Hs[DD], Dl[PWC], ob[kFoldAhead * PWC];
   __shared__ int ri;
   const int c = blockIdx.x;
   if (c >= ch.n) return;
@@ -159,50 +115,9 @@ __global__ void __launch_bounds__(64) k_fold_chains(Chains ch, const double* __r
   if (l == 0) ri = 1;
   __syncthreads();
   for (int e = l; e < PWC; e += 64) tr[e] = e < PL ? P[e] : (e < PL + DD ? 0.0 : 1.0);
-  static_assert(PWC <= 64 && DD <= 64, "one entry per lane");
-  double h_next = 0.0, d_next = 0.0;
-  if (n_poses > 1) {
-    if (l < DD) h_next = hinv[(size_t)o0 * DD + l];
-    if (l < PWC) d_next = delta[(size_t)o0 * PWC + l];
-  }
-  for (int k = 0; k + 1 < n_poses; ++k) {
-    if (l < DD) Hs[l] = h_next;
-    if (l < PWC) Dl[l] = d_next;
-    __syncthreads();
-    if (k + 2 < n_poses) {
-      if (l < DD) h_next = hinv[(size_t)(o0 + k + 1) * DD + l];
-      if (l < PWC) d_next = delta[(size_t)(o0 + k + 1) * PWC + l];
-    }
-    if (l < DD) {
-      const int r = l / D, cc = l % D;
-      double sacc = Hs[r * D] * S[cc];
-#pragma unroll
-      for (int m = 1; m < D; ++m) sacc += Hs[r * D + m] * S[m * D + cc];
-      T[l] = sacc;
-    }
-    double newpose = 0.0;
-    if (l >= DD && l < DD + PL) newpose = R64<DIM>::compose_entry(P, Dl, l - DD);
-    __syncthreads();
-    double* out = tr + (size_t)(k + 1) * PWC;
-    if (l < DD) {
-      const int r = l / D, cc = l % D;
-      double sacc = T[r * D] * Hs[cc * D];
-#pragma unroll
-      for (int m = 1; m < D; ++m) sacc += T[r * D + m] * Hs[cc * D + m];
-      sacc = sacc + Dl[PL + l];
-      S[l] = sacc;
-      out[PL + l] = sacc;
-    }
-    if (l >= DD && l < DD + PL) {
-      P[l - DD] = newpose;
-      out[l - DD] = newpose;
-    }
-    if (l == 0) {
-      ri = (ri && Dl[PL + DD] != 0.0) ? 1 : 0;
-      out[PL + DD] = ri ? 1.0 : 0.0;
-    }
-    __syncthreads();
-  }
+  if (n_poses > 1)
+    fold_run<DIM>(n_poses - 1, hinv + (size_t)o0 * DD, delta + (size_t)o0 * PWC, tr + PWC, S, T, P, Hs, Dl, &ri,
+                  ob, l);
 }
 
 // ---------------------------------------------------------------------------
