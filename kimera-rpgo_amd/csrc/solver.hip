@@ -491,10 +491,30 @@ __global__ void k_assemble(int N, const int* __restrict__ off, const int* __rest
 // blocks and the coupling across segment borders.  S = 1 is block-Jacobi.
 // ---------------------------------------------------------------------------
 
+// DPP row rotation of an FP64 value (both halves), all lanes of the row valid
+template <int N>
+__device__ __forceinline__ double row_ror_f64(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  return __hiloint2double(__builtin_amdgcn_update_dpp(0, hi, 0x120 + N, 0xF, 0xF, false),
+                          __builtin_amdgcn_update_dpp(0, lo, 0x120 + N, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                          __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+
+// Sum over the 64 lanes of a converged wavefront, in every lane: rotations
+// within each row of 16 lanes by DPP (VALU, a few cycles each), then the four
+// row sums by v_readlane.  A six-step shuffle reduction went through the LDS
+// crossbar (ds_bpermute) at every step, on the critical path of each PCG
+// launch.
 __device__ __forceinline__ double wave_sum_pre(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
+  v += row_ror_f64<8>(v);
+  v += row_ror_f64<4>(v);
+  v += row_ror_f64<2>(v);
+  v += row_ror_f64<1>(v);
+  return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
 }
 
 // block-wide FP64 sum (all threads call it; the result is valid in thread 0)
@@ -849,12 +869,6 @@ __global__ void __launch_bounds__(256) k_chol_update(int n, int k0, double* __re
 // row i of LT (coalesced), prefetched PF rows ahead.  The row-oriented form
 // (w_i from a dot product over j < i) needed a wave reduction per row:
 // 691 us at 630 unknowns, 5k poses.
-__device__ __forceinline__ double readlane_f64(double v, int lane) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
 template <int PL>  // rows per lane: n <= 64 PL
 __global__ void __launch_bounds__(256) k_tri_inv(int n, const double* __restrict__ LT, double* __restrict__ W) {
   constexpr int PF = 4;
@@ -1007,13 +1021,7 @@ __global__ void k_spmv(int N, const int* __restrict__ off, const int* __restrict
   for (int r = 0; r < 6; ++r) y[(size_t)p * 6 + r] = acc[r];
 }
 
-// FP64 dot, wave-reduced then atomically accumulated
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
-}
-
+// FP64 dot, block-reduced then atomically accumulated
 __global__ void k_dot(long long n, const double* __restrict__ a, const double* __restrict__ b,
                       double* __restrict__ out) {
   __shared__ double red[16];
@@ -1061,10 +1069,8 @@ __device__ __forceinline__ void red_add(double* sc, int which, int a, double v) 
 // the sum over the slots, in every lane of a converged wavefront
 __device__ __forceinline__ double red_sum(double* sc, int which, int a) {
   const int lane = threadIdx.x & 63;
-  double v = lane < kRedSlots ? *red_slot(sc, which, a, lane) : 0.0;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
+  const double v = lane < kRedSlots ? *red_slot(sc, which, a, lane) : 0.0;
+  return wave_sum_pre(v);
 }
 
 // clear the slots of (which, a): lanes < kRedSlots of one wavefront
