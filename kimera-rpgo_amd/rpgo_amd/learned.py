@@ -34,7 +34,7 @@ import numpy as np
 from . import _capi, problem, synth
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-MODEL_PATH = os.path.join(os.path.dirname(HERE), "models", "learned_init.npz")
+MODEL_PATH = os.path.join(os.path.dirname(HERE), "weights", "learned_init.npz")
 
 
 @dataclass
