@@ -435,7 +435,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
+        print(f"bench.py: the implementation runs on one GPU; --gpus {a.gpus} is not supported "
               "(the multi-GPU sharding is a design only)", file=sys.stderr)
         sys.exit(2)
 

@@ -1,6 +1,6 @@
 """GPU robust solve (rpgo_solve) against the FP64 oracle.
 
-Bar (north star): optimized poses within 1e-5 m / 1e-5 rad of the oracle's
+Bar: optimized poses within 1e-5 m / 1e-5 rad of the oracle's
 converged FP64 solution; GNC weight vectors equal; reference KATs of
 tests/testGnc.cpp:25-103 (robot_a) reproduced.
 """
