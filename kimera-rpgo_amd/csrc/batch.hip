@@ -120,7 +120,7 @@ __global__ void __launch_bounds__(64) k_fold_batch(int n_graphs, const long long
 }
 
 template <int DIM>
-__global__ void k_odom_batch(long long n_lc, const int* __restrict__ lc_graph, const long long* __restrict__ pose_off,
+__global__ void __launch_bounds__(256) k_odom_batch(long long n_lc, const int* __restrict__ lc_graph, const long long* __restrict__ pose_off,
                              const double* __restrict__ traj, const int* __restrict__ from,
                              const int* __restrict__ to, const double* __restrict__ lcm, double thr,
                              double* __restrict__ dist, int* __restrict__ flag) {
@@ -270,7 +270,7 @@ __global__ void __launch_bounds__(64) k_sym_batch(int n_graphs, long long n_bloc
 // overturned FP32 decision.  counters: u64[0] consistent pairs, u32[2] queue
 // length, u64[2] overturned decisions.
 template <int DIM>
-__global__ void k_recheck_batch(int n_graphs, const long long* __restrict__ pose_off,
+__global__ void __launch_bounds__(256) k_recheck_batch(int n_graphs, const long long* __restrict__ pose_off,
                                 const long long* __restrict__ lc_off, const long long* __restrict__ adj_off,
                                 const int* __restrict__ m_arr, const double* __restrict__ traj,
                                 const int* __restrict__ cf, const int* __restrict__ ct,

@@ -73,7 +73,7 @@ __device__ __forceinline__ double wave_sum_l(double v) {
 // 12 log m, 13 log N
 // 14 log(1 + distance of the two chain poses)
 // 15 log(1 + translation residual at the chain poses)
-__global__ void k_lc_features(int m, int n_poses, int W, const double* __restrict__ traj,
+__global__ void __launch_bounds__(128) k_lc_features(int m, int n_poses, int W, const double* __restrict__ traj,
                               const int* __restrict__ c_from, const int* __restrict__ c_to,
                               const double* __restrict__ c_lcm, const int* __restrict__ accepted,
                               const double* __restrict__ odom_dist, const uint64_t* __restrict__ adj,

@@ -199,7 +199,7 @@ __device__ __forceinline__ void block_local_body(int p, int n_poses, const doubl
 }
 
 // one thread per anchor block
-__global__ void k_block_local(int n_poses, int n_blocks, const double* __restrict__ delta,
+__global__ void __launch_bounds__(64) k_block_local(int n_poses, int n_blocks, const double* __restrict__ delta,
                               const double* __restrict__ hinv, PoseRec* __restrict__ prec,
                               double* __restrict__ pl64, double* __restrict__ pblock64,
                               double* __restrict__ bend64) {
@@ -663,7 +663,7 @@ __device__ __forceinline__ int batch_graph(const long long* __restrict__ off, in
   return lo;
 }
 
-__global__ void k_block_local_b(BatchFp32Args a) {
+__global__ void __launch_bounds__(64) k_block_local_b(BatchFp32Args a) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.n_blocks) return;
   const int g = batch_graph(a.blk_off, a.G, t);

@@ -124,7 +124,7 @@ __global__ void __launch_bounds__(64) k_fold_chains(Chains ch, const double* __r
 // odometry check (Pcm.h:604-629) : one thread per loop closure
 // ---------------------------------------------------------------------------
 template <int DIM>
-__global__ void k_odom_check(int n_lc, const double* __restrict__ traj,
+__global__ void __launch_bounds__(256) k_odom_check(int n_lc, const double* __restrict__ traj,
                              const int* __restrict__ from, const int* __restrict__ to,
                              const double* __restrict__ lcm, double thr,
                              double* __restrict__ dist, int* __restrict__ flag, Chains ch) {
@@ -395,7 +395,7 @@ __device__ __forceinline__ void recheck_one(int2 q, const double* __restrict__ t
 }
 
 template <int DIM>
-__global__ void k_recheck64(int n, const int2* __restrict__ queue, const double* __restrict__ traj,
+__global__ void __launch_bounds__(64) k_recheck64(int n, const int2* __restrict__ queue, const double* __restrict__ traj,
                             const int* __restrict__ cf, const int* __restrict__ ct,
                             const double* __restrict__ clcm, double thr, int m, int W,
                             unsigned long long* __restrict__ adj, double* __restrict__ dist,

@@ -607,7 +607,7 @@ __device__ void spd_inverse6(const double* A, double* Ainv) {
 
 // Factor: Dp_k = A_kk - U_{k-1}^T Dp_{k-1}^-1 U_{k-1}; stores Dp_k^-1 (Minv)
 // and U_k = H_{k,k+1} (Useg, zero at a segment end or without odometry).
-__global__ void k_seg_factor(int N, int S, const int* __restrict__ chain, const double* __restrict__ D,
+__global__ void __launch_bounds__(64) k_seg_factor(int N, int S, const int* __restrict__ chain, const double* __restrict__ D,
                              const double* __restrict__ hdiag, const double* __restrict__ Hij,
                              double lambda, int diag_damping, float* __restrict__ Minv,
                              float* __restrict__ Useg) {
@@ -718,7 +718,7 @@ __device__ __forceinline__ void ptmp(const float* P1, const double* M, const flo
 }
 
 // Ac(s,s) += P_k^T (D_k + lambda damp_k) P_k
-__global__ void k_coarse_diag(int N, int Sc, int nc, const float* __restrict__ Pb,
+__global__ void __launch_bounds__(128) k_coarse_diag(int N, int Sc, int nc, const float* __restrict__ Pb,
                               const double* __restrict__ D, const double* __restrict__ hdiag,
                               double lambda, int diag_damping, double* __restrict__ Ac) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -735,7 +735,7 @@ __global__ void k_coarse_diag(int N, int Sc, int nc, const float* __restrict__ P
 }
 
 // Ac(s,t) += P_i^T H_ij P_j and Ac(t,s) += its transpose, per between factor
-__global__ void k_coarse_offdiag(int F, int Sc, int nc, const int* __restrict__ kind,
+__global__ void __launch_bounds__(128) k_coarse_offdiag(int F, int Sc, int nc, const int* __restrict__ kind,
                                  const int* __restrict__ fi, const int* __restrict__ fj,
                                  const double* __restrict__ Hij, const float* __restrict__ Pb,
                                  double* __restrict__ Ac) {
