@@ -137,6 +137,18 @@ def fp32_error(ctx, w):
             "log_near_pi": "rotation within ~3e-3 rad of pi: replayed in FP64 (counted in pairs_replayed_fp64)"}
 
 
+def heu_clique(ctx):
+    """The reference's heuristic clique on the last adjacency, timed (HIP
+    events on the library stream) on the second of two calls, after the
+    first has loaded the kernels; both calls must agree."""
+    from rpgo_amd import _capi
+    k0, idx0, _ = ctx.max_clique(_capi.CLIQUE_HEU_REF)
+    k, idx, ms = ctx.max_clique(_capi.CLIQUE_HEU_REF)
+    if k != k0 or not np.array_equal(idx, idx0):
+        raise RuntimeError("heuristic clique differs between two calls on the same adjacency")
+    return k, idx, ms
+
+
 def exact_clique(ctx, heu_size):
     """Exact maximum clique (FMC::maxClique semantics) on the last adjacency:
     size, time, and whether it reached at least the heuristic's size; a node
@@ -480,7 +492,7 @@ def main():
         "n_rechecked": r["stats"][-1].n_rechecked,
         "n_flipped": r["stats"][-1].n_flipped,
     }
-    k, _, ms_clq = ctx.max_clique(_capi.CLIQUE_HEU_REF)
+    k, _, ms_clq = heu_clique(ctx)
     out["clique_heu_ref"] = {"size": k, "ms": ms_clq}
     precision = {}
     if a.side and a.precision == "fp32":
@@ -493,7 +505,7 @@ def main():
         go = graph(other, a.seed)
         progress(f"side workload: {other} pairwise stage")
         ro = pcm_pass(ctx, go, wo, prec, a.steps, a.warmup)
-        ko, _, mso = ctx.max_clique(_capi.CLIQUE_HEU_REF)
+        ko, _, mso = heu_clique(ctx)
         out[other] = {"pairs": ro["pairs"], "pair_checks_per_s": ro["pairs"] / (ro["ms_step"] * 1e-3),
                       "ms_per_step": ro["ms_step"], "ms_pairs_kernel": ro["ms_kernel"],
                       "tflops_kernel": FLOP_PER_PAIR[3] * ro["pairs"] / (ro["ms_kernel"] * 1e-3) / 1e12,

@@ -303,7 +303,7 @@ typedef struct {
   int32_t pair_waves;       /* wavefronts per workgroup of the pair stage      */
   int32_t clique_waves;     /* wavefronts per graph of the clique stage        */
   int32_t graph_replayed;   /* 1 when an existing hipGraph was replayed        */
-  int32_t clique_prefetch;  /* rows loaded ahead in the clique stage's walk    */
+  int32_t clique_prefetch;  /* rows loaded together in the clique walk         */
   int32_t fp32;             /* 1 when the FP32 pair stage ran (precision FP32_RECHECK,
                                dim 3, no covariance without rotation information,
                                at most 2^29 pairs in the batch)                 */

@@ -59,7 +59,7 @@ struct BatchState {
   DevBuf fp_prec, fp_lcr, fp_pl64, fp_pb64, fp_bend, fp_ta, fp_sanc, fp_terms, fp_queue;
   long long fp_qcap = 0;
   // launch shapes chosen by the autotuner, per tune_slot() (0: not tuned yet)
-  int tuned_pair[8] = {}, tuned_clique[8] = {}, tuned_prefetch[8] = {};
+  int tuned_pair[8] = {}, tuned_clique[8] = {};
   // captured stages and the layout they were captured for
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -469,8 +469,7 @@ static CliqueSpread spread_of(BatchState& s) {
   return sp;
 }
 
-static int launch_stages(rpgo_ctx* ctx, BatchState& s, const rpgo_pcm_params* prm, int pw, int cw, int pf,
-                         bool events) {
+static int launch_stages(rpgo_ctx* ctx, BatchState& s, const rpgo_pcm_params* prm, int pw, int cw, bool events) {
   hipStream_t st = ctx->stream;
   hipEvent_t* ev = ctx->ev;
   const int G = s.n_graphs, dim = s.dim;
@@ -529,7 +528,7 @@ static int launch_stages(rpgo_ctx* ctx, BatchState& s, const rpgo_pcm_params* pr
   if (events) RPGO_HIP_CHECK(hipEventRecord(ev[3], st));
   if (cw) {
     const CliqueSpread sp = spread_of(s);
-    if ((rc = clique_batch_launch(ctx, G, cw, pf, s.d_adj_off.as<long long>(), s.d_lc_off.as<long long>(),
+    if ((rc = clique_batch_launch(ctx, G, cw, s.d_adj_off.as<long long>(), s.d_lc_off.as<long long>(),
                                   s.m.as<int>(), s.adj.as<unsigned long long>(), s.deg.as<int>(), s.cidx.as<int>(),
                                   s.ck.as<int>(), &sp)))
       return rc;
@@ -540,16 +539,15 @@ static int launch_stages(rpgo_ctx* ctx, BatchState& s, const rpgo_pcm_params* pr
 
 // Launch-shape autotuner: runs the stages once, then times on this batch the
 // pair stage for 1, 2, 4 and 8 wavefronts per workgroup (best of two launches)
-// and the clique stage for 4, 8 and 16 wavefronts per graph, then the fastest
-// of those with row prefetch in the greedy walk (one launch each: a clique
-// launch of a large batch takes long enough to time once), and keeps the
-// fastest per dim.  Every candidate computes the same bits.
+// and the clique stage for 4, 8 and 16 wavefronts per graph (one launch each:
+// a clique launch of a large batch takes long enough to time once), and keeps
+// the fastest per dim.  Every candidate computes the same bits.
 static int autotune(rpgo_ctx* ctx, BatchState& s, const rpgo_pcm_params* prm, bool clique, float* ms_tune) {
   hipStream_t st = ctx->stream;
   hipEvent_t* ev = ctx->ev;
   int rc;
   RPGO_HIP_CHECK(hipEventRecord(ev[6], st));
-  if ((rc = launch_stages(ctx, s, prm, kDefaultPairWaves, clique ? kDefaultCliqueWaves : 0, 1, false))) return rc;
+  if ((rc = launch_stages(ctx, s, prm, kDefaultPairWaves, clique ? kDefaultCliqueWaves : 0, false))) return rc;
   auto timed = [&](auto&& launch, float* t_min, int reps = 2) -> int {
     *t_min = 1e30f;
     for (int rep = 0; rep < reps; ++rep) {
@@ -578,12 +576,12 @@ static int autotune(rpgo_ctx* ctx, BatchState& s, const rpgo_pcm_params* prm, bo
   s.tuned_pair[tune_slot(s)] = best_pw;
   if (clique) {
     best_t = 1e30f;
-    int best_cw = kDefaultCliqueWaves, best_pf = 1;
-    auto clique_time = [&](int cw, int pf, float* t) {
+    int best_cw = kDefaultCliqueWaves;
+    auto clique_time = [&](int cw, float* t) {
       return timed(
           [&] {
             const CliqueSpread sp = spread_of(s);
-            return clique_batch_launch(ctx, s.n_graphs, cw, pf, s.d_adj_off.as<long long>(),
+            return clique_batch_launch(ctx, s.n_graphs, cw, s.d_adj_off.as<long long>(),
                                        s.d_lc_off.as<long long>(), s.m.as<int>(), s.adj.as<unsigned long long>(),
                                        s.deg.as<int>(), s.cidx.as<int>(), s.ck.as<int>(), &sp);
           },
@@ -591,19 +589,13 @@ static int autotune(rpgo_ctx* ctx, BatchState& s, const rpgo_pcm_params* prm, bo
     };
     for (int cw : {4, 8, 16}) {
       float t;
-      if ((rc = clique_time(cw, 1, &t))) return rc;
+      if ((rc = clique_time(cw, &t))) return rc;
       if (t < best_t) {
         best_t = t;
         best_cw = cw;
       }
     }
-    {
-      float t;
-      if ((rc = clique_time(best_cw, clique_prefetch_depth(), &t))) return rc;
-      if (t < best_t) best_pf = clique_prefetch_depth();
-    }
     s.tuned_clique[tune_slot(s)] = best_cw;
-    s.tuned_prefetch[tune_slot(s)] = best_pf;
   }
   RPGO_HIP_CHECK(hipEventRecord(ev[9], st));
   RPGO_HIP_CHECK(hipEventSynchronize(ev[9]));
@@ -792,7 +784,6 @@ int rpgo_pcm_batch_run(rpgo_ctx* ctx, const rpgo_pcm_batch* B, const rpgo_pcm_pa
   }
   const int pw = s.tuned_pair[ts] ? s.tuned_pair[ts] : kDefaultPairWaves;
   const int cw = want_clique ? (s.tuned_clique[ts] ? s.tuned_clique[ts] : kDefaultCliqueWaves) : 0;
-  const int pf = s.tuned_prefetch[ts] ? s.tuned_prefetch[ts] : 1;
   hipEvent_t* ev = ctx->ev;
   int replayed = 0;
   if (B->flags & RPGO_BATCH_GRAPH) {
@@ -806,7 +797,7 @@ int rpgo_pcm_batch_run(rpgo_ctx* ctx, const rpgo_pcm_batch* B, const rpgo_pcm_pa
     std::memcpy(&bm, &prm->recheck_margin, 8);
     std::memcpy(&bc, &prm->recheck_chain_coef, 8);
     std::vector<long long> key = {dim, G, (long long)nlc, (long long)nod, s.tile_off[G], s.sym_off[G],
-                                  s.adj_off[G], spread_of(s).max_slots, pw, cw, pf, bo, bl, s.fp32 ? 1 : 0,
+                                  s.adj_off[G], spread_of(s).max_slots, pw, cw, bo, bl, s.fp32 ? 1 : 0,
                                   bm, bc, s.cb_xmax};
     if (s.fp32) {
       key.insert(key.end(), {s.fp32_S, s.blk_off[G], s.sq_off[G], s.pblk_off[G], s.fp_qcap});
@@ -829,7 +820,7 @@ int rpgo_pcm_batch_run(rpgo_ctx* ctx, const rpgo_pcm_batch* B, const rpgo_pcm_pa
       s.graph = nullptr;
       s.graph_key.clear();
       RPGO_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
-      const int rc_cap = launch_stages(ctx, s, prm, pw, cw, pf, false);
+      const int rc_cap = launch_stages(ctx, s, prm, pw, cw, false);
       hipGraph_t gr = nullptr;
       const hipError_t e_end = hipStreamEndCapture(st, &gr);
       if (rc_cap) {
@@ -845,7 +836,7 @@ int rpgo_pcm_batch_run(rpgo_ctx* ctx, const rpgo_pcm_batch* B, const rpgo_pcm_pa
     RPGO_HIP_CHECK(hipGraphLaunch(s.exec, st));
     RPGO_HIP_CHECK(hipEventRecord(ev[5], st));
   } else {
-    if ((rc = launch_stages(ctx, s, prm, pw, cw, pf, true))) return rc;
+    if ((rc = launch_stages(ctx, s, prm, pw, cw, true))) return rc;
   }
   s.m_host.assign(G, 0);
   RPGO_HIP_CHECK(hipMemcpyAsync(s.m_host.data(), s.m.ptr, sizeof(int) * G, hipMemcpyDeviceToHost, st));
@@ -902,7 +893,7 @@ int rpgo_pcm_batch_run(rpgo_ctx* ctx, const rpgo_pcm_batch* B, const rpgo_pcm_pa
     stats->ms_tune = ms_tune;
     stats->pair_waves = pw;
     stats->clique_waves = cw;
-    stats->clique_prefetch = want_clique ? pf : 0;
+    stats->clique_prefetch = want_clique ? clique_walk_chunk() : 0;
     stats->graph_replayed = replayed;
   }
   return RPGO_OK;

@@ -83,25 +83,63 @@ __device__ __forceinline__ int wave_popcount(const unsigned long long* S) {
   return __ockl_wfred_add_i32(cnt);  // DPP wavefront sum
 }
 
-// rows fetched ahead of the greedy walk (see icc_wave)
-constexpr int kPrefetch = 4;
+__device__ __forceinline__ unsigned long long readlane_u64(unsigned long long v, int l) {
+  const unsigned int lo = __builtin_amdgcn_readlane((int)(unsigned int)v, l);
+  const unsigned int hi = __builtin_amdgcn_readlane((int)(unsigned int)(v >> 32), l);
+  return ((unsigned long long)hi << 32) | lo;
+}
 
-// Greedy size icc(c, best) of candidate c, one wavefront (S in registers,
-// word w = lane + 64 q); dmask holds the deg >= best filter (LDS or global).
-// Only icc > best matters to the scan, so the walk stops early (INT_MIN) once
-// it cannot get there: every further step removes at least the picked vertex
-// from S, so at most |S| + 1 increments remain.  A candidate that beats best
-// never stops early and returns its exact size.
-// Each step's row load depends on the previous step, so the walk is bound by
-// memory latency.  The rows of the kPrefetch highest vertices of S are loaded
-// together: in a dense block the next pick is the next-highest vertex, and a
-// step whose pick differs from the prediction fetches anew from the current
-// S.  The steps taken are the same as one row at a time.
-// PF = 1 walks one row at a time (the cheaper form when many candidates run
-// at once and the chip is busy anyway); PF = kPrefetch hides the latency of a
-// lone walk.
-template <int PF>
-__device__ __forceinline__ int icc_wave(int W, int c, const unsigned long long* __restrict__ adj,
+// Most picks of one block of the walk whose rows are loaded together.
+constexpr int kWalkChunk = 16;
+
+// S /\ N(p) for the next K picks p of P (lowest first; P loses them) in
+// rows blk[p * W]: the rows' words are loaded together per word slot q.  A
+// pick past the end of P repeats the first (a row ANDed twice changes
+// nothing).  Lanes whose word of S is empty read word wt of the same rows,
+// one shared line, and keep their zero; a slot empty on every lane is skipped.
+template <int K>
+__device__ __forceinline__ void and_pick_rows(unsigned long long* S, unsigned long long& P,
+                                              const unsigned long long* __restrict__ blk, int W, int wt,
+                                              int lane) {
+  int pi[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    pi[k] = P ? __builtin_ctzll(P) : pi[0];
+    P &= P - 1ull;
+  }
+#pragma unroll
+  for (int q = 0; q < kMaxWPL; ++q) {
+    if (64 * q >= W || !__ballot(lane + 64 * q < W && S[q] != 0ull)) continue;  // slot q empty on every lane
+    const int w = lane + 64 * q;
+    const int wa = (w < W && S[q] != 0ull) ? w : wt;
+    unsigned long long r[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) r[k] = blk[(size_t)pi[k] * W + wa];
+    unsigned long long a = S[q];
+#pragma unroll
+    for (int k = 0; k < K; ++k) a &= r[k];
+    S[q] = a;
+  }
+}
+
+// Greedy size icc(c, best) of candidate c, one wavefront (S as a bitset in
+// registers, word w = lane + 64 q; dmask holds the deg >= best filter, LDS or
+// global).  The walk picks the highest vertex m of S and keeps S /\ N(m)
+// until S is empty; it is taken a word of S at a time.  While S
+// keeps a vertex of its highest non-empty word wt, every pick lies in wt: the
+// vertices of S outside wt are all lower.  So the picks inside wt follow from
+// the 64 x 64 diagonal block of the adjacency alone (one row word per vertex,
+// loaded at once, the walk then runs on scalars by v_readlane), and the word
+// leaves S when its last vertex is picked or removed.  The other words of S
+// keep the common neighbours of those picks, whose rows are loaded
+// kWalkChunk at a time with no load waiting on another.  A dense block then
+// costs two load round trips per 64 vertices instead of one per pick.  Only
+// icc > best matters to the scan, so the walk stops early (INT_MIN) after a
+// chunk of rows once it cannot get there (the word's picks are all counted,
+// and every later pick removes at least itself from S, so at most |S| + 1
+// increments remain); a candidate that beats best never stops early and
+// returns its exact size.
+__device__ __forceinline__ int icc_walk(int W, int c, const unsigned long long* __restrict__ adj,
                                         const unsigned long long* dmask, int lane, int best) {
   unsigned long long S[kMaxWPL];
 #pragma unroll
@@ -111,67 +149,200 @@ __device__ __forceinline__ int icc_wave(int W, int c, const unsigned long long* 
                       ((w == (c >> 6)) ? ~(1ull << (c & 63)) : ~0ull))
                    : 0ull;
   }
-  int icc = 0;
   if (wave_popcount(S) + 1 <= best) return INT_MIN;
-  if (PF == 1) {
-    for (;;) {
-      ++icc;
-      const int m = wave_top_bit(S, lane, W);
-      if (m < 0) return icc;  // S held only the seed c: removing it empties S
-      // S /\ N(m); m itself always leaves S, so the walk ends within n steps.
-      // Words of S already empty stay empty: their row words are not read.
-      const unsigned long long* row = adj + (size_t)m * W;
-#pragma unroll
-      for (int q = 0; q < kMaxWPL; ++q) {
-        const int w = lane + 64 * q;
-        if (w < W && S[q]) S[q] &= row[w] & ((w == (m >> 6)) ? ~(1ull << (m & 63)) : ~0ull);
-      }
-      if ((icc & 7) == 0 && icc + wave_popcount(S) + 1 <= best) return INT_MIN;
-    }
-  }
+  int picks = 0;
   for (;;) {
-    // predicted picks: the PF highest vertices of S, highest first
-    int cand[PF];
-    {
-      unsigned long long T[kMaxWPL];
+    // highest non-empty word wt of S and its vertices U (wave-uniform)
+    int wt = -1;
+    unsigned long long U = 0ull;
 #pragma unroll
-      for (int q = 0; q < kMaxWPL; ++q) T[q] = S[q];
-#pragma unroll
-      for (int p = 0; p < PF; ++p) {
-        const int t = wave_top_bit(T, lane, W);
-        cand[p] = t;
-        if (t >= 0) {
-          const int w = t >> 6;
-#pragma unroll
-          for (int q = 0; q < kMaxWPL; ++q)
-            if (lane + 64 * q == w) T[q] &= ~(1ull << (t & 63));
-        }
-      }
-    }
-    unsigned long long Rw[PF][kMaxWPL];
-#pragma unroll
-    for (int p = 0; p < PF; ++p)
-#pragma unroll
-      for (int q = 0; q < kMaxWPL; ++q) {
-        const int w = lane + 64 * q;
-        Rw[p][q] = (cand[p] >= 0 && w < W && S[q]) ? adj[(size_t)cand[p] * W + w] : 0ull;  // empty words stay empty
-      }
-#pragma unroll
-    for (int p = 0; p < PF; ++p) {
-      ++icc;
-      const int m = wave_top_bit(S, lane, W);
-      if (m < 0) return icc;  // S held only the seed c: removing it empties S
-      if (m != cand[p]) {     // prediction missed (wave-uniform): fetch again
-        --icc;
+    for (int q = kMaxWPL - 1; q >= 0; --q) {
+      if (64 * q >= W) continue;  // no words here (wave-uniform)
+      const unsigned long long lanes = __ballot(lane + 64 * q < W && S[q] != 0ull);
+      if (lanes) {
+        const int L = 63 - __clzll(lanes);
+        wt = 64 * q + L;
+        U = readlane_u64(S[q], L);
         break;
       }
-      // S /\ N(m); m itself always leaves S, so the walk ends within n steps
+    }
+    if (wt < 0) return picks + 1;  // S held only the seed c
+    const unsigned long long* blk = adj + (size_t)64 * wt * W;  // rows 64 wt .. 64 wt + 63
+    // lane k: word wt of row 64 wt + k for k in U (a lone vertex is its own pick);
+    // the other lanes read the top vertex's word and drop it
+    unsigned long long dw = 0ull;
+    if (U & (U - 1)) {
+      const bool in = (U >> lane) & 1ull;
+      const unsigned long long v = blk[(size_t)(in ? lane : 63 - __clzll(U)) * W + wt];
+      dw = in ? v : 0ull;
+    }
+    unsigned long long P = 0ull;  // picks in word wt, highest first
+    while (U) {
+      const int p = 63 - __clzll(U);
+      P |= 1ull << p;
+      U &= readlane_u64(dw, p) & ~(1ull << p);
+    }
+    picks += __popcll(P);
+#pragma unroll
+    for (int q = 0; q < kMaxWPL; ++q)
+      if (lane + 64 * q == wt) S[q] = 0ull;
+    // S /\ N(p) over the picks p, up to kWalkChunk rows at a time; a short
+    // remainder takes the next power of two, so few loads are wasted (the
+    // walk is bound by the loads it issues)
+    while (P) {
+      const int np = __popcll(P);
+      if (np > 8) and_pick_rows<kWalkChunk>(S, P, blk, W, wt, lane);
+      else if (np > 4) and_pick_rows<8>(S, P, blk, W, wt, lane);
+      else if (np > 2) and_pick_rows<4>(S, P, blk, W, wt, lane);
+      else if (np == 2) and_pick_rows<2>(S, P, blk, W, wt, lane);
+      else and_pick_rows<1>(S, P, blk, W, wt, lane);
+      // every pick of word wt is already counted and S only shrinks
+      if (picks + wave_popcount(S) + 1 <= best) return INT_MIN;
+    }
+  }
+}
+
+// Picks whose rows a replay stages in LDS at once: kReplayChunk x kMaxWPL x
+// 64 words of 8 B (32 KB) per replaying wavefront.
+constexpr int kReplayChunk = 8;
+constexpr int kReplayLds = kReplayChunk * kMaxWPL * 64;
+
+// Replay of the winning candidate c: the walk of icc_walk, one word of S at a
+// time, that also writes the reference's buffer positions 0..s0-1 (position p
+// keeps the value last written to it) and returns the greedy size.  One
+// wavefront; rl is its LDS staging area (kReplayLds words).
+// Step t (pick m_t, S_t -> S_{t+1}) leaves the list at s_{t+1} = 1 + |S_{t+1}|
+// entries (the seed stays in front) and positions [s_{t+1}, s_t) keep S_t's
+// entries: position p holds S_t's element of rank p - 1 in ascending order.
+// The writes need |S_{t+1}| after every pick, so within word wt the picks'
+// rows are staged in LDS kReplayChunk at a time (loads in flight together);
+// the steps then run on registers and LDS: S_t is U_t (the rest of word wt,
+// above every other vertex of S_t) and the lower words L_t.
+__device__ __forceinline__ int replay_walk(int W, int c, const unsigned long long* __restrict__ adj,
+                                           const unsigned long long* dmask, int* __restrict__ out, int lane,
+                                           unsigned long long* rl) {
+  unsigned long long S[kMaxWPL];
+#pragma unroll
+  for (int q = 0; q < kMaxWPL; ++q) {
+    const int w = lane + 64 * q;
+    S[q] = (w < W) ? (adj[(size_t)c * W + w] & dmask[w] &
+                      ((w == (c >> 6)) ? ~(1ull << (c & 63)) : ~0ull))
+                   : 0ull;
+  }
+  if (lane == 0) out[0] = c;
+  int s_t = 1 + wave_popcount(S);  // list size before the step
+  int picks = 0;
+  for (;;) {
+    int wt = -1;
+    unsigned long long U = 0ull;
+#pragma unroll
+    for (int q = kMaxWPL - 1; q >= 0; --q) {
+      if (64 * q >= W) continue;
+      const unsigned long long lanes = __ballot(lane + 64 * q < W && S[q] != 0ull);
+      if (lanes) {
+        const int L = 63 - __clzll(lanes);
+        wt = 64 * q + L;
+        U = readlane_u64(S[q], L);
+        break;
+      }
+    }
+    // S held only the seed: the last step leaves the list empty (s_t == 1,
+    // nothing is written)
+    if (wt < 0) return picks + 1;
+#pragma unroll
+    for (int q = 0; q < kMaxWPL; ++q)
+      if (lane + 64 * q == wt) S[q] = 0ull;  // S is L_t from here
+    const unsigned long long* blk = adj + (size_t)64 * wt * W;
+    unsigned long long dw = 0ull;
+    if (U & (U - 1)) {
+      const bool in = (U >> lane) & 1ull;
+      const unsigned long long v = blk[(size_t)(in ? lane : 63 - __clzll(U)) * W + wt];
+      dw = in ? v : 0ull;
+    }
+    unsigned long long P = 0ull, Ug = U;
+    while (Ug) {
+      const int p = 63 - __clzll(Ug);
+      P |= 1ull << p;
+      Ug &= readlane_u64(dw, p) & ~(1ull << p);
+    }
+    while (P) {
+      // the next kReplayChunk picks, highest first (a short chunk repeats
+      // its first pick: ANDing a row twice changes nothing)
+      int pk[kReplayChunk];
+      int nk = 0;
+#pragma unroll
+      for (int k = 0; k < kReplayChunk; ++k) {
+        pk[k] = P ? 63 - __clzll(P) : pk[0];
+        if (P) {
+          P &= ~(1ull << pk[k]);
+          ++nk;
+        }
+      }
+      // per word q: the chunk's row words loaded together, kept in LDS for
+      // the writes (word lane + 64 q of pick k at rl[(k kMaxWPL + q) 64 +
+      // lane]; each lane reads back only its own), and L after each pick with
+      // its size; the kReplayChunk wave sums are then taken together
+      unsigned long long L0[kMaxWPL];
+      int cnt[kReplayChunk];
+#pragma unroll
+      for (int k = 0; k < kReplayChunk; ++k) cnt[k] = 0;
 #pragma unroll
       for (int q = 0; q < kMaxWPL; ++q) {
+        L0[q] = S[q];
+        if (64 * q >= W) continue;
         const int w = lane + 64 * q;
-        if (w < W) S[q] &= Rw[p][q] & ((w == (m >> 6)) ? ~(1ull << (m & 63)) : ~0ull);
+        const int wa = (w < W && S[q] != 0ull) ? w : wt;  // dead lanes: one shared line
+        unsigned long long r[kReplayChunk];
+#pragma unroll
+        for (int k = 0; k < kReplayChunk; ++k) r[k] = blk[(size_t)pk[k] * W + wa];
+        unsigned long long a = S[q];
+#pragma unroll
+        for (int k = 0; k < kReplayChunk; ++k) {
+          rl[(k * kMaxWPL + q) * 64 + lane] = r[k];
+          a &= r[k];
+          cnt[k] += __popcll(a);
+        }
+        S[q] = a;
       }
-      if ((icc & 7) == 0 && icc + wave_popcount(S) + 1 <= best) return INT_MIN;
+#pragma unroll
+      for (int k = 0; k < kReplayChunk; ++k) cnt[k] = __ockl_wfred_add_i32(cnt[k]);
+#pragma unroll
+      for (int k = 0; k < kReplayChunk; ++k) {
+        if (k >= nk) break;
+        const int m = 63 - __clzll(U);  // the k-th staged pick
+        const unsigned long long Un = U & readlane_u64(dw, m) & ~(1ull << m);
+        const int s_next = 1 + __popcll(Un) + cnt[k];
+        // positions [s_next, s_t): the top of S_t, first U_t (highest bit
+        // first), then L_t, rebuilt from the staged rows when reached
+        int p = s_t - 1;
+        unsigned long long Ut = U;
+        for (; p >= s_next && Ut; --p) {
+          const int t = 63 - __clzll(Ut);
+          if (lane == 0) out[p] = 64 * wt + t;
+          Ut &= ~(1ull << t);
+        }
+        if (p >= s_next) {
+          unsigned long long Lp[kMaxWPL];
+#pragma unroll
+          for (int q = 0; q < kMaxWPL; ++q) {
+            Lp[q] = L0[q];
+#pragma unroll
+            for (int j = 0; j < k; ++j)
+              if (64 * q < W) Lp[q] &= rl[(j * kMaxWPL + q) * 64 + lane];
+          }
+          for (; p >= s_next; --p) {
+            const int t = wave_top_bit(Lp, lane, W);
+            if (lane == 0) out[p] = t;
+            const int w = t >> 6;
+#pragma unroll
+            for (int q = 0; q < kMaxWPL; ++q)
+              if (lane + 64 * q == w) Lp[q] &= ~(1ull << (t & 63));
+          }
+        }
+        U = Un;
+        s_t = s_next;
+        ++picks;
+      }
     }
   }
 }
@@ -189,7 +360,7 @@ __global__ void __launch_bounds__(256) k_icc(int n, int W, int pos, int best,
     if (lane == 0) icc_out[c] = INT_MIN;
     return;
   }
-  const int icc = icc_wave<1>(W, c, adj, dmask, lane, best);
+  const int icc = icc_walk(W, c, adj, dmask, lane, best);
   if (lane == 0) icc_out[c] = icc;
 }
 
@@ -200,98 +371,12 @@ __global__ void k_first_better(int n, int pos, int best, const int* __restrict__
   if (icc[c] > best) atomicMin(first, c);
 }
 
-// Replay of the winning candidate c: writes the reference's buffer positions
-// 0..s0-1 (position p keeps the value last written to it) and returns the
-// greedy size.  One wavefront.  The rows of the PF highest vertices of S
-// are fetched together, as in icc_wave; a step whose pick differs from the
-// prediction fetches again, so the steps and writes are those of a walk one
-// row at a time.
-template <int PF>
-__device__ __forceinline__ int replay_wave(int W, int c, const unsigned long long* __restrict__ adj,
-                                           const unsigned long long* dmask, int* __restrict__ out, int lane) {
-  unsigned long long S[kMaxWPL];
-#pragma unroll
-  for (int q = 0; q < kMaxWPL; ++q) {
-    const int w = lane + 64 * q;
-    S[q] = (w < W) ? (adj[(size_t)c * W + w] & dmask[w] &
-                      ((w == (c >> 6)) ? ~(1ull << (c & 63)) : ~0ull))
-                   : 0ull;
-  }
-  if (lane == 0) out[0] = c;
-  int s_t = 1 + wave_popcount(S);  // list size before the iteration
-  int icc = 0;
-  for (;;) {
-    // predicted picks: the PF highest vertices of S, highest first
-    int cand[PF];
-    {
-      unsigned long long T[kMaxWPL];
-#pragma unroll
-      for (int q = 0; q < kMaxWPL; ++q) T[q] = S[q];
-#pragma unroll
-      for (int f = 0; f < PF; ++f) {
-        const int t = wave_top_bit(T, lane, W);
-        cand[f] = t;
-        if (t >= 0) {
-          const int w = t >> 6;
-#pragma unroll
-          for (int q = 0; q < kMaxWPL; ++q)
-            if (lane + 64 * q == w) T[q] &= ~(1ull << (t & 63));
-        }
-      }
-    }
-    unsigned long long Rw[PF][kMaxWPL];
-#pragma unroll
-    for (int f = 0; f < PF; ++f)
-#pragma unroll
-      for (int q = 0; q < kMaxWPL; ++q) {
-        const int w = lane + 64 * q;
-        Rw[f][q] = (cand[f] >= 0 && w < W && S[q]) ? adj[(size_t)cand[f] * W + w] : 0ull;  // empty words stay empty
-      }
-#pragma unroll
-    for (int f = 0; f < PF; ++f) {
-      const int m = wave_top_bit(S, lane, W);
-      if (m >= 0 && m != cand[f]) break;  // prediction missed (wave-uniform): fetch again
-      ++icc;
-      unsigned long long N[kMaxWPL];
-#pragma unroll
-      for (int q = 0; q < kMaxWPL; ++q) {
-        const int w = lane + 64 * q;
-        N[q] = (m >= 0 && w < W) ? (S[q] & Rw[f][q] & ((w == (m >> 6)) ? ~(1ull << (m & 63)) : ~0ull)) : 0ull;
-      }
-      // list after the iteration: the seed survives unless it was the pick
-      const int s_next = (m >= 0) ? 1 + wave_popcount(N) : 0;
-      // positions [max(s_next,1), s_t) keep S_t's entries: position p holds the
-      // element of rank p-1 (ascending vertex order), i.e. the top s_t - lo
-      // vertices of S_t, highest at s_t - 1; the pick and its non-neighbours
-      // leave S, so there are as many as vertices removed at this step
-      const int lo = s_next > 1 ? s_next : 1;
-      if (lo < s_t) {
-        unsigned long long T[kMaxWPL];
-#pragma unroll
-        for (int q = 0; q < kMaxWPL; ++q) T[q] = S[q];
-        for (int p = s_t - 1; p >= lo; --p) {
-          const int t = wave_top_bit(T, lane, W);
-          if (lane == 0) out[p] = t;
-          const int w = t >> 6;
-#pragma unroll
-          for (int q = 0; q < kMaxWPL; ++q)
-            if (lane + 64 * q == w) T[q] &= ~(1ull << (t & 63));
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < kMaxWPL; ++q) S[q] = N[q];
-      s_t = s_next;
-      if (m < 0) return icc;
-    }
-  }
-}
-
-
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) k_replay(int n, int W, int c, int best,
                                                const unsigned long long* __restrict__ adj,
                                                const unsigned long long* __restrict__ dmask,
                                                int* __restrict__ out, int* __restrict__ icc_out) {
-  const int icc = replay_wave<kPrefetch>(W, c, adj, dmask, out, threadIdx.x);
+  __shared__ unsigned long long rl[kReplayLds];
+  const int icc = replay_walk(W, c, adj, dmask, out, threadIdx.x, rl);
   if (threadIdx.x == 0) *icc_out = icc;
   (void)best;
   (void)n;
@@ -321,7 +406,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
 // accepted loop closures per graph) the clique stage took 121 / 110 / 126 /
 // 167 ms with windows of 256 / 512 / 1,024 / 2,048 (wider windows re-evaluate
 // more candidates after each winner, narrower ones need more rounds)
-constexpr int kCbWindow = 512;   // candidates per graph per round
+constexpr int kCbWindow = 256;   // candidates per graph per round
 constexpr int kCbExtraRounds = 8;  // rounds for winners, beyond one per window of the largest
                                    // graph; the per-graph scan finishes what is left
 
@@ -406,22 +491,32 @@ __global__ void __launch_bounds__(256) k_cb_icc(int n_graphs, const long long* _
   // best < 1: the window's first candidate wins (every greedy size is >= 1),
   // so the others need no walk
   if (!(best > deg_all[lc_off[g] + c]) && (best >= 1 || k == 0))  // pruning 1
-    icc = icc_wave<1>((n + 63) / 64, c, adj_all + adj_off[g], dm + lc_off[g], lane, best);
+    icc = icc_walk((n + 63) / 64, c, adj_all + adj_off[g], dm + lc_off[g], lane, best);
   if (lane == 0) icc_out[t] = icc;
 }
 
-// one thread per graph: the window's first winner, or the next window
-__global__ void k_cb_select(int n_graphs, const int* __restrict__ m_arr, const int* __restrict__ icc,
-                            CbState* __restrict__ st) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+// one wavefront per graph: the window's first winner (by ballot over 64
+// slots at a time), or the next window
+__global__ void __launch_bounds__(256) k_cb_select(int n_graphs, const int* __restrict__ m_arr,
+                                                   const int* __restrict__ icc, CbState* __restrict__ st) {
+  const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (g >= n_graphs) return;
   CbState s = st[g];
   const int n = m_arr[g];
   if (s.pos >= n) return;
   const int span = min(kCbWindow, n - s.pos);
   const int* row = icc + (size_t)g * kCbWindow;
-  int k = 0;
-  while (k < span && !(row[k] > s.best)) ++k;
+  int k = span;
+  for (int base = 0; base < span; base += 64) {
+    const int kk = base + lane;
+    const unsigned long long b = __ballot(kk < span && row[kk] > s.best);
+    if (b) {
+      k = base + __builtin_ctzll(b);
+      break;
+    }
+  }
+  if (lane != 0) return;
   if (k < span) {
     s.win = s.pos + k;
     s.win_best = s.best;
@@ -433,7 +528,7 @@ __global__ void k_cb_select(int n_graphs, const int* __restrict__ m_arr, const i
   st[g] = s;
 }
 
-template <int NW, int PF>
+template <int NW>
 __global__ void __launch_bounds__(64 * NW) k_clique_batch(int n_graphs, const long long* __restrict__ adj_off,
                                                           const long long* __restrict__ lc_off,
                                                           const int* __restrict__ m_arr,
@@ -442,6 +537,7 @@ __global__ void __launch_bounds__(64 * NW) k_clique_batch(int n_graphs, const lo
                                                           int* __restrict__ k_out,
                                                           const CbState* __restrict__ start) {
   __shared__ unsigned long long dm[64 * kMaxWPL];
+  __shared__ unsigned long long rl[kReplayLds];  // wavefront 0's replay
   __shared__ int s_icc[NW];
   __shared__ int s_win, s_win_icc;
   const int g = blockIdx.x;
@@ -495,7 +591,7 @@ __global__ void __launch_bounds__(64 * NW) k_clique_batch(int n_graphs, const lo
       const int c = base + wid;
       int icc = INT_MIN;
       // pruning 1; with best < 1 the first candidate wins (greedy sizes are >= 1)
-      if (c < n && !(best > deg[c]) && (best >= 1 || wid == 0)) icc = icc_wave<PF>(W, c, A, dm, lane, best);
+      if (c < n && !(best > deg[c]) && (best >= 1 || wid == 0)) icc = icc_walk(W, c, A, dm, lane, best);
       if (lane == 0) s_icc[wid] = icc;
       __syncthreads();
       if (tid == 0) {
@@ -534,7 +630,7 @@ __global__ void __launch_bounds__(64 * NW) k_clique_batch(int n_graphs, const lo
     }
     __syncthreads();
     if (wid == 0) {
-      const int icc = replay_wave<PF>(W, win, A, dm, out, lane);
+      const int icc = replay_walk(W, win, A, dm, out, lane, rl);
       if (lane == 0) k_out[g] = icc == best ? best : INT_MIN;
     }
   } else if (tid == 0) {
@@ -547,12 +643,12 @@ size_t clique_spread_icc_bytes(int n_graphs) {
   return sizeof(int) * (size_t)kCbWindow * (size_t)(n_graphs > 0 ? n_graphs : 1);
 }
 
-int clique_batch_launch(rpgo_ctx* ctx, int n_graphs, int waves, int prefetch, const long long* d_adj_off,
+int clique_batch_launch(rpgo_ctx* ctx, int n_graphs, int waves, const long long* d_adj_off,
                         const long long* d_lc_off, const int* d_m, const unsigned long long* adj, int* deg,
                         int* out_idx, int* out_k, const CliqueSpread* spread) {
   hipStream_t st = ctx->stream;
-  if ((waves != 4 && waves != 8 && waves != 16) || (prefetch != 1 && prefetch != kPrefetch)) {
-    set_error("clique_batch_launch: unsupported shape (%d wavefronts, prefetch %d)", waves, prefetch);
+  if (waves != 4 && waves != 8 && waves != 16) {
+    set_error("clique_batch_launch: unsupported shape (%d wavefronts)", waves);
     return RPGO_ERR_ARG;
   }
   CbState* cs = nullptr;
@@ -568,28 +664,21 @@ int clique_batch_launch(rpgo_ctx* ctx, int n_graphs, int waves, int prefetch, co
       k_cb_mask<<<sg, 256, 0, st>>>(ns, spread->slot_graph, d_lc_off, d_m, deg, cs, spread->dm);
       k_cb_icc<<<gi, 256, 0, st>>>(n_graphs, d_adj_off, d_lc_off, d_m, adj, deg, spread->dm, cs, spread->xorder,
                                    spread->icc);
-      k_cb_select<<<(n_graphs + 255) / 256, 256, 0, st>>>(n_graphs, d_m, spread->icc, cs);
+      k_cb_select<<<(n_graphs + 3) / 4, 256, 0, st>>>(n_graphs, d_m, spread->icc, cs);
     }
     RPGO_HIP_CHECK(hipGetLastError());
   }
-#define RPGO_CB(NW, PF)                                                                                       \
-  k_clique_batch<NW, PF><<<n_graphs, 64 * NW, 0, st>>>(n_graphs, d_adj_off, d_lc_off, d_m, adj, deg, out_idx, \
-                                                       out_k, cs)
-  if (prefetch == 1) {
-    if (waves == 4) RPGO_CB(4, 1);
-    else if (waves == 8) RPGO_CB(8, 1);
-    else RPGO_CB(16, 1);
-  } else {
-    if (waves == 4) RPGO_CB(4, kPrefetch);
-    else if (waves == 8) RPGO_CB(8, kPrefetch);
-    else RPGO_CB(16, kPrefetch);
-  }
+#define RPGO_CB(NW) \
+  k_clique_batch<NW><<<n_graphs, 64 * NW, 0, st>>>(n_graphs, d_adj_off, d_lc_off, d_m, adj, deg, out_idx, out_k, cs)
+  if (waves == 4) RPGO_CB(4);
+  else if (waves == 8) RPGO_CB(8);
+  else RPGO_CB(16);
 #undef RPGO_CB
   RPGO_HIP_CHECK(hipGetLastError());
   return RPGO_OK;
 }
 
-int clique_prefetch_depth() { return kPrefetch; }
+int clique_walk_chunk() { return kWalkChunk; }
 
 int clique_max_vertices() { return 64 * 64 * kMaxWPL; }
 

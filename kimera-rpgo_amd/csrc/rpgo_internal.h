@@ -232,9 +232,7 @@ int clique_incremental(rpgo_ctx* ctx, const uint64_t* d_adj_u64, int n, int num_
                        int32_t* idx_out, int32_t* k_out);
 // clique_exact.hip: exact maximum clique given a clique size known to exist
 // batched FMC::maxCliqueHeu, one workgroup of `waves` wavefronts (4, 8, 16)
-// per graph; prefetch: rows loaded ahead in the greedy walk (1 or
-// clique_prefetch_depth()); out_k[g] = -1 for an empty graph, INT_MIN on an
-// internal error
+// per graph; out_k[g] = -1 for an empty graph, INT_MIN on an internal error
 // Optional chip-wide rounds before the per-graph scan (spread != NULL):
 // vertex slots are indexed like the loop closures (n_slots of them, graph of
 // each in slot_graph); state, icc and dm are scratch of
@@ -252,11 +250,11 @@ struct CliqueSpread {
 };
 size_t clique_spread_state_bytes(int n_graphs);
 size_t clique_spread_icc_bytes(int n_graphs);
-int clique_batch_launch(rpgo_ctx* ctx, int n_graphs, int waves, int prefetch, const long long* d_adj_off,
+int clique_batch_launch(rpgo_ctx* ctx, int n_graphs, int waves, const long long* d_adj_off,
                         const long long* d_lc_off, const int* d_m, const unsigned long long* adj, int* deg,
                         int* out_idx, int* out_k, const CliqueSpread* spread = nullptr);
 int clique_max_vertices();
-int clique_prefetch_depth();
+int clique_walk_chunk();  // rows the greedy walk loads together
 int clique_exact_run(rpgo_ctx* ctx, const unsigned long long* adj, int n, int lower_bound, int32_t* idx_out,
                      int32_t* k_out, long long node_budget);
 // clique.hip
