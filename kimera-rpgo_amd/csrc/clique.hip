@@ -402,10 +402,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
 // window), and candidates after a winner are evaluated again next round with
 // the new best.
 // ---------------------------------------------------------------------------
-// 512 candidates per graph per round: on the 256-graph batch (up to ~6,000
-// accepted loop closures per graph) the clique stage took 121 / 110 / 126 /
-// 167 ms with windows of 256 / 512 / 1,024 / 2,048 (wider windows re-evaluate
-// more candidates after each winner, narrower ones need more rounds)
+// 256 candidates per graph per round: on the 256-graph batch (up to ~6,000
+// accepted loop closures per graph) the clique stage took 22.4 / 19.5 / 21.4 /
+// 27.0 ms with windows of 128 / 256 / 512 / 1,024 on MI355X (wider windows
+// re-evaluate more candidates after each winner, narrower ones need more rounds)
 constexpr int kCbWindow = 256;   // candidates per graph per round
 constexpr int kCbExtraRounds = 8;  // rounds for winners, beyond one per window of the largest
                                    // graph; the per-graph scan finishes what is left
