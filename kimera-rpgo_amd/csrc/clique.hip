@@ -347,6 +347,91 @@ __device__ __forceinline__ int replay_walk(int W, int c, const unsigned long lon
   }
 }
 
+// A walk against best stays inside the vertices with deg >= best (dmask): S
+// starts within it and only shrinks, and the candidates that walk pass the
+// same filter.  The rounds of the single-graph scan therefore walk the
+// subgraph of those vertices, renumbered in the same order (so "highest"
+// picks the same vertex), with rows of ceil(n'/64) words instead of W: the
+// same sizes for fewer row bytes, which bound the rounds.
+// One workgroup of 512: new indices by a scan of the dmask words' popcounts.
+__global__ void __launch_bounds__(512) k_compact_index(int n, int W, const unsigned long long* __restrict__ dmask,
+                                                       int* __restrict__ cidx, int* __restrict__ ccol,
+                                                       unsigned long long* __restrict__ cdmask,
+                                                       int* __restrict__ cn) {
+  __shared__ int pre[512];
+  const int t = threadIdx.x;
+  const unsigned long long b = t < W ? dmask[t] : 0ull;
+  pre[t] = __popcll(b);
+  __syncthreads();
+  for (int d = 1; d < 512; d <<= 1) {  // inclusive scan
+    const int v = t >= d ? pre[t - d] : 0;
+    __syncthreads();
+    pre[t] += v;
+    __syncthreads();
+  }
+  const int total = pre[511];
+  int r = pre[t] - __popcll(b);
+  if (t < W) {
+    for (int k = 0; k < 64; ++k) {
+      const int v = 64 * t + k;
+      if (v >= n) break;
+      if ((b >> k) & 1ull) {
+        cidx[v] = r;
+        ccol[r++] = v;
+      } else {
+        cidx[v] = -1;
+      }
+    }
+  }
+  if (t < W) {
+    const int lo = 64 * t;
+    cdmask[t] = total >= lo + 64 ? ~0ull : (total > lo ? (1ull << (total - lo)) - 1ull : 0ull);
+  }
+  if (t == 0) *cn = total;
+}
+
+// word w' of new row v': bits of old row ccol[v'] at old columns
+// ccol[64 w' .. 64 w' + 63] (ascending, so the old words come in order)
+__global__ void k_compact_adj(int n, int W, const unsigned long long* __restrict__ adj,
+                              const int* __restrict__ ccol, const int* __restrict__ cn,
+                              unsigned long long* __restrict__ cadj) {
+  const int nc = *cn, Wc = (nc + 63) / 64;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nc * Wc) return;
+  const int vr = (int)(t / Wc), wr = (int)(t % Wc);
+  const unsigned long long* row = adj + (size_t)ccol[vr] * W;
+  unsigned long long out = 0ull, cur = 0ull;
+  int cw = -1;
+  for (int k = 0; k < 64; ++k) {
+    const int u2 = 64 * wr + k;
+    if (u2 >= nc) break;
+    const int u = ccol[u2];
+    if ((u >> 6) != cw) {
+      cw = u >> 6;
+      cur = row[cw];
+    }
+    out |= ((cur >> (u & 63)) & 1ull) << k;
+  }
+  cadj[(size_t)vr * Wc + wr] = out;
+}
+
+// k_icc on the compacted graph; results by old index.
+__global__ void __launch_bounds__(256) k_icc_compact(int n, int pos, int best, const int* __restrict__ deg,
+                                                     const int* __restrict__ cidx, const int* __restrict__ cn,
+                                                     const unsigned long long* __restrict__ cadj,
+                                                     const unsigned long long* __restrict__ cdmask,
+                                                     int* __restrict__ icc_out) {
+  const int c = pos + blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c >= n) return;
+  const int lane = threadIdx.x & 63;
+  if (best > deg[c]) {  // pruning 1
+    if (lane == 0) icc_out[c] = INT_MIN;
+    return;
+  }
+  const int icc = icc_walk((*cn + 63) / 64, cidx[c], cadj, cdmask, lane, best);
+  if (lane == 0) icc_out[c] = icc;
+}
+
 // One wavefront per candidate c >= pos: greedy size icc(c, best).
 __global__ void __launch_bounds__(256) k_icc(int n, int W, int pos, int best,
                                              const unsigned long long* __restrict__ adj,
@@ -690,6 +775,12 @@ struct ScanBufs {
   int* icc;
   int* out;
   int* best;
+  // compacted graph of the rounds (all null: the rounds walk the full rows)
+  int* cidx = nullptr;                  // n: new index of a vertex with deg >= best
+  int* ccol = nullptr;                  // n: old index of new vertex v'
+  unsigned long long* cdmask = nullptr;  // W: every new vertex
+  unsigned long long* cadj = nullptr;    // n x W words (rows of ceil(n'/64) words)
+  int* cn = nullptr;                     // n'
 };
 
 // Greedy scan over candidates pos0 .. n-1 starting from the bound best0;
@@ -815,8 +906,14 @@ static int clique_scan(rpgo_ctx* ctx, const uint64_t* d_adj_u64, int n, int mode
     if ((rc = s.icc.reserve(sizeof(int) * n))) return rc;
     if ((rc = s.out.reserve(sizeof(int) * (n + 1)))) return rc;
     if ((rc = s.best.reserve(sizeof(int) * 4))) return rc;
+    if ((rc = s.cidx.reserve(sizeof(int) * n))) return rc;
+    if ((rc = s.ccol.reserve(sizeof(int) * n))) return rc;
+    if ((rc = s.cdmask.reserve(sizeof(unsigned long long) * W))) return rc;
+    if ((rc = s.cadj.reserve(sizeof(unsigned long long) * (size_t)n * W))) return rc;
+    if ((rc = s.cn.reserve(sizeof(int)))) return rc;
     b = ScanBufs{s.deg.as<int>(), s.dmask.as<unsigned long long>(), s.icc.as<int>(), s.out.as<int>(),
-                 s.best.as<int>()};
+                 s.best.as<int>(), s.cidx.as<int>(), s.ccol.as<int>(), s.cdmask.as<unsigned long long>(),
+                 s.cadj.as<unsigned long long>(), s.cn.as<int>()};
   }
   hipStream_t st = ctx->stream;
   k_degrees<<<(n + 255) / 256, 256, 0, st>>>(n, W, adj, b.deg);
@@ -827,7 +924,15 @@ static int clique_scan(rpgo_ctx* ctx, const uint64_t* d_adj_u64, int n, int mode
     k_dmask<<<(W + 255) / 256, 256, 0, st>>>(n, W, best, b.deg, b.dmask, d_first);
     // best < 1: candidate pos wins (every greedy size is >= 1), so it alone is walked
     const int rem = best < 1 ? 1 : n - pos;
-    k_icc<<<(rem + 3) / 4, 256, 0, st>>>(n, W, pos, best, adj, b.deg, b.dmask, b.icc);
+    if (best >= 1 && b.cadj && W > 64) {
+      // the round's subgraph of deg >= best, renumbered in order (rows of at
+      // most 64 words take one load per lane either way: not compacted)
+      k_compact_index<<<1, 512, 0, st>>>(n, W, b.dmask, b.cidx, b.ccol, b.cdmask, b.cn);
+      k_compact_adj<<<(unsigned)(((long long)n * W + 255) / 256), 256, 0, st>>>(n, W, adj, b.ccol, b.cn, b.cadj);
+      k_icc_compact<<<(rem + 3) / 4, 256, 0, st>>>(n, pos, best, b.deg, b.cidx, b.cn, b.cadj, b.cdmask, b.icc);
+    } else {
+      k_icc<<<(rem + 3) / 4, 256, 0, st>>>(n, W, pos, best, adj, b.deg, b.dmask, b.icc);
+    }
     RPGO_HIP_CHECK(hipGetLastError());
     k_first_better<<<(rem + 255) / 256, 256, 0, st>>>(n, pos, best, b.icc, d_first);
     RPGO_HIP_CHECK(hipGetLastError());

@@ -119,6 +119,7 @@ struct PcmState {
 struct CliqueState {
   DevBuf adj;      // caller-provided adjacency (rpgo_max_clique_bits)
   DevBuf deg, dmask, icc, out, scratch, best;
+  DevBuf cidx, ccol, cdmask, cadj, cn;  // the heuristic rounds' compacted graph
   DevBuf ex_adj, ex_ctl, ex_stack, ex_path;  // exact mode (clique_exact.hip)
 };
 
