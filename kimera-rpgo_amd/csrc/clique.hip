@@ -220,7 +220,7 @@ constexpr int kReplayLds = kReplayChunk * kMaxWPL * 64;
 // above every other vertex of S_t) and the lower words L_t.
 __device__ __forceinline__ int replay_walk(int W, int c, const unsigned long long* __restrict__ adj,
                                            const unsigned long long* dmask, int* __restrict__ out, int lane,
-                                           unsigned long long* rl) {
+                                           unsigned long long* rl, int* s0 = nullptr) {
   unsigned long long S[kMaxWPL];
 #pragma unroll
   for (int q = 0; q < kMaxWPL; ++q) {
@@ -231,6 +231,7 @@ __device__ __forceinline__ int replay_walk(int W, int c, const unsigned long lon
   }
   if (lane == 0) out[0] = c;
   int s_t = 1 + wave_popcount(S);  // list size before the step
+  if (s0) *s0 = s_t;                // positions 0 .. s_t - 1 are written
   int picks = 0;
   for (;;) {
     int wt = -1;
@@ -430,6 +431,28 @@ __global__ void __launch_bounds__(256) k_icc_compact(int n, int pos, int best, c
   }
   const int icc = icc_walk((*cn + 63) / 64, cidx[c], cadj, cdmask, lane, best);
   if (lane == 0) icc_out[c] = icc;
+}
+
+// The winner's replay on the compacted graph (its dmask is deg >= the best
+// the winner beat, which the replay walk never leaves); the buffer holds new
+// indices until k_remap_out.  res[0] = greedy size, res[1] = positions written.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
+k_replay_compact(int c, const int* __restrict__ cidx, const int* __restrict__ cn,
+                 const unsigned long long* __restrict__ cadj, const unsigned long long* __restrict__ cdmask,
+                 int* __restrict__ out, int* __restrict__ res) {
+  __shared__ unsigned long long rl[kReplayLds];
+  int s0 = 0;
+  const int icc = replay_walk((*cn + 63) / 64, cidx[c], cadj, cdmask, out, threadIdx.x, rl, &s0);
+  if (threadIdx.x == 0) {
+    res[0] = icc;
+    res[1] = s0;
+  }
+}
+
+// buffer positions 0 .. res[1] - 1 back to old indices
+__global__ void k_remap_out(int* __restrict__ out, const int* __restrict__ ccol, const int* __restrict__ res) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < res[1]) out[p] = ccol[out[p]];
 }
 
 // One wavefront per candidate c >= pos: greedy size icc(c, best).
@@ -955,7 +978,15 @@ static int clique_scan(rpgo_ctx* ctx, const uint64_t* d_adj_u64, int n, int mode
   if (win >= 0) {
     // dmask for the winner's threshold
     k_dmask<<<(W + 255) / 256, 256, 0, st>>>(n, W, win_best, b.deg, b.dmask, nullptr);
-    k_replay<<<1, 64, 0, st>>>(n, W, win, win_best, adj, b.dmask, b.out, b.best + 1);
+    if (win_best >= 1 && b.cadj && W > 64) {
+      // on the subgraph of deg >= win_best, as the rounds
+      k_compact_index<<<1, 512, 0, st>>>(n, W, b.dmask, b.cidx, b.ccol, b.cdmask, b.cn);
+      k_compact_adj<<<(unsigned)(((long long)n * W + 255) / 256), 256, 0, st>>>(n, W, adj, b.ccol, b.cn, b.cadj);
+      k_replay_compact<<<1, 64, 0, st>>>(win, b.cidx, b.cn, b.cadj, b.cdmask, b.out, b.best + 1);
+      k_remap_out<<<(n + 255) / 256, 256, 0, st>>>(b.out, b.ccol, b.best + 1);
+    } else {
+      k_replay<<<1, 64, 0, st>>>(n, W, win, win_best, adj, b.dmask, b.out, b.best + 1);
+    }
     RPGO_HIP_CHECK(hipGetLastError());
     int icc_r = 0;
     RPGO_HIP_CHECK(hipMemcpyAsync(&icc_r, b.best + 1, sizeof(int), hipMemcpyDeviceToHost, st));

@@ -99,8 +99,9 @@ def test_noalloc_matches_clique_bits_on_two_calls():
 
 
 def test_compacted_rounds_match_full_rows():
-    """Above 64 words per row, rpgo_max_clique_bits walks each round against
-    the subgraph of vertices with deg >= best, renumbered in order; the
+    """Above 64 words per row, rpgo_max_clique_bits walks each round and the
+    winner's replay against the subgraph of vertices with deg >= best,
+    renumbered in order (the buffer mapped back after the replay); the
     allocation-free scan walks the full rows.  Both give the same clique and
     buffer (the full-row scan equals the FP64 oracle's in the test above)."""
     hip = _hip()
